@@ -1,0 +1,1067 @@
+/*
+ * oracle.c -- CPU restatement of libsupermesh's supermeshing hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).  Compiled with
+ * -O2 -ffp-contract=off -fno-fast-math so that every operation rounds as
+ * the reference's Fortran does on x86-64 (IEEE double, no FMA contraction,
+ * correctly rounded / and sqrt).
+ *
+ * Restated reference routines (all paths under src/):
+ *   orc_spacing, orc_epsilon      Fortran SPACING / EPSILON as used at
+ *                                 Triangle_Intersection.F90:174,
+ *                                 Tetrahedron_Intersection.F90:151,266
+ *   orc_norm2_3                   Fortran NORM2 in unit_cross,
+ *                                 Tetrahedron_Intersection.F90:669-677
+ *   orc_triangle_area             Triangle_Intersection.F90:453-477
+ *   orc_tetrahedron_volume_real   Tetrahedron_Intersection.F90:690-706
+ *   orc_tetrahedron_volume_tet    Tetrahedron_Intersection.F90:708-725
+ *   orc_intersect_tris            Triangle_Intersection.F90:144-185 (driver),
+ *                                 :282-335 (clip_buf), :395-420 (get_lines_tri)
+ *   orc_intersect_tets            Tetrahedron_Intersection.F90:137-179 (BFS
+ *                                 driver), :244-436 (clip_buf),
+ *                                 :634-667 (get_planes_tet), :679-688
+ *   orc_ksum_*                    Precision.F90:72-113
+ *   orc_brute_force_finder        Intersection_Finder.F90:990-1050 (strict);
+ *                                 closed variant = R-tree predicate,
+ *                                 R-Tree_Intersection_Finder_C++.h:233-251
+ *                                 with libspatialindex closed Region overlap
+ *   orc_tree_*                    Quadtree_Intersection_Finder.F90:174-420,
+ *                                 Octree_Intersection_Finder.F90:153-411
+ *   orc_mass_pairs,               test_parallel_p1_inner_product_2d.F90:59-68,
+ *   orc_inner_product             205-299; ..._3d.F90:61-80,226-337
+ *
+ * What pins each function (tests/test_oracle_*.py):
+ *   spacing / epsilon / norm2 / dot: bit-compared with the reference's own
+ *     compiler (amdflang) on probe vectors when amdflang is present.
+ *   tri / tet clippers: the reference tests' known answers on the reference's
+ *     fixture meshes (sum area 0.5, Kahan sum volume 1/3, cube.1 x cube.2 = 1),
+ *     the reference-measured candidate / non-empty / output counts on those
+ *     meshes, exact rational clipping of tiny inputs, self-intersection and
+ *     completeness (sum over A of |a ^ b| = |b|).
+ *   finders: tree set == brute-force set; brute force == an independent numpy
+ *     broadcast of the predicate; reference-measured candidate counts.
+ *   mass: the reference tests' integral values (8.3333333333333398e-2 in 2D,
+ *     0.12499999999999989 in 3D), 1^T M 1 = overlap measure, exactness of the
+ *     degree-2 rules on linear fields.
+ */
+#include "oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+/* ------------------------------------------------------------------------ */
+/* Fortran intrinsics                                                        */
+/* ------------------------------------------------------------------------ */
+
+/* SPACING(x) = 2**max(e-53, -1022) with x = m*2**e, 0.5 <= |m| < 1;
+ * TINY(x) for x == 0 (Fortran 2008, 13.7.157). */
+double orc_spacing(double x) {
+  if (x == 0.0) return DBL_MIN;
+  int e;
+  frexp(x, &e);
+  int k = e - 53;
+  if (k < -1022) k = -1022;
+  return ldexp(1.0, k);
+}
+
+double orc_epsilon(void) { return DBL_EPSILON; }
+
+/* NORM2 as the reference's compiler evaluates it for real(8): a running
+ * maximum m and a scaled sum of squares s, result m*sqrt(1+s). */
+double orc_norm2_3(const double v[3]) {
+  double mx = 0.0, s = 0.0;
+  for (int i = 0; i < 3; i++) {
+    double ax = fabs(v[i]);
+    if (mx == 0.0) {
+      mx = ax;
+    } else if (ax > mx) {
+      double t = mx / ax;
+      double tsq = t * t;
+      s = s * tsq;
+      s = s + tsq;
+      mx = ax;
+    } else {
+      double t = ax / mx;
+      s = s + t * t;
+    }
+  }
+  return mx * sqrt(1.0 + s);
+}
+
+/* DOT_PRODUCT over 3 reals: sequential accumulation. */
+double orc_dot3(const double a[3], const double b[3]) {
+  double s = a[0] * b[0];
+  s = s + a[1] * b[1];
+  s = s + a[2] * b[2];
+  return s;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Measures                                                                  */
+/* ------------------------------------------------------------------------ */
+
+double orc_triangle_area(const double t[6]) {
+  double ux = t[4] - t[0], uy = t[5] - t[1];
+  double vx = t[2] - t[0], vy = t[3] - t[1];
+  return 0.5 * fabs(uy * vx - ux * vy);
+}
+
+double orc_tetrahedron_volume_real(const double t[12]) {
+  double e1[3], e2[3], e3[3];
+  for (int i = 0; i < 3; i++) {
+    e1[i] = t[3 + i] - t[i];
+    e2[i] = t[6 + i] - t[i];
+    e3[i] = t[9 + i] - t[i];
+  }
+  const double sixth = 1.0 / 6.0;
+  double d = e1[0] * (e2[1] * e3[2] - e2[2] * e3[1]);
+  d = d + e1[1] * (e2[2] * e3[0] - e2[0] * e3[2]);
+  d = d + e1[2] * (e2[0] * e3[1] - e2[1] * e3[0]);
+  return sixth * fabs(d);
+}
+
+double orc_tetrahedron_volume_tet(const double t[12]) {
+  double a[3], b[3], c[3], cr[3];
+  for (int i = 0; i < 3; i++) {
+    a[i] = t[i] - t[9 + i];
+    b[i] = t[3 + i] - t[9 + i];
+    c[i] = t[6 + i] - t[9 + i];
+  }
+  cr[0] = b[1] * c[2] - b[2] * c[1];
+  cr[1] = b[2] * c[0] - b[0] * c[2];
+  cr[2] = b[0] * c[1] - b[1] * c[0];
+  return fabs(orc_dot3(a, cr)) / 6.0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Triangle clipping (Sutherland-Hodgman, then a fan)                        */
+/* ------------------------------------------------------------------------ */
+
+typedef struct { double n[2], p[2]; } orc_line;
+
+static void get_lines_tri(const double t[6], orc_line L[3]) {
+  for (int k = 0; k < 3; k++) {
+    int k1 = (k + 1) % 3;
+    L[k].n[0] = -(t[2 * k1 + 1] - t[2 * k + 1]);
+    L[k].n[1] = (t[2 * k1] - t[2 * k]);
+    L[k].p[0] = t[2 * k];
+    L[k].p[1] = t[2 * k + 1];
+  }
+  double d = (t[2] - t[0]) * L[2].n[0];
+  d = d + (t[3] - t[1]) * L[2].n[1];
+  if (d > 0.0) {
+    for (int k = 0; k < 3; k++) {
+      L[k].n[0] = -L[k].n[0];
+      L[k].n[1] = -L[k].n[1];
+    }
+  }
+}
+
+#define ORC_PTS (ORC_TRI_BUF_SIZE + 2)
+
+/* One Sutherland-Hodgman pass (initial and terminal roles swapped, as in the
+ * reference); returns the new point count. */
+static int clip_line(const orc_line *L, const double *pts, int n, double *out) {
+  double d[ORC_PTS];
+  for (int i = 0; i < n; i++) {
+    double s = L->n[0] * (pts[2 * i] - L->p[0]);
+    s = s + L->n[1] * (pts[2 * i + 1] - L->p[1]);
+    d[i] = s;
+  }
+  int m = 0;
+  for (int i = 0; i < n; i++) {
+    const double *p1 = pts + 2 * i;
+    const double *p2 = (i == n - 1) ? pts : pts + 2 * (i + 1);
+    double d1 = d[i], d2 = (i == n - 1) ? d[0] : d[i + 1];
+    if (d1 < 0.0) {
+      out[2 * m] = p1[0];
+      out[2 * m + 1] = p1[1];
+      m++;
+      if (!(d2 <= 0.0)) {
+        double f = fabs(d1) / (fabs(d1) + fabs(d2));
+        f = fmax(fmin(f, 1.0), 0.0);
+        out[2 * m] = p1[0] + f * (p2[0] - p1[0]);
+        out[2 * m + 1] = p1[1] + f * (p2[1] - p1[1]);
+        m++;
+      }
+    } else if (d1 == 0.0) {
+      out[2 * m] = p1[0];
+      out[2 * m + 1] = p1[1];
+      m++;
+    } else if (d2 < 0.0) {
+      double f = fabs(d1) / (fabs(d1) + fabs(d2));
+      f = fmax(fmin(f, 1.0), 0.0);
+      out[2 * m] = p1[0] + f * (p2[0] - p1[0]);
+      out[2 * m + 1] = p1[1] + f * (p2[1] - p1[1]);
+      m++;
+    }
+  }
+  return m;
+}
+
+int orc_intersect_tris(const double tri_a[6], const double tri_b[6], double *tris_c) {
+  orc_line L[3];
+  get_lines_tri(tri_b, L);
+  double buf0[2 * ORC_PTS], buf1[2 * ORC_PTS];
+  memcpy(buf0, tri_a, 6 * sizeof(double));
+  int n = clip_line(&L[0], buf0, 3, buf1);
+  if (n < 3) return 0;
+  n = clip_line(&L[1], buf1, n, buf0);
+  if (n < 3) return 0;
+  n = clip_line(&L[2], buf0, n, buf1);
+  if (n < 3) return 0;
+  double tol = 10.0 * fmin(orc_spacing(orc_triangle_area(tri_a)),
+                           orc_spacing(orc_triangle_area(tri_b)));
+  int nc = 0;
+  for (int i = 0; i < n - 2; i++) {
+    double *t = tris_c + 6 * nc;
+    t[0] = buf1[0];
+    t[1] = buf1[1];
+    t[2] = buf1[2 * (i + 1)];
+    t[3] = buf1[2 * (i + 1) + 1];
+    t[4] = buf1[2 * (i + 2)];
+    t[5] = buf1[2 * (i + 2) + 1];
+    if (!(orc_triangle_area(t) < tol)) nc++;
+  }
+  return nc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Tetrahedron clipping (plane at a time, breadth first)                     */
+/* ------------------------------------------------------------------------ */
+
+static void unit_cross(const double a[3], const double b[3], double out[3]) {
+  double c[3];
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+  double n = orc_norm2_3(c);
+  out[0] = c[0] / n;
+  out[1] = c[1] / n;
+  out[2] = c[2] / n;
+}
+
+void orc_get_planes_tet(const double t[12], double nrm[12], double c[4]) {
+  double e10[3], e20[3], e30[3], e21[3], e31[3];
+  for (int i = 0; i < 3; i++) {
+    e10[i] = t[3 + i] - t[i];
+    e20[i] = t[6 + i] - t[i];
+    e30[i] = t[9 + i] - t[i];
+    e21[i] = t[6 + i] - t[3 + i];
+    e31[i] = t[9 + i] - t[3 + i];
+  }
+  unit_cross(e20, e10, nrm + 0);
+  unit_cross(e10, e30, nrm + 3);
+  unit_cross(e30, e20, nrm + 6);
+  unit_cross(e21, e31, nrm + 9);
+  double det = orc_dot3(e10, nrm + 9);
+  if (det < 0.0) {
+    for (int i = 0; i < 12; i++) nrm[i] = -nrm[i];
+  }
+  for (int i = 0; i < 4; i++) c[i] = orc_dot3(t + 3 * i, nrm + 3 * i);
+}
+
+/* p = w0*v[ip] + w1*v[in], with the reference's weights for vertices ip
+ * (outside, d > 0) and in (inside, d < 0). */
+static void split_point(const double *v, const double *d, int ip, int in, double *p) {
+  double invdiff = 1.0 / (d[ip] - d[in]);
+  double w0 = -d[in] * invdiff;
+  double w1 = d[ip] * invdiff;
+  for (int k = 0; k < 3; k++) p[k] = w0 * v[3 * ip + k] + w1 * v[3 * in + k];
+}
+
+static void copy3(double *dst, const double *src) {
+  dst[0] = src[0];
+  dst[1] = src[1];
+  dst[2] = src[2];
+}
+
+/* Clip tet v against (n, c) and append 0..3 tets to out; returns new count. */
+static int clip_tet_plane(const double *n, double c, const double *v, double *out, int nout) {
+  double d[4];
+  int neg[4], pos[4], zer[4], nn = 0, np = 0, nz = 0;
+  for (int i = 0; i < 4; i++) {
+    d[i] = orc_dot3(n, v + 3 * i) - c;
+    if (fabs(d[i]) < DBL_EPSILON) zer[nz++] = i;
+    else if (d[i] < 0.0) neg[nn++] = i;
+    else if (d[i] > 0.0) pos[np++] = i;
+  }
+  if (nn == 0) return nout;
+  double *t = out + 12 * nout;
+  if (np == 0) {
+    memcpy(t, v, 12 * sizeof(double));
+    return nout + 1;
+  }
+  double tmp[4][3];
+  if (np == 3) {
+    memcpy(t, v, 12 * sizeof(double));
+    for (int i = 0; i < 3; i++) split_point(v, d, pos[i], neg[0], t + 3 * pos[i]);
+    return nout + 1;
+  } else if (np == 2) {
+    if (nn == 2) {
+      split_point(v, d, pos[0], neg[0], tmp[0]);
+      split_point(v, d, pos[1], neg[0], tmp[1]);
+      split_point(v, d, pos[0], neg[1], tmp[2]);
+      split_point(v, d, pos[1], neg[1], tmp[3]);
+      memcpy(t, v, 12 * sizeof(double));
+      copy3(t + 3 * pos[0], tmp[2]);
+      copy3(t + 3 * pos[1], tmp[1]);
+      t += 12;
+      copy3(t + 0, v + 3 * neg[1]);
+      copy3(t + 3, tmp[3]);
+      copy3(t + 6, tmp[2]);
+      copy3(t + 9, tmp[1]);
+      t += 12;
+      copy3(t + 0, v + 3 * neg[0]);
+      copy3(t + 3, tmp[0]);
+      copy3(t + 6, tmp[1]);
+      copy3(t + 9, tmp[2]);
+      return nout + 3;
+    } else if (nn == 1) {
+      memcpy(t, v, 12 * sizeof(double));
+      for (int i = 0; i < 2; i++) split_point(v, d, pos[i], neg[0], t + 3 * pos[i]);
+      return nout + 1;
+    }
+  } else if (np == 1) {
+    if (nn == 3) {
+      for (int i = 0; i < 3; i++) split_point(v, d, pos[0], neg[i], tmp[i]);
+      memcpy(t, v, 12 * sizeof(double));
+      copy3(t + 3 * pos[0], tmp[0]);
+      t += 12;
+      copy3(t + 0, tmp[0]);
+      copy3(t + 3, v + 3 * neg[1]);
+      copy3(t + 6, v + 3 * neg[2]);
+      copy3(t + 9, tmp[1]);
+      t += 12;
+      copy3(t + 0, v + 3 * neg[2]);
+      copy3(t + 3, tmp[1]);
+      copy3(t + 6, tmp[2]);
+      copy3(t + 9, tmp[0]);
+      return nout + 3;
+    } else if (nn == 2) {
+      for (int i = 0; i < 2; i++) split_point(v, d, pos[0], neg[i], tmp[i]);
+      memcpy(t, v, 12 * sizeof(double));
+      copy3(t + 3 * pos[0], tmp[0]);
+      t += 12;
+      copy3(t + 0, tmp[1]);
+      copy3(t + 3, v + 3 * zer[0]);
+      copy3(t + 6, v + 3 * neg[1]);
+      copy3(t + 9, tmp[0]);
+      return nout + 2;
+    } else if (nn == 1) {
+      memcpy(t, v, 12 * sizeof(double));
+      split_point(v, d, pos[0], neg[0], t + 3 * pos[0]);
+      return nout + 1;
+    }
+  }
+  return nout;
+}
+
+int orc_intersect_tets(const double tet_a[12], const double tet_b[12], double *tets_c) {
+  double nrm[12], c[4];
+  orc_get_planes_tet(tet_b, nrm, c);
+  double vol_b = orc_tetrahedron_volume_tet(tet_b);
+  double tol = 10.0 * fmin(orc_spacing(orc_tetrahedron_volume_tet(tet_a)), orc_spacing(vol_b));
+  double cur[ORC_TET_BUF_SIZE * 12], nxt[ORC_TET_BUF_SIZE * 12];
+  memcpy(cur, tet_a, 12 * sizeof(double));
+  int n = 1;
+  for (int i = 0; i < 4; i++) {
+    int m = 0;
+    for (int j = 0; j < n; j++) m = clip_tet_plane(nrm + 3 * i, c[i], cur + 12 * j, nxt, m);
+    if (i != 3) {
+      memcpy(cur, nxt, (size_t)m * 12 * sizeof(double));
+      n = m;
+    } else {
+      n = 0;
+      for (int j = 0; j < m; j++) {
+        if (orc_tetrahedron_volume_tet(nxt + 12 * j) >= tol) {
+          memcpy(tets_c + 12 * n, nxt + 12 * j, 12 * sizeof(double));
+          n++;
+        }
+      }
+    }
+  }
+  return n;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Kahan sum                                                                 */
+/* ------------------------------------------------------------------------ */
+
+void orc_ksum_init(orc_ksum *cs) { cs->sum = 0.0; cs->error = 0.0; }
+
+void orc_ksum_add(orc_ksum *cs, double x) {
+  double cx = cs->error + x;
+  double s = cs->sum + cx;
+  cs->error = cs->sum - s;
+  cs->error = cs->error + cx;
+  cs->sum = s;
+}
+
+double orc_ksum_value(const orc_ksum *cs) { return cs->sum + cs->error; }
+
+/* ------------------------------------------------------------------------ */
+/* Bounding boxes and finders                                                */
+/* ------------------------------------------------------------------------ */
+
+/* bbox[0..dim-1] = min, bbox[3..3+dim-1] = max (exact min/max). */
+static void element_bbox(int dim, int loc, const double *pos, const int *nodes, double *bb) {
+  for (int d = 0; d < dim; d++) {
+    double x = pos[(long)nodes[0] * dim + d];
+    bb[d] = x;
+    bb[3 + d] = x;
+  }
+  for (int j = 1; j < loc; j++) {
+    for (int d = 0; d < dim; d++) {
+      double x = pos[(long)nodes[j] * dim + d];
+      bb[d] = fmin(bb[d], x);
+      bb[3 + d] = fmax(bb[3 + d], x);
+    }
+  }
+}
+
+static void coords_bbox(int dim, int loc, const double *coords, double *bb) {
+  for (int d = 0; d < dim; d++) {
+    bb[d] = coords[d];
+    bb[3 + d] = coords[d];
+  }
+  for (int j = 1; j < loc; j++) {
+    for (int d = 0; d < dim; d++) {
+      double x = coords[j * dim + d];
+      bb[d] = fmin(bb[d], x);
+      bb[3 + d] = fmax(bb[3 + d], x);
+    }
+  }
+}
+
+static int boxes_strict(int dim, const double *a, const double *b) {
+  for (int d = 0; d < dim; d++)
+    if (!(b[3 + d] > a[d] && b[d] < a[3 + d])) return 0;
+  return 1;
+}
+
+static int boxes_closed(int dim, const double *a, const double *b) {
+  for (int d = 0; d < dim; d++)
+    if (!(b[3 + d] >= a[d] && b[d] <= a[3 + d])) return 0;
+  return 1;
+}
+
+long orc_brute_force_finder(int dim, int closed,
+                            long ne_a, int loc_a, const double *pos_a, const int *enl_a,
+                            long ne_b, int loc_b, const double *pos_b, const int *enl_b,
+                            long *indptr, int *indices) {
+  double *bbs = (double *)malloc(sizeof(double) * 6 * (size_t)(ne_b > 0 ? ne_b : 1));
+  for (long b = 0; b < ne_b; b++) element_bbox(dim, loc_b, pos_b, enl_b + b * loc_b, bbs + 6 * b);
+  long nnz = 0;
+  indptr[0] = 0;
+  for (long a = 0; a < ne_a; a++) {
+    double ba[6];
+    element_bbox(dim, loc_a, pos_a, enl_a + a * loc_a, ba);
+    for (long b = 0; b < ne_b; b++) {
+      int hit = closed ? boxes_closed(dim, ba, bbs + 6 * b) : boxes_strict(dim, ba, bbs + 6 * b);
+      if (hit) {
+        if (indices) indices[nnz] = (int)b;
+        nnz++;
+      }
+    }
+    indptr[a + 1] = nnz;
+  }
+  free(bbs);
+  return nnz;
+}
+
+/* Bucket quadtree / octree.  A leaf holds at most max_n elements and splits
+ * into 2**dim children at the midpoint when a further element arrives.
+ * Elements are inserted into every node whose box they strictly overlap. */
+typedef struct {
+  int n;          /* element count; INT32_MAX marks an internal node */
+  int child0;     /* index of the first child (internal nodes) */
+  long slot0;     /* first element slot (leaves) */
+  double bb[6];
+} orc_node;
+
+struct orc_tree {
+  int dim, nchild, max_n;
+  orc_node *nodes;
+  long nnodes, cap_nodes;
+  int *slot_ele;
+  double *slot_bb;
+  long nslots, cap_slots;
+};
+
+static long tree_new_slots(orc_tree *t) {
+  if (t->nslots + t->max_n > t->cap_slots) {
+    long nc = t->cap_slots * 2 + t->max_n;
+    t->slot_ele = (int *)realloc(t->slot_ele, sizeof(int) * (size_t)nc);
+    t->slot_bb = (double *)realloc(t->slot_bb, sizeof(double) * 6 * (size_t)nc);
+    t->cap_slots = nc;
+  }
+  long s = t->nslots;
+  t->nslots += t->max_n;
+  return s;
+}
+
+static int tree_new_nodes(orc_tree *t, int count) {
+  if (t->nnodes + count > t->cap_nodes) {
+    long nc = t->cap_nodes * 2 + count;
+    t->nodes = (orc_node *)realloc(t->nodes, sizeof(orc_node) * (size_t)nc);
+    t->cap_nodes = nc;
+  }
+  int first = (int)t->nnodes;
+  t->nnodes += count;
+  return first;
+}
+
+/* Child boxes in the reference's order (Quadtree_...F90:289-296,
+ * Octree_...F90:274-289): low corner selectors per child. */
+static void child_box(int dim, int k, const double *pb, const double *mid, double *cb) {
+  static const int qx[4] = {0, 0, 1, 1}, qy[4] = {0, 1, 1, 0};
+  static const int ox[8] = {0, 0, 1, 1, 0, 0, 1, 1}, oy[8] = {0, 1, 1, 0, 0, 1, 1, 0},
+                   oz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+  int sel[3];
+  if (dim == 2) {
+    sel[0] = qx[k];
+    sel[1] = qy[k];
+  } else {
+    sel[0] = ox[k];
+    sel[1] = oy[k];
+    sel[2] = oz[k];
+  }
+  for (int d = 0; d < dim; d++) {
+    if (sel[d]) {
+      cb[d] = mid[d];
+      cb[3 + d] = pb[3 + d];
+    } else {
+      cb[d] = pb[d];
+      cb[3 + d] = mid[d];
+    }
+  }
+}
+
+static void tree_add(orc_tree *t, int node, int ele, const double *bb) {
+  if (!boxes_strict(t->dim, t->nodes[node].bb, bb)) return;
+  orc_node *nd = &t->nodes[node];
+  if (nd->n <= t->max_n - 1) {
+    long s = nd->slot0 + nd->n;
+    t->slot_ele[s] = ele;
+    memcpy(t->slot_bb + 6 * s, bb, 6 * sizeof(double));
+    nd->n++;
+  } else if (nd->n == t->max_n) {
+    int c0 = tree_new_nodes(t, t->nchild);
+    nd = &t->nodes[node];
+    double mid[3];
+    for (int d = 0; d < t->dim; d++) mid[d] = 0.5 * (nd->bb[d] + nd->bb[3 + d]);
+    for (int k = 0; k < t->nchild; k++) {
+      orc_node *ch = &t->nodes[c0 + k];
+      ch->n = 0;
+      ch->child0 = -1;
+      memset(ch->bb, 0, sizeof(ch->bb));
+      child_box(t->dim, k, t->nodes[node].bb, mid, ch->bb);
+      long s = tree_new_slots(t);
+      t->nodes[c0 + k].slot0 = s;
+    }
+    long parent_slot0 = t->nodes[node].slot0;
+    t->nodes[node].child0 = c0;
+    t->nodes[node].n = INT32_MAX;
+    for (int i = 0; i < t->max_n; i++) {
+      int e = t->slot_ele[parent_slot0 + i];
+      double eb[6];
+      memcpy(eb, t->slot_bb + 6 * (parent_slot0 + i), sizeof(eb));
+      for (int k = 0; k < t->nchild; k++) tree_add(t, c0 + k, e, eb);
+    }
+    for (int k = 0; k < t->nchild; k++) tree_add(t, c0 + k, ele, bb);
+  } else {
+    int c0 = nd->child0;
+    for (int k = 0; k < t->nchild; k++) tree_add(t, c0 + k, ele, bb);
+  }
+}
+
+orc_tree *orc_tree_build(int dim, long nnodes_b, const double *pos_b,
+                         long ne_b, int loc_b, const int *enl_b) {
+  orc_tree *t = (orc_tree *)calloc(1, sizeof(orc_tree));
+  t->dim = dim;
+  t->nchild = 1 << dim;
+  /* max_n = max(max node degree, 256) (Quadtree_...F90:88-108,198-204) */
+  int *deg = (int *)calloc((size_t)(nnodes_b > 0 ? nnodes_b : 1), sizeof(int));
+  int maxdeg = 0;
+  for (long e = 0; e < ne_b; e++)
+    for (int j = 0; j < loc_b; j++) {
+      int v = enl_b[e * loc_b + j];
+      if (++deg[v] > maxdeg) maxdeg = deg[v];
+    }
+  free(deg);
+  t->max_n = maxdeg > 256 ? maxdeg : 256;
+  int root = tree_new_nodes(t, 1);
+  orc_node *r = &t->nodes[root];
+  r->n = 0;
+  r->child0 = -1;
+  memset(r->bb, 0, sizeof(r->bb));
+  for (int d = 0; d < dim; d++) {
+    r->bb[d] = pos_b[d];
+    r->bb[3 + d] = pos_b[d];
+  }
+  for (long v = 1; v < nnodes_b; v++)
+    for (int d = 0; d < dim; d++) {
+      r->bb[d] = fmin(r->bb[d], pos_b[v * dim + d]);
+      r->bb[3 + d] = fmax(r->bb[3 + d], pos_b[v * dim + d]);
+    }
+  long s = tree_new_slots(t);
+  t->nodes[root].slot0 = s;
+  for (long e = 0; e < ne_b; e++) {
+    double bb[6] = {0, 0, 0, 0, 0, 0};
+    element_bbox(dim, loc_b, pos_b, enl_b + e * loc_b, bb);
+    tree_add(t, root, (int)e, bb);
+  }
+  return t;
+}
+
+void orc_tree_free(orc_tree *t) {
+  if (!t) return;
+  free(t->nodes);
+  free(t->slot_ele);
+  free(t->slot_bb);
+  free(t);
+}
+
+static void tree_query_node(const orc_tree *t, int node, const double *qb,
+                            unsigned char *seen, int *out, int *nout) {
+  const orc_node *nd = &t->nodes[node];
+  if (nd->n == 0) return;
+  if (!boxes_strict(t->dim, nd->bb, qb)) return;
+  if (nd->n <= t->max_n) {
+    for (int i = 0; i < nd->n; i++) {
+      long s = nd->slot0 + i;
+      int e = t->slot_ele[s];
+      if (!seen[e] && boxes_strict(t->dim, t->slot_bb + 6 * s, qb)) {
+        out[(*nout)++] = e;
+        seen[e] = 1;
+      }
+    }
+  } else {
+    for (int k = 0; k < t->nchild; k++) tree_query_node(t, nd->child0 + k, qb, seen, out, nout);
+  }
+}
+
+int orc_tree_query(const orc_tree *t, int loc_a, const double *coords_a,
+                   unsigned char *seen, int *out) {
+  double qb[6] = {0, 0, 0, 0, 0, 0};
+  coords_bbox(t->dim, loc_a, coords_a, qb);
+  int n = 0;
+  tree_query_node(t, 0, qb, seen, out, &n);
+  for (int i = 0; i < n; i++) seen[out[i]] = 0;
+  return n;
+}
+
+long orc_tree_finder(int dim, long nnodes_b,
+                     long ne_a, int loc_a, const double *pos_a, const int *enl_a,
+                     long ne_b, int loc_b, const double *pos_b, const int *enl_b,
+                     long *indptr, int *indices) {
+  indptr[0] = 0;
+  if (ne_a == 0 || ne_b == 0) {
+    for (long a = 0; a < ne_a; a++) indptr[a + 1] = 0;
+    return 0;
+  }
+  orc_tree *t = orc_tree_build(dim, nnodes_b, pos_b, ne_b, loc_b, enl_b);
+  unsigned char *seen = (unsigned char *)calloc((size_t)ne_b, 1);
+  int *buf = (int *)malloc(sizeof(int) * (size_t)ne_b);
+  long nnz = 0;
+  double coords[4 * 3];
+  for (long a = 0; a < ne_a; a++) {
+    for (int j = 0; j < loc_a; j++)
+      for (int d = 0; d < dim; d++) coords[j * dim + d] = pos_a[(long)enl_a[a * loc_a + j] * dim + d];
+    int n = orc_tree_query(t, loc_a, coords, seen, buf);
+    if (indices) memcpy(indices + nnz, buf, sizeof(int) * (size_t)n);
+    nnz += n;
+    indptr[a + 1] = nnz;
+  }
+  free(buf);
+  free(seen);
+  orc_tree_free(t);
+  return nnz;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Pair clipping, mass and drivers                                           */
+/* ------------------------------------------------------------------------ */
+
+static void gather(int dim, const double *pos, const int *enl, long e, double *out) {
+  int loc = dim + 1;
+  for (int j = 0; j < loc; j++)
+    for (int d = 0; d < dim; d++) out[j * dim + d] = pos[(long)enl[e * loc + j] * dim + d];
+}
+
+static double measure_real(int dim, const double *el) {
+  return dim == 2 ? orc_triangle_area(el) : orc_tetrahedron_volume_real(el);
+}
+
+static int clip_pair(int dim, const double *ea, const double *eb, double *buf) {
+  return dim == 2 ? orc_intersect_tris(ea, eb, buf) : orc_intersect_tets(ea, eb, buf);
+}
+
+typedef struct {
+  int dim;
+  long p0, p1;
+  const int *pair_a, *pair_b;
+  const double *pos_a, *pos_b;
+  const int *enl_a, *enl_b;
+  int *counts;
+  double *measure;
+  int cap;
+  double *elems;
+  double *blocks;
+} clip_job;
+
+static void *clip_worker(void *arg) {
+  clip_job *j = (clip_job *)arg;
+  int dim = j->dim, esz = dim == 2 ? 6 : 12;
+  double ea[12], eb[12];
+  double buf[ORC_TET_BUF_SIZE * 12];
+  for (long p = j->p0; p < j->p1; p++) {
+    gather(dim, j->pos_a, j->enl_a, j->pair_a[p], ea);
+    gather(dim, j->pos_b, j->enl_b, j->pair_b[p], eb);
+    int n = clip_pair(dim, ea, eb, buf);
+    if (j->counts) j->counts[p] = n;
+    if (j->measure) {
+      double s = 0.0;
+      for (int k = 0; k < n; k++) s = s + measure_real(dim, buf + esz * k);
+      j->measure[p] = s;
+    }
+    if (j->elems) {
+      int m = n < j->cap ? n : j->cap;
+      memcpy(j->elems + (size_t)p * j->cap * esz, buf, sizeof(double) * (size_t)m * esz);
+    }
+  }
+  return NULL;
+}
+
+static void run_jobs(clip_job *base, long npairs, int nthreads, void *(*fn)(void *)) {
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 256) nthreads = 256;
+  pthread_t th[256];
+  clip_job jobs[256];
+  for (int t = 0; t < nthreads; t++) {
+    jobs[t] = *base;
+    jobs[t].p0 = npairs * t / nthreads;
+    jobs[t].p1 = npairs * (t + 1) / nthreads;
+  }
+  if (nthreads == 1) {
+    fn(&jobs[0]);
+    return;
+  }
+  for (int t = 0; t < nthreads; t++) pthread_create(&th[t], NULL, fn, &jobs[t]);
+  for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+}
+
+void orc_clip_pairs(int dim, long npairs, const int *pair_a, const int *pair_b,
+                    const double *pos_a, const int *enl_a,
+                    const double *pos_b, const int *enl_b,
+                    int *counts, double *measure, int cap, double *elems,
+                    int nthreads) {
+  clip_job base;
+  memset(&base, 0, sizeof(base));
+  base.dim = dim;
+  base.pair_a = pair_a;
+  base.pair_b = pair_b;
+  base.pos_a = pos_a;
+  base.pos_b = pos_b;
+  base.enl_a = enl_a;
+  base.enl_b = enl_b;
+  base.counts = counts;
+  base.measure = measure;
+  base.cap = cap;
+  base.elems = elems;
+  run_jobs(&base, npairs, nthreads, clip_worker);
+}
+
+/* P1 basis functions of a triangle at x: inverse-Jacobian closed form
+ * (test_parallel_p1_inner_product_2d.F90:205-228). */
+static void basis_p1_2d(const double *c, const double *x, double *fns) {
+  double e1x = c[2] - c[0], e1y = c[3] - c[1];
+  double e2x = c[4] - c[0], e2y = c[5] - c[1];
+  double j11 = e2y, j12 = -e2x, j21 = -e1y, j22 = e1x;
+  double det = j11 * j22 - j12 * j21;
+  j11 = j11 / det;
+  j12 = j12 / det;
+  j21 = j21 / det;
+  j22 = j22 / det;
+  double dx = x[0] - c[0], dy = x[1] - c[1];
+  double f2 = j11 * dx;
+  f2 = f2 + j12 * dy;
+  double f3 = j21 * dx;
+  f3 = f3 + j22 * dy;
+  fns[1] = f2;
+  fns[2] = f3;
+  fns[0] = 1.0 - f2 - f3;
+}
+
+/* P1 basis functions of a tet at x: Gaussian elimination with partial
+ * pivoting on [edges | x - c1] (test_parallel_p1_inner_product_3d.F90:227-265). */
+static void basis_p1_3d(const double *c, const double *x, double *fns) {
+  double m[3][4];
+  for (int r = 0; r < 3; r++) {
+    for (int k = 0; k < 3; k++) m[r][k] = c[3 * (k + 1) + r] - c[r];
+    m[r][3] = x[r] - c[r];
+  }
+  for (int i = 0; i < 3; i++) {
+    int j = i;
+    double best = fabs(m[i][i]);
+    for (int r = i + 1; r < 3; r++)
+      if (fabs(m[r][i]) > best) {
+        best = fabs(m[r][i]);
+        j = r;
+      }
+    if (j != i)
+      for (int k = 0; k < 4; k++) {
+        double tt = m[i][k];
+        m[i][k] = m[j][k];
+        m[j][k] = tt;
+      }
+    double piv = m[i][i];
+    for (int k = 0; k < 4; k++) m[i][k] = m[i][k] / piv;
+    for (int r = i + 1; r < 3; r++) {
+      double f = m[r][i];
+      for (int k = 0; k < 4; k++) m[r][k] = m[r][k] - f * m[i][k];
+    }
+  }
+  for (int i = 2; i >= 0; i--)
+    for (int r = i - 1; r >= 0; r--) {
+      double f = m[r][i];
+      for (int k = 0; k < 4; k++) m[r][k] = m[r][k] - f * m[i][k];
+    }
+  fns[1] = m[0][3];
+  fns[2] = m[1][3];
+  fns[3] = m[2][3];
+  fns[0] = 1.0 - fns[1] - fns[2] - fns[3];
+}
+
+/* Quadrature rules of the reference tests: Dunavant degree 2 (3 points,
+ * barycentric (4,1,1)/6, weights 1/3) and Stroud degree 2 (4 points). */
+static const double kQ2[3][3] = {{4.0 / 6.0, 1.0 / 6.0, 1.0 / 6.0},
+                                 {1.0 / 6.0, 4.0 / 6.0, 1.0 / 6.0},
+                                 {1.0 / 6.0, 1.0 / 6.0, 4.0 / 6.0}};
+#define QA 5.8541019662496852e-1
+#define QB 1.3819660112501050e-1
+static const double kQ3[4][4] = {{QA, QB, QB, QB}, {QB, QA, QB, QB}, {QB, QB, QA, QB}, {QB, QB, QB, QA}};
+
+static void quad_point(int dim, int q, const double *el, double *x) {
+  int loc = dim + 1;
+  for (int d = 0; d < dim; d++) {
+    double s = 0.0;
+    for (int v = 0; v < loc; v++) {
+      double w = dim == 2 ? kQ2[q][v] : kQ3[q][v];
+      s = s + w * el[v * dim + d];
+    }
+    x[d] = s;
+  }
+}
+
+static void *mass_worker(void *arg) {
+  clip_job *j = (clip_job *)arg;
+  int dim = j->dim, loc = dim + 1, esz = dim == 2 ? 6 : 12, nq = dim == 2 ? 3 : 4;
+  double wq = dim == 2 ? 1.0 / 3.0 : 1.0 / 4.0;
+  double ea[12], eb[12], buf[ORC_TET_BUF_SIZE * 12];
+  for (long p = j->p0; p < j->p1; p++) {
+    gather(dim, j->pos_a, j->enl_a, j->pair_a[p], ea);
+    gather(dim, j->pos_b, j->enl_b, j->pair_b[p], eb);
+    int n = clip_pair(dim, ea, eb, buf);
+    double *blk = j->blocks + (size_t)p * loc * loc;
+    for (int k = 0; k < loc * loc; k++) blk[k] = 0.0;
+    for (int c = 0; c < n; c++) {
+      const double *el = buf + esz * c;
+      double meas = measure_real(dim, el);
+      for (int q = 0; q < nq; q++) {
+        double x[3], fa[4], fb[4];
+        quad_point(dim, q, el, x);
+        if (dim == 2) {
+          basis_p1_2d(ea, x, fa);
+          basis_p1_2d(eb, x, fb);
+        } else {
+          basis_p1_3d(ea, x, fa);
+          basis_p1_3d(eb, x, fb);
+        }
+        double wm = wq * meas;
+        for (int a = 0; a < loc; a++)
+          for (int b = 0; b < loc; b++) blk[a * loc + b] = blk[a * loc + b] + wm * fa[a] * fb[b];
+      }
+    }
+  }
+  return NULL;
+}
+
+void orc_mass_pairs(int dim, long npairs, const int *pair_a, const int *pair_b,
+                    const double *pos_a, const int *enl_a,
+                    const double *pos_b, const int *enl_b,
+                    double *blocks, int nthreads) {
+  clip_job base;
+  memset(&base, 0, sizeof(base));
+  base.dim = dim;
+  base.pair_a = pair_a;
+  base.pair_b = pair_b;
+  base.pos_a = pos_a;
+  base.pos_b = pos_b;
+  base.enl_a = enl_a;
+  base.enl_b = enl_b;
+  base.blocks = blocks;
+  run_jobs(&base, npairs, nthreads, mass_worker);
+}
+
+void orc_inner_product(int dim, long npairs, const int *pair_a, const int *pair_b,
+                       const double *pos_a, const int *enl_a, const double *field_a,
+                       const double *pos_b, const int *enl_b, const double *field_b,
+                       double *measure_out, double *integral_out) {
+  int loc = dim + 1, esz = dim == 2 ? 6 : 12, nq = dim == 2 ? 3 : 4;
+  double wq = dim == 2 ? 1.0 / 3.0 : 1.0 / 4.0;
+  double ea[12], eb[12], buf[ORC_TET_BUF_SIZE * 12];
+  double area = 0.0, integral = 0.0;
+  orc_ksum kv, ki;
+  orc_ksum_init(&kv);
+  orc_ksum_init(&ki);
+  for (long p = 0; p < npairs; p++) {
+    long a = pair_a[p], b = pair_b[p];
+    gather(dim, pos_a, enl_a, a, ea);
+    gather(dim, pos_b, enl_b, b, eb);
+    int n = clip_pair(dim, ea, eb, buf);
+    for (int c = 0; c < n; c++) {
+      const double *el = buf + esz * c;
+      double meas = measure_real(dim, el);
+      if (dim == 2) area = area + meas;
+      else orc_ksum_add(&kv, meas);
+      for (int q = 0; q < nq; q++) {
+        double x[3], fa[4], fb[4];
+        quad_point(dim, q, el, x);
+        if (dim == 2) {
+          basis_p1_2d(eb, x, fb);
+          basis_p1_2d(ea, x, fa);
+        } else {
+          basis_p1_3d(eb, x, fb);
+          basis_p1_3d(ea, x, fa);
+        }
+        double ib = 0.0, ia = 0.0;
+        for (int v = 0; v < loc; v++) ib = ib + fb[v] * field_b[enl_b[b * loc + v]];
+        for (int v = 0; v < loc; v++) ia = ia + fa[v] * field_a[enl_a[a * loc + v]];
+        double term = wq * meas * ib * ia;
+        if (dim == 2) integral = integral + term;
+        else orc_ksum_add(&ki, term);
+      }
+    }
+  }
+  *measure_out = dim == 2 ? area : orc_ksum_value(&kv);
+  *integral_out = dim == 2 ? integral : orc_ksum_value(&ki);
+}
+
+static double now_s(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+typedef struct {
+  int dim, finder;
+  long a0, a1;
+  const double *pos_a, *pos_b;
+  const int *enl_a, *enl_b;
+  long ne_b;
+  const orc_tree *tree;
+  const double *bbs_b;
+  long long stats[4];
+  orc_ksum sum;
+} sm_job;
+
+static void *sm_worker(void *arg) {
+  sm_job *j = (sm_job *)arg;
+  int dim = j->dim, loc = dim + 1, esz = dim == 2 ? 6 : 12;
+  unsigned char *seen = (unsigned char *)calloc((size_t)(j->ne_b > 0 ? j->ne_b : 1), 1);
+  int *cand = (int *)malloc(sizeof(int) * (size_t)(j->ne_b > 0 ? j->ne_b : 1));
+  double ea[12], eb[12], buf[ORC_TET_BUF_SIZE * 12];
+  orc_ksum_init(&j->sum);
+  memset(j->stats, 0, sizeof(j->stats));
+  for (long a = j->a0; a < j->a1; a++) {
+    gather(dim, j->pos_a, j->enl_a, a, ea);
+    int nc = 0;
+    if (j->finder == 0) {
+      nc = orc_tree_query(j->tree, loc, ea, seen, cand);
+    } else {
+      double ba[6];
+      coords_bbox(dim, loc, ea, ba);
+      for (long b = 0; b < j->ne_b; b++) {
+        int hit = j->finder == 2 ? boxes_closed(dim, ba, j->bbs_b + 6 * b)
+                                 : boxes_strict(dim, ba, j->bbs_b + 6 * b);
+        if (hit) cand[nc++] = (int)b;
+      }
+    }
+    j->stats[0] += nc;
+    for (int i = 0; i < nc; i++) {
+      gather(dim, j->pos_b, j->enl_b, cand[i], eb);
+      int n = clip_pair(dim, ea, eb, buf);
+      if (n > 0) j->stats[1]++;
+      j->stats[2] += n;
+      if (n > j->stats[3]) j->stats[3] = n;
+      for (int k = 0; k < n; k++) orc_ksum_add(&j->sum, measure_real(dim, buf + esz * k));
+    }
+  }
+  free(seen);
+  free(cand);
+  return NULL;
+}
+
+void orc_supermesh(int dim, int finder, int nthreads,
+                   long nnodes_a, long ne_a, const double *pos_a, const int *enl_a,
+                   long nnodes_b, long ne_b, const double *pos_b, const int *enl_b,
+                   long long *stats, double *sum_out, double *times) {
+  (void)nnodes_a;
+  int loc = dim + 1;
+  double t0 = now_s();
+  orc_tree *tree = NULL;
+  double *bbs = NULL;
+  if (finder == 0 && ne_a > 0 && ne_b > 0) {
+    tree = orc_tree_build(dim, nnodes_b, pos_b, ne_b, loc, enl_b);
+  } else if (finder != 0) {
+    bbs = (double *)malloc(sizeof(double) * 6 * (size_t)(ne_b > 0 ? ne_b : 1));
+    for (long b = 0; b < ne_b; b++) element_bbox(dim, loc, pos_b, enl_b + b * loc, bbs + 6 * b);
+  }
+  double t1 = now_s();
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 256) nthreads = 256;
+  sm_job *jobs = (sm_job *)calloc((size_t)nthreads, sizeof(sm_job));
+  pthread_t th[256];
+  for (int t = 0; t < nthreads; t++) {
+    sm_job *j = &jobs[t];
+    j->dim = dim;
+    j->finder = finder;
+    j->a0 = tree || finder != 0 ? ne_a * t / nthreads : 0;
+    j->a1 = tree || finder != 0 ? ne_a * (t + 1) / nthreads : 0;
+    j->pos_a = pos_a;
+    j->pos_b = pos_b;
+    j->enl_a = enl_a;
+    j->enl_b = enl_b;
+    j->ne_b = ne_b;
+    j->tree = tree;
+    j->bbs_b = bbs;
+  }
+  if (nthreads == 1) sm_worker(&jobs[0]);
+  else {
+    for (int t = 0; t < nthreads; t++) pthread_create(&th[t], NULL, sm_worker, &jobs[t]);
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+  }
+  double t2 = now_s();
+  memset(stats, 0, 4 * sizeof(long long));
+  orc_ksum total;
+  orc_ksum_init(&total);
+  for (int t = 0; t < nthreads; t++) {
+    for (int k = 0; k < 3; k++) stats[k] += jobs[t].stats[k];
+    if (jobs[t].stats[3] > stats[3]) stats[3] = jobs[t].stats[3];
+    orc_ksum_add(&total, jobs[t].sum.sum);
+    orc_ksum_add(&total, jobs[t].sum.error);
+  }
+  *sum_out = orc_ksum_value(&total);
+  times[0] = t1 - t0;
+  times[1] = t2 - t1;
+  free(jobs);
+  free(bbs);
+  orc_tree_free(tree);
+}

@@ -1,0 +1,122 @@
+/*
+ * oracle.h -- CPU restatement of libsupermesh's supermeshing hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker the GPU library is compared
+ * against, and the CPU baseline timed beside it.  Only tests/, the smoke check
+ * in __graft_entry__.py and the cpu_baseline leg of bench.py may load it.  The
+ * product library (libsupermesh_amd) never links, loads or calls it.
+ *
+ * Layouts follow the reference's C ABI (include/libsupermesh-c.h):
+ *   positions  double[nnodes][dim]      (Fortran dim x nnodes, column-major)
+ *   enlist     int[nelements][loc]      0-based here (the C ABI passes 0-based
+ *                                       long; the Fortran core uses 1-based)
+ *   tri        double[3][2], tet double[4][3]
+ *
+ * Every routine is re-entrant: the reference keeps its scratch in module-level
+ * `save` variables (src/Triangle_Intersection.F90:101-102,
+ * src/Tetrahedron_Intersection.F90:102-103); here all scratch is on the stack
+ * or owned by the caller, so the driver may run one thread per core.
+ *
+ * Parity status of each function (what pins it) is listed in oracle.c's
+ * header comment.
+ */
+#ifndef LIBSUPERMESH_ORACLE_H
+#define LIBSUPERMESH_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_TRI_BUF_SIZE 22
+#define ORC_TET_BUF_SIZE 81
+
+/* Fortran intrinsics as the reference's compiler evaluates them. */
+double orc_spacing(double x);
+double orc_epsilon(void);
+double orc_norm2_3(const double v[3]);
+double orc_dot3(const double a[3], const double b[3]);
+
+/* Element measures. */
+double orc_triangle_area(const double tri[6]);
+double orc_tetrahedron_volume_real(const double tet[12]);
+double orc_tetrahedron_volume_tet(const double tet[12]);
+
+/* Per-pair clippers.  Output buffers hold ORC_TRI_BUF_SIZE / ORC_TET_BUF_SIZE
+ * elements.  Return the number of output elements. */
+int orc_intersect_tris(const double tri_a[6], const double tri_b[6], double *tris_c);
+int orc_intersect_tets(const double tet_a[12], const double tet_b[12], double *tets_c);
+/* Plane set of a tet, as get_planes_tet: normals[4][3], c[4]. */
+void orc_get_planes_tet(const double tet[12], double normals[12], double c[4]);
+
+/* Kahan compensated sum (src/Precision.F90:45-113). */
+typedef struct { double sum, error; } orc_ksum;
+void orc_ksum_init(orc_ksum *cs);
+void orc_ksum_add(orc_ksum *cs, double x);
+double orc_ksum_value(const orc_ksum *cs);
+
+/* Candidate finders.  Output is CSR over A elements (indptr[ne_a+1]); when
+ * indices is NULL only indptr is filled.  closed=0: strict AABB overlap (tree
+ * and brute-force finders); closed=1: closed intervals (R-tree finder).
+ * Brute force lists B ids ascending; the tree lists them in traversal order. */
+long orc_brute_force_finder(int dim, int closed,
+                            long ne_a, int loc_a, const double *pos_a, const int *enl_a,
+                            long ne_b, int loc_b, const double *pos_b, const int *enl_b,
+                            long *indptr, int *indices);
+
+typedef struct orc_tree orc_tree;
+orc_tree *orc_tree_build(int dim, long nnodes_b, const double *pos_b,
+                         long ne_b, int loc_b, const int *enl_b);
+void orc_tree_free(orc_tree *t);
+/* Query one A element (its vertex coordinates, loc x dim) into out (capacity
+ * >= ne_b); seen is a caller-owned zeroed byte array of ne_b entries, left
+ * zeroed on return.  Returns the number of B elements found. */
+int orc_tree_query(const orc_tree *t, int loc_a, const double *coords_a,
+                   unsigned char *seen, int *out);
+long orc_tree_finder(int dim, long nnodes_b,
+                     long ne_a, int loc_a, const double *pos_a, const int *enl_a,
+                     long ne_b, int loc_b, const double *pos_b, const int *enl_b,
+                     long *indptr, int *indices);
+
+/* Clip a list of candidate pairs.  counts[p] = number of output elements;
+ * measure[p] = sum, in output order, of the reference's Sigma measure
+ * (triangle_area / tetrahedron_volume_real) over them.  Either array may be
+ * NULL.  If elems is non-NULL it receives up to `cap` elements per pair
+ * (cap*6 or cap*12 doubles per pair). */
+void orc_clip_pairs(int dim, long npairs, const int *pair_a, const int *pair_b,
+                    const double *pos_a, const int *enl_a,
+                    const double *pos_b, const int *enl_b,
+                    int *counts, double *measure, int cap, double *elems,
+                    int nthreads);
+
+/* P1 x P1 mixed mass: for every pair, the local block
+ * block[p][i][j] = sum_c sum_q w_q |c| phi^A_i(x_q) phi^B_j(x_q)
+ * (3x3 in 2D with the Dunavant 3-point rule, 4x4 in 3D with the 4-point
+ * Stroud rule), with the supermesh elements c of the pair. */
+void orc_mass_pairs(int dim, long npairs, const int *pair_a, const int *pair_b,
+                    const double *pos_a, const int *enl_a,
+                    const double *pos_b, const int *enl_b,
+                    double *blocks, int nthreads);
+
+/* f^T M g evaluated like the reference's intersection_calculation
+ * callbacks, with f a nodal field on A and g on B (Kahan sum in 3D, plain in
+ * 2D), plus the Sigma measure.  pairs iterate in the given order. */
+void orc_inner_product(int dim, long npairs, const int *pair_a, const int *pair_b,
+                       const double *pos_a, const int *enl_a, const double *field_a,
+                       const double *pos_b, const int *enl_b, const double *field_b,
+                       double *measure_out, double *integral_out);
+
+/* Whole supermesh run (finder + clip + Sigma), the reference's serial caller
+ * loop.  finder: 0 tree (strict), 1 brute force strict, 2 brute force closed.
+ * Threads split the A elements into contiguous ranges; the tree is built once
+ * (serially, as the reference does) and shared read-only.
+ * stats[0]=candidates, [1]=non-empty pairs, [2]=output elements,
+ * [3]=max per pair.  times[0]=build seconds, [1]=query+clip seconds. */
+void orc_supermesh(int dim, int finder, int nthreads,
+                   long nnodes_a, long ne_a, const double *pos_a, const int *enl_a,
+                   long nnodes_b, long ne_b, const double *pos_b, const int *enl_b,
+                   long long *stats, double *sum_out, double *times);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
