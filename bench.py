@@ -1,0 +1,173 @@
+"""Benchmark: clipped candidate pairs per second on one MI355X.
+
+A step is one full supermesh pass over resident meshes: element staging,
+hierarchy build, candidate search, FP64 clipping of every candidate pair,
+reductions and (2D) P1 x P1 mixed-mass assembly -- one call of
+libsupermesh_gpu_supermesh.  The headline workload is the 2D split-grid pair
+1024^2 (2,097,152 triangles) vs 1000^2 (2,000,000 triangles), jitter 0.3h,
+seed 1; the 3D pair 64^3 x 6 (1,572,864 tets) vs 60^3 x 6 (1,296,000 tets),
+jitter 0.25h, seed 2, is reported beside it.
+
+Prints one JSON line.  `value` = candidate pairs clipped per second,
+end to end (inputs resident in HBM when timing starts).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "clipped candidate pairs/s (2D tri, 3D tet); learned+batched vs plain, 1 GPU"
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix), FMA counted as 2 flops
+HBM_PEAK_GBS = 8000.0
+
+# FP64 operations (add, sub, mul, div, sqrt, compare, min/max, abs) the
+# reference algorithm executes per candidate pair, counted with the
+# instrumented CPU restatement (oracle/count_flops.py) on these workloads.
+FLOPS_PER_PAIR = {"tri_1024_vs_1000": 415.8, "tet_64_vs_60": 691.1}
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--workload", default="tri_1024_vs_1000")
+    p.add_argument("--no-3d", action="store_true", help="skip the 3D workload")
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="also run a plain device copy of known size (PMC calibration)")
+    return p.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def host_threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def gpu_workload(ctx, name, steps, warmup, mass):
+    from libsupermesh_amd import gpu, meshgen
+    t0 = time.time()
+    pa, ea, pb, eb = meshgen.config_meshes(name)
+    log(f"[bench] {name}: meshes {ea.shape[0]} x {eb.shape[0]} elements ({time.time() - t0:.1f}s)")
+    A, B = gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pb, eb)
+    flags = gpu.TIMING | (gpu.MASS if mass else 0)
+    for _ in range(warmup):
+        r = ctx.supermesh(A, B, flags=flags)
+    gpu.synchronize()
+    t0 = time.perf_counter()
+    clip_ms = []
+    phases = []
+    for _ in range(steps):
+        r = ctx.supermesh(A, B, flags=flags)
+        clip_ms.append(r.ms[7])
+        phases.append(list(r.ms))
+    gpu.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    err = abs(r.sum - 1.0)
+    log(f"[bench] {name}: {r.npairs} pairs, {r.nonempty} non-empty, {r.nout} elements, "
+        f"{dt * 1e3:.2f} ms/step, |sum-1|={err:.2e}")
+    names = ["stage", "hierarchy", "search", "clip", "mass", None, "total", "clip_kernel"]
+    avg = [sum(p[i] for p in phases) / steps for i in range(8)]
+    return {
+        "workload": name, "elements_a": int(ea.shape[0]), "elements_b": int(eb.shape[0]),
+        "pairs": int(r.npairs), "nonempty": int(r.nonempty), "supermesh_elements": int(r.nout),
+        "max_per_pair": int(r.max_count), "sum_abs_err": err, "ms_per_step": dt * 1e3,
+        "pairs_per_s": r.npairs / dt, "elements_per_s": r.nout / dt,
+        "clip_kernel_ms": sum(clip_ms) / len(clip_ms),
+        "phase_ms": {n: round(v, 3) for n, v in zip(names, avg) if n},
+        "mass_nnz": int(r.mass_nnz) if mass else None,
+        "_meshes": (pa, ea, pb, eb),
+    }
+
+
+def cpu_baseline(name, meshes):
+    """The CPU restatement of the reference's serial loop (tree finder, then
+    the per-pair clip), one thread per host core over contiguous ranges of A."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    pa, ea, pb, eb = meshes
+    n = host_threads()
+    stats, s, (tb, tq) = O.supermesh(pa, ea, pb, eb, finder=0, nthreads=n)
+    return {
+        "value": stats["candidates"] / tq, "unit": "pairs/s", "cores": n, "kind": "port",
+        "sample": (f"whole {name} workload: serial tree build on B ({tb:.2f} s, excluded), "
+                   f"then tree query + clip of all {stats['candidates']} candidate pairs on "
+                   f"{n} threads ({tq:.2f} s); no mixed mass"),
+        "end_to_end_pairs_per_s": stats["candidates"] / (tb + tq),
+        "sum_abs_err": abs(s - 1.0),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on exactly one GPU: the path shards by "
+                          "Morton ranges of A across GPUs only by design (see DESIGN.md); "
+                          "run with --gpus 1"}))
+        sys.exit(2)
+    import numpy as np
+    from libsupermesh_amd import gpu
+    ctx = gpu.Context(0)
+    if args.calibrate_copy:
+        n = 1 << 28  # 256 MiB device-to-device copy of known size
+        src = gpu.DeviceArray(np.zeros(n, dtype=np.uint8))
+        dst = gpu.DeviceArray(np.zeros(n, dtype=np.uint8))
+        gpu._check(gpu.lib().libsupermesh_gpu_memcpy(dst.ptr, src.ptr, n, 3))
+        gpu.synchronize()
+        log(f"[bench] calibration copy of {n} bytes done")
+    main_w = gpu_workload(ctx, args.workload, args.steps, args.warmup, mass=args.workload.startswith("tri"))
+    extra = None
+    if not args.no_3d and args.workload != "tet_64_vs_60":
+        extra = gpu_workload(ctx, "tet_64_vs_60", max(1, args.steps), max(1, args.warmup), mass=False)
+    fpp = FLOPS_PER_PAIR.get(args.workload)
+    clip_s = main_w["clip_kernel_ms"] / 1e3
+    if fpp:
+        achieved = fpp * main_w["pairs"] / clip_s / 1e12
+        roofline = {"bound": "fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                    "kernel": "k_clip (count pass)", "flops_per_pair": fpp,
+                    "kernel_ms": main_w["clip_kernel_ms"]}
+    else:
+        roofline = {"bound": "fp64", "achieved": None, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": None, "traffic": None, "kernel": "k_clip (count pass)",
+                    "kernel_ms": main_w["clip_kernel_ms"]}
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline ...")
+        cpu = cpu_baseline(args.workload, main_w["_meshes"])
+        log(f"[bench] CPU baseline {cpu['value']:.3e} pairs/s on {cpu['cores']} threads")
+    for w in (main_w, extra):
+        if w:
+            w.pop("_meshes", None)
+    out = {
+        "metric": METRIC, "value": main_w["pairs_per_s"], "unit": "pairs/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": main_w["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "finder": "tree (strict AABB)",
+                   "mixed_mass": main_w["mass_nnz"] is not None,
+                   "pairs": main_w["pairs"], "supermesh_elements": main_w["supermesh_elements"]},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "detail": main_w,
+        "tet_64_vs_60": extra,
+    }
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

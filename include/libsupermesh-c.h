@@ -1,0 +1,80 @@
+/*
+ * libsupermesh-c.h -- drop-in C ABI of the MI355X supermeshing engine.
+ *
+ * Same names, argument meaning and layouts as the reference's
+ * include/libsupermesh-c.h (double precision build): positions are
+ * double[nnodes][dim], element-node lists long[nelements][loc] with 0-based
+ * node ids, triangles double[3][2], tetrahedra double[4][3].  Every function
+ * runs on the GPU (device 0); see libsupermesh_gpu.h for the bulk entry
+ * points that a GPU caller should prefer over per-pair calls.
+ *
+ * Errors: the reference aborts the process (src/Debug.F90:78-94); these
+ * functions instead return with the outputs zeroed and record the failure,
+ * readable through libsupermesh_gpu_last_error() / _code().
+ */
+#ifndef LIBSUPERMESH_C_H
+#define LIBSUPERMESH_C_H
+
+#define LIBSUPERMESH_DOUBLE_PRECISION
+#define TRI_BUF_SIZE 22
+#define TET_BUF_SIZE 81
+
+typedef double c_kind;
+
+#ifdef __cplusplus
+#include <cmath>
+extern "C" {
+#else
+#include <math.h>
+#endif
+
+/* Tree (quadtree / octree) candidate finder, strict AABB overlap.
+ * set_input runs the whole search; query_output returns the number of pairs;
+ * get_output fills indices[nindices] (B ids, ascending within each row) and
+ * ind_ptr[nelements + 1] and releases the result.
+ * Replaces src/C_bindings.F90:346-391 (declared at include/libsupermesh-c.h:21-23). */
+void libsupermesh_tree_intersection_finder_set_input(long* nnodes_a, int* dim_a, long* nelements_a, int* loc_a, long* nnodes_b, int* dim_b, long* nelements_b, int* loc_b, c_kind* positions_a, long* enlist_a, c_kind* positions_b, long* enlist_b);
+void libsupermesh_tree_intersection_finder_query_output(long* nindices);
+void libsupermesh_tree_intersection_finder_get_output(long* nelements, long* nindices, long* indices, long* ind_ptr);
+
+/* R-tree candidate finder semantics (closed AABB overlap), same calling
+ * sequence.  The reference reaches it only from Fortran
+ * (rtree_intersection_finder, src/Intersection_Finder.F90:577-643); these
+ * C symbols expose it to the Fortran module of this library. */
+void libsupermesh_rtree_intersection_finder_set_input(long* nnodes_a, int* dim_a, long* nelements_a, int* loc_a, long* nnodes_b, int* dim_b, long* nelements_b, int* loc_b, c_kind* positions_a, long* enlist_a, c_kind* positions_b, long* enlist_b);
+void libsupermesh_rtree_intersection_finder_query_output(long* nindices);
+void libsupermesh_rtree_intersection_finder_get_output(long* nelements, long* nindices, long* indices, long* ind_ptr);
+
+/* Triangle-triangle intersection: tris_c[TRI_BUF_SIZE][3][2], *n_tris_c outputs.
+ * Replaces src/C_bindings.F90:164-174 (include/libsupermesh-c.h:24). */
+void libsupermesh_intersect_tris_real(c_kind* tri_a, c_kind* tri_b, c_kind* tris_c, int* n_tris_c);
+/* Tet-tet intersection: tets_c[TET_BUF_SIZE][4][3].
+ * Replaces src/C_bindings.F90:152-162 (include/libsupermesh-c.h:25). */
+void libsupermesh_intersect_tets_real(c_kind* tet_a, c_kind* tet_b, c_kind* tets_c, int* n_tets_c);
+/* Replaces src/C_bindings.F90:143-150 (include/libsupermesh-c.h:26). */
+void libsupermesh_triangle_area(c_kind* tri, c_kind* area);
+/* Edges from the first vertex, times 1/6.
+ * Replaces src/C_bindings.F90:134-141 (include/libsupermesh-c.h:27). */
+void libsupermesh_tetrahedron_volume(c_kind* tet, c_kind* volume);
+
+/* Error of the last call on this thread: text ("" if none) and code
+ * (0 = ok; see libsupermesh_gpu.h).  No counterpart in the reference, which
+ * aborts instead. */
+const char* libsupermesh_gpu_last_error(void);
+int libsupermesh_gpu_last_error_code(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+static inline int libsupermesh_max_n_tris_c(int n_lines_b, int n_lines_a) {
+  return n_lines_a * (1 << n_lines_b) - 2;
+}
+
+static inline int libsupermesh_max_n_tets_c(int n_planes_b) {
+  int out = 1;
+  for (int i = 0; i < n_planes_b; i++) out *= 3;
+  return out;
+}
+
+#endif

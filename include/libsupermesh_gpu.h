@@ -1,0 +1,127 @@
+/*
+ * libsupermesh_gpu.h -- bulk entry points of the MI355X supermeshing engine.
+ *
+ * These have no one-call counterpart in the reference: there a caller loops
+ * over candidate pairs and calls intersect_tris / intersect_tets once per pair
+ * (doc/manual.tex:2064-2082, src/tests/test_tri_intersector.F90:89-104).  A
+ * GPU cannot amortise a launch per pair, so the whole loop -- candidate search,
+ * clipping, measures and the P1 x P1 mixed mass -- is one call here, on device
+ * arrays.  Results are bit-identical to the per-pair loop for candidate sets
+ * (sorted within each row), per-pair output counts and output elements.
+ *
+ * Memory: results live in buffers owned by the context and stay valid until
+ * the next call on the same context.  All device pointers are HIP device
+ * memory on the context's device.  Calls on one context are serialised.
+ */
+#ifndef LIBSUPERMESH_GPU_H
+#define LIBSUPERMESH_GPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LIBSUPERMESH_GPU_OK 0
+#define LIBSUPERMESH_GPU_ERROR_INVALID 1   /* bad argument (dimension, loc, sizes) */
+#define LIBSUPERMESH_GPU_ERROR_HIP 2       /* a HIP call failed (no device, OOM) */
+#define LIBSUPERMESH_GPU_ERROR_STATE 3     /* call sequence (get before set_input) */
+#define LIBSUPERMESH_GPU_ERROR_OVERFLOW 4  /* result larger than a 32-bit index */
+
+/* Candidate finder semantics. */
+#define LIBSUPERMESH_GPU_FINDER_TREE 0   /* strict AABB overlap: the quadtree /
+                                            octree finder and brute force
+                                            (Quadtree_Intersection_Finder.F90:411-420) */
+#define LIBSUPERMESH_GPU_FINDER_RTREE 1  /* closed AABB overlap: the R-tree finder
+                                            (R-Tree_Intersection_Finder_C++.h:233-251) */
+
+/* Flags for libsupermesh_gpu_supermesh. */
+#define LIBSUPERMESH_GPU_ELEMENTS 1   /* materialise the supermesh elements */
+#define LIBSUPERMESH_GPU_MASS 2       /* P1 x P1 mixed mass blocks + CSR */
+#define LIBSUPERMESH_GPU_TIMING 4     /* fill ms[] with per-phase times */
+#define LIBSUPERMESH_GPU_UNSORTED 8   /* keep candidate rows in traversal order */
+
+typedef struct libsupermesh_gpu_context libsupermesh_gpu_context;
+
+/* A simplex mesh in device memory: positions double[nnodes][dim],
+ * enlist int[nelements][dim+1], 0-based. */
+typedef struct {
+  int dim;
+  long nnodes;
+  long nelements;
+  const double* positions;
+  const int* enlist;
+} libsupermesh_gpu_mesh;
+
+typedef struct {
+  /* candidate pairs, CSR over A elements */
+  long nelements_a;
+  long npairs;
+  const long* indptr;        /* [nelements_a + 1] */
+  const int* indices;        /* [npairs] B element ids */
+  const int* rows;           /* [npairs] A element id of each pair */
+  /* per-pair clip results */
+  const int* counts;         /* [npairs] output elements per pair */
+  const double* measure;     /* [npairs] sum of output areas / volumes */
+  /* materialised supermesh (LIBSUPERMESH_GPU_ELEMENTS) */
+  const long* offsets;       /* [npairs + 1] first output element of each pair */
+  const double* elements;    /* [nout][dim+1][dim] */
+  /* P1 x P1 mixed mass (LIBSUPERMESH_GPU_MASS):
+   * blocks[p][i][j] = int over the pair's supermesh of phi^A_i phi^B_j,
+   * and the assembled matrix M[nnodes_a][nnodes_b] in CSR. */
+  const double* blocks;      /* [npairs][dim+1][dim+1] */
+  long mass_nnz;
+  const long* mass_indptr;   /* [nnodes_a + 1] */
+  const int* mass_indices;   /* [mass_nnz] */
+  const double* mass_values; /* [mass_nnz] */
+  /* reductions */
+  double sum;                /* compensated sum of all output measures */
+  long nonempty;             /* pairs with at least one output element */
+  long nout;                 /* output elements */
+  int max_count;             /* largest per-pair output count */
+  /* phase times in ms (LIBSUPERMESH_GPU_TIMING): [0] staging, [1] hierarchy
+   * build, [2] candidate search, [3] clipping (count pass + reductions +
+   * element fill), [4] mass assembly, [6] total, [7] the clip count kernel
+   * alone */
+  double ms[8];
+} libsupermesh_gpu_result;
+
+/* Context on HIP device `device` with its own stream (default priority). */
+int libsupermesh_gpu_context_create(int device, libsupermesh_gpu_context** ctx);
+void libsupermesh_gpu_context_destroy(libsupermesh_gpu_context* ctx);
+/* The context's hipStream_t, as void*. */
+void* libsupermesh_gpu_context_stream(libsupermesh_gpu_context* ctx);
+
+/* Text of the last error on this thread ("" if none) and its code. */
+const char* libsupermesh_gpu_last_error(void);
+int libsupermesh_gpu_last_error_code(void);
+
+/* Device memory helpers on the library's own HIP runtime, for callers that
+ * hold no other GPU runtime (Fortran, ctypes).  copy kinds: 1 host->device,
+ * 2 device->host, 3 device->device. */
+int libsupermesh_gpu_malloc(int device, size_t bytes, void** ptr);
+int libsupermesh_gpu_free(void* ptr);
+int libsupermesh_gpu_memcpy(void* dst, const void* src, size_t bytes, int kind);
+int libsupermesh_gpu_synchronize(int device);
+
+/* Candidate pairs only (finder = LIBSUPERMESH_GPU_FINDER_*). */
+int libsupermesh_gpu_intersection_finder(libsupermesh_gpu_context* ctx,
+                                         const libsupermesh_gpu_mesh* a,
+                                         const libsupermesh_gpu_mesh* b, int finder, int flags,
+                                         libsupermesh_gpu_result* out);
+
+/* Candidate pairs + clipping + reductions (+ elements / mass per flags). */
+int libsupermesh_gpu_supermesh(libsupermesh_gpu_context* ctx, const libsupermesh_gpu_mesh* a,
+                               const libsupermesh_gpu_mesh* b, int finder, int flags,
+                               libsupermesh_gpu_result* out);
+
+/* Clip an explicit list of pairs (device arrays pair_a[n], pair_b[n]);
+ * fills counts / measure (and elements, blocks per flags; no mass CSR). */
+int libsupermesh_gpu_clip_pairs(libsupermesh_gpu_context* ctx, const libsupermesh_gpu_mesh* a,
+                                const libsupermesh_gpu_mesh* b, long npairs, const int* pair_a,
+                                const int* pair_b, int flags, libsupermesh_gpu_result* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,517 @@
+// capi.hip -- the C ABI: libsupermesh-c.h drop-in symbols and the bulk GPU
+// entry points of libsupermesh_gpu.h.
+//
+// The reference exports void functions and aborts on bad input
+// (src/Debug.F90:78-94).  Here the drop-in symbols keep their void
+// signatures and never abort: failures are recorded and readable through
+// libsupermesh_gpu_last_error() / _code().  The reference keeps the finder's
+// CSR in module globals (src/C_bindings.F90:23-32) and frees it in
+// get_output; the same sequence is kept, guarded by a mutex.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "../../include/libsupermesh-c.h"
+#include "../../include/libsupermesh_gpu.h"
+#include "lsm_geometry.hpp"
+#include "lsm_runtime.hpp"
+
+struct libsupermesh_gpu_context {
+  lsm::Context ctx;
+};
+
+namespace lsm {
+
+static thread_local int tl_code = 0;
+static thread_local std::string tl_msg;
+
+void set_last_error(int code, const char* msg) {
+  tl_code = code;
+  tl_msg = msg ? msg : "";
+}
+
+static void clear_error() {
+  tl_code = 0;
+  tl_msg.clear();
+}
+
+static void init_context(Context& C, int device) {
+  int n = 0;
+  LSM_HIP(hipGetDeviceCount(&n));
+  LSM_REQUIRE(n > 0, LIBSUPERMESH_GPU_ERROR_HIP, "no HIP device");
+  LSM_REQUIRE(device >= 0 && device < n, LIBSUPERMESH_GPU_ERROR_INVALID, "invalid device index");
+  C.device = device;
+  LSM_HIP(hipSetDevice(device));
+  LSM_HIP(hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking));
+  for (int i = 0; i < 16; i++) LSM_HIP(hipEventCreate(&C.ev[i]));
+  C.events = true;
+}
+
+static void free_context(Context& C) {
+  hipSetDevice(C.device);
+  if (C.stream) hipStreamSynchronize(C.stream);
+  DevBuf* bufs[] = {&C.mesh_a.enl_copy, &C.mesh_a.pos_copy, &C.mesh_a.coords, &C.mesh_a.bbox,
+                    &C.mesh_a.meas_sp, &C.mesh_a.planes, &C.mesh_a.affine, &C.mesh_b.enl_copy,
+                    &C.mesh_b.pos_copy, &C.mesh_b.coords, &C.mesh_b.bbox, &C.mesh_b.meas_sp,
+                    &C.mesh_b.planes, &C.mesh_b.affine, &C.bvh.boxes, &C.bvh.perm, &C.keys,
+                    &C.keys_alt, &C.vals, &C.vals_alt, &C.sort_tmp, &C.scan_tmp, &C.qperm,
+                    &C.row_count, &C.indptr, &C.indices, &C.rows, &C.counts, &C.measure,
+                    &C.partials, &C.scalars, &C.offsets, &C.elements, &C.blocks, &C.mflag,
+                    &C.mkeys, &C.mvals, &C.mkeys_alt, &C.mvals_alt, &C.ukeys, &C.uvals, &C.nuniq,
+                    &C.mindptr, &C.mindices, &C.one_in, &C.one_out};
+  for (DevBuf* b : bufs) b->release();
+  if (C.pinned) hipHostFree(C.pinned);
+  C.pinned = nullptr;
+  if (C.events)
+    for (int i = 0; i < 16; i++) hipEventDestroy(C.ev[i]);
+  C.events = false;
+  if (C.stream) hipStreamDestroy(C.stream);
+  C.stream = nullptr;
+}
+
+Context* default_context() {
+  static std::unique_ptr<libsupermesh_gpu_context> g;
+  static std::mutex m;
+  std::lock_guard<std::mutex> lk(m);
+  if (!g) {
+    auto c = std::make_unique<libsupermesh_gpu_context>();
+    init_context(c->ctx, 0);
+    g = std::move(c);
+  }
+  return &g->ctx;
+}
+
+template <class F>
+static int guarded(F&& f) {
+  clear_error();
+  try {
+    f();
+    return LIBSUPERMESH_GPU_OK;
+  } catch (const Error& e) {
+    set_last_error(e.code, e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_last_error(LIBSUPERMESH_GPU_ERROR_HIP, "host allocation failed");
+    return LIBSUPERMESH_GPU_ERROR_HIP;
+  } catch (const std::exception& e) {
+    set_last_error(LIBSUPERMESH_GPU_ERROR_HIP, e.what());
+    return LIBSUPERMESH_GPU_ERROR_HIP;
+  }
+}
+
+static void check_mesh(const libsupermesh_gpu_mesh* m, const char* which) {
+  LSM_REQUIRE(m != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID, std::string("null mesh ") + which);
+  LSM_REQUIRE(m->dim == 2 || m->dim == 3, LIBSUPERMESH_GPU_ERROR_INVALID, "Invalid dimension");
+  LSM_REQUIRE(m->nelements == 0 || (m->positions && m->enlist), LIBSUPERMESH_GPU_ERROR_INVALID,
+              std::string("null arrays in mesh ") + which);
+}
+
+struct Phase {
+  Context& C;
+  bool on;
+  int n = 0;
+  explicit Phase(Context& c, bool enabled) : C(c), on(enabled) {}
+  void mark() {
+    if (on && n < 16) LSM_HIP(hipEventRecord(C.ev[n++], C.stream));
+  }
+  double between(int i, int j) {
+    float ms = 0.f;
+    if (!on || j >= n) return 0.0;
+    LSM_HIP(hipEventElapsedTime(&ms, C.ev[i], C.ev[j]));
+    return ms;
+  }
+};
+
+// Whole pipeline on device meshes.
+static void run_supermesh(Context& C, const libsupermesh_gpu_mesh* a, const libsupermesh_gpu_mesh* b,
+                          int finder, int flags, bool clip, libsupermesh_gpu_result* out) {
+  check_mesh(a, "a");
+  check_mesh(b, "b");
+  LSM_REQUIRE(a->dim == b->dim, LIBSUPERMESH_GPU_ERROR_INVALID, "Invalid dimension: meshes differ");
+  LSM_REQUIRE(finder == LIBSUPERMESH_GPU_FINDER_TREE || finder == LIBSUPERMESH_GPU_FINDER_RTREE,
+              LIBSUPERMESH_GPU_ERROR_INVALID, "unknown finder");
+  LSM_REQUIRE(out != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID, "null result");
+  LSM_HIP(hipSetDevice(C.device));
+  std::memset(out, 0, sizeof(*out));
+  const bool mass = clip && (flags & LIBSUPERMESH_GPU_MASS);
+  Phase ph(C, flags & LIBSUPERMESH_GPU_TIMING);
+  C.timing = ph.on;
+  ph.mark();
+  stage_mesh(C, C.mesh_a, a->dim, a->dim + 1, a->nnodes, a->nelements, a->positions, a->enlist,
+             false, mass);
+  stage_mesh(C, C.mesh_b, b->dim, b->dim + 1, b->nnodes, b->nelements, b->positions, b->enlist,
+             clip, mass);
+  ph.mark();
+  build_bvh(C, C.mesh_b);
+  ph.mark();
+  long npairs = find_pairs(C, C.mesh_a, C.mesh_b, finder, !(flags & LIBSUPERMESH_GPU_UNSORTED));
+  ph.mark();
+  out->nelements_a = a->nelements;
+  out->npairs = npairs;
+  out->indptr = static_cast<const long*>(C.indptr.ptr);
+  out->indices = static_cast<const int*>(C.indices.get(4));
+  out->rows = static_cast<const int*>(C.rows.get(4));
+  if (clip) {
+    clip_pairs(C, C.mesh_a, C.mesh_b, npairs, flags, out);
+    ph.mark();
+    if (mass) assemble_mass(C, C.mesh_a, C.mesh_b, npairs, out);
+    ph.mark();
+  }
+  LSM_HIP(hipStreamSynchronize(C.stream));
+  if (ph.on) {
+    out->ms[0] = ph.between(0, 1);
+    out->ms[1] = ph.between(1, 2);
+    out->ms[2] = ph.between(2, 3);
+    out->ms[3] = ph.between(3, 4);
+    out->ms[4] = ph.between(4, 5);
+    out->ms[6] = ph.between(0, ph.n - 1);
+  }
+  C.timing = false;
+}
+
+// ---------------------------------------------------------------------------
+// Per-pair kernels behind the reference's one-pair C functions
+// ---------------------------------------------------------------------------
+
+__global__ void k_one(int kind, const double* __restrict__ in, double* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (kind == 0) {  // intersect_tris_real
+    Line L[3];
+    tri_lines(in + 6, L);
+    double tol = tri_tol(triangle_area(in), triangle_area(in + 6));
+    int k = 0;
+    int n = clip_tri(in, L, tol, [&](double x0, double y0, double x1, double y1, double x2, double y2,
+                                     double) {
+      if (k < kTriBufSize) {
+        double* o = out + 1 + 6 * k;
+        o[0] = x0;
+        o[1] = y0;
+        o[2] = x1;
+        o[3] = y1;
+        o[4] = x2;
+        o[5] = y2;
+      }
+      k++;
+    });
+    out[0] = (double)n;
+  } else if (kind == 1) {  // intersect_tets_real
+    Planes P;
+    tet_planes(in + 12, P);
+    Tet A;
+    for (int i = 0; i < 12; i++) A.v[i] = in[i];
+    double tol = tet_tol(tet_volume_tet(in), tet_volume_tet(in + 12));
+    int k = 0;
+    int n = clip_tet(A, P, tol, [&](const Tet& T, double) {
+      if (k < kTetBufSize)
+        for (int i = 0; i < 12; i++) out[1 + 12 * k + i] = T.v[i];
+      k++;
+    });
+    out[0] = (double)n;
+  } else if (kind == 2) {
+    out[0] = triangle_area(in);
+  } else {
+    out[0] = tet_volume_real(in);
+  }
+}
+
+// Runs k_one on the default context: nin doubles in, nout doubles back.
+static void one_call(int kind, const double* in, int nin, double* result, int nout) {
+  Context& C = *default_context();
+  std::lock_guard<std::mutex> lk(C.mu);
+  LSM_HIP(hipSetDevice(C.device));
+  size_t need = sizeof(double) * (size_t)(24 + 1 + 81 * 12);
+  if (!C.pinned) {
+    LSM_HIP(hipHostMalloc(&C.pinned, 2 * need, hipHostMallocDefault));
+    C.pinned_bytes = 2 * need;
+  }
+  double* hin = static_cast<double*>(C.pinned);
+  double* hout = hin + need / sizeof(double);
+  std::memcpy(hin, in, sizeof(double) * nin);
+  double* din = C.one_in.as<double>(24);
+  double* dout = C.one_out.as<double>(1 + 81 * 12);
+  LSM_HIP(hipMemcpyAsync(din, hin, sizeof(double) * nin, hipMemcpyHostToDevice, C.stream));
+  hipLaunchKernelGGL(k_one, dim3(1), dim3(64), 0, C.stream, kind, din, dout);
+  LSM_HIP(hipGetLastError());
+  LSM_HIP(hipMemcpyAsync(hout, dout, sizeof(double) * nout, hipMemcpyDeviceToHost, C.stream));
+  LSM_HIP(hipStreamSynchronize(C.stream));
+  std::memcpy(result, hout, sizeof(double) * nout);
+}
+
+// ---------------------------------------------------------------------------
+// Finder state of the reference's set_input / query_output / get_output
+// ---------------------------------------------------------------------------
+
+struct FinderState {
+  std::mutex mu;
+  bool ready = false;
+  long nelements_a = 0;
+  std::vector<long> indptr;
+  std::vector<int> indices;
+};
+
+static FinderState g_tree, g_rtree;
+
+static void host_mesh_to_device(Context& C, DevMesh& M, long nnodes, int dim, long ne, int loc,
+                                const double* pos, const long* enl, libsupermesh_gpu_mesh& view) {
+  LSM_REQUIRE(dim == 2 || dim == 3, LIBSUPERMESH_GPU_ERROR_INVALID, "Invalid dimension");
+  LSM_REQUIRE(loc == dim + 1, LIBSUPERMESH_GPU_ERROR_INVALID,
+              "Unsupported element type: only simplices (loc = dim + 1)");
+  LSM_REQUIRE(nnodes >= 0 && ne >= 0 && nnodes < (1L << 31) && ne < (1L << 31),
+              LIBSUPERMESH_GPU_ERROR_INVALID, "Invalid mesh size");
+  std::vector<int> e32((size_t)ne * loc);
+  for (size_t i = 0; i < e32.size(); i++) {
+    long v = enl[i];
+    LSM_REQUIRE(v >= 0 && v < nnodes, LIBSUPERMESH_GPU_ERROR_INVALID, "Invalid node");
+    e32[i] = (int)v;
+  }
+  double* dpos = M.pos_copy.as<double>((size_t)nnodes * dim);
+  int* denl = M.enl_copy.as<int>((size_t)ne * loc);
+  if (nnodes > 0)
+    LSM_HIP(hipMemcpyAsync(dpos, pos, sizeof(double) * nnodes * dim, hipMemcpyHostToDevice, C.stream));
+  if (ne > 0)
+    LSM_HIP(hipMemcpyAsync(denl, e32.data(), sizeof(int) * ne * loc, hipMemcpyHostToDevice, C.stream));
+  LSM_HIP(hipStreamSynchronize(C.stream));
+  view.dim = dim;
+  view.nnodes = nnodes;
+  view.nelements = ne;
+  view.positions = dpos;
+  view.enlist = denl;
+}
+
+static void finder_set_input(FinderState& S, int finder, long nnodes_a, int dim_a, long ne_a,
+                             int loc_a, long nnodes_b, int dim_b, long ne_b, int loc_b,
+                             const double* pos_a, const long* enl_a, const double* pos_b,
+                             const long* enl_b) {
+  std::lock_guard<std::mutex> lk(S.mu);
+  S.ready = false;
+  S.indptr.clear();
+  S.indices.clear();
+  LSM_REQUIRE(dim_a == dim_b, LIBSUPERMESH_GPU_ERROR_INVALID, "Invalid dimension");
+  Context& C = *default_context();
+  std::lock_guard<std::mutex> lk2(C.mu);
+  LSM_HIP(hipSetDevice(C.device));
+  libsupermesh_gpu_mesh va, vb;
+  host_mesh_to_device(C, C.mesh_a, nnodes_a, dim_a, ne_a, loc_a, pos_a, enl_a, va);
+  host_mesh_to_device(C, C.mesh_b, nnodes_b, dim_b, ne_b, loc_b, pos_b, enl_b, vb);
+  libsupermesh_gpu_result r;
+  run_supermesh(C, &va, &vb, finder, 0, false, &r);
+  S.nelements_a = ne_a;
+  S.indptr.resize((size_t)ne_a + 1);
+  S.indices.resize((size_t)r.npairs);
+  LSM_HIP(hipMemcpy(S.indptr.data(), r.indptr, sizeof(long) * (ne_a + 1), hipMemcpyDeviceToHost));
+  if (r.npairs > 0)
+    LSM_HIP(hipMemcpy(S.indices.data(), r.indices, sizeof(int) * r.npairs, hipMemcpyDeviceToHost));
+  S.ready = true;
+}
+
+static void finder_query_output(FinderState& S, long* nindices) {
+  std::lock_guard<std::mutex> lk(S.mu);
+  LSM_REQUIRE(S.ready, LIBSUPERMESH_GPU_ERROR_STATE, "query_output called before set_input");
+  *nindices = (long)S.indices.size();
+}
+
+static void finder_get_output(FinderState& S, long nelements, long nindices, long* indices,
+                              long* ind_ptr) {
+  std::lock_guard<std::mutex> lk(S.mu);
+  LSM_REQUIRE(S.ready, LIBSUPERMESH_GPU_ERROR_STATE, "get_output called before set_input");
+  LSM_REQUIRE(nelements == S.nelements_a && nindices == (long)S.indices.size(),
+              LIBSUPERMESH_GPU_ERROR_INVALID, "get_output sizes differ from query_output");
+  for (long i = 0; i <= nelements; i++) ind_ptr[i] = S.indptr[i];
+  for (long i = 0; i < nindices; i++) indices[i] = S.indices[i];
+  S.ready = false;
+  std::vector<long>().swap(S.indptr);
+  std::vector<int>().swap(S.indices);
+}
+
+}  // namespace lsm
+
+using namespace lsm;
+
+extern "C" {
+
+// ---- bulk API (include/libsupermesh_gpu.h) --------------------------------
+
+int libsupermesh_gpu_context_create(int device, libsupermesh_gpu_context** ctx) {
+  return guarded([&] {
+    LSM_REQUIRE(ctx != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID, "null context pointer");
+    auto c = std::make_unique<libsupermesh_gpu_context>();
+    init_context(c->ctx, device);
+    *ctx = c.release();
+  });
+}
+
+void libsupermesh_gpu_context_destroy(libsupermesh_gpu_context* ctx) {
+  if (!ctx) return;
+  free_context(ctx->ctx);
+  delete ctx;
+}
+
+void* libsupermesh_gpu_context_stream(libsupermesh_gpu_context* ctx) {
+  return ctx ? (void*)ctx->ctx.stream : nullptr;
+}
+
+int libsupermesh_gpu_malloc(int device, size_t bytes, void** ptr) {
+  return guarded([&] {
+    LSM_REQUIRE(ptr != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID, "null pointer");
+    *ptr = nullptr;
+    LSM_HIP(hipSetDevice(device));
+    LSM_HIP(hipMalloc(ptr, bytes ? bytes : 1));
+  });
+}
+
+int libsupermesh_gpu_free(void* ptr) {
+  return guarded([&] {
+    if (ptr) LSM_HIP(hipFree(ptr));
+  });
+}
+
+int libsupermesh_gpu_memcpy(void* dst, const void* src, size_t bytes, int kind) {
+  return guarded([&] {
+    LSM_REQUIRE(kind >= 1 && kind <= 3, LIBSUPERMESH_GPU_ERROR_INVALID, "invalid copy kind");
+    if (bytes == 0) return;
+    hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice
+                                : (kind == 2 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+    LSM_HIP(hipMemcpy(dst, src, bytes, k));
+  });
+}
+
+int libsupermesh_gpu_synchronize(int device) {
+  return guarded([&] {
+    LSM_HIP(hipSetDevice(device));
+    LSM_HIP(hipDeviceSynchronize());
+  });
+}
+
+const char* libsupermesh_gpu_last_error(void) { return tl_msg.c_str(); }
+int libsupermesh_gpu_last_error_code(void) { return tl_code; }
+
+int libsupermesh_gpu_intersection_finder(libsupermesh_gpu_context* ctx, const libsupermesh_gpu_mesh* a,
+                                         const libsupermesh_gpu_mesh* b, int finder, int flags,
+                                         libsupermesh_gpu_result* out) {
+  return guarded([&] {
+    LSM_REQUIRE(ctx != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID, "null context");
+    std::lock_guard<std::mutex> lk(ctx->ctx.mu);
+    run_supermesh(ctx->ctx, a, b, finder, flags, false, out);
+  });
+}
+
+int libsupermesh_gpu_supermesh(libsupermesh_gpu_context* ctx, const libsupermesh_gpu_mesh* a,
+                               const libsupermesh_gpu_mesh* b, int finder, int flags,
+                               libsupermesh_gpu_result* out) {
+  return guarded([&] {
+    LSM_REQUIRE(ctx != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID, "null context");
+    std::lock_guard<std::mutex> lk(ctx->ctx.mu);
+    run_supermesh(ctx->ctx, a, b, finder, flags, true, out);
+  });
+}
+
+int libsupermesh_gpu_clip_pairs(libsupermesh_gpu_context* ctx, const libsupermesh_gpu_mesh* a,
+                                const libsupermesh_gpu_mesh* b, long npairs, const int* pair_a,
+                                const int* pair_b, int flags, libsupermesh_gpu_result* out) {
+  return guarded([&] {
+    LSM_REQUIRE(ctx != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID, "null context");
+    check_mesh(a, "a");
+    check_mesh(b, "b");
+    LSM_REQUIRE(a->dim == b->dim, LIBSUPERMESH_GPU_ERROR_INVALID, "Invalid dimension: meshes differ");
+    LSM_REQUIRE(npairs >= 0 && (npairs == 0 || (pair_a && pair_b)), LIBSUPERMESH_GPU_ERROR_INVALID,
+                "invalid pair list");
+    LSM_REQUIRE(out != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID, "null result");
+    Context& C = ctx->ctx;
+    std::lock_guard<std::mutex> lk(C.mu);
+    LSM_HIP(hipSetDevice(C.device));
+    std::memset(out, 0, sizeof(*out));
+    const bool mass = flags & LIBSUPERMESH_GPU_MASS;
+    stage_mesh(C, C.mesh_a, a->dim, a->dim + 1, a->nnodes, a->nelements, a->positions, a->enlist,
+               false, mass);
+    stage_mesh(C, C.mesh_b, b->dim, b->dim + 1, b->nnodes, b->nelements, b->positions, b->enlist,
+               true, mass);
+    out->npairs = npairs;
+    out->rows = pair_a;
+    out->indices = pair_b;
+    clip_pairs(C, C.mesh_a, C.mesh_b, npairs, flags, out);
+    LSM_HIP(hipStreamSynchronize(C.stream));
+  });
+}
+
+// ---- drop-in symbols (include/libsupermesh-c.h) ---------------------------
+
+void libsupermesh_tree_intersection_finder_set_input(long* nnodes_a, int* dim_a, long* nelements_a,
+                                                     int* loc_a, long* nnodes_b, int* dim_b,
+                                                     long* nelements_b, int* loc_b,
+                                                     double* positions_a, long* enlist_a,
+                                                     double* positions_b, long* enlist_b) {
+  guarded([&] {
+    finder_set_input(g_tree, LIBSUPERMESH_GPU_FINDER_TREE, *nnodes_a, *dim_a, *nelements_a, *loc_a,
+                     *nnodes_b, *dim_b, *nelements_b, *loc_b, positions_a, enlist_a, positions_b,
+                     enlist_b);
+  });
+}
+
+void libsupermesh_tree_intersection_finder_query_output(long* nindices) {
+  guarded([&] { finder_query_output(g_tree, nindices); });
+}
+
+void libsupermesh_tree_intersection_finder_get_output(long* nelements, long* nindices, long* indices,
+                                                      long* ind_ptr) {
+  guarded([&] { finder_get_output(g_tree, *nelements, *nindices, indices, ind_ptr); });
+}
+
+void libsupermesh_rtree_intersection_finder_set_input(long* nnodes_a, int* dim_a, long* nelements_a,
+                                                      int* loc_a, long* nnodes_b, int* dim_b,
+                                                      long* nelements_b, int* loc_b,
+                                                      double* positions_a, long* enlist_a,
+                                                      double* positions_b, long* enlist_b) {
+  guarded([&] {
+    finder_set_input(g_rtree, LIBSUPERMESH_GPU_FINDER_RTREE, *nnodes_a, *dim_a, *nelements_a, *loc_a,
+                     *nnodes_b, *dim_b, *nelements_b, *loc_b, positions_a, enlist_a, positions_b,
+                     enlist_b);
+  });
+}
+
+void libsupermesh_rtree_intersection_finder_query_output(long* nindices) {
+  guarded([&] { finder_query_output(g_rtree, nindices); });
+}
+
+void libsupermesh_rtree_intersection_finder_get_output(long* nelements, long* nindices, long* indices,
+                                                       long* ind_ptr) {
+  guarded([&] { finder_get_output(g_rtree, *nelements, *nindices, indices, ind_ptr); });
+}
+
+void libsupermesh_intersect_tris_real(double* tri_a, double* tri_b, double* tris_c, int* n_tris_c) {
+  int rc = guarded([&] {
+    double in[12], res[1 + 6 * kTriBufSize];
+    std::memcpy(in, tri_a, sizeof(double) * 6);
+    std::memcpy(in + 6, tri_b, sizeof(double) * 6);
+    one_call(0, in, 12, res, 1 + 6 * kTriBufSize);
+    int n = (int)res[0];
+    *n_tris_c = n;
+    std::memcpy(tris_c, res + 1, sizeof(double) * 6 * n);
+  });
+  if (rc != LIBSUPERMESH_GPU_OK) *n_tris_c = 0;
+}
+
+void libsupermesh_intersect_tets_real(double* tet_a, double* tet_b, double* tets_c, int* n_tets_c) {
+  int rc = guarded([&] {
+    double in[24];
+    std::vector<double> res(1 + 12 * kTetBufSize);
+    std::memcpy(in, tet_a, sizeof(double) * 12);
+    std::memcpy(in + 12, tet_b, sizeof(double) * 12);
+    one_call(1, in, 24, res.data(), 1 + 12 * kTetBufSize);
+    int n = (int)res[0];
+    *n_tets_c = n;
+    std::memcpy(tets_c, res.data() + 1, sizeof(double) * 12 * n);
+  });
+  if (rc != LIBSUPERMESH_GPU_OK) *n_tets_c = 0;
+}
+
+void libsupermesh_triangle_area(double* tri, double* area) {
+  guarded([&] { one_call(2, tri, 6, area, 1); });
+}
+
+void libsupermesh_tetrahedron_volume(double* tet, double* volume) {
+  guarded([&] { one_call(3, tet, 12, volume, 1); });
+}
+
+}  // extern "C"

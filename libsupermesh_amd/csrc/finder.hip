@@ -1,0 +1,538 @@
+// finder.hip -- candidate pair search on the device.
+//
+// Reference behaviour (what the result must equal): the quadtree / octree
+// finder (src/Quadtree_Intersection_Finder.F90, src/Octree_...F90) returns,
+// for every A element, the B elements whose bounding boxes overlap its own
+// strictly in every dimension (bboxes_intersect, Quadtree_...F90:411-420) --
+// the same set as brute force (src/Intersection_Finder.F90:1032-1050).  The
+// R-tree finder uses closed intervals.  Both predicates run here in FP64 on
+// exact min/max boxes.
+//
+// Design: B's element boxes are sorted by the 63/64-bit Morton code of their
+// centres and grouped into an implicit F-ary hierarchy (node i of level l
+// covers children [F*i, F*i+F) of level l-1; level 0 = elements).  Internal
+// boxes are exact unions, so pruning with the same predicate never loses a
+// pair.  A elements are visited in Morton order, one per lane, with a
+// stackless depth-first walk (level, index) that needs no per-lane stack.
+// A count pass sizes the CSR rows, a fill pass writes them, and each row is
+// then sorted by B id so the output is canonical.
+#include <cstring>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include "lsm_geometry.hpp"
+#include "lsm_runtime.hpp"
+
+namespace lsm {
+
+constexpr int kFanout = 8;
+constexpr int kBlock = 256;
+
+// ---------------------------------------------------------------------------
+// Mesh staging
+// ---------------------------------------------------------------------------
+
+template <int DIM>
+__global__ void __launch_bounds__(kBlock)
+k_stage(long ne, long nnodes, const double* __restrict__ pos, const int* __restrict__ enl,
+        double* __restrict__ coords, double* __restrict__ bbox, double* __restrict__ sp,
+        double* __restrict__ planes, double* __restrict__ affine, int* __restrict__ bad) {
+  constexpr int LOC = DIM + 1;
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  double c[LOC * DIM];
+#pragma unroll
+  for (int j = 0; j < LOC; j++) {
+    long v = enl[e * LOC + j];
+    if (v < 0 || v >= nnodes) {
+      atomicOr(bad, 1);
+      v = 0;
+    }
+#pragma unroll
+    for (int d = 0; d < DIM; d++) c[j * DIM + d] = pos[v * DIM + d];
+  }
+#pragma unroll
+  for (int k = 0; k < LOC * DIM; k++) coords[e * LOC * DIM + k] = c[k];
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    double lo = c[d], hi = c[d];
+#pragma unroll
+    for (int j = 1; j < LOC; j++) {
+      lo = fmin_f(lo, c[j * DIM + d]);
+      hi = fmax_f(hi, c[j * DIM + d]);
+    }
+    bbox[(long)d * ne + e] = lo;
+    bbox[(long)(DIM + d) * ne + e] = hi;
+  }
+  double m;
+  if constexpr (DIM == 2) m = triangle_area(c);
+  else m = tet_volume_tet(c);
+  sp[e] = fspacing(m);
+  if constexpr (DIM == 3) {
+   if (planes) {
+    Planes P;
+    tet_planes(c, P);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+#pragma unroll
+      for (int k = 0; k < 3; k++) planes[e * 16 + 4 * i + k] = P.n[i][k];
+      planes[e * 16 + 4 * i + 3] = P.c[i];
+    }
+   }
+  }
+  if (affine) {
+    // Barycentric coordinates as affine maps: lambda_{1..dim}(x) = Minv (x - v1),
+    // lambda_0 = 1 - sum.  Stored as [v1 (dim), Minv (dim x dim, row-major)].
+    double* out = affine + e * (DIM + DIM * DIM);
+    if constexpr (DIM == 2) {
+      double e1x = c[2] - c[0], e1y = c[3] - c[1];
+      double e2x = c[4] - c[0], e2y = c[5] - c[1];
+      double det = e2y * e1x - (-e2x) * (-e1y);
+      out[0] = c[0];
+      out[1] = c[1];
+      out[2] = e2y / det;
+      out[3] = -e2x / det;
+      out[4] = -e1y / det;
+      out[5] = e1x / det;
+    } else {
+      double m00 = c[3] - c[0], m01 = c[6] - c[0], m02 = c[9] - c[0];
+      double m10 = c[4] - c[1], m11 = c[7] - c[1], m12 = c[10] - c[1];
+      double m20 = c[5] - c[2], m21 = c[8] - c[2], m22 = c[11] - c[2];
+      double c00 = m11 * m22 - m12 * m21, c01 = m02 * m21 - m01 * m22, c02 = m01 * m12 - m02 * m11;
+      double c10 = m12 * m20 - m10 * m22, c11 = m00 * m22 - m02 * m20, c12 = m02 * m10 - m00 * m12;
+      double c20 = m10 * m21 - m11 * m20, c21 = m01 * m20 - m00 * m21, c22 = m00 * m11 - m01 * m10;
+      double det = m00 * c00 + m01 * c10 + m02 * c20;
+      out[0] = c[0];
+      out[1] = c[1];
+      out[2] = c[2];
+      out[3] = c00 / det;
+      out[4] = c01 / det;
+      out[5] = c02 / det;
+      out[6] = c10 / det;
+      out[7] = c11 / det;
+      out[8] = c12 / det;
+      out[9] = c20 / det;
+      out[10] = c21 / det;
+      out[11] = c22 / det;
+    }
+  }
+}
+
+void stage_mesh(Context& C, DevMesh& M, int dim, int loc, long nnodes, long ne,
+                const double* positions, const int* enlist, bool need_planes, bool need_affine) {
+  LSM_REQUIRE(dim == 2 || dim == 3, LIBSUPERMESH_GPU_ERROR_INVALID, "Invalid dimension");
+  LSM_REQUIRE(loc == dim + 1, LIBSUPERMESH_GPU_ERROR_INVALID,
+              "Unsupported element type: only simplices (loc = dim + 1)");
+  LSM_REQUIRE(nnodes >= 0 && ne >= 0, LIBSUPERMESH_GPU_ERROR_INVALID, "Negative mesh size");
+  LSM_REQUIRE(ne < (1L << 31) && nnodes < (1L << 31), LIBSUPERMESH_GPU_ERROR_OVERFLOW,
+              "Mesh larger than 2^31 elements or nodes");
+  LSM_REQUIRE(ne == 0 || nnodes > 0, LIBSUPERMESH_GPU_ERROR_INVALID, "No nodes");
+  M.dim = dim;
+  M.loc = loc;
+  M.nnodes = nnodes;
+  M.nelements = ne;
+  M.enlist_dev = enlist;
+  double* coords = M.coords.as<double>((size_t)ne * loc * dim);
+  double* bbox = M.bbox.as<double>((size_t)ne * 2 * dim);
+  double* sp = M.meas_sp.as<double>((size_t)ne);
+  double* planes = need_planes && dim == 3 ? M.planes.as<double>((size_t)ne * 16) : nullptr;
+  double* affine = need_affine ? M.affine.as<double>((size_t)ne * (dim + dim * dim)) : nullptr;
+  int* bad = C.scalars.as<int>(64);
+  LSM_HIP(hipMemsetAsync(bad, 0, sizeof(int), C.stream));
+  if (ne > 0) {
+    if (dim == 2)
+      hipLaunchKernelGGL(k_stage<2>, dim3(grid_for(ne, kBlock)), dim3(kBlock), 0, C.stream, ne,
+                         nnodes, positions, enlist, coords, bbox, sp, planes, affine, bad);
+    else
+      hipLaunchKernelGGL(k_stage<3>, dim3(grid_for(ne, kBlock)), dim3(kBlock), 0, C.stream, ne,
+                         nnodes, positions, enlist, coords, bbox, sp, planes, affine, bad);
+    LSM_HIP(hipGetLastError());
+  }
+  int hbad = 0;
+  LSM_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, C.stream));
+  LSM_HIP(hipStreamSynchronize(C.stream));
+  LSM_REQUIRE(hbad == 0, LIBSUPERMESH_GPU_ERROR_INVALID, "Invalid node: element-node index out of range");
+}
+
+// ---------------------------------------------------------------------------
+// Morton codes and bounds
+// ---------------------------------------------------------------------------
+
+template <int DIM>
+__global__ void __launch_bounds__(kBlock)
+k_center_bounds(long ne, const double* __restrict__ bbox, double* __restrict__ partial) {
+  __shared__ double red[2 * DIM][kBlock];
+  double lo[DIM], hi[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    lo[d] = __builtin_inf();
+    hi[d] = -__builtin_inf();
+  }
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (long)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      double c = 0.5 * (bbox[(long)d * ne + e] + bbox[(long)(DIM + d) * ne + e]);
+      lo[d] = fmin_f(lo[d], c);
+      hi[d] = fmax_f(hi[d], c);
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    red[d][threadIdx.x] = lo[d];
+    red[DIM + d][threadIdx.x] = hi[d];
+  }
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+#pragma unroll
+      for (int d = 0; d < DIM; d++) {
+        red[d][threadIdx.x] = fmin_f(red[d][threadIdx.x], red[d][threadIdx.x + s]);
+        red[DIM + d][threadIdx.x] = fmax_f(red[DIM + d][threadIdx.x], red[DIM + d][threadIdx.x + s]);
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < 2 * DIM; k++) partial[blockIdx.x * 2 * DIM + k] = red[k][0];
+  }
+}
+
+template <int DIM>
+__global__ void k_bounds_final(int nparts, double* __restrict__ partial) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double b[2 * DIM];
+#pragma unroll
+  for (int k = 0; k < 2 * DIM; k++) b[k] = partial[k];
+  for (int p = 1; p < nparts; p++) {
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      b[d] = fmin_f(b[d], partial[p * 2 * DIM + d]);
+      b[DIM + d] = fmax_f(b[DIM + d], partial[p * 2 * DIM + DIM + d]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 2 * DIM; k++) partial[k] = b[k];
+}
+
+__device__ __forceinline__ uint64_t spread2(uint64_t x) {  // 32 bits -> even bits
+  x &= 0xffffffffULL;
+  x = (x | (x << 16)) & 0x0000ffff0000ffffULL;
+  x = (x | (x << 8)) & 0x00ff00ff00ff00ffULL;
+  x = (x | (x << 4)) & 0x0f0f0f0f0f0f0f0fULL;
+  x = (x | (x << 2)) & 0x3333333333333333ULL;
+  x = (x | (x << 1)) & 0x5555555555555555ULL;
+  return x;
+}
+
+__device__ __forceinline__ uint64_t spread3(uint64_t x) {  // 21 bits -> every third bit
+  x &= 0x1fffffULL;
+  x = (x | (x << 32)) & 0x1f00000000ffffULL;
+  x = (x | (x << 16)) & 0x1f0000ff0000ffULL;
+  x = (x | (x << 8)) & 0x100f00f00f00f00fULL;
+  x = (x | (x << 4)) & 0x10c30c30c30c30c3ULL;
+  x = (x | (x << 2)) & 0x1249249249249249ULL;
+  return x;
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(kBlock)
+k_morton(long ne, const double* __restrict__ bbox, const double* __restrict__ bounds,
+         uint64_t* __restrict__ keys, int* __restrict__ vals) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  constexpr int BITS = DIM == 2 ? 32 : 21;
+  const double maxq = (double)((1ULL << BITS) - 1);
+  uint64_t code = 0;
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    double lo = bounds[d], hi = bounds[DIM + d];
+    double c = 0.5 * (bbox[(long)d * ne + e] + bbox[(long)(DIM + d) * ne + e]);
+    double t = hi > lo ? (c - lo) / (hi - lo) : 0.0;
+    t = fmin_f(fmax_f(t, 0.0), 1.0);
+    uint64_t q = (uint64_t)(t * maxq);
+    code |= (DIM == 2 ? spread2(q) : spread3(q)) << d;
+  }
+  keys[e] = code;
+  vals[e] = (int)e;
+}
+
+// Sorts the elements of M by Morton code; returns the permutation in `perm`.
+static void morton_order(Context& C, const DevMesh& M, DevBuf& perm) {
+  long ne = M.nelements;
+  int dim = M.dim;
+  int nparts = (int)std::min<long>(1024, std::max<long>(1, (ne + kBlock - 1) / kBlock));
+  double* part = C.partials.as<double>((size_t)nparts * 6 + 6);
+  uint64_t* keys = C.keys.as<uint64_t>(ne);
+  uint64_t* keys2 = C.keys_alt.as<uint64_t>(ne);
+  int* vals = C.vals.as<int>(ne);
+  int* out = perm.as<int>(ne);
+  const double* bbox = static_cast<const double*>(M.bbox.ptr);
+  if (dim == 2) {
+    hipLaunchKernelGGL(k_center_bounds<2>, dim3(nparts), dim3(kBlock), 0, C.stream, ne, bbox, part);
+    hipLaunchKernelGGL(k_bounds_final<2>, dim3(1), dim3(64), 0, C.stream, nparts, part);
+    hipLaunchKernelGGL(k_morton<2>, dim3(grid_for(ne, kBlock)), dim3(kBlock), 0, C.stream, ne, bbox,
+                       part, keys, vals);
+  } else {
+    hipLaunchKernelGGL(k_center_bounds<3>, dim3(nparts), dim3(kBlock), 0, C.stream, ne, bbox, part);
+    hipLaunchKernelGGL(k_bounds_final<3>, dim3(1), dim3(64), 0, C.stream, nparts, part);
+    hipLaunchKernelGGL(k_morton<3>, dim3(grid_for(ne, kBlock)), dim3(kBlock), 0, C.stream, ne, bbox,
+                       part, keys, vals);
+  }
+  LSM_HIP(hipGetLastError());
+  unsigned end_bit = dim == 2 ? 64 : 63;
+  size_t tmp_bytes = 0;
+  LSM_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys, keys2, vals, out, (size_t)ne, 0u,
+                                    end_bit, C.stream));
+  void* tmp = C.sort_tmp.get(tmp_bytes);
+  LSM_HIP(rocprim::radix_sort_pairs(tmp, tmp_bytes, keys, keys2, vals, out, (size_t)ne, 0u, end_bit,
+                                    C.stream));
+}
+
+// ---------------------------------------------------------------------------
+// Implicit hierarchy
+// ---------------------------------------------------------------------------
+
+struct LevelInfo {
+  int levels;
+  long count[32];
+  long offset[32];
+};
+
+// Level-0 boxes (node-major: [lo_0..lo_{dim-1}, hi_0..hi_{dim-1}]) in sorted order.
+template <int DIM>
+__global__ void __launch_bounds__(kBlock)
+k_leaf_boxes(long ne, const double* __restrict__ bbox, const int* __restrict__ perm,
+             double* __restrict__ boxes) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ne) return;
+  long e = perm[i];
+#pragma unroll
+  for (int k = 0; k < 2 * DIM; k++) boxes[i * 2 * DIM + k] = bbox[(long)k * ne + e];
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(kBlock)
+k_parent_boxes(long nchild, long nparent, const double* __restrict__ child, double* __restrict__ parent) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nparent) return;
+  double b[2 * DIM];
+  long c0 = i * kFanout;
+#pragma unroll
+  for (int k = 0; k < 2 * DIM; k++) b[k] = child[c0 * 2 * DIM + k];
+  for (long c = c0 + 1; c < c0 + kFanout && c < nchild; c++) {
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      b[d] = fmin_f(b[d], child[c * 2 * DIM + d]);
+      b[DIM + d] = fmax_f(b[DIM + d], child[c * 2 * DIM + DIM + d]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 2 * DIM; k++) parent[i * 2 * DIM + k] = b[k];
+}
+
+void build_bvh(Context& C, const DevMesh& B) {
+  DevBVH& H = C.bvh;
+  H.dim = B.dim;
+  H.n = B.nelements;
+  if (H.n == 0) {
+    H.levels = 0;
+    return;
+  }
+  morton_order(C, B, H.perm);
+  long n = H.n;
+  int L = 0;
+  long total = 0;
+  while (true) {
+    H.count[L] = n;
+    H.offset[L] = total;
+    total += n;
+    L++;
+    if (n == 1) break;
+    n = (n + kFanout - 1) / kFanout;
+    LSM_REQUIRE(L < 31, LIBSUPERMESH_GPU_ERROR_OVERFLOW, "hierarchy too deep");
+  }
+  H.levels = L;
+  H.total = total;
+  int w = 2 * B.dim;
+  double* boxes = H.boxes.as<double>((size_t)total * w);
+  const double* bbox = static_cast<const double*>(B.bbox.ptr);
+  const int* perm = static_cast<const int*>(H.perm.ptr);
+  if (B.dim == 2)
+    hipLaunchKernelGGL(k_leaf_boxes<2>, dim3(grid_for(H.n, kBlock)), dim3(kBlock), 0, C.stream, H.n,
+                       bbox, perm, boxes);
+  else
+    hipLaunchKernelGGL(k_leaf_boxes<3>, dim3(grid_for(H.n, kBlock)), dim3(kBlock), 0, C.stream, H.n,
+                       bbox, perm, boxes);
+  for (int l = 1; l < L; l++) {
+    const double* ch = boxes + H.offset[l - 1] * w;
+    double* pa = boxes + H.offset[l] * w;
+    if (B.dim == 2)
+      hipLaunchKernelGGL(k_parent_boxes<2>, dim3(grid_for(H.count[l], kBlock)), dim3(kBlock), 0,
+                         C.stream, H.count[l - 1], H.count[l], ch, pa);
+    else
+      hipLaunchKernelGGL(k_parent_boxes<3>, dim3(grid_for(H.count[l], kBlock)), dim3(kBlock), 0,
+                         C.stream, H.count[l - 1], H.count[l], ch, pa);
+  }
+  LSM_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Traversal
+// ---------------------------------------------------------------------------
+
+template <int DIM, bool CLOSED>
+__device__ __forceinline__ bool overlaps(const double* __restrict__ box, const double* qlo,
+                                         const double* qhi) {
+  bool hit = true;
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    double lo = box[d], hi = box[DIM + d];
+    // strict: b.max > a.min and b.min < a.max (Quadtree_...F90:411-420)
+    hit = hit && (CLOSED ? (hi >= qlo[d] && lo <= qhi[d]) : (hi > qlo[d] && lo < qhi[d]));
+  }
+  return hit;
+}
+
+template <int DIM, bool CLOSED, bool FILL>
+__global__ void __launch_bounds__(kBlock)
+k_query(long na, const int* __restrict__ qperm, const double* __restrict__ abox, LevelInfo info,
+        const double* __restrict__ boxes, const int* __restrict__ perm, long* __restrict__ row_count,
+        const long* __restrict__ indptr, int* __restrict__ indices, int* __restrict__ rows) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= na) return;
+  const int a = qperm[t];
+  double qlo[DIM], qhi[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    qlo[d] = abox[(long)d * na + a];
+    qhi[d] = abox[(long)(DIM + d) * na + a];
+  }
+  const int top = info.levels - 1;
+  long base = FILL ? indptr[a] : 0;
+  long cnt = 0;
+  int l = top;
+  long i = 0;
+  while (true) {
+    bool hit = false;
+    if (i < info.count[l]) hit = overlaps<DIM, CLOSED>(boxes + (info.offset[l] + i) * 2 * DIM, qlo, qhi);
+    if (hit) {
+      if (l == 0) {
+        if (FILL) {
+          indices[base + cnt] = perm[i];
+          rows[base + cnt] = a;
+        }
+        cnt++;
+      } else {
+        l--;
+        i *= kFanout;
+        continue;
+      }
+    }
+    // advance to the next sibling, climbing while at the end of a group
+    bool done = false;
+    while (true) {
+      if (l == top) {
+        done = true;
+        break;
+      }
+      if ((i % kFanout) != kFanout - 1 && i + 1 < info.count[l]) {
+        i++;
+        break;
+      }
+      i /= kFanout;
+      l++;
+    }
+    if (done) break;
+  }
+  if (!FILL) row_count[a] = cnt;
+}
+
+// Insertion sort of each row by B id (rows are short: tens of entries).
+__global__ void __launch_bounds__(kBlock)
+k_sort_rows(long na, const int* __restrict__ qperm, const long* __restrict__ indptr,
+            int* __restrict__ indices) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= na) return;
+  int a = qperm[t];
+  long b0 = indptr[a], b1 = indptr[a + 1];
+  for (long i = b0 + 1; i < b1; i++) {
+    int v = indices[i];
+    long j = i - 1;
+    while (j >= b0 && indices[j] > v) {
+      indices[j + 1] = indices[j];
+      j--;
+    }
+    indices[j + 1] = v;
+  }
+}
+
+long find_pairs(Context& C, const DevMesh& A, const DevMesh& B, int finder, int sort_rows) {
+  long na = A.nelements;
+  long* row_count = C.row_count.as<long>(na + 1);
+  long* indptr = C.indptr.as<long>(na + 1);
+  LSM_HIP(hipMemsetAsync(indptr, 0, sizeof(long) * (na + 1), C.stream));
+  if (na == 0 || B.nelements == 0) {
+    LSM_HIP(hipStreamSynchronize(C.stream));
+    return 0;
+  }
+  LSM_REQUIRE(A.dim == B.dim, LIBSUPERMESH_GPU_ERROR_INVALID, "Invalid dimension: meshes differ");
+  const DevBVH& H = C.bvh;
+  LevelInfo info;
+  info.levels = H.levels;
+  for (int l = 0; l < 32; l++) {
+    info.count[l] = l < H.levels ? H.count[l] : 0;
+    info.offset[l] = l < H.levels ? H.offset[l] : 0;
+  }
+  morton_order(C, A, C.qperm);
+  const int* qperm = static_cast<const int*>(C.qperm.ptr);
+  const double* abox = static_cast<const double*>(A.bbox.ptr);
+  const double* boxes = static_cast<const double*>(H.boxes.ptr);
+  const int* perm = static_cast<const int*>(H.perm.ptr);
+  dim3 grid(grid_for(na, kBlock)), block(kBlock);
+#define LSM_QUERY(DIM, CL, FILL)                                                                  \
+  hipLaunchKernelGGL((k_query<DIM, CL, FILL>), grid, block, 0, C.stream, na, qperm, abox, info, \
+                     boxes, perm, row_count, indptr, indices, rows)
+  int* indices = nullptr;
+  int* rows = nullptr;
+  bool closed = finder == LIBSUPERMESH_GPU_FINDER_RTREE;
+  if (A.dim == 2) {
+    if (closed) LSM_QUERY(2, true, false);
+    else LSM_QUERY(2, false, false);
+  } else {
+    if (closed) LSM_QUERY(3, true, false);
+    else LSM_QUERY(3, false, false);
+  }
+  LSM_HIP(hipGetLastError());
+  size_t tmp_bytes = 0;
+  LSM_HIP(rocprim::inclusive_scan(nullptr, tmp_bytes, row_count, indptr + 1, (size_t)na,
+                                  rocprim::plus<long>(), C.stream));
+  void* tmp = C.scan_tmp.get(tmp_bytes);
+  LSM_HIP(rocprim::inclusive_scan(tmp, tmp_bytes, row_count, indptr + 1, (size_t)na,
+                                  rocprim::plus<long>(), C.stream));
+  long npairs = 0;
+  LSM_HIP(hipMemcpyAsync(&npairs, indptr + na, sizeof(long), hipMemcpyDeviceToHost, C.stream));
+  LSM_HIP(hipStreamSynchronize(C.stream));
+  LSM_REQUIRE(npairs < (1L << 40), LIBSUPERMESH_GPU_ERROR_OVERFLOW, "too many candidate pairs");
+  indices = C.indices.as<int>(npairs);
+  rows = C.rows.as<int>(npairs);
+  if (npairs > 0) {
+    if (A.dim == 2) {
+      if (closed) LSM_QUERY(2, true, true);
+      else LSM_QUERY(2, false, true);
+    } else {
+      if (closed) LSM_QUERY(3, true, true);
+      else LSM_QUERY(3, false, true);
+    }
+    LSM_HIP(hipGetLastError());
+    if (sort_rows) {
+      hipLaunchKernelGGL(k_sort_rows, grid, block, 0, C.stream, na, qperm, indptr, indices);
+      LSM_HIP(hipGetLastError());
+    }
+  }
+#undef LSM_QUERY
+  return npairs;
+}
+
+}  // namespace lsm

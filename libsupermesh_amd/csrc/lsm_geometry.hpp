@@ -1,0 +1,550 @@
+// lsm_geometry.hpp -- FP64 element geometry and the per-pair clippers.
+//
+// Every routine here runs one candidate pair per lane with its whole state in
+// VGPRs: no per-lane arrays are indexed with run-time values, so nothing goes
+// to scratch.  The library is compiled with -ffp-contract=off and the code
+// spells out each operation in the reference's order, so every result rounds
+// exactly as the reference's Fortran does (IEEE double, no FMA, correctly
+// rounded division and square root).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lsm {
+
+#define LSM_HD __host__ __device__ __forceinline__
+
+constexpr int kTriBufSize = 22;  // TRI_BUF_SIZE, include/libsupermesh-c.h:3
+constexpr int kTetBufSize = 81;  // TET_BUF_SIZE, include/libsupermesh-c.h:4
+
+// ---------------------------------------------------------------------------
+// Fortran intrinsics
+// ---------------------------------------------------------------------------
+
+// SPACING(x): 2**max(e-53, -1022) with |x| = m*2**e, 0.5 <= m < 1; TINY for
+// zero and subnormals.  For a biased exponent E >= 53 that is 2**(E-1075),
+// whose biased exponent is E-52.
+LSM_HD double fspacing(double x) {
+  uint64_t bits;
+  __builtin_memcpy(&bits, &x, 8);
+  int64_t E = (int64_t)((bits >> 52) & 0x7ff);
+  uint64_t r = E >= 53 ? (uint64_t)(E - 52) << 52 : (uint64_t)1 << 52;
+  double out;
+  __builtin_memcpy(&out, &r, 8);
+  return out;
+}
+
+constexpr double kEpsilon = 2.220446049250313080847e-16;  // EPSILON(1.0d0)
+
+LSM_HD double fmin_f(double a, double b) { return b < a ? b : a; }
+LSM_HD double fmax_f(double a, double b) { return b > a ? b : a; }
+LSM_HD double fabs_f(double a) { return a < 0.0 ? -a : (a == 0.0 ? 0.0 : a); }
+
+// NORM2 of a 3-vector as the reference's compiler evaluates it: running
+// maximum m, scaled sum of squares s, result m*sqrt(1+s)
+// (Tetrahedron_Intersection.F90:676).
+LSM_HD double norm2_3(double x, double y, double z) {
+  double mx = 0.0, s = 0.0;
+  const double v[3] = {x, y, z};
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    double ax = __builtin_fabs(v[i]);
+    if (mx == 0.0) {
+      mx = ax;
+    } else if (ax > mx) {
+      double t = mx / ax;
+      double tsq = t * t;
+      s = s * tsq;
+      s = s + tsq;
+      mx = ax;
+    } else {
+      double t = ax / mx;
+      s = s + t * t;
+    }
+  }
+  return mx * __builtin_sqrt(1.0 + s);
+}
+
+LSM_HD double dot3(double a0, double a1, double a2, double b0, double b1, double b2) {
+  double s = a0 * b0;
+  s = s + a1 * b1;
+  s = s + a2 * b2;
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// Measures (Triangle_Intersection.F90:453-477, Tetrahedron_Intersection.F90:690-725)
+// ---------------------------------------------------------------------------
+
+LSM_HD double triangle_area(const double* t) {
+  double ux = t[4] - t[0], uy = t[5] - t[1];
+  double vx = t[2] - t[0], vy = t[3] - t[1];
+  return 0.5 * __builtin_fabs(uy * vx - ux * vy);
+}
+
+LSM_HD double triangle_area6(double x0, double y0, double x1, double y1, double x2, double y2) {
+  double ux = x2 - x0, uy = y2 - y0;
+  double vx = x1 - x0, vy = y1 - y0;
+  return 0.5 * __builtin_fabs(uy * vx - ux * vy);
+}
+
+// Edges from vertex 1, times 1/6 (the C ABI and the reference tests' Sigma).
+LSM_HD double tet_volume_real(const double* t) {
+  double e1x = t[3] - t[0], e1y = t[4] - t[1], e1z = t[5] - t[2];
+  double e2x = t[6] - t[0], e2y = t[7] - t[1], e2z = t[8] - t[2];
+  double e3x = t[9] - t[0], e3y = t[10] - t[1], e3z = t[11] - t[2];
+  double d = e1x * (e2y * e3z - e2z * e3y);
+  d = d + e1y * (e2z * e3x - e2x * e3z);
+  d = d + e1z * (e2x * e3y - e2y * e3x);
+  return (1.0 / 6.0) * __builtin_fabs(d);
+}
+
+// Edges from vertex 4, divided by 6 (the clipper's tolerance and keep test).
+LSM_HD double tet_volume_tet(const double* t) {
+  double ax = t[0] - t[9], ay = t[1] - t[10], az = t[2] - t[11];
+  double bx = t[3] - t[9], by = t[4] - t[10], bz = t[5] - t[11];
+  double cx = t[6] - t[9], cy = t[7] - t[10], cz = t[8] - t[11];
+  double c0 = by * cz - bz * cy;
+  double c1 = bz * cx - bx * cz;
+  double c2 = bx * cy - by * cx;
+  return __builtin_fabs(dot3(ax, ay, az, c0, c1, c2)) / 6.0;
+}
+
+// ---------------------------------------------------------------------------
+// Triangle clipping: Sutherland-Hodgman against B's three edge lines, run as
+// a pipeline (each stage keeps only its first and previous point), then a fan
+// from the first surviving vertex.  Output order and rounding are those of
+// Triangle_Intersection.F90:144-185,282-335.
+// ---------------------------------------------------------------------------
+
+struct Line {
+  double nx, ny, px, py;
+};
+
+// get_lines_tri (Triangle_Intersection.F90:395-420).
+LSM_HD void tri_lines(const double* t, Line L[3]) {
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const int k1 = (k + 1) % 3;
+    L[k].nx = -(t[2 * k1 + 1] - t[2 * k + 1]);
+    L[k].ny = (t[2 * k1] - t[2 * k]);
+    L[k].px = t[2 * k];
+    L[k].py = t[2 * k + 1];
+  }
+  double d = (t[2] - t[0]) * L[2].nx;
+  d = d + (t[3] - t[1]) * L[2].ny;
+  if (d > 0.0) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      L[k].nx = -L[k].nx;
+      L[k].ny = -L[k].ny;
+    }
+  }
+}
+
+struct ClipStage {
+  double fx, fy, fd;  // first point and its distance
+  double px, py, pd;  // previous point
+  int nin, nout;
+};
+
+LSM_HD void stage_init(ClipStage& s) {
+  s.fx = s.fy = s.fd = s.px = s.py = s.pd = 0.0;
+  s.nin = s.nout = 0;
+}
+
+// One edge (p1 -> p2) of a Sutherland-Hodgman pass: emits 0, 1 or 2 points.
+template <class Emit>
+LSM_HD void clip_edge(double x1, double y1, double d1, double x2, double y2, double d2,
+                      int& nout, Emit&& emit) {
+  if (d1 < 0.0) {
+    emit(x1, y1);
+    nout++;
+    if (!(d2 <= 0.0)) {
+      double f = __builtin_fabs(d1) / (__builtin_fabs(d1) + __builtin_fabs(d2));
+      f = fmax_f(fmin_f(f, 1.0), 0.0);
+      emit(x1 + f * (x2 - x1), y1 + f * (y2 - y1));
+      nout++;
+    }
+  } else if (d1 == 0.0) {
+    emit(x1, y1);
+    nout++;
+  } else if (d2 < 0.0) {
+    double f = __builtin_fabs(d1) / (__builtin_fabs(d1) + __builtin_fabs(d2));
+    f = fmax_f(fmin_f(f, 1.0), 0.0);
+    emit(x1 + f * (x2 - x1), y1 + f * (y2 - y1));
+    nout++;
+  }
+}
+
+template <class Emit>
+LSM_HD void stage_push(ClipStage& s, const Line& L, double x, double y, Emit&& emit) {
+  double d = L.nx * (x - L.px);
+  d = d + L.ny * (y - L.py);
+  if (s.nin == 0) {
+    s.fx = x;
+    s.fy = y;
+    s.fd = d;
+  } else {
+    clip_edge(s.px, s.py, s.pd, x, y, d, s.nout, emit);
+  }
+  s.px = x;
+  s.py = y;
+  s.pd = d;
+  s.nin++;
+}
+
+template <class Emit>
+LSM_HD void stage_close(ClipStage& s, Emit&& emit) {
+  if (s.nin > 0) clip_edge(s.px, s.py, s.pd, s.fx, s.fy, s.fd, s.nout, emit);
+}
+
+// Fan triangulation of the final polygon: triangle (q1, q_{k-1}, q_k) for each
+// arriving q_k, k >= 3; triangles with area < tol are dropped.
+struct Fan {
+  double x0, y0, x1, y1;
+  int n;
+};
+
+// Clips tri A against B's lines.  For each kept output triangle (in the
+// reference's order) calls sink(x0, y0, x1, y1, x2, y2, area).  Returns the
+// number of kept triangles; if any Sutherland-Hodgman pass ends with fewer
+// than 3 points the reference returns 0 (and the sink's calls must be
+// discarded by the caller, which only happens before a count is known).
+template <class Sink>
+LSM_HD int clip_tri(const double* A, const Line* L, double tol, Sink&& sink) {
+  ClipStage s1, s2, s3;
+  stage_init(s1);
+  stage_init(s2);
+  stage_init(s3);
+  Fan fan;
+  fan.x0 = fan.y0 = fan.x1 = fan.y1 = 0.0;
+  fan.n = 0;
+  int kept = 0;
+  auto fan_push = [&](double x, double y) {
+    if (fan.n == 0) {
+      fan.x0 = x;
+      fan.y0 = y;
+    } else if (fan.n >= 2) {
+      double a = triangle_area6(fan.x0, fan.y0, fan.x1, fan.y1, x, y);
+      if (!(a < tol)) {
+        sink(fan.x0, fan.y0, fan.x1, fan.y1, x, y, a);
+        kept++;
+      }
+    }
+    fan.x1 = x;
+    fan.y1 = y;
+    fan.n++;
+  };
+  auto push3 = [&](double x, double y) { stage_push(s3, L[2], x, y, fan_push); };
+  auto push2 = [&](double x, double y) { stage_push(s2, L[1], x, y, push3); };
+  stage_push(s1, L[0], A[0], A[1], push2);
+  stage_push(s1, L[0], A[2], A[3], push2);
+  stage_push(s1, L[0], A[4], A[5], push2);
+  stage_close(s1, push2);
+  stage_close(s2, push3);
+  stage_close(s3, fan_push);
+  if (s1.nout < 3 || s2.nout < 3 || s3.nout < 3) return 0;
+  return kept;
+}
+
+LSM_HD double tri_tol(double area_a, double area_b) {
+  return 10.0 * fmin_f(fspacing(area_a), fspacing(area_b));
+}
+
+// ---------------------------------------------------------------------------
+// Tetrahedron clipping: B's four unit-normal planes one at a time
+// (Tetrahedron_Intersection.F90:137-179,244-436), depth first.  A split tet
+// yields its children in the reference's order, so the depth-first preorder
+// over (plane-1 child, plane-2 child, ...) is the same lexicographic order as
+// the reference's breadth-first buffer, and every output tet is produced by
+// the same operations.
+// ---------------------------------------------------------------------------
+
+struct Planes {
+  double n[4][3];
+  double c[4];
+};
+
+LSM_HD void unit_cross(double a0, double a1, double a2, double b0, double b1, double b2,
+                       double* out) {
+  double c0 = a1 * b2 - a2 * b1;
+  double c1 = a2 * b0 - a0 * b2;
+  double c2 = a0 * b1 - a1 * b0;
+  double n = norm2_3(c0, c1, c2);
+  out[0] = c0 / n;
+  out[1] = c1 / n;
+  out[2] = c2 / n;
+}
+
+// get_planes_tet (Tetrahedron_Intersection.F90:634-667).
+LSM_HD void tet_planes(const double* t, Planes& P) {
+  double e10[3], e20[3], e30[3], e21[3], e31[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    e10[i] = t[3 + i] - t[i];
+    e20[i] = t[6 + i] - t[i];
+    e30[i] = t[9 + i] - t[i];
+    e21[i] = t[6 + i] - t[3 + i];
+    e31[i] = t[9 + i] - t[3 + i];
+  }
+  unit_cross(e20[0], e20[1], e20[2], e10[0], e10[1], e10[2], P.n[0]);
+  unit_cross(e10[0], e10[1], e10[2], e30[0], e30[1], e30[2], P.n[1]);
+  unit_cross(e30[0], e30[1], e30[2], e20[0], e20[1], e20[2], P.n[2]);
+  unit_cross(e21[0], e21[1], e21[2], e31[0], e31[1], e31[2], P.n[3]);
+  double det = dot3(e10[0], e10[1], e10[2], P.n[3][0], P.n[3][1], P.n[3][2]);
+  if (det < 0.0) {
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) P.n[i][k] = -P.n[i][k];
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+    P.c[i] = dot3(t[3 * i], t[3 * i + 1], t[3 * i + 2], P.n[i][0], P.n[i][1], P.n[i][2]);
+}
+
+struct Tet {
+  double v[12];
+};
+
+// Classification of a tet against one plane.  kind: 0 discard, 1 keep whole,
+// 31 (+++-), 22 (++--), 21 (++-0), 13 (+---), 12 (+--0), 11 (+-00).
+// pos/neg/zer hold ascending vertex indices, packed 2 bits each.
+struct Split {
+  double d[4];
+  int kind, nchild;
+  int pos, neg, zer;
+};
+
+LSM_HD int pick(int packed, int i) { return (packed >> (2 * i)) & 3; }
+
+LSM_HD double sel4(const double* a, int k) {
+  double r = a[3];
+  r = k == 2 ? a[2] : r;
+  r = k == 1 ? a[1] : r;
+  r = k == 0 ? a[0] : r;
+  return r;
+}
+
+LSM_HD Split classify(const Tet& T, const double* n, double c) {
+  Split s;
+  int np = 0, nn = 0, nz = 0;
+  s.pos = s.neg = s.zer = 0;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    double d = dot3(n[0], n[1], n[2], T.v[3 * i], T.v[3 * i + 1], T.v[3 * i + 2]) - c;
+    s.d[i] = d;
+    if (__builtin_fabs(d) < kEpsilon) {
+      s.zer |= i << (2 * nz);
+      nz++;
+    } else if (d < 0.0) {
+      s.neg |= i << (2 * nn);
+      nn++;
+    } else if (d > 0.0) {
+      s.pos |= i << (2 * np);
+      np++;
+    }
+  }
+  if (nn == 0) {
+    s.kind = 0;
+    s.nchild = 0;
+  } else if (np == 0) {
+    s.kind = 1;
+    s.nchild = 1;
+  } else {
+    s.kind = 10 * np + nn;
+    s.nchild = (s.kind == 22 || s.kind == 13) ? 3 : (s.kind == 12 ? 2 : 1);
+  }
+  return s;
+}
+
+// Split point between outside vertex ip and inside vertex in:
+// invdiff = 1/(d_pos - d_neg), w0 = -d_neg*invdiff, w1 = d_pos*invdiff,
+// p = w0*v_pos + w1*v_neg.
+LSM_HD void split_point(const Tet& T, const double* d, int ip, int in, double* p) {
+  double dp = sel4(d, ip), dn = sel4(d, in);
+  double invdiff = 1.0 / (dp - dn);
+  double w0 = -dn * invdiff;
+  double w1 = dp * invdiff;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    double vp = T.v[3 * 3 + k];
+    vp = ip == 2 ? T.v[3 * 2 + k] : vp;
+    vp = ip == 1 ? T.v[3 * 1 + k] : vp;
+    vp = ip == 0 ? T.v[k] : vp;
+    double vn = T.v[3 * 3 + k];
+    vn = in == 2 ? T.v[3 * 2 + k] : vn;
+    vn = in == 1 ? T.v[3 * 1 + k] : vn;
+    vn = in == 0 ? T.v[k] : vn;
+    p[k] = w0 * vp + w1 * vn;
+  }
+}
+
+LSM_HD void vertex(const Tet& T, int i, double* p) {
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    double v = T.v[9 + k];
+    v = i == 2 ? T.v[6 + k] : v;
+    v = i == 1 ? T.v[3 + k] : v;
+    v = i == 0 ? T.v[k] : v;
+    p[k] = v;
+  }
+}
+
+LSM_HD void set_slot(Tet& T, int s, const double* p) {
+  T.v[3 * s] = p[0];
+  T.v[3 * s + 1] = p[1];
+  T.v[3 * s + 2] = p[2];
+}
+
+// T with vertex slot `slot` (run-time) replaced by p.
+LSM_HD void replace_slot(Tet& T, int slot, const double* p) {
+#pragma unroll
+  for (int s = 0; s < 4; s++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) T.v[3 * s + k] = s == slot ? p[k] : T.v[3 * s + k];
+}
+
+// Child `ci` of parent T under split s, as appended by clip_buf
+// (Tetrahedron_Intersection.F90:292-434).
+LSM_HD Tet make_child(const Tet& T, const Split& s, int ci) {
+  Tet out = T;
+  double a[3], b[3], c[3];
+  switch (s.kind) {
+    case 1:
+      break;
+    case 31: {  // +++-
+      int n0 = pick(s.neg, 0);
+      for (int i = 0; i < 3; i++) {
+        int p = pick(s.pos, i);
+        split_point(T, s.d, p, n0, a);
+        replace_slot(out, p, a);
+      }
+      break;
+    }
+    case 22: {  // ++--
+      int p0 = pick(s.pos, 0), p1 = pick(s.pos, 1), n0 = pick(s.neg, 0), n1 = pick(s.neg, 1);
+      if (ci == 0) {
+        split_point(T, s.d, p0, n1, a);  // tmp3
+        split_point(T, s.d, p1, n0, b);  // tmp2
+        replace_slot(out, p0, a);
+        replace_slot(out, p1, b);
+      } else if (ci == 1) {
+        vertex(T, n1, a);
+        set_slot(out, 0, a);
+        split_point(T, s.d, p1, n1, a);  // tmp4
+        set_slot(out, 1, a);
+        split_point(T, s.d, p0, n1, a);  // tmp3
+        set_slot(out, 2, a);
+        split_point(T, s.d, p1, n0, a);  // tmp2
+        set_slot(out, 3, a);
+      } else {
+        vertex(T, n0, a);
+        set_slot(out, 0, a);
+        split_point(T, s.d, p0, n0, a);  // tmp1
+        set_slot(out, 1, a);
+        split_point(T, s.d, p1, n0, a);  // tmp2
+        set_slot(out, 2, a);
+        split_point(T, s.d, p0, n1, a);  // tmp3
+        set_slot(out, 3, a);
+      }
+      break;
+    }
+    case 21: {  // ++-0
+      int n0 = pick(s.neg, 0);
+      for (int i = 0; i < 2; i++) {
+        int p = pick(s.pos, i);
+        split_point(T, s.d, p, n0, a);
+        replace_slot(out, p, a);
+      }
+      break;
+    }
+    case 13: {  // +---
+      int p0 = pick(s.pos, 0), n0 = pick(s.neg, 0), n1 = pick(s.neg, 1), n2 = pick(s.neg, 2);
+      if (ci == 0) {
+        split_point(T, s.d, p0, n0, a);
+        replace_slot(out, p0, a);
+      } else if (ci == 1) {
+        split_point(T, s.d, p0, n0, a);  // tmp1
+        set_slot(out, 0, a);
+        vertex(T, n1, b);
+        set_slot(out, 1, b);
+        vertex(T, n2, b);
+        set_slot(out, 2, b);
+        split_point(T, s.d, p0, n1, c);  // tmp2
+        set_slot(out, 3, c);
+      } else {
+        vertex(T, n2, a);
+        set_slot(out, 0, a);
+        split_point(T, s.d, p0, n1, a);  // tmp2
+        set_slot(out, 1, a);
+        split_point(T, s.d, p0, n2, a);  // tmp3
+        set_slot(out, 2, a);
+        split_point(T, s.d, p0, n0, a);  // tmp1
+        set_slot(out, 3, a);
+      }
+      break;
+    }
+    case 12: {  // +--0
+      int p0 = pick(s.pos, 0), n0 = pick(s.neg, 0), n1 = pick(s.neg, 1), z0 = pick(s.zer, 0);
+      if (ci == 0) {
+        split_point(T, s.d, p0, n0, a);
+        replace_slot(out, p0, a);
+      } else {
+        split_point(T, s.d, p0, n1, a);  // tmp2
+        set_slot(out, 0, a);
+        vertex(T, z0, a);
+        set_slot(out, 1, a);
+        vertex(T, n1, a);
+        set_slot(out, 2, a);
+        split_point(T, s.d, p0, n0, a);  // tmp1
+        set_slot(out, 3, a);
+      }
+      break;
+    }
+    default: {  // 11: +-00
+      int p0 = pick(s.pos, 0), n0 = pick(s.neg, 0);
+      split_point(T, s.d, p0, n0, a);
+      replace_slot(out, p0, a);
+      break;
+    }
+  }
+  return out;
+}
+
+LSM_HD double tet_tol(double vol_a, double vol_b) {
+  return 10.0 * fmin_f(fspacing(vol_a), fspacing(vol_b));
+}
+
+// Clips tet A by the planes of B.  Calls sink(tet, vol_tet) for each kept
+// output tet in the reference's order; returns how many were kept.
+template <class Sink>
+LSM_HD int clip_tet(const Tet& A, const Planes& P, double tol, Sink&& sink) {
+  int kept = 0;
+  Split s1 = classify(A, P.n[0], P.c[0]);
+  for (int i1 = 0; i1 < s1.nchild; i1++) {
+    Tet T1 = make_child(A, s1, i1);
+    Split s2 = classify(T1, P.n[1], P.c[1]);
+    for (int i2 = 0; i2 < s2.nchild; i2++) {
+      Tet T2 = make_child(T1, s2, i2);
+      Split s3 = classify(T2, P.n[2], P.c[2]);
+      for (int i3 = 0; i3 < s3.nchild; i3++) {
+        Tet T3 = make_child(T2, s3, i3);
+        Split s4 = classify(T3, P.n[3], P.c[3]);
+        for (int i4 = 0; i4 < s4.nchild; i4++) {
+          Tet T4 = make_child(T3, s4, i4);
+          double vol = tet_volume_tet(T4.v);
+          if (vol >= tol) {
+            sink(T4, vol);
+            kept++;
+          }
+        }
+      }
+    }
+  }
+  return kept;
+}
+
+}  // namespace lsm

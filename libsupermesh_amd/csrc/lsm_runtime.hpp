@@ -1,0 +1,138 @@
+// lsm_runtime.hpp -- context, device buffers and error reporting shared by
+// the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/libsupermesh_gpu.h"
+
+namespace lsm {
+
+// Raised inside the library, turned into a status code + last-error text at
+// the C boundary.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(int code, const char* msg);
+
+#define LSM_HIP(call)                                                               \
+  do {                                                                              \
+    hipError_t _e = (call);                                                         \
+    if (_e != hipSuccess)                                                           \
+      throw ::lsm::Error(LIBSUPERMESH_GPU_ERROR_HIP,                                \
+                         std::string(#call " failed: ") + hipGetErrorString(_e));   \
+  } while (0)
+
+#define LSM_REQUIRE(cond, code, msg)                \
+  do {                                              \
+    if (!(cond)) throw ::lsm::Error((code), (msg)); \
+  } while (0)
+
+// A device buffer that only grows; reused across calls on one context.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  void* get(size_t need) {
+    if (need == 0) need = 1;
+    if (need > bytes) {
+      if (ptr) (void)hipFree(ptr);
+      ptr = nullptr;
+      bytes = 0;
+      size_t cap = need + need / 4;
+      LSM_HIP(hipMalloc(&ptr, cap));
+      bytes = cap;
+    }
+    return ptr;
+  }
+  template <class T>
+  T* as(size_t n) {
+    return static_cast<T*>(get(n * sizeof(T)));
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+};
+
+// Mesh data staged on the device in the layouts the kernels read.
+struct DevMesh {
+  int dim = 0, loc = 0;
+  long nnodes = 0, nelements = 0;
+  const int* enlist_dev = nullptr;  // element-node list on the device (borrowed)
+  DevBuf enl_copy, pos_copy;        // owned copies when the caller passed host arrays
+  DevBuf coords;   // double [ne][loc][dim]: element vertex coordinates
+  DevBuf bbox;     // double [2*dim][ne]: lo_x.. lo_z, hi_x.. hi_z (SoA)
+  DevBuf meas_sp;  // double [ne]: SPACING of the clipper's element measure
+  DevBuf planes;   // double [ne][16]: tet planes (normals, c) of B
+  DevBuf affine;   // double [ne][loc][dim+1]: barycentric maps (mass)
+};
+
+// Implicit F-ary bounding volume hierarchy over Morton-sorted elements.
+struct DevBVH {
+  int dim = 0;
+  long n = 0;
+  int levels = 0;           // level 0 = elements, levels-1 = root
+  long count[32];           // nodes per level
+  long offset[32];          // node offset of each level in the box arrays
+  long total = 0;
+  DevBuf boxes;             // double [2*dim][total] SoA
+  DevBuf perm;              // int [n]: sorted position -> element id
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  DevMesh mesh_a, mesh_b;
+  DevBVH bvh;
+  // finder
+  DevBuf keys, keys_alt, vals, vals_alt, sort_tmp, scan_tmp;
+  DevBuf qperm;                  // A elements in Morton order
+  DevBuf row_count, indptr, indices, rows;
+  // clipping
+  DevBuf counts, measure, partials, scalars, offsets, elements, blocks;
+  // mass assembly
+  DevBuf mflag, mkeys, mvals, mkeys_alt, mvals_alt, ukeys, uvals, nuniq, mindptr, mindices;
+  // per-pair host API staging
+  DevBuf one_in, one_out;
+  void* pinned = nullptr;
+  size_t pinned_bytes = 0;
+  // timing
+  hipEvent_t ev[16];
+  bool events = false;
+  bool timing = false;  // record per-phase / clip-kernel events this call
+};
+
+Context* default_context();
+
+// Mesh staging + BVH (finder.hip).
+void stage_mesh(Context& C, DevMesh& M, int dim, int loc, long nnodes, long nelements,
+                const double* positions, const int* enlist, bool need_planes, bool need_affine);
+void build_bvh(Context& C, const DevMesh& B);
+// Count / fill candidate pairs of A against the BVH of B.  Returns npairs.
+long find_pairs(Context& C, const DevMesh& A, const DevMesh& B, int finder, int sort_rows);
+
+// Clipping (supermesh.hip).
+void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int flags,
+                libsupermesh_gpu_result* out);
+void assemble_mass(Context& C, const DevMesh& A, const DevMesh& B, long npairs,
+                   libsupermesh_gpu_result* out);
+
+inline unsigned grid_for(long n, int block) {
+  long g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > 0x7fffffffL) g = 0x7fffffffL;
+  return (unsigned)g;
+}
+
+}  // namespace lsm

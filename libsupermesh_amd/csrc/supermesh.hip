@@ -1,0 +1,479 @@
+// supermesh.hip -- clipping kernels, reductions and P1 x P1 mixed mass.
+//
+// One candidate pair per lane.  The kernels read element vertex coordinates
+// from the element-major arrays staged by finder.hip (48 B per triangle,
+// 96 B per tet) and, for tets, B's four planes precomputed once per element
+// (get_planes_tet is a pure function of the element, so this is bitwise the
+// same as recomputing it per pair).  Per-pair counts and measures are
+// bit-identical to the reference's intersect_tris / intersect_tets followed
+// by a plain in-order sum of triangle_area / tetrahedron_volume; the global
+// sum uses double-double (TwoSum) accumulation in a fixed reduction tree, so
+// it is deterministic for a given input.
+#include <cstring>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include "lsm_geometry.hpp"
+#include "lsm_runtime.hpp"
+
+namespace lsm {
+
+constexpr int kClipBlock = 256;
+constexpr int kClipGrid = 2048;  // fixed: the reduction tree depends only on npairs
+
+struct Partial {
+  double hi, lo;
+  long long nonempty, nout;
+  long long maxc, pad;
+};
+
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+  s = a + b;
+  double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+__device__ __forceinline__ void dd_add(double& hi, double& lo, double x) {
+  double s, e;
+  two_sum(hi, x, s, e);
+  hi = s;
+  lo = lo + e;
+}
+
+__device__ __forceinline__ void dd_merge(double& hi, double& lo, double hi2, double lo2) {
+  double s, e;
+  two_sum(hi, hi2, s, e);
+  hi = s;
+  lo = lo + (e + lo2);
+}
+
+// Quadrature rules of the reference's mixed-mass tests: Dunavant degree 2 on
+// triangles (barycentric (4,1,1)/6, weights 1/3; test_parallel_p1_inner_product_2d.F90:59-68)
+// and the 4-point Stroud rule on tets (..._3d.F90:61-80).
+__constant__ double kQ2[3][3] = {{4.0 / 6.0, 1.0 / 6.0, 1.0 / 6.0},
+                                 {1.0 / 6.0, 4.0 / 6.0, 1.0 / 6.0},
+                                 {1.0 / 6.0, 1.0 / 6.0, 4.0 / 6.0}};
+#define LSM_QA 5.8541019662496852e-1
+#define LSM_QB 1.3819660112501050e-1
+__constant__ double kQ3[4][4] = {{LSM_QA, LSM_QB, LSM_QB, LSM_QB},
+                                 {LSM_QB, LSM_QA, LSM_QB, LSM_QB},
+                                 {LSM_QB, LSM_QB, LSM_QA, LSM_QB},
+                                 {LSM_QB, LSM_QB, LSM_QB, LSM_QA}};
+
+template <int DIM>
+__device__ __forceinline__ void basis(const double* __restrict__ aff, const double* x, double* f) {
+  if constexpr (DIM == 2) {
+    double dx = x[0] - aff[0], dy = x[1] - aff[1];
+    double f2 = aff[2] * dx;
+    f2 = f2 + aff[3] * dy;
+    double f3 = aff[4] * dx;
+    f3 = f3 + aff[5] * dy;
+    f[1] = f2;
+    f[2] = f3;
+    f[0] = 1.0 - f2 - f3;
+  } else {
+    double dx = x[0] - aff[0], dy = x[1] - aff[1], dz = x[2] - aff[2];
+    double s = 0.0;
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      double v = aff[3 + 3 * r] * dx;
+      v = v + aff[4 + 3 * r] * dy;
+      v = v + aff[5 + 3 * r] * dz;
+      f[1 + r] = v;
+    }
+    s = 1.0 - f[1] - f[2] - f[3];
+    f[0] = s;
+  }
+}
+
+// Adds the element's quadrature contributions to blk[LOC][LOC].
+template <int DIM>
+__device__ __forceinline__ void mass_element(const double* el, double meas, const double* affa,
+                                             const double* affb, double* blk) {
+  constexpr int LOC = DIM + 1;
+  constexpr int NQ = DIM == 2 ? 3 : 4;
+  const double wq = DIM == 2 ? 1.0 / 3.0 : 0.25;
+  const double wm = wq * meas;
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    double x[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      double s = 0.0;
+#pragma unroll
+      for (int v = 0; v < LOC; v++) s = s + (DIM == 2 ? kQ2[q][v] : kQ3[q][v]) * el[v * DIM + d];
+      x[d] = s;
+    }
+    double fa[LOC], fb[LOC];
+    basis<DIM>(affa, x, fa);
+    basis<DIM>(affb, x, fb);
+#pragma unroll
+    for (int i = 0; i < LOC; i++)
+#pragma unroll
+      for (int j = 0; j < LOC; j++) blk[i * LOC + j] = blk[i * LOC + j] + wm * fa[i] * fb[j];
+  }
+}
+
+struct ClipArgs {
+  long npairs;
+  const int* rows;
+  const int* cols;
+  const double* acoords;
+  const double* bcoords;
+  const double* asp;
+  const double* bsp;
+  const double* bplanes;
+  const double* aaff;
+  const double* baff;
+  int* counts;
+  double* measure;
+  double* blocks;
+  const long* offsets;
+  double* elements;
+  Partial* partials;
+};
+
+template <int DIM, bool MASS, bool FILL>
+__global__ void __launch_bounds__(kClipBlock) k_clip(ClipArgs g) {
+  constexpr int LOC = DIM + 1;
+  constexpr int ESZ = LOC * DIM;
+  double acc_hi = 0.0, acc_lo = 0.0;
+  long long nonempty = 0, nout = 0, maxc = 0;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < g.npairs; p += stride) {
+    const long a = g.rows[p], b = g.cols[p];
+    double A[ESZ], Bv[ESZ];
+#pragma unroll
+    for (int k = 0; k < ESZ; k++) A[k] = g.acoords[a * ESZ + k];
+    const double tol = 10.0 * fmin_f(g.asp[a], g.bsp[b]);
+    double m = 0.0;
+    double blk[LOC * LOC];
+    double affa[DIM + DIM * DIM], affb[DIM + DIM * DIM];
+    if (MASS) {
+#pragma unroll
+      for (int k = 0; k < LOC * LOC; k++) blk[k] = 0.0;
+#pragma unroll
+      for (int k = 0; k < DIM + DIM * DIM; k++) {
+        affa[k] = g.aaff[a * (DIM + DIM * DIM) + k];
+        affb[k] = g.baff[b * (DIM + DIM * DIM) + k];
+      }
+    }
+    long off = 0;
+    int cap = 0;  // fill pass: the count pass's result bounds the writes
+    if (FILL) {
+      cap = g.counts[p];
+      if (cap == 0) continue;
+      off = g.offsets[p];
+    }
+    int n;
+    if constexpr (DIM == 2) {
+#pragma unroll
+      for (int k = 0; k < ESZ; k++) Bv[k] = g.bcoords[b * ESZ + k];
+      Line L[3];
+      tri_lines(Bv, L);
+      int k = 0;
+      n = clip_tri(A, L, tol, [&](double x0, double y0, double x1, double y1, double x2, double y2,
+                                  double ar) {
+        m = m + ar;
+        if (MASS) {
+          double el[6] = {x0, y0, x1, y1, x2, y2};
+          mass_element<2>(el, ar, affa, affb, blk);
+        }
+        if (FILL && k < cap) {
+          double* o = g.elements + (off + k) * 6;
+          o[0] = x0;
+          o[1] = y0;
+          o[2] = x1;
+          o[3] = y1;
+          o[4] = x2;
+          o[5] = y2;
+        }
+        k++;
+      });
+    } else {
+      Planes P;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+#pragma unroll
+        for (int kk = 0; kk < 3; kk++) P.n[i][kk] = g.bplanes[b * 16 + 4 * i + kk];
+        P.c[i] = g.bplanes[b * 16 + 4 * i + 3];
+      }
+      Tet TA;
+#pragma unroll
+      for (int kk = 0; kk < 12; kk++) TA.v[kk] = A[kk];
+      int k = 0;
+      n = clip_tet(TA, P, tol, [&](const Tet& T, double) {
+        double vr = tet_volume_real(T.v);
+        m = m + vr;
+        if (MASS) mass_element<3>(T.v, vr, affa, affb, blk);
+        if (FILL && k < cap) {
+          double* o = g.elements + (off + k) * 12;
+#pragma unroll
+          for (int kk = 0; kk < 12; kk++) o[kk] = T.v[kk];
+        }
+        k++;
+      });
+    }
+    if (FILL) continue;
+    if (n == 0) m = 0.0;
+    g.counts[p] = n;
+    g.measure[p] = m;
+    if (MASS) {
+#pragma unroll
+      for (int k = 0; k < LOC * LOC; k++) g.blocks[p * LOC * LOC + k] = n == 0 ? 0.0 : blk[k];
+    }
+    dd_add(acc_hi, acc_lo, m);
+    nonempty += n > 0;
+    nout += n;
+    maxc = n > maxc ? n : maxc;
+  }
+  if (FILL) return;
+  __shared__ Partial red[kClipBlock];
+  red[threadIdx.x].hi = acc_hi;
+  red[threadIdx.x].lo = acc_lo;
+  red[threadIdx.x].nonempty = nonempty;
+  red[threadIdx.x].nout = nout;
+  red[threadIdx.x].maxc = maxc;
+  __syncthreads();
+  for (int s = kClipBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      Partial& x = red[threadIdx.x];
+      const Partial& y = red[threadIdx.x + s];
+      dd_merge(x.hi, x.lo, y.hi, y.lo);
+      x.nonempty += y.nonempty;
+      x.nout += y.nout;
+      x.maxc = y.maxc > x.maxc ? y.maxc : x.maxc;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) g.partials[blockIdx.x] = red[0];
+}
+
+__global__ void k_reduce_partials(int n, Partial* __restrict__ parts) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  Partial t = parts[0];
+  for (int i = 1; i < n; i++) {
+    dd_merge(t.hi, t.lo, parts[i].hi, parts[i].lo);
+    t.nonempty += parts[i].nonempty;
+    t.nout += parts[i].nout;
+    t.maxc = parts[i].maxc > t.maxc ? parts[i].maxc : t.maxc;
+  }
+  parts[0] = t;
+}
+
+template <int DIM>
+static void launch_clip(Context& C, const ClipArgs& args, bool mass, bool fill) {
+  dim3 grid(kClipGrid), block(kClipBlock);
+  if (fill) {
+    hipLaunchKernelGGL((k_clip<DIM, false, true>), grid, block, 0, C.stream, args);
+  } else if (mass) {
+    hipLaunchKernelGGL((k_clip<DIM, true, false>), grid, block, 0, C.stream, args);
+  } else {
+    hipLaunchKernelGGL((k_clip<DIM, false, false>), grid, block, 0, C.stream, args);
+  }
+  LSM_HIP(hipGetLastError());
+}
+
+__global__ void __launch_bounds__(256)
+k_count_to_long(long n, const int* __restrict__ counts, long* __restrict__ out) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = counts[i];
+}
+
+void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int flags,
+                libsupermesh_gpu_result* out) {
+  const int dim = A.dim, loc = dim + 1;
+  const bool mass = flags & LIBSUPERMESH_GPU_MASS;
+  const bool elems = flags & LIBSUPERMESH_GPU_ELEMENTS;
+  ClipArgs g = {};
+  g.npairs = npairs;
+  g.rows = out->rows;
+  g.cols = out->indices;
+  g.acoords = static_cast<const double*>(A.coords.ptr);
+  g.bcoords = static_cast<const double*>(B.coords.ptr);
+  g.asp = static_cast<const double*>(A.meas_sp.ptr);
+  g.bsp = static_cast<const double*>(B.meas_sp.ptr);
+  g.bplanes = static_cast<const double*>(B.planes.ptr);
+  g.aaff = static_cast<const double*>(A.affine.ptr);
+  g.baff = static_cast<const double*>(B.affine.ptr);
+  g.counts = C.counts.as<int>(npairs);
+  g.measure = C.measure.as<double>(npairs);
+  g.blocks = mass ? C.blocks.as<double>((size_t)npairs * loc * loc) : nullptr;
+  g.partials = C.partials.as<Partial>(kClipGrid);
+  if (C.timing) LSM_HIP(hipEventRecord(C.ev[14], C.stream));
+  if (dim == 2) launch_clip<2>(C, g, mass, false);
+  else launch_clip<3>(C, g, mass, false);
+  if (C.timing) LSM_HIP(hipEventRecord(C.ev[15], C.stream));
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, C.stream, kClipGrid, g.partials);
+  LSM_HIP(hipGetLastError());
+  Partial tot;
+  LSM_HIP(hipMemcpyAsync(&tot, g.partials, sizeof(Partial), hipMemcpyDeviceToHost, C.stream));
+  LSM_HIP(hipStreamSynchronize(C.stream));
+  if (C.timing) {
+    float ms = 0.f;
+    LSM_HIP(hipEventElapsedTime(&ms, C.ev[14], C.ev[15]));
+    out->ms[7] = ms;
+  }
+  out->counts = g.counts;
+  out->measure = g.measure;
+  out->blocks = g.blocks;
+  out->sum = npairs > 0 ? tot.hi + tot.lo : 0.0;
+  out->nonempty = npairs > 0 ? tot.nonempty : 0;
+  out->nout = npairs > 0 ? tot.nout : 0;
+  out->max_count = npairs > 0 ? (int)tot.maxc : 0;
+  out->offsets = nullptr;
+  out->elements = nullptr;
+  if (elems) {
+    long* offs = C.offsets.as<long>(npairs + 1);
+    LSM_HIP(hipMemsetAsync(offs, 0, sizeof(long), C.stream));
+    if (npairs > 0) {
+      long* tmp_counts = C.row_count.as<long>(npairs);
+      hipLaunchKernelGGL(k_count_to_long, dim3(grid_for(npairs, 256)), dim3(256), 0, C.stream, npairs,
+                         g.counts, tmp_counts);
+      size_t bytes = 0;
+      LSM_HIP(rocprim::inclusive_scan(nullptr, bytes, tmp_counts, offs + 1, (size_t)npairs,
+                                      rocprim::plus<long>(), C.stream));
+      void* tmp = C.scan_tmp.get(bytes);
+      LSM_HIP(rocprim::inclusive_scan(tmp, bytes, tmp_counts, offs + 1, (size_t)npairs,
+                                      rocprim::plus<long>(), C.stream));
+      g.offsets = offs;
+      g.elements = C.elements.as<double>((size_t)(out->nout > 0 ? out->nout : 1) * loc * dim);
+      if (dim == 2) launch_clip<2>(C, g, false, true);
+      else launch_clip<3>(C, g, false, true);
+    }
+    out->offsets = offs;
+    out->elements = static_cast<const double*>(C.elements.get(8));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Mixed mass assembly: (node_a, node_b, value) entries of non-empty pairs,
+// sorted by key (stable, so equal keys keep pair order), then summed per key
+// by a sequential walk -- deterministic.
+// ---------------------------------------------------------------------------
+
+template <int LOC>
+__global__ void __launch_bounds__(256)
+k_mass_entries(long npairs, const int* __restrict__ counts, const long* __restrict__ slot,
+               const int* __restrict__ rows, const int* __restrict__ cols,
+               const int* __restrict__ enla, const int* __restrict__ enlb,
+               const double* __restrict__ blocks, long nb, uint64_t* __restrict__ keys,
+               double* __restrict__ vals) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npairs || counts[p] == 0) return;
+  long s = slot[p] - 1;  // inclusive scan of the non-empty flags
+  long a = rows[p], b = cols[p];
+#pragma unroll
+  for (int i = 0; i < LOC; i++) {
+    uint64_t ni = (uint64_t)enla[a * LOC + i];
+#pragma unroll
+    for (int j = 0; j < LOC; j++) {
+      uint64_t nj = (uint64_t)enlb[b * LOC + j];
+      long idx = (s * LOC + i) * LOC + j;
+      keys[idx] = ni * (uint64_t)nb + nj;
+      vals[idx] = blocks[p * LOC * LOC + i * LOC + j];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_nonempty_flag(long n, const int* __restrict__ counts, long* __restrict__ flag) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flag[i] = counts[i] > 0;
+}
+
+__global__ void __launch_bounds__(256)
+k_heads(long n, const uint64_t* __restrict__ keys, long* __restrict__ head) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) head[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(256)
+k_segment_sum(long n, const uint64_t* __restrict__ keys, const double* __restrict__ vals,
+              const long* __restrict__ pos, long nb, int* __restrict__ mind,
+              double* __restrict__ mval, unsigned long long* __restrict__ rowcnt) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t k = keys[i];
+  if (i > 0 && keys[i - 1] == k) return;
+  double s = 0.0;
+  for (long j = i; j < n && keys[j] == k; j++) s = s + vals[j];
+  long o = pos[i] - 1;
+  mind[o] = (int)(k % (uint64_t)nb);
+  mval[o] = s;
+  atomicAdd(&rowcnt[k / (uint64_t)nb], 1ULL);
+}
+
+template <class T>
+static void inclusive_scan_long(Context& C, const T* in, long* out, long n) {
+  size_t bytes = 0;
+  LSM_HIP(rocprim::inclusive_scan(nullptr, bytes, in, out, (size_t)n, rocprim::plus<long>(), C.stream));
+  void* tmp = C.scan_tmp.get(bytes);
+  LSM_HIP(rocprim::inclusive_scan(tmp, bytes, in, out, (size_t)n, rocprim::plus<long>(), C.stream));
+}
+
+void assemble_mass(Context& C, const DevMesh& A, const DevMesh& B, long npairs,
+                   libsupermesh_gpu_result* out) {
+  const int loc = A.dim + 1;
+  const long na_nodes = A.nnodes, nb_nodes = B.nnodes;
+  long* mindptr = C.mindptr.as<long>(na_nodes + 1);
+  LSM_HIP(hipMemsetAsync(mindptr, 0, sizeof(long) * (na_nodes + 1), C.stream));
+  out->mass_nnz = 0;
+  out->mass_indptr = mindptr;
+  out->mass_indices = static_cast<const int*>(C.mindices.get(4));
+  out->mass_values = static_cast<const double*>(C.uvals.get(8));
+  if (out->nonempty == 0) {
+    LSM_HIP(hipStreamSynchronize(C.stream));
+    return;
+  }
+  long* flag = C.mflag.as<long>(npairs);
+  long* slot = C.row_count.as<long>(npairs);
+  hipLaunchKernelGGL(k_nonempty_flag, dim3(grid_for(npairs, 256)), dim3(256), 0, C.stream, npairs,
+                     out->counts, flag);
+  inclusive_scan_long(C, flag, slot, npairs);
+  const long nent = out->nonempty * loc * loc;
+  uint64_t* keys = C.mkeys.as<uint64_t>(nent);
+  double* vals = C.mvals.as<double>(nent);
+  uint64_t* keys2 = C.mkeys_alt.as<uint64_t>(nent);
+  double* vals2 = C.mvals_alt.as<double>(nent);
+  if (loc == 3)
+    hipLaunchKernelGGL(k_mass_entries<3>, dim3(grid_for(npairs, 256)), dim3(256), 0, C.stream, npairs,
+                       out->counts, slot, out->rows, out->indices, A.enlist_dev, B.enlist_dev,
+                       out->blocks, nb_nodes, keys, vals);
+  else
+    hipLaunchKernelGGL(k_mass_entries<4>, dim3(grid_for(npairs, 256)), dim3(256), 0, C.stream, npairs,
+                       out->counts, slot, out->rows, out->indices, A.enlist_dev, B.enlist_dev,
+                       out->blocks, nb_nodes, keys, vals);
+  LSM_HIP(hipGetLastError());
+  unsigned __int128 span = (unsigned __int128)na_nodes * (unsigned __int128)nb_nodes;
+  unsigned end_bit = 1;
+  while (end_bit < 64 && ((unsigned __int128)1 << end_bit) < span) end_bit++;
+  size_t bytes = 0;
+  LSM_HIP(rocprim::radix_sort_pairs(nullptr, bytes, keys, keys2, vals, vals2, (size_t)nent, 0u, end_bit,
+                                    C.stream));
+  void* tmp = C.sort_tmp.get(bytes);
+  LSM_HIP(rocprim::radix_sort_pairs(tmp, bytes, keys, keys2, vals, vals2, (size_t)nent, 0u, end_bit,
+                                    C.stream));
+  long* head = C.ukeys.as<long>(nent);
+  long* pos = C.nuniq.as<long>(nent);
+  hipLaunchKernelGGL(k_heads, dim3(grid_for(nent, 256)), dim3(256), 0, C.stream, nent, keys2, head);
+  inclusive_scan_long(C, head, pos, nent);
+  long nnz = 0;
+  LSM_HIP(hipMemcpyAsync(&nnz, pos + nent - 1, sizeof(long), hipMemcpyDeviceToHost, C.stream));
+  LSM_HIP(hipStreamSynchronize(C.stream));
+  int* mind = C.mindices.as<int>(nnz);
+  double* mval = C.uvals.as<double>(nnz);
+  unsigned long long* rowcnt = reinterpret_cast<unsigned long long*>(C.mkeys.as<uint64_t>(na_nodes + 1));
+  LSM_HIP(hipMemsetAsync(rowcnt, 0, sizeof(uint64_t) * (na_nodes + 1), C.stream));
+  hipLaunchKernelGGL(k_segment_sum, dim3(grid_for(nent, 256)), dim3(256), 0, C.stream, nent, keys2, vals2,
+                     pos, nb_nodes, mind, mval, rowcnt);
+  LSM_HIP(hipGetLastError());
+  inclusive_scan_long(C, reinterpret_cast<const long*>(rowcnt), mindptr + 1, na_nodes);
+  LSM_HIP(hipStreamSynchronize(C.stream));
+  out->mass_nnz = nnz;
+  out->mass_indices = mind;
+  out->mass_values = mval;
+}
+
+}  // namespace lsm

@@ -1,0 +1,334 @@
+"""ctypes binding of the C ABI (include/libsupermesh-c.h, include/libsupermesh_gpu.h).
+
+Two layers, both thin:
+
+* the reference's drop-in functions on host arrays -- ``tree_intersection_finder``,
+  ``rtree_intersection_finder`` (set_input / query_output / get_output),
+  ``intersect_tris_real``, ``intersect_tets_real``, ``triangle_area``,
+  ``tetrahedron_volume`` -- with the reference's argument meaning;
+* the bulk device API (``Context.supermesh`` / ``Context.intersection_finder`` /
+  ``Context.clip_pairs``) on meshes already resident in device memory.
+
+Device memory comes from the library's own HIP runtime
+(libsupermesh_gpu_malloc / _memcpy): a second GPU runtime in the same process
+(for instance one bundled with another Python package) would compete for the
+device.  Every computation runs in the HIP library, and loading fails loudly
+if it was not built.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_double, c_int, c_long, c_void_p
+
+import numpy as np
+
+from . import load
+
+FINDER_TREE = 0
+FINDER_RTREE = 1
+ELEMENTS = 1
+MASS = 2
+TIMING = 4
+UNSORTED = 8
+
+_lib = None
+
+
+class GpuError(RuntimeError):
+    """A failure reported by the library (code + text of last_error)."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"libsupermesh_gpu error {code}: {msg}")
+        self.code = code
+
+
+class Mesh(ctypes.Structure):
+    _fields_ = [("dim", c_int), ("nnodes", c_long), ("nelements", c_long),
+                ("positions", c_void_p), ("enlist", c_void_p)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("nelements_a", c_long), ("npairs", c_long), ("indptr", c_void_p),
+                ("indices", c_void_p), ("rows", c_void_p), ("counts", c_void_p),
+                ("measure", c_void_p), ("offsets", c_void_p), ("elements", c_void_p),
+                ("blocks", c_void_p), ("mass_nnz", c_long), ("mass_indptr", c_void_p),
+                ("mass_indices", c_void_p), ("mass_values", c_void_p), ("sum", c_double),
+                ("nonempty", c_long), ("nout", c_long), ("max_count", c_int),
+                ("ms", c_double * 8)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = load()
+        L.libsupermesh_gpu_last_error.restype = ctypes.c_char_p
+        L.libsupermesh_gpu_last_error_code.restype = c_int
+        L.libsupermesh_gpu_context_create.argtypes = [c_int, POINTER(c_void_p)]
+        L.libsupermesh_gpu_context_destroy.argtypes = [c_void_p]
+        L.libsupermesh_gpu_context_stream.argtypes = [c_void_p]
+        L.libsupermesh_gpu_context_stream.restype = c_void_p
+        for name in ("libsupermesh_gpu_supermesh", "libsupermesh_gpu_intersection_finder"):
+            f = getattr(L, name)
+            f.argtypes = [c_void_p, POINTER(Mesh), POINTER(Mesh), c_int, c_int, POINTER(Result)]
+            f.restype = c_int
+        L.libsupermesh_gpu_clip_pairs.argtypes = [c_void_p, POINTER(Mesh), POINTER(Mesh), c_long,
+                                                  c_void_p, c_void_p, c_int, POINTER(Result)]
+        L.libsupermesh_gpu_clip_pairs.restype = c_int
+        L.libsupermesh_gpu_malloc.argtypes = [c_int, ctypes.c_size_t, POINTER(c_void_p)]
+        L.libsupermesh_gpu_malloc.restype = c_int
+        L.libsupermesh_gpu_free.argtypes = [c_void_p]
+        L.libsupermesh_gpu_free.restype = c_int
+        L.libsupermesh_gpu_memcpy.argtypes = [c_void_p, c_void_p, ctypes.c_size_t, c_int]
+        L.libsupermesh_gpu_memcpy.restype = c_int
+        L.libsupermesh_gpu_synchronize.argtypes = [c_int]
+        L.libsupermesh_gpu_synchronize.restype = c_int
+        _lib = L
+    return _lib
+
+
+def last_error():
+    L = lib()
+    return L.libsupermesh_gpu_last_error_code(), L.libsupermesh_gpu_last_error().decode()
+
+
+def _check(rc):
+    if rc != 0:
+        code, msg = last_error()
+        raise GpuError(code, msg)
+
+
+def _raise_if_error():
+    code, msg = last_error()
+    if code != 0:
+        raise GpuError(code, msg)
+
+
+def device_to_numpy(ptr, count, dtype):
+    """Copy `count` items of `dtype` from device address `ptr` to a new array."""
+    out = np.empty(int(count), dtype=dtype)
+    if count > 0:
+        _check(lib().libsupermesh_gpu_memcpy(out.ctypes.data, ptr, out.nbytes, 2))
+    return out
+
+
+def synchronize(device=0):
+    _check(lib().libsupermesh_gpu_synchronize(device))
+
+
+class DeviceArray:
+    """A device allocation holding a copy of a numpy array."""
+
+    def __init__(self, host, device=0):
+        host = np.ascontiguousarray(host)
+        self.shape, self.dtype, self.nbytes = host.shape, host.dtype, host.nbytes
+        self.ptr = c_void_p()
+        _check(lib().libsupermesh_gpu_malloc(device, max(host.nbytes, 1), ctypes.byref(self.ptr)))
+        if host.nbytes:
+            _check(lib().libsupermesh_gpu_memcpy(self.ptr, host.ctypes.data, host.nbytes, 1))
+
+    def numpy(self):
+        return device_to_numpy(self.ptr.value, int(np.prod(self.shape)), self.dtype).reshape(self.shape)
+
+    def free(self):
+        if self.ptr:
+            lib().libsupermesh_gpu_free(self.ptr)
+            self.ptr = c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# Drop-in functions (host arrays, reference semantics)
+# ---------------------------------------------------------------------------
+
+def _pd(a):
+    return a.ctypes.data_as(POINTER(c_double))
+
+
+def _pl(a):
+    return a.ctypes.data_as(POINTER(c_long))
+
+
+def _finder(prefix, pos_a, enl_a, pos_b, enl_b):
+    L = lib()
+    pos_a = np.ascontiguousarray(pos_a, dtype=np.float64)
+    pos_b = np.ascontiguousarray(pos_b, dtype=np.float64)
+    enl_a = np.ascontiguousarray(enl_a, dtype=np.int64)
+    enl_b = np.ascontiguousarray(enl_b, dtype=np.int64)
+    nn_a, dim_a = c_long(pos_a.shape[0]), c_int(pos_a.shape[1])
+    ne_a, loc_a = c_long(enl_a.shape[0]), c_int(enl_a.shape[1])
+    nn_b, dim_b = c_long(pos_b.shape[0]), c_int(pos_b.shape[1])
+    ne_b, loc_b = c_long(enl_b.shape[0]), c_int(enl_b.shape[1])
+    getattr(L, prefix + "_set_input")(ctypes.byref(nn_a), ctypes.byref(dim_a), ctypes.byref(ne_a),
+                                      ctypes.byref(loc_a), ctypes.byref(nn_b), ctypes.byref(dim_b),
+                                      ctypes.byref(ne_b), ctypes.byref(loc_b), _pd(pos_a),
+                                      _pl(enl_a), _pd(pos_b), _pl(enl_b))
+    _raise_if_error()
+    nind = c_long(0)
+    getattr(L, prefix + "_query_output")(ctypes.byref(nind))
+    _raise_if_error()
+    indices = np.zeros(max(nind.value, 1), dtype=np.int64)
+    indptr = np.zeros(ne_a.value + 1, dtype=np.int64)
+    getattr(L, prefix + "_get_output")(ctypes.byref(ne_a), ctypes.byref(nind), _pl(indices), _pl(indptr))
+    _raise_if_error()
+    return indptr, indices[:nind.value]
+
+
+def tree_intersection_finder(pos_a, enl_a, pos_b, enl_b):
+    """Strict-overlap candidate pairs as CSR ``(indptr, indices)``, 0-based."""
+    return _finder("libsupermesh_tree_intersection_finder", pos_a, enl_a, pos_b, enl_b)
+
+
+def rtree_intersection_finder(pos_a, enl_a, pos_b, enl_b):
+    """Closed-overlap (R-tree semantics) candidate pairs as CSR."""
+    return _finder("libsupermesh_rtree_intersection_finder", pos_a, enl_a, pos_b, enl_b)
+
+
+def intersect_tris_real(tri_a, tri_b):
+    """Triangles of tri_a ^ tri_b, ``float64[n, 3, 2]``."""
+    a = np.ascontiguousarray(tri_a, dtype=np.float64)
+    b = np.ascontiguousarray(tri_b, dtype=np.float64)
+    out = np.zeros((22, 3, 2))
+    n = c_int(0)
+    lib().libsupermesh_intersect_tris_real(_pd(a), _pd(b), _pd(out), ctypes.byref(n))
+    _raise_if_error()
+    return out[:n.value].copy()
+
+
+def intersect_tets_real(tet_a, tet_b):
+    """Tetrahedra of tet_a ^ tet_b, ``float64[n, 4, 3]``."""
+    a = np.ascontiguousarray(tet_a, dtype=np.float64)
+    b = np.ascontiguousarray(tet_b, dtype=np.float64)
+    out = np.zeros((81, 4, 3))
+    n = c_int(0)
+    lib().libsupermesh_intersect_tets_real(_pd(a), _pd(b), _pd(out), ctypes.byref(n))
+    _raise_if_error()
+    return out[:n.value].copy()
+
+
+def triangle_area(tri):
+    t = np.ascontiguousarray(tri, dtype=np.float64)
+    r = c_double(0.0)
+    lib().libsupermesh_triangle_area(_pd(t), ctypes.byref(r))
+    _raise_if_error()
+    return r.value
+
+
+def tetrahedron_volume(tet):
+    t = np.ascontiguousarray(tet, dtype=np.float64)
+    r = c_double(0.0)
+    lib().libsupermesh_tetrahedron_volume(_pd(t), ctypes.byref(r))
+    _raise_if_error()
+    return r.value
+
+
+# ---------------------------------------------------------------------------
+# Bulk device API
+# ---------------------------------------------------------------------------
+
+class DeviceMesh:
+    """A mesh copied to device memory: positions float64[nnodes, dim],
+    enlist int32[nelements, dim+1] (0-based)."""
+
+    def __init__(self, positions, enlist, device=0):
+        pos = np.ascontiguousarray(positions, dtype=np.float64)
+        enl = np.ascontiguousarray(enlist, dtype=np.int32)
+        self.dim = pos.shape[1]
+        self.positions = DeviceArray(pos, device)
+        self.enlist = DeviceArray(enl, device)
+        self.struct = Mesh(self.dim, pos.shape[0], enl.shape[0], self.positions.ptr.value,
+                           self.enlist.ptr.value)
+
+    @property
+    def nnodes(self):
+        return self.struct.nnodes
+
+    @property
+    def nelements(self):
+        return self.struct.nelements
+
+
+class Context:
+    """A library context (its own HIP stream and reusable device buffers).
+    Results returned by a call stay valid until the next call."""
+
+    def __init__(self, device=0):
+        self.handle = c_void_p()
+        _check(lib().libsupermesh_gpu_context_create(device, ctypes.byref(self.handle)))
+
+    def close(self):
+        if self.handle:
+            lib().libsupermesh_gpu_context_destroy(self.handle)
+            self.handle = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def stream(self):
+        return lib().libsupermesh_gpu_context_stream(self.handle)
+
+    def supermesh(self, a: DeviceMesh, b: DeviceMesh, finder=FINDER_TREE, flags=0):
+        r = Result()
+        _check(lib().libsupermesh_gpu_supermesh(self.handle, ctypes.byref(a.struct),
+                                                ctypes.byref(b.struct), finder, flags,
+                                                ctypes.byref(r)))
+        return r
+
+    def intersection_finder(self, a: DeviceMesh, b: DeviceMesh, finder=FINDER_TREE, flags=0):
+        r = Result()
+        _check(lib().libsupermesh_gpu_intersection_finder(self.handle, ctypes.byref(a.struct),
+                                                          ctypes.byref(b.struct), finder, flags,
+                                                          ctypes.byref(r)))
+        return r
+
+    def clip_pairs(self, a: DeviceMesh, b: DeviceMesh, pair_a: DeviceArray, pair_b: DeviceArray,
+                   flags=0):
+        """pair_a / pair_b: int32 device arrays of equal length."""
+        r = Result()
+        n = int(np.prod(pair_a.shape))
+        _check(lib().libsupermesh_gpu_clip_pairs(self.handle, ctypes.byref(a.struct),
+                                                 ctypes.byref(b.struct), n, pair_a.ptr,
+                                                 pair_b.ptr, flags, ctypes.byref(r)))
+        return r
+
+
+def fetch(r: Result, dim, what=("indptr", "indices", "rows", "counts", "measure")):
+    """Copy the requested result arrays to host numpy arrays."""
+    loc = dim + 1
+    out = {}
+    for w in what:
+        if w == "indptr":
+            out[w] = device_to_numpy(r.indptr, r.nelements_a + 1, np.int64)
+        elif w == "indices":
+            out[w] = device_to_numpy(r.indices, r.npairs, np.int32)
+        elif w == "rows":
+            out[w] = device_to_numpy(r.rows, r.npairs, np.int32)
+        elif w == "counts":
+            out[w] = device_to_numpy(r.counts, r.npairs, np.int32)
+        elif w == "measure":
+            out[w] = device_to_numpy(r.measure, r.npairs, np.float64)
+        elif w == "offsets":
+            out[w] = device_to_numpy(r.offsets, r.npairs + 1, np.int64)
+        elif w == "elements":
+            out[w] = device_to_numpy(r.elements, r.nout * loc * dim, np.float64).reshape(-1, loc, dim)
+        elif w == "blocks":
+            out[w] = device_to_numpy(r.blocks, r.npairs * loc * loc, np.float64).reshape(-1, loc, loc)
+        else:
+            raise KeyError(w)
+    return out
+
+
+def fetch_mass(r: Result, nnodes_a):
+    """The assembled mixed mass matrix as CSR arrays (indptr, indices, values)."""
+    indptr = device_to_numpy(r.mass_indptr, nnodes_a + 1, np.int64)
+    indices = device_to_numpy(r.mass_indices, r.mass_nnz, np.int32)
+    values = device_to_numpy(r.mass_values, r.mass_nnz, np.float64)
+    return indptr, indices, values

@@ -54,6 +54,29 @@
 #include <string.h>
 #include <time.h>
 
+/* Operation counters (built into liboracle_count.so with -DORC_COUNT):
+ * F(n) counts n FP64 add/sub/mul/div/sqrt, K(n) counts n FP64 compares,
+ * abs, min or max.  Empty in the normal build. */
+#ifdef ORC_COUNT
+static __thread unsigned long long orc_nflop, orc_ncmp;
+#define F(n) (orc_nflop += (n))
+#define K(n) (orc_ncmp += (n))
+#else
+#define F(n) ((void)0)
+#define K(n) ((void)0)
+#endif
+
+void orc_op_counts(unsigned long long *out, int reset) {
+#ifdef ORC_COUNT
+  out[0] = orc_nflop;
+  out[1] = orc_ncmp;
+  if (reset) orc_nflop = orc_ncmp = 0;
+#else
+  (void)reset;
+  out[0] = out[1] = 0;
+#endif
+}
+
 /* ------------------------------------------------------------------------ */
 /* Fortran intrinsics                                                        */
 /* ------------------------------------------------------------------------ */
@@ -77,24 +100,29 @@ double orc_norm2_3(const double v[3]) {
   double mx = 0.0, s = 0.0;
   for (int i = 0; i < 3; i++) {
     double ax = fabs(v[i]);
+    K(2);
     if (mx == 0.0) {
       mx = ax;
-    } else if (ax > mx) {
+    } else if ((K(1), ax > mx)) {
+      F(4);
       double t = mx / ax;
       double tsq = t * t;
       s = s * tsq;
       s = s + tsq;
       mx = ax;
     } else {
+      F(3);
       double t = ax / mx;
       s = s + t * t;
     }
   }
+  F(3);
   return mx * sqrt(1.0 + s);
 }
 
 /* DOT_PRODUCT over 3 reals: sequential accumulation. */
 double orc_dot3(const double a[3], const double b[3]) {
+  F(5);
   double s = a[0] * b[0];
   s = s + a[1] * b[1];
   s = s + a[2] * b[2];
@@ -106,12 +134,16 @@ double orc_dot3(const double a[3], const double b[3]) {
 /* ------------------------------------------------------------------------ */
 
 double orc_triangle_area(const double t[6]) {
+  F(8);
+  K(1);
   double ux = t[4] - t[0], uy = t[5] - t[1];
   double vx = t[2] - t[0], vy = t[3] - t[1];
   return 0.5 * fabs(uy * vx - ux * vy);
 }
 
 double orc_tetrahedron_volume_real(const double t[12]) {
+  F(24);
+  K(1);
   double e1[3], e2[3], e3[3];
   for (int i = 0; i < 3; i++) {
     e1[i] = t[3 + i] - t[i];
@@ -126,6 +158,8 @@ double orc_tetrahedron_volume_real(const double t[12]) {
 }
 
 double orc_tetrahedron_volume_tet(const double t[12]) {
+  F(19);  /* + dot3 */
+  K(1);
   double a[3], b[3], c[3], cr[3];
   for (int i = 0; i < 3; i++) {
     a[i] = t[i] - t[9 + i];
@@ -145,6 +179,8 @@ double orc_tetrahedron_volume_tet(const double t[12]) {
 typedef struct { double n[2], p[2]; } orc_line;
 
 static void get_lines_tri(const double t[6], orc_line L[3]) {
+  F(11);
+  K(1);
   for (int k = 0; k < 3; k++) {
     int k1 = (k + 1) % 3;
     L[k].n[0] = -(t[2 * k1 + 1] - t[2 * k + 1]);
@@ -168,6 +204,7 @@ static void get_lines_tri(const double t[6], orc_line L[3]) {
  * reference); returns the new point count. */
 static int clip_line(const orc_line *L, const double *pts, int n, double *out) {
   double d[ORC_PTS];
+  F(5 * n);
   for (int i = 0; i < n; i++) {
     double s = L->n[0] * (pts[2 * i] - L->p[0]);
     s = s + L->n[1] * (pts[2 * i + 1] - L->p[1]);
@@ -178,22 +215,28 @@ static int clip_line(const orc_line *L, const double *pts, int n, double *out) {
     const double *p1 = pts + 2 * i;
     const double *p2 = (i == n - 1) ? pts : pts + 2 * (i + 1);
     double d1 = d[i], d2 = (i == n - 1) ? d[0] : d[i + 1];
+    K(1);
     if (d1 < 0.0) {
       out[2 * m] = p1[0];
       out[2 * m + 1] = p1[1];
       m++;
+      K(1);
       if (!(d2 <= 0.0)) {
+        F(8);
+        K(4);
         double f = fabs(d1) / (fabs(d1) + fabs(d2));
         f = fmax(fmin(f, 1.0), 0.0);
         out[2 * m] = p1[0] + f * (p2[0] - p1[0]);
         out[2 * m + 1] = p1[1] + f * (p2[1] - p1[1]);
         m++;
       }
-    } else if (d1 == 0.0) {
+    } else if ((K(1), d1 == 0.0)) {
       out[2 * m] = p1[0];
       out[2 * m + 1] = p1[1];
       m++;
-    } else if (d2 < 0.0) {
+    } else if ((K(1), d2 < 0.0)) {
+      F(8);
+      K(4);
       double f = fabs(d1) / (fabs(d1) + fabs(d2));
       f = fmax(fmin(f, 1.0), 0.0);
       out[2 * m] = p1[0] + f * (p2[0] - p1[0]);
@@ -215,6 +258,8 @@ int orc_intersect_tris(const double tri_a[6], const double tri_b[6], double *tri
   if (n < 3) return 0;
   n = clip_line(&L[2], buf0, n, buf1);
   if (n < 3) return 0;
+  F(1);
+  K(1);
   double tol = 10.0 * fmin(orc_spacing(orc_triangle_area(tri_a)),
                            orc_spacing(orc_triangle_area(tri_b)));
   int nc = 0;
@@ -226,6 +271,7 @@ int orc_intersect_tris(const double tri_a[6], const double tri_b[6], double *tri
     t[3] = buf1[2 * (i + 1) + 1];
     t[4] = buf1[2 * (i + 2)];
     t[5] = buf1[2 * (i + 2) + 1];
+    K(1);
     if (!(orc_triangle_area(t) < tol)) nc++;
   }
   return nc;
@@ -236,6 +282,7 @@ int orc_intersect_tris(const double tri_a[6], const double tri_b[6], double *tri
 /* ------------------------------------------------------------------------ */
 
 static void unit_cross(const double a[3], const double b[3], double out[3]) {
+  F(12);
   double c[3];
   c[0] = a[1] * b[2] - a[2] * b[1];
   c[1] = a[2] * b[0] - a[0] * b[2];
@@ -247,6 +294,8 @@ static void unit_cross(const double a[3], const double b[3], double out[3]) {
 }
 
 void orc_get_planes_tet(const double t[12], double nrm[12], double c[4]) {
+  F(15);
+  K(1);
   double e10[3], e20[3], e30[3], e21[3], e31[3];
   for (int i = 0; i < 3; i++) {
     e10[i] = t[3 + i] - t[i];
@@ -269,6 +318,7 @@ void orc_get_planes_tet(const double t[12], double nrm[12], double c[4]) {
 /* p = w0*v[ip] + w1*v[in], with the reference's weights for vertices ip
  * (outside, d > 0) and in (inside, d < 0). */
 static void split_point(const double *v, const double *d, int ip, int in, double *p) {
+  F(13);
   double invdiff = 1.0 / (d[ip] - d[in]);
   double w0 = -d[in] * invdiff;
   double w1 = d[ip] * invdiff;
@@ -287,6 +337,8 @@ static int clip_tet_plane(const double *n, double c, const double *v, double *ou
   int neg[4], pos[4], zer[4], nn = 0, np = 0, nz = 0;
   for (int i = 0; i < 4; i++) {
     d[i] = orc_dot3(n, v + 3 * i) - c;
+    F(1);
+    K(4);
     if (fabs(d[i]) < DBL_EPSILON) zer[nz++] = i;
     else if (d[i] < 0.0) neg[nn++] = i;
     else if (d[i] > 0.0) pos[np++] = i;
@@ -366,6 +418,8 @@ int orc_intersect_tets(const double tet_a[12], const double tet_b[12], double *t
   double nrm[12], c[4];
   orc_get_planes_tet(tet_b, nrm, c);
   double vol_b = orc_tetrahedron_volume_tet(tet_b);
+  F(1);
+  K(1);
   double tol = 10.0 * fmin(orc_spacing(orc_tetrahedron_volume_tet(tet_a)), orc_spacing(vol_b));
   double cur[ORC_TET_BUF_SIZE * 12], nxt[ORC_TET_BUF_SIZE * 12];
   memcpy(cur, tet_a, 12 * sizeof(double));
@@ -379,6 +433,7 @@ int orc_intersect_tets(const double tet_a[12], const double tet_b[12], double *t
     } else {
       n = 0;
       for (int j = 0; j < m; j++) {
+        K(1);
         if (orc_tetrahedron_volume_tet(nxt + 12 * j) >= tol) {
           memcpy(tets_c + 12 * n, nxt + 12 * j, 12 * sizeof(double));
           n++;
@@ -729,6 +784,7 @@ static void *clip_worker(void *arg) {
     if (j->measure) {
       double s = 0.0;
       for (int k = 0; k < n; k++) s = s + measure_real(dim, buf + esz * k);
+      F(n);
       j->measure[p] = s;
     }
     if (j->elems) {
@@ -781,6 +837,7 @@ void orc_clip_pairs(int dim, long npairs, const int *pair_a, const int *pair_b,
 /* P1 basis functions of a triangle at x: inverse-Jacobian closed form
  * (test_parallel_p1_inner_product_2d.F90:205-228). */
 static void basis_p1_2d(const double *c, const double *x, double *fns) {
+  F(21);
   double e1x = c[2] - c[0], e1y = c[3] - c[1];
   double e2x = c[4] - c[0], e2y = c[5] - c[1];
   double j11 = e2y, j12 = -e2x, j21 = -e1y, j22 = e1x;
@@ -802,6 +859,8 @@ static void basis_p1_2d(const double *c, const double *x, double *fns) {
 /* P1 basis functions of a tet at x: Gaussian elimination with partial
  * pivoting on [edges | x - c1] (test_parallel_p1_inner_product_3d.F90:227-265). */
 static void basis_p1_3d(const double *c, const double *x, double *fns) {
+  F(75);
+  K(9);
   double m[3][4];
   for (int r = 0; r < 3; r++) {
     for (int k = 0; k < 3; k++) m[r][k] = c[3 * (k + 1) + r] - c[r];
@@ -850,6 +909,7 @@ static const double kQ3[4][4] = {{QA, QB, QB, QB}, {QB, QA, QB, QB}, {QB, QB, QA
 
 static void quad_point(int dim, int q, const double *el, double *x) {
   int loc = dim + 1;
+  F(2 * dim * loc);
   for (int d = 0; d < dim; d++) {
     double s = 0.0;
     for (int v = 0; v < loc; v++) {
@@ -885,6 +945,7 @@ static void *mass_worker(void *arg) {
           basis_p1_3d(eb, x, fb);
         }
         double wm = wq * meas;
+        F(1 + 3 * loc * loc);
         for (int a = 0; a < loc; a++)
           for (int b = 0; b < loc; b++) blk[a * loc + b] = blk[a * loc + b] + wm * fa[a] * fb[b];
       }

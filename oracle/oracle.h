@@ -36,6 +36,11 @@ double orc_epsilon(void);
 double orc_norm2_3(const double v[3]);
 double orc_dot3(const double a[3], const double b[3]);
 
+/* Operation counts of the calling thread since the last reset:
+ * out[0] FP64 add/sub/mul/div/sqrt, out[1] FP64 compare/abs/min/max.
+ * Non-zero only in liboracle_count.so (built with -DORC_COUNT). */
+void orc_op_counts(unsigned long long *out, int reset);
+
 /* Element measures. */
 double orc_triangle_area(const double tri[6]);
 double orc_tetrahedron_volume_real(const double tet[12]);
