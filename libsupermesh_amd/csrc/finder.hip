@@ -37,7 +37,8 @@ template <int DIM>
 __global__ void __launch_bounds__(kBlock)
 k_stage(long ne, long nnodes, const double* __restrict__ pos, const int* __restrict__ enl,
         double* __restrict__ coords, double* __restrict__ bbox, double* __restrict__ sp,
-        double* __restrict__ planes, double* __restrict__ affine, int* __restrict__ bad) {
+        double* __restrict__ scale, double* __restrict__ planes, double* __restrict__ affine,
+        int* __restrict__ bad) {
   constexpr int LOC = DIM + 1;
   long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= ne) return;
@@ -52,8 +53,13 @@ k_stage(long ne, long nnodes, const double* __restrict__ pos, const int* __restr
 #pragma unroll
     for (int d = 0; d < DIM; d++) c[j * DIM + d] = pos[v * DIM + d];
   }
+  double sc = 1.0;
 #pragma unroll
-  for (int k = 0; k < LOC * DIM; k++) coords[e * LOC * DIM + k] = c[k];
+  for (int k = 0; k < LOC * DIM; k++) {
+    coords[e * LOC * DIM + k] = c[k];
+    sc = fmax_f(sc, __builtin_fabs(c[k]));
+  }
+  scale[e] = sc;
 #pragma unroll
   for (int d = 0; d < DIM; d++) {
     double lo = c[d], hi = c[d];
@@ -136,6 +142,7 @@ void stage_mesh(Context& C, DevMesh& M, int dim, int loc, long nnodes, long ne,
   double* coords = M.coords.as<double>((size_t)ne * loc * dim);
   double* bbox = M.bbox.as<double>((size_t)ne * 2 * dim);
   double* sp = M.meas_sp.as<double>((size_t)ne);
+  double* scale = M.scale.as<double>((size_t)ne);
   double* planes = need_planes && dim == 3 ? M.planes.as<double>((size_t)ne * 16) : nullptr;
   double* affine = need_affine ? M.affine.as<double>((size_t)ne * (dim + dim * dim)) : nullptr;
   int* bad = C.scalars.as<int>(64);
@@ -143,10 +150,10 @@ void stage_mesh(Context& C, DevMesh& M, int dim, int loc, long nnodes, long ne,
   if (ne > 0) {
     if (dim == 2)
       hipLaunchKernelGGL(k_stage<2>, dim3(grid_for(ne, kBlock)), dim3(kBlock), 0, C.stream, ne,
-                         nnodes, positions, enlist, coords, bbox, sp, planes, affine, bad);
+                         nnodes, positions, enlist, coords, bbox, sp, scale, planes, affine, bad);
     else
       hipLaunchKernelGGL(k_stage<3>, dim3(grid_for(ne, kBlock)), dim3(kBlock), 0, C.stream, ne,
-                         nnodes, positions, enlist, coords, bbox, sp, planes, affine, bad);
+                         nnodes, positions, enlist, coords, bbox, sp, scale, planes, affine, bad);
     LSM_HIP(hipGetLastError());
   }
   int hbad = 0;
@@ -200,12 +207,15 @@ k_center_bounds(long ne, const double* __restrict__ bbox, double* __restrict__ p
 }
 
 template <int DIM>
-__global__ void k_bounds_final(int nparts, double* __restrict__ partial) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ void __launch_bounds__(kBlock) k_bounds_final(int nparts, double* __restrict__ partial) {
+  __shared__ double red[2 * DIM][kBlock];
   double b[2 * DIM];
 #pragma unroll
-  for (int k = 0; k < 2 * DIM; k++) b[k] = partial[k];
-  for (int p = 1; p < nparts; p++) {
+  for (int d = 0; d < DIM; d++) {
+    b[d] = __builtin_inf();
+    b[DIM + d] = -__builtin_inf();
+  }
+  for (int p = threadIdx.x; p < nparts; p += kBlock) {
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
       b[d] = fmin_f(b[d], partial[p * 2 * DIM + d]);
@@ -213,7 +223,22 @@ __global__ void k_bounds_final(int nparts, double* __restrict__ partial) {
     }
   }
 #pragma unroll
-  for (int k = 0; k < 2 * DIM; k++) partial[k] = b[k];
+  for (int k = 0; k < 2 * DIM; k++) red[k][threadIdx.x] = b[k];
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+#pragma unroll
+      for (int d = 0; d < DIM; d++) {
+        red[d][threadIdx.x] = fmin_f(red[d][threadIdx.x], red[d][threadIdx.x + s]);
+        red[DIM + d][threadIdx.x] = fmax_f(red[DIM + d][threadIdx.x], red[DIM + d][threadIdx.x + s]);
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < 2 * DIM; k++) partial[k] = red[k][0];
+  }
 }
 
 __device__ __forceinline__ uint64_t spread2(uint64_t x) {  // 32 bits -> even bits
@@ -271,12 +296,12 @@ static void morton_order(Context& C, const DevMesh& M, DevBuf& perm) {
   const double* bbox = static_cast<const double*>(M.bbox.ptr);
   if (dim == 2) {
     hipLaunchKernelGGL(k_center_bounds<2>, dim3(nparts), dim3(kBlock), 0, C.stream, ne, bbox, part);
-    hipLaunchKernelGGL(k_bounds_final<2>, dim3(1), dim3(64), 0, C.stream, nparts, part);
+    hipLaunchKernelGGL(k_bounds_final<2>, dim3(1), dim3(kBlock), 0, C.stream, nparts, part);
     hipLaunchKernelGGL(k_morton<2>, dim3(grid_for(ne, kBlock)), dim3(kBlock), 0, C.stream, ne, bbox,
                        part, keys, vals);
   } else {
     hipLaunchKernelGGL(k_center_bounds<3>, dim3(nparts), dim3(kBlock), 0, C.stream, ne, bbox, part);
-    hipLaunchKernelGGL(k_bounds_final<3>, dim3(1), dim3(64), 0, C.stream, nparts, part);
+    hipLaunchKernelGGL(k_bounds_final<3>, dim3(1), dim3(kBlock), 0, C.stream, nparts, part);
     hipLaunchKernelGGL(k_morton<3>, dim3(grid_for(ne, kBlock)), dim3(kBlock), 0, C.stream, ne, bbox,
                        part, keys, vals);
   }

@@ -320,11 +320,30 @@ struct Split {
 
 LSM_HD int pick(int packed, int i) { return (packed >> (2 * i)) & 3; }
 
+// A value the optimiser must treat as already in a register.  Selecting
+// between elements of a small local array with a run-time index is otherwise
+// folded into one load at a computed address, which moves the whole array to
+// scratch memory.
+LSM_HD double in_reg(double x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  asm volatile("" : "+v"(x));
+#endif
+  return x;
+}
+
+LSM_HD int in_reg_i(int x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  asm volatile("" : "+v"(x));
+#endif
+  return x;
+}
+
 LSM_HD double sel4(const double* a, int k) {
-  double r = a[3];
-  r = k == 2 ? a[2] : r;
-  r = k == 1 ? a[1] : r;
-  r = k == 0 ? a[0] : r;
+  const double a0 = in_reg(a[0]), a1 = in_reg(a[1]), a2 = in_reg(a[2]), a3 = in_reg(a[3]);
+  double r = a3;
+  r = k == 2 ? a2 : r;
+  r = k == 1 ? a1 : r;
+  r = k == 0 ? a0 : r;
   return r;
 }
 
@@ -512,6 +531,302 @@ LSM_HD Tet make_child(const Tet& T, const Split& s, int ci) {
     }
   }
   return out;
+}
+
+// ---------------------------------------------------------------------------
+// Flat depth-first walk: the same clip as clip_tet, written as a loop whose
+// body always does the same work -- build one child, then classify it (or
+// measure it, at the last plane) -- so that lanes of a wave at different
+// depths, in different split cases, or on different pairs run in lockstep.
+// A child is built from a recipe (up to three split points, then a source for
+// each of the four output slots) that reproduces clip_buf's vertex order.
+// ---------------------------------------------------------------------------
+
+// Packed classification: bits 0-2 kind (0 discard, 1 keep, 2 +++-, 3 ++--,
+// 4 ++-0, 5 +---, 6 +--0, 7 +-00), 3-4 number of children, 5-10 pos
+// indices p0..p2, 11-16 neg indices n0..n2, 17-18 zero index z0.
+LSM_HD int pack_class(const double* d) {
+  int np = 0, nn = 0, nz = 0, pos = 0, neg = 0, zer = 0;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    if (__builtin_fabs(d[i]) < kEpsilon) {
+      zer |= i << (2 * nz);
+      nz++;
+    } else if (d[i] < 0.0) {
+      neg |= i << (2 * nn);
+      nn++;
+    } else if (d[i] > 0.0) {
+      pos |= i << (2 * np);
+      np++;
+    }
+  }
+  int kind, nch;
+  if (nn == 0) {
+    kind = 0;
+    nch = 0;
+  } else if (np == 0) {
+    kind = 1;
+    nch = 1;
+  } else if (np == 3) {
+    kind = 2;
+    nch = 1;
+  } else if (np == 2) {
+    kind = nn == 2 ? 3 : 4;
+    nch = nn == 2 ? 3 : 1;
+  } else {
+    kind = nn == 3 ? 5 : (nn == 2 ? 6 : 7);
+    nch = nn == 3 ? 3 : (nn == 2 ? 2 : 1);
+  }
+  return kind | (nch << 3) | ((pos & 63) << 5) | ((neg & 63) << 11) | ((zer & 3) << 17);
+}
+
+LSM_HD int class_nchild(int c) { return (c >> 3) & 3; }
+
+// Recipe of child ci: split points s0..s2 as (outside, inside) vertex pairs
+// and, per output slot, a source: 0-3 parent vertex, 4-6 split point.
+struct Recipe {
+  int sp[3], sn[3];
+  int src[4];
+};
+
+LSM_HD Recipe make_recipe(int c, int ci) {
+  const int kind = c & 7;
+  const int p0 = (c >> 5) & 3, p1 = (c >> 7) & 3, p2 = (c >> 9) & 3;
+  const int n0 = (c >> 11) & 3, n1 = (c >> 13) & 3, n2 = (c >> 15) & 3;
+  const int z0 = (c >> 17) & 3;
+  Recipe r;
+  r.sp[0] = r.sp[1] = r.sp[2] = 0;
+  r.sn[0] = r.sn[1] = r.sn[2] = 0;
+  r.src[0] = 0;
+  r.src[1] = 1;
+  r.src[2] = 2;
+  r.src[3] = 3;
+  // Slot-preserving children replace the outside vertices in place.
+  auto put = [&](int slot, int s) {
+    r.src[0] = slot == 0 ? s : r.src[0];
+    r.src[1] = slot == 1 ? s : r.src[1];
+    r.src[2] = slot == 2 ? s : r.src[2];
+    r.src[3] = slot == 3 ? s : r.src[3];
+  };
+  if (kind == 2) {  // +++-: outside vertices move to the plane
+    r.sp[0] = p0; r.sn[0] = n0;
+    r.sp[1] = p1; r.sn[1] = n0;
+    r.sp[2] = p2; r.sn[2] = n0;
+    put(p0, 4);
+    put(p1, 5);
+    put(p2, 6);
+  } else if (kind == 4) {  // ++-0
+    r.sp[0] = p0; r.sn[0] = n0;
+    r.sp[1] = p1; r.sn[1] = n0;
+    put(p0, 4);
+    put(p1, 5);
+  } else if (kind == 7) {  // +-00
+    r.sp[0] = p0; r.sn[0] = n0;
+    put(p0, 4);
+  } else if (kind == 3) {  // ++--
+    if (ci == 0) {
+      r.sp[0] = p0; r.sn[0] = n1;
+      r.sp[1] = p1; r.sn[1] = n0;
+      put(p0, 4);
+      put(p1, 5);
+    } else if (ci == 1) {
+      r.sp[0] = p1; r.sn[0] = n1;
+      r.sp[1] = p0; r.sn[1] = n1;
+      r.sp[2] = p1; r.sn[2] = n0;
+      r.src[0] = n1; r.src[1] = 4; r.src[2] = 5; r.src[3] = 6;
+    } else {
+      r.sp[0] = p0; r.sn[0] = n0;
+      r.sp[1] = p1; r.sn[1] = n0;
+      r.sp[2] = p0; r.sn[2] = n1;
+      r.src[0] = n0; r.src[1] = 4; r.src[2] = 5; r.src[3] = 6;
+    }
+  } else if (kind == 5) {  // +---
+    if (ci == 0) {
+      r.sp[0] = p0; r.sn[0] = n0;
+      put(p0, 4);
+    } else if (ci == 1) {
+      r.sp[0] = p0; r.sn[0] = n0;
+      r.sp[1] = p0; r.sn[1] = n1;
+      r.src[0] = 4; r.src[1] = n1; r.src[2] = n2; r.src[3] = 5;
+    } else {
+      r.sp[0] = p0; r.sn[0] = n1;
+      r.sp[1] = p0; r.sn[1] = n2;
+      r.sp[2] = p0; r.sn[2] = n0;
+      r.src[0] = n2; r.src[1] = 4; r.src[2] = 5; r.src[3] = 6;
+    }
+  } else if (kind == 6) {  // +--0
+    if (ci == 0) {
+      r.sp[0] = p0; r.sn[0] = n0;
+      put(p0, 4);
+    } else {
+      r.sp[0] = p0; r.sn[0] = n1;
+      r.sp[1] = p0; r.sn[1] = n0;
+      r.src[0] = 4; r.src[1] = z0; r.src[2] = n1; r.src[3] = 5;
+    }
+  }
+  return r;
+}
+
+// Child of parent T (distances d to the plane) from recipe r.  Unused split
+// points are computed from vertex 0 twice and never selected.
+LSM_HD void build_child(const double* T, const double* d, const Recipe& r, double* out) {
+  double S[3][3];
+#pragma unroll
+  for (int s = 0; s < 3; s++) {
+    const int ip = r.sp[s], in = r.sn[s];
+    double dp = sel4(d, ip), dn = sel4(d, in);
+    double invdiff = 1.0 / (dp - dn);
+    double w0 = -dn * invdiff;
+    double w1 = dp * invdiff;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const double t0 = in_reg(T[k]), t1 = in_reg(T[3 + k]), t2 = in_reg(T[6 + k]),
+                   t3 = in_reg(T[9 + k]);
+      double vp = t3;
+      vp = ip == 2 ? t2 : vp;
+      vp = ip == 1 ? t1 : vp;
+      vp = ip == 0 ? t0 : vp;
+      double vn = t3;
+      vn = in == 2 ? t2 : vn;
+      vn = in == 1 ? t1 : vn;
+      vn = in == 0 ? t0 : vn;
+      S[s][k] = w0 * vp + w1 * vn;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int src = r.src[j];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      double v = in_reg(S[2][k]);
+      v = src == 5 ? in_reg(S[1][k]) : v;
+      v = src == 4 ? in_reg(S[0][k]) : v;
+      v = src == 3 ? in_reg(T[9 + k]) : v;
+      v = src == 2 ? in_reg(T[6 + k]) : v;
+      v = src == 1 ? in_reg(T[3 + k]) : v;
+      v = src == 0 ? in_reg(T[k]) : v;
+      out[3 * j + k] = v;
+    }
+  }
+}
+
+// Walk state of one pair: tets at depths 1..3 (depth 0, tet A, is passed to
+// each step), their packed classes against the next plane, the depth of the
+// next child to build (1..4) and the next child index per depth.  A parent's
+// distances to its plane are recomputed when a child is built (the same
+// operations, hence the same values) rather than kept, to save registers.
+// B's planes are read through `planes` (16 doubles: n0 c0 n1 c1 ..).
+struct TetWalk {
+  double T[3][12];
+  int cls[4];
+  int level;
+  int digits;  // 2 bits per depth 1..4
+};
+
+LSM_HD void walk_distances(const double* T, const double* pl, double* d) {
+#pragma unroll
+  for (int v = 0; v < 4; v++) d[v] = dot3(pl[0], pl[1], pl[2], T[3 * v], T[3 * v + 1], T[3 * v + 2]) - pl[3];
+}
+
+LSM_HD void walk_start(TetWalk& W, const double* A, const double* planes) {
+  double d[4];
+  walk_distances(A, planes, d);
+  W.cls[0] = pack_class(d);
+  W.level = 1;
+  W.digits = 0;
+}
+
+// One step.  Returns false once the pair's walk is complete.  Calls
+// sink(tet, vol_tet) for each kept output tet, in the reference's order.
+template <class Sink>
+LSM_HD bool walk_step(TetWalk& W, const double* A, const double* planes, double tol, Sink&& sink) {
+  const int k = W.level;
+  const int ci = (W.digits >> (2 * k)) & 3;
+  int cls = in_reg_i(W.cls[3]);
+  cls = k == 3 ? in_reg_i(W.cls[2]) : cls;
+  cls = k == 2 ? in_reg_i(W.cls[1]) : cls;
+  cls = k == 1 ? in_reg_i(W.cls[0]) : cls;
+  if (ci >= class_nchild(cls)) {
+    // all children of this parent done; its own index was advanced on descent
+    if (k == 1) return false;
+    W.level = k - 1;
+    return true;
+  }
+  double parent[12], pd[4];
+#pragma unroll
+  for (int i = 0; i < 12; i++) {
+    double v = in_reg(W.T[2][i]);
+    v = k == 3 ? in_reg(W.T[1][i]) : v;
+    v = k == 2 ? in_reg(W.T[0][i]) : v;
+    v = k == 1 ? in_reg(A[i]) : v;
+    parent[i] = v;
+  }
+  walk_distances(parent, planes + 4 * (k - 1), pd);
+  double child[12];
+  build_child(parent, pd, make_recipe(cls, ci), child);
+  if (k == 4) {
+    W.digits += 1 << 8;
+    double vol = tet_volume_tet(child);
+    if (vol >= tol) sink(child, vol);
+    return true;
+  }
+  // descend: store the child at depth k, classify it against plane k+1
+  double dc[4];
+  walk_distances(child, planes + 4 * k, dc);
+  const int cc = pack_class(dc);
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+#pragma unroll
+    for (int i = 0; i < 12; i++) W.T[j][i] = (k == j + 1) ? child[i] : W.T[j][i];
+    W.cls[j + 1] = (k == j + 1) ? cc : W.cls[j + 1];
+  }
+  W.digits = (W.digits + (1 << (2 * k))) & ~(3 << (2 * (k + 1)));
+  W.level = k + 1;
+  return true;
+}
+
+// Whole clip of one pair with the flat walk (host checks and one-pair calls).
+template <class Sink>
+LSM_HD int clip_tet_walk(const double* A, const Planes& P, double tol, Sink&& sink) {
+  double planes[16];
+  for (int i = 0; i < 4; i++) {
+    for (int k = 0; k < 3; k++) planes[4 * i + k] = P.n[i][k];
+    planes[4 * i + 3] = P.c[i];
+  }
+  TetWalk W;
+  walk_start(W, A, planes);
+  int kept = 0;
+  while (walk_step(W, A, planes, tol, [&](const double* t, double v) {
+    sink(t, v);
+    kept++;
+  })) {
+  }
+  return kept;
+}
+
+// True when tet A lies beyond one of B's face planes by a margin that no
+// rounding in the clipper can bridge, so the clipper provably returns no
+// tets.  Every vertex the clipper creates is a rounded convex combination of
+// A's vertices: after three planes it is within ~30u*S of conv(A), S bounding
+// the coordinates of A and B, and its computed distance to the plane then
+// differs from the smallest distance of A's vertices by less than 1e-14*S.
+// With all of A's distances above eps + 1e-12*S, every sub-tet meeting that
+// plane is classified wholly outside and discarded
+// (Tetrahedron_Intersection.F90:278-281).  `scale` = max(1, max |coord|).
+LSM_HD bool tet_separated_by_b(const double* A, const Planes& P, double scale) {
+  const double T = kEpsilon + 1e-12 * scale;
+  bool sep = false;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    bool out = true;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      double d = dot3(P.n[i][0], P.n[i][1], P.n[i][2], A[3 * v], A[3 * v + 1], A[3 * v + 2]) - P.c[i];
+      out = out && (d > T);
+    }
+    sep = sep || out;
+  }
+  return sep;
 }
 
 LSM_HD double tet_tol(double vol_a, double vol_b) {

@@ -73,6 +73,7 @@ struct DevMesh {
   DevBuf coords;   // double [ne][loc][dim]: element vertex coordinates
   DevBuf bbox;     // double [2*dim][ne]: lo_x.. lo_z, hi_x.. hi_z (SoA)
   DevBuf meas_sp;  // double [ne]: SPACING of the clipper's element measure
+  DevBuf scale;    // double [ne]: max(1, max |vertex coordinate|)
   DevBuf planes;   // double [ne][16]: tet planes (normals, c) of B
   DevBuf affine;   // double [ne][loc][dim+1]: barycentric maps (mass)
 };
@@ -101,6 +102,7 @@ struct Context {
   DevBuf row_count, indptr, indices, rows;
   // clipping
   DevBuf counts, measure, partials, scalars, offsets, elements, blocks;
+  DevBuf work, nwork;            // compacted list of pairs that need the full clip
   // mass assembly
   DevBuf mflag, mkeys, mvals, mkeys_alt, mvals_alt, ukeys, uvals, nuniq, mindptr, mindices;
   // per-pair host API staging

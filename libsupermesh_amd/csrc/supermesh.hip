@@ -119,10 +119,14 @@ struct ClipArgs {
   long npairs;
   const int* rows;
   const int* cols;
+  const int* work;    // when non-null: process pairs work[0..nwork)
+  const unsigned long long* nwork;
   const double* acoords;
   const double* bcoords;
   const double* asp;
   const double* bsp;
+  const double* ascale;
+  const double* bscale;
   const double* bplanes;
   const double* aaff;
   const double* baff;
@@ -131,19 +135,76 @@ struct ClipArgs {
   double* blocks;
   const long* offsets;
   double* elements;
-  Partial* partials;
 };
+
+__device__ __forceinline__ void load_planes(const double* __restrict__ bp, long b, Planes& P) {
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) P.n[i][k] = bp[b * 16 + 4 * i + k];
+    P.c[i] = bp[b * 16 + 4 * i + 3];
+  }
+}
+
+// Tet pairs: pairs whose A lies provably beyond one of B's face planes get
+// their (empty) result here; the rest are appended to a compact work list, so
+// the clip kernel's waves carry only pairs that need the full clip.
+template <bool MASS>
+__global__ void __launch_bounds__(kClipBlock) k_tet_triage(ClipArgs g, int* __restrict__ work,
+                                                           unsigned long long* __restrict__ nwork) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  const int lane = threadIdx.x & 63;
+  for (long base = (long)blockIdx.x * blockDim.x; base < g.npairs; base += stride) {
+    const long p = base + threadIdx.x;
+    bool keep = false;
+    if (p < g.npairs) {
+      const long a = g.rows[p], b = g.cols[p];
+      double A[12];
+#pragma unroll
+      for (int k = 0; k < 12; k++) A[k] = g.acoords[a * 12 + k];
+      Planes P;
+      load_planes(g.bplanes, b, P);
+      const double sc = fmax_f(g.ascale[a], g.bscale[b]);
+      keep = !tet_separated_by_b(A, P, sc);
+      if (!keep) {
+        g.counts[p] = 0;
+        g.measure[p] = 0.0;
+        if (MASS) {
+#pragma unroll
+          for (int k = 0; k < 16; k++) g.blocks[p * 16 + k] = 0.0;
+        }
+      }
+    }
+    // wave-aggregated append to the work list
+    const unsigned long long m = __ballot(keep);
+    const int cnt = __popcll(m);
+    unsigned long long pos = 0;
+    if (cnt) {
+      const int leader = __ffsll((long long)m) - 1;
+      if (lane == leader) pos = atomicAdd(nwork, (unsigned long long)cnt);
+      pos = __shfl(pos, leader);
+      if (keep) work[pos + __popcll(m & ((1ULL << lane) - 1))] = (int)p;
+    }
+  }
+}
 
 template <int DIM, bool MASS, bool FILL>
 __global__ void __launch_bounds__(kClipBlock) k_clip(ClipArgs g) {
   constexpr int LOC = DIM + 1;
   constexpr int ESZ = LOC * DIM;
-  double acc_hi = 0.0, acc_lo = 0.0;
-  long long nonempty = 0, nout = 0, maxc = 0;
   const long stride = (long)gridDim.x * blockDim.x;
-  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < g.npairs; p += stride) {
+  const long n = g.work ? (long)*g.nwork : g.npairs;
+  for (long w = (long)blockIdx.x * blockDim.x + threadIdx.x; w < n; w += stride) {
+    const long p = g.work ? g.work[w] : w;
     const long a = g.rows[p], b = g.cols[p];
-    double A[ESZ], Bv[ESZ];
+    long off = 0;
+    int cap = 0;  // fill pass: the count pass's result bounds the writes
+    if (FILL) {
+      cap = g.counts[p];
+      if (cap == 0) continue;
+      off = g.offsets[p];
+    }
+    double A[ESZ];
 #pragma unroll
     for (int k = 0; k < ESZ; k++) A[k] = g.acoords[a * ESZ + k];
     const double tol = 10.0 * fmin_f(g.asp[a], g.bsp[b]);
@@ -159,22 +220,16 @@ __global__ void __launch_bounds__(kClipBlock) k_clip(ClipArgs g) {
         affb[k] = g.baff[b * (DIM + DIM * DIM) + k];
       }
     }
-    long off = 0;
-    int cap = 0;  // fill pass: the count pass's result bounds the writes
-    if (FILL) {
-      cap = g.counts[p];
-      if (cap == 0) continue;
-      off = g.offsets[p];
-    }
-    int n;
+    int nk;
     if constexpr (DIM == 2) {
+      double Bv[ESZ];
 #pragma unroll
       for (int k = 0; k < ESZ; k++) Bv[k] = g.bcoords[b * ESZ + k];
       Line L[3];
       tri_lines(Bv, L);
       int k = 0;
-      n = clip_tri(A, L, tol, [&](double x0, double y0, double x1, double y1, double x2, double y2,
-                                  double ar) {
+      nk = clip_tri(A, L, tol, [&](double x0, double y0, double x1, double y1, double x2, double y2,
+                                   double ar) {
         m = m + ar;
         if (MASS) {
           double el[6] = {x0, y0, x1, y1, x2, y2};
@@ -193,17 +248,12 @@ __global__ void __launch_bounds__(kClipBlock) k_clip(ClipArgs g) {
       });
     } else {
       Planes P;
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-#pragma unroll
-        for (int kk = 0; kk < 3; kk++) P.n[i][kk] = g.bplanes[b * 16 + 4 * i + kk];
-        P.c[i] = g.bplanes[b * 16 + 4 * i + 3];
-      }
+      load_planes(g.bplanes, b, P);
       Tet TA;
 #pragma unroll
       for (int kk = 0; kk < 12; kk++) TA.v[kk] = A[kk];
       int k = 0;
-      n = clip_tet(TA, P, tol, [&](const Tet& T, double) {
+      nk = clip_tet(TA, P, tol, [&](const Tet& T, double) {
         double vr = tet_volume_real(T.v);
         m = m + vr;
         if (MASS) mass_element<3>(T.v, vr, affa, affb, blk);
@@ -216,22 +266,107 @@ __global__ void __launch_bounds__(kClipBlock) k_clip(ClipArgs g) {
       });
     }
     if (FILL) continue;
-    if (n == 0) m = 0.0;
-    g.counts[p] = n;
+    if (nk == 0) m = 0.0;
+    g.counts[p] = nk;
     g.measure[p] = m;
     if (MASS) {
 #pragma unroll
-      for (int k = 0; k < LOC * LOC; k++) g.blocks[p * LOC * LOC + k] = n == 0 ? 0.0 : blk[k];
+      for (int k = 0; k < LOC * LOC; k++) g.blocks[p * LOC * LOC + k] = nk == 0 ? 0.0 : blk[k];
     }
-    dd_add(acc_hi, acc_lo, m);
-    nonempty += n > 0;
-    nout += n;
-    maxc = n > maxc ? n : maxc;
   }
-  if (FILL) return;
+}
+
+// Tet pairs on the compacted work list, one pair per lane at a time.  Each
+// lane walks its pair's clip tree with walk_step (every step builds one child
+// and classifies or measures it) and, when its pair is done, moves on to its
+// next work item by itself, so lanes never wait for the slowest pair of the
+// wave -- only for the slowest chain of work items, which evens out.
+template <bool MASS, bool FILL>
+__global__ void __launch_bounds__(kClipBlock) k_clip_tet_walk(ClipArgs g) {
+  const long n = (long)*g.nwork;
+  const long stride = (long)gridDim.x * blockDim.x;
+  long w = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long p = 0, off = 0;
+  int cap = 0, k = 0;
+  double A[12], tol = 0.0, m = 0.0;
+  double blk[16], affa[12], affb[12];
+  const double* planes = nullptr;
+  TetWalk W;
+  bool active = false;
+  while (true) {
+    if (!active) {
+      // next work item of this lane (fill pass: skip pairs with no output)
+      while (w < n) {
+        p = g.work[w];
+        if (!FILL) break;
+        cap = g.counts[p];
+        if (cap > 0) break;
+        w += stride;
+      }
+      if (w >= n) break;
+      if (FILL) off = g.offsets[p];
+      const long a = g.rows[p], b = g.cols[p];
+#pragma unroll
+      for (int i = 0; i < 12; i++) A[i] = g.acoords[a * 12 + i];
+      planes = g.bplanes + b * 16;
+      tol = 10.0 * fmin_f(g.asp[a], g.bsp[b]);
+      m = 0.0;
+      k = 0;
+      if (MASS) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) blk[i] = 0.0;
+#pragma unroll
+        for (int i = 0; i < 12; i++) {
+          affa[i] = g.aaff[a * 12 + i];
+          affb[i] = g.baff[b * 12 + i];
+        }
+      }
+      walk_start(W, A, planes);
+      active = true;
+    }
+    const bool more = walk_step(W, A, planes, tol, [&](const double* t, double) {
+      const double vr = tet_volume_real(t);
+      m = m + vr;
+      if (MASS) mass_element<3>(t, vr, affa, affb, blk);
+      if (FILL && k < cap) {
+        double* o = g.elements + (off + k) * 12;
+#pragma unroll
+        for (int i = 0; i < 12; i++) o[i] = t[i];
+      }
+      k++;
+    });
+    if (more) continue;
+    if (!FILL) {
+      g.counts[p] = k;
+      g.measure[p] = k == 0 ? 0.0 : m;
+      if (MASS) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) g.blocks[p * 16 + i] = k == 0 ? 0.0 : blk[i];
+      }
+    }
+    w += stride;
+    active = false;
+  }
+}
+
+// Totals over pairs in pair order: fixed grid, fixed tree, so the sum is
+// reproducible for a given input.
+__global__ void __launch_bounds__(kClipBlock)
+k_reduce_pairs(long npairs, const int* __restrict__ counts, const double* __restrict__ measure,
+               Partial* __restrict__ partials) {
+  double hi = 0.0, lo = 0.0;
+  long long nonempty = 0, nout = 0, maxc = 0;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += stride) {
+    int c = counts[p];
+    dd_add(hi, lo, measure[p]);
+    nonempty += c > 0;
+    nout += c;
+    maxc = c > maxc ? c : maxc;
+  }
   __shared__ Partial red[kClipBlock];
-  red[threadIdx.x].hi = acc_hi;
-  red[threadIdx.x].lo = acc_lo;
+  red[threadIdx.x].hi = hi;
+  red[threadIdx.x].lo = lo;
   red[threadIdx.x].nonempty = nonempty;
   red[threadIdx.x].nout = nout;
   red[threadIdx.x].maxc = maxc;
@@ -247,19 +382,32 @@ __global__ void __launch_bounds__(kClipBlock) k_clip(ClipArgs g) {
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) g.partials[blockIdx.x] = red[0];
+  if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
 }
 
-__global__ void k_reduce_partials(int n, Partial* __restrict__ parts) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  Partial t = parts[0];
-  for (int i = 1; i < n; i++) {
+__global__ void __launch_bounds__(kClipBlock) k_reduce_partials(int n, Partial* __restrict__ parts) {
+  __shared__ Partial red[kClipBlock];
+  Partial t = {0.0, 0.0, 0, 0, 0, 0};
+  for (int i = threadIdx.x; i < n; i += kClipBlock) {
     dd_merge(t.hi, t.lo, parts[i].hi, parts[i].lo);
     t.nonempty += parts[i].nonempty;
     t.nout += parts[i].nout;
     t.maxc = parts[i].maxc > t.maxc ? parts[i].maxc : t.maxc;
   }
-  parts[0] = t;
+  red[threadIdx.x] = t;
+  __syncthreads();
+  for (int s = kClipBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      Partial& x = red[threadIdx.x];
+      const Partial& y = red[threadIdx.x + s];
+      dd_merge(x.hi, x.lo, y.hi, y.lo);
+      x.nonempty += y.nonempty;
+      x.nout += y.nout;
+      x.maxc = y.maxc > x.maxc ? y.maxc : x.maxc;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) parts[0] = red[0];
 }
 
 template <int DIM>
@@ -272,6 +420,29 @@ static void launch_clip(Context& C, const ClipArgs& args, bool mass, bool fill) 
   } else {
     hipLaunchKernelGGL((k_clip<DIM, false, false>), grid, block, 0, C.stream, args);
   }
+  LSM_HIP(hipGetLastError());
+}
+
+static void launch_tet_walk(Context& C, const ClipArgs& args, bool mass, bool fill) {
+  // persistent lanes: one full wave of resident blocks
+  static int blocks_per_cu[3] = {0, 0, 0};
+  int which = fill ? 2 : (mass ? 1 : 0);
+  if (blocks_per_cu[which] == 0) {
+    int nb = 0;
+    if (fill)
+      LSM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_clip_tet_walk<false, true>, kClipBlock, 0));
+    else if (mass)
+      LSM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_clip_tet_walk<true, false>, kClipBlock, 0));
+    else
+      LSM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_clip_tet_walk<false, false>, kClipBlock, 0));
+    blocks_per_cu[which] = nb > 0 ? nb : 1;
+  }
+  int cus = 0;
+  LSM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, C.device));
+  dim3 grid((unsigned)(blocks_per_cu[which] * (cus > 0 ? cus : 256))), block(kClipBlock);
+  if (fill) hipLaunchKernelGGL((k_clip_tet_walk<false, true>), grid, block, 0, C.stream, args);
+  else if (mass) hipLaunchKernelGGL((k_clip_tet_walk<true, false>), grid, block, 0, C.stream, args);
+  else hipLaunchKernelGGL((k_clip_tet_walk<false, false>), grid, block, 0, C.stream, args);
   LSM_HIP(hipGetLastError());
 }
 
@@ -294,21 +465,37 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
   g.bcoords = static_cast<const double*>(B.coords.ptr);
   g.asp = static_cast<const double*>(A.meas_sp.ptr);
   g.bsp = static_cast<const double*>(B.meas_sp.ptr);
+  g.ascale = static_cast<const double*>(A.scale.ptr);
+  g.bscale = static_cast<const double*>(B.scale.ptr);
   g.bplanes = static_cast<const double*>(B.planes.ptr);
   g.aaff = static_cast<const double*>(A.affine.ptr);
   g.baff = static_cast<const double*>(B.affine.ptr);
   g.counts = C.counts.as<int>(npairs);
   g.measure = C.measure.as<double>(npairs);
   g.blocks = mass ? C.blocks.as<double>((size_t)npairs * loc * loc) : nullptr;
-  g.partials = C.partials.as<Partial>(kClipGrid);
+  Partial* parts = C.partials.as<Partial>(kClipGrid);
+  if (dim == 3 && npairs > 0) {
+    int* work = C.work.as<int>(npairs);
+    unsigned long long* nwork = C.nwork.as<unsigned long long>(1);
+    LSM_HIP(hipMemsetAsync(nwork, 0, sizeof(unsigned long long), C.stream));
+    if (mass)
+      hipLaunchKernelGGL(k_tet_triage<true>, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, g, work, nwork);
+    else
+      hipLaunchKernelGGL(k_tet_triage<false>, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, g, work, nwork);
+    LSM_HIP(hipGetLastError());
+    g.work = work;
+    g.nwork = nwork;
+  }
   if (C.timing) LSM_HIP(hipEventRecord(C.ev[14], C.stream));
   if (dim == 2) launch_clip<2>(C, g, mass, false);
-  else launch_clip<3>(C, g, mass, false);
+  else launch_tet_walk(C, g, mass, false);
   if (C.timing) LSM_HIP(hipEventRecord(C.ev[15], C.stream));
-  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, C.stream, kClipGrid, g.partials);
+  hipLaunchKernelGGL(k_reduce_pairs, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, npairs, g.counts,
+                     g.measure, parts);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kClipBlock), 0, C.stream, kClipGrid, parts);
   LSM_HIP(hipGetLastError());
   Partial tot;
-  LSM_HIP(hipMemcpyAsync(&tot, g.partials, sizeof(Partial), hipMemcpyDeviceToHost, C.stream));
+  LSM_HIP(hipMemcpyAsync(&tot, parts, sizeof(Partial), hipMemcpyDeviceToHost, C.stream));
   LSM_HIP(hipStreamSynchronize(C.stream));
   if (C.timing) {
     float ms = 0.f;
@@ -340,7 +527,7 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
       g.offsets = offs;
       g.elements = C.elements.as<double>((size_t)(out->nout > 0 ? out->nout : 1) * loc * dim);
       if (dim == 2) launch_clip<2>(C, g, false, true);
-      else launch_clip<3>(C, g, false, true);
+      else launch_tet_walk(C, g, false, true);
     }
     out->offsets = offs;
     out->elements = static_cast<const double*>(C.elements.get(8));
