@@ -6,8 +6,8 @@
  * (doc/manual.tex:2064-2082, src/tests/test_tri_intersector.F90:89-104).  A
  * GPU cannot amortise a launch per pair, so the whole loop -- candidate search,
  * clipping, measures and the P1 x P1 mixed mass -- is one call here, on device
- * arrays.  Results are bit-identical to the per-pair loop for candidate sets
- * (sorted within each row), per-pair output counts and output elements.
+ * arrays.  Results are bit-identical to the per-pair loop for candidate sets,
+ * per-pair output counts, measures and output elements.
  *
  * Memory: results live in buffers owned by the context and stay valid until
  * the next call on the same context.  All device pointers are HIP device
@@ -39,7 +39,9 @@ extern "C" {
 #define LIBSUPERMESH_GPU_ELEMENTS 1   /* materialise the supermesh elements */
 #define LIBSUPERMESH_GPU_MASS 2       /* P1 x P1 mixed mass blocks + CSR */
 #define LIBSUPERMESH_GPU_TIMING 4     /* fill ms[] with per-phase times */
-#define LIBSUPERMESH_GPU_UNSORTED 8   /* keep candidate rows in traversal order */
+#define LIBSUPERMESH_GPU_SORTED 8     /* sort each candidate row by B id (otherwise
+                                         rows keep the hierarchy's traversal
+                                         order, which is deterministic) */
 
 typedef struct libsupermesh_gpu_context libsupermesh_gpu_context;
 

@@ -146,7 +146,7 @@ static void run_supermesh(Context& C, const libsupermesh_gpu_mesh* a, const libs
   ph.mark();
   build_bvh(C, C.mesh_b);
   ph.mark();
-  long npairs = find_pairs(C, C.mesh_a, C.mesh_b, finder, !(flags & LIBSUPERMESH_GPU_UNSORTED));
+  long npairs = find_pairs(C, C.mesh_a, C.mesh_b, finder, flags & LIBSUPERMESH_GPU_SORTED);
   ph.mark();
   out->nelements_a = a->nelements;
   out->npairs = npairs;
@@ -296,7 +296,7 @@ static void finder_set_input(FinderState& S, int finder, long nnodes_a, int dim_
   host_mesh_to_device(C, C.mesh_a, nnodes_a, dim_a, ne_a, loc_a, pos_a, enl_a, va);
   host_mesh_to_device(C, C.mesh_b, nnodes_b, dim_b, ne_b, loc_b, pos_b, enl_b, vb);
   libsupermesh_gpu_result r;
-  run_supermesh(C, &va, &vb, finder, 0, false, &r);
+  run_supermesh(C, &va, &vb, finder, LIBSUPERMESH_GPU_SORTED, false, &r);
   S.nelements_a = ne_a;
   S.indptr.resize((size_t)ne_a + 1);
   S.indices.resize((size_t)r.npairs);

@@ -147,17 +147,22 @@ __device__ __forceinline__ void load_planes(const double* __restrict__ bp, long 
 }
 
 // Tet pairs: pairs whose A lies provably beyond one of B's face planes get
-// their (empty) result here; the rest are appended to a compact work list, so
-// the clip kernel's waves carry only pairs that need the full clip.
+// their (empty) result here; the rest are listed for the full clip.  Block b
+// owns pairs [b*chunk, (b+1)*chunk) and writes the survivors, in pair order,
+// to its own region work[b*chunk ..] (no atomics); k_gather_work then packs
+// the regions into one dense list.
 template <bool MASS>
-__global__ void __launch_bounds__(kClipBlock) k_tet_triage(ClipArgs g, int* __restrict__ work,
-                                                           unsigned long long* __restrict__ nwork) {
-  const long stride = (long)gridDim.x * blockDim.x;
-  const int lane = threadIdx.x & 63;
-  for (long base = (long)blockIdx.x * blockDim.x; base < g.npairs; base += stride) {
+__global__ void __launch_bounds__(kClipBlock) k_tet_triage(ClipArgs g, long chunk, int* __restrict__ work,
+                                                           long* __restrict__ block_count) {
+  __shared__ int wave_count[kClipBlock / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long lo = (long)blockIdx.x * chunk;
+  const long hi = lo + chunk < g.npairs ? lo + chunk : g.npairs;
+  long kept = 0;
+  for (long base = lo; base < hi; base += kClipBlock) {
     const long p = base + threadIdx.x;
     bool keep = false;
-    if (p < g.npairs) {
+    if (p < hi) {
       const long a = g.rows[p], b = g.cols[p];
       double A[12];
 #pragma unroll
@@ -175,17 +180,28 @@ __global__ void __launch_bounds__(kClipBlock) k_tet_triage(ClipArgs g, int* __re
         }
       }
     }
-    // wave-aggregated append to the work list
     const unsigned long long m = __ballot(keep);
-    const int cnt = __popcll(m);
-    unsigned long long pos = 0;
-    if (cnt) {
-      const int leader = __ffsll((long long)m) - 1;
-      if (lane == leader) pos = atomicAdd(nwork, (unsigned long long)cnt);
-      pos = __shfl(pos, leader);
-      if (keep) work[pos + __popcll(m & ((1ULL << lane) - 1))] = (int)p;
+    if (lane == 0) wave_count[wave] = __popcll(m);
+    __syncthreads();
+    int before = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kClipBlock / 64; w++) {
+      before += w < wave ? wave_count[w] : 0;
+      total += wave_count[w];
     }
+    if (keep) work[lo + kept + before + __popcll(m & ((1ULL << lane) - 1))] = (int)p;
+    kept += total;
+    __syncthreads();
   }
+  if (threadIdx.x == 0) block_count[blockIdx.x] = kept;
+}
+
+__global__ void __launch_bounds__(kClipBlock)
+k_gather_work(long chunk, const long* __restrict__ block_offset, const int* __restrict__ src,
+              int* __restrict__ dst) {
+  const long b = blockIdx.x;
+  const long o = block_offset[b], n = block_offset[b + 1] - o;
+  for (long i = threadIdx.x; i < n; i += kClipBlock) dst[o + i] = src[b * chunk + i];
 }
 
 template <int DIM, bool MASS, bool FILL>
@@ -475,16 +491,30 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
   g.blocks = mass ? C.blocks.as<double>((size_t)npairs * loc * loc) : nullptr;
   Partial* parts = C.partials.as<Partial>(kClipGrid);
   if (dim == 3 && npairs > 0) {
-    int* work = C.work.as<int>(npairs);
-    unsigned long long* nwork = C.nwork.as<unsigned long long>(1);
-    LSM_HIP(hipMemsetAsync(nwork, 0, sizeof(unsigned long long), C.stream));
+    const long chunk = (npairs + kClipGrid - 1) / kClipGrid;
+    int* region = C.mflag.as<int>(npairs);
+    long* bcount = C.nwork.as<long>(2 * kClipGrid + 2);
+    long* boff = bcount + kClipGrid + 1;
     if (mass)
-      hipLaunchKernelGGL(k_tet_triage<true>, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, g, work, nwork);
+      hipLaunchKernelGGL(k_tet_triage<true>, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, g, chunk,
+                         region, bcount);
     else
-      hipLaunchKernelGGL(k_tet_triage<false>, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, g, work, nwork);
+      hipLaunchKernelGGL(k_tet_triage<false>, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, g, chunk,
+                         region, bcount);
+    LSM_HIP(hipGetLastError());
+    LSM_HIP(hipMemsetAsync(boff, 0, sizeof(long), C.stream));
+    size_t bytes = 0;
+    LSM_HIP(rocprim::inclusive_scan(nullptr, bytes, bcount, boff + 1, (size_t)kClipGrid,
+                                    rocprim::plus<long>(), C.stream));
+    void* tmp = C.scan_tmp.get(bytes);
+    LSM_HIP(rocprim::inclusive_scan(tmp, bytes, bcount, boff + 1, (size_t)kClipGrid,
+                                    rocprim::plus<long>(), C.stream));
+    int* work = C.work.as<int>(npairs);
+    hipLaunchKernelGGL(k_gather_work, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, chunk, boff, region,
+                       work);
     LSM_HIP(hipGetLastError());
     g.work = work;
-    g.nwork = nwork;
+    g.nwork = reinterpret_cast<const unsigned long long*>(boff + kClipGrid);
   }
   if (C.timing) LSM_HIP(hipEventRecord(C.ev[14], C.stream));
   if (dim == 2) launch_clip<2>(C, g, mass, false);

@@ -29,7 +29,7 @@ FINDER_RTREE = 1
 ELEMENTS = 1
 MASS = 2
 TIMING = 4
-UNSORTED = 8
+SORTED = 8
 
 _lib = None
 

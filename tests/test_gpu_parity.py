@@ -35,7 +35,7 @@ def ctx():
 def run(ctx, pa, ea, pb, eb, finder=gpu.FINDER_TREE, flags=0):
     A = gpu.DeviceMesh(pa, ea)
     B = gpu.DeviceMesh(pb, eb)
-    r = ctx.supermesh(A, B, finder=finder, flags=flags)
+    r = ctx.supermesh(A, B, finder=finder, flags=flags | gpu.SORTED)
     dim = pa.shape[1]
     what = ["indptr", "indices", "rows", "counts", "measure"]
     if flags & gpu.ELEMENTS:
@@ -151,14 +151,26 @@ def test_closed_finder_on_touching_grids(ctx):
         assert abs(r.sum - 1.0) <= 1e-12
 
 
-def test_unsorted_rows_same_set(ctx):
-    pa, ea, pb, eb = meshgen.config_meshes("tri_64_vs_50")
+def test_default_row_order_same_pairs_and_results(ctx):
+    # Without SORTED, rows keep the traversal order: same pair set and the
+    # same per-pair results once both sides are ordered by (row, col).
+    pa, ea = meshgen.split_grid(3, 9, 5)
+    pb, eb = meshgen.split_grid(3, 8, 6, mesh_id=1)
     A, B = gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pb, eb)
-    r = ctx.intersection_finder(A, B, flags=gpu.UNSORTED)
-    out = gpu.fetch(r, 2, ["indptr", "indices"])
+    r = ctx.supermesh(A, B, flags=gpu.ELEMENTS)
+    out = gpu.fetch(r, 3, ["indptr", "indices", "rows", "counts", "measure", "offsets", "elements"])
     ip, ix = O.brute_force(pa, ea, pb, eb)
     assert np.array_equal(out["indptr"], ip)
     assert np.array_equal(sorted_csr(out["indptr"], out["indices"]), ix)
+    order = np.lexsort((out["indices"], out["rows"]))
+    rows, cols = O.csr_to_pairs(ip, ix)
+    cnt, meas = O.clip_pairs(pa, ea, pb, eb, rows, cols)
+    assert np.array_equal(out["counts"][order], cnt)
+    assert np.array_equal(out["measure"][order], meas)
+    offs = out["offsets"]
+    _, _, els = O.clip_pairs(pa, ea, pb, eb, rows, cols, cap=max(int(cnt.max()), 1))
+    for k, p in enumerate(order):
+        assert np.array_equal(out["elements"][offs[p]:offs[p + 1]], els[k, :cnt[k]])
 
 
 def test_deterministic_sum(ctx):
