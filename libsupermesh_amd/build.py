@@ -45,6 +45,35 @@ def _compile(src):
     return obj
 
 
+def build_fortran(lib_path, verbose=True):
+    """The Fortran interface module (ISO_C_BINDING over the C ABI) and the
+    Fortran driver used by the tests; skipped when amdflang is absent."""
+    fc = shutil.which("amdflang") or ("/opt/rocm/bin/amdflang" if os.path.exists("/opt/rocm/bin/amdflang") else None)
+    if fc is None:
+        if verbose:
+            print("amdflang not found: Fortran interface not built")
+        return None
+    src = os.path.join(PKG, "fortran", "libsupermesh.F90")
+    mod_dir = os.path.join(LIBDIR, "fortran")
+    os.makedirs(mod_dir, exist_ok=True)
+    flib = os.path.join(LIBDIR, "libsupermesh_fortran.so")
+    cmd = [fc, "-O2", "-fPIC", "-shared", "-module-dir", mod_dir, src, "-o", flib,
+           "-L" + LIBDIR, "-lsupermesh_gpu", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"Fortran build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    test_src = os.path.join(ROOT, "tests", "fortran", "test_intersector_gpu.F90")
+    exe = os.path.join(LIBDIR, "test_intersector_gpu")
+    cmd = [fc, "-O2", "-I", mod_dir, test_src, "-o", exe, "-L" + LIBDIR, "-lsupermesh_fortran",
+           "-lsupermesh_gpu", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"Fortran test build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {flib} and {exe}")
+    return flib
+
+
 def build(verbose=True):
     os.makedirs(OBJDIR, exist_ok=True)
     with concurrent.futures.ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
@@ -58,6 +87,7 @@ def build(verbose=True):
     os.replace(tmp, out)
     if verbose:
         print(f"built {out}")
+    build_fortran(out, verbose)
     return out
 
 
