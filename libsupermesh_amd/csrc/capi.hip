@@ -58,7 +58,7 @@ static void free_context(Context& C) {
                     &C.mesh_b.planes, &C.mesh_b.affine, &C.bvh.boxes, &C.bvh.perm, &C.keys,
                     &C.keys_alt, &C.vals, &C.vals_alt, &C.sort_tmp, &C.scan_tmp, &C.qperm,
                     &C.row_count, &C.indptr, &C.indices, &C.rows, &C.counts, &C.measure,
-                    &C.partials, &C.scalars, &C.offsets, &C.elements, &C.blocks, &C.mflag,
+                    &C.partials, &C.scalars, &C.offsets, &C.elements, &C.blocks, &C.mflag, &C.mslot, &C.work, &C.nwork,
                     &C.mkeys, &C.mvals, &C.mkeys_alt, &C.mvals_alt, &C.ukeys, &C.uvals, &C.nuniq,
                     &C.mindptr, &C.mindices, &C.one_in, &C.one_out};
   for (DevBuf* b : bufs) b->release();

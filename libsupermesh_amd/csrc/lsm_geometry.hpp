@@ -207,17 +207,35 @@ struct Fan {
   int n;
 };
 
-// Clips tri A against B's lines.  For each kept output triangle (in the
-// reference's order) calls sink(x0, y0, x1, y1, x2, y2, area).  Returns the
-// number of kept triangles; if any Sutherland-Hodgman pass ends with fewer
-// than 3 points the reference returns 0 (and the sink's calls must be
-// discarded by the caller, which only happens before a count is known).
-template <class Sink>
-LSM_HD int clip_tri(const double* A, const Line* L, double tol, Sink&& sink) {
-  ClipStage s1, s2, s3;
-  stage_init(s1);
-  stage_init(s2);
-  stage_init(s3);
+// Pushes a point into pass S of an NL-pass pipeline (pass NL is the fan).
+template <int S, int NL, class Last>
+LSM_HD void pipe_push(ClipStage* st, const Line* L, double x, double y, Last& last) {
+  if constexpr (S == NL) {
+    last(x, y);
+  } else {
+    stage_push(st[S], L[S], x, y, [&](double u, double v) { pipe_push<S + 1, NL>(st, L, u, v, last); });
+  }
+}
+
+template <int S, int NL, class Last>
+LSM_HD void pipe_close(ClipStage* st, const Line* L, Last& last) {
+  if constexpr (S < NL) {
+    stage_close(st[S], [&](double u, double v) { pipe_push<S + 1, NL>(st, L, u, v, last); });
+    pipe_close<S + 1, NL>(st, L, last);
+  }
+}
+
+// Clips triangle A (3 points) against NL lines, passes in line order, then
+// fans the result (intersect_polys_lines, Triangle_Intersection.F90:216-276).
+// For each kept output triangle (in the reference's order) calls
+// sink(x0, y0, x1, y1, x2, y2, area).  Returns the number kept; if any pass
+// ends with fewer than 3 points the reference returns 0 (and calls made to
+// the sink must then be discarded, which only matters before a count is known).
+template <int NL, class Sink>
+LSM_HD int clip_poly(const double* A, const Line* L, double tol, Sink&& sink) {
+  ClipStage st[NL];
+#pragma unroll
+  for (int i = 0; i < NL; i++) stage_init(st[i]);
   Fan fan;
   fan.x0 = fan.y0 = fan.x1 = fan.y1 = 0.0;
   fan.n = 0;
@@ -237,16 +255,44 @@ LSM_HD int clip_tri(const double* A, const Line* L, double tol, Sink&& sink) {
     fan.y1 = y;
     fan.n++;
   };
-  auto push3 = [&](double x, double y) { stage_push(s3, L[2], x, y, fan_push); };
-  auto push2 = [&](double x, double y) { stage_push(s2, L[1], x, y, push3); };
-  stage_push(s1, L[0], A[0], A[1], push2);
-  stage_push(s1, L[0], A[2], A[3], push2);
-  stage_push(s1, L[0], A[4], A[5], push2);
-  stage_close(s1, push2);
-  stage_close(s2, push3);
-  stage_close(s3, fan_push);
-  if (s1.nout < 3 || s2.nout < 3 || s3.nout < 3) return 0;
+  pipe_push<0, NL>(st, L, A[0], A[1], fan_push);
+  pipe_push<0, NL>(st, L, A[2], A[3], fan_push);
+  pipe_push<0, NL>(st, L, A[4], A[5], fan_push);
+  pipe_close<0, NL>(st, L, fan_push);
+#pragma unroll
+  for (int i = 0; i < NL; i++)
+    if (st[i].nout < 3) return 0;
   return kept;
+}
+
+// Triangle A against triangle B's lines (intersect_tris_tri,
+// Triangle_Intersection.F90:144-185).
+template <class Sink>
+LSM_HD int clip_tri(const double* A, const Line* L, double tol, Sink&& sink) {
+  return clip_poly<3>(A, L, tol, sink);
+}
+
+// get_lines_poly for a 4-vertex polygon (Triangle_Intersection.F90:422-451):
+// normals (dy, -dx) per edge, all flipped if vertex 1 is on the positive side
+// of line 2.
+LSM_HD void quad_lines(const double* q, Line L[4]) {
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int k1 = (k + 1) % 4;
+    L[k].nx = (q[2 * k1 + 1] - q[2 * k + 1]);
+    L[k].ny = -(q[2 * k1] - q[2 * k]);
+    L[k].px = q[2 * k];
+    L[k].py = q[2 * k + 1];
+  }
+  double d = (q[0] - L[1].px) * L[1].nx;
+  d = d + (q[1] - L[1].py) * L[1].ny;
+  if (d > 0.0) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      L[k].nx = -L[k].nx;
+      L[k].ny = -L[k].ny;
+    }
+  }
 }
 
 LSM_HD double tri_tol(double area_a, double area_b) {
@@ -710,25 +756,28 @@ LSM_HD void build_child(const double* T, const double* d, const Recipe& r, doubl
   }
 }
 
-// Walk state of one pair: tets at depths 1..3 (depth 0, tet A, is passed to
-// each step), their packed classes against the next plane, the depth of the
-// next child to build (1..4) and the next child index per depth.  A parent's
-// distances to its plane are recomputed when a child is built (the same
-// operations, hence the same values) rather than kept, to save registers.
-// B's planes are read through `planes` (16 doubles: n0 c0 n1 c1 ..).
-struct TetWalk {
-  double T[3][12];
-  int cls[4];
+// Walk state of one pair clipped by NP planes: tets at depths 1..NP-1 (depth
+// 0, tet A, is passed to each step), their packed classes against the next
+// plane, the depth of the next child to build (1..NP) and the next child index
+// per depth.  A parent's distances to its plane are recomputed when a child is
+// built (the same operations, hence the same values) rather than kept, to save
+// registers.  The planes are read through `planes` (NP x [n0 n1 n2 c]).
+template <int NP>
+struct TetWalkN {
+  double T[NP - 1][12];
+  int cls[NP];
   int level;
-  int digits;  // 2 bits per depth 1..4
+  int digits;  // 2 bits per depth 1..NP
 };
+using TetWalk = TetWalkN<4>;
 
 LSM_HD void walk_distances(const double* T, const double* pl, double* d) {
 #pragma unroll
   for (int v = 0; v < 4; v++) d[v] = dot3(pl[0], pl[1], pl[2], T[3 * v], T[3 * v + 1], T[3 * v + 2]) - pl[3];
 }
 
-LSM_HD void walk_start(TetWalk& W, const double* A, const double* planes) {
+template <int NP>
+LSM_HD void walk_start(TetWalkN<NP>& W, const double* A, const double* planes) {
   double d[4];
   walk_distances(A, planes, d);
   W.cls[0] = pack_class(d);
@@ -738,14 +787,13 @@ LSM_HD void walk_start(TetWalk& W, const double* A, const double* planes) {
 
 // One step.  Returns false once the pair's walk is complete.  Calls
 // sink(tet, vol_tet) for each kept output tet, in the reference's order.
-template <class Sink>
-LSM_HD bool walk_step(TetWalk& W, const double* A, const double* planes, double tol, Sink&& sink) {
+template <int NP, class Sink>
+LSM_HD bool walk_step(TetWalkN<NP>& W, const double* A, const double* planes, double tol, Sink&& sink) {
   const int k = W.level;
   const int ci = (W.digits >> (2 * k)) & 3;
-  int cls = in_reg_i(W.cls[3]);
-  cls = k == 3 ? in_reg_i(W.cls[2]) : cls;
-  cls = k == 2 ? in_reg_i(W.cls[1]) : cls;
-  cls = k == 1 ? in_reg_i(W.cls[0]) : cls;
+  int cls = in_reg_i(W.cls[0]);
+#pragma unroll
+  for (int j = 1; j < NP; j++) cls = k == j + 1 ? in_reg_i(W.cls[j]) : cls;
   if (ci >= class_nchild(cls)) {
     // all children of this parent done; its own index was advanced on descent
     if (k == 1) return false;
@@ -755,17 +803,16 @@ LSM_HD bool walk_step(TetWalk& W, const double* A, const double* planes, double 
   double parent[12], pd[4];
 #pragma unroll
   for (int i = 0; i < 12; i++) {
-    double v = in_reg(W.T[2][i]);
-    v = k == 3 ? in_reg(W.T[1][i]) : v;
-    v = k == 2 ? in_reg(W.T[0][i]) : v;
-    v = k == 1 ? in_reg(A[i]) : v;
+    double v = in_reg(A[i]);
+#pragma unroll
+    for (int j = 0; j < NP - 1; j++) v = k == j + 2 ? in_reg(W.T[j][i]) : v;
     parent[i] = v;
   }
   walk_distances(parent, planes + 4 * (k - 1), pd);
   double child[12];
   build_child(parent, pd, make_recipe(cls, ci), child);
-  if (k == 4) {
-    W.digits += 1 << 8;
+  if (k == NP) {
+    W.digits += 1 << (2 * NP);
     double vol = tet_volume_tet(child);
     if (vol >= tol) sink(child, vol);
     return true;
@@ -775,7 +822,7 @@ LSM_HD bool walk_step(TetWalk& W, const double* A, const double* planes, double 
   walk_distances(child, planes + 4 * k, dc);
   const int cc = pack_class(dc);
 #pragma unroll
-  for (int j = 0; j < 3; j++) {
+  for (int j = 0; j < NP - 1; j++) {
 #pragma unroll
     for (int i = 0; i < 12; i++) W.T[j][i] = (k == j + 1) ? child[i] : W.T[j][i];
     W.cls[j + 1] = (k == j + 1) ? cc : W.cls[j + 1];
@@ -794,9 +841,48 @@ LSM_HD int clip_tet_walk(const double* A, const Planes& P, double tol, Sink&& si
     planes[4 * i + 3] = P.c[i];
   }
   TetWalk W;
-  walk_start(W, A, planes);
+  walk_start<4>(W, A, planes);
   int kept = 0;
-  while (walk_step(W, A, planes, tol, [&](const double* t, double v) {
+  while (walk_step<4>(W, A, planes, tol, [&](const double* t, double v) {
+    sink(t, v);
+    kept++;
+  })) {
+  }
+  return kept;
+}
+
+
+// The six face planes of a hexahedron in Gmsh node order, normals from
+// unit_cross as written in intersect_tet_hex (Supermesh.F90:269-280); no
+// orientation correction.  planes: 6 x [n0 n1 n2 c].
+LSM_HD void hex_planes(const double* h, double* planes) {
+  // (edge a = h[a1] - h[a0], edge b = h[b1] - h[a0], plane point h[a0])
+  const int a0[6] = {0, 4, 5, 1, 0, 7}, a1[6] = {1, 5, 1, 0, 4, 6}, b1[6] = {5, 6, 2, 3, 7, 2};
+#pragma unroll
+  for (int f = 0; f < 6; f++) {
+    const double* o = h + 3 * a0[f];
+    const double* p = h + 3 * a1[f];
+    const double* q = h + 3 * b1[f];
+    double n[3];
+    unit_cross(p[0] - o[0], p[1] - o[1], p[2] - o[2], q[0] - o[0], q[1] - o[1], q[2] - o[2], n);
+    planes[4 * f] = n[0];
+    planes[4 * f + 1] = n[1];
+    planes[4 * f + 2] = n[2];
+    planes[4 * f + 3] = dot3(o[0], o[1], o[2], n[0], n[1], n[2]);
+  }
+}
+
+// Tet A clipped by the 6 planes of hex B, tolerance from A's volume only
+// (intersect_tet_hex -> intersect_tet_planes, Tetrahedron_Intersection.F90:191-241).
+template <class Sink>
+LSM_HD int clip_tet_hex(const double* A, const double* hex, Sink&& sink) {
+  double planes[24];
+  hex_planes(hex, planes);
+  const double tol = 10.0 * fspacing(tet_volume_tet(A));
+  TetWalkN<6> W;
+  walk_start<6>(W, A, planes);
+  int kept = 0;
+  while (walk_step<6>(W, A, planes, tol, [&](const double* t, double v) {
     sink(t, v);
     kept++;
   })) {

@@ -44,7 +44,11 @@ struct DevBuf {
   void* get(size_t need) {
     if (need == 0) need = 1;
     if (need > bytes) {
-      if (ptr) (void)hipFree(ptr);
+      // queued kernels may still use the old allocation
+      if (ptr) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(ptr);
+      }
       ptr = nullptr;
       bytes = 0;
       size_t cap = need + need / 4;
@@ -104,7 +108,7 @@ struct Context {
   DevBuf counts, measure, partials, scalars, offsets, elements, blocks;
   DevBuf work, nwork;            // compacted list of pairs that need the full clip
   // mass assembly
-  DevBuf mflag, mkeys, mvals, mkeys_alt, mvals_alt, ukeys, uvals, nuniq, mindptr, mindices;
+  DevBuf mflag, mslot, mkeys, mvals, mkeys_alt, mvals_alt, ukeys, uvals, nuniq, mindptr, mindices;
   // per-pair host API staging
   DevBuf one_in, one_out;
   void* pinned = nullptr;

@@ -12,6 +12,7 @@
 #include <cstring>
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_reduce.hpp>
 #include <rocprim/device/device_scan.hpp>
 
 #include "lsm_geometry.hpp"
@@ -337,10 +338,10 @@ __global__ void __launch_bounds__(kClipBlock) k_clip_tet_walk(ClipArgs g) {
           affb[i] = g.baff[b * 12 + i];
         }
       }
-      walk_start(W, A, planes);
+      walk_start<4>(W, A, planes);
       active = true;
     }
-    const bool more = walk_step(W, A, planes, tol, [&](const double* t, double) {
+    const bool more = walk_step<4>(W, A, planes, tol, [&](const double* t, double) {
       const double vr = tet_volume_real(t);
       m = m + vr;
       if (MASS) mass_element<3>(t, vr, affa, affb, blk);
@@ -622,6 +623,226 @@ k_segment_sum(long n, const uint64_t* __restrict__ keys, const double* __restric
   atomicAdd(&rowcnt[k / (uint64_t)nb], 1ULL);
 }
 
+// ---------------------------------------------------------------------------
+// Row-wise mixed mass assembly.  Row i of M (A node i) collects, from every A
+// element e containing i (local vertex li) and every non-empty pair p of e,
+// the entries (B node of local vertex j, blocks[p][li][j]).  One wave owns a
+// row: it gathers the row's entries into LDS in a fixed order (elements of i
+// in ascending (e, li), pairs of e in CSR order, j ascending), sorts them by
+// B node with a bitonic network, and sums each run of equal keys in sorted
+// order -- no global sort, and the same result on every run.
+// ---------------------------------------------------------------------------
+
+constexpr int kRowBlock = 64;
+
+__global__ void __launch_bounds__(256)
+k_node_count(long ne, int loc, const int* __restrict__ enl, unsigned int* __restrict__ ncount) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ne * loc) return;
+  atomicAdd(&ncount[enl[t]], 1u);
+}
+
+__global__ void __launch_bounds__(256)
+k_node_fill(long ne, int loc, const int* __restrict__ enl, const long* __restrict__ nptr,
+            unsigned int* __restrict__ cursor, int* __restrict__ nlist) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ne * loc) return;
+  int v = enl[t];
+  unsigned int k = atomicAdd(&cursor[v], 1u);
+  nlist[nptr[v] + k] = (int)t;  // t = e*loc + li
+}
+
+// Sorts each node's entries (arrival order of the atomics is arbitrary) and
+// counts, per entry (node, e, li), the row entries it contributes: loc per
+// non-empty pair of e.
+__global__ void __launch_bounds__(256)
+k_node_rows(long nn, int loc, const long* __restrict__ nptr, int* __restrict__ nlist,
+            const int* __restrict__ elem_nonempty, long* __restrict__ contrib) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  long b0 = nptr[i], b1 = nptr[i + 1];
+  for (long x = b0; x < b1; x++) {
+    int v = nlist[x];
+    long y = x - 1;
+    while (y >= b0 && nlist[y] > v) {
+      nlist[y + 1] = nlist[y];
+      y--;
+    }
+    nlist[y + 1] = v;
+  }
+  for (long x = b0; x < b1; x++) contrib[x] = (long)elem_nonempty[nlist[x] / loc] * loc;
+}
+
+// Writes the row entries of node-list entry x (element e, local vertex li) at
+// slot[x]: B node and block value for each non-empty pair of e, in CSR order.
+template <int LOC>
+__global__ void __launch_bounds__(256)
+k_mass_scatter(long nent, const int* __restrict__ nlist, const long* __restrict__ slot,
+               const long* __restrict__ indptr, const int* __restrict__ cols,
+               const int* __restrict__ counts, const double* __restrict__ blocks,
+               const int* __restrict__ enlb, int* __restrict__ rkey, double* __restrict__ rval) {
+  long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= nent) return;
+  const int el = nlist[x];
+  const int e = el / LOC, li = el - e * LOC;
+  long o = slot[x];
+  for (long p = indptr[e]; p < indptr[e + 1]; p++) {
+    if (counts[p] == 0) continue;
+    const long b = cols[p];
+#pragma unroll
+    for (int j = 0; j < LOC; j++) {
+      rkey[o + j] = enlb[b * LOC + j];
+      rval[o + j] = blocks[p * LOC * LOC + li * LOC + j];
+    }
+    o += LOC;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_row_bounds(long nn, const long* __restrict__ nptr, const long* __restrict__ slot,
+             long* __restrict__ raw_off, long* __restrict__ raw) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > nn) return;
+  raw_off[i] = slot[nptr[i]];
+  if (i < nn) raw[i] = slot[nptr[i + 1]] - slot[nptr[i]];
+}
+
+__global__ void __launch_bounds__(256)
+k_elem_nonempty(long ne, const long* __restrict__ indptr, const int* __restrict__ counts,
+                int* __restrict__ out) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  int c = 0;
+  for (long p = indptr[e]; p < indptr[e + 1]; p++) c += counts[p] > 0;
+  out[e] = c;
+}
+
+// Compare-exchange of registers r and r ^ JR (distance 64*JR) when the
+// stage's distance is 64*jr == 64*JR; JR is a compile-time constant so the
+// register indices stay static.
+template <int EPL, int JR>
+__device__ __forceinline__ void reg_stage(int* key, double* val, int lane, int n2, int k, int jr) {
+  if constexpr (JR < EPL) {
+    if (jr != JR) return;
+#pragma unroll
+    for (int r = 0; r < EPL; r++) {
+      const int r2 = r ^ JR;
+      if (r2 > r && 64 * r2 < n2) {
+        const int t = lane + 64 * r;
+        const bool up = (t & k) == 0;
+        if ((key[r] > key[r2]) == up) {
+          const int kt = key[r];
+          key[r] = key[r2];
+          key[r2] = kt;
+          const double vt = val[r];
+          val[r] = val[r2];
+          val[r2] = vt;
+        }
+      }
+    }
+  }
+}
+
+// Bitonic sort of n2 = 64*EPL (key, val) pairs held in registers: element
+// t = lane + 64*r.  Partners at distance j < 64 sit in another lane (same r)
+// and are exchanged with __shfl_xor; larger distances pair registers r and
+// r ^ (j/64) of the same lane.
+template <int EPL>
+__device__ __forceinline__ void wave_bitonic(int* key, double* val, int lane, int n2) {
+  for (int k = 2; k <= n2; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j < 64) {
+#pragma unroll
+        for (int r = 0; r < EPL; r++) {
+          const int t = lane + 64 * r;
+          const int ok = __shfl_xor(key[r], j);
+          const double ov = __shfl_xor(val[r], j);
+          const bool lower = (t & j) == 0;        // t is the lower index of its pair
+          const bool up = (t & k) == 0;           // ascending block
+          const bool take_min = lower == up;
+          const bool swap = take_min ? (ok < key[r]) : (ok > key[r]);
+          if (t < n2 && swap) {
+            key[r] = ok;
+            val[r] = ov;
+          }
+        }
+      } else {
+        const int jr = j >> 6;
+        reg_stage<EPL, 1>(key, val, lane, n2, k, jr);
+        reg_stage<EPL, 2>(key, val, lane, n2, k, jr);
+        reg_stage<EPL, 4>(key, val, lane, n2, k, jr);
+        reg_stage<EPL, 8>(key, val, lane, n2, k, jr);
+        reg_stage<EPL, 16>(key, val, lane, n2, k, jr);
+      }
+    }
+  }
+}
+
+template <int EPL>
+__global__ void __launch_bounds__(kRowBlock)
+k_mass_rows(long nn, const long* __restrict__ raw_off, int* __restrict__ tkey, double* __restrict__ tval,
+            long* __restrict__ ucount) {
+  constexpr int CAP = 64 * EPL;
+  __shared__ int skey[CAP];
+  __shared__ double sval[CAP];
+  const int lane = threadIdx.x;
+  for (long i = blockIdx.x; i < nn; i += gridDim.x) {
+    const long o0 = raw_off[i];
+    const int n = (int)(raw_off[i + 1] - o0);
+    int n2 = 64;
+    while (n2 < n) n2 <<= 1;
+    int key[EPL];
+    double val[EPL];
+#pragma unroll
+    for (int r = 0; r < EPL; r++) {
+      const int t = lane + 64 * r;
+      key[r] = t < n ? tkey[o0 + t] : 0x7fffffff;
+      val[r] = t < n ? tval[o0 + t] : 0.0;
+    }
+    wave_bitonic<EPL>(key, val, lane, n2);
+#pragma unroll
+    for (int r = 0; r < EPL; r++) {
+      const int t = lane + 64 * r;
+      if (t < n2) {
+        skey[t] = key[r];
+        sval[t] = val[r];
+      }
+    }
+    __syncthreads();
+    // runs of equal keys: the head of each run sums it in sorted order, and
+    // the row is rewritten in place
+    int before = 0;
+    for (int t0 = 0; t0 < n; t0 += kRowBlock) {
+      const int t = t0 + lane;
+      const bool head = t < n && (t == 0 || skey[t] != skey[t - 1]);
+      const unsigned long long m = __ballot(head);
+      if (head) {
+        double sacc = 0.0;
+        for (int r = t; r < n && skey[r] == skey[t]; r++) sacc = sacc + sval[r];
+        const long o = o0 + before + __popcll(m & ((1ULL << lane) - 1));
+        tkey[o] = skey[t];
+        tval[o] = sacc;
+      }
+      before += __popcll(m);
+    }
+    if (lane == 0) ucount[i] = before;
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_mass_copy(long nn, const long* __restrict__ raw_off, const long* __restrict__ mptr,
+            const int* __restrict__ tkey, const double* __restrict__ tval, int* __restrict__ mind,
+            double* __restrict__ mval) {
+  const long i = (long)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  if (i >= nn) return;
+  const long n = mptr[i + 1] - mptr[i];
+  for (long t = threadIdx.x & 63; t < n; t += 64) {
+    mind[mptr[i] + t] = tkey[raw_off[i] + t];
+    mval[mptr[i] + t] = tval[raw_off[i] + t];
+  }
+}
+
 template <class T>
 static void inclusive_scan_long(Context& C, const T* in, long* out, long n) {
   size_t bytes = 0;
@@ -630,7 +851,7 @@ static void inclusive_scan_long(Context& C, const T* in, long* out, long n) {
   LSM_HIP(rocprim::inclusive_scan(tmp, bytes, in, out, (size_t)n, rocprim::plus<long>(), C.stream));
 }
 
-void assemble_mass(Context& C, const DevMesh& A, const DevMesh& B, long npairs,
+static void assemble_mass_sorted(Context& C, const DevMesh& A, const DevMesh& B, long npairs,
                    libsupermesh_gpu_result* out) {
   const int loc = A.dim + 1;
   const long na_nodes = A.nnodes, nb_nodes = B.nnodes;
@@ -687,6 +908,102 @@ void assemble_mass(Context& C, const DevMesh& A, const DevMesh& B, long npairs,
                      pos, nb_nodes, mind, mval, rowcnt);
   LSM_HIP(hipGetLastError());
   inclusive_scan_long(C, reinterpret_cast<const long*>(rowcnt), mindptr + 1, na_nodes);
+  LSM_HIP(hipStreamSynchronize(C.stream));
+  out->mass_nnz = nnz;
+  out->mass_indices = mind;
+  out->mass_values = mval;
+}
+
+template <int EPL>
+static void launch_rows(Context& C, long nn, const long* raw_off, int* tkey, double* tval, long* ucount) {
+  hipLaunchKernelGGL((k_mass_rows<EPL>), dim3(8192), dim3(kRowBlock), 0, C.stream, nn, raw_off, tkey, tval,
+                     ucount);
+}
+
+void assemble_mass(Context& C, const DevMesh& A, const DevMesh& B, long npairs,
+                   libsupermesh_gpu_result* out) {
+  const int loc = A.dim + 1;
+  const long na = A.nelements, nn = A.nnodes;
+  long* mindptr = C.mindptr.as<long>(nn + 1);
+  LSM_HIP(hipMemsetAsync(mindptr, 0, sizeof(long) * (nn + 1), C.stream));
+  out->mass_nnz = 0;
+  out->mass_indptr = mindptr;
+  out->mass_indices = static_cast<const int*>(C.mindices.get(4));
+  out->mass_values = static_cast<const double*>(C.uvals.get(8));
+  if (out->nonempty == 0 || nn == 0) {
+    LSM_HIP(hipStreamSynchronize(C.stream));
+    return;
+  }
+  // A's node -> (element, local vertex) lists
+  unsigned int* ncount = reinterpret_cast<unsigned int*>(C.mflag.as<long>(2 * nn + 2));
+  unsigned int* cursor = ncount + nn + 1;
+  LSM_HIP(hipMemsetAsync(ncount, 0, sizeof(unsigned int) * (2 * nn + 2), C.stream));
+  const long nent = na * loc;
+  hipLaunchKernelGGL(k_node_count, dim3(grid_for(nent, 256)), dim3(256), 0, C.stream, na, loc,
+                     A.enlist_dev, ncount);
+  long* nptr = C.nuniq.as<long>(3 * (nn + 1));
+  long* raw = nptr + (nn + 1);
+  long* raw_off = raw + (nn + 1);
+  LSM_HIP(hipMemsetAsync(nptr, 0, sizeof(long), C.stream));
+  LSM_HIP(hipMemsetAsync(raw_off, 0, sizeof(long), C.stream));
+  inclusive_scan_long(C, ncount, nptr + 1, nn);
+  int* nlist = C.ukeys.as<int>(nent + 1);
+  hipLaunchKernelGGL(k_node_fill, dim3(grid_for(nent, 256)), dim3(256), 0, C.stream, na, loc,
+                     A.enlist_dev, nptr, cursor, nlist);
+  int* elem_ne = C.mkeys_alt.as<int>(na + 1);
+  hipLaunchKernelGGL(k_elem_nonempty, dim3(grid_for(na, 256)), dim3(256), 0, C.stream, na, out->indptr,
+                     out->counts, elem_ne);
+  long* contrib = C.mslot.as<long>(2 * nent + 2);
+  long* slot = contrib + nent + 1;
+  hipLaunchKernelGGL(k_node_rows, dim3(grid_for(nn, 256)), dim3(256), 0, C.stream, nn, loc, nptr, nlist,
+                     elem_ne, contrib);
+  LSM_HIP(hipGetLastError());
+  LSM_HIP(hipMemsetAsync(slot, 0, sizeof(long), C.stream));
+  inclusive_scan_long(C, contrib, slot + 1, nent);
+  hipLaunchKernelGGL(k_row_bounds, dim3(grid_for(nn + 1, 256)), dim3(256), 0, C.stream, nn, nptr, slot,
+                     raw_off, raw);
+  LSM_HIP(hipGetLastError());
+  long maxraw = 0;
+  {
+    // the longest row decides the LDS capacity of the row kernel
+    size_t bytes = 0;
+    long* dmax = reinterpret_cast<long*>(C.scalars.as<long>(64)) + 8;
+    LSM_HIP(rocprim::reduce(nullptr, bytes, raw, dmax, 0L, (size_t)nn, rocprim::maximum<long>(), C.stream));
+    void* tmp = C.sort_tmp.get(bytes);
+    LSM_HIP(rocprim::reduce(tmp, bytes, raw, dmax, 0L, (size_t)nn, rocprim::maximum<long>(), C.stream));
+    LSM_HIP(hipMemcpyAsync(&maxraw, dmax, sizeof(long), hipMemcpyDeviceToHost, C.stream));
+  }
+  long total_raw = 0;
+  LSM_HIP(hipMemcpyAsync(&total_raw, raw_off + nn, sizeof(long), hipMemcpyDeviceToHost, C.stream));
+  LSM_HIP(hipStreamSynchronize(C.stream));
+  if (maxraw > 2048) {  // rows too long for a register-resident sort: global sort path
+    assemble_mass_sorted(C, A, B, npairs, out);
+    return;
+  }
+  int* tkey = C.mkeys.as<int>(total_raw + 1);
+  double* tval = C.mvals.as<double>(total_raw + 1);
+  long* ucount = C.mvals_alt.as<long>(nn + 1);
+  if (loc == 3)
+    hipLaunchKernelGGL(k_mass_scatter<3>, dim3(grid_for(nent, 256)), dim3(256), 0, C.stream, nent, nlist,
+                       slot, out->indptr, out->indices, out->counts, out->blocks, B.enlist_dev, tkey, tval);
+  else
+    hipLaunchKernelGGL(k_mass_scatter<4>, dim3(grid_for(nent, 256)), dim3(256), 0, C.stream, nent, nlist,
+                       slot, out->indptr, out->indices, out->counts, out->blocks, B.enlist_dev, tkey, tval);
+  if (maxraw <= 128) launch_rows<2>(C, nn, raw_off, tkey, tval, ucount);
+  else if (maxraw <= 256) launch_rows<4>(C, nn, raw_off, tkey, tval, ucount);
+  else if (maxraw <= 512) launch_rows<8>(C, nn, raw_off, tkey, tval, ucount);
+  else if (maxraw <= 1024) launch_rows<16>(C, nn, raw_off, tkey, tval, ucount);
+  else launch_rows<32>(C, nn, raw_off, tkey, tval, ucount);
+  LSM_HIP(hipGetLastError());
+  inclusive_scan_long(C, ucount, mindptr + 1, nn);
+  long nnz = 0;
+  LSM_HIP(hipMemcpyAsync(&nnz, mindptr + nn, sizeof(long), hipMemcpyDeviceToHost, C.stream));
+  LSM_HIP(hipStreamSynchronize(C.stream));
+  int* mind = C.mindices.as<int>(nnz);
+  double* mval = C.uvals.as<double>(nnz);
+  hipLaunchKernelGGL(k_mass_copy, dim3(grid_for(nn, 4)), dim3(256), 0, C.stream, nn, raw_off, mindptr, tkey,
+                     tval, mind, mval);
+  LSM_HIP(hipGetLastError());
   LSM_HIP(hipStreamSynchronize(C.stream));
   out->mass_nnz = nnz;
   out->mass_indices = mind;
