@@ -4,9 +4,11 @@
  * Same names, argument meaning and layouts as the reference's
  * include/libsupermesh-c.h (double precision build): positions are
  * double[nnodes][dim], element-node lists long[nelements][loc] with 0-based
- * node ids, triangles double[3][2], tetrahedra double[4][3].  Every function
- * runs on the GPU (device 0); see libsupermesh_gpu.h for the bulk entry
- * points that a GPU caller should prefer over per-pair calls.
+ * node ids, triangles double[3][2], tetrahedra double[4][3], quads double[4][2],
+ * hexahedra double[8][3] (Gmsh vertex order), intervals double[2].  The 2D and
+ * 3D functions run on the GPU (device 0); the 1D ones (intervals and the sort
+ * finder) are host code.  See libsupermesh_gpu.h for the bulk entry points
+ * that a GPU caller should prefer over per-pair calls.
  *
  * Errors: the reference aborts the process (src/Debug.F90:78-94); these
  * functions instead return with the outputs zeroed and record the failure,
@@ -37,6 +39,14 @@ void libsupermesh_tree_intersection_finder_set_input(long* nnodes_a, int* dim_a,
 void libsupermesh_tree_intersection_finder_query_output(long* nindices);
 void libsupermesh_tree_intersection_finder_get_output(long* nelements, long* nindices, long* indices, long* ind_ptr);
 
+/* 1D sort finder (intervals, loc 2): sorts both meshes by left end point and
+ * sweeps A with one cursor into B, as the reference does; the rows hold B ids
+ * in sweep order.  Host code.
+ * Replaces src/C_bindings.F90:66-111 (include/libsupermesh-c.h:18-20). */
+void libsupermesh_sort_intersection_finder_set_input(long* nnodes_a, int* dim_a, long* nelements_a, int* loc_a, long* nnodes_b, int* dim_b, long* nelements_b, int* loc_b, c_kind* positions_a, long* enlist_a, c_kind* positions_b, long* enlist_b);
+void libsupermesh_sort_intersection_finder_query_output(long* nindices);
+void libsupermesh_sort_intersection_finder_get_output(long* nelements, long* nindices, long* indices, long* ind_ptr);
+
 /* R-tree candidate finder semantics (closed AABB overlap), same calling
  * sequence.  The reference reaches it only from Fortran
  * (rtree_intersection_finder, src/Intersection_Finder.F90:577-643); these
@@ -56,6 +66,19 @@ void libsupermesh_triangle_area(c_kind* tri, c_kind* area);
 /* Edges from the first vertex, times 1/6.
  * Replaces src/C_bindings.F90:134-141 (include/libsupermesh-c.h:27). */
 void libsupermesh_tetrahedron_volume(c_kind* tet, c_kind* volume);
+
+/* |interval[1] - interval[0]|.  Host code.
+ * Replaces src/C_bindings.F90:125-132 (include/libsupermesh-c.h:28). */
+void libsupermesh_interval_size(c_kind* interval, c_kind* size);
+/* Interval-interval intersection: *n_intervals_c is 0 or 1.  Host code.
+ * Replaces src/C_bindings.F90:113-123 (include/libsupermesh-c.h:29). */
+void libsupermesh_intersect_intervals(c_kind* interval_a, c_kind* interval_b, c_kind* intervals_c, int* n_intervals_c);
+/* Tet clipped by the 6 face planes of a hexahedron: tets_c[729][4][3].
+ * Replaces src/C_bindings.F90:42-52 (include/libsupermesh-c.h:30). */
+void libsupermesh_intersect_tet_hex(c_kind* tet_a, c_kind* hex_b, c_kind* tets_c, int* n_tets_c);
+/* Triangle clipped by the 4 edge lines of a quad: tris_c[46][3][2].
+ * Replaces src/C_bindings.F90:54-64 (include/libsupermesh-c.h:31). */
+void libsupermesh_intersect_tri_quad(c_kind* tri_a, c_kind* quad_b, c_kind* tris_c, int* n_tris_c);
 
 /* Error of the last call on this thread: text ("" if none) and code
  * (0 = ok; see libsupermesh_gpu.h).  No counterpart in the reference, which

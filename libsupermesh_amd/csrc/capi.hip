@@ -211,17 +211,48 @@ __global__ void k_one(int kind, const double* __restrict__ in, double* __restric
     out[0] = (double)n;
   } else if (kind == 2) {
     out[0] = triangle_area(in);
-  } else {
+  } else if (kind == 3) {
     out[0] = tet_volume_real(in);
+  } else if (kind == 4) {  // intersect_tri_quad: tri (6) then quad (8)
+    Line L[4];
+    quad_lines(in + 6, L);
+    const double tol = 10.0 * fspacing(triangle_area(in));
+    int k = 0;
+    int n = clip_poly<4>(in, L, tol, [&](double x0, double y0, double x1, double y1, double x2,
+                                         double y2, double) {
+      if (k < kTriQuadBufSize) {
+        double* o = out + 1 + 6 * k;
+        o[0] = x0;
+        o[1] = y0;
+        o[2] = x1;
+        o[3] = y1;
+        o[4] = x2;
+        o[5] = y2;
+      }
+      k++;
+    });
+    out[0] = (double)n;
+  } else {  // intersect_tet_hex: tet (12) then hex (24)
+    int k = 0;
+    int n = clip_tet_hex(in, in + 12, [&](const double* t, double) {
+      if (k < kTetHexBufSize)
+        for (int i = 0; i < 12; i++) out[1 + 12 * k + i] = t[i];
+      k++;
+    });
+    out[0] = (double)n;
   }
 }
+
+constexpr int kOneIn = 36;                          // largest input: tet + hex
+constexpr int kOneOut = 1 + 12 * kTetHexBufSize;  // largest output: 729 tets
 
 // Runs k_one on the default context: nin doubles in, nout doubles back.
 static void one_call(int kind, const double* in, int nin, double* result, int nout) {
   Context& C = *default_context();
   std::lock_guard<std::mutex> lk(C.mu);
   LSM_HIP(hipSetDevice(C.device));
-  size_t need = sizeof(double) * (size_t)(24 + 1 + 81 * 12);
+  LSM_REQUIRE(nin <= kOneIn && nout <= kOneOut, LIBSUPERMESH_GPU_ERROR_INVALID, "per-pair buffer size");
+  size_t need = sizeof(double) * (size_t)(kOneIn + kOneOut);
   if (!C.pinned) {
     LSM_HIP(hipHostMalloc(&C.pinned, 2 * need, hipHostMallocDefault));
     C.pinned_bytes = 2 * need;
@@ -229,8 +260,8 @@ static void one_call(int kind, const double* in, int nin, double* result, int no
   double* hin = static_cast<double*>(C.pinned);
   double* hout = hin + need / sizeof(double);
   std::memcpy(hin, in, sizeof(double) * nin);
-  double* din = C.one_in.as<double>(24);
-  double* dout = C.one_out.as<double>(1 + 81 * 12);
+  double* din = C.one_in.as<double>(kOneIn);
+  double* dout = C.one_out.as<double>(kOneOut);
   LSM_HIP(hipMemcpyAsync(din, hin, sizeof(double) * nin, hipMemcpyHostToDevice, C.stream));
   hipLaunchKernelGGL(k_one, dim3(1), dim3(64), 0, C.stream, kind, din, dout);
   LSM_HIP(hipGetLastError());
@@ -251,7 +282,7 @@ struct FinderState {
   std::vector<int> indices;
 };
 
-static FinderState g_tree, g_rtree;
+static FinderState g_tree, g_rtree, g_sort;
 
 static void host_mesh_to_device(Context& C, DevMesh& M, long nnodes, int dim, long ne, int loc,
                                 const double* pos, const long* enl, libsupermesh_gpu_mesh& view) {
@@ -512,6 +543,71 @@ void libsupermesh_triangle_area(double* tri, double* area) {
 
 void libsupermesh_tetrahedron_volume(double* tet, double* volume) {
   guarded([&] { one_call(3, tet, 12, volume, 1); });
+}
+
+void libsupermesh_intersect_tri_quad(double* tri_a, double* quad_b, double* tris_c, int* n_tris_c) {
+  int rc = guarded([&] {
+    double in[14], res[1 + 6 * kTriQuadBufSize];
+    std::memcpy(in, tri_a, sizeof(double) * 6);
+    std::memcpy(in + 6, quad_b, sizeof(double) * 8);
+    one_call(4, in, 14, res, 1 + 6 * kTriQuadBufSize);
+    int n = (int)res[0];
+    *n_tris_c = n;
+    std::memcpy(tris_c, res + 1, sizeof(double) * 6 * n);
+  });
+  if (rc != LIBSUPERMESH_GPU_OK) *n_tris_c = 0;
+}
+
+void libsupermesh_intersect_tet_hex(double* tet_a, double* hex_b, double* tets_c, int* n_tets_c) {
+  int rc = guarded([&] {
+    double in[36];
+    std::vector<double> res(1 + 12 * kTetHexBufSize);
+    std::memcpy(in, tet_a, sizeof(double) * 12);
+    std::memcpy(in + 12, hex_b, sizeof(double) * 24);
+    one_call(5, in, 36, res.data(), 1 + 12 * kTetHexBufSize);
+    int n = (int)res[0];
+    *n_tets_c = n;
+    std::memcpy(tets_c, res.data() + 1, sizeof(double) * 12 * n);
+  });
+  if (rc != LIBSUPERMESH_GPU_OK) *n_tets_c = 0;
+}
+
+void libsupermesh_interval_size(double* interval, double* size) {
+  guarded([&] { *size = lsm::interval_size(interval); });
+}
+
+void libsupermesh_intersect_intervals(double* interval_a, double* interval_b, double* intervals_c,
+                                      int* n_intervals_c) {
+  int rc = guarded([&] { *n_intervals_c = lsm::intersect_intervals(interval_a, interval_b, intervals_c); });
+  if (rc != LIBSUPERMESH_GPU_OK) *n_intervals_c = 0;
+}
+
+void libsupermesh_sort_intersection_finder_set_input(long* nnodes_a, int* dim_a, long* nelements_a,
+                                                     int* loc_a, long* nnodes_b, int* dim_b,
+                                                     long* nelements_b, int* loc_b, double* positions_a,
+                                                     long* enlist_a, double* positions_b, long* enlist_b) {
+  guarded([&] {
+    std::lock_guard<std::mutex> lk(g_sort.mu);
+    g_sort.ready = false;
+    g_sort.indptr.clear();
+    g_sort.indices.clear();
+    LSM_REQUIRE(*dim_a == 1 && *dim_b == 1, LIBSUPERMESH_GPU_ERROR_INVALID, "Invalid dimension");
+    LSM_REQUIRE(*loc_a == 2 && *loc_b == 2, LIBSUPERMESH_GPU_ERROR_INVALID,
+                "Unsupported element type: intervals have 2 vertices");
+    lsm::sort_finder(*nnodes_a, *nelements_a, positions_a, enlist_a, *nnodes_b, *nelements_b,
+                     positions_b, enlist_b, g_sort.indptr, g_sort.indices);
+    g_sort.nelements_a = *nelements_a;
+    g_sort.ready = true;
+  });
+}
+
+void libsupermesh_sort_intersection_finder_query_output(long* nindices) {
+  guarded([&] { finder_query_output(g_sort, nindices); });
+}
+
+void libsupermesh_sort_intersection_finder_get_output(long* nelements, long* nindices, long* indices,
+                                                      long* ind_ptr) {
+  guarded([&] { finder_get_output(g_sort, *nelements, *nindices, indices, ind_ptr); });
 }
 
 }  // extern "C"

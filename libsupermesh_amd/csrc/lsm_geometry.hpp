@@ -17,6 +17,8 @@ namespace lsm {
 
 constexpr int kTriBufSize = 22;  // TRI_BUF_SIZE, include/libsupermesh-c.h:3
 constexpr int kTetBufSize = 81;  // TET_BUF_SIZE, include/libsupermesh-c.h:4
+constexpr int kTriQuadBufSize = 46;  // tris_c(2, 3, 46), C_bindings.F90:59
+constexpr int kTetHexBufSize = 729;  // tets_c(3, 4, 729), C_bindings.F90:47
 
 // ---------------------------------------------------------------------------
 // Fortran intrinsics

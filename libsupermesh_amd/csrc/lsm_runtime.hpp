@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <mutex>
+#include <vector>
 #include <stdexcept>
 #include <string>
 
@@ -140,5 +141,12 @@ inline unsigned grid_for(long n, int block) {
   if (g > 0x7fffffffL) g = 0x7fffffffL;
   return (unsigned)g;
 }
+
+// 1D entry points (interval.cpp), host code.
+double interval_size(const double* v);
+int intersect_intervals(const double* a, const double* b, double* c);
+void sort_finder(long nnodes_a, long ne_a, const double* pos_a, const long* enl_a, long nnodes_b,
+                 long ne_b, const double* pos_b, const long* enl_b, std::vector<long>& indptr,
+                 std::vector<int>& indices);
 
 }  // namespace lsm

@@ -188,6 +188,51 @@ def rtree_intersection_finder(pos_a, enl_a, pos_b, enl_b):
     return _finder("libsupermesh_rtree_intersection_finder", pos_a, enl_a, pos_b, enl_b)
 
 
+def sort_intersection_finder(pos_a, enl_a, pos_b, enl_b):
+    """1D (intervals) candidate pairs of the reference's sort finder, as CSR.
+    Host code in the library: no GPU is touched."""
+    return _finder("libsupermesh_sort_intersection_finder", pos_a, enl_a, pos_b, enl_b)
+
+
+def _pair_call(fn, a, b, shape):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    out = np.zeros(shape)
+    n = c_int(0)
+    fn(_pd(a), _pd(b), _pd(out), ctypes.byref(n))
+    _raise_if_error()
+    return out[:n.value].copy()
+
+
+def intersect_tri_quad(tri_a, quad_b):
+    """Triangles of tri_a ^ quad_b, ``float64[n, 3, 2]`` (at most 46)."""
+    return _pair_call(lib().libsupermesh_intersect_tri_quad, tri_a, quad_b, (46, 3, 2))
+
+
+def intersect_tet_hex(tet_a, hex_b):
+    """Tetrahedra of tet_a ^ hex_b (Gmsh vertex order), ``float64[n, 4, 3]`` (at most 729)."""
+    return _pair_call(lib().libsupermesh_intersect_tet_hex, tet_a, hex_b, (729, 4, 3))
+
+
+def intersect_intervals(interval_a, interval_b):
+    """``(n, interval)``: n is 0 or 1.  Host code in the library."""
+    a = np.ascontiguousarray(interval_a, dtype=np.float64).reshape(2)
+    b = np.ascontiguousarray(interval_b, dtype=np.float64).reshape(2)
+    out = np.zeros(2)
+    n = c_int(0)
+    lib().libsupermesh_intersect_intervals(_pd(a), _pd(b), _pd(out), ctypes.byref(n))
+    _raise_if_error()
+    return n.value, out
+
+
+def interval_size(interval):
+    v = np.ascontiguousarray(interval, dtype=np.float64).reshape(2)
+    r = c_double(0.0)
+    lib().libsupermesh_interval_size(_pd(v), ctypes.byref(r))
+    _raise_if_error()
+    return r.value
+
+
 def intersect_tris_real(tri_a, tri_b):
     """Triangles of tri_a ^ tri_b, ``float64[n, 3, 2]``."""
     a = np.ascontiguousarray(tri_a, dtype=np.float64)

@@ -20,6 +20,14 @@
  *   orc_intersect_tets            Tetrahedron_Intersection.F90:137-179 (BFS
  *                                 driver), :244-436 (clip_buf),
  *                                 :634-667 (get_planes_tet), :679-688
+ *   orc_intersect_tri_quad,       Supermesh.F90:211-334, get_lines_poly
+ *   orc_intersect_quads,          Triangle_Intersection.F90:422-451,
+ *   orc_intersect_tet_hex,        intersect_polys_lines :216-276,
+ *   orc_intersect_hexes           intersect_tet_planes
+ *                                 Tetrahedron_Intersection.F90:189-241
+ *   orc_intersect_intervals,      Interval_Intersection.F90:45-110
+ *   orc_interval_size
+ *   orc_sort_finder               Intersection_Finder.F90:811-921
  *   orc_ksum_*                    Precision.F90:72-113
  *   orc_brute_force_finder        Intersection_Finder.F90:990-1050 (strict);
  *                                 closed variant = R-tree predicate,
@@ -442,6 +450,247 @@ int orc_intersect_tets(const double tet_a[12], const double tet_b[12], double *t
     }
   }
   return n;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Mixed element types (Supermesh.F90:211-334)                               */
+/* ------------------------------------------------------------------------ */
+
+/* get_lines_poly, Triangle_Intersection.F90:422-451: normals (dy, -dx), all
+ * flipped when vertex 1 lies on the positive side of line 2. */
+static void get_lines_poly(const double *q, int loc, orc_line *L) {
+  for (int k = 0; k < loc; k++) {
+    int k1 = (k + 1) % loc;
+    L[k].n[0] = (q[2 * k1 + 1] - q[2 * k + 1]);
+    L[k].n[1] = -(q[2 * k1] - q[2 * k]);
+    L[k].p[0] = q[2 * k];
+    L[k].p[1] = q[2 * k + 1];
+  }
+  double d = (q[0] - L[1].p[0]) * L[1].n[0];
+  d = d + (q[1] - L[1].p[1]) * L[1].n[1];
+  if (d > 0.0) {
+    for (int k = 0; k < loc; k++) {
+      L[k].n[0] = -L[k].n[0];
+      L[k].n[1] = -L[k].n[1];
+    }
+  }
+}
+
+/* intersect_polys_lines, Triangle_Intersection.F90:216-276: a polygon of
+ * na points clipped by nl lines, then fanned; tol as passed. */
+static int polys_lines(const double *poly, int na, const orc_line *L, int nl, double tol,
+                       double *tris_c) {
+  double buf0[2 * 80], buf1[2 * 80];
+  memcpy(buf0, poly, (size_t)na * 2 * sizeof(double));
+  int n = na;
+  double *cur = buf0, *nxt = buf1;
+  for (int i = 0; i < nl; i++) {
+    n = clip_line(&L[i], cur, n, nxt);
+    if (n < 3) return 0;
+    double *t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  int nc = 0;
+  for (int i = 0; i < n - 2; i++) {
+    double *t = tris_c + 6 * nc;
+    t[0] = cur[0];
+    t[1] = cur[1];
+    t[2] = cur[2 * (i + 1)];
+    t[3] = cur[2 * (i + 1) + 1];
+    t[4] = cur[2 * (i + 2)];
+    t[5] = cur[2 * (i + 2) + 1];
+    if (!(orc_triangle_area(t) < tol)) nc++;
+  }
+  return nc;
+}
+
+/* intersect_tri_quad (Supermesh.F90:211-229) -> intersect_tri_lines with no
+ * area for B: tol = 10*spacing(area(A)). */
+int orc_intersect_tri_quad(const double tri_a[6], const double quad_b[8], double *tris_c) {
+  orc_line L[4];
+  get_lines_poly(quad_b, 4, L);
+  return polys_lines(tri_a, 3, L, 4, 10.0 * orc_spacing(orc_triangle_area(tri_a)), tris_c);
+}
+
+/* intersect_quads (Supermesh.F90:231-248) -> intersect_polys_real with no
+ * areas: tol = 10*tiny. */
+int orc_intersect_quads(const double quad_a[8], const double quad_b[8], double *tris_c) {
+  orc_line L[4];
+  get_lines_poly(quad_b, 4, L);
+  return polys_lines(quad_a, 4, L, 4, 10.0 * DBL_MIN, tris_c);
+}
+
+void orc_hex_planes(const double hex[24], double nrm[18], double c[6]) {
+  /* plane f: unit_cross(h[a1] - h[a0], h[b1] - h[a0]), c = h[a0] . n */
+  static const int a0[6] = {0, 4, 5, 1, 0, 7}, a1[6] = {1, 5, 1, 0, 4, 6},
+                   b1[6] = {5, 6, 2, 3, 7, 2};
+  for (int f = 0; f < 6; f++) {
+    double ea[3], eb[3];
+    for (int i = 0; i < 3; i++) {
+      ea[i] = hex[3 * a1[f] + i] - hex[3 * a0[f] + i];
+      eb[i] = hex[3 * b1[f] + i] - hex[3 * a0[f] + i];
+    }
+    unit_cross(ea, eb, nrm + 3 * f);
+    c[f] = orc_dot3(hex + 3 * a0[f], nrm + 3 * f);
+  }
+}
+
+/* intersect_tet_hex (Supermesh.F90:250-301) -> intersect_tet_planes
+ * (Tetrahedron_Intersection.F90:189-241): 6 planes breadth first, keep
+ * vol_tet >= 10*spacing(vol_tet(A)). */
+int orc_intersect_tet_hex(const double tet_a[12], const double hex_b[24], double *tets_c) {
+  double nrm[18], c[6];
+  orc_hex_planes(hex_b, nrm, c);
+  double tol = 10.0 * orc_spacing(orc_tetrahedron_volume_tet(tet_a));
+  static __thread double cur[ORC_TET_HEX_BUF * 12], nxt[ORC_TET_HEX_BUF * 12];
+  memcpy(cur, tet_a, 12 * sizeof(double));
+  int n = 1;
+  for (int i = 0; i < 6; i++) {
+    int m = 0;
+    for (int j = 0; j < n; j++) m = clip_tet_plane(nrm + 3 * i, c[i], cur + 12 * j, nxt, m);
+    if (i != 5) {
+      memcpy(cur, nxt, (size_t)m * 12 * sizeof(double));
+      n = m;
+    } else {
+      n = 0;
+      for (int j = 0; j < m; j++) {
+        if (orc_tetrahedron_volume_tet(nxt + 12 * j) >= tol) {
+          memcpy(tets_c + 12 * n, nxt + 12 * j, 12 * sizeof(double));
+          n++;
+        }
+      }
+    }
+  }
+  return n;
+}
+
+/* intersect_hexes (Supermesh.F90:303-334): A split into 5 tets (Gmsh
+ * vertices 3678, 1348, 1568, 1236, 1368), each clipped by B's planes. */
+int orc_intersect_hexes(const double hex_a[24], const double hex_b[24], double *tets_c) {
+  static const int split[5][4] = {{2, 5, 6, 7}, {0, 2, 3, 7}, {0, 4, 5, 7}, {0, 1, 2, 5}, {0, 2, 5, 7}};
+  int n = 0;
+  for (int s = 0; s < 5; s++) {
+    double t[12];
+    for (int v = 0; v < 4; v++) memcpy(t + 3 * v, hex_a + 3 * split[s][v], 3 * sizeof(double));
+    n += orc_intersect_tet_hex(t, hex_b, tets_c + 12 * n);
+  }
+  return n;
+}
+
+/* ------------------------------------------------------------------------ */
+/* 1D (Interval_Intersection.F90:45-110, Intersection_Finder.F90:811-921)    */
+/* ------------------------------------------------------------------------ */
+
+double orc_interval_size(const double v[2]) { return fabs(v[1] - v[0]); }
+
+/* Fortran MIN/MAX of two values: first argument on ties. */
+static double fmin2(double a, double b) { return b < a ? b : a; }
+static double fmax2(double a, double b) { return b > a ? b : a; }
+
+int orc_intersect_intervals(const double a[2], const double b[2], double c[2]) {
+  double min_a = fmin2(a[0], a[1]), max_a = fmax2(a[0], a[1]);
+  double min_b = fmin2(b[0], b[1]), max_b = fmax2(b[0], b[1]);
+  if (max_b <= min_a || min_b >= max_a) return 0;
+  c[0] = fmax2(min_a, min_b);
+  c[1] = fmin2(max_a, max_b);
+  double tol = 10.0 * fmin2(orc_spacing(orc_interval_size(a)), orc_spacing(orc_interval_size(b)));
+  return orc_interval_size(c) < tol ? 0 : 1;
+}
+
+/* Stable sort of left end points, ties by element number.  The reference
+ * (Intersection_Finder.F90:869-917) merge-sorts with a base case for blocks
+ * of at most 4 entries that gives slot i the first strict minimum of
+ * v(i..n): the identity when the block is sorted, and otherwise an index list
+ * with repeats, after which elements go unvisited and the reference reads an
+ * undefined row count.  On every input where its result is defined, it equals
+ * this stable sort. */
+static void sort_merge(const double *v, int *idx, int *work, long n) {
+  if (n <= 4) {
+    for (long i = 0; i < n; i++) {
+      long k = i;
+      idx[i] = (int)i;
+      while (k > 0 && v[idx[k - 1]] > v[idx[k]]) {
+        int t = idx[k - 1];
+        idx[k - 1] = idx[k];
+        idx[k] = t;
+        k--;
+      }
+    }
+    return;
+  }
+  long h = n / 2;
+  sort_merge(v, idx, work, h);
+  sort_merge(v + h, idx + h, work + h, n - h);
+  for (long i = 0; i < h; i++) work[i] = idx[i];
+  for (long i = h; i < n; i++) work[i] = idx[i] + (int)h;
+  long i1 = 0, i2 = h;
+  for (long i = 0; i < n; i++) {
+    if (i1 >= h) {
+      for (; i < n; i++) idx[i] = work[i2++];
+      break;
+    } else if (i2 >= n) {
+      for (; i < n; i++) idx[i] = work[i1++];
+      break;
+    } else if (v[work[i1]] <= v[work[i2]]) {
+      idx[i] = work[i1++];
+    } else {
+      idx[i] = work[i2++];
+    }
+  }
+}
+
+long orc_sort_finder(long ne_a, const double *pos_a, const int *enl_a,
+                     long ne_b, const double *pos_b, const int *enl_b,
+                     long *indptr, int *indices, long cap) {
+  double *left_a = malloc(sizeof(double) * (ne_a + 1)), *left_b = malloc(sizeof(double) * (ne_b + 1));
+  int *ia = malloc(sizeof(int) * (ne_a + 1)), *ib = malloc(sizeof(int) * (ne_b + 1));
+  int *work = malloc(sizeof(int) * (ne_a + ne_b + 1));
+  long *rows = malloc(sizeof(long) * (ne_a + 1));
+  long *first = malloc(sizeof(long) * (ne_a + 1));
+  for (long e = 0; e < ne_a; e++) left_a[e] = fmin2(pos_a[enl_a[2 * e]], pos_a[enl_a[2 * e + 1]]);
+  for (long e = 0; e < ne_b; e++) left_b[e] = fmin2(pos_b[enl_b[2 * e]], pos_b[enl_b[2 * e + 1]]);
+  sort_merge(left_a, ia, work, ne_a);
+  sort_merge(left_b, ib, work, ne_b);
+  /* pass 1: per A element (in sorted order) the B ids, stored contiguously in
+   * visiting order; rows[e] = count, first[e] = offset in the visit list */
+  int *visit = malloc(sizeof(int) * (ne_a + 1) * 4 + sizeof(int) * (ne_b + 1));
+  long nvisit = 0, vcap = (ne_a + 1) * 4 + ne_b + 1;
+  long j = 0;
+  for (long i = 0; i < ne_a; i++) {
+    long ea = ia[i];
+    double a0 = pos_a[enl_a[2 * ea]], a1 = pos_a[enl_a[2 * ea + 1]];
+    double min_a = fmin2(a0, a1), max_a = fmax2(a0, a1);
+    long n = 0;
+    first[ea] = nvisit;
+    while (j < ne_b) {
+      long eb = ib[j];
+      double b0 = pos_b[enl_b[2 * eb]], b1 = pos_b[enl_b[2 * eb + 1]];
+      if (fmax2(b0, b1) <= min_a) {
+        j++;
+      } else if (fmin2(b0, b1) >= max_a) {
+        break;
+      } else {
+        if (nvisit == vcap) {
+          vcap *= 2;
+          visit = realloc(visit, sizeof(int) * vcap);
+        }
+        visit[nvisit++] = (int)eb;
+        n++;
+        j++;
+      }
+    }
+    if (n > 0) j--;
+    rows[ea] = n;
+  }
+  indptr[0] = 0;
+  for (long e = 0; e < ne_a; e++) indptr[e + 1] = indptr[e] + rows[e];
+  if (indices)
+    for (long e = 0; e < ne_a; e++)
+      for (long k = 0; k < rows[e] && indptr[e] + k < cap; k++) indices[indptr[e] + k] = visit[first[e] + k];
+  long total = indptr[ne_a];
+  free(left_a); free(left_b); free(ia); free(ib); free(work); free(rows); free(first); free(visit);
+  return total;
 }
 
 /* ------------------------------------------------------------------------ */

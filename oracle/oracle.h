@@ -53,6 +53,26 @@ int orc_intersect_tets(const double tet_a[12], const double tet_b[12], double *t
 /* Plane set of a tet, as get_planes_tet: normals[4][3], c[4]. */
 void orc_get_planes_tet(const double tet[12], double normals[12], double c[4]);
 
+/* Mixed element types and 1D (Supermesh.F90:211-334, Interval_Intersection.F90:45-110).
+ * Buffers: tri_quad 46 triangles, quads 62, tet_hex 729 tets, hexes 5*729. */
+#define ORC_TRI_QUAD_BUF 46
+#define ORC_QUADS_BUF 62
+#define ORC_TET_HEX_BUF 729
+int orc_intersect_tri_quad(const double tri_a[6], const double quad_b[8], double *tris_c);
+int orc_intersect_quads(const double quad_a[8], const double quad_b[8], double *tris_c);
+int orc_intersect_tet_hex(const double tet_a[12], const double hex_b[24], double *tets_c);
+int orc_intersect_hexes(const double hex_a[24], const double hex_b[24], double *tets_c);
+/* Hex face planes in Gmsh order, as intersect_tet_hex builds them: normals[6][3], c[6]. */
+void orc_hex_planes(const double hex[24], double normals[18], double c[6]);
+double orc_interval_size(const double interval[2]);
+int orc_intersect_intervals(const double a[2], const double b[2], double c[2]);
+/* 1D sort finder (Intersection_Finder.F90:811-921), enlists 0-based, loc 2.
+ * Fills indptr[ne_a+1] and, when indices is non-NULL, indices (capacity cap);
+ * returns the total count. */
+long orc_sort_finder(long ne_a, const double *pos_a, const int *enl_a,
+                     long ne_b, const double *pos_b, const int *enl_b,
+                     long *indptr, int *indices, long cap);
+
 /* Kahan compensated sum (src/Precision.F90:45-113). */
 typedef struct { double sum, error; } orc_ksum;
 void orc_ksum_init(orc_ksum *cs);

@@ -20,7 +20,7 @@ from libsupermesh_amd.triangle_io import read_mesh  # noqa: E402
 
 REF_DATA = "/root/reference/src/tests/data"
 
-SMALL = [("triangle.1", 2), ("triangle.2", 2), ("square.1", 2), ("square.2", 2),
+SMALL = [("line.1", 1), ("line.2", 1), ("triangle.1", 2), ("triangle.2", 2), ("square.1", 2), ("square.2", 2),
          ("square.3", 2), ("square.4", 2), ("cube.1", 3), ("cube.2", 3),
          ("cube.3", 3), ("cube.4", 3), ("tet", 3)]
 LARGE = [("2D_0_01.tar.bz2", [("triangle_0_01", 2), ("square_0_01", 2)]),

@@ -63,6 +63,16 @@ def lib():
         L.orc_supermesh.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_long, ctypes.c_long, _dp, _ip,
                                     ctypes.c_long, ctypes.c_long, _dp, _ip, _llp, _dp, _dp]
+        for name in ("orc_intersect_tri_quad", "orc_intersect_quads", "orc_intersect_tet_hex",
+                     "orc_intersect_hexes", "orc_intersect_intervals"):
+            getattr(L, name).restype = ctypes.c_int
+            getattr(L, name).argtypes = [_dp, _dp, _dp]
+        L.orc_hex_planes.argtypes = [_dp, _dp, _dp]
+        L.orc_interval_size.restype = ctypes.c_double
+        L.orc_interval_size.argtypes = [_dp]
+        L.orc_sort_finder.restype = ctypes.c_long
+        L.orc_sort_finder.argtypes = [ctypes.c_long, _dp, _ip, ctypes.c_long, _dp, _ip, _lp, _ip,
+                                      ctypes.c_long]
         _lib = L
     return _lib
 
@@ -124,6 +134,64 @@ def intersect_tets(a, b):
     out = np.zeros((81, 4, 3))
     n = lib().orc_intersect_tets(_d(a), _d(b), _d(out))
     return out[:n].copy()
+
+
+def _mixed(fn, a, b, shape):
+    a, b = _f64(a), _f64(b)
+    out = np.zeros(shape)
+    n = fn(_d(a), _d(b), _d(out))
+    return out[:n].copy()
+
+
+def intersect_tri_quad(tri, quad):
+    return _mixed(lib().orc_intersect_tri_quad, tri, quad, (46, 3, 2))
+
+
+def intersect_quads(qa, qb):
+    return _mixed(lib().orc_intersect_quads, qa, qb, (62, 3, 2))
+
+
+def intersect_tet_hex(tet, hex_):
+    return _mixed(lib().orc_intersect_tet_hex, tet, hex_, (729, 4, 3))
+
+
+def intersect_hexes(ha, hb):
+    return _mixed(lib().orc_intersect_hexes, ha, hb, (5 * 729, 4, 3))
+
+
+def hex_planes(h):
+    h = _f64(h)
+    n = np.zeros((6, 3))
+    c = np.zeros(6)
+    lib().orc_hex_planes(_d(h), _d(n), _d(c))
+    return n, c
+
+
+def interval_size(v):
+    v = _f64(v)
+    return lib().orc_interval_size(_d(v))
+
+
+def intersect_intervals(a, b):
+    """(n, interval) as intersect_intervals: n is 0 or 1."""
+    a, b = _f64(a), _f64(b)
+    c = np.zeros(2)
+    n = lib().orc_intersect_intervals(_d(a), _d(b), _d(c))
+    return n, c
+
+
+def sort_finder(pos_a, enl_a, pos_b, enl_b):
+    pos_a, pos_b = _f64(pos_a).reshape(-1), _f64(pos_b).reshape(-1)
+    enl_a, enl_b = _i32(enl_a), _i32(enl_b)
+    ne_a, ne_b = enl_a.shape[0], enl_b.shape[0]
+    indptr = np.zeros(ne_a + 1, dtype=np.int64)
+    L = lib()
+    n = L.orc_sort_finder(ne_a, _d(pos_a), _i(enl_a), ne_b, _d(pos_b), _i(enl_b),
+                          indptr.ctypes.data_as(_lp), None, 0)
+    indices = np.zeros(max(n, 1), dtype=np.int32)
+    L.orc_sort_finder(ne_a, _d(pos_a), _i(enl_a), ne_b, _d(pos_b), _i(enl_b),
+                      indptr.ctypes.data_as(_lp), _i(indices), n)
+    return indptr, indices[:n]
 
 
 def get_planes_tet(t):

@@ -28,11 +28,17 @@ def declared_functions():
 
 def test_headers_declare_the_reference_symbols():
     names = declared_functions()
-    for ref in ["libsupermesh_tree_intersection_finder_set_input",
+    # every function of the reference's include/libsupermesh-c.h:18-31
+    for ref in ["libsupermesh_sort_intersection_finder_set_input",
+                "libsupermesh_sort_intersection_finder_query_output",
+                "libsupermesh_sort_intersection_finder_get_output",
+                "libsupermesh_tree_intersection_finder_set_input",
                 "libsupermesh_tree_intersection_finder_query_output",
                 "libsupermesh_tree_intersection_finder_get_output",
                 "libsupermesh_intersect_tris_real", "libsupermesh_intersect_tets_real",
-                "libsupermesh_triangle_area", "libsupermesh_tetrahedron_volume"]:
+                "libsupermesh_triangle_area", "libsupermesh_tetrahedron_volume",
+                "libsupermesh_interval_size", "libsupermesh_intersect_intervals",
+                "libsupermesh_intersect_tet_hex", "libsupermesh_intersect_tri_quad"]:
         assert ref in names
 
 
