@@ -117,6 +117,25 @@ int libsupermesh_gpu_supermesh(libsupermesh_gpu_context* ctx, const libsupermesh
                                const libsupermesh_gpu_mesh* b, int finder, int flags,
                                libsupermesh_gpu_result* out);
 
+/* Callback driver: the single-GPU form of the self-self loop of the
+ * reference's parallel_supermesh (src/Parallel_Supermesh.F90:436-453, its
+ * intersection_calculation interface :97-113; the MPI exchange is left out).
+ * Candidates use closed overlap (R-tree semantics).  For each B element in
+ * order, and each of its A candidates in ascending id, the callback receives
+ * host pointers: element_a[dim+1][dim], element_b[dim+1][dim], the clipped
+ * elements_c[n_elements_c][dim+1][dim] (n_elements_c may be 0), nodes_b[dim+1]
+ * (B's 0-based node ids), and the 0-based element ids.  The pointers are valid
+ * only during the call.  Clipping runs on the device in bulk; the callbacks run
+ * on the calling thread. */
+typedef void (*libsupermesh_gpu_pair_callback)(const double* element_a, const double* element_b,
+                                               const double* elements_c, int n_elements_c,
+                                               const long* nodes_b, long ele_a, long ele_b,
+                                               void* user);
+int libsupermesh_gpu_supermesh_callback(libsupermesh_gpu_context* ctx,
+                                        const libsupermesh_gpu_mesh* a,
+                                        const libsupermesh_gpu_mesh* b,
+                                        libsupermesh_gpu_pair_callback callback, void* user);
+
 /* Clip an explicit list of pairs (device arrays pair_a[n], pair_b[n]);
  * fills counts / measure (and elements, blocks per flags; no mass CSR). */
 int libsupermesh_gpu_clip_pairs(libsupermesh_gpu_context* ctx, const libsupermesh_gpu_mesh* a,

@@ -9,6 +9,7 @@
 // get_output; the same sequence is kept, guarded by a mutex.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -169,6 +170,75 @@ static void run_supermesh(Context& C, const libsupermesh_gpu_mesh* a, const libs
     out->ms[6] = ph.between(0, ph.n - 1);
   }
   C.timing = false;
+}
+
+// Single-GPU form of the self-self loop of parallel_supermesh
+// (Parallel_Supermesh.F90:436-453): closed-overlap candidates with A as the
+// searched mesh, B elements in order, the A candidates of each in ascending
+// id; every candidate pair, empty or not, is handed to the callback with
+// the clipped elements of intersect_elements(A, B).  Pairs are clipped on the
+// device in chunks and the callbacks run on the calling thread.
+static void run_callback_driver(Context& C, const libsupermesh_gpu_mesh* a,
+                                const libsupermesh_gpu_mesh* b, libsupermesh_gpu_pair_callback cb,
+                                void* user) {
+  libsupermesh_gpu_result r;
+  run_supermesh(C, b, a, LIBSUPERMESH_GPU_FINDER_RTREE, LIBSUPERMESH_GPU_SORTED, false, &r);
+  const long npairs = r.npairs;
+  const int dim = a->dim, loc = dim + 1, per = loc * dim;
+  // host copies of both meshes, for the element coordinates and nodes_b
+  std::vector<double> pos_a((size_t)a->nnodes * dim), pos_b((size_t)b->nnodes * dim);
+  std::vector<int> enl_a((size_t)a->nelements * loc), enl_b((size_t)b->nelements * loc);
+  auto d2h = [&](void* dst, const void* src, size_t bytes) {
+    if (bytes) LSM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, C.stream));
+  };
+  d2h(pos_a.data(), a->positions, sizeof(double) * pos_a.size());
+  d2h(pos_b.data(), b->positions, sizeof(double) * pos_b.size());
+  d2h(enl_a.data(), a->enlist, sizeof(int) * enl_a.size());
+  d2h(enl_b.data(), b->enlist, sizeof(int) * enl_b.size());
+  LSM_HIP(hipStreamSynchronize(C.stream));
+  // pair lists: B element (CSR row) and A element (CSR entry) of each pair
+  const int* d_pair_b = r.rows;
+  const int* d_pair_a = r.indices;
+  stage_mesh(C, C.mesh_a, dim, loc, a->nnodes, a->nelements, a->positions, a->enlist, false, false);
+  stage_mesh(C, C.mesh_b, dim, loc, b->nnodes, b->nelements, b->positions, b->enlist, true, false);
+  constexpr long kChunk = 1L << 21;
+  std::vector<int> pa, pb, counts;
+  std::vector<long> offs;
+  std::vector<double> elems;
+  double ea[4 * 3], eb[4 * 3];
+  long nodes_b[4];
+  for (long p0 = 0; p0 < npairs; p0 += kChunk) {
+    const long n = std::min(kChunk, npairs - p0);
+    libsupermesh_gpu_result cr;
+    std::memset(&cr, 0, sizeof(cr));
+    cr.npairs = n;
+    cr.rows = d_pair_a + p0;
+    cr.indices = d_pair_b + p0;
+    clip_pairs(C, C.mesh_a, C.mesh_b, n, LIBSUPERMESH_GPU_ELEMENTS, &cr);
+    pa.resize(n);
+    pb.resize(n);
+    counts.resize(n);
+    offs.resize(n + 1);
+    elems.resize((size_t)std::max(cr.nout, 1L) * per);
+    d2h(pa.data(), cr.rows, sizeof(int) * n);
+    d2h(pb.data(), cr.indices, sizeof(int) * n);
+    d2h(counts.data(), cr.counts, sizeof(int) * n);
+    d2h(offs.data(), cr.offsets, sizeof(long) * (n + 1));
+    d2h(elems.data(), cr.elements, sizeof(double) * (size_t)cr.nout * per);
+    LSM_HIP(hipStreamSynchronize(C.stream));
+    for (long i = 0; i < n; i++) {
+      const long ia = pa[i], ib = pb[i];
+      for (int v = 0; v < loc; v++) {
+        const long na = enl_a[(size_t)ia * loc + v], nb = enl_b[(size_t)ib * loc + v];
+        nodes_b[v] = nb;
+        for (int d = 0; d < dim; d++) {
+          ea[v * dim + d] = pos_a[(size_t)na * dim + d];
+          eb[v * dim + d] = pos_b[(size_t)nb * dim + d];
+        }
+      }
+      cb(ea, eb, elems.data() + (size_t)offs[i] * per, counts[i], nodes_b, ia, ib, user);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -435,6 +505,18 @@ int libsupermesh_gpu_supermesh(libsupermesh_gpu_context* ctx, const libsupermesh
     LSM_REQUIRE(ctx != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID, "null context");
     std::lock_guard<std::mutex> lk(ctx->ctx.mu);
     run_supermesh(ctx->ctx, a, b, finder, flags, true, out);
+  });
+}
+
+int libsupermesh_gpu_supermesh_callback(libsupermesh_gpu_context* ctx, const libsupermesh_gpu_mesh* a,
+                                        const libsupermesh_gpu_mesh* b,
+                                        libsupermesh_gpu_pair_callback callback, void* user) {
+  return guarded([&] {
+    LSM_REQUIRE(ctx != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID, "null context");
+    LSM_REQUIRE(callback != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID, "null callback");
+    Context& C = ctx->ctx;
+    std::lock_guard<std::mutex> lk(C.mu);
+    run_callback_driver(C, a, b, callback, user);
   });
 }
 

@@ -12,7 +12,7 @@
 module libsupermesh
 
   use iso_c_binding, only : c_double, c_int, c_long, c_char, c_ptr, c_null_char, &
-    & c_f_pointer, c_associated
+    & c_f_pointer, c_associated, c_loc, c_funloc, c_funptr, c_null_ptr, c_size_t
 
   implicit none
 
@@ -22,9 +22,37 @@ module libsupermesh
   integer, parameter, public :: tri_buf_size = 22
   integer, parameter, public :: tet_buf_size = 81
 
+  integer, parameter, public :: interval_buf_size = 1
+
   public :: tree_intersection_finder, rtree_intersection_finder, &
-    & intersect_tris, intersect_tets, triangle_area, tetrahedron_volume, &
-    & libsupermesh_gpu_error
+    & sort_intersection_finder, intersect_tris, intersect_tets, &
+    & triangle_area, tetrahedron_volume, intersect_intervals, interval_size, &
+    & intersect_tri_quad, intersect_tet_hex, supermesh, libsupermesh_gpu_error
+
+  ! The per-pair callback of supermesh: the interface of the reference's
+  ! intersection_calculation (Parallel_Supermesh.F90:97-113).
+  abstract interface
+    subroutine intersection_calculation_interface(element_a, element_b, elements_c, nodes_b, &
+        & ele_a, ele_b, local)
+      import :: c_double
+      real(kind = c_double), dimension(:, :), intent(in) :: element_a, element_b
+      real(kind = c_double), dimension(:, :, :), intent(in) :: elements_c
+      integer, dimension(:), intent(in) :: nodes_b
+      integer, intent(in) :: ele_a, ele_b
+      logical, intent(in) :: local
+    end subroutine intersection_calculation_interface
+  end interface
+
+  ! libsupermesh_gpu_mesh (include/libsupermesh_gpu.h)
+  type, bind(c) :: gpu_mesh
+    integer(kind = c_int) :: dim
+    integer(kind = c_long) :: nnodes, nelements
+    type(c_ptr) :: positions, enlist
+  end type gpu_mesh
+
+  procedure(intersection_calculation_interface), pointer, save :: user_calculation => null()
+  integer, save :: callback_dim = 0
+  type(c_ptr), save :: gpu_context = c_null_ptr
 
   interface tree_intersection_finder
     module procedure tree_intersection_finder_csr_sparsity
@@ -41,6 +69,10 @@ module libsupermesh
   interface intersect_tets
     module procedure intersect_tets_real
   end interface intersect_tets
+
+  interface sort_intersection_finder
+    module procedure sort_intersection_finder_csr_sparsity
+  end interface sort_intersection_finder
 
   interface
     subroutine c_tree_set_input(nnodes_a, dim_a, nelements_a, loc_a, nnodes_b, dim_b, &
@@ -105,6 +137,83 @@ module libsupermesh
       real(kind = c_double), dimension(*) :: tet
       real(kind = c_double) :: volume
     end subroutine c_tetrahedron_volume
+    subroutine c_sort_set_input(nnodes_a, dim_a, nelements_a, loc_a, nnodes_b, dim_b, &
+        & nelements_b, loc_b, positions_a, enlist_a, positions_b, enlist_b) &
+        & bind(c, name = "libsupermesh_sort_intersection_finder_set_input")
+      import :: c_long, c_int, c_double
+      integer(kind = c_long) :: nnodes_a, nelements_a, nnodes_b, nelements_b
+      integer(kind = c_int) :: dim_a, loc_a, dim_b, loc_b
+      real(kind = c_double), dimension(*) :: positions_a, positions_b
+      integer(kind = c_long), dimension(*) :: enlist_a, enlist_b
+    end subroutine c_sort_set_input
+    subroutine c_sort_query_output(nindices) &
+        & bind(c, name = "libsupermesh_sort_intersection_finder_query_output")
+      import :: c_long
+      integer(kind = c_long) :: nindices
+    end subroutine c_sort_query_output
+    subroutine c_sort_get_output(nelements, nindices, indices, ind_ptr) &
+        & bind(c, name = "libsupermesh_sort_intersection_finder_get_output")
+      import :: c_long
+      integer(kind = c_long) :: nelements, nindices
+      integer(kind = c_long), dimension(*) :: indices, ind_ptr
+    end subroutine c_sort_get_output
+    subroutine c_intersect_tri_quad(tri_a, quad_b, tris_c, n_tris_c) &
+        & bind(c, name = "libsupermesh_intersect_tri_quad")
+      import :: c_double, c_int
+      real(kind = c_double), dimension(*) :: tri_a, quad_b, tris_c
+      integer(kind = c_int) :: n_tris_c
+    end subroutine c_intersect_tri_quad
+    subroutine c_intersect_tet_hex(tet_a, hex_b, tets_c, n_tets_c) &
+        & bind(c, name = "libsupermesh_intersect_tet_hex")
+      import :: c_double, c_int
+      real(kind = c_double), dimension(*) :: tet_a, hex_b, tets_c
+      integer(kind = c_int) :: n_tets_c
+    end subroutine c_intersect_tet_hex
+    subroutine c_intersect_intervals(interval_a, interval_b, intervals_c, n_intervals_c) &
+        & bind(c, name = "libsupermesh_intersect_intervals")
+      import :: c_double, c_int
+      real(kind = c_double), dimension(*) :: interval_a, interval_b, intervals_c
+      integer(kind = c_int) :: n_intervals_c
+    end subroutine c_intersect_intervals
+    subroutine c_interval_size(interval, size) bind(c, name = "libsupermesh_interval_size")
+      import :: c_double
+      real(kind = c_double), dimension(*) :: interval
+      real(kind = c_double) :: size
+    end subroutine c_interval_size
+    function c_context_create(device, ctx) bind(c, name = "libsupermesh_gpu_context_create")
+      import :: c_int, c_ptr
+      integer(kind = c_int), value :: device
+      type(c_ptr) :: ctx
+      integer(kind = c_int) :: c_context_create
+    end function c_context_create
+    function c_gpu_malloc(device, bytes, ptr) bind(c, name = "libsupermesh_gpu_malloc")
+      import :: c_int, c_size_t, c_ptr
+      integer(kind = c_int), value :: device
+      integer(kind = c_size_t), value :: bytes
+      type(c_ptr) :: ptr
+      integer(kind = c_int) :: c_gpu_malloc
+    end function c_gpu_malloc
+    function c_gpu_free(ptr) bind(c, name = "libsupermesh_gpu_free")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ptr
+      integer(kind = c_int) :: c_gpu_free
+    end function c_gpu_free
+    function c_gpu_memcpy(dst, src, bytes, kind) bind(c, name = "libsupermesh_gpu_memcpy")
+      import :: c_int, c_size_t, c_ptr
+      type(c_ptr), value :: dst, src
+      integer(kind = c_size_t), value :: bytes
+      integer(kind = c_int), value :: kind
+      integer(kind = c_int) :: c_gpu_memcpy
+    end function c_gpu_memcpy
+    function c_supermesh_callback(ctx, a, b, callback, user) &
+        & bind(c, name = "libsupermesh_gpu_supermesh_callback")
+      import :: c_int, c_ptr, c_funptr, gpu_mesh
+      type(c_ptr), value :: ctx
+      type(gpu_mesh) :: a, b
+      type(c_funptr), value :: callback
+      type(c_ptr), value :: user
+      integer(kind = c_int) :: c_supermesh_callback
+    end function c_supermesh_callback
     function c_last_error() bind(c, name = "libsupermesh_gpu_last_error")
       import :: c_ptr
       type(c_ptr) :: c_last_error
@@ -211,6 +320,175 @@ contains
     map_ab_indices = int(ind(:nind)) + 1
     map_ab_indptr = int(ptr) + 1
   end subroutine finder_csr
+
+  ! CSR candidate pairs of the 1D sort finder (Intersection_Finder.F90:924-948).
+  subroutine sort_intersection_finder_csr_sparsity(positions_a, enlist_a, positions_b, enlist_b, &
+      & map_ab_indices, map_ab_indptr)
+    real(kind = real_kind), dimension(:, :), intent(in) :: positions_a, positions_b
+    integer, dimension(:, :), intent(in) :: enlist_a, enlist_b
+    integer, dimension(:), allocatable, intent(out) :: map_ab_indices
+    integer, dimension(:), intent(out) :: map_ab_indptr
+    integer(kind = c_long) :: nn_a, ne_a, nn_b, ne_b, nind
+    integer(kind = c_int) :: dim_a, loc_a, dim_b, loc_b
+    integer(kind = c_long), dimension(:, :), allocatable :: ea, eb
+    integer(kind = c_long), dimension(:), allocatable :: ind, ptr
+    real(kind = c_double), dimension(:, :), allocatable :: pa, pb
+    dim_a = size(positions_a, 1)
+    nn_a = size(positions_a, 2)
+    loc_a = size(enlist_a, 1)
+    ne_a = size(enlist_a, 2)
+    dim_b = size(positions_b, 1)
+    nn_b = size(positions_b, 2)
+    loc_b = size(enlist_b, 1)
+    ne_b = size(enlist_b, 2)
+    allocate(pa(dim_a, nn_a), pb(dim_b, nn_b), ea(loc_a, ne_a), eb(loc_b, ne_b))
+    pa = positions_a
+    pb = positions_b
+    ea = enlist_a - 1
+    eb = enlist_b - 1
+    call c_sort_set_input(nn_a, dim_a, ne_a, loc_a, nn_b, dim_b, ne_b, loc_b, pa, ea, pb, eb)
+    nind = 0
+    if(c_last_error_code() == 0) call c_sort_query_output(nind)
+    allocate(ind(max(nind, 1_c_long)), ptr(ne_a + 1))
+    ptr = 0
+    if(c_last_error_code() == 0) call c_sort_get_output(ne_a, nind, ind, ptr)
+    allocate(map_ab_indices(nind))
+    map_ab_indices = int(ind(:nind)) + 1
+    map_ab_indptr = int(ptr) + 1
+  end subroutine sort_intersection_finder_csr_sparsity
+
+  ! intersect_intervals (Interval_Intersection.F90:45-70).
+  subroutine intersect_intervals(interval_a, interval_b, intervals_c, n_intervals_c)
+    real(kind = real_kind), dimension(2), intent(in) :: interval_a, interval_b
+    real(kind = real_kind), dimension(2), intent(out) :: intervals_c
+    integer, intent(out) :: n_intervals_c
+    real(kind = c_double), dimension(2) :: a, b
+    integer(kind = c_int) :: n
+    a = interval_a
+    b = interval_b
+    call c_intersect_intervals(a, b, intervals_c, n)
+    n_intervals_c = n
+  end subroutine intersect_intervals
+
+  function interval_size(interval) result(size)
+    real(kind = real_kind), dimension(2), intent(in) :: interval
+    real(kind = real_kind) :: size
+    real(kind = c_double), dimension(2) :: v
+    v = interval
+    call c_interval_size(v, size)
+  end function interval_size
+
+  ! intersect_tri_quad (Supermesh.F90:211-230): tris_c(2, 3, 46).
+  subroutine intersect_tri_quad(tri_a, quad_b, tris_c, n_tris_c)
+    real(kind = real_kind), dimension(2, 3), intent(in) :: tri_a
+    real(kind = real_kind), dimension(2, 4), intent(in) :: quad_b
+    real(kind = real_kind), dimension(:, :, :), intent(inout) :: tris_c
+    integer, intent(out) :: n_tris_c
+    real(kind = c_double), dimension(2, 3) :: a
+    real(kind = c_double), dimension(2, 4) :: b
+    real(kind = c_double), dimension(2, 3, 46) :: c
+    integer(kind = c_int) :: n
+    a = tri_a
+    b = quad_b
+    call c_intersect_tri_quad(a, b, c, n)
+    n_tris_c = n
+    if(n > 0) tris_c(:, :, :n) = c(:, :, :n)
+  end subroutine intersect_tri_quad
+
+  ! intersect_tet_hex (Supermesh.F90:251-301): tets_c(3, 4, 729).
+  subroutine intersect_tet_hex(tet_a, hex_b, tets_c, n_tets_c)
+    real(kind = real_kind), dimension(3, 4), intent(in) :: tet_a
+    real(kind = real_kind), dimension(3, 8), intent(in) :: hex_b
+    real(kind = real_kind), dimension(:, :, :), intent(inout) :: tets_c
+    integer, intent(out) :: n_tets_c
+    real(kind = c_double), dimension(3, 4) :: a
+    real(kind = c_double), dimension(3, 8) :: b
+    real(kind = c_double), dimension(:, :, :), allocatable :: c
+    integer(kind = c_int) :: n
+    allocate(c(3, 4, 729))
+    a = tet_a
+    b = hex_b
+    call c_intersect_tet_hex(a, b, c, n)
+    n_tets_c = n
+    if(n > 0) tets_c(:, :, :n) = c(:, :, :n)
+  end subroutine intersect_tet_hex
+
+  ! Callback driver: the single-GPU form of the self-self loop of
+  ! parallel_supermesh (Parallel_Supermesh.F90:436-453) without the MPI
+  ! exchange.  intersection_calculation is called for every closed-overlap
+  ! candidate pair, B element by B element, with local = .true.
+  subroutine supermesh(positions_a, enlist_a, positions_b, enlist_b, intersection_calculation)
+    real(kind = real_kind), dimension(:, :), intent(in) :: positions_a, positions_b
+    integer, dimension(:, :), intent(in) :: enlist_a, enlist_b
+    procedure(intersection_calculation_interface) :: intersection_calculation
+    type(gpu_mesh) :: ma, mb
+    type(c_ptr) :: bufs(4)
+    integer(kind = c_int) :: rc
+    integer :: i
+    bufs = c_null_ptr
+    if(.not. c_associated(gpu_context)) then
+      rc = c_context_create(0_c_int, gpu_context)
+      if(rc /= 0) return
+    end if
+    call upload(positions_a, enlist_a, ma, bufs(1), bufs(2), rc)
+    if(rc == 0) call upload(positions_b, enlist_b, mb, bufs(3), bufs(4), rc)
+    if(rc == 0) then
+      user_calculation => intersection_calculation
+      callback_dim = size(positions_a, 1)
+      rc = c_supermesh_callback(gpu_context, ma, mb, c_funloc(callback_trampoline), c_null_ptr)
+      user_calculation => null()
+    end if
+    do i = 1, 4
+      if(c_associated(bufs(i))) rc = c_gpu_free(bufs(i))
+    end do
+  end subroutine supermesh
+
+  subroutine upload(positions, enlist, mesh, dpos, denl, rc)
+    real(kind = real_kind), dimension(:, :), intent(in) :: positions
+    integer, dimension(:, :), intent(in) :: enlist
+    type(gpu_mesh), intent(out) :: mesh
+    type(c_ptr), intent(inout) :: dpos, denl
+    integer(kind = c_int), intent(out) :: rc
+    real(kind = c_double), dimension(:, :), allocatable, target :: p
+    integer(kind = c_int), dimension(:, :), allocatable, target :: e
+    integer(kind = c_size_t) :: np, ne
+    allocate(p(size(positions, 1), size(positions, 2)), e(size(enlist, 1), size(enlist, 2)))
+    p = positions
+    e = enlist - 1
+    np = 8_c_size_t * max(size(p), 1)
+    ne = 4_c_size_t * max(size(e), 1)
+    mesh%dim = size(positions, 1)
+    mesh%nnodes = size(positions, 2)
+    mesh%nelements = size(enlist, 2)
+    rc = c_gpu_malloc(0_c_int, np, dpos)
+    if(rc == 0) rc = c_gpu_malloc(0_c_int, ne, denl)
+    if(rc == 0 .and. size(p) > 0) rc = c_gpu_memcpy(dpos, c_loc(p), 8_c_size_t * size(p), 1_c_int)
+    if(rc == 0 .and. size(e) > 0) rc = c_gpu_memcpy(denl, c_loc(e), 4_c_size_t * size(e), 1_c_int)
+    mesh%positions = dpos
+    mesh%enlist = denl
+  end subroutine upload
+
+  subroutine callback_trampoline(element_a, element_b, elements_c, n_elements_c, nodes_b, &
+      & ele_a, ele_b, user) bind(c)
+    type(c_ptr), value :: element_a, element_b, elements_c, nodes_b, user
+    integer(kind = c_int), value :: n_elements_c
+    integer(kind = c_long), value :: ele_a, ele_b
+    real(kind = c_double), dimension(:, :), pointer :: ea, eb
+    real(kind = c_double), dimension(:, :, :), pointer :: ec
+    real(kind = c_double), dimension(callback_dim, callback_dim + 1, 0), target :: empty
+    integer(kind = c_long), dimension(:), pointer :: nb
+    integer :: d
+    d = callback_dim
+    call c_f_pointer(element_a, ea, [d, d + 1])
+    call c_f_pointer(element_b, eb, [d, d + 1])
+    call c_f_pointer(nodes_b, nb, [d + 1])
+    if(n_elements_c > 0) then
+      call c_f_pointer(elements_c, ec, [d, d + 1, int(n_elements_c)])
+    else
+      ec => empty
+    end if
+    call user_calculation(ea, eb, ec, int(nb) + 1, int(ele_a) + 1, int(ele_b) + 1, .true.)
+  end subroutine callback_trampoline
 
   ! intersect_tris_real (Triangle_Intersection.F90:125-142).
   subroutine intersect_tris_real(tri_a, tri_b, tris_c, n_tris_c)

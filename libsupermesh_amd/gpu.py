@@ -57,6 +57,11 @@ class Result(ctypes.Structure):
                 ("ms", c_double * 8)]
 
 
+# libsupermesh_gpu_pair_callback
+CALLBACK = ctypes.CFUNCTYPE(None, POINTER(c_double), POINTER(c_double), POINTER(c_double), c_int,
+                            POINTER(c_long), c_long, c_long, c_void_p)
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -74,6 +79,9 @@ def lib():
         L.libsupermesh_gpu_clip_pairs.argtypes = [c_void_p, POINTER(Mesh), POINTER(Mesh), c_long,
                                                   c_void_p, c_void_p, c_int, POINTER(Result)]
         L.libsupermesh_gpu_clip_pairs.restype = c_int
+        L.libsupermesh_gpu_supermesh_callback.argtypes = [c_void_p, POINTER(Mesh), POINTER(Mesh),
+                                                          CALLBACK, c_void_p]
+        L.libsupermesh_gpu_supermesh_callback.restype = c_int
         L.libsupermesh_gpu_malloc.argtypes = [c_int, ctypes.c_size_t, POINTER(c_void_p)]
         L.libsupermesh_gpu_malloc.restype = c_int
         L.libsupermesh_gpu_free.argtypes = [c_void_p]
@@ -333,6 +341,34 @@ class Context:
                                                           ctypes.byref(b.struct), finder, flags,
                                                           ctypes.byref(r)))
         return r
+
+    def supermesh_callback(self, a: DeviceMesh, b: DeviceMesh, fn):
+        """The callback driver: ``fn(element_a, element_b, elements_c, nodes_b,
+        ele_a, ele_b)`` for every closed-overlap candidate pair, B element by B
+        element, A candidates ascending (0-based ids, numpy arrays valid only
+        during the call)."""
+        dim = a.struct.dim
+        loc = dim + 1
+        err = []
+
+        def tramp(pa, pb, pc, n, nodes, ea, eb, user):
+            if err:
+                return
+            try:
+                A = np.ctypeslib.as_array(pa, shape=(loc, dim))
+                B = np.ctypeslib.as_array(pb, shape=(loc, dim))
+                Cc = (np.ctypeslib.as_array(pc, shape=(n, loc, dim)) if n > 0
+                      else np.zeros((0, loc, dim)))
+                nb = np.ctypeslib.as_array(nodes, shape=(loc,))
+                fn(A, B, Cc, nb, ea, eb)
+            except BaseException as e:  # surfaced after the driver returns
+                err.append(e)
+
+        cb = CALLBACK(tramp)
+        _check(lib().libsupermesh_gpu_supermesh_callback(self.handle, ctypes.byref(a.struct),
+                                                         ctypes.byref(b.struct), cb, None))
+        if err:
+            raise err[0]
 
     def clip_pairs(self, a: DeviceMesh, b: DeviceMesh, pair_a: DeviceArray, pair_b: DeviceArray,
                    flags=0):

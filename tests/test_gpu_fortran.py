@@ -27,15 +27,47 @@ def write_mesh(path, pos, enl):
     ("cube.1", "cube.2", None, None, 1.0),
 ])
 def test_fortran_driver(tmp_path, fixture_meshes, a, b, npairs, npieces, total):
+    _run_driver(tmp_path, fixture_meshes, a, b, npairs, npieces, total, "pairs", closed=False)
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran interface not built (no amdflang)")
+@pytest.mark.parametrize("a,b,total", [("triangle_0_01", "square_0_01", 0.5), ("cube.1", "cube.2", 1.0)])
+def test_fortran_callback_driver(tmp_path, fixture_meshes, a, b, total):
+    # supermesh(): every closed-overlap candidate reaches the callback
+    _run_driver(tmp_path, fixture_meshes, a, b, None, None, total, "callback", closed=True)
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran interface not built (no amdflang)")
+@pytest.mark.parametrize("a,b,total", [("square.1", "square.4", 1.0), ("cube.1", "cube.3", 0.5),
+                                       ("line.1", "line.2", 1.0)])
+def test_fortran_mixed_and_1d(tmp_path, fixture_meshes, a, b, total):
+    # src/tests/test_element_intersector.F90 / test_interval_intersector.F90 totals
     pa, ea = fixture_meshes[a]
     pb, eb = fixture_meshes[b]
     fa, fb = tmp_path / "a.bin", tmp_path / "b.bin"
     write_mesh(fa, pa, ea)
     write_mesh(fb, pb, eb)
-    r = subprocess.run([EXE, str(fa), str(fb)], capture_output=True, text=True, timeout=240)
+    r = subprocess.run([EXE, str(fa), str(fb), "mixed"], capture_output=True, text=True, timeout=240)
     assert r.returncode == 0, r.stdout + r.stderr
     dim, n, pieces, s = r.stdout.split()
-    ip, ix = O.brute_force(pa, ea, pb, eb)
+    if int(dim) == 1:
+        ip, ix = O.sort_finder(pa, ea, pb, eb)
+    else:
+        ip, ix = O.brute_force(pa, ea, pb, eb, closed=True)
+    assert int(n) == len(ix)
+    assert abs(float(s) - total) < 100.0 * max(np.finfo(float).eps, O.spacing(total))
+
+
+def _run_driver(tmp_path, fixture_meshes, a, b, npairs, npieces, total, mode, closed):
+    pa, ea = fixture_meshes[a]
+    pb, eb = fixture_meshes[b]
+    fa, fb = tmp_path / "a.bin", tmp_path / "b.bin"
+    write_mesh(fa, pa, ea)
+    write_mesh(fb, pb, eb)
+    r = subprocess.run([EXE, str(fa), str(fb), mode], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    dim, n, pieces, s = r.stdout.split()
+    ip, ix = O.brute_force(pa, ea, pb, eb, closed=closed)
     rows, cols = O.csr_to_pairs(ip, ix)
     cnt, meas = O.clip_pairs(pa, ea, pb, eb, rows, cols)
     assert int(n) == len(ix)

@@ -291,3 +291,27 @@ def test_invalid_inputs_raise(ctx):
     # the context is still usable afterwards
     r, out = run(ctx, pa, ea, pa, ea)
     assert abs(r.sum - 1.0) < 1e-14
+
+
+@pytest.mark.parametrize("names", [("triangle.1", "triangle.2"), ("cube.1", "cube.2")])
+def test_callback_driver(fixture_meshes, names):
+    """Callback driver: B-major order, A candidates ascending (closed overlap),
+    every pair's pieces equal to the oracle's intersect of (A, B)."""
+    pa, ea = fixture_meshes[names[0]]
+    pb, eb = fixture_meshes[names[1]]
+    ctx = gpu.Context()
+    A, B = gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pb, eb)
+    seen = []
+
+    def fn(xa, xb, xc, nodes_b, ia, ib):
+        assert np.array_equal(xa, pa[ea[ia]]) and np.array_equal(xb, pb[eb[ib]])
+        assert np.array_equal(nodes_b, eb[ib])
+        ref = O.intersect_tris(xa, xb) if pa.shape[1] == 2 else O.intersect_tets(xa, xb)
+        assert np.array_equal(xc, ref)
+        seen.append((ib, ia))
+
+    ctx.supermesh_callback(A, B, fn)
+    ip, ix = O.brute_force(pb, eb, pa, ea, closed=True)   # B rows, A ids ascending
+    rows, cols = O.csr_to_pairs(ip, ix)
+    expect = [(int(r), int(c)) for r, c in zip(rows, cols)]
+    assert seen == expect
