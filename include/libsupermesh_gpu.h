@@ -117,6 +117,40 @@ int libsupermesh_gpu_supermesh(libsupermesh_gpu_context* ctx, const libsupermesh
                                const libsupermesh_gpu_mesh* b, int finder, int flags,
                                libsupermesh_gpu_result* out);
 
+/* A batch of independent meshes of one dimension, concatenated: mesh m owns
+ * elements element_offsets[m] .. element_offsets[m+1]-1 (device array,
+ * [nmeshes + 1], from 0 to nelements).  enlist holds node ids into the whole
+ * positions array (0-based).  Mesh m of batch A is paired with mesh m of
+ * batch B only. */
+typedef struct {
+  int dim;
+  long nmeshes;
+  long nnodes;
+  long nelements;
+  const double* positions;     /* [nnodes][dim] */
+  const int* enlist;           /* [nelements][dim+1] */
+  const long* element_offsets; /* [nmeshes + 1] */
+} libsupermesh_gpu_mesh_batch;
+
+typedef struct {
+  long nmeshes;
+  const long* pair_offsets;    /* [nmeshes + 1]: candidate pairs of mesh m are
+                                  pair_offsets[m] .. pair_offsets[m+1]-1 of the
+                                  result's pair arrays */
+  const double* sums;          /* [nmeshes]: compensated measure sum per mesh */
+} libsupermesh_gpu_batch_result;
+
+/* Batched supermesh: every mesh pair of the batch in one pass (one candidate
+ * search over all meshes, one clip over all candidate pairs, per-mesh sums).
+ * `out` holds the pairs of all meshes in batch order, with the same per-pair
+ * content as separate libsupermesh_gpu_supermesh calls (element ids are
+ * batch-global). */
+int libsupermesh_gpu_supermesh_batched(libsupermesh_gpu_context* ctx,
+                                       const libsupermesh_gpu_mesh_batch* a,
+                                       const libsupermesh_gpu_mesh_batch* b, int finder, int flags,
+                                       libsupermesh_gpu_result* out,
+                                       libsupermesh_gpu_batch_result* batch);
+
 /* Callback driver: the single-GPU form of the self-self loop of the
  * reference's parallel_supermesh (src/Parallel_Supermesh.F90:436-453, its
  * intersection_calculation interface :97-113; the MPI exchange is left out).

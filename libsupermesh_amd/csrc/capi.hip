@@ -125,9 +125,11 @@ struct Phase {
   }
 };
 
-// Whole pipeline on device meshes.
+// Whole pipeline on device meshes.  In a batch, a_off / b_off (device,
+// [ngroups + 1]) split A and B into ngroups meshes, paired by index.
 static void run_supermesh(Context& C, const libsupermesh_gpu_mesh* a, const libsupermesh_gpu_mesh* b,
-                          int finder, int flags, bool clip, libsupermesh_gpu_result* out) {
+                          int finder, int flags, bool clip, libsupermesh_gpu_result* out,
+                          long ngroups = 0, const long* a_off = nullptr, const long* b_off = nullptr) {
   check_mesh(a, "a");
   check_mesh(b, "b");
   LSM_REQUIRE(a->dim == b->dim, LIBSUPERMESH_GPU_ERROR_INVALID, "Invalid dimension: meshes differ");
@@ -144,6 +146,8 @@ static void run_supermesh(Context& C, const libsupermesh_gpu_mesh* a, const libs
              false, mass);
   stage_mesh(C, C.mesh_b, b->dim, b->dim + 1, b->nnodes, b->nelements, b->positions, b->enlist,
              clip, mass);
+  stage_groups(C, C.mesh_a, ngroups, a_off);
+  stage_groups(C, C.mesh_b, ngroups, b_off);
   ph.mark();
   build_bvh(C, C.mesh_b);
   ph.mark();
@@ -156,6 +160,7 @@ static void run_supermesh(Context& C, const libsupermesh_gpu_mesh* a, const libs
   out->rows = static_cast<const int*>(C.rows.get(4));
   if (clip) {
     clip_pairs(C, C.mesh_a, C.mesh_b, npairs, flags, out);
+    if (ngroups > 0) batch_sums(C, ngroups, a_off, out->indptr, out->measure);
     ph.mark();
     if (mass) assemble_mass(C, C.mesh_a, C.mesh_b, npairs, out);
     ph.mark();
@@ -505,6 +510,30 @@ int libsupermesh_gpu_supermesh(libsupermesh_gpu_context* ctx, const libsupermesh
     LSM_REQUIRE(ctx != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID, "null context");
     std::lock_guard<std::mutex> lk(ctx->ctx.mu);
     run_supermesh(ctx->ctx, a, b, finder, flags, true, out);
+  });
+}
+
+int libsupermesh_gpu_supermesh_batched(libsupermesh_gpu_context* ctx,
+                                       const libsupermesh_gpu_mesh_batch* a,
+                                       const libsupermesh_gpu_mesh_batch* b, int finder, int flags,
+                                       libsupermesh_gpu_result* out,
+                                       libsupermesh_gpu_batch_result* batch) {
+  return guarded([&] {
+    LSM_REQUIRE(ctx != nullptr && a != nullptr && b != nullptr && batch != nullptr,
+                LIBSUPERMESH_GPU_ERROR_INVALID, "null argument");
+    LSM_REQUIRE(a->nmeshes == b->nmeshes && a->nmeshes >= 1, LIBSUPERMESH_GPU_ERROR_INVALID,
+                "batches of A and B hold different mesh counts");
+    LSM_REQUIRE(a->element_offsets && b->element_offsets, LIBSUPERMESH_GPU_ERROR_INVALID,
+                "null batch element offsets");
+    Context& C = ctx->ctx;
+    std::lock_guard<std::mutex> lk(C.mu);
+    libsupermesh_gpu_mesh ma = {a->dim, a->nnodes, a->nelements, a->positions, a->enlist};
+    libsupermesh_gpu_mesh mb = {b->dim, b->nnodes, b->nelements, b->positions, b->enlist};
+    run_supermesh(C, &ma, &mb, finder, flags, true, out, a->nmeshes, a->element_offsets,
+                  b->element_offsets);
+    batch->nmeshes = a->nmeshes;
+    batch->pair_offsets = static_cast<const long*>(C.gpairs.ptr);
+    batch->sums = static_cast<const double*>(C.gsum.ptr);
   });
 }
 

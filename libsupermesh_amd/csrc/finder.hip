@@ -17,6 +17,7 @@
 // A count pass sizes the CSR rows, a fill pass writes them, and each row is
 // then sorted by B id so the output is canonical.
 #include <cstring>
+#include <vector>
 
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
@@ -138,6 +139,7 @@ void stage_mesh(Context& C, DevMesh& M, int dim, int loc, long nnodes, long ne,
   M.loc = loc;
   M.nnodes = nnodes;
   M.nelements = ne;
+  M.ngroups = 0;
   M.enlist_dev = enlist;
   double* coords = M.coords.as<double>((size_t)ne * loc * dim);
   double* bbox = M.bbox.as<double>((size_t)ne * 2 * dim);
@@ -160,6 +162,30 @@ void stage_mesh(Context& C, DevMesh& M, int dim, int loc, long nnodes, long ne,
   LSM_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, C.stream));
   LSM_HIP(hipStreamSynchronize(C.stream));
   LSM_REQUIRE(hbad == 0, LIBSUPERMESH_GPU_ERROR_INVALID, "Invalid node: element-node index out of range");
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_groups(const long* __restrict__ offsets, int* __restrict__ group) {
+  const long g = blockIdx.x;
+  for (long e = offsets[g] + threadIdx.x; e < offsets[g + 1]; e += kBlock) group[e] = (int)g;
+}
+
+void stage_groups(Context& C, DevMesh& M, long ngroups, const long* offsets) {
+  M.ngroups = 0;
+  if (ngroups <= 0) return;
+  LSM_REQUIRE(ngroups < (1L << 24), LIBSUPERMESH_GPU_ERROR_INVALID, "too many meshes in a batch");
+  std::vector<long> h((size_t)ngroups + 1);
+  LSM_HIP(hipMemcpyAsync(h.data(), offsets, sizeof(long) * (ngroups + 1), hipMemcpyDeviceToHost,
+                         C.stream));
+  LSM_HIP(hipStreamSynchronize(C.stream));
+  LSM_REQUIRE(h[0] == 0 && h[ngroups] == M.nelements, LIBSUPERMESH_GPU_ERROR_INVALID,
+              "batch element offsets must run from 0 to the element count");
+  for (long g = 0; g < ngroups; g++)
+    LSM_REQUIRE(h[g + 1] >= h[g], LIBSUPERMESH_GPU_ERROR_INVALID, "batch element offsets decrease");
+  int* group = M.group.as<int>((size_t)M.nelements);
+  hipLaunchKernelGGL(k_groups, dim3((unsigned)ngroups), dim3(kBlock), 0, C.stream, offsets, group);
+  LSM_HIP(hipGetLastError());
+  M.ngroups = ngroups;
 }
 
 // ---------------------------------------------------------------------------
@@ -261,10 +287,14 @@ __device__ __forceinline__ uint64_t spread3(uint64_t x) {  // 21 bits -> every t
   return x;
 }
 
+// Sort key of element e: its Morton code, or in a batch the mesh id in the
+// top gbits bits above the code's leading bits, so that each mesh is one
+// contiguous run of the order.
 template <int DIM>
 __global__ void __launch_bounds__(kBlock)
 k_morton(long ne, const double* __restrict__ bbox, const double* __restrict__ bounds,
-         uint64_t* __restrict__ keys, int* __restrict__ vals) {
+         const int* __restrict__ group, int gbits, uint64_t* __restrict__ keys,
+         int* __restrict__ vals) {
   long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= ne) return;
   constexpr int BITS = DIM == 2 ? 32 : 21;
@@ -279,9 +309,14 @@ k_morton(long ne, const double* __restrict__ bbox, const double* __restrict__ bo
     uint64_t q = (uint64_t)(t * maxq);
     code |= (DIM == 2 ? spread2(q) : spread3(q)) << d;
   }
+  if (group && gbits > 0) {
+    constexpr int KEYBITS = DIM == 2 ? 64 : 63;
+    code = ((uint64_t)group[e] << (KEYBITS - gbits)) | (code >> gbits);
+  }
   keys[e] = code;
   vals[e] = (int)e;
 }
+
 
 // Sorts the elements of M by Morton code; returns the permutation in `perm`.
 static void morton_order(Context& C, const DevMesh& M, DevBuf& perm) {
@@ -294,16 +329,19 @@ static void morton_order(Context& C, const DevMesh& M, DevBuf& perm) {
   int* vals = C.vals.as<int>(ne);
   int* out = perm.as<int>(ne);
   const double* bbox = static_cast<const double*>(M.bbox.ptr);
+  const int* group = M.ngroups > 0 ? static_cast<const int*>(M.group.ptr) : nullptr;
+  int gbits = 0;
+  while (group && (1L << gbits) < M.ngroups) gbits++;
   if (dim == 2) {
     hipLaunchKernelGGL(k_center_bounds<2>, dim3(nparts), dim3(kBlock), 0, C.stream, ne, bbox, part);
     hipLaunchKernelGGL(k_bounds_final<2>, dim3(1), dim3(kBlock), 0, C.stream, nparts, part);
     hipLaunchKernelGGL(k_morton<2>, dim3(grid_for(ne, kBlock)), dim3(kBlock), 0, C.stream, ne, bbox,
-                       part, keys, vals);
+                       part, group, gbits, keys, vals);
   } else {
     hipLaunchKernelGGL(k_center_bounds<3>, dim3(nparts), dim3(kBlock), 0, C.stream, ne, bbox, part);
     hipLaunchKernelGGL(k_bounds_final<3>, dim3(1), dim3(kBlock), 0, C.stream, nparts, part);
     hipLaunchKernelGGL(k_morton<3>, dim3(grid_for(ne, kBlock)), dim3(kBlock), 0, C.stream, ne, bbox,
-                       part, keys, vals);
+                       part, group, gbits, keys, vals);
   }
   LSM_HIP(hipGetLastError());
   unsigned end_bit = dim == 2 ? 64 : 63;
@@ -329,24 +367,30 @@ struct LevelInfo {
 template <int DIM>
 __global__ void __launch_bounds__(kBlock)
 k_leaf_boxes(long ne, const double* __restrict__ bbox, const int* __restrict__ perm,
-             double* __restrict__ boxes) {
+             double* __restrict__ boxes, const int* __restrict__ group, int* __restrict__ granges) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ne) return;
   long e = perm[i];
 #pragma unroll
   for (int k = 0; k < 2 * DIM; k++) boxes[i * 2 * DIM + k] = bbox[(long)k * ne + e];
+  if (group) {
+    granges[2 * i] = group[e];
+    granges[2 * i + 1] = group[e];
+  }
 }
 
 template <int DIM>
 __global__ void __launch_bounds__(kBlock)
-k_parent_boxes(long nchild, long nparent, const double* __restrict__ child, double* __restrict__ parent) {
+k_parent_boxes(long nchild, long nparent, const double* __restrict__ child, double* __restrict__ parent,
+               const int* __restrict__ cg, int* __restrict__ pg) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nparent) return;
   double b[2 * DIM];
   long c0 = i * kFanout;
+  long c1 = c0 + kFanout < nchild ? c0 + kFanout : nchild;
 #pragma unroll
   for (int k = 0; k < 2 * DIM; k++) b[k] = child[c0 * 2 * DIM + k];
-  for (long c = c0 + 1; c < c0 + kFanout && c < nchild; c++) {
+  for (long c = c0 + 1; c < c1; c++) {
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
       b[d] = fmin_f(b[d], child[c * 2 * DIM + d]);
@@ -355,6 +399,11 @@ k_parent_boxes(long nchild, long nparent, const double* __restrict__ child, doub
   }
 #pragma unroll
   for (int k = 0; k < 2 * DIM; k++) parent[i * 2 * DIM + k] = b[k];
+  if (cg) {
+    // children are in mesh order: the first one holds the lowest id
+    pg[2 * i] = cg[2 * c0];
+    pg[2 * i + 1] = cg[2 * (c1 - 1) + 1];
+  }
 }
 
 void build_bvh(Context& C, const DevMesh& B) {
@@ -384,21 +433,26 @@ void build_bvh(Context& C, const DevMesh& B) {
   double* boxes = H.boxes.as<double>((size_t)total * w);
   const double* bbox = static_cast<const double*>(B.bbox.ptr);
   const int* perm = static_cast<const int*>(H.perm.ptr);
+  H.grouped = B.ngroups > 0;
+  const int* group = H.grouped ? static_cast<const int*>(B.group.ptr) : nullptr;
+  int* gr = H.grouped ? H.granges.as<int>((size_t)total * 2) : nullptr;
   if (B.dim == 2)
     hipLaunchKernelGGL(k_leaf_boxes<2>, dim3(grid_for(H.n, kBlock)), dim3(kBlock), 0, C.stream, H.n,
-                       bbox, perm, boxes);
+                       bbox, perm, boxes, group, gr);
   else
     hipLaunchKernelGGL(k_leaf_boxes<3>, dim3(grid_for(H.n, kBlock)), dim3(kBlock), 0, C.stream, H.n,
-                       bbox, perm, boxes);
+                       bbox, perm, boxes, group, gr);
   for (int l = 1; l < L; l++) {
     const double* ch = boxes + H.offset[l - 1] * w;
     double* pa = boxes + H.offset[l] * w;
+    const int* cg = gr ? gr + 2 * H.offset[l - 1] : nullptr;
+    int* pg = gr ? gr + 2 * H.offset[l] : nullptr;
     if (B.dim == 2)
       hipLaunchKernelGGL(k_parent_boxes<2>, dim3(grid_for(H.count[l], kBlock)), dim3(kBlock), 0,
-                         C.stream, H.count[l - 1], H.count[l], ch, pa);
+                         C.stream, H.count[l - 1], H.count[l], ch, pa, cg, pg);
     else
       hipLaunchKernelGGL(k_parent_boxes<3>, dim3(grid_for(H.count[l], kBlock)), dim3(kBlock), 0,
-                         C.stream, H.count[l - 1], H.count[l], ch, pa);
+                         C.stream, H.count[l - 1], H.count[l], ch, pa, cg, pg);
   }
   LSM_HIP(hipGetLastError());
 }
@@ -420,14 +474,17 @@ __device__ __forceinline__ bool overlaps(const double* __restrict__ box, const d
   return hit;
 }
 
-template <int DIM, bool CLOSED, bool FILL>
+template <int DIM, bool CLOSED, bool FILL, bool GROUPED>
 __global__ void __launch_bounds__(kBlock)
 k_query(long na, const int* __restrict__ qperm, const double* __restrict__ abox, LevelInfo info,
         const double* __restrict__ boxes, const int* __restrict__ perm, long* __restrict__ row_count,
-        const long* __restrict__ indptr, int* __restrict__ indices, int* __restrict__ rows) {
+        const long* __restrict__ indptr, int* __restrict__ indices, int* __restrict__ rows,
+        const int* __restrict__ agroup, const int* __restrict__ granges) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= na) return;
   const int a = qperm[t];
+  // in a batch, only nodes whose mesh range holds A's mesh are visited
+  const int g = GROUPED ? agroup[a] : 0;
   double qlo[DIM], qhi[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; d++) {
@@ -441,7 +498,13 @@ k_query(long na, const int* __restrict__ qperm, const double* __restrict__ abox,
   long i = 0;
   while (true) {
     bool hit = false;
-    if (i < info.count[l]) hit = overlaps<DIM, CLOSED>(boxes + (info.offset[l] + i) * 2 * DIM, qlo, qhi);
+    if (i < info.count[l]) {
+      hit = overlaps<DIM, CLOSED>(boxes + (info.offset[l] + i) * 2 * DIM, qlo, qhi);
+      if (GROUPED) {
+        const int* r = granges + 2 * (info.offset[l] + i);
+        hit = hit && r[0] <= g && g <= r[1];
+      }
+    }
     if (hit) {
       if (l == 0) {
         if (FILL) {
@@ -516,9 +579,20 @@ long find_pairs(Context& C, const DevMesh& A, const DevMesh& B, int finder, int 
   const double* boxes = static_cast<const double*>(H.boxes.ptr);
   const int* perm = static_cast<const int*>(H.perm.ptr);
   dim3 grid(grid_for(na, kBlock)), block(kBlock);
-#define LSM_QUERY(DIM, CL, FILL)                                                                  \
-  hipLaunchKernelGGL((k_query<DIM, CL, FILL>), grid, block, 0, C.stream, na, qperm, abox, info, \
-                     boxes, perm, row_count, indptr, indices, rows)
+  LSM_REQUIRE((A.ngroups > 0) == H.grouped && (!H.grouped || A.ngroups == B.ngroups),
+              LIBSUPERMESH_GPU_ERROR_INVALID, "batches of A and B hold different mesh counts");
+  const int* agroup = H.grouped ? static_cast<const int*>(A.group.ptr) : nullptr;
+  const int* granges = H.grouped ? static_cast<const int*>(H.granges.ptr) : nullptr;
+#define LSM_QUERY(DIM, CL, FILL)                                                                    \
+  do {                                                                                            \
+    if (H.grouped)                                                                                \
+      hipLaunchKernelGGL((k_query<DIM, CL, FILL, true>), grid, block, 0, C.stream, na, qperm, abox, \
+                         info, boxes, perm, row_count, indptr, indices, rows, agroup, granges);   \
+    else                                                                                          \
+      hipLaunchKernelGGL((k_query<DIM, CL, FILL, false>), grid, block, 0, C.stream, na, qperm,      \
+                         abox, info, boxes, perm, row_count, indptr, indices, rows, agroup,        \
+                         granges);                                                                \
+  } while (0)
   int* indices = nullptr;
   int* rows = nullptr;
   bool closed = finder == LIBSUPERMESH_GPU_FINDER_RTREE;

@@ -81,6 +81,9 @@ struct DevMesh {
   DevBuf scale;    // double [ne]: max(1, max |vertex coordinate|)
   DevBuf planes;   // double [ne][16]: tet planes (normals, c) of B
   DevBuf affine;   // double [ne][loc][dim+1]: barycentric maps (mass)
+  // batches: element e belongs to mesh group[e]; ngroups == 0 outside batches
+  long ngroups = 0;
+  DevBuf group;    // int [ne]
 };
 
 // Implicit F-ary bounding volume hierarchy over Morton-sorted elements.
@@ -91,8 +94,10 @@ struct DevBVH {
   long count[32];           // nodes per level
   long offset[32];          // node offset of each level in the box arrays
   long total = 0;
-  DevBuf boxes;             // double [2*dim][total] SoA
+  DevBuf boxes;             // double [total][2*dim]: lo_0.., hi_0.. per node
   DevBuf perm;              // int [n]: sorted position -> element id
+  bool grouped = false;     // batches: granges holds each node's mesh range
+  DevBuf granges;           // int [total][2]: lowest and highest mesh id below the node
 };
 
 struct Context {
@@ -110,6 +115,8 @@ struct Context {
   DevBuf work, nwork;            // compacted list of pairs that need the full clip
   // mass assembly
   DevBuf mflag, mslot, mkeys, mvals, mkeys_alt, mvals_alt, ukeys, uvals, nuniq, mindptr, mindices;
+  // batches: per-mesh pair offsets and measure sums
+  DevBuf gpairs, gsum;
   // per-pair host API staging
   DevBuf one_in, one_out;
   void* pinned = nullptr;
@@ -125,6 +132,9 @@ Context* default_context();
 // Mesh staging + BVH (finder.hip).
 void stage_mesh(Context& C, DevMesh& M, int dim, int loc, long nnodes, long nelements,
                 const double* positions, const int* enlist, bool need_planes, bool need_affine);
+// Mesh id of every element of a batch from its element offsets (device,
+// [ngroups + 1]); validates the offsets.
+void stage_groups(Context& C, DevMesh& M, long ngroups, const long* offsets);
 void build_bvh(Context& C, const DevMesh& B);
 // Count / fill candidate pairs of A against the BVH of B.  Returns npairs.
 long find_pairs(Context& C, const DevMesh& A, const DevMesh& B, int finder, int sort_rows);
@@ -134,6 +144,10 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
                 libsupermesh_gpu_result* out);
 void assemble_mass(Context& C, const DevMesh& A, const DevMesh& B, long npairs,
                    libsupermesh_gpu_result* out);
+// Batches: per-mesh candidate pair offsets (gpairs, from the CSR row pointers
+// at the A mesh offsets) and compensated measure sums per mesh (gsum).
+void batch_sums(Context& C, long ngroups, const long* a_offsets, const long* indptr,
+                const double* measure);
 
 inline unsigned grid_for(long n, int block) {
   long g = (n + block - 1) / block;

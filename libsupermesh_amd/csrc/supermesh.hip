@@ -427,6 +427,50 @@ __global__ void __launch_bounds__(kClipBlock) k_reduce_partials(int n, Partial* 
   if (threadIdx.x == 0) parts[0] = red[0];
 }
 
+// Batches: candidate pair range of each mesh (its A elements are the rows
+// a_off[g] .. a_off[g+1]) ...
+__global__ void __launch_bounds__(kClipBlock)
+k_batch_pairs(long ngroups, const long* __restrict__ a_off, const long* __restrict__ indptr,
+              long* __restrict__ gpairs) {
+  long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g <= ngroups) gpairs[g] = indptr[a_off[g]];
+}
+
+// ... and its measure sum: one block per mesh, double-double in a fixed order.
+__global__ void __launch_bounds__(kClipBlock)
+k_batch_sum(const long* __restrict__ gpairs, const double* __restrict__ measure,
+            double* __restrict__ gsum) {
+  const long g = blockIdx.x;
+  double hi = 0.0, lo = 0.0;
+  for (long p = gpairs[g] + threadIdx.x; p < gpairs[g + 1]; p += kClipBlock) dd_add(hi, lo, measure[p]);
+  __shared__ double rh[kClipBlock], rl[kClipBlock];
+  rh[threadIdx.x] = hi;
+  rl[threadIdx.x] = lo;
+  __syncthreads();
+  for (int s = kClipBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      double h = rh[threadIdx.x], l = rl[threadIdx.x];
+      dd_merge(h, l, rh[threadIdx.x + s], rl[threadIdx.x + s]);
+      rh[threadIdx.x] = h;
+      rl[threadIdx.x] = l;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) gsum[g] = rh[0] + rl[0];
+}
+
+void batch_sums(Context& C, long ngroups, const long* a_offsets, const long* indptr,
+                const double* measure) {
+  long* gp = C.gpairs.as<long>((size_t)ngroups + 1);
+  double* gs = C.gsum.as<double>((size_t)ngroups);
+  hipLaunchKernelGGL(k_batch_pairs, dim3(grid_for(ngroups + 1, kClipBlock)), dim3(kClipBlock), 0,
+                     C.stream, ngroups, a_offsets, indptr, gp);
+  if (ngroups > 0)
+    hipLaunchKernelGGL(k_batch_sum, dim3((unsigned)ngroups), dim3(kClipBlock), 0, C.stream, gp,
+                       measure, gs);
+  LSM_HIP(hipGetLastError());
+}
+
 template <int DIM>
 static void launch_clip(Context& C, const ClipArgs& args, bool mass, bool fill) {
   dim3 grid(kClipGrid), block(kClipBlock);

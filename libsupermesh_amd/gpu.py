@@ -82,6 +82,10 @@ def lib():
         L.libsupermesh_gpu_supermesh_callback.argtypes = [c_void_p, POINTER(Mesh), POINTER(Mesh),
                                                           CALLBACK, c_void_p]
         L.libsupermesh_gpu_supermesh_callback.restype = c_int
+        L.libsupermesh_gpu_supermesh_batched.argtypes = [c_void_p, POINTER(MeshBatch),
+                                                         POINTER(MeshBatch), c_int, c_int,
+                                                         POINTER(Result), POINTER(BatchResult)]
+        L.libsupermesh_gpu_supermesh_batched.restype = c_int
         L.libsupermesh_gpu_malloc.argtypes = [c_int, ctypes.c_size_t, POINTER(c_void_p)]
         L.libsupermesh_gpu_malloc.restype = c_int
         L.libsupermesh_gpu_free.argtypes = [c_void_p]
@@ -305,6 +309,41 @@ class DeviceMesh:
         return self.struct.nelements
 
 
+class MeshBatch(ctypes.Structure):
+    _fields_ = [("dim", c_int), ("nmeshes", c_long), ("nnodes", c_long), ("nelements", c_long),
+                ("positions", c_void_p), ("enlist", c_void_p), ("element_offsets", c_void_p)]
+
+
+class BatchResult(ctypes.Structure):
+    _fields_ = [("nmeshes", c_long), ("pair_offsets", c_void_p), ("sums", c_void_p)]
+
+
+class DeviceMeshBatch:
+    """Meshes of one dimension concatenated on the device: node ids are made
+    batch-global, ``element_offsets[m]`` is the first element of mesh m."""
+
+    def __init__(self, meshes, device=0):
+        dim = meshes[0][0].shape[1]
+        node_off = np.zeros(len(meshes) + 1, dtype=np.int64)
+        elem_off = np.zeros(len(meshes) + 1, dtype=np.int64)
+        for m, (pos, enl) in enumerate(meshes):
+            assert pos.shape[1] == dim
+            node_off[m + 1] = node_off[m] + pos.shape[0]
+            elem_off[m + 1] = elem_off[m] + enl.shape[0]
+        pos = np.concatenate([np.asarray(p, dtype=np.float64) for p, _ in meshes])
+        enl = np.concatenate([np.asarray(e, dtype=np.int64) + node_off[m]
+                              for m, (_, e) in enumerate(meshes)]).astype(np.int32)
+        self.dim = dim
+        self.node_offsets = node_off
+        self.element_offsets = elem_off
+        self.positions = DeviceArray(np.ascontiguousarray(pos), device)
+        self.enlist = DeviceArray(np.ascontiguousarray(enl), device)
+        self.offsets = DeviceArray(elem_off, device)
+        self.struct = MeshBatch(dim, len(meshes), pos.shape[0], enl.shape[0],
+                                self.positions.ptr.value, self.enlist.ptr.value,
+                                self.offsets.ptr.value)
+
+
 class Context:
     """A library context (its own HIP stream and reusable device buffers).
     Results returned by a call stay valid until the next call."""
@@ -341,6 +380,15 @@ class Context:
                                                           ctypes.byref(b.struct), finder, flags,
                                                           ctypes.byref(r)))
         return r
+
+    def supermesh_batched(self, a: DeviceMeshBatch, b: DeviceMeshBatch, finder=FINDER_TREE, flags=0):
+        """(Result over all meshes, BatchResult with per-mesh pair offsets and sums)."""
+        r = Result()
+        br = BatchResult()
+        _check(lib().libsupermesh_gpu_supermesh_batched(self.handle, ctypes.byref(a.struct),
+                                                        ctypes.byref(b.struct), finder, flags,
+                                                        ctypes.byref(r), ctypes.byref(br)))
+        return r, br
 
     def supermesh_callback(self, a: DeviceMesh, b: DeviceMesh, fn):
         """The callback driver: ``fn(element_a, element_b, elements_c, nodes_b,
@@ -405,6 +453,13 @@ def fetch(r: Result, dim, what=("indptr", "indices", "rows", "counts", "measure"
         else:
             raise KeyError(w)
     return out
+
+
+def fetch_batch(br: BatchResult):
+    """Per-mesh pair offsets and measure sums of a batched call, as numpy."""
+    n = br.nmeshes
+    return (device_to_numpy(br.pair_offsets, n + 1, np.int64),
+            device_to_numpy(br.sums, n, np.float64))
 
 
 def fetch_mass(r: Result, nnodes_a):
