@@ -40,6 +40,8 @@ def parse():
     p.add_argument("--workload", default="tri_1024_vs_1000")
     p.add_argument("--no-3d", action="store_true", help="skip the 3D workload")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--no-batched", action="store_true", help="skip the batched workload")
+    p.add_argument("--batch-count", type=int, default=4096, help="mesh pairs in the batched workload")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also run a plain device copy of known size (PMC calibration)")
     return p.parse_args()
@@ -93,6 +95,72 @@ def gpu_workload(ctx, name, steps, warmup, mass):
     }
 
 
+def batched_workload(ctx, steps, warmup, count=4096):
+    """The batched workload: `count` independent mesh pairs (2D or 3D with
+    equal odds, element counts log-uniform in [50, 50000] per mesh, seed 3).
+    Plain = one libsupermesh_gpu_supermesh call per mesh pair; batched = one
+    libsupermesh_gpu_supermesh_batched call per dimension.  Both compute the
+    candidate pairs, their clips and per-pair / per-mesh measure sums."""
+    import numpy as np
+    from libsupermesh_amd import gpu, meshgen
+    t0 = time.time()
+    pairs = meshgen.batched_pairs(count=count, seed=3)
+    log(f"[bench] batched: {len(pairs)} mesh pairs generated ({time.time() - t0:.1f}s)")
+    plain = [(gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pb, eb)) for _, pa, ea, pb, eb in pairs]
+    batches = {}
+    for dim in (2, 3):
+        sel = [k for k, p in enumerate(pairs) if p[0] == dim]
+        if sel:
+            batches[dim] = (sel, gpu.DeviceMeshBatch([(pairs[k][1], pairs[k][2]) for k in sel]),
+                            gpu.DeviceMeshBatch([(pairs[k][3], pairs[k][4]) for k in sel]))
+    log(f"[bench] batched: meshes resident ({time.time() - t0:.1f}s)")
+
+    def run_plain():
+        tot, sums = 0, np.zeros(len(pairs))
+        for k, (A, B) in enumerate(plain):
+            r = ctx.supermesh(A, B)
+            tot += r.npairs
+            sums[k] = r.sum
+        return tot, sums
+
+    def run_batched():
+        tot, sums = 0, np.zeros(len(pairs))
+        for dim, (sel, BA, BB) in batches.items():
+            r, br = ctx.supermesh_batched(BA, BB)
+            tot += r.npairs
+            sums[sel] = gpu.fetch_batch(br)[1]
+        return tot, sums
+
+    out = {}
+    for name, fn in (("plain", run_plain), ("batched", run_batched)):
+        for _ in range(warmup):
+            fn()
+        gpu.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            npairs, sums = fn()
+        gpu.synchronize()
+        dt = (time.perf_counter() - t1) / steps
+        out[name] = {"ms_per_step": dt * 1e3, "pairs": int(npairs), "pairs_per_s": npairs / dt,
+                     "_sums": sums}
+        log(f"[bench] batched workload, {name}: {npairs} pairs, {dt * 1e3:.1f} ms/step")
+    dev = np.abs(out["plain"]["_sums"] - out["batched"]["_sums"])
+    ref = np.maximum(1.0, np.abs(out["plain"]["_sums"]))
+    unit_err = float(np.abs(out["batched"]["_sums"] - 1.0).max())   # every mesh covers [0,1]^d
+    for v in out.values():
+        v.pop("_sums")
+    n2 = len(batches.get(2, ([],))[0])
+    return {
+        "workload": f"{len(pairs)} mesh pairs ({n2} 2D, {len(pairs) - n2} 3D), "
+                    "elements log-uniform in [50, 50000] per mesh, seed 3",
+        "plain": out["plain"], "batched": out["batched"],
+        "speedup": out["plain"]["ms_per_step"] / out["batched"]["ms_per_step"],
+        "same_pairs": out["plain"]["pairs"] == out["batched"]["pairs"],
+        "max_rel_sum_diff": float((dev / ref).max()),
+        "max_abs_sum_err": unit_err,
+    }
+
+
 def cpu_baseline(name, meshes):
     """The CPU restatement of the reference's serial loop (tree finder, then
     the per-pair clip), one thread per host core over contiguous ranges of A."""
@@ -132,6 +200,9 @@ def main():
     extra = None
     if not args.no_3d and args.workload != "tet_64_vs_60":
         extra = gpu_workload(ctx, "tet_64_vs_60", max(1, args.steps), max(1, args.warmup), mass=False)
+    batched = None
+    if not args.no_batched:
+        batched = batched_workload(ctx, max(1, min(args.steps, 3)), 1, count=args.batch_count)
     fpp = FLOPS_PER_PAIR.get(args.workload)
     clip_s = main_w["clip_kernel_ms"] / 1e3
     if fpp:
@@ -164,6 +235,7 @@ def main():
         "cpu_baseline": cpu,
         "detail": main_w,
         "tet_64_vs_60": extra,
+        "batched": batched,
     }
     ctx.close()
     print(json.dumps(out))

@@ -42,6 +42,10 @@ extern "C" {
 #define LIBSUPERMESH_GPU_SORTED 8     /* sort each candidate row by B id (otherwise
                                          rows keep the hierarchy's traversal
                                          order, which is deterministic) */
+#define LIBSUPERMESH_GPU_LEARNED 16   /* 3D with ELEMENTS: size the output from the
+                                         learned predictor and clip once (pairs
+                                         that outgrow their budget are clipped
+                                         again); the output is identical */
 
 typedef struct libsupermesh_gpu_context libsupermesh_gpu_context;
 
@@ -83,9 +87,13 @@ typedef struct {
   int max_count;             /* largest per-pair output count */
   /* phase times in ms (LIBSUPERMESH_GPU_TIMING): [0] staging, [1] hierarchy
    * build, [2] candidate search, [3] clipping (count pass + reductions +
-   * element fill), [4] mass assembly, [6] total, [7] the clip count kernel
-   * alone */
+   * element fill), [4] mass assembly, [5] the predictor kernel (LEARNED),
+   * [6] total, [7] the clip count kernel alone (the single pass with LEARNED) */
   double ms[8];
+  /* LIBSUPERMESH_GPU_LEARNED: output slots allocated from the predicted
+   * budgets, and pairs clipped again because they outgrew their budget */
+  long learned_slots;
+  long learned_overflow;
 } libsupermesh_gpu_result;
 
 /* Context on HIP device `device` with its own stream (default priority). */
@@ -150,6 +158,38 @@ int libsupermesh_gpu_supermesh_batched(libsupermesh_gpu_context* ctx,
                                        const libsupermesh_gpu_mesh_batch* b, int finder, int flags,
                                        libsupermesh_gpu_result* out,
                                        libsupermesh_gpu_batch_result* batch);
+
+/* Learned output-size predictor (a BF16 MLP on the MFMA units; see
+ * LIBSUPERMESH_GPU_LEARNED).  Parameters are LIBSUPERMESH_GPU_PREDICTOR_PARAMS
+ * floats: W1[64][24] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[33][64]
+ * b4[33] (W[out][in]); inputs are the pair's 8 vertices normalised to the
+ * pair's bounding box, outputs the logits of counts 0..32 (32 = "32 or more").
+ * set/get copy host arrays; quantile sets the budget rule (the smallest count
+ * whose predicted cumulative probability reaches q, default 0.99); collect
+ * clips A x B with the plain path and keeps up to max_samples (features,
+ * count) of its fully clipped pairs as training data; train runs `epochs`
+ * passes of Adam over the collected samples (batch a multiple of 64),
+ * writing each epoch's mean cross-entropy to loss_per_epoch; clear drops the
+ * samples.  No counterpart in the reference. */
+#define LIBSUPERMESH_GPU_PREDICTOR_PARAMS 12065
+int libsupermesh_gpu_predictor_set(libsupermesh_gpu_context* ctx, const float* params);
+int libsupermesh_gpu_predictor_get(libsupermesh_gpu_context* ctx, float* params);
+int libsupermesh_gpu_predictor_quantile(libsupermesh_gpu_context* ctx, float q);
+int libsupermesh_gpu_predictor_collect(libsupermesh_gpu_context* ctx, const libsupermesh_gpu_mesh* a,
+                                       const libsupermesh_gpu_mesh* b, long max_samples,
+                                       long* nsamples);
+int libsupermesh_gpu_predictor_train(libsupermesh_gpu_context* ctx, int epochs, int batch, float lr,
+                                     unsigned seed, float* loss_per_epoch);
+int libsupermesh_gpu_predictor_clear(libsupermesh_gpu_context* ctx);
+/* Adds n training samples from host arrays: features[n][24] (as the
+ * predictor's inputs) and their output counts[n] (clamped to 0..32). */
+int libsupermesh_gpu_predictor_add_samples(libsupermesh_gpu_context* ctx, long n, const float* features,
+                                           const int* counts);
+/* Predicted output budget of each listed tet pair (device arrays pair_a[n],
+ * pair_b[n] -> budgets[n]), by the budget rule above. */
+int libsupermesh_gpu_predictor_infer(libsupermesh_gpu_context* ctx, const libsupermesh_gpu_mesh* a,
+                                     const libsupermesh_gpu_mesh* b, long npairs, const int* pair_a,
+                                     const int* pair_b, int* budgets);
 
 /* Callback driver: the single-GPU form of the self-self loop of the
  * reference's parallel_supermesh (src/Parallel_Supermesh.F90:436-453, its

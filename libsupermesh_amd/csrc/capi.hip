@@ -18,6 +18,7 @@
 #include "../../include/libsupermesh_gpu.h"
 #include "lsm_geometry.hpp"
 #include "lsm_runtime.hpp"
+#include "predictor.hpp"
 
 struct libsupermesh_gpu_context {
   lsm::Context ctx;
@@ -61,7 +62,13 @@ static void free_context(Context& C) {
                     &C.row_count, &C.indptr, &C.indices, &C.rows, &C.counts, &C.measure,
                     &C.partials, &C.scalars, &C.offsets, &C.elements, &C.blocks, &C.mflag, &C.mslot, &C.work, &C.nwork,
                     &C.mkeys, &C.mvals, &C.mkeys_alt, &C.mvals_alt, &C.ukeys, &C.uvals, &C.nuniq,
-                    &C.mindptr, &C.mindices, &C.one_in, &C.one_out};
+                    &C.mindptr, &C.mindices, &C.one_in, &C.one_out, &C.mesh_a.scale,
+                    &C.mesh_b.scale, &C.mesh_a.group, &C.mesh_b.group, &C.bvh.granges, &C.gpairs,
+                    &C.gsum, &C.predictor.weights, &C.predictor.bias, &C.predictor.master,
+                    &C.predictor.adam_m, &C.predictor.adam_v, &C.predictor.grad,
+                    &C.predictor.features, &C.predictor.labels, &C.predictor.act,
+                    &C.predictor.loss, &C.predictor.index, &C.predictor.budget,
+                    &C.predictor.slots, &C.predictor.slot_elems, &C.predictor.overflow};
   for (DevBuf* b : bufs) b->release();
   if (C.pinned) hipHostFree(C.pinned);
   C.pinned = nullptr;
@@ -129,7 +136,8 @@ struct Phase {
 // [ngroups + 1]) split A and B into ngroups meshes, paired by index.
 static void run_supermesh(Context& C, const libsupermesh_gpu_mesh* a, const libsupermesh_gpu_mesh* b,
                           int finder, int flags, bool clip, libsupermesh_gpu_result* out,
-                          long ngroups = 0, const long* a_off = nullptr, const long* b_off = nullptr) {
+                          long ngroups = 0, const long* a_off = nullptr, const long* b_off = nullptr,
+                          int loc_a = 0, int loc_b = 0) {
   check_mesh(a, "a");
   check_mesh(b, "b");
   LSM_REQUIRE(a->dim == b->dim, LIBSUPERMESH_GPU_ERROR_INVALID, "Invalid dimension: meshes differ");
@@ -142,10 +150,10 @@ static void run_supermesh(Context& C, const libsupermesh_gpu_mesh* a, const libs
   Phase ph(C, flags & LIBSUPERMESH_GPU_TIMING);
   C.timing = ph.on;
   ph.mark();
-  stage_mesh(C, C.mesh_a, a->dim, a->dim + 1, a->nnodes, a->nelements, a->positions, a->enlist,
-             false, mass);
-  stage_mesh(C, C.mesh_b, b->dim, b->dim + 1, b->nnodes, b->nelements, b->positions, b->enlist,
-             clip, mass);
+  stage_mesh(C, C.mesh_a, a->dim, loc_a ? loc_a : a->dim + 1, a->nnodes, a->nelements, a->positions,
+             a->enlist, false, mass);
+  stage_mesh(C, C.mesh_b, b->dim, loc_b ? loc_b : b->dim + 1, b->nnodes, b->nelements, b->positions,
+             b->enlist, clip, mass);
   stage_groups(C, C.mesh_a, ngroups, a_off);
   stage_groups(C, C.mesh_b, ngroups, b_off);
   ph.mark();
@@ -362,8 +370,8 @@ static FinderState g_tree, g_rtree, g_sort;
 static void host_mesh_to_device(Context& C, DevMesh& M, long nnodes, int dim, long ne, int loc,
                                 const double* pos, const long* enl, libsupermesh_gpu_mesh& view) {
   LSM_REQUIRE(dim == 2 || dim == 3, LIBSUPERMESH_GPU_ERROR_INVALID, "Invalid dimension");
-  LSM_REQUIRE(loc == dim + 1, LIBSUPERMESH_GPU_ERROR_INVALID,
-              "Unsupported element type: only simplices (loc = dim + 1)");
+  LSM_REQUIRE(loc == dim + 1 || loc == (1 << dim), LIBSUPERMESH_GPU_ERROR_INVALID,
+              "Unsupported element type: simplices, quads or hexahedra only");
   LSM_REQUIRE(nnodes >= 0 && ne >= 0 && nnodes < (1L << 31) && ne < (1L << 31),
               LIBSUPERMESH_GPU_ERROR_INVALID, "Invalid mesh size");
   std::vector<int> e32((size_t)ne * loc);
@@ -402,7 +410,8 @@ static void finder_set_input(FinderState& S, int finder, long nnodes_a, int dim_
   host_mesh_to_device(C, C.mesh_a, nnodes_a, dim_a, ne_a, loc_a, pos_a, enl_a, va);
   host_mesh_to_device(C, C.mesh_b, nnodes_b, dim_b, ne_b, loc_b, pos_b, enl_b, vb);
   libsupermesh_gpu_result r;
-  run_supermesh(C, &va, &vb, finder, LIBSUPERMESH_GPU_SORTED, false, &r);
+  run_supermesh(C, &va, &vb, finder, LIBSUPERMESH_GPU_SORTED, false, &r, 0, nullptr, nullptr, loc_a,
+                loc_b);
   S.nelements_a = ne_a;
   S.indptr.resize((size_t)ne_a + 1);
   S.indices.resize((size_t)r.npairs);
@@ -534,6 +543,107 @@ int libsupermesh_gpu_supermesh_batched(libsupermesh_gpu_context* ctx,
     batch->nmeshes = a->nmeshes;
     batch->pair_offsets = static_cast<const long*>(C.gpairs.ptr);
     batch->sums = static_cast<const double*>(C.gsum.ptr);
+  });
+}
+
+int libsupermesh_gpu_predictor_set(libsupermesh_gpu_context* ctx, const float* params) {
+  return guarded([&] {
+    LSM_REQUIRE(ctx != nullptr && params != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(ctx->ctx.mu);
+    LSM_HIP(hipSetDevice(ctx->ctx.device));
+    predictor_set(ctx->ctx, params);
+  });
+}
+
+int libsupermesh_gpu_predictor_get(libsupermesh_gpu_context* ctx, float* params) {
+  return guarded([&] {
+    LSM_REQUIRE(ctx != nullptr && params != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(ctx->ctx.mu);
+    LSM_HIP(hipSetDevice(ctx->ctx.device));
+    predictor_get(ctx->ctx, params);
+  });
+}
+
+int libsupermesh_gpu_predictor_quantile(libsupermesh_gpu_context* ctx, float q) {
+  return guarded([&] {
+    LSM_REQUIRE(ctx != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID, "null context");
+    LSM_REQUIRE(q > 0.f && q <= 1.f, LIBSUPERMESH_GPU_ERROR_INVALID, "quantile must be in (0, 1]");
+    std::lock_guard<std::mutex> lk(ctx->ctx.mu);
+    ctx->ctx.predictor.quantile = q;
+  });
+}
+
+int libsupermesh_gpu_predictor_collect(libsupermesh_gpu_context* ctx, const libsupermesh_gpu_mesh* a,
+                                       const libsupermesh_gpu_mesh* b, long max_samples,
+                                       long* nsamples) {
+  return guarded([&] {
+    LSM_REQUIRE(ctx != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID, "null context");
+    LSM_REQUIRE(a != nullptr && b != nullptr && a->dim == 3 && b->dim == 3, LIBSUPERMESH_GPU_ERROR_INVALID,
+                "the predictor learns 3D (tet) pairs");
+    Context& C = ctx->ctx;
+    std::lock_guard<std::mutex> lk(C.mu);
+    libsupermesh_gpu_result r;
+    run_supermesh(C, a, b, LIBSUPERMESH_GPU_FINDER_TREE, 0, true, &r);
+    const int* work = nullptr;
+    const long nwork = r.npairs > 0 ? last_work_list(C, &work) : 0;
+    predictor_collect(C, nwork, work, r.rows, r.indices, r.counts,
+                      static_cast<const double*>(C.mesh_a.coords.ptr),
+                      static_cast<const double*>(C.mesh_b.coords.ptr), max_samples);
+    if (nsamples) *nsamples = C.predictor.nsamples;
+  });
+}
+
+int libsupermesh_gpu_predictor_train(libsupermesh_gpu_context* ctx, int epochs, int batch, float lr,
+                                     unsigned seed, float* loss_per_epoch) {
+  return guarded([&] {
+    LSM_REQUIRE(ctx != nullptr && epochs >= 0, LIBSUPERMESH_GPU_ERROR_INVALID, "invalid argument");
+    std::lock_guard<std::mutex> lk(ctx->ctx.mu);
+    LSM_HIP(hipSetDevice(ctx->ctx.device));
+    predictor_train(ctx->ctx, epochs, batch, lr, seed, loss_per_epoch);
+  });
+}
+
+int libsupermesh_gpu_predictor_infer(libsupermesh_gpu_context* ctx, const libsupermesh_gpu_mesh* a,
+                                     const libsupermesh_gpu_mesh* b, long npairs, const int* pair_a,
+                                     const int* pair_b, int* budgets) {
+  return guarded([&] {
+    LSM_REQUIRE(ctx != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID, "null context");
+    check_mesh(a, "a");
+    check_mesh(b, "b");
+    LSM_REQUIRE(a->dim == 3 && b->dim == 3, LIBSUPERMESH_GPU_ERROR_INVALID, "the predictor takes tet pairs");
+    LSM_REQUIRE(npairs >= 0 && (npairs == 0 || (pair_a && pair_b && budgets)),
+                LIBSUPERMESH_GPU_ERROR_INVALID, "invalid pair list");
+    Context& C = ctx->ctx;
+    std::lock_guard<std::mutex> lk(C.mu);
+    LSM_HIP(hipSetDevice(C.device));
+    LSM_REQUIRE(C.predictor.ready, LIBSUPERMESH_GPU_ERROR_STATE, "no predictor weights loaded");
+    stage_mesh(C, C.mesh_a, 3, 4, a->nnodes, a->nelements, a->positions, a->enlist, false, false);
+    stage_mesh(C, C.mesh_b, 3, 4, b->nnodes, b->nelements, b->positions, b->enlist, false, false);
+    PredictArgs pa = {npairs, nullptr, pair_a, pair_b, static_cast<const double*>(C.mesh_a.coords.ptr),
+                      static_cast<const double*>(C.mesh_b.coords.ptr),
+                      static_cast<const __bf16*>(C.predictor.weights.ptr),
+                      static_cast<const float*>(C.predictor.bias.ptr), C.predictor.quantile, budgets};
+    predict_budgets(C, pa);
+    LSM_HIP(hipStreamSynchronize(C.stream));
+  });
+}
+
+int libsupermesh_gpu_predictor_add_samples(libsupermesh_gpu_context* ctx, long n, const float* features,
+                                           const int* counts) {
+  return guarded([&] {
+    LSM_REQUIRE(ctx != nullptr && n >= 0 && (n == 0 || (features && counts)), LIBSUPERMESH_GPU_ERROR_INVALID,
+                "invalid samples");
+    std::lock_guard<std::mutex> lk(ctx->ctx.mu);
+    LSM_HIP(hipSetDevice(ctx->ctx.device));
+    predictor_add_samples(ctx->ctx, n, features, counts);
+  });
+}
+
+int libsupermesh_gpu_predictor_clear(libsupermesh_gpu_context* ctx) {
+  return guarded([&] {
+    LSM_REQUIRE(ctx != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID, "null context");
+    std::lock_guard<std::mutex> lk(ctx->ctx.mu);
+    ctx->ctx.predictor.nsamples = 0;
   });
 }
 

@@ -34,13 +34,15 @@ constexpr int kBlock = 256;
 // Mesh staging
 // ---------------------------------------------------------------------------
 
-template <int DIM>
+// Elements are simplices (LOC = DIM + 1) or, for the candidate finder only,
+// quads / hexahedra (LOC = 2^DIM): those get coordinates, boxes and scale.
+template <int DIM, int LOC>
 __global__ void __launch_bounds__(kBlock)
 k_stage(long ne, long nnodes, const double* __restrict__ pos, const int* __restrict__ enl,
         double* __restrict__ coords, double* __restrict__ bbox, double* __restrict__ sp,
         double* __restrict__ scale, double* __restrict__ planes, double* __restrict__ affine,
         int* __restrict__ bad) {
-  constexpr int LOC = DIM + 1;
+  constexpr bool SIMPLEX = LOC == DIM + 1;
   long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= ne) return;
   double c[LOC * DIM];
@@ -72,6 +74,10 @@ k_stage(long ne, long nnodes, const double* __restrict__ pos, const int* __restr
     bbox[(long)d * ne + e] = lo;
     bbox[(long)(DIM + d) * ne + e] = hi;
   }
+  if constexpr (!SIMPLEX) {
+    sp[e] = 0.0;
+    return;
+  } else {
   double m;
   if constexpr (DIM == 2) m = triangle_area(c);
   else m = tet_volume_tet(c);
@@ -124,13 +130,17 @@ k_stage(long ne, long nnodes, const double* __restrict__ pos, const int* __restr
       out[11] = c22 / det;
     }
   }
+  }
 }
 
 void stage_mesh(Context& C, DevMesh& M, int dim, int loc, long nnodes, long ne,
                 const double* positions, const int* enlist, bool need_planes, bool need_affine) {
   LSM_REQUIRE(dim == 2 || dim == 3, LIBSUPERMESH_GPU_ERROR_INVALID, "Invalid dimension");
-  LSM_REQUIRE(loc == dim + 1, LIBSUPERMESH_GPU_ERROR_INVALID,
-              "Unsupported element type: only simplices (loc = dim + 1)");
+  const bool simplex = loc == dim + 1;
+  LSM_REQUIRE(simplex || loc == (1 << dim), LIBSUPERMESH_GPU_ERROR_INVALID,
+              "Unsupported element type: simplices, quads or hexahedra only");
+  LSM_REQUIRE(simplex || (!need_planes && !need_affine), LIBSUPERMESH_GPU_ERROR_INVALID,
+              "Unsupported element type: clipping and mass take simplices only");
   LSM_REQUIRE(nnodes >= 0 && ne >= 0, LIBSUPERMESH_GPU_ERROR_INVALID, "Negative mesh size");
   LSM_REQUIRE(ne < (1L << 31) && nnodes < (1L << 31), LIBSUPERMESH_GPU_ERROR_OVERFLOW,
               "Mesh larger than 2^31 elements or nodes");
@@ -150,12 +160,17 @@ void stage_mesh(Context& C, DevMesh& M, int dim, int loc, long nnodes, long ne,
   int* bad = C.scalars.as<int>(64);
   LSM_HIP(hipMemsetAsync(bad, 0, sizeof(int), C.stream));
   if (ne > 0) {
-    if (dim == 2)
-      hipLaunchKernelGGL(k_stage<2>, dim3(grid_for(ne, kBlock)), dim3(kBlock), 0, C.stream, ne,
-                         nnodes, positions, enlist, coords, bbox, sp, scale, planes, affine, bad);
-    else
-      hipLaunchKernelGGL(k_stage<3>, dim3(grid_for(ne, kBlock)), dim3(kBlock), 0, C.stream, ne,
-                         nnodes, positions, enlist, coords, bbox, sp, scale, planes, affine, bad);
+#define LSM_STAGE(D, L)                                                                         \
+  hipLaunchKernelGGL((k_stage<D, L>), dim3(grid_for(ne, kBlock)), dim3(kBlock), 0, C.stream, ne,  \
+                     nnodes, positions, enlist, coords, bbox, sp, scale, planes, affine, bad)
+    if (dim == 2) {
+      if (simplex) LSM_STAGE(2, 3);
+      else LSM_STAGE(2, 4);
+    } else {
+      if (simplex) LSM_STAGE(3, 4);
+      else LSM_STAGE(3, 8);
+    }
+#undef LSM_STAGE
     LSM_HIP(hipGetLastError());
   }
   int hbad = 0;

@@ -100,6 +100,19 @@ struct DevBVH {
   DevBuf granges;           // int [total][2]: lowest and highest mesh id below the node
 };
 
+// Learned output-size predictor state (predictor.hip).
+struct Predictor {
+  bool ready = false;
+  long steps = 0;           // Adam steps taken
+  float quantile = 0.99f;   // budget = this quantile of the predicted count
+  DevBuf weights, bias;     // BF16 layouts, FP32 biases [4][64]
+  DevBuf master, adam_m, adam_v, grad;
+  DevBuf features, labels;  // collected training samples
+  long nsamples = 0;
+  DevBuf act, loss, index;  // training scratch
+  DevBuf budget, slots, slot_elems, overflow;  // single-pass clip buffers
+};
+
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -121,6 +134,7 @@ struct Context {
   DevBuf one_in, one_out;
   void* pinned = nullptr;
   size_t pinned_bytes = 0;
+  Predictor predictor;
   // timing
   hipEvent_t ev[16];
   bool events = false;
@@ -144,6 +158,8 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
                 libsupermesh_gpu_result* out);
 void assemble_mass(Context& C, const DevMesh& A, const DevMesh& B, long npairs,
                    libsupermesh_gpu_result* out);
+// Work list (pairs that needed the full clip) of the last 3D clip_pairs call.
+long last_work_list(Context& C, const int** work);
 // Batches: per-mesh candidate pair offsets (gpairs, from the CSR row pointers
 // at the A mesh offsets) and compensated measure sums per mesh (gsum).
 void batch_sums(Context& C, long ngroups, const long* a_offsets, const long* indptr,

@@ -1,0 +1,580 @@
+// predictor.hip -- learned output-size predictor for the single-pass tet clip.
+//
+// A small MLP, 24 -> 64 -> 64 -> 64 -> 33 with ReLU, maps a candidate pair
+// (both tets' vertices, normalised to the pair's joint bounding box) to a
+// distribution over its output count 0..32 (class 32 = "32 or more").  The
+// weights are BF16 and every product accumulates in FP32 on the MFMA units
+// (v_mfma_f32_32x32x16_bf16): a wave computes a 32-pair x 32-unit tile per
+// instruction sequence.  Here:
+//   * k_predict: features + forward + softmax; writes each pair's output
+//     budget, the smallest count whose predicted cumulative probability
+//     reaches the quantile q (81 = the tet clip's maximum when that is the
+//     last class);
+//   * k_train_step / k_train_wgrad / k_train_adam: one minibatch of training
+//     (forward, softmax cross-entropy, backward through the ReLU layers,
+//     weight gradients as MFMA reductions over the batch, Adam in FP32 on the
+//     master weights, then the BF16 copies the MFMA kernels read).
+// The budgets only size the output; the clip itself is the exact FP64 walk,
+// and pairs whose output exceeds their budget are clipped again by the plain
+// fill pass, so results never depend on the predictor.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "lsm_geometry.hpp"
+#include "lsm_runtime.hpp"
+#include "predictor.hpp"
+
+namespace lsm {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// BF16 weight layouts (element offsets into one array):
+//   forward, [out][in] with in padded to 32 / 64: F1 [64][32], F2 [64][64],
+//   F3 [64][64], F4 [64][64] (rows >= 33 zero);
+//   backward, [in][out]: G2 [64][64], G3 [64][64], G4 [64][64] (cols >= 33 zero).
+constexpr int kF1 = 0, kF2 = kF1 + 64 * 32, kF3 = kF2 + 64 * 64, kF4 = kF3 + 64 * 64;
+constexpr int kFwdSize = kF4 + 64 * 64;  // 14336
+constexpr int kG2 = kFwdSize, kG3 = kG2 + 64 * 64, kG4 = kG3 + 64 * 64;
+constexpr int kAllSize = kG4 + 64 * 64;
+constexpr int kLd = 64;  // row stride (elements) of activation tiles in LDS
+
+__device__ __forceinline__ int lane64() { return threadIdx.x & 63; }
+
+// Row of accumulator element r in a 32x32 MFMA result (column = lane & 31).
+__device__ __forceinline__ int acc_row(int r) { return 8 * (r >> 2) + 4 * (lane64() >> 5) + (r & 3); }
+
+// acc (32 x 32) += A (32 x K) * B (K x 32).  A: rows at a + i * lda;
+// B: given by columns, column j at bt + j * ldb, contiguous in k.
+__device__ __forceinline__ f32x16 mma_tile(const __bf16* a, int lda, const __bf16* bt, int ldb, int K,
+                                           f32x16 acc) {
+  const int i = lane64() & 31, h = (lane64() >> 5) * 8;
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    const bf16x8 va = *reinterpret_cast<const bf16x8*>(a + i * lda + k0 + h);
+    const bf16x8 vb = *reinterpret_cast<const bf16x8*>(bt + i * ldb + k0 + h);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(va, vb, acc, 0, 0, 0);
+  }
+  return acc;
+}
+
+// Y (32 x 64, bf16) = act(X (32 x K) W^T + b); W in the forward layout.
+template <int K, bool RELU>
+__device__ __forceinline__ void dense(const __bf16* X, const __bf16* W, const float* bias, __bf16* Y) {
+#pragma unroll
+  for (int t = 0; t < 2; t++) {
+    f32x16 acc = {};
+    acc = mma_tile(X, kLd, W + 32 * t * K, K, K, acc);
+    const int j = 32 * t + (lane64() & 31);
+    const float bj = bias[j];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      float v = acc[r] + bj;
+      if (RELU) v = v > 0.f ? v : 0.f;
+      Y[acc_row(r) * kLd + j] = (__bf16)v;
+    }
+  }
+}
+
+// Features of pair (a, b): the 8 vertices normalised to their joint box.
+__device__ __forceinline__ void pair_features(const double* __restrict__ ta, const double* __restrict__ tb,
+                                              __bf16* row) {
+  double lo[3], hi[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    lo[d] = fmin_f(fmin_f(ta[d], ta[3 + d]), fmin_f(ta[6 + d], ta[9 + d]));
+    hi[d] = fmax_f(fmax_f(ta[d], ta[3 + d]), fmax_f(ta[6 + d], ta[9 + d]));
+    lo[d] = fmin_f(lo[d], fmin_f(fmin_f(tb[d], tb[3 + d]), fmin_f(tb[6 + d], tb[9 + d])));
+    hi[d] = fmax_f(hi[d], fmax_f(fmax_f(tb[d], tb[3 + d]), fmax_f(tb[6 + d], tb[9 + d])));
+  }
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+    const int d = k % 3;
+    const double s = hi[d] > lo[d] ? 1.0 / (hi[d] - lo[d]) : 0.0;
+    row[k] = (__bf16)(float)((ta[k] - lo[d]) * s);
+    row[12 + k] = (__bf16)(float)((tb[k] - lo[d]) * s);
+  }
+#pragma unroll
+  for (int k = 24; k < 32; k++) row[k] = (__bf16)0.f;
+}
+
+// Smallest count whose cumulative predicted probability reaches q.
+__device__ __forceinline__ int budget_of(const float* logit, float q) {
+  float m = logit[0];
+  for (int c = 1; c < 33; c++) m = fmaxf(m, logit[c]);
+  float s = 0.f;
+  for (int c = 0; c < 33; c++) s += __expf(logit[c] - m);
+  const float target = q * s;
+  float acc = 0.f;
+  for (int c = 0; c < 32; c++) {
+    acc += __expf(logit[c] - m);
+    if (acc >= target) return c;
+  }
+  return kTetBufSize;  // "32 or more": the clip's maximum
+}
+
+// ---------------------------------------------------------------------------
+// Inference
+// ---------------------------------------------------------------------------
+
+constexpr int kPredWaves = 4;
+
+__global__ void __launch_bounds__(64 * kPredWaves) k_predict(PredictArgs g) {
+  __shared__ __align__(16) __bf16 sW[kFwdSize];
+  __shared__ float sB[4 * 64];
+  __shared__ __align__(16) __bf16 sX[kPredWaves][2][32 * kLd];
+  for (int i = threadIdx.x; i < kFwdSize; i += blockDim.x) sW[i] = g.weights[i];
+  for (int i = threadIdx.x; i < 4 * 64; i += blockDim.x) sB[i] = g.bias[i];
+  const int wv = threadIdx.x >> 6, l = lane64();
+  __bf16* X0 = sX[wv][0];
+  __bf16* X1 = sX[wv][1];
+  float* L = reinterpret_cast<float*>(sX[wv][0]);  // 32 x 33 logits over both buffers
+  const long ntiles = (g.nwork + 31) / 32;
+  // uniform trip count per block, so the block barriers below are safe
+  const long per_iter = (long)gridDim.x * kPredWaves;
+  const long iters = (ntiles + per_iter - 1) / per_iter;
+  __syncthreads();
+  for (long it = 0; it < iters; it++) {
+    const long tile = (it * gridDim.x + blockIdx.x) * kPredWaves + wv;
+    const long w = tile * 32 + (l & 31);
+    const bool valid = tile < ntiles && w < g.nwork;
+    if (l < 32) {
+      __bf16* row = X0 + l * kLd;
+      if (valid) {
+        const long p = g.work ? g.work[w] : w;
+        pair_features(g.acoords + (long)g.rows[p] * 12, g.bcoords + (long)g.cols[p] * 12, row);
+      } else {
+        for (int k = 0; k < 32; k++) row[k] = (__bf16)0.f;
+      }
+    }
+    __syncthreads();
+    dense<32, true>(X0, sW + kF1, sB, X1);
+    __syncthreads();
+    dense<64, true>(X1, sW + kF2, sB + 64, X0);
+    __syncthreads();
+    dense<64, true>(X0, sW + kF3, sB + 128, X1);
+    __syncthreads();
+    f32x16 a0 = {}, a1 = {};
+    a0 = mma_tile(X1, kLd, sW + kF4, 64, 64, a0);
+    a1 = mma_tile(X1, kLd, sW + kF4 + 32 * 64, 64, 64, a1);
+    __syncthreads();
+    {
+      const int j = l & 31;
+#pragma unroll
+      for (int r = 0; r < 16; r++) L[acc_row(r) * 33 + j] = a0[r] + sB[192 + j];
+      if (j == 0) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) L[acc_row(r) * 33 + 32] = a1[r] + sB[192 + 32];
+      }
+    }
+    __syncthreads();
+    if (l < 32 && valid) g.budget[w] = budget_of(L + l * 33, g.quantile);
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Training
+// ---------------------------------------------------------------------------
+
+constexpr int kTrainWaves = 2;
+
+// Forward and backward of one 32-sample tile per wave.  Writes the transposed
+// activations HT_l [64][B] (l = 0: the features, [32][B]) and output-side
+// gradients DT_l [64][B] (l = 1..4) that the weight-gradient kernel reduces.
+__global__ void __launch_bounds__(64 * kTrainWaves) k_train_step(TrainArgs g) {
+  __shared__ __align__(16) __bf16 sH[kTrainWaves][4][32 * kLd];  // H1 H2 H3 D
+  __shared__ float sL[kTrainWaves][32 * 33];
+  const int wv = threadIdx.x >> 6, l = lane64();
+  // the batch is a multiple of 32 * kTrainWaves: every wave has a tile
+  const long tile = (long)blockIdx.x * kTrainWaves + wv;
+  const long base = tile * 32;
+  __bf16* H1 = sH[wv][0];
+  __bf16* H2 = sH[wv][1];
+  __bf16* H3 = sH[wv][2];
+  __bf16* D = sH[wv][3];
+  float* L = sL[wv];
+  const __bf16* W = g.weights;
+  const long B = g.batch;
+  // features of the samples -> D (as X0) and HT_0
+  if (l < 32) {
+    const long s = g.index[base + l];
+    for (int k = 0; k < 32; k++) {
+      const __bf16 v = g.features[s * 32 + k];
+      D[l * kLd + k] = v;
+      g.ht0[(long)k * B + base + l] = v;
+    }
+  }
+  __syncthreads();
+  dense<32, true>(D, W + kF1, g.bias, H1);
+  __syncthreads();
+  dense<64, true>(H1, W + kF2, g.bias + 64, H2);
+  __syncthreads();
+  dense<64, true>(H2, W + kF3, g.bias + 128, H3);
+  __syncthreads();
+  f32x16 a0 = {}, a1 = {};
+  a0 = mma_tile(H3, kLd, W + kF4, 64, 64, a0);
+  a1 = mma_tile(H3, kLd, W + kF4 + 32 * 64, 64, 64, a1);
+  {
+    const int j = l & 31;
+#pragma unroll
+    for (int r = 0; r < 16; r++) L[acc_row(r) * 33 + j] = a0[r] + g.bias[192 + j];
+    if (j == 0) {
+#pragma unroll
+      for (int r = 0; r < 16; r++) L[acc_row(r) * 33 + 32] = a1[r] + g.bias[192 + 32];
+    }
+  }
+  // transposed activations for the weight gradients
+  for (int k = 0; k < 64; k++) {
+    if (l < 32) {
+      g.ht1[(long)k * B + base + l] = H1[l * kLd + k];
+      g.ht2[(long)k * B + base + l] = H2[l * kLd + k];
+      g.ht3[(long)k * B + base + l] = H3[l * kLd + k];
+    }
+  }
+  __syncthreads();
+  // softmax cross-entropy: D = dL/dlogits = (p - onehot) / B
+  if (l < 32) {
+    const float* row = L + l * 33;
+    const int y = g.labels[g.index[base + l]];
+    float m = row[0];
+    for (int c = 1; c < 33; c++) m = fmaxf(m, row[c]);
+    float s = 0.f;
+    for (int c = 0; c < 33; c++) s += __expf(row[c] - m);
+    const float inv = 1.f / s;
+    for (int c = 0; c < 64; c++) {
+      float d = 0.f;
+      if (c < 33) d = (__expf(row[c] - m) * inv - (c == y ? 1.f : 0.f)) / (float)B;
+      const __bf16 v = (__bf16)d;
+      D[l * kLd + c] = v;
+      g.dt4[(long)c * B + base + l] = v;
+    }
+    atomicAdd(g.loss, (m + __logf(s)) - row[y]);
+  }
+  __syncthreads();
+  // back through layers 4, 3, 2: dZ_{l} = (dZ_{l+1} W_{l+1}) * (H_l > 0)
+  const __bf16* Gs[3] = {W + kG4, W + kG3, W + kG2};
+  __bf16* Hs[3] = {H3, H2, H1};
+  __bf16* DT[3] = {g.dt3, g.dt2, g.dt1};
+  for (int s = 0; s < 3; s++) {
+    f32x16 c0 = {}, c1 = {};
+    c0 = mma_tile(D, kLd, Gs[s], 64, 64, c0);
+    c1 = mma_tile(D, kLd, Gs[s] + 32 * 64, 64, 64, c1);
+    __syncthreads();
+    const int j = l & 31;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const int i = acc_row(r);
+      const float h0 = (float)Hs[s][i * kLd + j], h1 = (float)Hs[s][i * kLd + 32 + j];
+      D[i * kLd + j] = (__bf16)(h0 > 0.f ? c0[r] : 0.f);
+      D[i * kLd + 32 + j] = (__bf16)(h1 > 0.f ? c1[r] : 0.f);
+    }
+    __syncthreads();
+    for (int k = 0; k < 64; k++)
+      if (l < 32) DT[s][(long)k * B + base + l] = D[l * kLd + k];
+    __syncthreads();
+  }
+}
+
+// Weight gradients: grad_l [out][in] = sum_k DT_l[out][k] HT_{l-1}[in][k],
+// one wave per 32 x 32 tile; bias gradients from the first column tile.
+__global__ void __launch_bounds__(64) k_train_wgrad(TrainArgs g) {
+  // tile list: layer 1: 2 (out 64 x in 32), layers 2..4: 4 each
+  int t = blockIdx.x, layer, mt, nt;
+  if (t < 2) {
+    layer = 0;
+    mt = t;
+    nt = 0;
+  } else {
+    t -= 2;
+    layer = 1 + t / 4;
+    mt = (t % 4) / 2;
+    nt = t % 2;
+  }
+  const __bf16* dts[4] = {g.dt1, g.dt2, g.dt3, g.dt4};
+  const __bf16* hts[4] = {g.ht0, g.ht1, g.ht2, g.ht3};
+  const int nin = layer == 0 ? 24 : 64, nout = layer == 3 ? 33 : 64;
+  const int woff[4] = {kOffW1, kOffW2, kOffW3, kOffW4}, boff[4] = {kOffB1, kOffB2, kOffB3, kOffB4};
+  const long B = g.batch;
+  const __bf16* A = dts[layer] + (long)(32 * mt) * B;
+  const __bf16* Bt = hts[layer] + (long)(32 * nt) * B;
+  f32x16 acc = {};
+  acc = mma_tile(A, (int)B, Bt, (int)B, (int)B, acc);
+  const int j = 32 * nt + (lane64() & 31);
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    const int o = 32 * mt + acc_row(r);
+    if (o < nout && j < nin) g.grad[woff[layer] + o * nin + j] = acc[r];
+  }
+  if (nt == 0 && lane64() < 32) {
+    const int o = 32 * mt + lane64();
+    if (o < nout) {
+      float s = 0.f;
+      const __bf16* row = dts[layer] + (long)o * B;
+      for (long k = 0; k < B; k++) s += (float)row[k];
+      g.grad[boff[layer] + o] = s;
+    }
+  }
+}
+
+// Writes the BF16 forward / backward copies and biases of parameter i.
+__device__ __forceinline__ void publish(int i, float w, __bf16* W, float* bias) {
+  const __bf16 v = (__bf16)w;
+  if (i < kOffB1) {
+    const int o = (i - kOffW1) / 24, n = (i - kOffW1) % 24;
+    W[kF1 + o * 32 + n] = v;
+  } else if (i < kOffW2) {
+    bias[i - kOffB1] = w;
+  } else if (i < kOffB2) {
+    const int o = (i - kOffW2) / 64, n = (i - kOffW2) % 64;
+    W[kF2 + o * 64 + n] = v;
+    W[kG2 + n * 64 + o] = v;
+  } else if (i < kOffW3) {
+    bias[64 + i - kOffB2] = w;
+  } else if (i < kOffB3) {
+    const int o = (i - kOffW3) / 64, n = (i - kOffW3) % 64;
+    W[kF3 + o * 64 + n] = v;
+    W[kG3 + n * 64 + o] = v;
+  } else if (i < kOffW4) {
+    bias[128 + i - kOffB3] = w;
+  } else if (i < kOffB4) {
+    const int o = (i - kOffW4) / 64, n = (i - kOffW4) % 64;
+    W[kF4 + o * 64 + n] = v;
+    W[kG4 + n * 64 + o] = v;
+  } else {
+    bias[192 + i - kOffB4] = w;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_train_adam(TrainArgs g, float lr, float c1, float c2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= kNumParams) return;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const float gr = g.grad[i];
+  const float m = b1 * g.adam_m[i] + (1.f - b1) * gr;
+  const float v = b2 * g.adam_v[i] + (1.f - b2) * gr * gr;
+  g.adam_m[i] = m;
+  g.adam_v[i] = v;
+  const float w = g.master[i] - lr * (m / c1) / (sqrtf(v / c2) + eps);
+  g.master[i] = w;
+  publish(i, w, g.weights, g.bias);
+}
+
+__global__ void __launch_bounds__(256) k_publish(const float* __restrict__ master, __bf16* W, float* bias) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < kNumParams) publish(i, master[i], W, bias);
+}
+
+__global__ void __launch_bounds__(256)
+k_zero_bf16(long n, __bf16* __restrict__ p) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = (__bf16)0.f;
+}
+
+// Features and labels of sampled work pairs (one thread per sample).
+__global__ void __launch_bounds__(256)
+k_collect(long n, long stride, const int* __restrict__ work, const int* __restrict__ rows,
+          const int* __restrict__ cols, const int* __restrict__ counts, const double* __restrict__ ac,
+          const double* __restrict__ bc, __bf16* __restrict__ features, int* __restrict__ labels) {
+  const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const long p = work[s * stride];
+  __bf16 row[32];
+  pair_features(ac + (long)rows[p] * 12, bc + (long)cols[p] * 12, row);
+  for (int k = 0; k < 32; k++) features[s * 32 + k] = row[k];
+  const int c = counts[p];
+  labels[s] = c < 32 ? c : 32;
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+static void ensure_weights(Context& C) {
+  Predictor& P = C.predictor;
+  if (P.weights.ptr) return;
+  __bf16* W = P.weights.as<__bf16>(kAllSize);
+  hipLaunchKernelGGL(k_zero_bf16, dim3(grid_for(kAllSize, 256)), dim3(256), 0, C.stream, (long)kAllSize, W);
+  P.bias.as<float>(256);
+  LSM_HIP(hipMemsetAsync(P.bias.ptr, 0, 256 * sizeof(float), C.stream));
+  LSM_HIP(hipGetLastError());
+}
+
+void predictor_set(Context& C, const float* params) {
+  Predictor& P = C.predictor;
+  ensure_weights(C);
+  float* m = P.master.as<float>(kNumParams);
+  LSM_HIP(hipMemcpyAsync(m, params, sizeof(float) * kNumParams, hipMemcpyHostToDevice, C.stream));
+  LSM_HIP(hipMemsetAsync(P.adam_m.as<float>(kNumParams), 0, sizeof(float) * kNumParams, C.stream));
+  LSM_HIP(hipMemsetAsync(P.adam_v.as<float>(kNumParams), 0, sizeof(float) * kNumParams, C.stream));
+  hipLaunchKernelGGL(k_publish, dim3(grid_for(kNumParams, 256)), dim3(256), 0, C.stream, m,
+                     static_cast<__bf16*>(P.weights.ptr), static_cast<float*>(P.bias.ptr));
+  LSM_HIP(hipGetLastError());
+  LSM_HIP(hipStreamSynchronize(C.stream));
+  P.ready = true;
+  P.steps = 0;
+}
+
+void predictor_get(Context& C, float* params) {
+  Predictor& P = C.predictor;
+  LSM_REQUIRE(P.ready, LIBSUPERMESH_GPU_ERROR_STATE, "no predictor weights loaded");
+  LSM_HIP(hipMemcpyAsync(params, P.master.ptr, sizeof(float) * kNumParams, hipMemcpyDeviceToHost, C.stream));
+  LSM_HIP(hipStreamSynchronize(C.stream));
+}
+
+void predictor_init(Context& C, unsigned seed) {
+  // He-uniform weights, zero biases
+  std::vector<float> p(kNumParams, 0.f);
+  std::mt19937 rng(seed);
+  auto fill = [&](int off, int nout, int nin) {
+    const float a = std::sqrt(6.f / (float)nin);
+    std::uniform_real_distribution<float> U(-a, a);
+    for (int i = 0; i < nout * nin; i++) p[off + i] = U(rng);
+  };
+  fill(kOffW1, 64, 24);
+  fill(kOffW2, 64, 64);
+  fill(kOffW3, 64, 64);
+  fill(kOffW4, 33, 64);
+  predictor_set(C, p.data());
+}
+
+static long reserve_samples(Context& C, long extra);
+
+void predictor_collect(Context& C, long nwork, const int* work, const int* rows, const int* cols,
+                       const int* counts, const double* acoords, const double* bcoords, long max_samples) {
+  Predictor& P = C.predictor;
+  if (nwork <= 0 || max_samples <= 0) return;
+  const long stride = (nwork + max_samples - 1) / max_samples;
+  const long n = (nwork + stride - 1) / stride;
+  const long old = reserve_samples(C, n);
+  __bf16* f = static_cast<__bf16*>(P.features.ptr) + 32 * old;
+  int* lab = static_cast<int*>(P.labels.ptr) + old;
+  hipLaunchKernelGGL(k_collect, dim3(grid_for(n, 256)), dim3(256), 0, C.stream, n, stride, work, rows, cols,
+                     counts, acoords, bcoords, f, lab);
+  LSM_HIP(hipGetLastError());
+  LSM_HIP(hipStreamSynchronize(C.stream));
+  P.nsamples = old + n;
+}
+
+__global__ void __launch_bounds__(256)
+k_samples_from_host(long n, const float* __restrict__ f, const int* __restrict__ y, __bf16* __restrict__ features,
+                    int* __restrict__ labels) {
+  const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  for (int k = 0; k < 32; k++) features[s * 32 + k] = (__bf16)(k < 24 ? f[s * 24 + k] : 0.f);
+  const int c = y[s];
+  labels[s] = c < 0 ? 0 : (c < 32 ? c : 32);
+}
+
+// Grows the sample store to hold `extra` more samples; returns the old count.
+static long reserve_samples(Context& C, long extra) {
+  Predictor& P = C.predictor;
+  const long old = P.nsamples;
+  const size_t fbytes = sizeof(__bf16) * 32 * (size_t)(old + extra), lbytes = sizeof(int) * (size_t)(old + extra);
+  if (fbytes > P.features.bytes || lbytes > P.labels.bytes) {
+    DevBuf nf, nl;
+    nf.get(fbytes);
+    nl.get(lbytes);
+    if (old > 0) {
+      LSM_HIP(hipMemcpyAsync(nf.ptr, P.features.ptr, sizeof(__bf16) * 32 * old, hipMemcpyDeviceToDevice, C.stream));
+      LSM_HIP(hipMemcpyAsync(nl.ptr, P.labels.ptr, sizeof(int) * old, hipMemcpyDeviceToDevice, C.stream));
+    }
+    LSM_HIP(hipStreamSynchronize(C.stream));
+    P.features.release();
+    P.labels.release();
+    P.features = nf;
+    P.labels = nl;
+  }
+  return old;
+}
+
+void predictor_add_samples(Context& C, long n, const float* features, const int* labels) {
+  if (n <= 0) return;
+  Predictor& P = C.predictor;
+  const long old = reserve_samples(C, n);
+  DevBuf hf, hy;
+  float* df = static_cast<float*>(hf.get(sizeof(float) * 24 * n));
+  int* dy = static_cast<int*>(hy.get(sizeof(int) * n));
+  LSM_HIP(hipMemcpyAsync(df, features, sizeof(float) * 24 * n, hipMemcpyHostToDevice, C.stream));
+  LSM_HIP(hipMemcpyAsync(dy, labels, sizeof(int) * n, hipMemcpyHostToDevice, C.stream));
+  hipLaunchKernelGGL(k_samples_from_host, dim3(grid_for(n, 256)), dim3(256), 0, C.stream, n, df, dy,
+                     static_cast<__bf16*>(P.features.ptr) + 32 * old, static_cast<int*>(P.labels.ptr) + old);
+  LSM_HIP(hipGetLastError());
+  LSM_HIP(hipStreamSynchronize(C.stream));
+  hf.release();
+  hy.release();
+  P.nsamples = old + n;
+}
+
+void predictor_train(Context& C, int epochs, int batch, float lr, unsigned seed, float* loss_out) {
+  Predictor& P = C.predictor;
+  if (!P.ready) predictor_init(C, seed);
+  LSM_REQUIRE(batch >= 64 && batch % 64 == 0, LIBSUPERMESH_GPU_ERROR_INVALID,
+              "training batch must be a positive multiple of 64");
+  LSM_REQUIRE(P.nsamples >= batch, LIBSUPERMESH_GPU_ERROR_INVALID, "fewer training samples than one batch");
+  const long n = P.nsamples, steps = n / batch;
+  TrainArgs g = {};
+  g.batch = batch;
+  g.features = static_cast<const __bf16*>(P.features.ptr);
+  g.labels = static_cast<const int*>(P.labels.ptr);
+  g.weights = static_cast<__bf16*>(P.weights.ptr);
+  g.bias = static_cast<float*>(P.bias.ptr);
+  g.master = static_cast<float*>(P.master.ptr);
+  g.adam_m = static_cast<float*>(P.adam_m.ptr);
+  g.adam_v = static_cast<float*>(P.adam_v.ptr);
+  g.grad = P.grad.as<float>(kNumParams);
+  LSM_HIP(hipMemsetAsync(g.grad, 0, sizeof(float) * kNumParams, C.stream));
+  __bf16* act = P.act.as<__bf16>((size_t)batch * (32 + 64 * 3 + 64 * 4));
+  g.ht0 = act;
+  g.ht1 = g.ht0 + (size_t)32 * batch;
+  g.ht2 = g.ht1 + (size_t)64 * batch;
+  g.ht3 = g.ht2 + (size_t)64 * batch;
+  g.dt1 = g.ht3 + (size_t)64 * batch;
+  g.dt2 = g.dt1 + (size_t)64 * batch;
+  g.dt3 = g.dt2 + (size_t)64 * batch;
+  g.dt4 = g.dt3 + (size_t)64 * batch;
+  float* dloss = P.loss.as<float>(1);
+  int* didx = P.index.as<int>((size_t)n);
+  std::vector<int> perm((size_t)n);
+  for (long i = 0; i < n; i++) perm[i] = (int)i;
+  std::mt19937 rng(seed * 2654435761u + 1u);
+  const float b1 = 0.9f, b2 = 0.999f;
+  for (int e = 0; e < epochs; e++) {
+    std::shuffle(perm.begin(), perm.end(), rng);
+    LSM_HIP(hipMemcpyAsync(didx, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, C.stream));
+    LSM_HIP(hipMemsetAsync(dloss, 0, sizeof(float), C.stream));
+    g.loss = dloss;
+    for (long s = 0; s < steps; s++) {
+      g.index = didx + s * batch;
+      hipLaunchKernelGGL(k_train_step, dim3((unsigned)(batch / (32 * kTrainWaves))),
+                         dim3(64 * kTrainWaves), 0, C.stream, g);
+      hipLaunchKernelGGL(k_train_wgrad, dim3(14), dim3(64), 0, C.stream, g);
+      P.steps++;
+      const float c1 = 1.f - std::pow(b1, (float)P.steps), c2 = 1.f - std::pow(b2, (float)P.steps);
+      hipLaunchKernelGGL(k_train_adam, dim3(grid_for(kNumParams, 256)), dim3(256), 0, C.stream, g, lr, c1, c2);
+    }
+    LSM_HIP(hipGetLastError());
+    float h = 0.f;
+    LSM_HIP(hipMemcpyAsync(&h, dloss, sizeof(float), hipMemcpyDeviceToHost, C.stream));
+    LSM_HIP(hipStreamSynchronize(C.stream));
+    if (loss_out) loss_out[e] = steps > 0 ? h / (float)(steps * batch) : 0.f;
+  }
+  // a held copy of the perm vector is not needed beyond this call
+}
+
+void predict_budgets(Context& C, const PredictArgs& args) {
+  if (args.nwork <= 0) return;
+  int cus = 0;
+  LSM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, C.device));
+  const long tiles = (args.nwork + 31) / 32;
+  long blocks = std::min<long>((long)(cus > 0 ? cus : 256) * 2, (tiles + kPredWaves - 1) / kPredWaves);
+  blocks = std::max<long>(blocks, 1);
+  hipLaunchKernelGGL(k_predict, dim3((unsigned)blocks), dim3(64 * kPredWaves), 0, C.stream, args);
+  LSM_HIP(hipGetLastError());
+}
+
+}  // namespace lsm

@@ -17,6 +17,7 @@
 
 #include "lsm_geometry.hpp"
 #include "lsm_runtime.hpp"
+#include "predictor.hpp"
 
 namespace lsm {
 
@@ -136,6 +137,11 @@ struct ClipArgs {
   double* blocks;
   const long* offsets;
   double* elements;
+  // single-pass (predicted budget) clip: per work item budget and slot offset
+  const int* budget;
+  const long* slots;
+  double* slot_elems;
+  const unsigned char* ovf;  // fill pass: when non-null, only items flagged here
 };
 
 __device__ __forceinline__ void load_planes(const double* __restrict__ bp, long b, Planes& P) {
@@ -298,7 +304,11 @@ __global__ void __launch_bounds__(kClipBlock) k_clip(ClipArgs g) {
 // and classifies or measures it) and, when its pair is done, moves on to its
 // next work item by itself, so lanes never wait for the slowest pair of the
 // wave -- only for the slowest chain of work items, which evens out.
-template <bool MASS, bool FILL>
+// MODES: count (FILL and LEARNED false): counts, measures (and mass blocks);
+// fill (FILL): writes each pair's counts[p] outputs at offsets[p];
+// single pass (LEARNED): the count pass that also writes up to budget[w]
+// outputs of work item w at slots[w] of the slot buffer.
+template <bool MASS, bool FILL, bool LEARNED = false>
 __global__ void __launch_bounds__(kClipBlock) k_clip_tet_walk(ClipArgs g) {
   const long n = (long)*g.nwork;
   const long stride = (long)gridDim.x * blockDim.x;
@@ -317,11 +327,15 @@ __global__ void __launch_bounds__(kClipBlock) k_clip_tet_walk(ClipArgs g) {
         p = g.work[w];
         if (!FILL) break;
         cap = g.counts[p];
-        if (cap > 0) break;
+        if (cap > 0 && (!g.ovf || g.ovf[w])) break;
         w += stride;
       }
       if (w >= n) break;
       if (FILL) off = g.offsets[p];
+      if (LEARNED) {
+        cap = g.budget[w];
+        off = g.slots[w];
+      }
       const long a = g.rows[p], b = g.cols[p];
 #pragma unroll
       for (int i = 0; i < 12; i++) A[i] = g.acoords[a * 12 + i];
@@ -345,8 +359,8 @@ __global__ void __launch_bounds__(kClipBlock) k_clip_tet_walk(ClipArgs g) {
       const double vr = tet_volume_real(t);
       m = m + vr;
       if (MASS) mass_element<3>(t, vr, affa, affb, blk);
-      if (FILL && k < cap) {
-        double* o = g.elements + (off + k) * 12;
+      if ((FILL || LEARNED) && k < cap) {
+        double* o = (LEARNED ? g.slot_elems : g.elements) + (off + k) * 12;
 #pragma unroll
         for (int i = 0; i < 12; i++) o[i] = t[i];
       }
@@ -459,6 +473,15 @@ k_batch_sum(const long* __restrict__ gpairs, const double* __restrict__ measure,
   if (threadIdx.x == 0) gsum[g] = rh[0] + rl[0];
 }
 
+long last_work_list(Context& C, const int** work) {
+  long n = 0;
+  const long* bcount = static_cast<const long*>(C.nwork.ptr);
+  LSM_HIP(hipMemcpyAsync(&n, bcount + 2 * kClipGrid + 1, sizeof(long), hipMemcpyDeviceToHost, C.stream));
+  LSM_HIP(hipStreamSynchronize(C.stream));
+  *work = static_cast<const int*>(C.work.ptr);
+  return n;
+}
+
 void batch_sums(Context& C, long ngroups, const long* a_offsets, const long* indptr,
                 const double* measure) {
   long* gp = C.gpairs.as<long>((size_t)ngroups + 1);
@@ -484,27 +507,61 @@ static void launch_clip(Context& C, const ClipArgs& args, bool mass, bool fill) 
   LSM_HIP(hipGetLastError());
 }
 
-static void launch_tet_walk(Context& C, const ClipArgs& args, bool mass, bool fill) {
+// which: 0 count, 1 count + mass, 2 fill, 3 single pass, 4 single pass + mass
+static void launch_tet_walk(Context& C, const ClipArgs& args, int which) {
   // persistent lanes: one full wave of resident blocks
-  static int blocks_per_cu[3] = {0, 0, 0};
-  int which = fill ? 2 : (mass ? 1 : 0);
+  static int blocks_per_cu[5] = {0, 0, 0, 0, 0};
+  static const void* kernels[5] = {
+      reinterpret_cast<const void*>(k_clip_tet_walk<false, false>),
+      reinterpret_cast<const void*>(k_clip_tet_walk<true, false>),
+      reinterpret_cast<const void*>(k_clip_tet_walk<false, true>),
+      reinterpret_cast<const void*>(k_clip_tet_walk<false, false, true>),
+      reinterpret_cast<const void*>(k_clip_tet_walk<true, false, true>)};
   if (blocks_per_cu[which] == 0) {
     int nb = 0;
-    if (fill)
-      LSM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_clip_tet_walk<false, true>, kClipBlock, 0));
-    else if (mass)
-      LSM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_clip_tet_walk<true, false>, kClipBlock, 0));
-    else
-      LSM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_clip_tet_walk<false, false>, kClipBlock, 0));
+    LSM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernels[which], kClipBlock, 0));
     blocks_per_cu[which] = nb > 0 ? nb : 1;
   }
   int cus = 0;
   LSM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, C.device));
   dim3 grid((unsigned)(blocks_per_cu[which] * (cus > 0 ? cus : 256))), block(kClipBlock);
-  if (fill) hipLaunchKernelGGL((k_clip_tet_walk<false, true>), grid, block, 0, C.stream, args);
-  else if (mass) hipLaunchKernelGGL((k_clip_tet_walk<true, false>), grid, block, 0, C.stream, args);
-  else hipLaunchKernelGGL((k_clip_tet_walk<false, false>), grid, block, 0, C.stream, args);
+  switch (which) {
+    case 0: hipLaunchKernelGGL((k_clip_tet_walk<false, false>), grid, block, 0, C.stream, args); break;
+    case 1: hipLaunchKernelGGL((k_clip_tet_walk<true, false>), grid, block, 0, C.stream, args); break;
+    case 2: hipLaunchKernelGGL((k_clip_tet_walk<false, true>), grid, block, 0, C.stream, args); break;
+    case 3: hipLaunchKernelGGL((k_clip_tet_walk<false, false, true>), grid, block, 0, C.stream, args); break;
+    default: hipLaunchKernelGGL((k_clip_tet_walk<true, false, true>), grid, block, 0, C.stream, args); break;
+  }
   LSM_HIP(hipGetLastError());
+}
+
+static void launch_tet_walk(Context& C, const ClipArgs& args, bool mass, bool fill) {
+  launch_tet_walk(C, args, fill ? 2 : (mass ? 1 : 0));
+}
+
+// Single pass: moves each work item's outputs from its slot to the final
+// layout, or flags it when its output did not fit its budget.
+__global__ void __launch_bounds__(kClipBlock)
+k_compact_slots(const unsigned long long* __restrict__ nwork, const int* __restrict__ work,
+                const int* __restrict__ counts, const int* __restrict__ budget,
+                const long* __restrict__ slots, const double* __restrict__ slot_elems,
+                const long* __restrict__ offsets, double* __restrict__ elements,
+                unsigned char* __restrict__ ovf, unsigned long long* __restrict__ novf) {
+  const long n = (long)*nwork;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long w = (long)blockIdx.x * blockDim.x + threadIdx.x; w < n; w += stride) {
+    const long p = work[w];
+    const int c = counts[p];
+    const bool over = c > budget[w];
+    ovf[w] = over;
+    if (over) {
+      atomicAdd(novf, 1ULL);
+      continue;
+    }
+    const double2* src = reinterpret_cast<const double2*>(slot_elems + slots[w] * 12);
+    double2* dst = reinterpret_cast<double2*>(elements + offsets[p] * 12);
+    for (int i = 0; i < 6 * c; i++) dst[i] = src[i];
+  }
 }
 
 __global__ void __launch_bounds__(256)
@@ -561,8 +618,44 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
     g.work = work;
     g.nwork = reinterpret_cast<const unsigned long long*>(boff + kClipGrid);
   }
+  const bool learned = elems && dim == 3 && (flags & LIBSUPERMESH_GPU_LEARNED) && npairs > 0;
+  long nwork = 0, nslots = 0;
+  if (learned) {
+    // single pass: predicted budgets -> slots -> one walk that counts and writes
+    Predictor& P = C.predictor;
+    LSM_REQUIRE(P.ready, LIBSUPERMESH_GPU_ERROR_STATE,
+                "LIBSUPERMESH_GPU_LEARNED needs predictor weights (libsupermesh_gpu_predictor_set)");
+    LSM_HIP(hipMemcpyAsync(&nwork, g.nwork, sizeof(long), hipMemcpyDeviceToHost, C.stream));
+    LSM_HIP(hipStreamSynchronize(C.stream));
+    int* budget = P.budget.as<int>((size_t)nwork);
+    long* slots = P.slots.as<long>((size_t)nwork + 1);
+    PredictArgs pa = {nwork, g.work, g.rows, g.cols, g.acoords, g.bcoords,
+                      static_cast<const __bf16*>(P.weights.ptr), static_cast<const float*>(P.bias.ptr),
+                      P.quantile, budget};
+    if (C.timing) LSM_HIP(hipEventRecord(C.ev[12], C.stream));
+    predict_budgets(C, pa);
+    if (C.timing) LSM_HIP(hipEventRecord(C.ev[13], C.stream));
+    LSM_HIP(hipMemsetAsync(slots, 0, sizeof(long), C.stream));
+    if (nwork > 0) {
+      long* wide = C.row_count.as<long>((size_t)nwork);
+      hipLaunchKernelGGL(k_count_to_long, dim3(grid_for(nwork, 256)), dim3(256), 0, C.stream, nwork, budget,
+                         wide);
+      size_t bytes = 0;
+      LSM_HIP(rocprim::inclusive_scan(nullptr, bytes, wide, slots + 1, (size_t)nwork, rocprim::plus<long>(),
+                                      C.stream));
+      void* tmp = C.scan_tmp.get(bytes);
+      LSM_HIP(rocprim::inclusive_scan(tmp, bytes, wide, slots + 1, (size_t)nwork, rocprim::plus<long>(),
+                                      C.stream));
+    }
+    LSM_HIP(hipMemcpyAsync(&nslots, slots + nwork, sizeof(long), hipMemcpyDeviceToHost, C.stream));
+    LSM_HIP(hipStreamSynchronize(C.stream));
+    g.budget = budget;
+    g.slots = slots;
+    g.slot_elems = P.slot_elems.as<double>((size_t)(nslots > 0 ? nslots : 1) * 12);
+  }
   if (C.timing) LSM_HIP(hipEventRecord(C.ev[14], C.stream));
   if (dim == 2) launch_clip<2>(C, g, mass, false);
+  else if (learned) launch_tet_walk(C, g, mass ? 4 : 3);
   else launch_tet_walk(C, g, mass, false);
   if (C.timing) LSM_HIP(hipEventRecord(C.ev[15], C.stream));
   hipLaunchKernelGGL(k_reduce_pairs, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, npairs, g.counts,
@@ -576,6 +669,10 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
     float ms = 0.f;
     LSM_HIP(hipEventElapsedTime(&ms, C.ev[14], C.ev[15]));
     out->ms[7] = ms;
+    if (learned) {
+      LSM_HIP(hipEventElapsedTime(&ms, C.ev[12], C.ev[13]));
+      out->ms[5] = ms;
+    }
   }
   out->counts = g.counts;
   out->measure = g.measure;
@@ -601,8 +698,30 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
                                       rocprim::plus<long>(), C.stream));
       g.offsets = offs;
       g.elements = C.elements.as<double>((size_t)(out->nout > 0 ? out->nout : 1) * loc * dim);
-      if (dim == 2) launch_clip<2>(C, g, false, true);
-      else launch_tet_walk(C, g, false, true);
+      if (learned) {
+        Predictor& P = C.predictor;
+        unsigned char* ovf = P.overflow.as<unsigned char>((size_t)(nwork > 0 ? nwork : 1));
+        unsigned long long* novf = C.scalars.as<unsigned long long>(8) + 1;
+        LSM_HIP(hipMemsetAsync(novf, 0, sizeof(unsigned long long), C.stream));
+        hipLaunchKernelGGL(k_compact_slots, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, g.nwork, g.work,
+                           g.counts, g.budget, g.slots, g.slot_elems, offs, g.elements, ovf, novf);
+        LSM_HIP(hipGetLastError());
+        unsigned long long h = 0;
+        LSM_HIP(hipMemcpyAsync(&h, novf, sizeof(h), hipMemcpyDeviceToHost, C.stream));
+        LSM_HIP(hipStreamSynchronize(C.stream));
+        if (h > 0) {
+          // pairs whose output outgrew the budget: the plain fill pass, for them only
+          g.ovf = ovf;
+          launch_tet_walk(C, g, 2);
+          g.ovf = nullptr;
+        }
+        out->learned_slots = nslots;
+        out->learned_overflow = (long)h;
+      } else if (dim == 2) {
+        launch_clip<2>(C, g, false, true);
+      } else {
+        launch_tet_walk(C, g, false, true);
+      }
     }
     out->offsets = offs;
     out->elements = static_cast<const double*>(C.elements.get(8));

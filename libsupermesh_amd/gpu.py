@@ -30,6 +30,8 @@ ELEMENTS = 1
 MASS = 2
 TIMING = 4
 SORTED = 8
+LEARNED = 16
+PREDICTOR_PARAMS = 12065
 
 _lib = None
 
@@ -54,7 +56,7 @@ class Result(ctypes.Structure):
                 ("blocks", c_void_p), ("mass_nnz", c_long), ("mass_indptr", c_void_p),
                 ("mass_indices", c_void_p), ("mass_values", c_void_p), ("sum", c_double),
                 ("nonempty", c_long), ("nout", c_long), ("max_count", c_int),
-                ("ms", c_double * 8)]
+                ("ms", c_double * 8), ("learned_slots", c_long), ("learned_overflow", c_long)]
 
 
 # libsupermesh_gpu_pair_callback
@@ -86,6 +88,20 @@ def lib():
                                                          POINTER(MeshBatch), c_int, c_int,
                                                          POINTER(Result), POINTER(BatchResult)]
         L.libsupermesh_gpu_supermesh_batched.restype = c_int
+        fp = POINTER(ctypes.c_float)
+        L.libsupermesh_gpu_predictor_set.argtypes = [c_void_p, fp]
+        L.libsupermesh_gpu_predictor_get.argtypes = [c_void_p, fp]
+        L.libsupermesh_gpu_predictor_quantile.argtypes = [c_void_p, ctypes.c_float]
+        L.libsupermesh_gpu_predictor_collect.argtypes = [c_void_p, POINTER(Mesh), POINTER(Mesh), c_long,
+                                                         POINTER(c_long)]
+        L.libsupermesh_gpu_predictor_train.argtypes = [c_void_p, c_int, c_int, ctypes.c_float,
+                                                       ctypes.c_uint, fp]
+        L.libsupermesh_gpu_predictor_clear.argtypes = [c_void_p]
+        L.libsupermesh_gpu_predictor_infer.argtypes = [c_void_p, POINTER(Mesh), POINTER(Mesh), c_long,
+                                                       c_void_p, c_void_p, c_void_p]
+        L.libsupermesh_gpu_predictor_add_samples.argtypes = [c_void_p, c_long, fp, POINTER(c_int)]
+        for name in ("set", "get", "quantile", "collect", "train", "clear", "infer", "add_samples"):
+            getattr(L, "libsupermesh_gpu_predictor_" + name).restype = c_int
         L.libsupermesh_gpu_malloc.argtypes = [c_int, ctypes.c_size_t, POINTER(c_void_p)]
         L.libsupermesh_gpu_malloc.restype = c_int
         L.libsupermesh_gpu_free.argtypes = [c_void_p]
@@ -380,6 +396,55 @@ class Context:
                                                           ctypes.byref(b.struct), finder, flags,
                                                           ctypes.byref(r)))
         return r
+
+    # -- learned output-size predictor -------------------------------------
+    def predictor_set(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32).reshape(-1)
+        assert p.size == PREDICTOR_PARAMS
+        _check(lib().libsupermesh_gpu_predictor_set(self.handle, p.ctypes.data_as(POINTER(ctypes.c_float))))
+
+    def predictor_get(self):
+        p = np.zeros(PREDICTOR_PARAMS, dtype=np.float32)
+        _check(lib().libsupermesh_gpu_predictor_get(self.handle, p.ctypes.data_as(POINTER(ctypes.c_float))))
+        return p
+
+    def predictor_quantile(self, q):
+        _check(lib().libsupermesh_gpu_predictor_quantile(self.handle, ctypes.c_float(q)))
+
+    def predictor_collect(self, a: DeviceMesh, b: DeviceMesh, max_samples):
+        n = c_long(0)
+        _check(lib().libsupermesh_gpu_predictor_collect(self.handle, ctypes.byref(a.struct),
+                                                        ctypes.byref(b.struct), max_samples,
+                                                        ctypes.byref(n)))
+        return n.value
+
+    def predictor_train(self, epochs, batch=4096, lr=2e-3, seed=0):
+        loss = np.zeros(max(epochs, 1), dtype=np.float32)
+        _check(lib().libsupermesh_gpu_predictor_train(self.handle, epochs, batch, ctypes.c_float(lr),
+                                                      seed, loss.ctypes.data_as(POINTER(ctypes.c_float))))
+        return loss[:epochs]
+
+    def predictor_infer(self, a: DeviceMesh, b: DeviceMesh, pair_a, pair_b):
+        """Predicted output budget of each pair (host int arrays in, numpy out)."""
+        pa = DeviceArray(np.ascontiguousarray(pair_a, dtype=np.int32))
+        pb = DeviceArray(np.ascontiguousarray(pair_b, dtype=np.int32))
+        n = len(pair_a)
+        out = DeviceArray(np.zeros(max(n, 1), dtype=np.int32))
+        _check(lib().libsupermesh_gpu_predictor_infer(self.handle, ctypes.byref(a.struct),
+                                                      ctypes.byref(b.struct), n, pa.ptr, pb.ptr,
+                                                      out.ptr))
+        return out.numpy()[:n]
+
+    def predictor_add_samples(self, features, counts):
+        f = np.ascontiguousarray(features, dtype=np.float32).reshape(-1, 24)
+        y = np.ascontiguousarray(counts, dtype=np.int32).reshape(-1)
+        assert f.shape[0] == y.shape[0]
+        _check(lib().libsupermesh_gpu_predictor_add_samples(
+            self.handle, f.shape[0], f.ctypes.data_as(POINTER(ctypes.c_float)),
+            y.ctypes.data_as(POINTER(c_int))))
+
+    def predictor_clear(self):
+        _check(lib().libsupermesh_gpu_predictor_clear(self.handle))
 
     def supermesh_batched(self, a: DeviceMeshBatch, b: DeviceMeshBatch, finder=FINDER_TREE, flags=0):
         """(Result over all meshes, BatchResult with per-mesh pair offsets and sums)."""

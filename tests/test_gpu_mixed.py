@@ -69,3 +69,18 @@ def test_reference_mixed_meshes(fixture_meshes, names, expected):
         assert np.array_equal(got, ref_fn(pa[ea[a]], pb[eb[b]]))
         total += sum(meas(t) for t in got)
     assert abs(total - expected) < 1e-12
+
+
+@pytest.mark.parametrize("names", [("square.1", "square.4"), ("square.3", "square.2"),
+                                   ("square.3", "square.4"), ("cube.1", "cube.3"),
+                                   ("cube.3", "cube.4")])
+@pytest.mark.parametrize("closed", [False, True])
+def test_finders_on_quads_and_hexes(fixture_meshes, names, closed):
+    # the candidate finders take any element type (boxes only)
+    pa, ea = fixture_meshes[names[0]]
+    pb, eb = fixture_meshes[names[1]]
+    fn = gpu.rtree_intersection_finder if closed else gpu.tree_intersection_finder
+    indptr, indices = fn(pa, ea, pb, eb)
+    ip, ix = O.brute_force(pa, ea, pb, eb, closed=closed)
+    assert np.array_equal(indptr, ip)
+    assert np.array_equal(indices, ix)

@@ -1,0 +1,98 @@
+"""The learned output-size predictor: MFMA inference and training kernels
+against a numpy restatement, and the single-pass (LEARNED) tet clip against
+the plain count-then-fill path -- identical outputs whatever the weights."""
+import numpy as np
+import pytest
+
+import mlp_ref as R
+import oracle_lib as O
+from libsupermesh_amd import gpu, meshgen
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def meshes():
+    pa, ea = meshgen.split_grid(3, 6, 21, mesh_id=0, diagonal=0)
+    pb, eb = meshgen.split_grid(3, 5, 21, mesh_id=1, diagonal=1)
+    return pa, ea, pb, eb
+
+
+def test_inference_matches_numpy(meshes):
+    pa, ea, pb, eb = meshes
+    ip, ix = O.brute_force(pa, ea, pb, eb)
+    rows, cols = O.csr_to_pairs(ip, ix)
+    ctx = gpu.Context()
+    params = R.init(3, scale=3.0)
+    ctx.predictor_set(params)
+    got = ctx.predictor_infer(gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pb, eb), rows, cols)
+    x = R.features(pa[ea[rows]], pb[eb[cols]])
+    logits, _ = R.forward(params, x)
+    ref = R.budgets(logits)
+    assert len(set(ref.tolist())) > 3   # the weights give varied budgets
+    agree = np.mean(got == ref)
+    assert agree >= 0.97, agree          # off only where BF16 rounding flips a threshold
+    assert np.all(np.abs(got - ref)[got != ref] <= 81)
+    assert np.array_equal(ctx.predictor_get(), params)
+
+
+def test_training_step_matches_numpy_gradient():
+    rng = np.random.default_rng(4)
+    n = 4096
+    x = rng.uniform(0, 1, (n, 24)).astype(np.float32)
+    y = rng.integers(0, 33, n).astype(np.int32)
+    params = R.init(5)
+    ctx = gpu.Context()
+    ctx.predictor_set(params)
+    ctx.predictor_add_samples(x, y)
+    loss = ctx.predictor_train(epochs=1, batch=n, lr=1e-3, seed=0)   # one Adam step
+    ref_loss, grad = R.loss_and_grads(params, R.bf16(x), y)
+    assert abs(loss[0] - ref_loss) <= 1e-3 * ref_loss
+    step = ctx.predictor_get() - params
+    # Adam's first step moves each weight by lr * sign(gradient)
+    big = np.abs(grad) > 1e-2 * np.abs(grad).max()
+    assert big.sum() > 1000
+    agree = np.mean(np.sign(step[big]) == -np.sign(grad[big]))
+    assert agree >= 0.99, agree
+    assert np.allclose(np.abs(step[big]), 1e-3, rtol=1e-2)
+
+
+def test_training_learns_clip_counts(meshes):
+    pa, ea, pb, eb = meshes
+    ctx = gpu.Context()
+    A, B = gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pb, eb)
+    n = ctx.predictor_collect(A, B, 1 << 20)
+    assert n > 2000
+    loss = ctx.predictor_train(epochs=12, batch=512, lr=3e-3, seed=7)
+    assert np.all(np.isfinite(loss))
+    assert loss[-1] < 0.6 * loss[0], loss
+
+
+@pytest.mark.parametrize("weights", ["trained", "random", "zero"])
+@pytest.mark.parametrize("quantile", [0.5, 0.99])
+def test_learned_single_pass_is_identical(meshes, weights, quantile):
+    pa, ea, pb, eb = meshes
+    ctx = gpu.Context()
+    A, B = gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pb, eb)
+    if weights == "trained":
+        ctx.predictor_collect(A, B, 1 << 20)
+        ctx.predictor_train(epochs=6, batch=512, lr=3e-3, seed=1)
+    elif weights == "random":
+        ctx.predictor_set(R.init(9, scale=3.0))
+    else:
+        ctx.predictor_set(np.zeros(gpu.PREDICTOR_PARAMS, np.float32))
+    ctx.predictor_quantile(quantile)
+    what = ["indptr", "indices", "counts", "measure", "offsets", "elements"]
+    r0 = ctx.supermesh(A, B, flags=gpu.ELEMENTS | gpu.SORTED)
+    plain = gpu.fetch(r0, 3, what)
+    n0 = (r0.npairs, r0.nout, r0.sum)
+    r1 = ctx.supermesh(A, B, flags=gpu.ELEMENTS | gpu.SORTED | gpu.LEARNED)
+    learned = gpu.fetch(r1, 3, what)
+    assert (r1.npairs, r1.nout, r1.sum) == n0
+    for k in what:
+        assert np.array_equal(plain[k], learned[k]), k
+    if weights == "zero":
+        assert r1.learned_overflow == 0   # uniform prediction: every budget is the maximum
+    if weights == "random":
+        assert r1.learned_overflow > 0    # the fallback path was exercised
+    assert r1.learned_slots >= 0
