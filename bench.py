@@ -42,6 +42,9 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--no-batched", action="store_true", help="skip the batched workload")
     p.add_argument("--batch-count", type=int, default=4096, help="mesh pairs in the batched workload")
+    p.add_argument("--no-learned", action="store_true", help="skip the learned single-pass workload")
+    p.add_argument("--learned-workload", default="tet_96_vs_90")
+    p.add_argument("--weights", default=os.path.join(ROOT, "libsupermesh_amd", "data", "tet_counts_mlp.f32"))
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also run a plain device copy of known size (PMC calibration)")
     return p.parse_args()
@@ -161,6 +164,47 @@ def batched_workload(ctx, steps, warmup, count=4096):
     }
 
 
+def learned_workload(ctx, name, steps, warmup, weights):
+    """Materialised tet supermesh (LIBSUPERMESH_GPU_ELEMENTS): plain
+    count-then-fill against the single pass sized by the learned predictor
+    (LIBSUPERMESH_GPU_LEARNED).  Outputs are compared by device checksums of
+    the per-pair counts, the offsets and every output tet."""
+    import numpy as np
+    from libsupermesh_amd import gpu, meshgen
+    t0 = time.time()
+    pa, ea, pb, eb = meshgen.config_meshes(name)
+    log(f"[bench] learned {name}: meshes {ea.shape[0]} x {eb.shape[0]} ({time.time() - t0:.1f}s)")
+    A, B = gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pb, eb)
+    ctx.predictor_set(np.fromfile(weights, dtype=np.float32))
+    out = {}
+    for mode, flags in (("plain", gpu.ELEMENTS | gpu.TIMING), ("learned", gpu.ELEMENTS | gpu.LEARNED | gpu.TIMING)):
+        for _ in range(warmup):
+            ctx.supermesh(A, B, flags=flags)
+        gpu.synchronize()
+        t1 = time.perf_counter()
+        walk = []
+        pred = []
+        for _ in range(steps):
+            r = ctx.supermesh(A, B, flags=flags)
+            walk.append(r.ms[7])
+            pred.append(r.ms[5])
+        gpu.synchronize()
+        dt = (time.perf_counter() - t1) / steps
+        sums = (ctx.checksum(r.counts, 4 * r.npairs), ctx.checksum(r.offsets, 8 * (r.npairs + 1)),
+                ctx.checksum(r.elements, 96 * r.nout))
+        out[mode] = {"ms_per_step": dt * 1e3, "pairs": int(r.npairs), "supermesh_elements": int(r.nout),
+                     "pairs_per_s": r.npairs / dt, "elements_per_s": r.nout / dt,
+                     "first_walk_ms": sum(walk) / steps, "_sums": sums}
+        if mode == "learned":
+            out[mode].update({"predictor_ms": sum(pred) / steps, "slots": int(r.learned_slots),
+                              "overflow_pairs": int(r.learned_overflow)})
+        log(f"[bench] learned {name}, {mode}: {r.npairs} pairs, {r.nout} tets, {dt * 1e3:.1f} ms/step")
+    same = out["plain"].pop("_sums") == out["learned"].pop("_sums")
+    return {"workload": name, "plain": out["plain"], "learned": out["learned"],
+            "speedup": out["plain"]["ms_per_step"] / out["learned"]["ms_per_step"],
+            "identical_outputs": same, "weights": os.path.relpath(weights, ROOT)}
+
+
 def cpu_baseline(name, meshes):
     """The CPU restatement of the reference's serial loop (tree finder, then
     the per-pair clip), one thread per host core over contiguous ranges of A."""
@@ -203,6 +247,9 @@ def main():
     batched = None
     if not args.no_batched:
         batched = batched_workload(ctx, max(1, min(args.steps, 3)), 1, count=args.batch_count)
+    learned = None
+    if not args.no_learned and os.path.exists(args.weights):
+        learned = learned_workload(ctx, args.learned_workload, max(1, min(args.steps, 3)), 1, args.weights)
     fpp = FLOPS_PER_PAIR.get(args.workload)
     clip_s = main_w["clip_kernel_ms"] / 1e3
     if fpp:
@@ -236,6 +283,7 @@ def main():
         "detail": main_w,
         "tet_64_vs_60": extra,
         "batched": batched,
+        "learned": learned,
     }
     ctx.close()
     print(json.dumps(out))

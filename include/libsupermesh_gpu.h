@@ -114,6 +114,12 @@ int libsupermesh_gpu_free(void* ptr);
 int libsupermesh_gpu_memcpy(void* dst, const void* src, size_t bytes, int kind);
 int libsupermesh_gpu_synchronize(int device);
 
+/* Order-sensitive checksum of `bytes` (a multiple of 4) of device memory:
+ * the sum of its 32-bit words w_i times (2 i + 1), modulo 2^64.  For comparing
+ * large results without copying them to the host. */
+int libsupermesh_gpu_checksum(libsupermesh_gpu_context* ctx, const void* ptr, size_t bytes,
+                              unsigned long long* out);
+
 /* Candidate pairs only (finder = LIBSUPERMESH_GPU_FINDER_*). */
 int libsupermesh_gpu_intersection_finder(libsupermesh_gpu_context* ctx,
                                          const libsupermesh_gpu_mesh* a,

@@ -546,6 +546,18 @@ int libsupermesh_gpu_supermesh_batched(libsupermesh_gpu_context* ctx,
   });
 }
 
+int libsupermesh_gpu_checksum(libsupermesh_gpu_context* ctx, const void* ptr, size_t bytes,
+                              unsigned long long* out) {
+  return guarded([&] {
+    LSM_REQUIRE(ctx != nullptr && out != nullptr && (bytes == 0 || ptr != nullptr),
+                LIBSUPERMESH_GPU_ERROR_INVALID, "invalid argument");
+    LSM_REQUIRE(bytes % 4 == 0, LIBSUPERMESH_GPU_ERROR_INVALID, "checksum size must be whole 32-bit words");
+    std::lock_guard<std::mutex> lk(ctx->ctx.mu);
+    LSM_HIP(hipSetDevice(ctx->ctx.device));
+    *out = checksum(ctx->ctx, ptr, bytes);
+  });
+}
+
 int libsupermesh_gpu_predictor_set(libsupermesh_gpu_context* ctx, const float* params) {
   return guarded([&] {
     LSM_REQUIRE(ctx != nullptr && params != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID, "null argument");

@@ -158,6 +158,8 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
                 libsupermesh_gpu_result* out);
 void assemble_mass(Context& C, const DevMesh& A, const DevMesh& B, long npairs,
                    libsupermesh_gpu_result* out);
+// Order-sensitive checksum of a device buffer (whole 32-bit words).
+unsigned long long checksum(Context& C, const void* ptr, size_t bytes);
 // Work list (pairs that needed the full clip) of the last 3D clip_pairs call.
 long last_work_list(Context& C, const int** work);
 // Batches: per-mesh candidate pair offsets (gpairs, from the CSR row pointers

@@ -473,6 +473,39 @@ k_batch_sum(const long* __restrict__ gpairs, const double* __restrict__ measure,
   if (threadIdx.x == 0) gsum[g] = rh[0] + rl[0];
 }
 
+// Order-sensitive 64-bit checksum of a device buffer: sum over its 32-bit
+// words w_i of w_i * (2 i + 1), modulo 2^64 (wrapping adds commute, so the
+// value does not depend on the reduction order).
+__global__ void __launch_bounds__(256)
+k_checksum(long nwords, const unsigned* __restrict__ words, unsigned long long* __restrict__ out) {
+  unsigned long long h = 0;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += stride)
+    h += (unsigned long long)words[i] * (unsigned long long)(2 * i + 1);
+  __shared__ unsigned long long red[256];
+  red[threadIdx.x] = h;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(out, red[0]);
+}
+
+unsigned long long checksum(Context& C, const void* ptr, size_t bytes) {
+  unsigned long long* d = C.scalars.as<unsigned long long>(8) + 2;
+  LSM_HIP(hipMemsetAsync(d, 0, sizeof(*d), C.stream));
+  const long n = (long)(bytes / 4);
+  if (n > 0)
+    hipLaunchKernelGGL(k_checksum, dim3((unsigned)std::min<long>(4096, (n + 255) / 256)), dim3(256), 0,
+                       C.stream, n, static_cast<const unsigned*>(ptr), d);
+  LSM_HIP(hipGetLastError());
+  unsigned long long h = 0;
+  LSM_HIP(hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, C.stream));
+  LSM_HIP(hipStreamSynchronize(C.stream));
+  return h;
+}
+
 long last_work_list(Context& C, const int** work) {
   long n = 0;
   const long* bcount = static_cast<const long*>(C.nwork.ptr);

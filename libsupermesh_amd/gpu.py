@@ -100,6 +100,9 @@ def lib():
         L.libsupermesh_gpu_predictor_infer.argtypes = [c_void_p, POINTER(Mesh), POINTER(Mesh), c_long,
                                                        c_void_p, c_void_p, c_void_p]
         L.libsupermesh_gpu_predictor_add_samples.argtypes = [c_void_p, c_long, fp, POINTER(c_int)]
+        L.libsupermesh_gpu_checksum.argtypes = [c_void_p, c_void_p, ctypes.c_size_t,
+                                                POINTER(ctypes.c_ulonglong)]
+        L.libsupermesh_gpu_checksum.restype = c_int
         for name in ("set", "get", "quantile", "collect", "train", "clear", "infer", "add_samples"):
             getattr(L, "libsupermesh_gpu_predictor_" + name).restype = c_int
         L.libsupermesh_gpu_malloc.argtypes = [c_int, ctypes.c_size_t, POINTER(c_void_p)]
@@ -396,6 +399,12 @@ class Context:
                                                           ctypes.byref(b.struct), finder, flags,
                                                           ctypes.byref(r)))
         return r
+
+    def checksum(self, ptr, nbytes):
+        """Order-sensitive 64-bit checksum of device memory (whole 32-bit words)."""
+        h = ctypes.c_ulonglong(0)
+        _check(lib().libsupermesh_gpu_checksum(self.handle, c_void_p(ptr), nbytes, ctypes.byref(h)))
+        return h.value
 
     # -- learned output-size predictor -------------------------------------
     def predictor_set(self, params):

@@ -315,3 +315,16 @@ def test_callback_driver(fixture_meshes, names):
     rows, cols = O.csr_to_pairs(ip, ix)
     expect = [(int(r), int(c)) for r, c in zip(rows, cols)]
     assert seen == expect
+
+
+def test_device_checksum_matches_numpy():
+    rng = np.random.default_rng(12)
+    host = rng.integers(0, 2**32, 100003, dtype=np.uint64).astype(np.uint32)
+    d = gpu.DeviceArray(host)
+    ctx = gpu.Context()
+    i = np.arange(host.size, dtype=np.uint64)
+    ref = int(np.sum(host.astype(np.uint64) * (2 * i + 1), dtype=np.uint64))
+    assert ctx.checksum(d.ptr.value, host.nbytes) == ref
+    host2 = host.copy()
+    host2[[5, 6]] = host2[[6, 5]]   # order matters
+    assert ctx.checksum(gpu.DeviceArray(host2).ptr.value, host.nbytes) != ref
