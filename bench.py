@@ -164,7 +164,7 @@ def batched_workload(ctx, steps, warmup, count=4096):
     }
 
 
-def learned_workload(ctx, name, steps, warmup, weights):
+def learned_workload(ctx, name, steps, warmup, weights, quantiles=(0.7, 0.9, 0.99)):
     """Materialised tet supermesh (LIBSUPERMESH_GPU_ELEMENTS): plain
     count-then-fill against the single pass sized by the learned predictor
     (LIBSUPERMESH_GPU_LEARNED).  Outputs are compared by device checksums of
@@ -177,32 +177,46 @@ def learned_workload(ctx, name, steps, warmup, weights):
     A, B = gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pb, eb)
     ctx.predictor_set(np.fromfile(weights, dtype=np.float32))
     out = {}
-    for mode, flags in (("plain", gpu.ELEMENTS | gpu.TIMING), ("learned", gpu.ELEMENTS | gpu.LEARNED | gpu.TIMING)):
+    modes = [("plain", gpu.ELEMENTS | gpu.TIMING, None)]
+    modes += [(f"learned_q{q}", gpu.ELEMENTS | gpu.LEARNED | gpu.TIMING, q) for q in quantiles]
+    for mode, flags, q in modes:
+        if q is not None:
+            ctx.predictor_quantile(q)
         for _ in range(warmup):
             ctx.supermesh(A, B, flags=flags)
         gpu.synchronize()
         t1 = time.perf_counter()
         walk = []
         pred = []
+        phases = []
         for _ in range(steps):
             r = ctx.supermesh(A, B, flags=flags)
             walk.append(r.ms[7])
             pred.append(r.ms[5])
+            phases.append(list(r.ms))
         gpu.synchronize()
         dt = (time.perf_counter() - t1) / steps
         sums = (ctx.checksum(r.counts, 4 * r.npairs), ctx.checksum(r.offsets, 8 * (r.npairs + 1)),
                 ctx.checksum(r.elements, 96 * r.nout))
         out[mode] = {"ms_per_step": dt * 1e3, "pairs": int(r.npairs), "supermesh_elements": int(r.nout),
                      "pairs_per_s": r.npairs / dt, "elements_per_s": r.nout / dt,
-                     "first_walk_ms": sum(walk) / steps, "_sums": sums}
-        if mode == "learned":
-            out[mode].update({"predictor_ms": sum(pred) / steps, "slots": int(r.learned_slots),
+                     "first_walk_ms": sum(walk) / steps, "_sums": sums,
+                     "phase_ms": {k: round(sum(p[i] for p in phases) / steps, 3)
+                                  for k, i in (("search", 2), ("clip", 3), ("total", 6))}}
+        if q is not None:
+            out[mode].update({"quantile": q, "predictor_ms": sum(pred) / steps, "slots": int(r.learned_slots),
                               "overflow_pairs": int(r.learned_overflow)})
-        log(f"[bench] learned {name}, {mode}: {r.npairs} pairs, {r.nout} tets, {dt * 1e3:.1f} ms/step")
-    same = out["plain"].pop("_sums") == out["learned"].pop("_sums")
-    return {"workload": name, "plain": out["plain"], "learned": out["learned"],
-            "speedup": out["plain"]["ms_per_step"] / out["learned"]["ms_per_step"],
-            "identical_outputs": same, "weights": os.path.relpath(weights, ROOT)}
+        log(f"[bench] learned {name}, {mode}: {r.npairs} pairs, {r.nout} tets, {dt * 1e3:.1f} ms/step, "
+            f"phases {out[mode]['phase_ms']}")
+    ref = out["plain"].pop("_sums")
+    same = all(v.pop("_sums") == ref for k, v in out.items() if k != "plain")
+    best = min((k for k in out if k != "plain"), key=lambda k: out[k]["ms_per_step"])
+    return {"workload": name, "plain": out["plain"], "learned": out[best],
+            "speedup": out["plain"]["ms_per_step"] / out[best]["ms_per_step"],
+            "identical_outputs": same, "weights": os.path.relpath(weights, ROOT),
+            "quantile_sweep": {k: {"ms_per_step": round(v["ms_per_step"], 3), "slots": v["slots"],
+                                   "overflow_pairs": v["overflow_pairs"]}
+                               for k, v in out.items() if k != "plain"}}
 
 
 def cpu_baseline(name, meshes):

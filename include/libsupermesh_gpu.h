@@ -171,7 +171,7 @@ int libsupermesh_gpu_supermesh_batched(libsupermesh_gpu_context* ctx,
  * b4[33] (W[out][in]); inputs are the pair's 8 vertices normalised to the
  * pair's bounding box, outputs the logits of counts 0..32 (32 = "32 or more").
  * set/get copy host arrays; quantile sets the budget rule (the smallest count
- * whose predicted cumulative probability reaches q, default 0.99); collect
+ * whose predicted cumulative probability reaches q, default 0.9); collect
  * clips A x B with the plain path and keeps up to max_samples (features,
  * count) of its fully clipped pairs as training data; train runs `epochs`
  * passes of Adam over the collected samples (batch a multiple of 64),
@@ -181,6 +181,10 @@ int libsupermesh_gpu_supermesh_batched(libsupermesh_gpu_context* ctx,
 int libsupermesh_gpu_predictor_set(libsupermesh_gpu_context* ctx, const float* params);
 int libsupermesh_gpu_predictor_get(libsupermesh_gpu_context* ctx, float* params);
 int libsupermesh_gpu_predictor_quantile(libsupermesh_gpu_context* ctx, float q);
+/* Outputs beyond a pair's budget go to an arena of fraction x the predicted
+ * output plus min_tets tets (defaults 0.25 and 2^20), split among the clip's
+ * lanes; pairs that find their lane's share full are clipped again. */
+int libsupermesh_gpu_predictor_arena(libsupermesh_gpu_context* ctx, float fraction, long min_tets);
 int libsupermesh_gpu_predictor_collect(libsupermesh_gpu_context* ctx, const libsupermesh_gpu_mesh* a,
                                        const libsupermesh_gpu_mesh* b, long max_samples,
                                        long* nsamples);

@@ -68,7 +68,8 @@ static void free_context(Context& C) {
                     &C.predictor.adam_m, &C.predictor.adam_v, &C.predictor.grad,
                     &C.predictor.features, &C.predictor.labels, &C.predictor.act,
                     &C.predictor.loss, &C.predictor.index, &C.predictor.budget,
-                    &C.predictor.slots, &C.predictor.slot_elems, &C.predictor.overflow};
+                    &C.predictor.slots, &C.predictor.slot_elems, &C.predictor.overflow,
+                    &C.predictor.arena, &C.predictor.aoff};
   for (DevBuf* b : bufs) b->release();
   if (C.pinned) hipHostFree(C.pinned);
   C.pinned = nullptr;
@@ -252,6 +253,11 @@ static void run_callback_driver(Context& C, const libsupermesh_gpu_mesh* a,
       cb(ea, eb, elems.data() + (size_t)offs[i] * per, counts[i], nodes_b, ia, ib, user);
     }
   }
+}
+
+__global__ void __launch_bounds__(256) k_low_byte(long n, int* __restrict__ v) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] &= 0xff;
 }
 
 // ---------------------------------------------------------------------------
@@ -585,6 +591,16 @@ int libsupermesh_gpu_predictor_quantile(libsupermesh_gpu_context* ctx, float q) 
   });
 }
 
+int libsupermesh_gpu_predictor_arena(libsupermesh_gpu_context* ctx, float fraction, long min_tets) {
+  return guarded([&] {
+    LSM_REQUIRE(ctx != nullptr && fraction >= 0.f && min_tets >= 1, LIBSUPERMESH_GPU_ERROR_INVALID,
+                "invalid arena size");
+    std::lock_guard<std::mutex> lk(ctx->ctx.mu);
+    ctx->ctx.predictor.arena_fraction = fraction;
+    ctx->ctx.predictor.arena_min = min_tets;
+  });
+}
+
 int libsupermesh_gpu_predictor_collect(libsupermesh_gpu_context* ctx, const libsupermesh_gpu_mesh* a,
                                        const libsupermesh_gpu_mesh* b, long max_samples,
                                        long* nsamples) {
@@ -631,11 +647,23 @@ int libsupermesh_gpu_predictor_infer(libsupermesh_gpu_context* ctx, const libsup
     LSM_REQUIRE(C.predictor.ready, LIBSUPERMESH_GPU_ERROR_STATE, "no predictor weights loaded");
     stage_mesh(C, C.mesh_a, 3, 4, a->nnodes, a->nelements, a->positions, a->enlist, false, false);
     stage_mesh(C, C.mesh_b, 3, 4, b->nnodes, b->nelements, b->positions, b->enlist, false, false);
-    PredictArgs pa = {npairs, nullptr, pair_a, pair_b, static_cast<const double*>(C.mesh_a.coords.ptr),
-                      static_cast<const double*>(C.mesh_b.coords.ptr),
-                      static_cast<const __bf16*>(C.predictor.weights.ptr),
-                      static_cast<const float*>(C.predictor.bias.ptr), C.predictor.quantile, budgets};
+    PredictArgs pa = {};
+    pa.nwork = npairs;
+    pa.rows = pair_a;
+    pa.cols = pair_b;
+    pa.acoords = static_cast<const double*>(C.mesh_a.coords.ptr);
+    pa.bcoords = static_cast<const double*>(C.mesh_b.coords.ptr);
+    pa.weights = static_cast<const __bf16*>(C.predictor.weights.ptr);
+    pa.bias = static_cast<const float*>(C.predictor.bias.ptr);
+    budget_levels(C.predictor.quantile, pa.levels);
+    // packed levels land in the caller's array; level 0 is the low byte
+    pa.budget = reinterpret_cast<unsigned*>(budgets);
+    pa.sums = C.scalars.as<unsigned long long>(16) + 8;
+    LSM_HIP(hipMemsetAsync(pa.sums, 0, 5 * sizeof(unsigned long long), C.stream));
     predict_budgets(C, pa);
+    if (npairs > 0)
+      hipLaunchKernelGGL(k_low_byte, dim3(grid_for(npairs, 256)), dim3(256), 0, C.stream, npairs, budgets);
+    LSM_HIP(hipGetLastError());
     LSM_HIP(hipStreamSynchronize(C.stream));
   });
 }

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstddef>
 #include <cstdint>
 #include <cstdio>
@@ -52,7 +53,7 @@ struct DevBuf {
       }
       ptr = nullptr;
       bytes = 0;
-      size_t cap = need + need / 4;
+      size_t cap = need + std::min(need / 4, (size_t)256 << 20);
       LSM_HIP(hipMalloc(&ptr, cap));
       bytes = cap;
     }
@@ -104,13 +105,16 @@ struct DevBVH {
 struct Predictor {
   bool ready = false;
   long steps = 0;           // Adam steps taken
-  float quantile = 0.99f;   // budget = this quantile of the predicted count
+  float quantile = 0.9f;    // budget = this quantile of the predicted count
+  int level = 0;            // quantile level the last single pass used (0 = quantile)
+  float arena_fraction = 0.25f;  // arena size: this fraction of the expected output ...
+  long arena_min = 1L << 20;     // ... plus this many tets
   DevBuf weights, bias;     // BF16 layouts, FP32 biases [4][64]
   DevBuf master, adam_m, adam_v, grad;
   DevBuf features, labels;  // collected training samples
   long nsamples = 0;
   DevBuf act, loss, index;  // training scratch
-  DevBuf budget, slots, slot_elems, overflow;  // single-pass clip buffers
+  DevBuf budget, slots, slot_elems, overflow, arena, aoff;  // single-pass clip buffers
 };
 
 struct Context {

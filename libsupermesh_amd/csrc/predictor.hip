@@ -102,19 +102,39 @@ __device__ __forceinline__ void pair_features(const double* __restrict__ ta, con
   for (int k = 24; k < 32; k++) row[k] = (__bf16)0.f;
 }
 
-// Smallest count whose cumulative predicted probability reaches q.
-__device__ __forceinline__ int budget_of(const float* logit, float q) {
+// Budgets at four quantile levels (highest first), packed one per byte: for
+// each level, the smallest count whose cumulative predicted probability
+// reaches it, or the clip's maximum (81) when only the last class ("32 or
+// more") does.  Also returns 1024 x the expected count.
+__device__ __forceinline__ unsigned budgets_of(const float* logit, const float* levels, unsigned& expect) {
   float m = logit[0];
   for (int c = 1; c < 33; c++) m = fmaxf(m, logit[c]);
   float s = 0.f;
   for (int c = 0; c < 33; c++) s += __expf(logit[c] - m);
-  const float target = q * s;
-  float acc = 0.f;
-  for (int c = 0; c < 32; c++) {
-    acc += __expf(logit[c] - m);
-    if (acc >= target) return c;
+  float t[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) t[k] = levels[k] * s;
+  unsigned b[4] = {(unsigned)kTetBufSize, (unsigned)kTetBufSize, (unsigned)kTetBufSize, (unsigned)kTetBufSize};
+  float acc = 0.f, ex = 0.f;
+  for (int c = 0; c < 33; c++) {
+    const float e = __expf(logit[c] - m);
+    acc += e;
+    ex += (float)c * e;
+    if (c < 32) {
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        if (b[k] == (unsigned)kTetBufSize && acc >= t[k]) b[k] = (unsigned)c;
+    }
   }
-  return kTetBufSize;  // "32 or more": the clip's maximum
+  expect = (unsigned)(1024.f * ex / s);
+  return b[0] | (b[1] << 8) | (b[2] << 16) | (b[3] << 24);
+}
+
+void budget_levels(float q, float* levels) {
+  levels[0] = q;
+  levels[1] = q < 0.9f ? q : 0.9f;
+  levels[2] = q < 0.7f ? q : 0.7f;
+  levels[3] = q < 0.5f ? q : 0.5f;
 }
 
 // ---------------------------------------------------------------------------
@@ -133,6 +153,8 @@ __global__ void __launch_bounds__(64 * kPredWaves) k_predict(PredictArgs g) {
   __bf16* X0 = sX[wv][0];
   __bf16* X1 = sX[wv][1];
   float* L = reinterpret_cast<float*>(sX[wv][0]);  // 32 x 33 logits over both buffers
+  __shared__ unsigned long long sTot[kPredWaves][5];
+  unsigned long long tot[5] = {0, 0, 0, 0, 0};
   const long ntiles = (g.nwork + 31) / 32;
   // uniform trip count per block, so the block barriers below are safe
   const long per_iter = (long)gridDim.x * kPredWaves;
@@ -172,8 +194,28 @@ __global__ void __launch_bounds__(64 * kPredWaves) k_predict(PredictArgs g) {
       }
     }
     __syncthreads();
-    if (l < 32 && valid) g.budget[w] = budget_of(L + l * 33, g.quantile);
+    if (l < 32 && valid) {
+      unsigned ex = 0;
+      const unsigned b = budgets_of(L + l * 33, g.levels, ex);
+      g.budget[w] = b;
+#pragma unroll
+      for (int k = 0; k < 4; k++) tot[k] += (b >> (8 * k)) & 0xffu;
+      tot[4] += ex;
+    }
     __syncthreads();
+  }
+  // block totals: lanes of a wave, then the block's waves, then one atomic each
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    unsigned long long v = tot[k];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (l == 0) sTot[wv][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    unsigned long long v = 0;
+    for (int i = 0; i < kPredWaves; i++) v += sTot[i][threadIdx.x];
+    atomicAdd(g.sums + threadIdx.x, v);
   }
 }
 

@@ -23,8 +23,9 @@ struct PredictArgs {
   const double* bcoords;
   const __bf16* weights;  // forward layouts
   const float* bias;      // [4][64]
-  float quantile;
-  int* budget;  // [nwork]
+  float levels[4];           // quantiles, highest first
+  unsigned* budget;          // [nwork]: byte k = the budget at levels[k]
+  unsigned long long* sums;  // [5]: budget sum per level; 1024 x expected count sum
 };
 
 struct TrainArgs {
@@ -50,5 +51,7 @@ void predictor_collect(Context& C, long nwork, const int* work, const int* rows,
 void predictor_add_samples(Context& C, long n, const float* features, const int* labels);
 void predictor_train(Context& C, int epochs, int batch, float lr, unsigned seed, float* loss_out);
 void predict_budgets(Context& C, const PredictArgs& args);
+// The four budget levels for a quantile q: q and, below it, 0.9, 0.7, 0.5.
+void budget_levels(float q, float* levels);
 
 }  // namespace lsm

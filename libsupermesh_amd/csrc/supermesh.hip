@@ -138,9 +138,18 @@ struct ClipArgs {
   const long* offsets;
   double* elements;
   // single-pass (predicted budget) clip: per work item budget and slot offset
-  const int* budget;
+  const unsigned* budget;  // packed budget levels; this call uses byte bshift / 8
+  int bshift;
   const long* slots;
   double* slot_elems;
+  // outputs beyond a budget go to the walking lane's own arena region (no
+  // atomics: a lane writes one pair at a time, so a pair's extra outputs are
+  // contiguous); aoff[w] = their start, or -1
+  double* arena;
+  long arena_per_lane;
+  long* aoff;
+  unsigned char* dropped;      // single pass: the lane's arena ran out, outputs dropped
+  unsigned long long* ndropped;
   const unsigned char* ovf;  // fill pass: when non-null, only items flagged here
 };
 
@@ -315,6 +324,10 @@ __global__ void __launch_bounds__(kClipBlock) k_clip_tet_walk(ClipArgs g) {
   long w = (long)blockIdx.x * blockDim.x + threadIdx.x;
   long p = 0, off = 0;
   int cap = 0, k = 0;
+  // single pass: this lane's arena region and its fill level
+  const long abase = LEARNED ? ((long)blockIdx.x * blockDim.x + threadIdx.x) * g.arena_per_lane : 0;
+  long atop = 0;
+  bool lost = false;
   double A[12], tol = 0.0, m = 0.0;
   double blk[16], affa[12], affb[12];
   const double* planes = nullptr;
@@ -333,8 +346,9 @@ __global__ void __launch_bounds__(kClipBlock) k_clip_tet_walk(ClipArgs g) {
       if (w >= n) break;
       if (FILL) off = g.offsets[p];
       if (LEARNED) {
-        cap = g.budget[w];
+        cap = (int)((g.budget[w] >> g.bshift) & 0xffu);
         off = g.slots[w];
+        lost = false;
       }
       const long a = g.rows[p], b = g.cols[p];
 #pragma unroll
@@ -363,10 +377,26 @@ __global__ void __launch_bounds__(kClipBlock) k_clip_tet_walk(ClipArgs g) {
         double* o = (LEARNED ? g.slot_elems : g.elements) + (off + k) * 12;
 #pragma unroll
         for (int i = 0; i < 12; i++) o[i] = t[i];
+      } else if (LEARNED) {
+        const long e = atop + (k - cap);
+        if (e < g.arena_per_lane) {
+          double* o = g.arena + (abase + e) * 12;
+#pragma unroll
+          for (int i = 0; i < 12; i++) o[i] = t[i];
+        } else {
+          lost = true;
+        }
       }
       k++;
     });
     if (more) continue;
+    if (LEARNED) {
+      const bool extra = k > cap && !lost;
+      g.aoff[w] = extra ? abase + atop : -1;
+      if (extra) atop += k - cap;
+      g.dropped[w] = lost;
+      if (lost) atomicAdd(g.ndropped, 1ULL);
+    }
     if (!FILL) {
       g.counts[p] = k;
       g.measure[p] = k == 0 ? 0.0 : m;
@@ -540,9 +570,9 @@ static void launch_clip(Context& C, const ClipArgs& args, bool mass, bool fill) 
   LSM_HIP(hipGetLastError());
 }
 
-// which: 0 count, 1 count + mass, 2 fill, 3 single pass, 4 single pass + mass
-static void launch_tet_walk(Context& C, const ClipArgs& args, int which) {
-  // persistent lanes: one full wave of resident blocks
+// which: 0 count, 1 count + mass, 2 fill, 3 single pass, 4 single pass + mass.
+// Persistent lanes: one full wave of resident blocks.
+static unsigned tet_walk_blocks(Context& C, int which) {
   static int blocks_per_cu[5] = {0, 0, 0, 0, 0};
   static const void* kernels[5] = {
       reinterpret_cast<const void*>(k_clip_tet_walk<false, false>),
@@ -557,7 +587,13 @@ static void launch_tet_walk(Context& C, const ClipArgs& args, int which) {
   }
   int cus = 0;
   LSM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, C.device));
-  dim3 grid((unsigned)(blocks_per_cu[which] * (cus > 0 ? cus : 256))), block(kClipBlock);
+  return (unsigned)(blocks_per_cu[which] * (cus > 0 ? cus : 256));
+}
+
+static long tet_walk_lanes(Context& C, int which) { return (long)tet_walk_blocks(C, which) * kClipBlock; }
+
+static void launch_tet_walk(Context& C, const ClipArgs& args, int which) {
+  dim3 grid(tet_walk_blocks(C, which)), block(kClipBlock);
   switch (which) {
     case 0: hipLaunchKernelGGL((k_clip_tet_walk<false, false>), grid, block, 0, C.stream, args); break;
     case 1: hipLaunchKernelGGL((k_clip_tet_walk<true, false>), grid, block, 0, C.stream, args); break;
@@ -572,28 +608,49 @@ static void launch_tet_walk(Context& C, const ClipArgs& args, bool mass, bool fi
   launch_tet_walk(C, args, fill ? 2 : (mass ? 1 : 0));
 }
 
-// Single pass: moves each work item's outputs from its slot to the final
-// layout, or flags it when its output did not fit its budget.
+// Single pass: moves each work item's outputs (its slot, then its arena run)
+// to the final layout.  Items whose outputs were dropped are left to the fill
+// pass.  A wave takes 64 consecutive work items and copies them one after
+// another, its lanes on consecutive 16-byte words.
 __global__ void __launch_bounds__(kClipBlock)
 k_compact_slots(const unsigned long long* __restrict__ nwork, const int* __restrict__ work,
-                const int* __restrict__ counts, const int* __restrict__ budget,
+                const int* __restrict__ counts, const unsigned* __restrict__ budget, int bshift,
                 const long* __restrict__ slots, const double* __restrict__ slot_elems,
-                const long* __restrict__ offsets, double* __restrict__ elements,
-                unsigned char* __restrict__ ovf, unsigned long long* __restrict__ novf) {
+                const long* __restrict__ aoff, const double* __restrict__ arena,
+                const unsigned char* __restrict__ dropped, const long* __restrict__ offsets,
+                double* __restrict__ elements) {
   const long n = (long)*nwork;
-  const long stride = (long)gridDim.x * blockDim.x;
-  for (long w = (long)blockIdx.x * blockDim.x + threadIdx.x; w < n; w += stride) {
-    const long p = work[w];
-    const int c = counts[p];
-    const bool over = c > budget[w];
-    ovf[w] = over;
-    if (over) {
-      atomicAdd(novf, 1ULL);
-      continue;
+  const int lane = threadIdx.x & 63;
+  const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long nwaves = ((long)gridDim.x * blockDim.x) >> 6;
+  for (long base = wave * 64; base < n; base += nwaves * 64) {
+    const long w = base + lane;
+    long src = 0, dst = 0, asrc = 0;
+    int c1 = 0, c2 = 0;
+    if (w < n && !dropped[w]) {
+      const long p = work[w];
+      const int c = counts[p];
+      const int b = (int)((budget[w] >> bshift) & 0xffu);
+      c1 = c < b ? c : b;
+      c2 = c - c1;
+      src = slots[w];
+      asrc = aoff[w];
+      dst = offsets[p];
     }
-    const double2* src = reinterpret_cast<const double2*>(slot_elems + slots[w] * 12);
-    double2* dst = reinterpret_cast<double2*>(elements + offsets[p] * 12);
-    for (int i = 0; i < 6 * c; i++) dst[i] = src[i];
+    for (int k = 0; k < 64; k++) {
+      const int n1 = __shfl(c1, k, 64), n2 = __shfl(c2, k, 64);
+      if (n1 + n2 == 0) continue;
+      const long dk = __shfl(dst, k, 64);
+      double2* d2 = reinterpret_cast<double2*>(elements + dk * 12);
+      if (n1 > 0) {
+        const double2* s2 = reinterpret_cast<const double2*>(slot_elems + __shfl(src, k, 64) * 12);
+        for (int i = lane; i < 6 * n1; i += 64) d2[i] = s2[i];
+      }
+      if (n2 > 0) {
+        const double2* a2 = reinterpret_cast<const double2*>(arena + __shfl(asrc, k, 64) * 12);
+        for (int i = lane; i < 6 * n2; i += 64) d2[6 * n1 + i] = a2[i];
+      }
+    }
   }
 }
 
@@ -601,6 +658,12 @@ __global__ void __launch_bounds__(256)
 k_count_to_long(long n, const int* __restrict__ counts, long* __restrict__ out) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = counts[i];
+}
+
+__global__ void __launch_bounds__(256)
+k_budget_to_long(long n, const unsigned* __restrict__ budget, int shift, long* __restrict__ out) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (budget[i] >> shift) & 0xffu;
 }
 
 void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int flags,
@@ -651,8 +714,8 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
     g.work = work;
     g.nwork = reinterpret_cast<const unsigned long long*>(boff + kClipGrid);
   }
-  const bool learned = elems && dim == 3 && (flags & LIBSUPERMESH_GPU_LEARNED) && npairs > 0;
-  long nwork = 0, nslots = 0;
+  bool learned = elems && dim == 3 && (flags & LIBSUPERMESH_GPU_LEARNED) && npairs > 0;
+  long nwork = 0, nslots = 0, arena_tets = 0;
   if (learned) {
     // single pass: predicted budgets -> slots -> one walk that counts and writes
     Predictor& P = C.predictor;
@@ -660,31 +723,71 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
                 "LIBSUPERMESH_GPU_LEARNED needs predictor weights (libsupermesh_gpu_predictor_set)");
     LSM_HIP(hipMemcpyAsync(&nwork, g.nwork, sizeof(long), hipMemcpyDeviceToHost, C.stream));
     LSM_HIP(hipStreamSynchronize(C.stream));
-    int* budget = P.budget.as<int>((size_t)nwork);
+    unsigned* budget = P.budget.as<unsigned>((size_t)(nwork > 0 ? nwork : 1));
     long* slots = P.slots.as<long>((size_t)nwork + 1);
-    PredictArgs pa = {nwork, g.work, g.rows, g.cols, g.acoords, g.bcoords,
-                      static_cast<const __bf16*>(P.weights.ptr), static_cast<const float*>(P.bias.ptr),
-                      P.quantile, budget};
+    unsigned long long* sums = C.scalars.as<unsigned long long>(16) + 8;
+    LSM_HIP(hipMemsetAsync(sums, 0, 5 * sizeof(unsigned long long), C.stream));
+    PredictArgs pa = {};
+    pa.nwork = nwork;
+    pa.work = g.work;
+    pa.rows = g.rows;
+    pa.cols = g.cols;
+    pa.acoords = g.acoords;
+    pa.bcoords = g.bcoords;
+    pa.weights = static_cast<const __bf16*>(P.weights.ptr);
+    pa.bias = static_cast<const float*>(P.bias.ptr);
+    budget_levels(P.quantile, pa.levels);
+    pa.budget = budget;
+    pa.sums = sums;
     if (C.timing) LSM_HIP(hipEventRecord(C.ev[12], C.stream));
     predict_budgets(C, pa);
     if (C.timing) LSM_HIP(hipEventRecord(C.ev[13], C.stream));
-    LSM_HIP(hipMemsetAsync(slots, 0, sizeof(long), C.stream));
-    if (nwork > 0) {
-      long* wide = C.row_count.as<long>((size_t)nwork);
-      hipLaunchKernelGGL(k_count_to_long, dim3(grid_for(nwork, 256)), dim3(256), 0, C.stream, nwork, budget,
-                         wide);
-      size_t bytes = 0;
-      LSM_HIP(rocprim::inclusive_scan(nullptr, bytes, wide, slots + 1, (size_t)nwork, rocprim::plus<long>(),
-                                      C.stream));
-      void* tmp = C.scan_tmp.get(bytes);
-      LSM_HIP(rocprim::inclusive_scan(tmp, bytes, wide, slots + 1, (size_t)nwork, rocprim::plus<long>(),
-                                      C.stream));
-    }
-    LSM_HIP(hipMemcpyAsync(&nslots, slots + nwork, sizeof(long), hipMemcpyDeviceToHost, C.stream));
+    unsigned long long hs[5] = {0, 0, 0, 0, 0};
+    LSM_HIP(hipMemcpyAsync(hs, sums, sizeof(hs), hipMemcpyDeviceToHost, C.stream));
     LSM_HIP(hipStreamSynchronize(C.stream));
-    g.budget = budget;
-    g.slots = slots;
-    g.slot_elems = P.slot_elems.as<double>((size_t)(nslots > 0 ? nslots : 1) * 12);
+    // the highest quantile level whose slots, with room for the expected
+    // output beside them, fit in the device memory left
+    size_t free_b = 0, total_b = 0;
+    LSM_HIP(hipMemGetInfo(&free_b, &total_b));
+    const double avail = (double)free_b + (double)P.slot_elems.bytes + (double)P.arena.bytes +
+                         (double)C.elements.bytes - 4e9;
+    const double expect = (double)hs[4] / 1024.0;
+    // the arena holds outputs beyond the budgets: a quarter of the expected
+    // output, spread over the walk's lanes
+    arena_tets = (long)(P.arena_fraction * expect) + P.arena_min;
+    int level = -1;
+    for (int k = 0; k < 4 && level < 0; k++)
+      if (((double)hs[k] + (double)arena_tets + 1.1 * expect + 1.0) * 96.0 <= avail) level = k;
+    if (level < 0) {
+      learned = false;  // no room: the plain count-then-fill path (same output)
+    } else {
+      g.bshift = 8 * level;
+      P.level = level;
+      LSM_HIP(hipMemsetAsync(slots, 0, sizeof(long), C.stream));
+      if (nwork > 0) {
+        long* wide = C.row_count.as<long>((size_t)nwork);
+        hipLaunchKernelGGL(k_budget_to_long, dim3(grid_for(nwork, 256)), dim3(256), 0, C.stream, nwork, budget,
+                           g.bshift, wide);
+        size_t bytes = 0;
+        LSM_HIP(rocprim::inclusive_scan(nullptr, bytes, wide, slots + 1, (size_t)nwork, rocprim::plus<long>(),
+                                        C.stream));
+        void* tmp = C.scan_tmp.get(bytes);
+        LSM_HIP(rocprim::inclusive_scan(tmp, bytes, wide, slots + 1, (size_t)nwork, rocprim::plus<long>(),
+                                        C.stream));
+      }
+      LSM_HIP(hipMemcpyAsync(&nslots, slots + nwork, sizeof(long), hipMemcpyDeviceToHost, C.stream));
+      LSM_HIP(hipStreamSynchronize(C.stream));
+      g.budget = budget;
+      g.slots = slots;
+      g.slot_elems = P.slot_elems.as<double>((size_t)(nslots > 0 ? nslots : 1) * 12);
+      const long lanes = tet_walk_lanes(C, mass ? 4 : 3);  // the grid of the single-pass launch
+      g.arena_per_lane = std::max(1L, arena_tets / lanes);
+      g.arena = P.arena.as<double>((size_t)(g.arena_per_lane * lanes) * 12);
+      g.aoff = P.aoff.as<long>((size_t)(nwork > 0 ? nwork : 1));
+      g.dropped = P.overflow.as<unsigned char>((size_t)(nwork > 0 ? nwork : 1));
+      g.ndropped = C.scalars.as<unsigned long long>(16) + 1;
+      LSM_HIP(hipMemsetAsync(g.ndropped, 0, sizeof(unsigned long long), C.stream));
+    }
   }
   if (C.timing) LSM_HIP(hipEventRecord(C.ev[14], C.stream));
   if (dim == 2) launch_clip<2>(C, g, mass, false);
@@ -732,19 +835,17 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
       g.offsets = offs;
       g.elements = C.elements.as<double>((size_t)(out->nout > 0 ? out->nout : 1) * loc * dim);
       if (learned) {
-        Predictor& P = C.predictor;
-        unsigned char* ovf = P.overflow.as<unsigned char>((size_t)(nwork > 0 ? nwork : 1));
-        unsigned long long* novf = C.scalars.as<unsigned long long>(8) + 1;
-        LSM_HIP(hipMemsetAsync(novf, 0, sizeof(unsigned long long), C.stream));
         hipLaunchKernelGGL(k_compact_slots, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, g.nwork, g.work,
-                           g.counts, g.budget, g.slots, g.slot_elems, offs, g.elements, ovf, novf);
+                           g.counts, g.budget, g.bshift, g.slots, g.slot_elems, g.aoff, g.arena, g.dropped,
+                           offs, g.elements);
         LSM_HIP(hipGetLastError());
         unsigned long long h = 0;
-        LSM_HIP(hipMemcpyAsync(&h, novf, sizeof(h), hipMemcpyDeviceToHost, C.stream));
+        LSM_HIP(hipMemcpyAsync(&h, g.ndropped, sizeof(h), hipMemcpyDeviceToHost, C.stream));
         LSM_HIP(hipStreamSynchronize(C.stream));
         if (h > 0) {
-          // pairs whose output outgrew the budget: the plain fill pass, for them only
-          g.ovf = ovf;
+          // pairs whose outputs did not fit budget and arena: the plain fill
+          // pass, for them only
+          g.ovf = g.dropped;
           launch_tet_walk(C, g, 2);
           g.ovf = nullptr;
         }

@@ -100,6 +100,8 @@ def lib():
         L.libsupermesh_gpu_predictor_infer.argtypes = [c_void_p, POINTER(Mesh), POINTER(Mesh), c_long,
                                                        c_void_p, c_void_p, c_void_p]
         L.libsupermesh_gpu_predictor_add_samples.argtypes = [c_void_p, c_long, fp, POINTER(c_int)]
+        L.libsupermesh_gpu_predictor_arena.argtypes = [c_void_p, ctypes.c_float, c_long]
+        L.libsupermesh_gpu_predictor_arena.restype = c_int
         L.libsupermesh_gpu_checksum.argtypes = [c_void_p, c_void_p, ctypes.c_size_t,
                                                 POINTER(ctypes.c_ulonglong)]
         L.libsupermesh_gpu_checksum.restype = c_int
@@ -419,6 +421,9 @@ class Context:
 
     def predictor_quantile(self, q):
         _check(lib().libsupermesh_gpu_predictor_quantile(self.handle, ctypes.c_float(q)))
+
+    def predictor_arena(self, fraction, min_tets):
+        _check(lib().libsupermesh_gpu_predictor_arena(self.handle, ctypes.c_float(fraction), min_tets))
 
     def predictor_collect(self, a: DeviceMesh, b: DeviceMesh, max_samples):
         n = c_long(0)

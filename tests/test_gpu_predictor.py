@@ -25,6 +25,7 @@ def test_inference_matches_numpy(meshes):
     ctx = gpu.Context()
     params = R.init(3, scale=3.0)
     ctx.predictor_set(params)
+    ctx.predictor_quantile(0.99)
     got = ctx.predictor_infer(gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pb, eb), rows, cols)
     x = R.features(pa[ea[rows]], pb[eb[cols]])
     logits, _ = R.forward(params, x)
@@ -71,9 +72,12 @@ def test_training_learns_clip_counts(meshes):
 
 @pytest.mark.parametrize("weights", ["trained", "random", "zero"])
 @pytest.mark.parametrize("quantile", [0.5, 0.99])
-def test_learned_single_pass_is_identical(meshes, weights, quantile):
+@pytest.mark.parametrize("arena", ["default", "tiny"])
+def test_learned_single_pass_is_identical(meshes, weights, quantile, arena):
     pa, ea, pb, eb = meshes
     ctx = gpu.Context()
+    if arena == "tiny":
+        ctx.predictor_arena(0.0, 1)   # one tet per lane: most extras are dropped and re-clipped
     A, B = gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pb, eb)
     if weights == "trained":
         ctx.predictor_collect(A, B, 1 << 20)
@@ -95,6 +99,6 @@ def test_learned_single_pass_is_identical(meshes, weights, quantile):
     if weights == "zero" and quantile == 0.99:
         # uniform prediction: the 0.99 quantile is the last class, budget = maximum
         assert r1.learned_overflow == 0
-    if weights == "random":
+    if weights == "random" and arena == "tiny":
         assert r1.learned_overflow > 0    # the fallback path was exercised
     assert r1.learned_slots >= 0
