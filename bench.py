@@ -262,7 +262,9 @@ def main():
     if not args.no_batched:
         batched = batched_workload(ctx, max(1, min(args.steps, 3)), 1, count=args.batch_count)
     learned = None
-    if not args.no_learned and os.path.exists(args.weights):
+    if not args.no_learned:
+        if not os.path.exists(args.weights):
+            raise SystemExit(f"predictor weights not found: {args.weights}")
         learned = learned_workload(ctx, args.learned_workload, max(1, min(args.steps, 3)), 1, args.weights)
     fpp = FLOPS_PER_PAIR.get(args.workload)
     clip_s = main_w["clip_kernel_ms"] / 1e3
