@@ -925,12 +925,12 @@ k_segment_sum(long n, const uint64_t* __restrict__ keys, const double* __restric
 // element e containing i (local vertex li) and every non-empty pair p of e,
 // the entries (B node of local vertex j, blocks[p][li][j]).  One wave owns a
 // row: it gathers the row's entries into LDS in a fixed order (elements of i
-// in ascending (e, li), pairs of e in CSR order, j ascending), sorts them by
-// B node with a bitonic network, and sums each run of equal keys in sorted
-// order -- no global sort, and the same result on every run.
+// in ascending (e, li), pairs of e in CSR order, j ascending), finds the
+// distinct B nodes with an LDS hash table, sorts them, and each lane sums
+// its columns' entries in gather order -- no global sort, and the same result
+// on every run.  Rows too long for LDS take the global sort path.
 // ---------------------------------------------------------------------------
 
-constexpr int kRowBlock = 64;
 
 __global__ void __launch_bounds__(256)
 k_node_count(long ne, int loc, const int* __restrict__ enl, unsigned int* __restrict__ ncount) {
@@ -970,31 +970,6 @@ k_node_rows(long nn, int loc, const long* __restrict__ nptr, int* __restrict__ n
   for (long x = b0; x < b1; x++) contrib[x] = (long)elem_nonempty[nlist[x] / loc] * loc;
 }
 
-// Writes the row entries of node-list entry x (element e, local vertex li) at
-// slot[x]: B node and block value for each non-empty pair of e, in CSR order.
-template <int LOC>
-__global__ void __launch_bounds__(256)
-k_mass_scatter(long nent, const int* __restrict__ nlist, const long* __restrict__ slot,
-               const long* __restrict__ indptr, const int* __restrict__ cols,
-               const int* __restrict__ counts, const double* __restrict__ blocks,
-               const int* __restrict__ enlb, int* __restrict__ rkey, double* __restrict__ rval) {
-  long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= nent) return;
-  const int el = nlist[x];
-  const int e = el / LOC, li = el - e * LOC;
-  long o = slot[x];
-  for (long p = indptr[e]; p < indptr[e + 1]; p++) {
-    if (counts[p] == 0) continue;
-    const long b = cols[p];
-#pragma unroll
-    for (int j = 0; j < LOC; j++) {
-      rkey[o + j] = enlb[b * LOC + j];
-      rval[o + j] = blocks[p * LOC * LOC + li * LOC + j];
-    }
-    o += LOC;
-  }
-}
-
 __global__ void __launch_bounds__(256)
 k_row_bounds(long nn, const long* __restrict__ nptr, const long* __restrict__ slot,
              long* __restrict__ raw_off, long* __restrict__ raw) {
@@ -1012,119 +987,6 @@ k_elem_nonempty(long ne, const long* __restrict__ indptr, const int* __restrict_
   int c = 0;
   for (long p = indptr[e]; p < indptr[e + 1]; p++) c += counts[p] > 0;
   out[e] = c;
-}
-
-// Compare-exchange of registers r and r ^ JR (distance 64*JR) when the
-// stage's distance is 64*jr == 64*JR; JR is a compile-time constant so the
-// register indices stay static.
-template <int EPL, int JR>
-__device__ __forceinline__ void reg_stage(int* key, double* val, int lane, int n2, int k, int jr) {
-  if constexpr (JR < EPL) {
-    if (jr != JR) return;
-#pragma unroll
-    for (int r = 0; r < EPL; r++) {
-      const int r2 = r ^ JR;
-      if (r2 > r && 64 * r2 < n2) {
-        const int t = lane + 64 * r;
-        const bool up = (t & k) == 0;
-        if ((key[r] > key[r2]) == up) {
-          const int kt = key[r];
-          key[r] = key[r2];
-          key[r2] = kt;
-          const double vt = val[r];
-          val[r] = val[r2];
-          val[r2] = vt;
-        }
-      }
-    }
-  }
-}
-
-// Bitonic sort of n2 = 64*EPL (key, val) pairs held in registers: element
-// t = lane + 64*r.  Partners at distance j < 64 sit in another lane (same r)
-// and are exchanged with __shfl_xor; larger distances pair registers r and
-// r ^ (j/64) of the same lane.
-template <int EPL>
-__device__ __forceinline__ void wave_bitonic(int* key, double* val, int lane, int n2) {
-  for (int k = 2; k <= n2; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      if (j < 64) {
-#pragma unroll
-        for (int r = 0; r < EPL; r++) {
-          const int t = lane + 64 * r;
-          const int ok = __shfl_xor(key[r], j);
-          const double ov = __shfl_xor(val[r], j);
-          const bool lower = (t & j) == 0;        // t is the lower index of its pair
-          const bool up = (t & k) == 0;           // ascending block
-          const bool take_min = lower == up;
-          const bool swap = take_min ? (ok < key[r]) : (ok > key[r]);
-          if (t < n2 && swap) {
-            key[r] = ok;
-            val[r] = ov;
-          }
-        }
-      } else {
-        const int jr = j >> 6;
-        reg_stage<EPL, 1>(key, val, lane, n2, k, jr);
-        reg_stage<EPL, 2>(key, val, lane, n2, k, jr);
-        reg_stage<EPL, 4>(key, val, lane, n2, k, jr);
-        reg_stage<EPL, 8>(key, val, lane, n2, k, jr);
-        reg_stage<EPL, 16>(key, val, lane, n2, k, jr);
-      }
-    }
-  }
-}
-
-template <int EPL>
-__global__ void __launch_bounds__(kRowBlock)
-k_mass_rows(long nn, const long* __restrict__ raw_off, int* __restrict__ tkey, double* __restrict__ tval,
-            long* __restrict__ ucount) {
-  constexpr int CAP = 64 * EPL;
-  __shared__ int skey[CAP];
-  __shared__ double sval[CAP];
-  const int lane = threadIdx.x;
-  for (long i = blockIdx.x; i < nn; i += gridDim.x) {
-    const long o0 = raw_off[i];
-    const int n = (int)(raw_off[i + 1] - o0);
-    int n2 = 64;
-    while (n2 < n) n2 <<= 1;
-    int key[EPL];
-    double val[EPL];
-#pragma unroll
-    for (int r = 0; r < EPL; r++) {
-      const int t = lane + 64 * r;
-      key[r] = t < n ? tkey[o0 + t] : 0x7fffffff;
-      val[r] = t < n ? tval[o0 + t] : 0.0;
-    }
-    wave_bitonic<EPL>(key, val, lane, n2);
-#pragma unroll
-    for (int r = 0; r < EPL; r++) {
-      const int t = lane + 64 * r;
-      if (t < n2) {
-        skey[t] = key[r];
-        sval[t] = val[r];
-      }
-    }
-    __syncthreads();
-    // runs of equal keys: the head of each run sums it in sorted order, and
-    // the row is rewritten in place
-    int before = 0;
-    for (int t0 = 0; t0 < n; t0 += kRowBlock) {
-      const int t = t0 + lane;
-      const bool head = t < n && (t == 0 || skey[t] != skey[t - 1]);
-      const unsigned long long m = __ballot(head);
-      if (head) {
-        double sacc = 0.0;
-        for (int r = t; r < n && skey[r] == skey[t]; r++) sacc = sacc + sval[r];
-        const long o = o0 + before + __popcll(m & ((1ULL << lane) - 1));
-        tkey[o] = skey[t];
-        tval[o] = sacc;
-      }
-      before += __popcll(m);
-    }
-    if (lane == 0) ucount[i] = before;
-    __syncthreads();
-  }
 }
 
 __global__ void __launch_bounds__(256)
@@ -1211,10 +1073,182 @@ static void assemble_mass_sorted(Context& C, const DevMesh& A, const DevMesh& B,
   out->mass_values = mval;
 }
 
+// Bitonic sort of up to 64*EPL int keys in registers (element t = lane + 64*r),
+// keys only.
 template <int EPL>
-static void launch_rows(Context& C, long nn, const long* raw_off, int* tkey, double* tval, long* ucount) {
-  hipLaunchKernelGGL((k_mass_rows<EPL>), dim3(8192), dim3(kRowBlock), 0, C.stream, nn, raw_off, tkey, tval,
-                     ucount);
+__device__ __forceinline__ void wave_bitonic_keys(int* key, int lane) {
+  constexpr int N2 = 64 * EPL;
+  for (int k = 2; k <= N2; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j < 64) {
+#pragma unroll
+        for (int r = 0; r < EPL; r++) {
+          const int t = lane + 64 * r;
+          const int ok = __shfl_xor(key[r], j);
+          const bool take_min = ((t & j) == 0) == ((t & k) == 0);
+          key[r] = take_min ? (ok < key[r] ? ok : key[r]) : (ok > key[r] ? ok : key[r]);
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < EPL; r++) {
+#pragma unroll
+          for (int r2 = 0; r2 < EPL; r2++) {
+            if (r2 == (r ^ (j >> 6)) && r2 > r) {
+              const int t = lane + 64 * r;
+              const bool up = (t & k) == 0;
+              if ((key[r] > key[r2]) == up) {
+                const int kt = key[r];
+                key[r] = key[r2];
+                key[r2] = kt;
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+constexpr int kHashSlots = 512;   // per-row table of distinct B nodes
+constexpr int kMaxDistinct = 256;  // columns a row may hold on this path
+
+// One wave per row of the mixed mass: gathers the row's contributions (its
+// incident A elements in node-list order, their non-empty pairs in CSR order,
+// B's local nodes in order) into LDS, finds the distinct columns with an LDS
+// hash table, sorts them, and sums each column's contributions in gather
+// order -- a fixed order, so the result is reproducible.  Writes the row's
+// (column, value) list at raw_off[i] and its length to ucount[i]; rows whose
+// distinct columns exceed kMaxDistinct raise *overflow (the caller then uses
+// the global sort path).
+template <int LOC, int CAP, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES)
+k_mass_rows_gather(long nn, const long* __restrict__ nptr, const int* __restrict__ nlist,
+                   const long* __restrict__ indptr, const int* __restrict__ cols,
+                   const int* __restrict__ counts, const double* __restrict__ blocks,
+                   const int* __restrict__ enlb, const long* __restrict__ raw_off, int* __restrict__ tkey,
+                   double* __restrict__ tval, long* __restrict__ ucount, int* __restrict__ overflow) {
+  __shared__ int s_rk[WAVES][CAP];
+  __shared__ double s_rv[WAVES][CAP];
+  __shared__ int s_ht[WAVES][kHashSlots];
+  __shared__ int s_dk[WAVES][kMaxDistinct];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int* rk = s_rk[wv];
+  double* rv = s_rv[wv];
+  int* ht = s_ht[wv];
+  int* dk = s_dk[wv];
+  const long nwaves = (long)gridDim.x * WAVES;
+  // the same trip count and barrier sequence for every wave of the block
+  for (long base = (long)blockIdx.x * WAVES; base < nn; base += nwaves) {
+    const long i = base + wv;
+    const bool valid = i < nn;
+    // 1. gather: up to 64 incident elements at a time, one per lane; their
+    // pair ranges are laid end to end and the lanes walk that flat list
+    int n = 0;
+    if (valid) {
+      const long x0 = nptr[i], x1 = nptr[i + 1];
+      for (long xb = x0; xb < x1; xb += 64) {
+        const long x = xb + lane;
+        long p0 = 0;
+        int np = 0, li = 0;
+        if (x < x1) {
+          const int el = nlist[x];
+          const int e = el / LOC;
+          li = el - e * LOC;
+          p0 = indptr[e];
+          np = (int)(indptr[e + 1] - p0);
+        }
+        int incl = np;
+        for (int o = 1; o < 64; o <<= 1) {
+          const int v = __shfl_up(incl, o, 64);
+          if (lane >= o) incl += v;
+        }
+        const int excl = incl - np;
+        const int total = __shfl(incl, 63, 64);
+        for (int f0 = 0; f0 < total; f0 += 64) {
+          const int f = f0 + lane;
+          // owner: the last lane whose range starts at or before f
+          int own = 0;
+          for (int step = 32; step > 0; step >>= 1) {
+            const int cand = own + step;
+            const int ev = __shfl(excl, cand < 64 ? cand : 63, 64);
+            if (cand < 64 && ev <= f) own = cand;
+          }
+          const long p = __shfl(p0, own, 64) + (f - __shfl(excl, own, 64));
+          const int lio = __shfl(li, own, 64);
+          const bool ne = f < total && counts[p] > 0;
+          const unsigned long long m = __ballot(ne);
+          if (ne) {
+            const int pos = n + LOC * __popcll(m & ((1ULL << lane) - 1));
+            const long b = cols[p];
+#pragma unroll
+            for (int j = 0; j < LOC; j++) {
+              rk[pos + j] = enlb[b * LOC + j];
+              rv[pos + j] = blocks[p * LOC * LOC + lio * LOC + j];
+            }
+          }
+          n += LOC * __popcll(m);
+        }
+      }
+    }
+    for (int t = lane; t < kHashSlots; t += 64) ht[t] = -1;
+    __syncthreads();
+    // 2. distinct columns (bounded probing: at most kHashSlots tries)
+    bool full = false;
+    for (int t = lane; t < n; t += 64) {
+      const int key = rk[t];
+      unsigned h = ((unsigned)key * 2654435761u) >> (32 - 9);
+      int tries = 0;
+      while (true) {
+        const int prev = atomicCAS(&ht[h], -1, key);
+        if (prev == -1 || prev == key) break;
+        h = (h + 1) & (kHashSlots - 1);
+        if (++tries >= kHashSlots) {
+          full = true;
+          break;
+        }
+      }
+    }
+    __syncthreads();
+    int nd = 0;
+    for (int s0 = 0; s0 < kHashSlots; s0 += 64) {
+      const int key = ht[s0 + lane];
+      const bool occ = key != -1;
+      const unsigned long long m = __ballot(occ);
+      const int pos = nd + __popcll(m & ((1ULL << lane) - 1));
+      if (occ && pos < kMaxDistinct) dk[pos] = key;
+      nd += __popcll(m);
+    }
+    const bool bad = __ballot(full) != 0ULL || nd > kMaxDistinct;
+    if (bad && lane == 0) atomicOr(overflow, 1);
+    __syncthreads();
+    if (valid && !bad) {
+      // 3. sort the distinct columns, 4. sum each in gather order
+      int key[4];
+#pragma unroll
+      for (int r = 0; r < 4; r++) key[r] = lane + 64 * r < nd ? dk[lane + 64 * r] : 0x7fffffff;
+      if (nd <= 64) wave_bitonic_keys<1>(key, lane);
+      else wave_bitonic_keys<4>(key, lane);
+      double acc[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int t = 0; t < n; t++) {
+        const int kt = rk[t];
+        const double vt = rv[t];
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+          if (kt == key[r]) acc[r] = acc[r] + vt;
+      }
+      const long o = raw_off[i];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int t = lane + 64 * r;
+        if (t < nd) {
+          tkey[o + t] = key[r];
+          tval[o + t] = acc[r];
+        }
+      }
+      if (lane == 0) ucount[i] = nd;
+    }
+    __syncthreads();  // the LDS rows are refilled next
+  }
 }
 
 void assemble_mass(Context& C, const DevMesh& A, const DevMesh& B, long npairs,
@@ -1280,18 +1314,35 @@ void assemble_mass(Context& C, const DevMesh& A, const DevMesh& B, long npairs,
   int* tkey = C.mkeys.as<int>(total_raw + 1);
   double* tval = C.mvals.as<double>(total_raw + 1);
   long* ucount = C.mvals_alt.as<long>(nn + 1);
-  if (loc == 3)
-    hipLaunchKernelGGL(k_mass_scatter<3>, dim3(grid_for(nent, 256)), dim3(256), 0, C.stream, nent, nlist,
-                       slot, out->indptr, out->indices, out->counts, out->blocks, B.enlist_dev, tkey, tval);
-  else
-    hipLaunchKernelGGL(k_mass_scatter<4>, dim3(grid_for(nent, 256)), dim3(256), 0, C.stream, nent, nlist,
-                       slot, out->indptr, out->indices, out->counts, out->blocks, B.enlist_dev, tkey, tval);
-  if (maxraw <= 128) launch_rows<2>(C, nn, raw_off, tkey, tval, ucount);
-  else if (maxraw <= 256) launch_rows<4>(C, nn, raw_off, tkey, tval, ucount);
-  else if (maxraw <= 512) launch_rows<8>(C, nn, raw_off, tkey, tval, ucount);
-  else if (maxraw <= 1024) launch_rows<16>(C, nn, raw_off, tkey, tval, ucount);
-  else launch_rows<32>(C, nn, raw_off, tkey, tval, ucount);
+  int* ovf = C.scalars.as<int>(64) + 4;
+  LSM_HIP(hipMemsetAsync(ovf, 0, sizeof(int), C.stream));
+  {
+    // rows gathered, deduplicated and summed in one pass (LDS capacity by the
+    // longest row)
+    int cus = 0;
+    LSM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, C.device));
+    const dim3 grid((unsigned)std::max<long>(1, std::min<long>((long)(cus > 0 ? cus : 256) * 16, nn)));
+#define LSM_ROWS(L, CAPV, W)                                                                           \
+  hipLaunchKernelGGL((k_mass_rows_gather<L, CAPV, W>), grid, dim3(64 * W), 0, C.stream, nn, nptr, nlist,  \
+                     out->indptr, out->indices, out->counts, out->blocks, B.enlist_dev, raw_off, tkey,     \
+                     tval, ucount, ovf)
+    if (loc == 3) {
+      if (maxraw <= 512) LSM_ROWS(3, 512, 4);
+      else LSM_ROWS(3, 2048, 2);
+    } else {
+      if (maxraw <= 512) LSM_ROWS(4, 512, 4);
+      else LSM_ROWS(4, 2048, 2);
+    }
+#undef LSM_ROWS
+  }
   LSM_HIP(hipGetLastError());
+  int hovf = 0;
+  LSM_HIP(hipMemcpyAsync(&hovf, ovf, sizeof(int), hipMemcpyDeviceToHost, C.stream));
+  LSM_HIP(hipStreamSynchronize(C.stream));
+  if (hovf) {  // a row with more distinct columns than the row kernel holds
+    assemble_mass_sorted(C, A, B, npairs, out);
+    return;
+  }
   inclusive_scan_long(C, ucount, mindptr + 1, nn);
   long nnz = 0;
   LSM_HIP(hipMemcpyAsync(&nnz, mindptr + nn, sizeof(long), hipMemcpyDeviceToHost, C.stream));
