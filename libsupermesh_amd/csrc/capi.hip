@@ -69,7 +69,7 @@ static void free_context(Context& C) {
                     &C.predictor.features, &C.predictor.labels, &C.predictor.act,
                     &C.predictor.loss, &C.predictor.index, &C.predictor.budget,
                     &C.predictor.slots, &C.predictor.slot_elems, &C.predictor.overflow,
-                    &C.predictor.arena, &C.predictor.aoff};
+                    &C.predictor.arena, &C.predictor.aoff, &C.fscratch};
   for (DevBuf* b : bufs) b->release();
   if (C.pinned) hipHostFree(C.pinned);
   C.pinned = nullptr;

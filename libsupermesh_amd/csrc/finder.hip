@@ -20,6 +20,7 @@
 #include <vector>
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_reduce.hpp>
 #include <rocprim/device/device_scan.hpp>
 
 #include "lsm_geometry.hpp"
@@ -494,10 +495,12 @@ __global__ void __launch_bounds__(kBlock)
 k_query(long na, const int* __restrict__ qperm, const double* __restrict__ abox, LevelInfo info,
         const double* __restrict__ boxes, const int* __restrict__ perm, long* __restrict__ row_count,
         const long* __restrict__ indptr, int* __restrict__ indices, int* __restrict__ rows,
-        const int* __restrict__ agroup, const int* __restrict__ granges) {
+        const int* __restrict__ agroup, const int* __restrict__ granges, long skip_upto) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= na) return;
   const int a = qperm[t];
+  // fill pass after the single pass: only the rows that outgrew its scratch
+  if (FILL && row_count[a] <= skip_upto) return;
   // in a batch, only nodes whose mesh range holds A's mesh are visited
   const int g = GROUPED ? agroup[a] : 0;
   double qlo[DIM], qhi[DIM];
@@ -552,6 +555,125 @@ k_query(long na, const int* __restrict__ qperm, const double* __restrict__ abox,
   if (!FILL) row_count[a] = cnt;
 }
 
+// Boxes of the up to F children first .. first+F-1 of level K that overlap
+// the query, as a bit mask.
+template <int DIM, bool CLOSED, bool GROUPED>
+__device__ __forceinline__ unsigned child_mask(const LevelInfo& info, const double* __restrict__ boxes,
+                                               const int* __restrict__ granges, int K, long first,
+                                               const double* qlo, const double* qhi, int g) {
+  const long n = info.count[K];
+  const long base = info.offset[K] + first;
+  unsigned m = 0;
+#pragma unroll
+  for (int c = 0; c < kFanout; c++) {
+    if (first + c < n) {
+      bool hit = overlaps<DIM, CLOSED>(boxes + (base + c) * 2 * DIM, qlo, qhi);
+      if (GROUPED) {
+        const int* r = granges + 2 * (base + c);
+        hit = hit && r[0] <= g && g <= r[1];
+      }
+      m |= (unsigned)hit << c;
+    }
+  }
+  return m;
+}
+
+// Single-pass search: the same depth-first order as k_query, but a node's
+// children are tested together (their boxes are contiguous) and the walk
+// keeps, per level, a byte of the hit children not yet visited; the node
+// index is implicit (parent = child / F).  The first `cap` results of a query
+// go to scratch (slot-major: scratch[k * na + t]); all are counted.
+template <int DIM, bool CLOSED, bool GROUPED>
+__global__ void __launch_bounds__(kBlock)
+k_query_mask(long na, const int* __restrict__ qperm, const double* __restrict__ abox, LevelInfo info,
+             const double* __restrict__ boxes, const int* __restrict__ perm, int cap,
+             int* __restrict__ scratch, long* __restrict__ row_count, const int* __restrict__ agroup,
+             const int* __restrict__ granges) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= na) return;
+  const int a = qperm[t];
+  const int g = GROUPED ? agroup[a] : 0;
+  double qlo[DIM], qhi[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    qlo[d] = abox[(long)d * na + a];
+    qhi[d] = abox[(long)(DIM + d) * na + a];
+  }
+  long cnt = 0;
+  auto emit = [&](long leaf) {
+    if (cnt < cap) scratch[cnt * na + t] = perm[leaf];
+    cnt++;
+  };
+  const int top = info.levels - 1;
+  if (top == 0) {
+    if (child_mask<DIM, CLOSED, GROUPED>(info, boxes, granges, 0, 0, qlo, qhi, g) & 1u) emit(0);
+  } else if (child_mask<DIM, CLOSED, GROUPED>(info, boxes, granges, top, 0, qlo, qhi, g) & 1u) {
+    // masks of levels 1..16, a byte each
+    unsigned long long mlo = 0, mhi = 0;
+    auto get = [&](int L) -> unsigned {
+      return L <= 8 ? (unsigned)(mlo >> (8 * (L - 1))) & 0xffu : (unsigned)(mhi >> (8 * (L - 9))) & 0xffu;
+    };
+    auto set = [&](int L, unsigned m) {
+      if (L <= 8) {
+        const int sh = 8 * (L - 1);
+        mlo = (mlo & ~(0xffULL << sh)) | ((unsigned long long)m << sh);
+      } else {
+        const int sh = 8 * (L - 9);
+        mhi = (mhi & ~(0xffULL << sh)) | ((unsigned long long)m << sh);
+      }
+    };
+    int L = top;
+    long node = 0;
+    set(L, child_mask<DIM, CLOSED, GROUPED>(info, boxes, granges, top - 1, 0, qlo, qhi, g));
+    while (true) {
+      unsigned m = get(L);
+      if (m == 0) {
+        if (L == top) break;
+        node /= kFanout;
+        L++;
+        continue;
+      }
+      if (L == 1) {  // children are elements
+        while (m) {
+          const int c = __builtin_ctz(m);
+          m &= m - 1;
+          emit(node * kFanout + c);
+        }
+        set(1, 0);
+        continue;
+      }
+      const int c = __builtin_ctz(m);
+      set(L, m & (m - 1));
+      const long child = node * kFanout + c;
+      const unsigned mm = child_mask<DIM, CLOSED, GROUPED>(info, boxes, granges, L - 2, child * kFanout, qlo,
+                                                           qhi, g);
+      if (mm) {
+        node = child;
+        L--;
+        set(L, mm);
+      }
+    }
+  }
+  row_count[a] = cnt;
+}
+
+// Rows that fit the scratch: copy them into the CSR arrays.
+__global__ void __launch_bounds__(kBlock)
+k_rows_from_scratch(long na, const int* __restrict__ qperm, int cap, const int* __restrict__ scratch,
+                    const long* __restrict__ row_count, const long* __restrict__ indptr,
+                    int* __restrict__ indices, int* __restrict__ rows) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= na) return;
+  const int a = qperm[t];
+  const long n = row_count[a];
+  if (n > cap) return;
+  const long o = indptr[a];
+  for (long k = 0; k < n; k++) {
+    indices[o + k] = scratch[k * na + t];
+    rows[o + k] = a;
+  }
+}
+
 // Insertion sort of each row by B id (rows are short: tens of entries).
 __global__ void __launch_bounds__(kBlock)
 k_sort_rows(long na, const int* __restrict__ qperm, const long* __restrict__ indptr,
@@ -598,25 +720,40 @@ long find_pairs(Context& C, const DevMesh& A, const DevMesh& B, int finder, int 
               LIBSUPERMESH_GPU_ERROR_INVALID, "batches of A and B hold different mesh counts");
   const int* agroup = H.grouped ? static_cast<const int*>(A.group.ptr) : nullptr;
   const int* granges = H.grouped ? static_cast<const int*>(H.granges.ptr) : nullptr;
-#define LSM_QUERY(DIM, CL, FILL)                                                                    \
+#define LSM_QUERY(DIM, CL, FILL, SKIP)                                                              \
   do {                                                                                            \
     if (H.grouped)                                                                                \
       hipLaunchKernelGGL((k_query<DIM, CL, FILL, true>), grid, block, 0, C.stream, na, qperm, abox, \
-                         info, boxes, perm, row_count, indptr, indices, rows, agroup, granges);   \
+                         info, boxes, perm, row_count, indptr, indices, rows, agroup, granges,    \
+                         (long)(SKIP));                                                           \
     else                                                                                          \
       hipLaunchKernelGGL((k_query<DIM, CL, FILL, false>), grid, block, 0, C.stream, na, qperm,      \
                          abox, info, boxes, perm, row_count, indptr, indices, rows, agroup,        \
-                         granges);                                                                \
+                         granges, (long)(SKIP));                                                  \
+  } while (0)
+#define LSM_MASK(DIM, CL)                                                                           \
+  do {                                                                                            \
+    if (H.grouped)                                                                                \
+      hipLaunchKernelGGL((k_query_mask<DIM, CL, true>), grid, block, 0, C.stream, na, qperm, abox,  \
+                         info, boxes, perm, cap, scratch, row_count, agroup, granges);            \
+    else                                                                                          \
+      hipLaunchKernelGGL((k_query_mask<DIM, CL, false>), grid, block, 0, C.stream, na, qperm, abox, \
+                         info, boxes, perm, cap, scratch, row_count, agroup, granges);            \
   } while (0)
   int* indices = nullptr;
   int* rows = nullptr;
   bool closed = finder == LIBSUPERMESH_GPU_FINDER_RTREE;
+  // single pass into a scratch of `cap` results per query; longer rows are
+  // searched again by the fill pass
+  const int cap = A.dim == 2 ? 32 : 96;
+  LSM_REQUIRE(H.levels <= 17, LIBSUPERMESH_GPU_ERROR_OVERFLOW, "hierarchy too deep");
+  int* scratch = C.fscratch.as<int>((size_t)cap * na);
   if (A.dim == 2) {
-    if (closed) LSM_QUERY(2, true, false);
-    else LSM_QUERY(2, false, false);
+    if (closed) LSM_MASK(2, true);
+    else LSM_MASK(2, false);
   } else {
-    if (closed) LSM_QUERY(3, true, false);
-    else LSM_QUERY(3, false, false);
+    if (closed) LSM_MASK(3, true);
+    else LSM_MASK(3, false);
   }
   LSM_HIP(hipGetLastError());
   size_t tmp_bytes = 0;
@@ -625,19 +762,33 @@ long find_pairs(Context& C, const DevMesh& A, const DevMesh& B, int finder, int 
   void* tmp = C.scan_tmp.get(tmp_bytes);
   LSM_HIP(rocprim::inclusive_scan(tmp, tmp_bytes, row_count, indptr + 1, (size_t)na,
                                   rocprim::plus<long>(), C.stream));
-  long npairs = 0;
-  LSM_HIP(hipMemcpyAsync(&npairs, indptr + na, sizeof(long), hipMemcpyDeviceToHost, C.stream));
+  // longest row: decides whether the fill pass is needed
+  long* dmax = C.scalars.as<long>(64) + 16;
+  {
+    size_t bytes = 0;
+    LSM_HIP(rocprim::reduce(nullptr, bytes, row_count, dmax, 0L, (size_t)na, rocprim::maximum<long>(), C.stream));
+    void* t2 = C.sort_tmp.get(bytes);
+    LSM_HIP(rocprim::reduce(t2, bytes, row_count, dmax, 0L, (size_t)na, rocprim::maximum<long>(), C.stream));
+  }
+  long hdr[2] = {0, 0};
+  LSM_HIP(hipMemcpyAsync(&hdr[0], indptr + na, sizeof(long), hipMemcpyDeviceToHost, C.stream));
+  LSM_HIP(hipMemcpyAsync(&hdr[1], dmax, sizeof(long), hipMemcpyDeviceToHost, C.stream));
   LSM_HIP(hipStreamSynchronize(C.stream));
+  const long npairs = hdr[0], maxrow = hdr[1];
   LSM_REQUIRE(npairs < (1L << 40), LIBSUPERMESH_GPU_ERROR_OVERFLOW, "too many candidate pairs");
   indices = C.indices.as<int>(npairs);
   rows = C.rows.as<int>(npairs);
   if (npairs > 0) {
-    if (A.dim == 2) {
-      if (closed) LSM_QUERY(2, true, true);
-      else LSM_QUERY(2, false, true);
-    } else {
-      if (closed) LSM_QUERY(3, true, true);
-      else LSM_QUERY(3, false, true);
+    hipLaunchKernelGGL(k_rows_from_scratch, grid, block, 0, C.stream, na, qperm, cap, scratch, row_count,
+                       indptr, indices, rows);
+    if (maxrow > cap) {
+      if (A.dim == 2) {
+        if (closed) LSM_QUERY(2, true, true, cap);
+        else LSM_QUERY(2, false, true, cap);
+      } else {
+        if (closed) LSM_QUERY(3, true, true, cap);
+        else LSM_QUERY(3, false, true, cap);
+      }
     }
     LSM_HIP(hipGetLastError());
     if (sort_rows) {
@@ -645,6 +796,7 @@ long find_pairs(Context& C, const DevMesh& A, const DevMesh& B, int finder, int 
       LSM_HIP(hipGetLastError());
     }
   }
+#undef LSM_MASK
 #undef LSM_QUERY
   return npairs;
 }

@@ -127,6 +127,7 @@ struct Context {
   DevBuf keys, keys_alt, vals, vals_alt, sort_tmp, scan_tmp;
   DevBuf qperm;                  // A elements in Morton order
   DevBuf row_count, indptr, indices, rows;
+  DevBuf fscratch;               // single-pass search results (slot-major)
   // clipping
   DevBuf counts, measure, partials, scalars, offsets, elements, blocks;
   DevBuf work, nwork;            // compacted list of pairs that need the full clip
