@@ -657,20 +657,53 @@ k_query_mask(long na, const int* __restrict__ qperm, const double* __restrict__ 
   row_count[a] = cnt;
 }
 
-// Rows that fit the scratch: copy them into the CSR arrays.
+// Rows that fit the scratch: copy them into the CSR arrays.  A wave takes 64
+// consecutive queries and transposes their slot-major scratch through LDS in
+// 64-slot tiles, so that both the scratch reads and the row writes are
+// contiguous across the wave.
 __global__ void __launch_bounds__(kBlock)
 k_rows_from_scratch(long na, const int* __restrict__ qperm, int cap, const int* __restrict__ scratch,
                     const long* __restrict__ row_count, const long* __restrict__ indptr,
                     int* __restrict__ indices, int* __restrict__ rows) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= na) return;
-  const int a = qperm[t];
-  const long n = row_count[a];
-  if (n > cap) return;
-  const long o = indptr[a];
-  for (long k = 0; k < n; k++) {
-    indices[o + k] = scratch[k * na + t];
-    rows[o + k] = a;
+  __shared__ int tile[kBlock / 64][64][65];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long base = ((long)blockIdx.x * (kBlock / 64) + wv) * 64;
+  if (base >= na) return;  // whole waves only; no block barriers below
+  const long t = base + lane;
+  int a = 0;
+  long n = 0, o = 0;
+  if (t < na) {
+    a = qperm[t];
+    n = row_count[a];
+    o = indptr[a];
+    if (n > cap) n = 0;  // left to the fill pass
+  }
+  long nmax = n;
+  for (int off = 32; off > 0; off >>= 1) {
+    const long v = __shfl_xor(nmax, off, 64);
+    nmax = v > nmax ? v : nmax;
+  }
+  int (*T)[65] = tile[wv];
+  for (long s0 = 0; s0 < nmax; s0 += 64) {
+    // tile[j][q] = slot s0 + j of query base + q
+    const int nj = nmax - s0 < 64 ? (int)(nmax - s0) : 64;
+    for (int j = 0; j < nj; j++) {
+      const long s = s0 + j;
+      T[j][lane] = (s < cap && t < na) ? scratch[s * na + t] : 0;
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    for (int q = 0; q < 64; q++) {
+      const long nq = __shfl(n, q, 64), oq = __shfl(o, q, 64);
+      const int aq = __shfl(a, q, 64);
+      const long s = s0 + lane;
+      if (s < nq) {
+        indices[oq + s] = T[lane][q];
+        rows[oq + s] = aq;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
   }
 }
 
@@ -745,7 +778,7 @@ long find_pairs(Context& C, const DevMesh& A, const DevMesh& B, int finder, int 
   bool closed = finder == LIBSUPERMESH_GPU_FINDER_RTREE;
   // single pass into a scratch of `cap` results per query; longer rows are
   // searched again by the fill pass
-  const int cap = A.dim == 2 ? 32 : 96;
+  const int cap = A.dim == 2 ? 32 : 160;
   LSM_REQUIRE(H.levels <= 17, LIBSUPERMESH_GPU_ERROR_OVERFLOW, "hierarchy too deep");
   int* scratch = C.fscratch.as<int>((size_t)cap * na);
   if (A.dim == 2) {
