@@ -140,6 +140,9 @@ struct Context {
   void* pinned = nullptr;
   size_t pinned_bytes = 0;
   Predictor predictor;
+  // 2D mixed mass: rows laid out before the clip and filled by it
+  bool mass_rows_ready = false;
+  long mass_maxraw = 0;
   // timing
   hipEvent_t ev[16];
   bool events = false;

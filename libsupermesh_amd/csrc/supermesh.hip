@@ -151,6 +151,14 @@ struct ClipArgs {
   unsigned char* dropped;      // single pass: the lane's arena ran out, outputs dropped
   unsigned long long* ndropped;
   const unsigned char* ovf;  // fill pass: when non-null, only items flagged here
+  // mass row slots (2D): the count pass also writes each pair's contributions
+  // (B node, value) to the rows of A's nodes, at
+  // rbase[a * LOC + li] + (p - indptr[a]) * LOC + j; empty pairs write node -1
+  const long* rbase;
+  const long* indptr;
+  const int* enlb;
+  int* rkey;
+  double* rval;
 };
 
 __device__ __forceinline__ void load_planes(const double* __restrict__ bp, long b, Planes& P) {
@@ -304,6 +312,21 @@ __global__ void __launch_bounds__(kClipBlock) k_clip(ClipArgs g) {
     if (MASS) {
 #pragma unroll
       for (int k = 0; k < LOC * LOC; k++) g.blocks[p * LOC * LOC + k] = nk == 0 ? 0.0 : blk[k];
+      if (g.rkey) {
+        const long r = (p - g.indptr[a]) * LOC;
+        int nodes[LOC];
+#pragma unroll
+        for (int j = 0; j < LOC; j++) nodes[j] = nk == 0 ? -1 : g.enlb[b * LOC + j];
+#pragma unroll
+        for (int li = 0; li < LOC; li++) {
+          const long o = g.rbase[a * LOC + li] + r;
+#pragma unroll
+          for (int j = 0; j < LOC; j++) {
+            g.rkey[o + j] = nodes[j];
+            g.rval[o + j] = nk == 0 ? 0.0 : blk[li * LOC + j];
+          }
+        }
+      }
     }
   }
 }
@@ -666,6 +689,8 @@ k_budget_to_long(long n, const unsigned* __restrict__ budget, int shift, long* _
   if (i < n) out[i] = (budget[i] >> shift) & 0xffu;
 }
 
+bool mass_prepare_rows(Context& C, const DevMesh& A, const DevMesh& B, const long* indptr, ClipArgs& g);
+
 void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int flags,
                 libsupermesh_gpu_result* out) {
   const int dim = A.dim, loc = dim + 1;
@@ -789,6 +814,8 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
       LSM_HIP(hipMemsetAsync(g.ndropped, 0, sizeof(unsigned long long), C.stream));
     }
   }
+  C.mass_rows_ready = false;
+  if (mass && dim == 2 && out->indptr) mass_prepare_rows(C, A, B, out->indptr, g);
   if (C.timing) LSM_HIP(hipEventRecord(C.ev[14], C.stream));
   if (dim == 2) launch_clip<2>(C, g, mass, false);
   else if (learned) launch_tet_walk(C, g, mass ? 4 : 3);
@@ -1120,7 +1147,7 @@ constexpr int kMaxDistinct = 256;  // columns a row may hold on this path
 // (column, value) list at raw_off[i] and its length to ucount[i]; rows whose
 // distinct columns exceed kMaxDistinct raise *overflow (the caller then uses
 // the global sort path).
-template <int LOC, int CAP, int WAVES>
+template <int LOC, int CAP, int WAVES, bool SLOTS = false>
 __global__ void __launch_bounds__(64 * WAVES)
 k_mass_rows_gather(long nn, const long* __restrict__ nptr, const int* __restrict__ nlist,
                    const long* __restrict__ indptr, const int* __restrict__ cols,
@@ -1142,9 +1169,26 @@ k_mass_rows_gather(long nn, const long* __restrict__ nptr, const int* __restrict
     const long i = base + wv;
     const bool valid = i < nn;
     // 1. gather: up to 64 incident elements at a time, one per lane; their
-    // pair ranges are laid end to end and the lanes walk that flat list
+    // pair ranges are laid end to end and the lanes walk that flat list.  With
+    // SLOTS the clip pass already wrote the row in that order (empty pairs as
+    // node -1): it is read as is.
     int n = 0;
-    if (valid) {
+    if (SLOTS && valid) {
+      const long r0 = raw_off[i], r1 = raw_off[i + 1];
+      for (long rb = r0; rb < r1; rb += 64) {
+        const long r = rb + lane;
+        const int key = r < r1 ? tkey[r] : -1;
+        const double v = r < r1 ? tval[r] : 0.0;
+        const unsigned long long m = __ballot(key >= 0);
+        if (key >= 0) {
+          const int pos = n + __popcll(m & ((1ULL << lane) - 1));
+          rk[pos] = key;
+          rv[pos] = v;
+        }
+        n += __popcll(m);
+      }
+    }
+    if (!SLOTS && valid) {
       const long x0 = nptr[i], x1 = nptr[i + 1];
       for (long xb = x0; xb < x1; xb += 64) {
         const long x = xb + lane;
@@ -1229,12 +1273,20 @@ k_mass_rows_gather(long nn, const long* __restrict__ nptr, const int* __restrict
       if (nd <= 64) wave_bitonic_keys<1>(key, lane);
       else wave_bitonic_keys<4>(key, lane);
       double acc[4] = {0.0, 0.0, 0.0, 0.0};
-      for (int t = 0; t < n; t++) {
-        const int kt = rk[t];
-        const double vt = rv[t];
+      if (nd <= 64) {
+        for (int t = 0; t < n; t++) {
+          const int kt = rk[t];
+          const double vt = rv[t];
+          if (kt == key[0]) acc[0] = acc[0] + vt;
+        }
+      } else {
+        for (int t = 0; t < n; t++) {
+          const int kt = rk[t];
+          const double vt = rv[t];
 #pragma unroll
-        for (int r = 0; r < 4; r++)
-          if (kt == key[r]) acc[r] = acc[r] + vt;
+          for (int r = 0; r < 4; r++)
+            if (kt == key[r]) acc[r] = acc[r] + vt;
+        }
       }
       const long o = raw_off[i];
 #pragma unroll
@@ -1251,8 +1303,303 @@ k_mass_rows_gather(long nn, const long* __restrict__ nptr, const int* __restrict
   }
 }
 
+// Deduplicates and sums one row held in LDS (rk/rv, n entries): distinct
+// columns through an LDS hash table, sorted, each summed in entry order.
+// Writes them at tkey/tval + o; returns the count, or -1 if the row has more
+// than kMaxDistinct columns.  One wave; `ht` has kHashSlots entries.
+__device__ __forceinline__ int row_reduce(const int* rk, const double* rv, int n, int* ht, int* dk, int lane,
+                                          int* __restrict__ tkey, double* __restrict__ tval, long o) {
+  for (int t = lane; t < kHashSlots; t += 64) ht[t] = -1;
+  __syncthreads();
+  bool full = false;
+  for (int t = lane; t < n; t += 64) {
+    const int key = rk[t];
+    unsigned h = ((unsigned)key * 2654435761u) >> (32 - 9);
+    int tries = 0;
+    while (true) {
+      const int prev = atomicCAS(&ht[h], -1, key);
+      if (prev == -1 || prev == key) break;
+      h = (h + 1) & (kHashSlots - 1);
+      if (++tries >= kHashSlots) {
+        full = true;
+        break;
+      }
+    }
+  }
+  __syncthreads();
+  int nd = 0;
+  for (int s0 = 0; s0 < kHashSlots; s0 += 64) {
+    const int key = ht[s0 + lane];
+    const bool occ = key != -1;
+    const unsigned long long m = __ballot(occ);
+    const int pos = nd + __popcll(m & ((1ULL << lane) - 1));
+    if (occ && pos < kMaxDistinct) dk[pos] = key;
+    nd += __popcll(m);
+  }
+  const bool bad = __ballot(full) != 0ULL || nd > kMaxDistinct;
+  __syncthreads();
+  if (bad) return -1;
+  int key[4];
+#pragma unroll
+  for (int r = 0; r < 4; r++) key[r] = lane + 64 * r < nd ? dk[lane + 64 * r] : 0x7fffffff;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  // entries are read 8 at a time (the tail past n holds key -2, which matches
+  // no column) so that the LDS loads overlap
+  const int n8 = (n + 7) & ~7;
+  if (nd <= 64) {
+    wave_bitonic_keys<1>(key, lane);
+    for (int t = 0; t < n8; t += 8) {
+      const int4 k0 = *reinterpret_cast<const int4*>(rk + t), k1 = *reinterpret_cast<const int4*>(rk + t + 4);
+      const double2 v0 = *reinterpret_cast<const double2*>(rv + t);
+      const double2 v1 = *reinterpret_cast<const double2*>(rv + t + 2);
+      const double2 v2 = *reinterpret_cast<const double2*>(rv + t + 4);
+      const double2 v3 = *reinterpret_cast<const double2*>(rv + t + 6);
+      const int kk[8] = {k0.x, k0.y, k0.z, k0.w, k1.x, k1.y, k1.z, k1.w};
+      const double vv[8] = {v0.x, v0.y, v1.x, v1.y, v2.x, v2.y, v3.x, v3.y};
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        if (kk[u] == key[0]) acc[0] = acc[0] + vv[u];
+    }
+  } else {
+    wave_bitonic_keys<4>(key, lane);
+    for (int t = 0; t < n8; t += 8) {
+      const int4 k0 = *reinterpret_cast<const int4*>(rk + t), k1 = *reinterpret_cast<const int4*>(rk + t + 4);
+      const double2 v0 = *reinterpret_cast<const double2*>(rv + t);
+      const double2 v1 = *reinterpret_cast<const double2*>(rv + t + 2);
+      const double2 v2 = *reinterpret_cast<const double2*>(rv + t + 4);
+      const double2 v3 = *reinterpret_cast<const double2*>(rv + t + 6);
+      const int kk[8] = {k0.x, k0.y, k0.z, k0.w, k1.x, k1.y, k1.z, k1.w};
+      const double vv[8] = {v0.x, v0.y, v1.x, v1.y, v2.x, v2.y, v3.x, v3.y};
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+          if (kk[u] == key[r]) acc[r] = acc[r] + vv[u];
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int t = lane + 64 * r;
+    if (t < nd) {
+      tkey[o + t] = key[r];
+      tval[o + t] = acc[r];
+    }
+  }
+  return nd;
+}
+
+// Rows laid out by the clip pass (2D): one wave per block, rows i, i + grid,
+// ...; while a row is reduced, the next row's entries (up to 64 * PF) and the
+// one after's bounds are already being loaded.
+template <int CAP, int PF>
+__global__ void __launch_bounds__(64)
+k_mass_rows_slots(long nn, const long* __restrict__ raw_off, int* __restrict__ tkey, double* __restrict__ tval,
+                  long* __restrict__ ucount, int* __restrict__ overflow) {
+  __shared__ __align__(16) int rk[CAP + 8];
+  __shared__ __align__(16) double rv[CAP + 8];
+  __shared__ int ht[kHashSlots];
+  __shared__ int dk[kMaxDistinct];
+  const int lane = threadIdx.x;
+  const long stride = gridDim.x;
+  long i = blockIdx.x;
+  if (i >= nn) return;
+  long a0 = raw_off[i], a1 = raw_off[i + 1];
+  int ck[PF];
+  double cv[PF];
+#pragma unroll
+  for (int q = 0; q < PF; q++) {
+    const long r = a0 + lane + 64 * q;
+    ck[q] = r < a1 ? tkey[r] : -1;
+    cv[q] = r < a1 ? tval[r] : 0.0;
+  }
+  long j = i + stride;
+  long b0 = 0, b1 = 0;
+  if (j < nn) {
+    b0 = raw_off[j];
+    b1 = raw_off[j + 1];
+  }
+  for (; i < nn; i = j, j += stride) {
+    // row i into LDS, skipping empty pairs (node -1), in entry order
+    int n = 0;
+#pragma unroll
+    for (int q = 0; q < PF; q++) {
+      const unsigned long long m = __ballot(ck[q] >= 0);
+      if (ck[q] >= 0) {
+        const int pos = n + __popcll(m & ((1ULL << lane) - 1));
+        rk[pos] = ck[q];
+        rv[pos] = cv[q];
+      }
+      n += __popcll(m);
+    }
+    for (long rb = a0 + 64 * PF; rb < a1; rb += 64) {  // the rest of a long row
+      const long r = rb + lane;
+      const int key = r < a1 ? tkey[r] : -1;
+      const double v = r < a1 ? tval[r] : 0.0;
+      const unsigned long long m = __ballot(key >= 0);
+      if (key >= 0) {
+        const int pos = n + __popcll(m & ((1ULL << lane) - 1));
+        rk[pos] = key;
+        rv[pos] = v;
+      }
+      n += __popcll(m);
+    }
+    // next row's entries and the bounds of the one after
+    const long jn = j + stride;
+    long c0 = 0, c1 = 0;
+    if (j < nn) {
+#pragma unroll
+      for (int q = 0; q < PF; q++) {
+        const long r = b0 + lane + 64 * q;
+        ck[q] = r < b1 ? tkey[r] : -1;
+        cv[q] = r < b1 ? tval[r] : 0.0;
+      }
+      if (jn < nn) {
+        c0 = raw_off[jn];
+        c1 = raw_off[jn + 1];
+      }
+    }
+    if (lane < 8) rk[n + lane] = -2;  // tail of the last 8-entry group
+    __syncthreads();
+    const int nd = row_reduce(rk, rv, n, ht, dk, lane, tkey, tval, a0);
+    if (nd < 0) {
+      if (lane == 0) atomicOr(overflow, 1);
+    } else if (lane == 0) {
+      ucount[i] = nd;
+    }
+    __syncthreads();
+    a0 = b0;
+    a1 = b1;
+    b0 = c0;
+    b1 = c1;
+  }
+}
+
+// After the clip pass filled the row slots (mass_prepare_rows).
+static void assemble_mass_slots(Context& C, const DevMesh& A, const DevMesh& B, libsupermesh_gpu_result* out) {
+  const long nn = A.nnodes;
+  long* mindptr = C.mindptr.as<long>(nn + 1);
+  LSM_HIP(hipMemsetAsync(mindptr, 0, sizeof(long) * (nn + 1), C.stream));
+  out->mass_nnz = 0;
+  out->mass_indptr = mindptr;
+  out->mass_indices = static_cast<const int*>(C.mindices.get(4));
+  out->mass_values = static_cast<const double*>(C.uvals.get(8));
+  long* nptr = static_cast<long*>(C.nuniq.ptr);
+  const long* raw_off = nptr + 2 * (nn + 1);
+  int* tkey = static_cast<int*>(C.mkeys.ptr);
+  double* tval = static_cast<double*>(C.mvals.ptr);
+  long* ucount = C.mvals_alt.as<long>(nn + 1);
+  int* ovf = C.scalars.as<int>(64) + 4;
+  LSM_HIP(hipMemsetAsync(ovf, 0, sizeof(int), C.stream));
+  int cus = 0;
+  LSM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, C.device));
+  const dim3 grid((unsigned)std::max<long>(1, std::min<long>((long)(cus > 0 ? cus : 256) * 16, nn)));
+  if (C.mass_maxraw <= 512)
+    hipLaunchKernelGGL((k_mass_rows_slots<512, 4>), dim3(grid.x * 4), dim3(64), 0, C.stream, nn, raw_off, tkey,
+                       tval, ucount, ovf);
+  else
+    hipLaunchKernelGGL((k_mass_rows_gather<3, 2048, 2, true>), grid, dim3(128), 0, C.stream, nn, nptr, nullptr,
+                       nullptr, nullptr, nullptr, nullptr, nullptr, raw_off, tkey, tval, ucount, ovf);
+  LSM_HIP(hipGetLastError());
+  int hovf = 0;
+  LSM_HIP(hipMemcpyAsync(&hovf, ovf, sizeof(int), hipMemcpyDeviceToHost, C.stream));
+  LSM_HIP(hipStreamSynchronize(C.stream));
+  if (hovf) {  // a row with more distinct columns than the row kernel holds
+    assemble_mass_sorted(C, A, B, out->npairs, out);
+    return;
+  }
+  inclusive_scan_long(C, ucount, mindptr + 1, nn);
+  long nnz = 0;
+  LSM_HIP(hipMemcpyAsync(&nnz, mindptr + nn, sizeof(long), hipMemcpyDeviceToHost, C.stream));
+  LSM_HIP(hipStreamSynchronize(C.stream));
+  int* mind = C.mindices.as<int>(nnz);
+  double* mval = C.uvals.as<double>(nnz);
+  hipLaunchKernelGGL(k_mass_copy, dim3(grid_for(nn, 4)), dim3(256), 0, C.stream, nn, raw_off, mindptr, tkey, tval,
+                     mind, mval);
+  LSM_HIP(hipGetLastError());
+  LSM_HIP(hipStreamSynchronize(C.stream));
+  out->mass_nnz = nnz;
+  out->mass_indices = mind;
+  out->mass_values = mval;
+}
+
+__global__ void __launch_bounds__(256)
+k_elem_pairs(long ne, const long* __restrict__ indptr, int* __restrict__ out) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < ne) out[e] = (int)(indptr[e + 1] - indptr[e]);
+}
+
+__global__ void __launch_bounds__(256)
+k_elem_base(long nent, const int* __restrict__ nlist, const long* __restrict__ slot, long* __restrict__ rbase) {
+  const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x < nent) rbase[nlist[x]] = slot[x];
+}
+
+// 2D mass rows laid out before the clip: row i of A node i gets, for each
+// incident element (e, li) in ascending order, LOC slots per candidate pair of
+// e; the clip pass fills them.  Returns false (and lays out nothing) when the
+// longest row would not fit the row kernel.
+bool mass_prepare_rows(Context& C, const DevMesh& A, const DevMesh& B, const long* indptr, ClipArgs& g) {
+  const int loc = A.dim + 1;
+  const long na = A.nelements, nn = A.nnodes;
+  if (A.dim != 2 || na == 0 || nn == 0) return false;
+  unsigned int* ncount = reinterpret_cast<unsigned int*>(C.mflag.as<long>(2 * nn + 2));
+  unsigned int* cursor = ncount + nn + 1;
+  LSM_HIP(hipMemsetAsync(ncount, 0, sizeof(unsigned int) * (2 * nn + 2), C.stream));
+  const long nent = na * loc;
+  hipLaunchKernelGGL(k_node_count, dim3(grid_for(nent, 256)), dim3(256), 0, C.stream, na, loc, A.enlist_dev,
+                     ncount);
+  long* nptr = C.nuniq.as<long>(3 * (nn + 1));
+  long* raw = nptr + (nn + 1);
+  long* raw_off = raw + (nn + 1);
+  LSM_HIP(hipMemsetAsync(nptr, 0, sizeof(long), C.stream));
+  LSM_HIP(hipMemsetAsync(raw_off, 0, sizeof(long), C.stream));
+  inclusive_scan_long(C, ncount, nptr + 1, nn);
+  int* nlist = C.ukeys.as<int>(nent + 1);
+  hipLaunchKernelGGL(k_node_fill, dim3(grid_for(nent, 256)), dim3(256), 0, C.stream, na, loc, A.enlist_dev, nptr,
+                     cursor, nlist);
+  int* elem_np = C.mkeys_alt.as<int>(na + 1);
+  hipLaunchKernelGGL(k_elem_pairs, dim3(grid_for(na, 256)), dim3(256), 0, C.stream, na, indptr, elem_np);
+  long* contrib = C.mslot.as<long>(2 * nent + 2);
+  long* slot = contrib + nent + 1;
+  hipLaunchKernelGGL(k_node_rows, dim3(grid_for(nn, 256)), dim3(256), 0, C.stream, nn, loc, nptr, nlist, elem_np,
+                     contrib);
+  LSM_HIP(hipMemsetAsync(slot, 0, sizeof(long), C.stream));
+  inclusive_scan_long(C, contrib, slot + 1, nent);
+  hipLaunchKernelGGL(k_row_bounds, dim3(grid_for(nn + 1, 256)), dim3(256), 0, C.stream, nn, nptr, slot, raw_off,
+                     raw);
+  long* rbase = C.row_count.as<long>(nent + 1);
+  hipLaunchKernelGGL(k_elem_base, dim3(grid_for(nent, 256)), dim3(256), 0, C.stream, nent, nlist, slot, rbase);
+  LSM_HIP(hipGetLastError());
+  long maxraw = 0, total_raw = 0;
+  {
+    size_t bytes = 0;
+    long* dmax = reinterpret_cast<long*>(C.scalars.as<long>(64)) + 8;
+    LSM_HIP(rocprim::reduce(nullptr, bytes, raw, dmax, 0L, (size_t)nn, rocprim::maximum<long>(), C.stream));
+    void* tmp = C.sort_tmp.get(bytes);
+    LSM_HIP(rocprim::reduce(tmp, bytes, raw, dmax, 0L, (size_t)nn, rocprim::maximum<long>(), C.stream));
+    LSM_HIP(hipMemcpyAsync(&maxraw, dmax, sizeof(long), hipMemcpyDeviceToHost, C.stream));
+    LSM_HIP(hipMemcpyAsync(&total_raw, raw_off + nn, sizeof(long), hipMemcpyDeviceToHost, C.stream));
+    LSM_HIP(hipStreamSynchronize(C.stream));
+  }
+  if (maxraw > 2048) return false;
+  C.mass_rows_ready = true;
+  C.mass_maxraw = maxraw;
+  g.rbase = rbase;
+  g.indptr = indptr;
+  g.enlb = B.enlist_dev;
+  g.rkey = C.mkeys.as<int>(total_raw + 1);
+  g.rval = C.mvals.as<double>(total_raw + 1);
+  return true;
+}
+
 void assemble_mass(Context& C, const DevMesh& A, const DevMesh& B, long npairs,
                    libsupermesh_gpu_result* out) {
+  if (C.mass_rows_ready) {
+    C.mass_rows_ready = false;
+    assemble_mass_slots(C, A, B, out);
+    return;
+  }
   const int loc = A.dim + 1;
   const long na = A.nelements, nn = A.nnodes;
   long* mindptr = C.mindptr.as<long>(nn + 1);
