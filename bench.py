@@ -31,6 +31,17 @@ HBM_PEAK_GBS = 8000.0
 # instrumented CPU restatement (oracle/count_flops.py) on these workloads.
 FLOPS_PER_PAIR = {"tri_1024_vs_1000": 415.8, "tet_64_vs_60": 691.1}
 
+# Algorithmic bytes per candidate pair of the clip count pass: the B element
+# gathered (tri 6 doubles + 3 ids = 60 B; tet 12 doubles + 4 ids = 112 B),
+# the pair's two ids (8 B), its count (4 B) and measure (8 B); with the mixed
+# mass (2D) also its 3x3 block (72 B) and its 9 row-slot entries (9 x 12 B).
+BYTES_PER_PAIR = {"tri_1024_vs_1000": 60 + 8 + 4 + 8 + 72 + 108, "tet_64_vs_60": 112 + 8 + 4 + 8}
+
+# HBM bytes per launch of the same kernel on the same workload, from the two
+# counter runs summarised in profiles/traffic_tri1024_tet64.md (FETCH_SIZE x
+# 1.9997 + WRITE_SIZE x 1.0, the factors a 256 MiB copy calibrates).
+PROFILED_TRAFFIC = {"tri_1024_vs_1000": 5.29e9, "tet_64_vs_60": 36.86e9}
+
 
 def parse():
     p = argparse.ArgumentParser()
@@ -219,6 +230,28 @@ def learned_workload(ctx, name, steps, warmup, weights, quantiles=(0.7, 0.9, 0.9
                                for k, v in out.items() if k != "plain"}}
 
 
+def clip_roofline(name, w):
+    """Roofline of the clip count pass (the dominant kernel of the 2D step
+    and of the 3D step), timed with HIP events on the library's stream over
+    the timed steps.  Both its ceilings are given: HBM (the bound at this
+    arithmetic intensity) and FP64."""
+    secs = w["clip_kernel_ms"] / 1e3
+    bpp, fpp = BYTES_PER_PAIR.get(name), FLOPS_PER_PAIR.get(name)
+    if not bpp or secs <= 0:
+        return {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+                "traffic": None, "kernel_ms": w["clip_kernel_ms"]}
+    gbs = bpp * w["pairs"] / secs / 1e9
+    out = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+           "traffic": PROFILED_TRAFFIC.get(name), "kernel": "k_clip<2> count pass with mixed mass"
+           if name.startswith("tri") else "k_clip_tet_walk count pass",
+           "bytes_per_pair": bpp, "pairs_per_launch": w["pairs"], "kernel_ms": w["clip_kernel_ms"]}
+    if fpp:
+        tf = fpp * w["pairs"] / secs / 1e12
+        out["fp64"] = {"achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tf / FP64_PEAK_TFLOPS,
+                       "flops_per_pair": fpp}
+    return out
+
+
 def cpu_baseline(name, meshes):
     """The CPU restatement of the reference's serial loop (tree finder, then
     the per-pair clip), one thread per host core over contiguous ranges of A."""
@@ -266,23 +299,17 @@ def main():
         if not os.path.exists(args.weights):
             raise SystemExit(f"predictor weights not found: {args.weights}")
         learned = learned_workload(ctx, args.learned_workload, max(1, min(args.steps, 3)), 1, args.weights)
-    fpp = FLOPS_PER_PAIR.get(args.workload)
-    clip_s = main_w["clip_kernel_ms"] / 1e3
-    if fpp:
-        achieved = fpp * main_w["pairs"] / clip_s / 1e12
-        roofline = {"bound": "fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                    "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                    "kernel": "k_clip (count pass)", "flops_per_pair": fpp,
-                    "kernel_ms": main_w["clip_kernel_ms"]}
-    else:
-        roofline = {"bound": "fp64", "achieved": None, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": None, "traffic": None, "kernel": "k_clip (count pass)",
-                    "kernel_ms": main_w["clip_kernel_ms"]}
+    roofline = clip_roofline(args.workload, main_w)
+    if extra:
+        extra["roofline"] = clip_roofline(extra["workload"], extra)
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline ...")
         cpu = cpu_baseline(args.workload, main_w["_meshes"])
         log(f"[bench] CPU baseline {cpu['value']:.3e} pairs/s on {cpu['cores']} threads")
+        if extra:
+            extra["cpu_baseline"] = cpu_baseline(extra["workload"], extra["_meshes"])
+            log(f"[bench] CPU baseline {extra['workload']}: {extra['cpu_baseline']['value']:.3e} pairs/s")
     for w in (main_w, extra):
         if w:
             w.pop("_meshes", None)

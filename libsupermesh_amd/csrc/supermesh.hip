@@ -1147,162 +1147,6 @@ constexpr int kMaxDistinct = 256;  // columns a row may hold on this path
 // (column, value) list at raw_off[i] and its length to ucount[i]; rows whose
 // distinct columns exceed kMaxDistinct raise *overflow (the caller then uses
 // the global sort path).
-template <int LOC, int CAP, int WAVES, bool SLOTS = false>
-__global__ void __launch_bounds__(64 * WAVES)
-k_mass_rows_gather(long nn, const long* __restrict__ nptr, const int* __restrict__ nlist,
-                   const long* __restrict__ indptr, const int* __restrict__ cols,
-                   const int* __restrict__ counts, const double* __restrict__ blocks,
-                   const int* __restrict__ enlb, const long* __restrict__ raw_off, int* __restrict__ tkey,
-                   double* __restrict__ tval, long* __restrict__ ucount, int* __restrict__ overflow) {
-  __shared__ int s_rk[WAVES][CAP];
-  __shared__ double s_rv[WAVES][CAP];
-  __shared__ int s_ht[WAVES][kHashSlots];
-  __shared__ int s_dk[WAVES][kMaxDistinct];
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  int* rk = s_rk[wv];
-  double* rv = s_rv[wv];
-  int* ht = s_ht[wv];
-  int* dk = s_dk[wv];
-  const long nwaves = (long)gridDim.x * WAVES;
-  // the same trip count and barrier sequence for every wave of the block
-  for (long base = (long)blockIdx.x * WAVES; base < nn; base += nwaves) {
-    const long i = base + wv;
-    const bool valid = i < nn;
-    // 1. gather: up to 64 incident elements at a time, one per lane; their
-    // pair ranges are laid end to end and the lanes walk that flat list.  With
-    // SLOTS the clip pass already wrote the row in that order (empty pairs as
-    // node -1): it is read as is.
-    int n = 0;
-    if (SLOTS && valid) {
-      const long r0 = raw_off[i], r1 = raw_off[i + 1];
-      for (long rb = r0; rb < r1; rb += 64) {
-        const long r = rb + lane;
-        const int key = r < r1 ? tkey[r] : -1;
-        const double v = r < r1 ? tval[r] : 0.0;
-        const unsigned long long m = __ballot(key >= 0);
-        if (key >= 0) {
-          const int pos = n + __popcll(m & ((1ULL << lane) - 1));
-          rk[pos] = key;
-          rv[pos] = v;
-        }
-        n += __popcll(m);
-      }
-    }
-    if (!SLOTS && valid) {
-      const long x0 = nptr[i], x1 = nptr[i + 1];
-      for (long xb = x0; xb < x1; xb += 64) {
-        const long x = xb + lane;
-        long p0 = 0;
-        int np = 0, li = 0;
-        if (x < x1) {
-          const int el = nlist[x];
-          const int e = el / LOC;
-          li = el - e * LOC;
-          p0 = indptr[e];
-          np = (int)(indptr[e + 1] - p0);
-        }
-        int incl = np;
-        for (int o = 1; o < 64; o <<= 1) {
-          const int v = __shfl_up(incl, o, 64);
-          if (lane >= o) incl += v;
-        }
-        const int excl = incl - np;
-        const int total = __shfl(incl, 63, 64);
-        for (int f0 = 0; f0 < total; f0 += 64) {
-          const int f = f0 + lane;
-          // owner: the last lane whose range starts at or before f
-          int own = 0;
-          for (int step = 32; step > 0; step >>= 1) {
-            const int cand = own + step;
-            const int ev = __shfl(excl, cand < 64 ? cand : 63, 64);
-            if (cand < 64 && ev <= f) own = cand;
-          }
-          const long p = __shfl(p0, own, 64) + (f - __shfl(excl, own, 64));
-          const int lio = __shfl(li, own, 64);
-          const bool ne = f < total && counts[p] > 0;
-          const unsigned long long m = __ballot(ne);
-          if (ne) {
-            const int pos = n + LOC * __popcll(m & ((1ULL << lane) - 1));
-            const long b = cols[p];
-#pragma unroll
-            for (int j = 0; j < LOC; j++) {
-              rk[pos + j] = enlb[b * LOC + j];
-              rv[pos + j] = blocks[p * LOC * LOC + lio * LOC + j];
-            }
-          }
-          n += LOC * __popcll(m);
-        }
-      }
-    }
-    for (int t = lane; t < kHashSlots; t += 64) ht[t] = -1;
-    __syncthreads();
-    // 2. distinct columns (bounded probing: at most kHashSlots tries)
-    bool full = false;
-    for (int t = lane; t < n; t += 64) {
-      const int key = rk[t];
-      unsigned h = ((unsigned)key * 2654435761u) >> (32 - 9);
-      int tries = 0;
-      while (true) {
-        const int prev = atomicCAS(&ht[h], -1, key);
-        if (prev == -1 || prev == key) break;
-        h = (h + 1) & (kHashSlots - 1);
-        if (++tries >= kHashSlots) {
-          full = true;
-          break;
-        }
-      }
-    }
-    __syncthreads();
-    int nd = 0;
-    for (int s0 = 0; s0 < kHashSlots; s0 += 64) {
-      const int key = ht[s0 + lane];
-      const bool occ = key != -1;
-      const unsigned long long m = __ballot(occ);
-      const int pos = nd + __popcll(m & ((1ULL << lane) - 1));
-      if (occ && pos < kMaxDistinct) dk[pos] = key;
-      nd += __popcll(m);
-    }
-    const bool bad = __ballot(full) != 0ULL || nd > kMaxDistinct;
-    if (bad && lane == 0) atomicOr(overflow, 1);
-    __syncthreads();
-    if (valid && !bad) {
-      // 3. sort the distinct columns, 4. sum each in gather order
-      int key[4];
-#pragma unroll
-      for (int r = 0; r < 4; r++) key[r] = lane + 64 * r < nd ? dk[lane + 64 * r] : 0x7fffffff;
-      if (nd <= 64) wave_bitonic_keys<1>(key, lane);
-      else wave_bitonic_keys<4>(key, lane);
-      double acc[4] = {0.0, 0.0, 0.0, 0.0};
-      if (nd <= 64) {
-        for (int t = 0; t < n; t++) {
-          const int kt = rk[t];
-          const double vt = rv[t];
-          if (kt == key[0]) acc[0] = acc[0] + vt;
-        }
-      } else {
-        for (int t = 0; t < n; t++) {
-          const int kt = rk[t];
-          const double vt = rv[t];
-#pragma unroll
-          for (int r = 0; r < 4; r++)
-            if (kt == key[r]) acc[r] = acc[r] + vt;
-        }
-      }
-      const long o = raw_off[i];
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int t = lane + 64 * r;
-        if (t < nd) {
-          tkey[o + t] = key[r];
-          tval[o + t] = acc[r];
-        }
-      }
-      if (lane == 0) ucount[i] = nd;
-    }
-    __syncthreads();  // the LDS rows are refilled next
-  }
-}
-
 // Deduplicates and sums one row held in LDS (rk/rv, n entries): distinct
 // columns through an LDS hash table, sorted, each summed in entry order.
 // Writes them at tkey/tval + o; returns the count, or -1 if the row has more
@@ -1387,6 +1231,106 @@ __device__ __forceinline__ int row_reduce(const int* rk, const double* rv, int n
     }
   }
   return nd;
+}
+
+template <int LOC, int CAP, int WAVES, bool SLOTS = false>
+__global__ void __launch_bounds__(64 * WAVES)
+k_mass_rows_gather(long nn, const long* __restrict__ nptr, const int* __restrict__ nlist,
+                   const long* __restrict__ indptr, const int* __restrict__ cols,
+                   const int* __restrict__ counts, const double* __restrict__ blocks,
+                   const int* __restrict__ enlb, const long* __restrict__ raw_off, int* __restrict__ tkey,
+                   double* __restrict__ tval, long* __restrict__ ucount, int* __restrict__ overflow) {
+  __shared__ __align__(16) int s_rk[WAVES][CAP + 8];
+  __shared__ __align__(16) double s_rv[WAVES][CAP + 8];
+  __shared__ int s_ht[WAVES][kHashSlots];
+  __shared__ int s_dk[WAVES][kMaxDistinct];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int* rk = s_rk[wv];
+  double* rv = s_rv[wv];
+  int* ht = s_ht[wv];
+  int* dk = s_dk[wv];
+  const long nwaves = (long)gridDim.x * WAVES;
+  // the same trip count and barrier sequence for every wave of the block
+  for (long base = (long)blockIdx.x * WAVES; base < nn; base += nwaves) {
+    const long i = base + wv;
+    const bool valid = i < nn;
+    // 1. gather: up to 64 incident elements at a time, one per lane; their
+    // pair ranges are laid end to end and the lanes walk that flat list.  With
+    // SLOTS the clip pass already wrote the row in that order (empty pairs as
+    // node -1): it is read as is.
+    int n = 0;
+    if (SLOTS && valid) {
+      const long r0 = raw_off[i], r1 = raw_off[i + 1];
+      for (long rb = r0; rb < r1; rb += 64) {
+        const long r = rb + lane;
+        const int key = r < r1 ? tkey[r] : -1;
+        const double v = r < r1 ? tval[r] : 0.0;
+        const unsigned long long m = __ballot(key >= 0);
+        if (key >= 0) {
+          const int pos = n + __popcll(m & ((1ULL << lane) - 1));
+          rk[pos] = key;
+          rv[pos] = v;
+        }
+        n += __popcll(m);
+      }
+    }
+    if (!SLOTS && valid) {
+      const long x0 = nptr[i], x1 = nptr[i + 1];
+      for (long xb = x0; xb < x1; xb += 64) {
+        const long x = xb + lane;
+        long p0 = 0;
+        int np = 0, li = 0;
+        if (x < x1) {
+          const int el = nlist[x];
+          const int e = el / LOC;
+          li = el - e * LOC;
+          p0 = indptr[e];
+          np = (int)(indptr[e + 1] - p0);
+        }
+        int incl = np;
+        for (int o = 1; o < 64; o <<= 1) {
+          const int v = __shfl_up(incl, o, 64);
+          if (lane >= o) incl += v;
+        }
+        const int excl = incl - np;
+        const int total = __shfl(incl, 63, 64);
+        for (int f0 = 0; f0 < total; f0 += 64) {
+          const int f = f0 + lane;
+          // owner: the last lane whose range starts at or before f
+          int own = 0;
+          for (int step = 32; step > 0; step >>= 1) {
+            const int cand = own + step;
+            const int ev = __shfl(excl, cand < 64 ? cand : 63, 64);
+            if (cand < 64 && ev <= f) own = cand;
+          }
+          const long p = __shfl(p0, own, 64) + (f - __shfl(excl, own, 64));
+          const int lio = __shfl(li, own, 64);
+          const bool ne = f < total && counts[p] > 0;
+          const unsigned long long m = __ballot(ne);
+          if (ne) {
+            const int pos = n + LOC * __popcll(m & ((1ULL << lane) - 1));
+            const long b = cols[p];
+#pragma unroll
+            for (int j = 0; j < LOC; j++) {
+              rk[pos + j] = enlb[b * LOC + j];
+              rv[pos + j] = blocks[p * LOC * LOC + lio * LOC + j];
+            }
+          }
+          n += LOC * __popcll(m);
+        }
+      }
+    }
+    if (lane < 8) rk[n + lane] = -2;  // tail of the last 8-entry group
+    __syncthreads();
+    // 2-4. distinct columns, sorted, summed in gather order (every wave of the
+    // block calls this, valid row or not, so that the barriers match)
+    const int nd = row_reduce(rk, rv, valid ? n : 0, ht, dk, lane, tkey, tval, valid ? raw_off[i] : 0);
+    if (valid && lane == 0) {
+      if (nd < 0) atomicOr(overflow, 1);
+      else ucount[i] = nd;
+    }
+    __syncthreads();  // the LDS rows are refilled next
+  }
 }
 
 // Rows laid out by the clip pass (2D): one wave per block, rows i, i + grid,
