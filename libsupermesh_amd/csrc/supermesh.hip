@@ -336,12 +336,69 @@ __global__ void __launch_bounds__(kClipBlock) k_clip(ClipArgs g) {
 // and classifies or measures it) and, when its pair is done, moves on to its
 // next work item by itself, so lanes never wait for the slowest pair of the
 // wave -- only for the slowest chain of work items, which evens out.
+// The walk of k_clip_tet_walk with its stack of tets (depths 1..3) in LDS
+// instead of registers: a parent is one LDS read per coordinate and a child
+// one write, where the register stack needed a select over every depth.
+// Lane t's depth-j tet, coordinate i, is stk[((j - 1) * 12 + i) * kWalkBlock]
+// with stk = stack + t (consecutive lanes, consecutive words).
+constexpr int kWalkBlock = 128;
+
+struct TetWalkLds {
+  int cls[4];
+  int level;
+  int digits;  // 2 bits per depth 1..4
+};
+
+template <class Sink>
+__device__ __forceinline__ bool walk_step_lds(TetWalkLds& W, const double* A, double* stk,
+                                              const double* planes, double tol, Sink&& sink) {
+  const int k = W.level;
+  const int ci = (W.digits >> (2 * k)) & 3;
+  int cls = in_reg_i(W.cls[0]);
+#pragma unroll
+  for (int j = 1; j < 4; j++) cls = k == j + 1 ? in_reg_i(W.cls[j]) : cls;
+  if (ci >= class_nchild(cls)) {
+    if (k == 1) return false;
+    W.level = k - 1;
+    return true;
+  }
+  double parent[12], pd[4];
+  const double* src = stk + (k >= 2 ? k - 2 : 0) * 12 * kWalkBlock;
+#pragma unroll
+  for (int i = 0; i < 12; i++) {
+    const double v = src[i * kWalkBlock];
+    parent[i] = k == 1 ? in_reg(A[i]) : v;
+  }
+  walk_distances(parent, planes + 4 * (k - 1), pd);
+  double child[12];
+  build_child(parent, pd, make_recipe(cls, ci), child);
+  if (k == 4) {
+    W.digits += 1 << 8;
+    const double vol = tet_volume_tet(child);
+    if (vol >= tol) sink(child, vol);
+    return true;
+  }
+  double dc[4];
+  walk_distances(child, planes + 4 * k, dc);
+  const int cc = pack_class(dc);
+  double* dst = stk + (k - 1) * 12 * kWalkBlock;
+#pragma unroll
+  for (int i = 0; i < 12; i++) dst[i * kWalkBlock] = child[i];
+#pragma unroll
+  for (int j = 1; j < 4; j++) W.cls[j] = k == j ? cc : W.cls[j];
+  W.digits = (W.digits + (1 << (2 * k))) & ~(3 << (2 * (k + 1)));
+  W.level = k + 1;
+  return true;
+}
+
 // MODES: count (FILL and LEARNED false): counts, measures (and mass blocks);
 // fill (FILL): writes each pair's counts[p] outputs at offsets[p];
 // single pass (LEARNED): the count pass that also writes up to budget[w]
 // outputs of work item w at slots[w] of the slot buffer.
 template <bool MASS, bool FILL, bool LEARNED = false>
-__global__ void __launch_bounds__(kClipBlock) k_clip_tet_walk(ClipArgs g) {
+__global__ void __launch_bounds__(kWalkBlock) k_clip_tet_walk(ClipArgs g) {
+  __shared__ double stack[3 * 12 * kWalkBlock];
+  double* stk = stack + threadIdx.x;
   const long n = (long)*g.nwork;
   const long stride = (long)gridDim.x * blockDim.x;
   long w = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -354,7 +411,7 @@ __global__ void __launch_bounds__(kClipBlock) k_clip_tet_walk(ClipArgs g) {
   double A[12], tol = 0.0, m = 0.0;
   double blk[16], affa[12], affb[12];
   const double* planes = nullptr;
-  TetWalk W;
+  TetWalkLds W;
   bool active = false;
   while (true) {
     if (!active) {
@@ -389,10 +446,16 @@ __global__ void __launch_bounds__(kClipBlock) k_clip_tet_walk(ClipArgs g) {
           affb[i] = g.baff[b * 12 + i];
         }
       }
-      walk_start<4>(W, A, planes);
+      {
+        double d0[4];
+        walk_distances(A, planes, d0);
+        W.cls[0] = pack_class(d0);
+        W.level = 1;
+        W.digits = 0;
+      }
       active = true;
     }
-    const bool more = walk_step<4>(W, A, planes, tol, [&](const double* t, double) {
+    const bool more = walk_step_lds(W, A, stk, planes, tol, [&](const double* t, double) {
       const double vr = tet_volume_real(t);
       m = m + vr;
       if (MASS) mass_element<3>(t, vr, affa, affb, blk);
@@ -605,7 +668,7 @@ static unsigned tet_walk_blocks(Context& C, int which) {
       reinterpret_cast<const void*>(k_clip_tet_walk<true, false, true>)};
   if (blocks_per_cu[which] == 0) {
     int nb = 0;
-    LSM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernels[which], kClipBlock, 0));
+    LSM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernels[which], kWalkBlock, 0));
     blocks_per_cu[which] = nb > 0 ? nb : 1;
   }
   int cus = 0;
@@ -613,10 +676,10 @@ static unsigned tet_walk_blocks(Context& C, int which) {
   return (unsigned)(blocks_per_cu[which] * (cus > 0 ? cus : 256));
 }
 
-static long tet_walk_lanes(Context& C, int which) { return (long)tet_walk_blocks(C, which) * kClipBlock; }
+static long tet_walk_lanes(Context& C, int which) { return (long)tet_walk_blocks(C, which) * kWalkBlock; }
 
 static void launch_tet_walk(Context& C, const ClipArgs& args, int which) {
-  dim3 grid(tet_walk_blocks(C, which)), block(kClipBlock);
+  dim3 grid(tet_walk_blocks(C, which)), block(kWalkBlock);
   switch (which) {
     case 0: hipLaunchKernelGGL((k_clip_tet_walk<false, false>), grid, block, 0, C.stream, args); break;
     case 1: hipLaunchKernelGGL((k_clip_tet_walk<true, false>), grid, block, 0, C.stream, args); break;
