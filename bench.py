@@ -252,6 +252,86 @@ def clip_roofline(name, w):
     return out
 
 
+def learned_batched_workload(ctx, steps, warmup, weights, count=1024):
+    """The materialised supermesh (ELEMENTS, rows sorted) of `count` mesh pairs
+    of the batched generator: plain = one count-then-fill call per mesh pair;
+    learned+batched = one batched call per dimension, the 3D one sized by the
+    predictor.  Outputs are compared mesh by mesh by device checksums of each
+    mesh's per-pair counts and output elements."""
+    import numpy as np
+    from libsupermesh_amd import gpu, meshgen
+    t0 = time.time()
+    pairs = meshgen.batched_pairs(count=count, seed=3)
+    plain = [(gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pb, eb)) for _, pa, ea, pb, eb in pairs]
+    batches = {}
+    for dim in (2, 3):
+        sel = [k for k, p in enumerate(pairs) if p[0] == dim]
+        if sel:
+            batches[dim] = (sel, gpu.DeviceMeshBatch([(pairs[k][1], pairs[k][2]) for k in sel]),
+                            gpu.DeviceMeshBatch([(pairs[k][3], pairs[k][4]) for k in sel]))
+    ctx.predictor_set(np.fromfile(weights, dtype=np.float32))
+    log(f"[bench] learned+batched: {len(pairs)} mesh pairs resident ({time.time() - t0:.1f}s)")
+    base = gpu.ELEMENTS | gpu.SORTED
+
+    def sums_plain():
+        out = []
+        for A, B in plain:
+            r = ctx.supermesh(A, B, flags=base)
+            dim = A.dim
+            out.append((r.npairs, r.nout, ctx.checksum(r.counts, 4 * r.npairs),
+                        ctx.checksum(r.elements, 8 * (dim + 1) * dim * r.nout)))
+        return out
+
+    def sums_batched():
+        out = [None] * len(pairs)
+        for dim, (sel, BA, BB) in batches.items():
+            r, br = ctx.supermesh_batched(BA, BB, flags=base | (gpu.LEARNED if dim == 3 else 0))
+            poff, _ = gpu.fetch_batch(br)
+            offs = gpu.device_to_numpy(r.offsets, r.npairs + 1, np.int64)
+            esz = 8 * (dim + 1) * dim
+            for m, k in enumerate(sel):
+                p0, p1 = int(poff[m]), int(poff[m + 1])
+                e0, e1 = int(offs[p0]), int(offs[p1])
+                out[k] = (p1 - p0, e1 - e0, ctx.checksum((r.counts or 0) + 4 * p0, 4 * (p1 - p0)),
+                          ctx.checksum((r.elements or 0) + esz * e0, esz * (e1 - e0)))
+        return out
+
+    def run_plain():
+        tot = nout = 0
+        for A, B in plain:
+            r = ctx.supermesh(A, B, flags=base)
+            tot += r.npairs
+            nout += r.nout
+        return tot, nout
+
+    def run_batched():
+        tot = nout = 0
+        for dim, (sel, BA, BB) in batches.items():
+            r, _ = ctx.supermesh_batched(BA, BB, flags=base | (gpu.LEARNED if dim == 3 else 0))
+            tot += r.npairs
+            nout += r.nout
+        return tot, nout
+
+    same = sums_plain() == sums_batched()
+    out = {}
+    for name, fn in (("plain", run_plain), ("learned_batched", run_batched)):
+        for _ in range(warmup):
+            fn()
+        gpu.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            npairs, nout = fn()
+        gpu.synchronize()
+        dt = (time.perf_counter() - t1) / steps
+        out[name] = {"ms_per_step": dt * 1e3, "pairs": int(npairs), "supermesh_elements": int(nout),
+                     "pairs_per_s": npairs / dt, "elements_per_s": nout / dt}
+        log(f"[bench] learned+batched workload, {name}: {npairs} pairs, {nout} elements, {dt * 1e3:.1f} ms/step")
+    return {"workload": f"{len(pairs)} mesh pairs of the batched generator, materialised supermesh, rows sorted",
+            "plain": out["plain"], "learned_batched": out["learned_batched"],
+            "speedup": out["plain"]["ms_per_step"] / out["learned_batched"]["ms_per_step"],
+            "identical_outputs": same}
+
+
 def cpu_baseline(name, meshes):
     """The CPU restatement of the reference's serial loop (tree finder, then
     the per-pair clip), one thread per host core over contiguous ranges of A."""
@@ -294,6 +374,10 @@ def main():
     batched = None
     if not args.no_batched:
         batched = batched_workload(ctx, max(1, min(args.steps, 3)), 1, count=args.batch_count)
+    learned_batched = None
+    if not args.no_learned and not args.no_batched:
+        learned_batched = learned_batched_workload(ctx, max(1, min(args.steps, 2)), 1, args.weights,
+                                                   count=max(1, args.batch_count // 4))
     learned = None
     if not args.no_learned:
         if not os.path.exists(args.weights):
@@ -327,6 +411,7 @@ def main():
         "tet_64_vs_60": extra,
         "batched": batched,
         "learned": learned,
+        "learned_batched": learned_batched,
     }
     ctx.close()
     print(json.dumps(out))

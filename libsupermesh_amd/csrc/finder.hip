@@ -17,10 +17,12 @@
 // A count pass sizes the CSR rows, a fill pass writes them, and each row is
 // then sorted by B id so the output is canonical.
 #include <cstring>
+#include <utility>
 #include <vector>
 
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_reduce.hpp>
+#include <rocprim/device/device_segmented_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
 #include "lsm_geometry.hpp"
@@ -825,7 +827,24 @@ long find_pairs(Context& C, const DevMesh& A, const DevMesh& B, int finder, int 
     }
     LSM_HIP(hipGetLastError());
     if (sort_rows) {
-      hipLaunchKernelGGL(k_sort_rows, grid, block, 0, C.stream, na, qperm, indptr, indices);
+      if (maxrow <= 64) {
+        hipLaunchKernelGGL(k_sort_rows, grid, block, 0, C.stream, na, qperm, indptr, indices);
+      } else {
+        // long rows (a coarse A element over a fine B mesh): segmented radix
+        // sort of the rows, then the sorted copy becomes the indices buffer
+        LSM_REQUIRE(npairs < (1L << 32) && na < (1L << 32), LIBSUPERMESH_GPU_ERROR_OVERFLOW,
+                    "too many pairs to sort rows");
+        int* sorted = C.vals_alt.as<int>(npairs);
+        unsigned end_bit = 1;
+        while (end_bit < 31 && (1L << end_bit) < B.nelements) end_bit++;
+        size_t bytes = 0;
+        LSM_HIP(rocprim::segmented_radix_sort_keys(nullptr, bytes, indices, sorted, (unsigned)npairs, (unsigned)na,
+                                                   indptr, indptr + 1, 0u, end_bit, C.stream));
+        void* t3 = C.sort_tmp.get(bytes);
+        LSM_HIP(rocprim::segmented_radix_sort_keys(t3, bytes, indices, sorted, (unsigned)npairs, (unsigned)na,
+                                                   indptr, indptr + 1, 0u, end_bit, C.stream));
+        std::swap(C.indices, C.vals_alt);
+      }
       LSM_HIP(hipGetLastError());
     }
   }
