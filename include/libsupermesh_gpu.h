@@ -131,6 +131,18 @@ int libsupermesh_gpu_supermesh(libsupermesh_gpu_context* ctx, const libsupermesh
                                const libsupermesh_gpu_mesh* b, int finder, int flags,
                                libsupermesh_gpu_result* out);
 
+/* The same on meshes of any element type: loc_a / loc_b are dim+1 (triangles,
+ * tetrahedra) or 2^dim (quadrilaterals, hexahedra, convex, nodes in the
+ * reference's order), and each pair is clipped as intersect_elements does
+ * (Supermesh.F90:162-209): tri x quad as intersect_tri_quad, hex x hex as
+ * intersect_hexes (Supermesh.F90:303-334), and so on.  The output elements are
+ * always simplices.  Flags: ELEMENTS, SORTED, TIMING (MASS and LEARNED need
+ * simplex meshes: use libsupermesh_gpu_supermesh).  enlist holds loc ids per
+ * element. */
+int libsupermesh_gpu_supermesh_mixed(libsupermesh_gpu_context* ctx, const libsupermesh_gpu_mesh* a,
+                                     int loc_a, const libsupermesh_gpu_mesh* b, int loc_b, int finder,
+                                     int flags, libsupermesh_gpu_result* out);
+
 /* A batch of independent meshes of one dimension, concatenated: mesh m owns
  * elements element_offsets[m] .. element_offsets[m+1]-1 (device array,
  * [nmeshes + 1], from 0 to nelements).  enlist holds node ids into the whole

@@ -148,13 +148,20 @@ static void run_supermesh(Context& C, const libsupermesh_gpu_mesh* a, const libs
   LSM_HIP(hipSetDevice(C.device));
   std::memset(out, 0, sizeof(*out));
   const bool mass = clip && (flags & LIBSUPERMESH_GPU_MASS);
+  if (!loc_a) loc_a = a->dim + 1;
+  if (!loc_b) loc_b = b->dim + 1;
+  // quads / hexahedra on either side: the mixed clip kernels, which need no
+  // staged planes
+  const bool mixed = loc_a != a->dim + 1 || loc_b != b->dim + 1;
+  if (mixed && clip)
+    LSM_REQUIRE(!(flags & (LIBSUPERMESH_GPU_MASS | LIBSUPERMESH_GPU_LEARNED)) && ngroups == 0,
+                LIBSUPERMESH_GPU_ERROR_INVALID, "MASS, LEARNED and batches need simplex meshes");
   Phase ph(C, flags & LIBSUPERMESH_GPU_TIMING);
   C.timing = ph.on;
   ph.mark();
-  stage_mesh(C, C.mesh_a, a->dim, loc_a ? loc_a : a->dim + 1, a->nnodes, a->nelements, a->positions,
-             a->enlist, false, mass);
-  stage_mesh(C, C.mesh_b, b->dim, loc_b ? loc_b : b->dim + 1, b->nnodes, b->nelements, b->positions,
-             b->enlist, clip, mass);
+  stage_mesh(C, C.mesh_a, a->dim, loc_a, a->nnodes, a->nelements, a->positions, a->enlist, false, mass);
+  stage_mesh(C, C.mesh_b, b->dim, loc_b, b->nnodes, b->nelements, b->positions, b->enlist,
+             clip && !mixed, mass);
   stage_groups(C, C.mesh_a, ngroups, a_off);
   stage_groups(C, C.mesh_b, ngroups, b_off);
   ph.mark();
@@ -167,7 +174,11 @@ static void run_supermesh(Context& C, const libsupermesh_gpu_mesh* a, const libs
   out->indptr = static_cast<const long*>(C.indptr.ptr);
   out->indices = static_cast<const int*>(C.indices.get(4));
   out->rows = static_cast<const int*>(C.rows.get(4));
-  if (clip) {
+  if (clip && mixed) {
+    clip_pairs_mixed(C, C.mesh_a, C.mesh_b, npairs, flags, out);
+    ph.mark();
+    ph.mark();
+  } else if (clip) {
     clip_pairs(C, C.mesh_a, C.mesh_b, npairs, flags, out);
     if (ngroups > 0) batch_sums(C, ngroups, a_off, out->indptr, out->measure);
     ph.mark();
@@ -525,6 +536,20 @@ int libsupermesh_gpu_supermesh(libsupermesh_gpu_context* ctx, const libsupermesh
     LSM_REQUIRE(ctx != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID, "null context");
     std::lock_guard<std::mutex> lk(ctx->ctx.mu);
     run_supermesh(ctx->ctx, a, b, finder, flags, true, out);
+  });
+}
+
+int libsupermesh_gpu_supermesh_mixed(libsupermesh_gpu_context* ctx, const libsupermesh_gpu_mesh* a, int loc_a,
+                                     const libsupermesh_gpu_mesh* b, int loc_b, int finder, int flags,
+                                     libsupermesh_gpu_result* out) {
+  return guarded([&] {
+    LSM_REQUIRE(ctx != nullptr && a != nullptr && b != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID,
+                "null argument");
+    const int d = a->dim;
+    LSM_REQUIRE((loc_a == d + 1 || loc_a == 1 << d) && (loc_b == d + 1 || loc_b == 1 << d),
+                LIBSUPERMESH_GPU_ERROR_INVALID, "Invalid element type: loc must be dim+1 or 2^dim");
+    std::lock_guard<std::mutex> lk(ctx->ctx.mu);
+    run_supermesh(ctx->ctx, a, b, finder, flags, true, out, 0, nullptr, nullptr, loc_a, loc_b);
   });
 }
 

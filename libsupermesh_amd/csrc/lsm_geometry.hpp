@@ -227,13 +227,13 @@ LSM_HD void pipe_close(ClipStage* st, const Line* L, Last& last) {
   }
 }
 
-// Clips triangle A (3 points) against NL lines, passes in line order, then
+// Clips polygon A (NA points, a triangle by default) against NL lines, passes in line order, then
 // fans the result (intersect_polys_lines, Triangle_Intersection.F90:216-276).
 // For each kept output triangle (in the reference's order) calls
 // sink(x0, y0, x1, y1, x2, y2, area).  Returns the number kept; if any pass
 // ends with fewer than 3 points the reference returns 0 (and calls made to
 // the sink must then be discarded, which only matters before a count is known).
-template <int NL, class Sink>
+template <int NL, int NA = 3, class Sink>
 LSM_HD int clip_poly(const double* A, const Line* L, double tol, Sink&& sink) {
   ClipStage st[NL];
 #pragma unroll
@@ -257,9 +257,8 @@ LSM_HD int clip_poly(const double* A, const Line* L, double tol, Sink&& sink) {
     fan.y1 = y;
     fan.n++;
   };
-  pipe_push<0, NL>(st, L, A[0], A[1], fan_push);
-  pipe_push<0, NL>(st, L, A[2], A[3], fan_push);
-  pipe_push<0, NL>(st, L, A[4], A[5], fan_push);
+#pragma unroll
+  for (int v = 0; v < NA; v++) pipe_push<0, NL>(st, L, A[2 * v], A[2 * v + 1], fan_push);
   pipe_close<0, NL>(st, L, fan_push);
 #pragma unroll
   for (int i = 0; i < NL; i++)

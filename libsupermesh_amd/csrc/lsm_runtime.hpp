@@ -166,6 +166,10 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
                 libsupermesh_gpu_result* out);
 void assemble_mass(Context& C, const DevMesh& A, const DevMesh& B, long npairs,
                    libsupermesh_gpu_result* out);
+// Clipping of candidate pairs where A or B is not a simplex mesh (quads in
+// 2D, hexahedra in 3D).
+void clip_pairs_mixed(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int flags,
+                      libsupermesh_gpu_result* out);
 // Order-sensitive checksum of a device buffer (whole 32-bit words).
 unsigned long long checksum(Context& C, const void* ptr, size_t bytes);
 // Work list (pairs that needed the full clip) of the last 3D clip_pairs call.

@@ -557,6 +557,186 @@ __global__ void __launch_bounds__(kClipBlock) k_reduce_partials(int n, Partial* 
   if (threadIdx.x == 0) parts[0] = red[0];
 }
 
+// ---------------------------------------------------------------------------
+// Mixed element types in bulk: the dispatch of intersect_elements
+// (Supermesh.F90:162-209) for one pair per lane.  2D: tri x quad clips the
+// triangle by the quad's lines (tol from the triangle), quad x tri clips B's
+// triangle by A's lines, quad x quad clips A's quad by B's lines (tol 10*tiny);
+// 3D: tet x hex clips the tet by the hex's 6 planes, hex x tet the reverse,
+// hex x hex clips A's five tets (Supermesh.F90:303-334) in turn.  Counts,
+// measures (triangle_area / tetrahedron_volume_real of each output) and, in
+// the fill pass, the output elements.
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256)
+k_count_to_long(long n, const int* __restrict__ counts, long* __restrict__ out) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = counts[i];
+}
+
+struct MixedArgs {
+  long npairs;
+  const int* rows;
+  const int* cols;
+  const double* acoords;
+  const double* bcoords;
+  int* counts;
+  double* measure;
+  const long* offsets;
+  double* elements;
+};
+
+// The five tets of a hexahedron in Gmsh order: vertices 3678, 1348, 1568,
+// 1236, 1368 (1-based), intersect_hexes (Supermesh.F90:318-332).
+__constant__ int kHexSplit[5][4] = {{2, 5, 6, 7}, {0, 2, 3, 7}, {0, 4, 5, 7}, {0, 1, 2, 5}, {0, 2, 5, 7}};
+
+template <int DIM, int LA, int LB, bool FILL>
+__global__ void __launch_bounds__(kClipBlock) k_clip_mixed(MixedArgs g) {
+  constexpr int ESZ = (DIM + 1) * DIM;  // output simplex
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < g.npairs; p += stride) {
+    long off = 0;
+    int cap = 0;
+    if (FILL) {
+      cap = g.counts[p];
+      if (cap == 0) continue;
+      off = g.offsets[p];
+    }
+    const long a = g.rows[p], b = g.cols[p];
+    double A[LA * DIM], B[LB * DIM];
+#pragma unroll
+    for (int k = 0; k < LA * DIM; k++) A[k] = g.acoords[a * LA * DIM + k];
+#pragma unroll
+    for (int k = 0; k < LB * DIM; k++) B[k] = g.bcoords[b * LB * DIM + k];
+    double m = 0.0;
+    int k = 0;
+    auto put = [&](const double* t) {
+      if (FILL && k < cap) {
+        double* o = g.elements + (off + k) * ESZ;
+#pragma unroll
+        for (int i = 0; i < ESZ; i++) o[i] = t[i];
+      }
+      k++;
+    };
+    if constexpr (DIM == 2) {
+      auto tri_sink = [&](double x0, double y0, double x1, double y1, double x2, double y2, double ar) {
+        const double t[6] = {x0, y0, x1, y1, x2, y2};
+        m = m + ar;
+        put(t);
+      };
+      Line L[4];
+      int n = 0;
+      if constexpr (LA == 3 && LB == 4) {
+        quad_lines(B, L);
+        n = clip_poly<4, 3>(A, L, 10.0 * fspacing(triangle_area(A)), tri_sink);
+      } else if constexpr (LA == 4 && LB == 3) {
+        quad_lines(A, L);
+        n = clip_poly<4, 3>(B, L, 10.0 * fspacing(triangle_area(B)), tri_sink);
+      } else {
+        quad_lines(B, L);
+        n = clip_poly<4, 4>(A, L, 10.0 * 2.2250738585072014e-308, tri_sink);
+      }
+      if (n == 0) {
+        k = 0;
+        m = 0.0;
+      }
+    } else {
+      auto tet_sink = [&](const double* t, double) {
+        m = m + tet_volume_real(t);
+        put(t);
+      };
+      if constexpr (LA == 4 && LB == 8) {
+        clip_tet_hex(A, B, tet_sink);
+      } else if constexpr (LA == 8 && LB == 4) {
+        clip_tet_hex(B, A, tet_sink);
+      } else {
+        for (int s = 0; s < 5; s++) {
+          double t[12];
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            const int hv = kHexSplit[s][v];
+            t[3 * v] = A[3 * hv];
+            t[3 * v + 1] = A[3 * hv + 1];
+            t[3 * v + 2] = A[3 * hv + 2];
+          }
+          clip_tet_hex(t, B, tet_sink);
+        }
+      }
+    }
+    if (FILL) continue;
+    g.counts[p] = k;
+    g.measure[p] = k == 0 ? 0.0 : m;
+  }
+}
+
+template <bool FILL>
+static void launch_mixed(Context& C, int dim, int la, int lb, const MixedArgs& g) {
+  const dim3 grid(kClipGrid), block(kClipBlock);
+  if (dim == 2) {
+    if (la == 3 && lb == 4) hipLaunchKernelGGL((k_clip_mixed<2, 3, 4, FILL>), grid, block, 0, C.stream, g);
+    else if (la == 4 && lb == 3) hipLaunchKernelGGL((k_clip_mixed<2, 4, 3, FILL>), grid, block, 0, C.stream, g);
+    else hipLaunchKernelGGL((k_clip_mixed<2, 4, 4, FILL>), grid, block, 0, C.stream, g);
+  } else {
+    if (la == 4 && lb == 8) hipLaunchKernelGGL((k_clip_mixed<3, 4, 8, FILL>), grid, block, 0, C.stream, g);
+    else if (la == 8 && lb == 4) hipLaunchKernelGGL((k_clip_mixed<3, 8, 4, FILL>), grid, block, 0, C.stream, g);
+    else hipLaunchKernelGGL((k_clip_mixed<3, 8, 8, FILL>), grid, block, 0, C.stream, g);
+  }
+  LSM_HIP(hipGetLastError());
+}
+
+void clip_pairs_mixed(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int flags,
+                      libsupermesh_gpu_result* out) {
+  const int dim = A.dim;
+  MixedArgs g = {};
+  g.npairs = npairs;
+  g.rows = out->rows;
+  g.cols = out->indices;
+  g.acoords = static_cast<const double*>(A.coords.ptr);
+  g.bcoords = static_cast<const double*>(B.coords.ptr);
+  g.counts = C.counts.as<int>(npairs);
+  g.measure = C.measure.as<double>(npairs);
+  if (npairs > 0) launch_mixed<false>(C, dim, A.loc, B.loc, g);
+  Partial* parts = C.partials.as<Partial>(kClipGrid);
+  hipLaunchKernelGGL(k_reduce_pairs, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, npairs, g.counts, g.measure,
+                     parts);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kClipBlock), 0, C.stream, kClipGrid, parts);
+  LSM_HIP(hipGetLastError());
+  Partial tot;
+  LSM_HIP(hipMemcpyAsync(&tot, parts, sizeof(Partial), hipMemcpyDeviceToHost, C.stream));
+  LSM_HIP(hipStreamSynchronize(C.stream));
+  out->counts = g.counts;
+  out->measure = g.measure;
+  out->blocks = nullptr;
+  out->sum = npairs > 0 ? tot.hi + tot.lo : 0.0;
+  out->nonempty = npairs > 0 ? tot.nonempty : 0;
+  out->nout = npairs > 0 ? tot.nout : 0;
+  out->max_count = npairs > 0 ? (int)tot.maxc : 0;
+  out->offsets = nullptr;
+  out->elements = nullptr;
+  if (flags & LIBSUPERMESH_GPU_ELEMENTS) {
+    const int esz = (dim + 1) * dim;
+    long* offs = C.offsets.as<long>(npairs + 1);
+    LSM_HIP(hipMemsetAsync(offs, 0, sizeof(long), C.stream));
+    if (npairs > 0) {
+      long* wide = C.row_count.as<long>(npairs);
+      hipLaunchKernelGGL(k_count_to_long, dim3(grid_for(npairs, 256)), dim3(256), 0, C.stream, npairs, g.counts,
+                         wide);
+      size_t bytes = 0;
+      LSM_HIP(rocprim::inclusive_scan(nullptr, bytes, wide, offs + 1, (size_t)npairs, rocprim::plus<long>(),
+                                      C.stream));
+      void* tmp = C.scan_tmp.get(bytes);
+      LSM_HIP(rocprim::inclusive_scan(tmp, bytes, wide, offs + 1, (size_t)npairs, rocprim::plus<long>(),
+                                      C.stream));
+      g.offsets = offs;
+      g.elements = C.elements.as<double>((size_t)(out->nout > 0 ? out->nout : 1) * esz);
+      launch_mixed<true>(C, dim, A.loc, B.loc, g);
+    }
+    LSM_HIP(hipStreamSynchronize(C.stream));
+    out->offsets = offs;
+    out->elements = static_cast<const double*>(C.elements.get(8));
+  }
+}
+
 // Batches: candidate pair range of each mesh (its A elements are the rows
 // a_off[g] .. a_off[g+1]) ...
 __global__ void __launch_bounds__(kClipBlock)
@@ -738,12 +918,6 @@ k_compact_slots(const unsigned long long* __restrict__ nwork, const int* __restr
       }
     }
   }
-}
-
-__global__ void __launch_bounds__(256)
-k_count_to_long(long n, const int* __restrict__ counts, long* __restrict__ out) {
-  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = counts[i];
 }
 
 __global__ void __launch_bounds__(256)

@@ -78,6 +78,8 @@ def lib():
             f = getattr(L, name)
             f.argtypes = [c_void_p, POINTER(Mesh), POINTER(Mesh), c_int, c_int, POINTER(Result)]
             f.restype = c_int
+        L.libsupermesh_gpu_supermesh_mixed.argtypes = [c_void_p, POINTER(Mesh), c_int, POINTER(Mesh),
+                                                       c_int, c_int, c_int, POINTER(Result)]
         L.libsupermesh_gpu_clip_pairs.argtypes = [c_void_p, POINTER(Mesh), POINTER(Mesh), c_long,
                                                   c_void_p, c_void_p, c_int, POINTER(Result)]
         L.libsupermesh_gpu_clip_pairs.restype = c_int
@@ -310,12 +312,14 @@ def tetrahedron_volume(tet):
 
 class DeviceMesh:
     """A mesh copied to device memory: positions float64[nnodes, dim],
-    enlist int32[nelements, dim+1] (0-based)."""
+    enlist int32[nelements, loc] (0-based; loc = dim+1 for simplices, 2^dim
+    for quadrilaterals / hexahedra)."""
 
     def __init__(self, positions, enlist, device=0):
         pos = np.ascontiguousarray(positions, dtype=np.float64)
         enl = np.ascontiguousarray(enlist, dtype=np.int32)
         self.dim = pos.shape[1]
+        self.loc = enl.shape[1]
         self.positions = DeviceArray(pos, device)
         self.enlist = DeviceArray(enl, device)
         self.struct = Mesh(self.dim, pos.shape[0], enl.shape[0], self.positions.ptr.value,
@@ -393,6 +397,15 @@ class Context:
         _check(lib().libsupermesh_gpu_supermesh(self.handle, ctypes.byref(a.struct),
                                                 ctypes.byref(b.struct), finder, flags,
                                                 ctypes.byref(r)))
+        return r
+
+    def supermesh_mixed(self, a: DeviceMesh, b: DeviceMesh, finder=FINDER_RTREE, flags=0):
+        """Supermesh of meshes of any element type (triangles / quadrilaterals
+        in 2D, tetrahedra / hexahedra in 3D), clipped as intersect_elements."""
+        r = Result()
+        _check(lib().libsupermesh_gpu_supermesh_mixed(self.handle, ctypes.byref(a.struct), a.loc,
+                                                      ctypes.byref(b.struct), b.loc, finder, flags,
+                                                      ctypes.byref(r)))
         return r
 
     def intersection_finder(self, a: DeviceMesh, b: DeviceMesh, finder=FINDER_TREE, flags=0):

@@ -84,3 +84,119 @@ def test_finders_on_quads_and_hexes(fixture_meshes, names, closed):
     ip, ix = O.brute_force(pa, ea, pb, eb, closed=closed)
     assert np.array_equal(indptr, ip)
     assert np.array_equal(indices, ix)
+
+
+def _oracle_elements(ea_xy, eb_xy):
+    """intersect_elements (Supermesh.F90:162-209) over the oracle's routines."""
+    dim, la, lb = ea_xy.shape[1], ea_xy.shape[0], eb_xy.shape[0]
+    if dim == 2:
+        if la == 3 and lb == 4:
+            return O.intersect_tri_quad(ea_xy, eb_xy)
+        if la == 4 and lb == 3:
+            return O.intersect_tri_quad(eb_xy, ea_xy)
+        return O.intersect_quads(ea_xy, eb_xy)
+    if la == 4 and lb == 8:
+        return O.intersect_tet_hex(ea_xy, eb_xy)
+    if la == 8 and lb == 4:
+        return O.intersect_tet_hex(eb_xy, ea_xy)
+    return O.intersect_hexes(ea_xy, eb_xy)
+
+
+def _check_bulk(pa, ea, pb, eb, finder, sort, expected=None):
+    dim = pa.shape[1]
+    ctx = gpu.Context()
+    flags = gpu.ELEMENTS | (gpu.SORTED if sort else 0)
+    r = ctx.supermesh_mixed(gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pb, eb), finder=finder, flags=flags)
+    out = gpu.fetch(r, dim, ["indptr", "indices", "rows", "counts", "measure", "offsets", "elements"])
+    ip, ix = O.brute_force(pa, ea, pb, eb, closed=finder == gpu.FINDER_RTREE)
+    assert np.array_equal(out["indptr"], ip)
+    if sort:
+        assert np.array_equal(out["indices"], ix)
+    else:
+        for a in range(len(ip) - 1):
+            assert sorted(out["indices"][ip[a]:ip[a + 1]]) == list(ix[ip[a]:ip[a + 1]])
+    meas = O.triangle_area if dim == 2 else O.tet_volume_real
+    total, nout = 0.0, 0
+    for p, (a, b) in enumerate(zip(out["rows"], out["indices"])):
+        ref = _oracle_elements(pa[ea[a]], pb[eb[b]])
+        assert out["counts"][p] == len(ref), p
+        m = 0.0
+        for t in ref:
+            m = m + meas(t)
+        assert out["measure"][p] == m, p
+        o0, o1 = out["offsets"][p], out["offsets"][p + 1]
+        assert o1 - o0 == len(ref)
+        assert np.array_equal(out["elements"][o0:o1], ref.reshape(-1, dim + 1, dim)), p
+        total += m
+        nout += len(ref)
+    assert r.nout == nout and r.npairs == len(ix)
+    assert abs(r.sum - total) <= 1e-13 * max(1.0, total)
+    if expected is not None:
+        assert abs(r.sum - expected) < 1e-12
+    return r
+
+
+@pytest.mark.parametrize("names,expected", [
+    (("square.1", "square.4"), 1.0), (("square.4", "square.1"), 1.0), (("square.3", "square.2"), 1.0),
+    (("square.3", "square.4"), 1.0), (("cube.1", "cube.3"), 0.5), (("cube.3", "cube.2"), 0.5),
+    (("cube.3", "cube.4"), 0.5)])
+def test_bulk_mixed_reference_meshes(fixture_meshes, names, expected):
+    # the meshes and totals of src/tests/test_element_intersector.F90:136-332,
+    # every candidate pair clipped in one call, pair by pair equal to the oracle
+    pa, ea = fixture_meshes[names[0]]
+    pb, eb = fixture_meshes[names[1]]
+    _check_bulk(pa, ea, pb, eb, gpu.FINDER_RTREE, True, expected)
+
+
+def _jitter_quads(dim, n, seed):
+    """A structured grid of the unit square / cube, nodes in the reference's
+    element order: convex quadrilaterals with jittered interior nodes (2D), or
+    hexahedra on unevenly spaced axes, so that their faces stay planar and
+    the pieces tile the cube exactly (3D)."""
+    rng = np.random.default_rng(seed)
+    axes = []
+    for _ in range(dim):
+        ax = np.linspace(0.0, 1.0, n + 1)
+        if dim == 3:
+            ax[1:-1] += rng.uniform(-0.3, 0.3, n - 1) / n
+        axes.append(ax)
+    grid = np.stack(np.meshgrid(*axes, indexing="ij"), -1).reshape(-1, dim)
+    if dim == 2:
+        interior = np.all((grid > 0) & (grid < 1), axis=1)
+        grid[interior] += rng.uniform(-0.2, 0.2, (interior.sum(), dim)) / n
+    idx = np.arange((n + 1) ** dim).reshape([n + 1] * dim)
+    if dim == 2:
+        i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+        i, j = i.ravel(), j.ravel()
+        enl = np.stack([idx[i, j], idx[i + 1, j], idx[i + 1, j + 1], idx[i, j + 1]], 1)
+    else:
+        i, j, k = (x.ravel() for x in np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij"))
+        enl = np.stack([idx[i, j, k], idx[i + 1, j, k], idx[i + 1, j + 1, k], idx[i, j + 1, k],
+                        idx[i, j, k + 1], idx[i + 1, j, k + 1], idx[i + 1, j + 1, k + 1],
+                        idx[i, j + 1, k + 1]], 1)
+    return grid, enl.astype(np.int32)
+
+
+@pytest.mark.parametrize("dim,na,nb", [(2, 9, 7), (3, 3, 4)])
+@pytest.mark.parametrize("kind", ["simplex_x_quad", "quad_x_simplex", "quad_x_quad"])
+@pytest.mark.parametrize("finder,sort", [(gpu.FINDER_RTREE, True), (gpu.FINDER_TREE, False)])
+def test_bulk_mixed_jittered(dim, na, nb, kind, finder, sort):
+    from libsupermesh_amd import meshgen
+    qa = _jitter_quads(dim, na, 11)
+    qb = _jitter_quads(dim, nb, 12)
+    sa = meshgen.split_grid(dim, na, 5, mesh_id=0, diagonal=0)
+    sb = meshgen.split_grid(dim, nb, 5, mesh_id=1, diagonal=1)
+    A, B = {"simplex_x_quad": (sa, qb), "quad_x_simplex": (qa, sb), "quad_x_quad": (qa, qb)}[kind]
+    r = _check_bulk(A[0], A[1], B[0], B[1], finder, sort)
+    assert abs(r.sum - 1.0) < 1e-10
+
+
+def test_bulk_mixed_rejects_mass_and_bad_loc():
+    pa, ea = _jitter_quads(2, 3, 1)
+    ctx = gpu.Context()
+    m = gpu.DeviceMesh(pa, ea)
+    with pytest.raises(gpu.GpuError, match="simplex"):
+        ctx.supermesh_mixed(m, m, flags=gpu.MASS)
+    bad = gpu.DeviceMesh(pa, np.zeros((2, 5), dtype=np.int32))
+    with pytest.raises(gpu.GpuError, match="loc"):
+        ctx.supermesh_mixed(m, bad)
