@@ -257,8 +257,13 @@ LSM_HD int clip_poly(const double* A, const Line* L, double tol, Sink&& sink) {
     fan.y1 = y;
     fan.n++;
   };
-#pragma unroll
-  for (int v = 0; v < NA; v++) pipe_push<0, NL>(st, L, A[2 * v], A[2 * v + 1], fan_push);
+  static_assert(NA == 3 || NA == 4, "triangles and quadrilaterals");
+  // one call per vertex, written out: a loop here is inlined before it is
+  // unrolled and costs the triangle kernels registers
+  pipe_push<0, NL>(st, L, A[0], A[1], fan_push);
+  pipe_push<0, NL>(st, L, A[2], A[3], fan_push);
+  pipe_push<0, NL>(st, L, A[4], A[5], fan_push);
+  if constexpr (NA == 4) pipe_push<0, NL>(st, L, A[6], A[7], fan_push);
   pipe_close<0, NL>(st, L, fan_push);
 #pragma unroll
   for (int i = 0; i < NL; i++)
