@@ -52,6 +52,7 @@ def parse():
     p.add_argument("--no-3d", action="store_true", help="skip the 3D workload")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--no-batched", action="store_true", help="skip the batched workload")
+    p.add_argument("--no-mixed", action="store_true", help="skip the tri/quad and tet/hex workloads")
     p.add_argument("--batch-count", type=int, default=4096, help="mesh pairs in the batched workload")
     p.add_argument("--no-learned", action="store_true", help="skip the learned single-pass workload")
     p.add_argument("--learned-workload", default="tet_96_vs_90")
@@ -107,6 +108,44 @@ def gpu_workload(ctx, name, steps, warmup, mass):
         "mass_nnz": int(r.mass_nnz) if mass else None,
         "_meshes": (pa, ea, pb, eb),
     }
+
+
+MIXED_WORKLOADS = {
+    # name: (dim, simplex grid n, cell grid n)
+    "tri_1024_vs_quad_1000": (2, 1024, 1000),
+    "tet_64_vs_hex_60": (3, 64, 60),
+}
+
+
+def mixed_workload(ctx, steps, warmup):
+    """Simplex mesh A against a quadrilateral / hexahedral mesh B, every
+    candidate pair clipped as intersect_elements does (tri x quad, tet x hex),
+    counts, measures and the compensated sum, R-tree (closed) candidates."""
+    from libsupermesh_amd import gpu, meshgen
+    out = {}
+    for name, (dim, na, nb) in MIXED_WORKLOADS.items():
+        pa, ea = meshgen.split_grid(dim, na, 6, mesh_id=0)
+        pb, eb = meshgen.cell_grid(dim, nb, 6, mesh_id=1)
+        A, B = gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pb, eb)
+        flags = gpu.TIMING
+        for _ in range(warmup):
+            r = ctx.supermesh_mixed(A, B, flags=flags)
+        gpu.synchronize()
+        t0 = time.perf_counter()
+        phases = []
+        for _ in range(steps):
+            r = ctx.supermesh_mixed(A, B, flags=flags)
+            phases.append(list(r.ms))
+        gpu.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        err = abs(r.sum - 1.0)
+        log(f"[bench] {name}: {r.npairs} pairs, {r.nout} elements, {dt * 1e3:.2f} ms/step, |sum-1|={err:.2e}")
+        names = ["stage", "hierarchy", "search", "clip"]
+        out[name] = {"elements_a": int(ea.shape[0]), "elements_b": int(eb.shape[0]), "pairs": int(r.npairs),
+                     "nonempty": int(r.nonempty), "supermesh_elements": int(r.nout), "sum_abs_err": err,
+                     "ms_per_step": dt * 1e3, "pairs_per_s": r.npairs / dt,
+                     "phase_ms": {n: round(sum(p[i] for p in phases) / steps, 3) for i, n in enumerate(names)}}
+    return out
 
 
 def batched_workload(ctx, steps, warmup, count=4096):
@@ -371,6 +410,9 @@ def main():
     extra = None
     if not args.no_3d and args.workload != "tet_64_vs_60":
         extra = gpu_workload(ctx, "tet_64_vs_60", max(1, args.steps), max(1, args.warmup), mass=False)
+    mixed = None
+    if not args.no_mixed:
+        mixed = mixed_workload(ctx, max(1, min(args.steps, 3)), 1)
     batched = None
     if not args.no_batched:
         batched = batched_workload(ctx, max(1, min(args.steps, 3)), 1, count=args.batch_count)
@@ -409,6 +451,7 @@ def main():
         "cpu_baseline": cpu,
         "detail": main_w,
         "tet_64_vs_60": extra,
+        "mixed_types": mixed,
         "batched": batched,
         "learned": learned,
         "learned_batched": learned_batched,

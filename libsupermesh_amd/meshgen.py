@@ -152,6 +152,43 @@ def split_grid(dim, n, seed, mesh_id=0, alpha=None, diagonal=0, max_redraw=16):
         pos = base + disp
 
 
+def cell_grid(dim, n, seed, mesh_id=0, alpha=0.2):
+    """Grid of the unit square / cube in quadrilaterals (2D, ``n^2``) or
+    hexahedra (3D, ``n^3``), nodes in the reference's element order (2D:
+    counter-clockwise; 3D: the z-low face counter-clockwise, then the z-high
+    face).  2D: interior vertices jittered by up to ``alpha h`` per coordinate
+    (convex cells for alpha < 0.25).  3D: the grid lines of each axis are
+    jittered instead, so every face stays planar and the hexahedra tile the
+    cube exactly.  Returns ``(positions float64[nn, dim], enlist int32[ne,
+    2^dim])``."""
+    if dim not in (2, 3):
+        raise ValueError("Invalid dimension")
+    if n < 1:
+        raise ValueError("n must be >= 1")
+    h = 1.0 / float(n)
+    base = _grid_vertices(dim, n)
+    nn = base.shape[0]
+    if dim == 2:
+        interior = _interior_mask(dim, n)
+        u = uniform01(seed, mesh_id, 0, np.arange(nn * dim, dtype=np.uint64)).reshape(nn, dim)
+        pos = base + np.where(interior[:, None], (2.0 * u - 1.0) * (alpha * h), 0.0)
+        i, j = (x.ravel() for x in np.meshgrid(np.arange(n), np.arange(n), indexing="ij"))
+        v = lambda a, b: b * (n + 1) + a  # noqa: E731
+        enl = np.stack([v(i, j), v(i + 1, j), v(i + 1, j + 1), v(i, j + 1)], axis=1)
+    else:
+        u = uniform01(seed, mesh_id, 0, np.arange(3 * (n + 1), dtype=np.uint64)).reshape(3, n + 1)
+        t = np.arange(n + 1, dtype=np.float64) * h
+        axes = [t + np.where((t > 0) & (t < 1), (2.0 * u[d] - 1.0) * (alpha * h), 0.0) for d in range(3)]
+        z, y, x = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
+        pos = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
+        i, j, k = (x.ravel() for x in np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij"))
+        v = lambda a, b, c: (c * (n + 1) + b) * (n + 1) + a  # noqa: E731
+        enl = np.stack([v(i, j, k), v(i + 1, j, k), v(i + 1, j + 1, k), v(i, j + 1, k),
+                        v(i, j, k + 1), v(i + 1, j, k + 1), v(i + 1, j + 1, k + 1), v(i, j + 1, k + 1)],
+                       axis=1)
+    return pos, enl.astype(np.int32)
+
+
 # Named workloads: (dim, n_a, n_b, seed, alpha)
 CONFIGS = {
     "tri_64_vs_50": (2, 64, 50, 0, 0.3),

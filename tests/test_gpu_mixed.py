@@ -148,42 +148,13 @@ def test_bulk_mixed_reference_meshes(fixture_meshes, names, expected):
     _check_bulk(pa, ea, pb, eb, gpu.FINDER_RTREE, True, expected)
 
 
-def _jitter_quads(dim, n, seed):
-    """A structured grid of the unit square / cube, nodes in the reference's
-    element order: convex quadrilaterals with jittered interior nodes (2D), or
-    hexahedra on unevenly spaced axes, so that their faces stay planar and
-    the pieces tile the cube exactly (3D)."""
-    rng = np.random.default_rng(seed)
-    axes = []
-    for _ in range(dim):
-        ax = np.linspace(0.0, 1.0, n + 1)
-        if dim == 3:
-            ax[1:-1] += rng.uniform(-0.3, 0.3, n - 1) / n
-        axes.append(ax)
-    grid = np.stack(np.meshgrid(*axes, indexing="ij"), -1).reshape(-1, dim)
-    if dim == 2:
-        interior = np.all((grid > 0) & (grid < 1), axis=1)
-        grid[interior] += rng.uniform(-0.2, 0.2, (interior.sum(), dim)) / n
-    idx = np.arange((n + 1) ** dim).reshape([n + 1] * dim)
-    if dim == 2:
-        i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
-        i, j = i.ravel(), j.ravel()
-        enl = np.stack([idx[i, j], idx[i + 1, j], idx[i + 1, j + 1], idx[i, j + 1]], 1)
-    else:
-        i, j, k = (x.ravel() for x in np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij"))
-        enl = np.stack([idx[i, j, k], idx[i + 1, j, k], idx[i + 1, j + 1, k], idx[i, j + 1, k],
-                        idx[i, j, k + 1], idx[i + 1, j, k + 1], idx[i + 1, j + 1, k + 1],
-                        idx[i, j + 1, k + 1]], 1)
-    return grid, enl.astype(np.int32)
-
-
 @pytest.mark.parametrize("dim,na,nb", [(2, 9, 7), (3, 3, 4)])
 @pytest.mark.parametrize("kind", ["simplex_x_quad", "quad_x_simplex", "quad_x_quad"])
 @pytest.mark.parametrize("finder,sort", [(gpu.FINDER_RTREE, True), (gpu.FINDER_TREE, False)])
 def test_bulk_mixed_jittered(dim, na, nb, kind, finder, sort):
     from libsupermesh_amd import meshgen
-    qa = _jitter_quads(dim, na, 11)
-    qb = _jitter_quads(dim, nb, 12)
+    qa = meshgen.cell_grid(dim, na, 11)
+    qb = meshgen.cell_grid(dim, nb, 12, mesh_id=1)
     sa = meshgen.split_grid(dim, na, 5, mesh_id=0, diagonal=0)
     sb = meshgen.split_grid(dim, nb, 5, mesh_id=1, diagonal=1)
     A, B = {"simplex_x_quad": (sa, qb), "quad_x_simplex": (qa, sb), "quad_x_quad": (qa, qb)}[kind]
@@ -192,7 +163,8 @@ def test_bulk_mixed_jittered(dim, na, nb, kind, finder, sort):
 
 
 def test_bulk_mixed_rejects_mass_and_bad_loc():
-    pa, ea = _jitter_quads(2, 3, 1)
+    from libsupermesh_amd import meshgen
+    pa, ea = meshgen.cell_grid(2, 3, 1)
     ctx = gpu.Context()
     m = gpu.DeviceMesh(pa, ea)
     with pytest.raises(gpu.GpuError, match="simplex"):

@@ -261,3 +261,27 @@ def test_sort_finder_equals_strict_brute_force_on_meshes(na, nb, shuffle):
     for a in range(ea.shape[0]):
         row = indices[indptr[a]:indptr[a + 1]]
         assert np.all(np.diff(lb[row]) > 0)
+
+
+@pytest.mark.parametrize("dim,n", [(2, 6), (3, 3)])
+def test_cell_grid_tiles_the_unit_domain(dim, n):
+    # the quad / hex generator of the mixed-type workloads: convex cells in
+    # the reference's node order whose clips against a simplex mesh of the
+    # same domain sum to exactly its measure
+    from libsupermesh_amd import meshgen
+    pq, eq = meshgen.cell_grid(dim, n, 3)
+    ps, es = meshgen.split_grid(dim, n + 1, 4)
+    if dim == 2:
+        q = pq[eq]
+        d = np.roll(q, -1, axis=1) - q
+        cross = d[:, :, 0] * np.roll(d, -1, axis=1)[:, :, 1] - d[:, :, 1] * np.roll(d, -1, axis=1)[:, :, 0]
+        assert (cross > 0).all()  # counter-clockwise and strictly convex
+    else:
+        h = pq[eq]
+        assert np.array_equal(h[:, 1, 1:], h[:, 0, 1:]) and np.array_equal(h[:, 4, :2], h[:, 0, :2])
+    meas = O.triangle_area if dim == 2 else O.tet_volume_real
+    fn = O.intersect_tri_quad if dim == 2 else O.intersect_tet_hex
+    total = 0.0
+    for a, b in _pairs_closed(ps, es, pq, eq):
+        total += sum(meas(t) for t in fn(ps[es[a]], pq[eq[b]]))
+    assert abs(total - 1.0) < 1e-12
