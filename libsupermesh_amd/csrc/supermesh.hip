@@ -9,6 +9,7 @@
 // by a plain in-order sum of triangle_area / tetrahedron_volume; the global
 // sum uses double-double (TwoSum) accumulation in a fixed reduction tree, so
 // it is deterministic for a given input.
+#include <algorithm>
 #include <cstring>
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -343,37 +344,43 @@ __global__ void __launch_bounds__(kClipBlock) k_clip(ClipArgs g) {
 // with stk = stack + t (consecutive lanes, consecutive words).
 constexpr int kWalkBlock = 128;
 
-struct TetWalkLds {
-  int cls[4];
+// NP planes: class of the tet at each depth 0..NP-1, current depth, and the
+// child index reached at each depth 1..NP (2 bits each).
+template <int NP>
+struct TetWalkLdsN {
+  int cls[NP];
   int level;
-  int digits;  // 2 bits per depth 1..4
+  int digits;
 };
+using TetWalkLds = TetWalkLdsN<4>;
 
-template <class Sink>
-__device__ __forceinline__ bool walk_step_lds(TetWalkLds& W, const double* A, double* stk,
+// One step of the walk over NP planes with the depth 1..NP-1 tets in LDS;
+// lane stride LANES between consecutive words of one lane.
+template <int NP, int LANES, class Sink>
+__device__ __forceinline__ bool walk_step_lds(TetWalkLdsN<NP>& W, const double* A, double* stk,
                                               const double* planes, double tol, Sink&& sink) {
   const int k = W.level;
   const int ci = (W.digits >> (2 * k)) & 3;
   int cls = in_reg_i(W.cls[0]);
 #pragma unroll
-  for (int j = 1; j < 4; j++) cls = k == j + 1 ? in_reg_i(W.cls[j]) : cls;
+  for (int j = 1; j < NP; j++) cls = k == j + 1 ? in_reg_i(W.cls[j]) : cls;
   if (ci >= class_nchild(cls)) {
     if (k == 1) return false;
     W.level = k - 1;
     return true;
   }
   double parent[12], pd[4];
-  const double* src = stk + (k >= 2 ? k - 2 : 0) * 12 * kWalkBlock;
+  const double* src = stk + (k >= 2 ? k - 2 : 0) * 12 * LANES;
 #pragma unroll
   for (int i = 0; i < 12; i++) {
-    const double v = src[i * kWalkBlock];
+    const double v = src[i * LANES];
     parent[i] = k == 1 ? in_reg(A[i]) : v;
   }
   walk_distances(parent, planes + 4 * (k - 1), pd);
   double child[12];
   build_child(parent, pd, make_recipe(cls, ci), child);
-  if (k == 4) {
-    W.digits += 1 << 8;
+  if (k == NP) {
+    W.digits += 1 << (2 * NP);
     const double vol = tet_volume_tet(child);
     if (vol >= tol) sink(child, vol);
     return true;
@@ -381,11 +388,11 @@ __device__ __forceinline__ bool walk_step_lds(TetWalkLds& W, const double* A, do
   double dc[4];
   walk_distances(child, planes + 4 * k, dc);
   const int cc = pack_class(dc);
-  double* dst = stk + (k - 1) * 12 * kWalkBlock;
+  double* dst = stk + (k - 1) * 12 * LANES;
 #pragma unroll
-  for (int i = 0; i < 12; i++) dst[i * kWalkBlock] = child[i];
+  for (int i = 0; i < 12; i++) dst[i * LANES] = child[i];
 #pragma unroll
-  for (int j = 1; j < 4; j++) W.cls[j] = k == j ? cc : W.cls[j];
+  for (int j = 1; j < NP; j++) W.cls[j] = k == j ? cc : W.cls[j];
   W.digits = (W.digits + (1 << (2 * k))) & ~(3 << (2 * (k + 1)));
   W.level = k + 1;
   return true;
@@ -455,7 +462,7 @@ __global__ void __launch_bounds__(kWalkBlock) k_clip_tet_walk(ClipArgs g) {
       }
       active = true;
     }
-    const bool more = walk_step_lds(W, A, stk, planes, tol, [&](const double* t, double) {
+    const bool more = walk_step_lds<4, kWalkBlock>(W, A, stk, planes, tol, [&](const double* t, double) {
       const double vr = tet_volume_real(t);
       m = m + vr;
       if (MASS) mass_element<3>(t, vr, affa, affb, blk);
@@ -586,13 +593,8 @@ struct MixedArgs {
   double* elements;
 };
 
-// The five tets of a hexahedron in Gmsh order: vertices 3678, 1348, 1568,
-// 1236, 1368 (1-based), intersect_hexes (Supermesh.F90:318-332).
-__constant__ int kHexSplit[5][4] = {{2, 5, 6, 7}, {0, 2, 3, 7}, {0, 4, 5, 7}, {0, 1, 2, 5}, {0, 2, 5, 7}};
-
-template <int DIM, int LA, int LB, bool FILL>
+template <int LA, int LB, bool FILL>
 __global__ void __launch_bounds__(kClipBlock) k_clip_mixed(MixedArgs g) {
-  constexpr int ESZ = (DIM + 1) * DIM;  // output simplex
   const long stride = (long)gridDim.x * blockDim.x;
   for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < g.npairs; p += stride) {
     long off = 0;
@@ -603,85 +605,202 @@ __global__ void __launch_bounds__(kClipBlock) k_clip_mixed(MixedArgs g) {
       off = g.offsets[p];
     }
     const long a = g.rows[p], b = g.cols[p];
-    double A[LA * DIM], B[LB * DIM];
+    double A[LA * 2], B[LB * 2];
 #pragma unroll
-    for (int k = 0; k < LA * DIM; k++) A[k] = g.acoords[a * LA * DIM + k];
+    for (int k = 0; k < LA * 2; k++) A[k] = g.acoords[a * LA * 2 + k];
 #pragma unroll
-    for (int k = 0; k < LB * DIM; k++) B[k] = g.bcoords[b * LB * DIM + k];
+    for (int k = 0; k < LB * 2; k++) B[k] = g.bcoords[b * LB * 2 + k];
     double m = 0.0;
     int k = 0;
-    auto put = [&](const double* t) {
+    auto sink = [&](double x0, double y0, double x1, double y1, double x2, double y2, double ar) {
+      m = m + ar;
       if (FILL && k < cap) {
-        double* o = g.elements + (off + k) * ESZ;
-#pragma unroll
-        for (int i = 0; i < ESZ; i++) o[i] = t[i];
+        double* o = g.elements + (off + k) * 6;
+        o[0] = x0;
+        o[1] = y0;
+        o[2] = x1;
+        o[3] = y1;
+        o[4] = x2;
+        o[5] = y2;
       }
       k++;
     };
-    if constexpr (DIM == 2) {
-      auto tri_sink = [&](double x0, double y0, double x1, double y1, double x2, double y2, double ar) {
-        const double t[6] = {x0, y0, x1, y1, x2, y2};
-        m = m + ar;
-        put(t);
-      };
-      Line L[4];
-      int n = 0;
-      if constexpr (LA == 3 && LB == 4) {
-        quad_lines(B, L);
-        n = clip_poly<4, 3>(A, L, 10.0 * fspacing(triangle_area(A)), tri_sink);
-      } else if constexpr (LA == 4 && LB == 3) {
-        quad_lines(A, L);
-        n = clip_poly<4, 3>(B, L, 10.0 * fspacing(triangle_area(B)), tri_sink);
-      } else {
-        quad_lines(B, L);
-        n = clip_poly<4, 4>(A, L, 10.0 * 2.2250738585072014e-308, tri_sink);
-      }
-      if (n == 0) {
-        k = 0;
-        m = 0.0;
-      }
+    Line L[4];
+    int n = 0;
+    if constexpr (LA == 3 && LB == 4) {
+      quad_lines(B, L);
+      n = clip_poly<4, 3>(A, L, 10.0 * fspacing(triangle_area(A)), sink);
+    } else if constexpr (LA == 4 && LB == 3) {
+      quad_lines(A, L);
+      n = clip_poly<4, 3>(B, L, 10.0 * fspacing(triangle_area(B)), sink);
     } else {
-      auto tet_sink = [&](const double* t, double) {
-        m = m + tet_volume_real(t);
-        put(t);
-      };
-      if constexpr (LA == 4 && LB == 8) {
-        clip_tet_hex(A, B, tet_sink);
-      } else if constexpr (LA == 8 && LB == 4) {
-        clip_tet_hex(B, A, tet_sink);
-      } else {
-        for (int s = 0; s < 5; s++) {
-          double t[12];
-#pragma unroll
-          for (int v = 0; v < 4; v++) {
-            const int hv = kHexSplit[s][v];
-            t[3 * v] = A[3 * hv];
-            t[3 * v + 1] = A[3 * hv + 1];
-            t[3 * v + 2] = A[3 * hv + 2];
-          }
-          clip_tet_hex(t, B, tet_sink);
-        }
-      }
+      quad_lines(B, L);
+      n = clip_poly<4, 4>(A, L, 10.0 * 2.2250738585072014e-308, sink);
     }
     if (FILL) continue;
-    g.counts[p] = k;
-    g.measure[p] = k == 0 ? 0.0 : m;
+    // a pass left with fewer than 3 points: no output, whatever was fanned
+    g.counts[p] = n == 0 ? 0 : k;
+    g.measure[p] = n == 0 ? 0.0 : m;
   }
 }
 
-template <bool FILL>
-static void launch_mixed(Context& C, int dim, int la, int lb, const MixedArgs& g) {
-  const dim3 grid(kClipGrid), block(kClipBlock);
-  if (dim == 2) {
-    if (la == 3 && lb == 4) hipLaunchKernelGGL((k_clip_mixed<2, 3, 4, FILL>), grid, block, 0, C.stream, g);
-    else if (la == 4 && lb == 3) hipLaunchKernelGGL((k_clip_mixed<2, 4, 3, FILL>), grid, block, 0, C.stream, g);
-    else hipLaunchKernelGGL((k_clip_mixed<2, 4, 4, FILL>), grid, block, 0, C.stream, g);
-  } else {
-    if (la == 4 && lb == 8) hipLaunchKernelGGL((k_clip_mixed<3, 4, 8, FILL>), grid, block, 0, C.stream, g);
-    else if (la == 8 && lb == 4) hipLaunchKernelGGL((k_clip_mixed<3, 8, 4, FILL>), grid, block, 0, C.stream, g);
-    else hipLaunchKernelGGL((k_clip_mixed<3, 8, 8, FILL>), grid, block, 0, C.stream, g);
+// 3D: one pair per lane at a time over the six planes of the hexahedron
+// (walk_step_lds<6>, depths 1..5 in LDS); a lane moves on to its next pair
+// as soon as its walk ends.  MODE 0: tet A x hex B; 1: hex A x tet B (the
+// tet is B's); 2: hex A x hex B, A's five tets in turn, outputs and measure
+// accumulated over them in order.  hplanes: planes of the hexahedra,
+// [n][6][4] (hex_planes).
+struct HexWalkArgs {
+  long npairs;
+  const int* rows;
+  const int* cols;
+  const double* acoords;
+  const double* bcoords;
+  const double* hplanes;
+  int* counts;
+  double* measure;
+  const long* offsets;
+  double* elements;
+};
+
+// Split tet s of a hexahedron (intersect_hexes, Supermesh.F90:318-332):
+// vertex ids 2567, 0237, 0457, 0125, 0257, one nibble each.
+__device__ __forceinline__ void hex_split_tet(const double* hex, int s, double* T) {
+  const unsigned code = s == 0 ? 0x7652u : s == 1 ? 0x7320u : s == 2 ? 0x7540u : s == 3 ? 0x5210u : 0x7520u;
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const int hv = (code >> (4 * v)) & 15;
+    T[3 * v] = hex[3 * hv];
+    T[3 * v + 1] = hex[3 * hv + 1];
+    T[3 * v + 2] = hex[3 * hv + 2];
   }
+}
+
+// One wave per block: 30 KB of LDS stack per block, five blocks per CU.
+constexpr int kHexBlock = 64;
+
+template <int MODE, bool FILL>
+__global__ void __launch_bounds__(kHexBlock) k_clip_hex_walk(HexWalkArgs g) {
+  __shared__ double stack[5 * 12 * kHexBlock];
+  double* stk = stack + threadIdx.x;
+  const long n = g.npairs;
+  const long stride = (long)gridDim.x * blockDim.x;
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long off = 0, a = 0;
+  int cap = 0, k = 0, sub = 0;
+  double T[12], tol = 0.0, m = 0.0;
+  const double* planes = nullptr;
+  TetWalkLdsN<6> W;
+  bool active = false;
+  auto start = [&]() {
+    tol = 10.0 * fspacing(tet_volume_tet(T));
+    double d0[4];
+    walk_distances(T, planes, d0);
+    W.cls[0] = pack_class(d0);
+    W.level = 1;
+    W.digits = 0;
+  };
+  while (true) {
+    if (!active) {
+      while (p < n) {
+        if (!FILL) break;
+        cap = g.counts[p];
+        if (cap > 0) break;
+        p += stride;
+      }
+      if (p >= n) break;
+      if (FILL) off = g.offsets[p];
+      a = g.rows[p];
+      const long b = g.cols[p];
+      if (MODE == 0) {
+#pragma unroll
+        for (int i = 0; i < 12; i++) T[i] = g.acoords[a * 12 + i];
+        planes = g.hplanes + b * 24;
+      } else if (MODE == 1) {
+#pragma unroll
+        for (int i = 0; i < 12; i++) T[i] = g.bcoords[b * 12 + i];
+        planes = g.hplanes + a * 24;
+      } else {
+        sub = 0;
+        hex_split_tet(g.acoords + a * 24, 0, T);
+        planes = g.hplanes + b * 24;
+      }
+      m = 0.0;
+      k = 0;
+      start();
+      active = true;
+    }
+    const bool more = walk_step_lds<6, kHexBlock>(W, T, stk, planes, tol, [&](const double* t, double) {
+      m = m + tet_volume_real(t);
+      if (FILL && k < cap) {
+        double* o = g.elements + (off + k) * 12;
+#pragma unroll
+        for (int i = 0; i < 12; i++) o[i] = t[i];
+      }
+      k++;
+    });
+    if (more) continue;
+    if (MODE == 2 && sub < 4) {
+      sub++;
+      hex_split_tet(g.acoords + a * 24, sub, T);
+      start();
+      continue;
+    }
+    if (!FILL) {
+      g.counts[p] = k;
+      g.measure[p] = k == 0 ? 0.0 : m;
+    }
+    p += stride;
+    active = false;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_hex_planes(long n, const double* __restrict__ coords,
+                                                    double* __restrict__ planes) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  double h[24];
+#pragma unroll
+  for (int i = 0; i < 24; i++) h[i] = coords[e * 24 + i];
+  double pl[24];
+  hex_planes(h, pl);
+#pragma unroll
+  for (int i = 0; i < 24; i++) planes[e * 24 + i] = pl[i];
+}
+
+template <bool FILL>
+static void launch_mixed(Context& C, int la, int lb, const MixedArgs& g) {
+  const dim3 grid(kClipGrid), block(kClipBlock);
+  if (la == 3 && lb == 4) hipLaunchKernelGGL((k_clip_mixed<3, 4, FILL>), grid, block, 0, C.stream, g);
+  else if (la == 4 && lb == 3) hipLaunchKernelGGL((k_clip_mixed<4, 3, FILL>), grid, block, 0, C.stream, g);
+  else hipLaunchKernelGGL((k_clip_mixed<4, 4, FILL>), grid, block, 0, C.stream, g);
   LSM_HIP(hipGetLastError());
+}
+
+// Persistent lanes for the hexahedron walk: one full wave of resident blocks.
+template <int MODE, bool FILL>
+static void launch_hex_walk_as(Context& C, const HexWalkArgs& g) {
+  static int blocks_per_cu = 0;
+  if (blocks_per_cu == 0) {
+    int nb = 0;
+    LSM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &nb, reinterpret_cast<const void*>(k_clip_hex_walk<MODE, FILL>), kHexBlock, 0));
+    blocks_per_cu = nb > 0 ? nb : 1;
+  }
+  int cus = 0;
+  LSM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, C.device));
+  const long lanes = (long)blocks_per_cu * (cus > 0 ? cus : 256) * kHexBlock;
+  const long need = (g.npairs + kHexBlock - 1) / kHexBlock;
+  const unsigned blocks = (unsigned)std::max(1L, std::min(lanes / kHexBlock, need));
+  hipLaunchKernelGGL((k_clip_hex_walk<MODE, FILL>), dim3(blocks), dim3(kHexBlock), 0, C.stream, g);
+  LSM_HIP(hipGetLastError());
+}
+
+template <bool FILL>
+static void launch_hex_walk(Context& C, int la, int lb, const HexWalkArgs& g) {
+  if (la == 4 && lb == 8) launch_hex_walk_as<0, FILL>(C, g);
+  else if (la == 8 && lb == 4) launch_hex_walk_as<1, FILL>(C, g);
+  else launch_hex_walk_as<2, FILL>(C, g);
 }
 
 void clip_pairs_mixed(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int flags,
@@ -695,7 +814,28 @@ void clip_pairs_mixed(Context& C, const DevMesh& A, const DevMesh& B, long npair
   g.bcoords = static_cast<const double*>(B.coords.ptr);
   g.counts = C.counts.as<int>(npairs);
   g.measure = C.measure.as<double>(npairs);
-  if (npairs > 0) launch_mixed<false>(C, dim, A.loc, B.loc, g);
+  HexWalkArgs h = {};
+  if (dim == 3) {
+    // planes of the hexahedra (B's, or A's when only A is one)
+    DevMesh& H = const_cast<DevMesh&>(A.loc == 8 && B.loc == 4 ? A : B);
+    h.npairs = npairs;
+    h.rows = g.rows;
+    h.cols = g.cols;
+    h.acoords = g.acoords;
+    h.bcoords = g.bcoords;
+    h.counts = g.counts;
+    h.measure = g.measure;
+    double* hp = H.planes.as<double>((size_t)(H.nelements > 0 ? H.nelements : 1) * 24);
+    h.hplanes = hp;
+    if (H.nelements > 0)
+      hipLaunchKernelGGL(k_hex_planes, dim3(grid_for(H.nelements, 256)), dim3(256), 0, C.stream, H.nelements,
+                         static_cast<const double*>(H.coords.ptr), hp);
+    LSM_HIP(hipGetLastError());
+  }
+  if (npairs > 0) {
+    if (dim == 2) launch_mixed<false>(C, A.loc, B.loc, g);
+    else launch_hex_walk<false>(C, A.loc, B.loc, h);
+  }
   Partial* parts = C.partials.as<Partial>(kClipGrid);
   hipLaunchKernelGGL(k_reduce_pairs, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, npairs, g.counts, g.measure,
                      parts);
@@ -729,7 +869,13 @@ void clip_pairs_mixed(Context& C, const DevMesh& A, const DevMesh& B, long npair
                                       C.stream));
       g.offsets = offs;
       g.elements = C.elements.as<double>((size_t)(out->nout > 0 ? out->nout : 1) * esz);
-      launch_mixed<true>(C, dim, A.loc, B.loc, g);
+      if (dim == 2) {
+        launch_mixed<true>(C, A.loc, B.loc, g);
+      } else {
+        h.offsets = offs;
+        h.elements = g.elements;
+        launch_hex_walk<true>(C, A.loc, B.loc, h);
+      }
     }
     LSM_HIP(hipStreamSynchronize(C.stream));
     out->offsets = offs;
