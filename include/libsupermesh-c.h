@@ -80,6 +80,38 @@ void libsupermesh_intersect_tet_hex(c_kind* tet_a, c_kind* hex_b, c_kind* tets_c
  * Replaces src/C_bindings.F90:54-64 (include/libsupermesh-c.h:31). */
 void libsupermesh_intersect_tri_quad(c_kind* tri_a, c_kind* quad_b, c_kind* tris_c, int* n_tris_c);
 
+/* The advancing front finder, in the tree finder's set_input / query_output /
+ * get_output form.  The reference has it in Fortran only
+ * (advancing_front_intersection_finder, src/Intersection_Finder.F90:138-141,
+ * 270-454, 456-476); these are its C form, and the Fortran module's
+ * advancing_front_intersection_finder calls them. */
+void libsupermesh_advancing_front_intersection_finder_set_input(long* nnodes_a, int* dim_a, long* nelements_a, int* loc_a, long* nnodes_b, int* dim_b, long* nelements_b, int* loc_b, c_kind* positions_a, long* enlist_a, c_kind* positions_b, long* enlist_b);
+void libsupermesh_advancing_front_intersection_finder_query_output(long* nindices);
+void libsupermesh_advancing_front_intersection_finder_get_output(long* nelements, long* nindices, long* indices, long* ind_ptr);
+
+/* Mesh adjacency, exported by the reference's C_bindings.F90 but not declared
+ * in its header.  All ids 0-based, CSR index pointers starting at 0.
+ * node -> elements, ascending element order, one entry per occurrence:
+ * replaces src/C_bindings.F90:176-213 (mesh_nelist, src/Graphs.F90:76-116). */
+void libsupermesh_mesh_nelist_set_input(long* nnodes, long* nelements, long* loc, long* enlist);
+void libsupermesh_mesh_nelist_query_output(long* nindices);
+void libsupermesh_mesh_nelist_get_output(long* nnodes, long* nindices, long* indices, long* ind_ptr);
+/* node -> nodes sharing an element, ascending: replaces src/C_bindings.F90:215-295. */
+void libsupermesh_mesh_nnlist_set_input(long* nnodes, long* nelements, long* loc, long* enlist);
+void libsupermesh_mesh_nnlist_query_output(long* nindices);
+void libsupermesh_mesh_nnlist_get_output(long* nnodes, long* nindices, long* indices, long* ind_ptr);
+/* element -> elements sharing sloc nodes, in the reference's discovery order,
+ * at most loc per element: replaces src/C_bindings.F90:297-344
+ * (mesh_eelist, src/Graphs.F90:118-187). */
+void libsupermesh_mesh_eelist_set_input(long* nnodes, long* nelements, long* loc, long* sloc, long* enlist);
+void libsupermesh_mesh_eelist_query_output(long* nindices);
+void libsupermesh_mesh_eelist_get_output(long* nelements, long* nindices, long* indices, long* ind_ptr);
+/* CSR transpose (rows of each column in ascending order): replaces
+ * src/C_bindings.F90:397-430.  indices_t gets ind_ptr[nnodes] - ind_ptr[0]
+ * entries. */
+void libsupermesh_transpose_sparsity(long* nnodes, long* nnodes_t, long* nindices, long* indices, long* ind_ptr,
+                                     long* indices_t, long* ind_ptr_t);
+
 /* Error of the last call on this thread: text ("" if none) and code
  * (0 = ok; see libsupermesh_gpu.h).  No counterpart in the reference, which
  * aborts instead. */

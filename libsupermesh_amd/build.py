@@ -19,7 +19,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIBDIR = os.path.join(PKG, "lib")
 OBJDIR = os.path.join(LIBDIR, "obj")
 LIBNAME = "libsupermesh_gpu.so"
-SOURCES = ["finder.hip", "supermesh.hip", "predictor.hip", "capi.hip", "interval.cpp"]
+SOURCES = ["finder.hip", "supermesh.hip", "predictor.hip", "graphs.hip", "capi.hip", "interval.cpp", "advancing_front.cpp"]
 ARCH = os.environ.get("LIBSUPERMESH_GPU_ARCH", "gfx950")
 
 FLAGS = [

@@ -382,7 +382,48 @@ struct FinderState {
   std::vector<int> indices;
 };
 
-static FinderState g_tree, g_rtree, g_sort;
+static FinderState g_tree, g_rtree, g_sort, g_front;
+
+// State of the mesh_*list set_input / query_output / get_output calls
+// (C_bindings.F90:176-344): the CSR computed by set_input, freed by
+// get_output.
+struct GraphState {
+  std::mutex mu;
+  bool ready = false;
+  std::vector<long> indptr, indices;
+};
+
+static GraphState g_nelist, g_nnlist, g_eelist;
+
+template <class F>
+static void graph_set_input(GraphState& S, F&& compute) {
+  std::lock_guard<std::mutex> lk(S.mu);
+  S.ready = false;
+  S.indptr.clear();
+  S.indices.clear();
+  Context& C = *default_context();
+  std::lock_guard<std::mutex> lk2(C.mu);
+  compute(C, S.indptr, S.indices);
+  S.ready = true;
+}
+
+static void graph_query_output(GraphState& S, long* nindices) {
+  std::lock_guard<std::mutex> lk(S.mu);
+  LSM_REQUIRE(S.ready, LIBSUPERMESH_GPU_ERROR_STATE, "query_output called before set_input");
+  *nindices = (long)S.indices.size();
+}
+
+static void graph_get_output(GraphState& S, long nrows, long nindices, long* indices, long* ind_ptr) {
+  std::lock_guard<std::mutex> lk(S.mu);
+  LSM_REQUIRE(S.ready, LIBSUPERMESH_GPU_ERROR_STATE, "get_output called before set_input");
+  LSM_REQUIRE(nrows + 1 == (long)S.indptr.size() && nindices == (long)S.indices.size(),
+              LIBSUPERMESH_GPU_ERROR_INVALID, "get_output sizes differ from set_input's result");
+  std::memcpy(ind_ptr, S.indptr.data(), sizeof(long) * S.indptr.size());
+  if (nindices > 0) std::memcpy(indices, S.indices.data(), sizeof(long) * nindices);
+  S.ready = false;
+  std::vector<long>().swap(S.indptr);
+  std::vector<long>().swap(S.indices);
+}
 
 static void host_mesh_to_device(Context& C, DevMesh& M, long nnodes, int dim, long ne, int loc,
                                 const double* pos, const long* enl, libsupermesh_gpu_mesh& view) {
@@ -894,6 +935,92 @@ void libsupermesh_sort_intersection_finder_query_output(long* nindices) {
 void libsupermesh_sort_intersection_finder_get_output(long* nelements, long* nindices, long* indices,
                                                       long* ind_ptr) {
   guarded([&] { finder_get_output(g_sort, *nelements, *nindices, indices, ind_ptr); });
+}
+
+void libsupermesh_advancing_front_intersection_finder_set_input(
+    long* nnodes_a, int* dim_a, long* nelements_a, int* loc_a, long* nnodes_b, int* dim_b, long* nelements_b,
+    int* loc_b, double* positions_a, long* enlist_a, double* positions_b, long* enlist_b) {
+  guarded([&] {
+    std::lock_guard<std::mutex> lk(g_front.mu);
+    g_front.ready = false;
+    g_front.indptr.clear();
+    g_front.indices.clear();
+    LSM_REQUIRE(*dim_a == *dim_b, LIBSUPERMESH_GPU_ERROR_INVALID, "Invalid dimension");
+    Context& C = *default_context();
+    std::lock_guard<std::mutex> lk2(C.mu);
+    LSM_HIP(hipSetDevice(C.device));
+    lsm::advancing_front_finder(C, *dim_a, *nnodes_a, *nelements_a, *loc_a, positions_a, enlist_a, *nnodes_b,
+                                *nelements_b, *loc_b, positions_b, enlist_b, g_front.indptr, g_front.indices);
+    g_front.nelements_a = *nelements_a;
+    g_front.ready = true;
+  });
+}
+
+void libsupermesh_advancing_front_intersection_finder_query_output(long* nindices) {
+  guarded([&] { finder_query_output(g_front, nindices); });
+}
+
+void libsupermesh_advancing_front_intersection_finder_get_output(long* nelements, long* nindices, long* indices,
+                                                                 long* ind_ptr) {
+  guarded([&] { finder_get_output(g_front, *nelements, *nindices, indices, ind_ptr); });
+}
+
+void libsupermesh_mesh_nelist_set_input(long* nnodes, long* nelements, long* loc, long* enlist) {
+  guarded([&] {
+    graph_set_input(g_nelist, [&](Context& C, std::vector<long>& ptr, std::vector<long>& idx) {
+      lsm::mesh_nelist(C, *nnodes, *nelements, *loc, enlist, ptr, idx);
+    });
+  });
+}
+
+void libsupermesh_mesh_nelist_query_output(long* nindices) {
+  guarded([&] { graph_query_output(g_nelist, nindices); });
+}
+
+void libsupermesh_mesh_nelist_get_output(long* nnodes, long* nindices, long* indices, long* ind_ptr) {
+  guarded([&] { graph_get_output(g_nelist, *nnodes, *nindices, indices, ind_ptr); });
+}
+
+void libsupermesh_mesh_nnlist_set_input(long* nnodes, long* nelements, long* loc, long* enlist) {
+  guarded([&] {
+    graph_set_input(g_nnlist, [&](Context& C, std::vector<long>& ptr, std::vector<long>& idx) {
+      lsm::mesh_nnlist(C, *nnodes, *nelements, *loc, enlist, ptr, idx);
+    });
+  });
+}
+
+void libsupermesh_mesh_nnlist_query_output(long* nindices) {
+  guarded([&] { graph_query_output(g_nnlist, nindices); });
+}
+
+void libsupermesh_mesh_nnlist_get_output(long* nnodes, long* nindices, long* indices, long* ind_ptr) {
+  guarded([&] { graph_get_output(g_nnlist, *nnodes, *nindices, indices, ind_ptr); });
+}
+
+void libsupermesh_mesh_eelist_set_input(long* nnodes, long* nelements, long* loc, long* sloc, long* enlist) {
+  guarded([&] {
+    // mesh_eelist without nneigh_ele: room for loc neighbours per element
+    graph_set_input(g_eelist, [&](Context& C, std::vector<long>& ptr, std::vector<long>& idx) {
+      lsm::mesh_eelist(C, *nnodes, *nelements, *loc, *sloc, *loc, enlist, ptr, idx);
+    });
+  });
+}
+
+void libsupermesh_mesh_eelist_query_output(long* nindices) {
+  guarded([&] { graph_query_output(g_eelist, nindices); });
+}
+
+void libsupermesh_mesh_eelist_get_output(long* nelements, long* nindices, long* indices, long* ind_ptr) {
+  guarded([&] { graph_get_output(g_eelist, *nelements, *nindices, indices, ind_ptr); });
+}
+
+void libsupermesh_transpose_sparsity(long* nnodes, long* nnodes_t, long* nindices, long* indices, long* ind_ptr,
+                                     long* indices_t, long* ind_ptr_t) {
+  guarded([&] {
+    Context& C = *default_context();
+    std::lock_guard<std::mutex> lk(C.mu);
+    lsm::transpose_sparsity(C, *nnodes, *nnodes_t, *nindices, indices, ind_ptr, indices_t, ind_ptr_t);
+  });
 }
 
 }  // extern "C"

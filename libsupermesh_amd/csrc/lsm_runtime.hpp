@@ -166,6 +166,21 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
                 libsupermesh_gpu_result* out);
 void assemble_mass(Context& C, const DevMesh& A, const DevMesh& B, long npairs,
                    libsupermesh_gpu_result* out);
+// Mesh adjacency (graphs.hip): host element-node lists in, host CSR out.
+void mesh_nelist(Context& C, long nnodes, long ne, long loc, const long* enlist, std::vector<long>& indptr,
+                 std::vector<long>& indices);
+void mesh_nnlist(Context& C, long nnodes, long ne, long loc, const long* enlist, std::vector<long>& indptr,
+                 std::vector<long>& indices);
+void mesh_eelist(Context& C, long nnodes, long ne, long loc, long sloc, long nneigh, const long* enlist,
+                 std::vector<long>& indptr, std::vector<long>& indices);
+void transpose_sparsity(Context& C, long nrows, long ncols, long nindices, const long* indices, const long* ind_ptr,
+                        long* indices_t, long* ind_ptr_t);
+
+// The advancing front finder (advancing_front.cpp): host arrays, 0-based.
+void advancing_front_finder(Context& C, int dim, long nn_a, long ne_a, int loc_a, const double* pos_a,
+                            const long* enl_a, long nn_b, long ne_b, int loc_b, const double* pos_b,
+                            const long* enl_b, std::vector<long>& indptr, std::vector<int>& indices);
+
 // Clipping of candidate pairs where A or B is not a simplex mesh (quads in
 // 2D, hexahedra in 3D).
 void clip_pairs_mixed(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int flags,

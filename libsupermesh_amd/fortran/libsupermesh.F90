@@ -25,7 +25,8 @@ module libsupermesh
   integer, parameter, public :: interval_buf_size = 1
 
   public :: tree_intersection_finder, rtree_intersection_finder, &
-    & sort_intersection_finder, intersect_tris, intersect_tets, &
+    & sort_intersection_finder, advancing_front_intersection_finder, &
+    & intersection_finder, intersect_tris, intersect_tets, &
     & triangle_area, tetrahedron_volume, intersect_intervals, interval_size, &
     & intersect_tri_quad, intersect_tet_hex, supermesh, libsupermesh_gpu_error
 
@@ -74,6 +75,17 @@ module libsupermesh
     module procedure sort_intersection_finder_csr_sparsity
   end interface sort_intersection_finder
 
+  interface advancing_front_intersection_finder
+    module procedure advancing_front_intersection_finder_csr_sparsity
+  end interface advancing_front_intersection_finder
+
+  interface intersection_finder
+    module procedure intersection_finder_csr_sparsity
+  end interface intersection_finder
+
+  ! finders behind finder_csr
+  integer, parameter :: finder_tree = 0, finder_rtree = 1, finder_sort = 2, finder_front = 3
+
   interface
     subroutine c_tree_set_input(nnodes_a, dim_a, nelements_a, loc_a, nnodes_b, dim_b, &
         & nelements_b, loc_b, positions_a, enlist_a, positions_b, enlist_b) &
@@ -115,6 +127,26 @@ module libsupermesh
       integer(kind = c_long) :: nelements, nindices
       integer(kind = c_long), dimension(*) :: indices, ind_ptr
     end subroutine c_rtree_get_output
+    subroutine c_front_set_input(nnodes_a, dim_a, nelements_a, loc_a, nnodes_b, dim_b, &
+        & nelements_b, loc_b, positions_a, enlist_a, positions_b, enlist_b) &
+        & bind(c, name = "libsupermesh_advancing_front_intersection_finder_set_input")
+      import :: c_long, c_int, c_double
+      integer(kind = c_long) :: nnodes_a, nelements_a, nnodes_b, nelements_b
+      integer(kind = c_int) :: dim_a, loc_a, dim_b, loc_b
+      real(kind = c_double), dimension(*) :: positions_a, positions_b
+      integer(kind = c_long), dimension(*) :: enlist_a, enlist_b
+    end subroutine c_front_set_input
+    subroutine c_front_query_output(nindices) &
+        & bind(c, name = "libsupermesh_advancing_front_intersection_finder_query_output")
+      import :: c_long
+      integer(kind = c_long) :: nindices
+    end subroutine c_front_query_output
+    subroutine c_front_get_output(nelements, nindices, indices, ind_ptr) &
+        & bind(c, name = "libsupermesh_advancing_front_intersection_finder_get_output")
+      import :: c_long
+      integer(kind = c_long) :: nelements, nindices
+      integer(kind = c_long), dimension(*) :: indices, ind_ptr
+    end subroutine c_front_get_output
     subroutine c_intersect_tris_real(tri_a, tri_b, tris_c, n_tris_c) &
         & bind(c, name = "libsupermesh_intersect_tris_real")
       import :: c_double, c_int
@@ -254,7 +286,7 @@ contains
     integer, dimension(:, :), intent(in) :: enlist_a, enlist_b
     integer, dimension(:), allocatable, intent(out) :: map_ab_indices
     integer, dimension(:), intent(out) :: map_ab_indptr
-    call finder_csr(.false., positions_a, enlist_a, positions_b, enlist_b, map_ab_indices, &
+    call finder_csr(finder_tree, positions_a, enlist_a, positions_b, enlist_b, map_ab_indices, &
       & map_ab_indptr)
   end subroutine tree_intersection_finder_csr_sparsity
 
@@ -265,13 +297,41 @@ contains
     integer, dimension(:, :), intent(in) :: enlist_a, enlist_b
     integer, dimension(:), allocatable, intent(out) :: map_ab_indices
     integer, dimension(:), intent(out) :: map_ab_indptr
-    call finder_csr(.true., positions_a, enlist_a, positions_b, enlist_b, map_ab_indices, &
+    call finder_csr(finder_rtree, positions_a, enlist_a, positions_b, enlist_b, map_ab_indices, &
       & map_ab_indptr)
   end subroutine rtree_intersection_finder_csr_sparsity
 
-  subroutine finder_csr(closed, positions_a, enlist_a, positions_b, enlist_b, map_ab_indices, &
+  ! CSR candidate pairs of the advancing front finder
+  ! (Intersection_Finder.F90:270-476): closed overlap, rows in the front's
+  ! visiting order.
+  subroutine advancing_front_intersection_finder_csr_sparsity(positions_a, enlist_a, positions_b, &
+      & enlist_b, map_ab_indices, map_ab_indptr)
+    real(kind = real_kind), dimension(:, :), intent(in) :: positions_a, positions_b
+    integer, dimension(:, :), intent(in) :: enlist_a, enlist_b
+    integer, dimension(:), allocatable, intent(out) :: map_ab_indices
+    integer, dimension(:), intent(out) :: map_ab_indptr
+    call finder_csr(finder_front, positions_a, enlist_a, positions_b, enlist_b, map_ab_indices, &
       & map_ab_indptr)
-    logical, intent(in) :: closed
+  end subroutine advancing_front_intersection_finder_csr_sparsity
+
+  ! The default finder by dimension (Intersection_Finder.F90:234-262): the
+  ! sort finder in 1D, the R-tree finder in 2D and 3D.
+  subroutine intersection_finder_csr_sparsity(positions_a, enlist_a, positions_b, enlist_b, &
+      & map_ab_indices, map_ab_indptr)
+    real(kind = real_kind), dimension(:, :), intent(in) :: positions_a, positions_b
+    integer, dimension(:, :), intent(in) :: enlist_a, enlist_b
+    integer, dimension(:), allocatable, intent(out) :: map_ab_indices
+    integer, dimension(:), intent(out) :: map_ab_indptr
+    integer :: kind
+    kind = finder_rtree
+    if(size(positions_a, 1) == 1) kind = finder_sort
+    call finder_csr(kind, positions_a, enlist_a, positions_b, enlist_b, map_ab_indices, &
+      & map_ab_indptr)
+  end subroutine intersection_finder_csr_sparsity
+
+  subroutine finder_csr(kind, positions_a, enlist_a, positions_b, enlist_b, map_ab_indices, &
+      & map_ab_indptr)
+    integer, intent(in) :: kind
     real(kind = real_kind), dimension(:, :), intent(in) :: positions_a, positions_b
     integer, dimension(:, :), intent(in) :: enlist_a, enlist_b
     integer, dimension(:), allocatable, intent(out) :: map_ab_indices
@@ -294,27 +354,42 @@ contains
     pb = positions_b
     ea = enlist_a - 1
     eb = enlist_b - 1
-    if(closed) then
-      call c_rtree_set_input(nn_a, dim_a, ne_a, loc_a, nn_b, dim_b, ne_b, loc_b, pa, ea, pb, eb)
-    else
-      call c_tree_set_input(nn_a, dim_a, ne_a, loc_a, nn_b, dim_b, ne_b, loc_b, pa, ea, pb, eb)
-    end if
+    select case(kind)
+      case(finder_tree)
+        call c_tree_set_input(nn_a, dim_a, ne_a, loc_a, nn_b, dim_b, ne_b, loc_b, pa, ea, pb, eb)
+      case(finder_rtree)
+        call c_rtree_set_input(nn_a, dim_a, ne_a, loc_a, nn_b, dim_b, ne_b, loc_b, pa, ea, pb, eb)
+      case(finder_sort)
+        call c_sort_set_input(nn_a, dim_a, ne_a, loc_a, nn_b, dim_b, ne_b, loc_b, pa, ea, pb, eb)
+      case default
+        call c_front_set_input(nn_a, dim_a, ne_a, loc_a, nn_b, dim_b, ne_b, loc_b, pa, ea, pb, eb)
+    end select
     nind = 0
     if(c_last_error_code() == 0) then
-      if(closed) then
-        call c_rtree_query_output(nind)
-      else
-        call c_tree_query_output(nind)
-      end if
+      select case(kind)
+        case(finder_tree)
+          call c_tree_query_output(nind)
+        case(finder_rtree)
+          call c_rtree_query_output(nind)
+        case(finder_sort)
+          call c_sort_query_output(nind)
+        case default
+          call c_front_query_output(nind)
+      end select
     end if
     allocate(ind(max(nind, 1_c_long)), ptr(ne_a + 1))
     ptr = 0
     if(c_last_error_code() == 0) then
-      if(closed) then
-        call c_rtree_get_output(ne_a, nind, ind, ptr)
-      else
-        call c_tree_get_output(ne_a, nind, ind, ptr)
-      end if
+      select case(kind)
+        case(finder_tree)
+          call c_tree_get_output(ne_a, nind, ind, ptr)
+        case(finder_rtree)
+          call c_rtree_get_output(ne_a, nind, ind, ptr)
+        case(finder_sort)
+          call c_sort_get_output(ne_a, nind, ind, ptr)
+        case default
+          call c_front_get_output(ne_a, nind, ind, ptr)
+      end select
     end if
     allocate(map_ab_indices(nind))
     map_ab_indices = int(ind(:nind)) + 1
@@ -328,33 +403,8 @@ contains
     integer, dimension(:, :), intent(in) :: enlist_a, enlist_b
     integer, dimension(:), allocatable, intent(out) :: map_ab_indices
     integer, dimension(:), intent(out) :: map_ab_indptr
-    integer(kind = c_long) :: nn_a, ne_a, nn_b, ne_b, nind
-    integer(kind = c_int) :: dim_a, loc_a, dim_b, loc_b
-    integer(kind = c_long), dimension(:, :), allocatable :: ea, eb
-    integer(kind = c_long), dimension(:), allocatable :: ind, ptr
-    real(kind = c_double), dimension(:, :), allocatable :: pa, pb
-    dim_a = size(positions_a, 1)
-    nn_a = size(positions_a, 2)
-    loc_a = size(enlist_a, 1)
-    ne_a = size(enlist_a, 2)
-    dim_b = size(positions_b, 1)
-    nn_b = size(positions_b, 2)
-    loc_b = size(enlist_b, 1)
-    ne_b = size(enlist_b, 2)
-    allocate(pa(dim_a, nn_a), pb(dim_b, nn_b), ea(loc_a, ne_a), eb(loc_b, ne_b))
-    pa = positions_a
-    pb = positions_b
-    ea = enlist_a - 1
-    eb = enlist_b - 1
-    call c_sort_set_input(nn_a, dim_a, ne_a, loc_a, nn_b, dim_b, ne_b, loc_b, pa, ea, pb, eb)
-    nind = 0
-    if(c_last_error_code() == 0) call c_sort_query_output(nind)
-    allocate(ind(max(nind, 1_c_long)), ptr(ne_a + 1))
-    ptr = 0
-    if(c_last_error_code() == 0) call c_sort_get_output(ne_a, nind, ind, ptr)
-    allocate(map_ab_indices(nind))
-    map_ab_indices = int(ind(:nind)) + 1
-    map_ab_indptr = int(ptr) + 1
+    call finder_csr(finder_sort, positions_a, enlist_a, positions_b, enlist_b, map_ab_indices, &
+      & map_ab_indptr)
   end subroutine sort_intersection_finder_csr_sparsity
 
   ! intersect_intervals (Interval_Intersection.F90:45-70).

@@ -229,6 +229,68 @@ def sort_intersection_finder(pos_a, enl_a, pos_b, enl_b):
     return _finder("libsupermesh_sort_intersection_finder", pos_a, enl_a, pos_b, enl_b)
 
 
+def advancing_front_intersection_finder(pos_a, enl_a, pos_b, enl_b):
+    """Candidate pairs of the reference's advancing front finder (closed
+    overlap, rows in the front's visiting order), as CSR.  The front runs on
+    the host; the face-neighbour lists come from the GPU."""
+    return _finder("libsupermesh_advancing_front_intersection_finder", pos_a, enl_a, pos_b, enl_b)
+
+
+def _graph_call(prefix, nrows_out, *args):
+    L = lib()
+    getattr(L, prefix + "_set_input")(*args)
+    _raise_if_error()
+    nind = c_long(0)
+    getattr(L, prefix + "_query_output")(ctypes.byref(nind))
+    _raise_if_error()
+    indptr = np.zeros(nrows_out + 1, dtype=np.int64)
+    indices = np.zeros(max(nind.value, 1), dtype=np.int64)
+    getattr(L, prefix + "_get_output")(ctypes.byref(c_long(nrows_out)), ctypes.byref(nind), _pl(indices),
+                                       _pl(indptr))
+    _raise_if_error()
+    return indptr, indices[:nind.value]
+
+
+def mesh_nelist(nnodes, enlist):
+    """Node -> element CSR ``(indptr, indices)`` (mesh_nelist), 0-based."""
+    enl = np.ascontiguousarray(enlist, dtype=np.int64)
+    ne, loc = enl.shape
+    return _graph_call("libsupermesh_mesh_nelist", nnodes, ctypes.byref(c_long(nnodes)),
+                       ctypes.byref(c_long(ne)), ctypes.byref(c_long(loc)), _pl(enl))
+
+
+def mesh_nnlist(nnodes, enlist):
+    """Node -> node CSR (nodes sharing an element, ascending), 0-based."""
+    enl = np.ascontiguousarray(enlist, dtype=np.int64)
+    ne, loc = enl.shape
+    return _graph_call("libsupermesh_mesh_nnlist", nnodes, ctypes.byref(c_long(nnodes)),
+                       ctypes.byref(c_long(ne)), ctypes.byref(c_long(loc)), _pl(enl))
+
+
+def mesh_eelist(nnodes, enlist, sloc):
+    """Element -> element CSR (elements sharing ``sloc`` nodes, in the
+    reference's discovery order, at most loc each), 0-based."""
+    enl = np.ascontiguousarray(enlist, dtype=np.int64)
+    ne, loc = enl.shape
+    return _graph_call("libsupermesh_mesh_eelist", ne, ctypes.byref(c_long(nnodes)),
+                       ctypes.byref(c_long(ne)), ctypes.byref(c_long(loc)), ctypes.byref(c_long(sloc)),
+                       _pl(enl))
+
+
+def transpose_sparsity(ncols, indptr, indices):
+    """CSR transpose ``(indptr_t, indices_t)``: rows of each column ascending."""
+    ptr = np.ascontiguousarray(indptr, dtype=np.int64)
+    idx = np.ascontiguousarray(indices, dtype=np.int64)
+    nrows = len(ptr) - 1
+    ptr_t = np.zeros(ncols + 1, dtype=np.int64)
+    idx_t = np.zeros(max(len(idx), 1), dtype=np.int64)
+    lib().libsupermesh_transpose_sparsity(ctypes.byref(c_long(nrows)), ctypes.byref(c_long(ncols)),
+                                          ctypes.byref(c_long(len(idx))), _pl(idx), _pl(ptr), _pl(idx_t),
+                                          _pl(ptr_t))
+    _raise_if_error()
+    return ptr_t, idx_t[:ptr_t[-1]]
+
+
 def _pair_call(fn, a, b, shape):
     a = np.ascontiguousarray(a, dtype=np.float64)
     b = np.ascontiguousarray(b, dtype=np.float64)

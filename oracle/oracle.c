@@ -37,6 +37,11 @@
  *                                 Octree_Intersection_Finder.F90:153-411
  *   orc_mass_pairs,               test_parallel_p1_inner_product_2d.F90:59-68,
  *   orc_inner_product             205-299; ..._3d.F90:61-80,226-337
+ *   orc_mesh_nelist               Graphs.F90:76-116
+ *   orc_mesh_nnlist               C_bindings.F90:215-255
+ *   orc_mesh_eelist               Graphs.F90:118-187 (release build)
+ *   orc_transpose_sparsity        C_bindings.F90:397-430
+ *   orc_advancing_front_finder    Intersection_Finder.F90:270-454
  *
  * What pins each function (tests/test_oracle_*.py):
  *   spacing / epsilon / norm2 / dot: bit-compared with the reference's own
@@ -51,6 +56,14 @@
  *   mass: the reference tests' integral values (8.3333333333333398e-2 in 2D,
  *     0.12499999999999989 in 3D), 1^T M 1 = overlap measure, exactness of the
  *     degree-2 rules on linear fields.
+ *   adjacency: nelist == a stable numpy argsort of the element-node list;
+ *     nnlist == the sorted scipy adjacency of the node graph; eelist: the
+ *     face-neighbour sets of structured grids in closed form, symmetry, and
+ *     the discovery order of hand-worked meshes; transpose == scipy's CSR
+ *     transpose, twice == identity.
+ *   advancing front: on connected meshes every row's set equals the closed
+ *     brute-force set; the row order (the front's visiting order) is pinned
+ *     only by a hand-worked two-mesh example.
  */
 #include "oracle.h"
 
@@ -1374,4 +1387,268 @@ void orc_supermesh(int dim, int finder, int nthreads,
   free(jobs);
   free(bbs);
   orc_tree_free(tree);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Mesh adjacency (Graphs.F90, C_bindings.F90:176-430) and the advancing    */
+/* front finder (Intersection_Finder.F90:270-454).  0-based ids throughout. */
+/* ------------------------------------------------------------------------ */
+
+void orc_free(void *p) { free(p); }
+
+/* mesh_nelist, Graphs.F90:76-116: count per node, then fill in element
+ * order. */
+void orc_mesh_nelist(long nnodes, long ne, int loc, const long *enl, long *indptr, long *indices) {
+  long *n = calloc((size_t)nnodes + 1, sizeof(long));
+  for (long i = 0; i < ne; i++)
+    for (int j = 0; j < loc; j++) n[enl[i * loc + j]]++;
+  indptr[0] = 0;
+  for (long x = 0; x < nnodes; x++) indptr[x + 1] = indptr[x] + n[x];
+  memset(n, 0, sizeof(long) * ((size_t)nnodes + 1));
+  for (long i = 0; i < ne; i++)
+    for (int j = 0; j < loc; j++) {
+      long x = enl[i * loc + j];
+      indices[indptr[x] + n[x]] = i;
+      n[x]++;
+    }
+  free(n);
+}
+
+static int cmp_long(const void *a, const void *b) {
+  long x = *(const long *)a, y = *(const long *)b;
+  return x < y ? -1 : x > y;
+}
+
+/* mesh_nnlist, C_bindings.F90:215-255: per node the set (ascending, no
+ * repeats) of nodes inserted by every local pair i < j of every element, in
+ * both directions.  indices needs room for ne*loc*(loc-1); returns the
+ * number of entries. */
+long orc_mesh_nnlist(long nnodes, long ne, int loc, const long *enl, long *indptr, long *indices) {
+  long *cap = calloc((size_t)nnodes + 1, sizeof(long));
+  for (long e = 0; e < ne; e++)
+    for (int i = 0; i < loc; i++) cap[enl[e * loc + i]] += loc - 1;
+  long *start = malloc(sizeof(long) * ((size_t)nnodes + 1));
+  start[0] = 0;
+  for (long x = 0; x < nnodes; x++) start[x + 1] = start[x] + cap[x];
+  long *buf = malloc(sizeof(long) * (size_t)(start[nnodes] > 0 ? start[nnodes] : 1));
+  memset(cap, 0, sizeof(long) * ((size_t)nnodes + 1));
+  for (long e = 0; e < ne; e++)
+    for (int i = 0; i < loc; i++)
+      for (int j = i + 1; j < loc; j++) {
+        long a = enl[e * loc + i], b = enl[e * loc + j];
+        buf[start[a] + cap[a]++] = b;
+        buf[start[b] + cap[b]++] = a;
+      }
+  long m = 0;
+  indptr[0] = 0;
+  for (long x = 0; x < nnodes; x++) {
+    long *v = buf + start[x];
+    qsort(v, (size_t)cap[x], sizeof(long), cmp_long);
+    for (long k = 0; k < cap[x]; k++)
+      if (k == 0 || v[k] != v[k - 1]) indices[m++] = v[k];
+    indptr[x + 1] = m;
+  }
+  free(buf);
+  free(start);
+  free(cap);
+  return m;
+}
+
+/* mesh_eelist, Graphs.F90:118-187, as the release (NDEBUG) build runs it:
+ * per element, walk its nodes' element lists, count shared nodes per other
+ * element, list an element when its count reaches sloc, stop at nneigh.
+ * indices needs room for ne*nneigh; returns the number of entries. */
+long orc_mesh_eelist(long nnodes, long ne, int loc, long sloc, int nneigh, const long *enl, long *indptr,
+                     long *indices) {
+  long *nptr = malloc(sizeof(long) * ((size_t)nnodes + 1));
+  long *nidx = malloc(sizeof(long) * (size_t)(ne * loc > 0 ? ne * loc : 1));
+  orc_mesh_nelist(nnodes, ne, loc, enl, nptr, nidx);
+  long *seen = calloc((size_t)(ne > 0 ? ne : 1), sizeof(long));
+  long *seen_eles = malloc(sizeof(long) * (size_t)(ne > 0 ? ne : 1));
+  long m = 0;
+  indptr[0] = 0;
+  for (long i = 0; i < ne; i++) {
+    long nseen = 0, n = 0;
+    for (int j = 0; j < loc && !(n == nneigh && n > 0); j++) {
+      long x = enl[i * loc + j];
+      for (long k = nptr[x]; k < nptr[x + 1]; k++) {
+        long e = nidx[k];
+        if (e == i) continue;
+        seen[e]++;
+        if (seen[e] == 1) seen_eles[nseen++] = e;
+        if (seen[e] == sloc) {
+          indices[m + n] = e;
+          n++;
+          if (n == nneigh) break;
+        }
+      }
+      if (n == nneigh) break;
+    }
+    for (long k = 0; k < nseen; k++) seen[seen_eles[k]] = 0;
+    m += n;
+    indptr[i + 1] = m;
+  }
+  free(seen_eles);
+  free(seen);
+  free(nidx);
+  free(nptr);
+  return m;
+}
+
+/* transpose_sparsity, C_bindings.F90:397-430: column counts, then rows in
+ * ascending order per column. */
+void orc_transpose_sparsity(long nrows, long ncols, const long *indices, const long *ind_ptr, long *indices_t,
+                            long *ind_ptr_t) {
+  long *n = calloc((size_t)ncols + 1, sizeof(long));
+  for (long r = 0; r < nrows; r++)
+    for (long i = ind_ptr[r]; i < ind_ptr[r + 1]; i++) n[indices[i]]++;
+  ind_ptr_t[0] = 0;
+  for (long c = 0; c < ncols; c++) ind_ptr_t[c + 1] = ind_ptr_t[c] + n[c];
+  memset(n, 0, sizeof(long) * ((size_t)ncols + 1));
+  for (long r = 0; r < nrows; r++)
+    for (long i = ind_ptr[r]; i < ind_ptr[r + 1]; i++) {
+      long c = indices[i];
+      indices_t[ind_ptr_t[c] + n[c]] = r;
+      n[c]++;
+    }
+  free(n);
+}
+
+static int sloc_of(int dim, int loc) {
+  if (loc == dim + 1) return loc - 1;
+  if (loc == (1 << dim)) return loc / 2;
+  return -1;
+}
+
+static int nneigh_of(int dim, int loc) {
+  if (loc == dim + 1) return loc;
+  if (loc == (1 << dim)) return 2 * dim;
+  return -1;
+}
+
+/* Closed box test of the advancing front (Intersection_Finder.F90:429-450):
+ * disjoint only when b.max < a.min or b.min > a.max on some axis. */
+static int af_boxes_meet(int dim, const double *a, const double *b) {
+  for (int d = 0; d < dim; d++)
+    if (b[dim + d] < a[d] || b[d] > a[dim + d]) return 0;
+  return 1;
+}
+
+static void af_bbox(int dim, int loc, const double *pos, const long *nodes, double *bb) {
+  for (int d = 0; d < dim; d++) bb[d] = bb[dim + d] = pos[nodes[0] * dim + d];
+  for (int v = 1; v < loc; v++)
+    for (int d = 0; d < dim; d++) {
+      double x = pos[nodes[v] * dim + d];
+      if (x < bb[d]) bb[d] = x;
+      if (x > bb[dim + d]) bb[dim + d] = x;
+    }
+}
+
+/* advancing_front_intersection_finder_intersections
+ * (Intersection_Finder.F90:270-454): the first A element of each connected
+ * part of A is searched by brute force; every other A element, taken from a
+ * stack of A's unvisited face neighbours, starts from the list of the element
+ * that pushed it and advances through B's face neighbours while their boxes
+ * meet.  Returns the number of pairs; *indices (malloc'd, free with orc_free)
+ * holds the rows, indptr[ne_a + 1]. */
+long orc_advancing_front_finder(int dim, long nn_a, long ne_a, int loc_a, const double *pos_a, const long *enl_a,
+                                long nn_b, long ne_b, int loc_b, const double *pos_b, const long *enl_b,
+                                long *indptr, long **indices) {
+  long **rows = calloc((size_t)(ne_a > 0 ? ne_a : 1), sizeof(long *));
+  long *nrow = calloc((size_t)(ne_a > 0 ? ne_a : 1), sizeof(long));
+  if (ne_a > 0 && ne_b > 0) {
+    int nb_b = nneigh_of(dim, loc_b), nb_a = nneigh_of(dim, loc_a);
+    long *eptr_b = malloc(sizeof(long) * ((size_t)ne_b + 1));
+    long *eidx_b = malloc(sizeof(long) * (size_t)(ne_b * nb_b + 1));
+    long *eptr_a = malloc(sizeof(long) * ((size_t)ne_a + 1));
+    long *eidx_a = malloc(sizeof(long) * (size_t)(ne_a * nb_a + 1));
+    orc_mesh_eelist(nn_b, ne_b, loc_b, sloc_of(dim, loc_b), nb_b, enl_b, eptr_b, eidx_b);
+    orc_mesh_eelist(nn_a, ne_a, loc_a, sloc_of(dim, loc_a), nb_a, enl_a, eptr_a, eidx_a);
+    double *bb_b = malloc(sizeof(double) * 2 * dim * (size_t)ne_b);
+    for (long e = 0; e < ne_b; e++) af_bbox(dim, loc_b, pos_b, enl_b + e * loc_b, bb_b + 2 * dim * e);
+    char *seen_b = calloc((size_t)ne_b, 1), *seen_a = calloc((size_t)ne_a, 1);
+    long *front_b = malloc(sizeof(long) * (size_t)ne_b), *ints = malloc(sizeof(long) * (size_t)ne_b);
+    long *front_a = malloc(sizeof(long) * 2 * (size_t)ne_a);
+    double bb_a[6];
+    long seed = 0;
+    while (1) {
+      seen_a[seed] = 1;
+      af_bbox(dim, loc_a, pos_a, enl_a + seed * loc_a, bb_a);
+      long nints = 0;
+      for (long b = 0; b < ne_b; b++)
+        if (af_boxes_meet(dim, bb_a, bb_b + 2 * dim * b)) ints[nints++] = b;
+      rows[seed] = malloc(sizeof(long) * (size_t)(nints > 0 ? nints : 1));
+      memcpy(rows[seed], ints, sizeof(long) * (size_t)nints);
+      nrow[seed] = nints;
+      long nfa = 0;
+      for (long k = eptr_a[seed]; k < eptr_a[seed + 1]; k++) {
+        long na = eidx_a[k];
+        if (!seen_a[na]) {
+          front_a[2 * nfa] = na;
+          front_a[2 * nfa + 1] = seed;
+          nfa++;
+          seen_a[na] = 1;
+        }
+      }
+      while (nfa > 0) {
+        nfa--;
+        long a = front_a[2 * nfa], clue = front_a[2 * nfa + 1];
+        af_bbox(dim, loc_a, pos_a, enl_a + a * loc_a, bb_a);
+        long nfb = nrow[clue];
+        memcpy(front_b, rows[clue], sizeof(long) * (size_t)nfb);
+        for (long k = 0; k < nfb; k++) seen_b[front_b[k]] = 1;
+        nints = 0;
+        for (long i = 0; i < nfb; i++) {
+          long b = front_b[i];
+          if (af_boxes_meet(dim, bb_a, bb_b + 2 * dim * b)) {
+            ints[nints++] = b;
+            for (long k = eptr_b[b]; k < eptr_b[b + 1]; k++) {
+              long nb = eidx_b[k];
+              if (!seen_b[nb]) {
+                front_b[nfb++] = nb;
+                seen_b[nb] = 1;
+              }
+            }
+          }
+        }
+        for (long k = 0; k < nfb; k++) seen_b[front_b[k]] = 0;
+        rows[a] = malloc(sizeof(long) * (size_t)(nints > 0 ? nints : 1));
+        memcpy(rows[a], ints, sizeof(long) * (size_t)nints);
+        nrow[a] = nints;
+        for (long k = eptr_a[a]; k < eptr_a[a + 1]; k++) {
+          long na = eidx_a[k];
+          if (!seen_a[na]) {
+            front_a[2 * nfa] = na;
+            front_a[2 * nfa + 1] = a;
+            nfa++;
+            seen_a[na] = 1;
+          }
+        }
+      }
+      while (seed < ne_a && seen_a[seed]) seed++;
+      if (seed >= ne_a) break;
+    }
+    free(front_a);
+    free(ints);
+    free(front_b);
+    free(seen_a);
+    free(seen_b);
+    free(bb_b);
+    free(eidx_a);
+    free(eptr_a);
+    free(eidx_b);
+    free(eptr_b);
+  }
+  indptr[0] = 0;
+  for (long a = 0; a < ne_a; a++) indptr[a + 1] = indptr[a] + nrow[a];
+  long total = indptr[ne_a];
+  long *out = malloc(sizeof(long) * (size_t)(total > 0 ? total : 1));
+  for (long a = 0; a < ne_a; a++) {
+    if (nrow[a]) memcpy(out + indptr[a], rows[a], sizeof(long) * (size_t)nrow[a]);
+    free(rows[a]);
+  }
+  free(rows);
+  free(nrow);
+  *indices = out;
+  return total;
 }

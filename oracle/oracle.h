@@ -141,6 +141,23 @@ void orc_supermesh(int dim, int finder, int nthreads,
                    long nnodes_b, long ne_b, const double *pos_b, const int *enl_b,
                    long long *stats, double *sum_out, double *times);
 
+/* Mesh adjacency and the advancing front finder; ids 0-based, CSR from 0. */
+void orc_free(void *p);
+/* mesh_nelist (Graphs.F90:76-116): indptr[nnodes+1], indices[ne*loc]. */
+void orc_mesh_nelist(long nnodes, long ne, int loc, const long *enl, long *indptr, long *indices);
+/* mesh_nnlist (C_bindings.F90:215-255): indices needs ne*loc*(loc-1). */
+long orc_mesh_nnlist(long nnodes, long ne, int loc, const long *enl, long *indptr, long *indices);
+/* mesh_eelist (Graphs.F90:118-187), release build: indices needs ne*nneigh. */
+long orc_mesh_eelist(long nnodes, long ne, int loc, long sloc, int nneigh, const long *enl, long *indptr,
+                     long *indices);
+/* transpose_sparsity (C_bindings.F90:397-430). */
+void orc_transpose_sparsity(long nrows, long ncols, const long *indices, const long *ind_ptr, long *indices_t,
+                            long *ind_ptr_t);
+/* advancing_front_intersection_finder (Intersection_Finder.F90:270-454). */
+long orc_advancing_front_finder(int dim, long nn_a, long ne_a, int loc_a, const double *pos_a, const long *enl_a,
+                                long nn_b, long ne_b, int loc_b, const double *pos_b, const long *enl_b,
+                                long *indptr, long **indices);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,7 +7,10 @@
 !   mode "callback": the supermesh callback driver, summing in the callback;
 !   mode "mixed": simplex mesh A against quad / hex mesh B (R-tree finder,
 !     intersect_tri_quad / intersect_tet_hex), or 1D meshes (sort finder,
-!     intersect_intervals).
+!     intersect_intervals);
+!   mode "front" / "finder": advancing_front_intersection_finder /
+!     intersection_finder only; the third number printed is then a checksum
+!     of the CSR, sum of indices(i) * i plus sum of indptr(i) * (i + 7).
 ! Mesh file (stream, little endian): int32 nnodes, dim, nelements, loc;
 ! real(8) positions(dim, nnodes); int32 enlist(loc, nelements) (1-based).
 ! Prints: dim, number of candidate pairs, number of pieces, measure sum.
@@ -60,6 +63,7 @@ program test_intersector_gpu
   real(kind = real_kind), dimension(:, :, :), allocatable :: tets_h
   real(kind = real_kind), dimension(2) :: interval_c
   integer :: a, i, b, c, n
+  integer(kind = 8) :: chk
 
   call get_command_argument(1, fa)
   call get_command_argument(2, fb)
@@ -107,6 +111,23 @@ program test_intersector_gpu
       end do
     end do
     call check_error()
+  case("front", "finder")
+    allocate(indptr(size(ea, 2) + 1))
+    if(trim(mode) == "front") then
+      call advancing_front_intersection_finder(pa, ea, pb, eb, indices, indptr)
+    else
+      call intersection_finder(pa, ea, pb, eb, indices, indptr)
+    end if
+    call check_error()
+    chk = 0
+    do i = 1, size(indices)
+      chk = chk + int(indices(i), 8) * int(i, 8)
+    end do
+    do i = 1, size(indptr)
+      chk = chk + int(indptr(i), 8) * int(i + 7, 8)
+    end do
+    print "(i0,1x,i0,1x,i0,1x,es25.17)", dim, size(indices), chk, 0.0_real_kind
+    stop
   case default
     allocate(indptr(size(ea, 2) + 1))
     call tree_intersection_finder(pa, ea, pb, eb, indices, indptr)

@@ -73,6 +73,18 @@ def lib():
         L.orc_sort_finder.restype = ctypes.c_long
         L.orc_sort_finder.argtypes = [ctypes.c_long, _dp, _ip, ctypes.c_long, _dp, _ip, _lp, _ip,
                                       ctypes.c_long]
+        _l = ctypes.c_long
+        L.orc_mesh_nelist.argtypes = [_l, _l, ctypes.c_int, _lp, _lp, _lp]
+        L.orc_mesh_nnlist.restype = _l
+        L.orc_mesh_nnlist.argtypes = [_l, _l, ctypes.c_int, _lp, _lp, _lp]
+        L.orc_mesh_eelist.restype = _l
+        L.orc_mesh_eelist.argtypes = [_l, _l, ctypes.c_int, _l, ctypes.c_int, _lp, _lp, _lp]
+        L.orc_transpose_sparsity.argtypes = [_l, _l, _lp, _lp, _lp, _lp]
+        L.orc_advancing_front_finder.restype = _l
+        L.orc_advancing_front_finder.argtypes = [ctypes.c_int, _l, _l, ctypes.c_int, _dp, _lp,
+                                                 _l, _l, ctypes.c_int, _dp, _lp, _lp,
+                                                 ctypes.POINTER(_lp)]
+        L.orc_free.argtypes = [ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -298,3 +310,64 @@ def load_fixture_meshes():
     z = np.load(os.path.join(ROOT, "tests", "golden", "meshes.npz"))
     names = sorted({k.split("/")[0] for k in z.files})
     return {n: (z[n + "/pos"], z[n + "/enl"]) for n in names}
+
+
+# -- mesh adjacency and the advancing front finder -------------------------
+
+def _i64(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+def _lpp(a):
+    return a.ctypes.data_as(_lp)
+
+
+def mesh_nelist(nnodes, enl):
+    enl = _i64(enl)
+    ne, loc = enl.shape
+    indptr = np.zeros(nnodes + 1, dtype=np.int64)
+    indices = np.zeros(max(ne * loc, 1), dtype=np.int64)
+    lib().orc_mesh_nelist(nnodes, ne, loc, _lpp(enl), _lpp(indptr), _lpp(indices))
+    return indptr, indices[:ne * loc]
+
+
+def mesh_nnlist(nnodes, enl):
+    enl = _i64(enl)
+    ne, loc = enl.shape
+    indptr = np.zeros(nnodes + 1, dtype=np.int64)
+    indices = np.zeros(max(ne * loc * (loc - 1), 1), dtype=np.int64)
+    m = lib().orc_mesh_nnlist(nnodes, ne, loc, _lpp(enl), _lpp(indptr), _lpp(indices))
+    return indptr, indices[:m]
+
+
+def mesh_eelist(nnodes, enl, sloc, nneigh):
+    enl = _i64(enl)
+    ne, loc = enl.shape
+    indptr = np.zeros(ne + 1, dtype=np.int64)
+    indices = np.zeros(max(ne * nneigh, 1), dtype=np.int64)
+    m = lib().orc_mesh_eelist(nnodes, ne, loc, sloc, nneigh, _lpp(enl), _lpp(indptr), _lpp(indices))
+    return indptr, indices[:m]
+
+
+def transpose_sparsity(ncols, indptr, indices):
+    indptr, indices = _i64(indptr), _i64(indices)
+    nrows = len(indptr) - 1
+    ptr_t = np.zeros(ncols + 1, dtype=np.int64)
+    idx_t = np.zeros(max(len(indices), 1), dtype=np.int64)
+    lib().orc_transpose_sparsity(nrows, ncols, _lpp(indices), _lpp(indptr), _lpp(idx_t), _lpp(ptr_t))
+    return ptr_t, idx_t[:ptr_t[-1]]
+
+
+def advancing_front(pos_a, enl_a, pos_b, enl_b):
+    pos_a, pos_b = _f64(pos_a), _f64(pos_b)
+    enl_a, enl_b = _i64(enl_a), _i64(enl_b)
+    dim = pos_a.shape[1]
+    indptr = np.zeros(enl_a.shape[0] + 1, dtype=np.int64)
+    out = _lp()
+    L = lib()
+    n = L.orc_advancing_front_finder(dim, pos_a.shape[0], enl_a.shape[0], enl_a.shape[1], _d(pos_a),
+                                     _lpp(enl_a), pos_b.shape[0], enl_b.shape[0], enl_b.shape[1],
+                                     _d(pos_b), _lpp(enl_b), _lpp(indptr), ctypes.byref(out))
+    indices = np.ctypeslib.as_array(out, shape=(max(n, 1),))[:n].copy()
+    L.orc_free(ctypes.cast(out, ctypes.c_void_p))
+    return indptr, indices

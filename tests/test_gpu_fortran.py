@@ -76,3 +76,28 @@ def _run_driver(tmp_path, fixture_meshes, a, b, npairs, npieces, total, mode, cl
         assert int(n) == npairs and int(pieces) == npieces
     tol = 100.0 * max(np.finfo(float).eps, O.spacing(total))
     assert abs(float(s) - total) < tol
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran interface not built (no amdflang)")
+@pytest.mark.parametrize("mode,a,b", [("front", "triangle_0_01", "square_0_01"), ("front", "cube.1", "cube.3"),
+                                      ("finder", "square.1", "square.2"), ("finder", "line.1", "line.2")])
+def test_fortran_front_and_default_finder(tmp_path, fixture_meshes, mode, a, b):
+    # advancing_front_intersection_finder / intersection_finder through the
+    # Fortran module: the CSR (1-based) checksum equals the oracle's
+    pa, ea = fixture_meshes[a]
+    pb, eb = fixture_meshes[b]
+    fa, fb = tmp_path / "a.bin", tmp_path / "b.bin"
+    write_mesh(fa, pa, ea)
+    write_mesh(fb, pb, eb)
+    r = subprocess.run([EXE, str(fa), str(fb), mode], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    dim, n, chk, _ = r.stdout.split()
+    if mode == "front":
+        ip, ix = O.advancing_front(pa, ea, pb, eb)
+    elif pa.shape[1] == 1:
+        ip, ix = O.sort_finder(pa, ea, pb, eb)
+    else:
+        ip, ix = O.brute_force(pa, ea, pb, eb, closed=True)
+    ix = np.asarray(ix, dtype=np.int64) + 1
+    ref = int(np.sum(ix * np.arange(1, len(ix) + 1)) + np.sum((np.asarray(ip) + 1) * np.arange(8, len(ip) + 8)))
+    assert int(n) == len(ix) and int(chk) == ref

@@ -1,0 +1,125 @@
+"""GPU parity of the mesh adjacency entry points (mesh_nelist / nnlist /
+eelist, transpose_sparsity; C_bindings.F90:176-430) and of the advancing
+front finder (Intersection_Finder.F90:270-454) against the oracle: equal CSR
+arrays, order included."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from libsupermesh_amd import gpu, meshgen
+
+pytestmark = pytest.mark.gpu
+
+
+def _meshes():
+    rng = np.random.default_rng(9)
+    out = {}
+    for dim in (2, 3):
+        p, e = meshgen.split_grid(dim, 40 if dim == 2 else 8, 1)
+        out[f"simplex{dim}"] = (dim, p.shape[0], e)
+        p, e = meshgen.cell_grid(dim, 30 if dim == 2 else 7, 1)
+        out[f"cells{dim}"] = (dim, p.shape[0], e)
+    out["random"] = (2, 50, rng.integers(0, 40, (200, 3)))  # repeats, unused nodes
+    out["empty"] = (2, 5, np.zeros((0, 3), dtype=np.int64))
+    return out
+
+
+MESHES = _meshes()
+
+
+@pytest.mark.parametrize("name", sorted(MESHES))
+def test_nelist_nnlist(name):
+    dim, nn, enl = MESHES[name]
+    for g, o in ((gpu.mesh_nelist, O.mesh_nelist), (gpu.mesh_nnlist, O.mesh_nnlist)):
+        gp, gi = g(nn, enl)
+        op, oi = o(nn, enl)
+        assert np.array_equal(gp, op) and np.array_equal(gi, oi)
+
+
+@pytest.mark.parametrize("name", sorted(MESHES))
+@pytest.mark.parametrize("sloc_shift", [0, -1, 1])
+def test_eelist(name, sloc_shift):
+    dim, nn, enl = MESHES[name]
+    loc = enl.shape[1]
+    sloc = (loc - 1 if loc == dim + 1 else loc // 2) + sloc_shift
+    gp, gi = gpu.mesh_eelist(nn, enl, sloc)
+    op, oi = O.mesh_eelist(nn, enl, sloc, loc)  # the C binding gives room for loc
+    assert np.array_equal(gp, op) and np.array_equal(gi, oi)
+
+
+def test_eelist_hand_worked_order():
+    enl = np.array([[0, 1, 4], [1, 2, 4], [2, 3, 4], [3, 0, 4], [1, 0, 5]])
+    ptr, idx = gpu.mesh_eelist(6, enl, 2)
+    assert [list(idx[ptr[i]:ptr[i + 1]]) for i in range(5)] == [[4, 1, 3], [0, 2], [1, 3], [0, 2], [0]]
+
+
+@pytest.mark.parametrize("shape,density", [((300, 200), 0.05), ((1, 7), 1.0), ((5, 3), 0.0)])
+def test_transpose(shape, density):
+    import scipy.sparse as sp
+    rng = np.random.default_rng(3)
+    A = sp.random(*shape, density=density, format="csr", random_state=rng)
+    # unsorted rows and a repeated column: order must follow the input
+    idx = A.indices.copy()
+    for r in range(shape[0]):
+        rng.shuffle(idx[A.indptr[r]:A.indptr[r + 1]])
+    if len(idx) > 1:
+        idx[1] = idx[0]
+    gp, gi = gpu.transpose_sparsity(shape[1], A.indptr, idx)
+    op, oi = O.transpose_sparsity(shape[1], A.indptr, idx)
+    assert np.array_equal(gp, op) and np.array_equal(gi, oi)
+
+
+def test_graph_errors():
+    with pytest.raises(gpu.GpuError, match="out of range"):
+        gpu.mesh_nelist(3, np.array([[0, 1, 3]]))
+    with pytest.raises(gpu.GpuError, match="out of range"):
+        gpu.transpose_sparsity(2, np.array([0, 1]), np.array([2]))
+    with pytest.raises(gpu.GpuError, match="decreases"):
+        gpu.transpose_sparsity(2, np.array([0, 2, 1]), np.array([0, 1]))
+    n = gpu.c_long(0)
+    gpu.lib().libsupermesh_mesh_nnlist_query_output(gpu.ctypes.byref(n))
+    with pytest.raises(gpu.GpuError, match="before set_input"):
+        gpu._raise_if_error()
+
+
+def _af_cases(fixture_meshes):
+    pa, ea, pb, eb = meshgen.config_meshes("tri_64_vs_50")
+    yield "tri_64_vs_50", pa, ea, pb, eb
+    pa, ea = meshgen.split_grid(3, 5, 2)
+    pb, eb = meshgen.split_grid(3, 6, 3, diagonal=1)
+    yield "tets", pa, ea, pb, eb
+    pa, ea = meshgen.split_grid(2, 12, 2)
+    pb, eb = meshgen.cell_grid(2, 9, 3)
+    yield "tri_x_quad", pa, ea, pb, eb
+    pa, ea = meshgen.cell_grid(3, 4, 2)
+    pb, eb = meshgen.split_grid(3, 3, 3)
+    yield "hex_x_tet", pa, ea, pb, eb
+    for a, b in (("square.1", "square.2"), ("cube.1", "cube.2"), ("line.1", "line.2"),
+                 ("triangle_0_01", "square_0_01")):
+        pa, ea = fixture_meshes[a]
+        pb, eb = fixture_meshes[b]
+        yield f"{a}_x_{b}", pa, ea, pb, eb
+    # A in two disconnected parts: a second seed
+    p1, e1 = meshgen.split_grid(2, 6, 4)
+    p2 = p1 * 0.3 + 0.65
+    pa = np.concatenate([p1 * 0.5, p2])
+    ea = np.concatenate([e1, e1 + p1.shape[0]])
+    pb, eb = meshgen.split_grid(2, 9, 5)
+    yield "two_parts", pa, ea, pb, eb
+
+
+def test_advancing_front_matches_oracle(fixture_meshes):
+    n = 0
+    for name, pa, ea, pb, eb in _af_cases(fixture_meshes):
+        gp, gi = gpu.advancing_front_intersection_finder(pa, ea, pb, eb)
+        op, oi = O.advancing_front(pa, ea, pb, eb)
+        assert np.array_equal(gp, op), name
+        assert np.array_equal(gi, oi), name
+        n += len(oi)
+    assert n > 100000
+
+
+def test_advancing_front_empty():
+    pa, ea = meshgen.split_grid(2, 3, 1)
+    gp, gi = gpu.advancing_front_intersection_finder(pa, ea, pa[:0], np.zeros((0, 3), dtype=np.int64))
+    assert np.array_equal(gp, np.zeros(len(ea) + 1)) and len(gi) == 0
