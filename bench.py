@@ -281,7 +281,7 @@ def clip_roofline(name, w):
                 "traffic": None, "kernel_ms": w["clip_kernel_ms"]}
     gbs = bpp * w["pairs"] / secs / 1e9
     out = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-           "traffic": PROFILED_TRAFFIC.get(name), "kernel": "k_clip<2> count pass with mixed mass"
+           "traffic": PROFILED_TRAFFIC.get(name), "kernel": "k_clip_tri count pass with mixed mass"
            if name.startswith("tri") else "k_clip_tet_walk count pass",
            "bytes_per_pair": bpp, "pairs_per_launch": w["pairs"], "kernel_ms": w["clip_kernel_ms"]}
     if fpp:

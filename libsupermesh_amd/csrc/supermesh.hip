@@ -152,6 +152,10 @@ struct ClipArgs {
   unsigned char* dropped;      // single pass: the lane's arena ran out, outputs dropped
   unsigned long long* ndropped;
   const unsigned char* ovf;  // fill pass: when non-null, only items flagged here
+  // 2D: pairs whose working polygon outgrew its LDS slots, for the
+  // streaming clipper
+  int* olist;
+  unsigned long long* nolist;
   // mass row slots (2D): the count pass also writes each pair's contributions
   // (B node, value) to the rows of A's nodes, at
   // rbase[a * LOC + li] + (p - indptr[a]) * LOC + j; empty pairs write node -1
@@ -325,6 +329,192 @@ __global__ void __launch_bounds__(kClipBlock) k_clip(ClipArgs g) {
           for (int j = 0; j < LOC; j++) {
             g.rkey[o + j] = nodes[j];
             g.rval[o + j] = nk == 0 ? 0.0 : blk[li * LOC + j];
+          }
+        }
+      }
+    }
+  }
+}
+
+// The polygon clip of intersect_tris / intersect_polys_lines
+// (Triangle_Intersection.F90:144-185, 216-276, 282-335) with the
+// intermediate polygons in LDS: pass 0 reads A from registers, passes
+// 1..NL-2 read the previous pass's polygon from LDS through one load site and
+// write theirs through one store site, and the last pass streams into the fan.
+// The streaming clip_poly instead inlines every later pass, and the sink, at
+// every emit site of every earlier pass.  Same operations in the same order as
+// clip_poly.  Pass s writes buffer s & 1 (capacity NA + s + 1, the most a
+// convex polygon reaches); lane t's point k of buffer b is at
+// q[(off(b) + k) * 2 * LANES] (x) and the next word row (y), q = base + t.
+// Returns -1 when a pass would overflow its buffer (only possible when
+// rounding makes the polygon non-convex); the caller then hands the pair to
+// clip_poly.
+template <int NA, int NL>
+struct LdsPoly {
+  static constexpr int cap(int b) {
+    int c = 0;
+    for (int s = b; s < NL - 1; s += 2) c = NA + s + 1 > c ? NA + s + 1 : c;
+    return c;
+  }
+  static constexpr int kCap0 = cap(0), kCap1 = cap(1);
+  static constexpr int kPoints = kCap0 + kCap1;  // LDS points per lane
+};
+
+template <int NL, int NA, int LANES, class Sink>
+__device__ __forceinline__ int clip_poly_lds(const double* A, const Line* L, double tol, double* q,
+                                             Sink&& sink) {
+  using P = LdsPoly<NA, NL>;
+  auto X = [&](int b, int k) -> double& { return q[((b ? P::kCap0 : 0) + k) * 2 * LANES]; };
+  auto Y = [&](int b, int k) -> double& { return q[(((b ? P::kCap0 : 0) + k) * 2 + 1) * LANES]; };
+  // the fan of the last pass's points (triangle (q1, q_{k-1}, q_k) for k >= 3)
+  double fx0 = 0.0, fy0 = 0.0, fx1 = 0.0, fy1 = 0.0;
+  int fn = 0, kept = 0;
+  auto fan_push = [&](double x, double y) {
+    if (fn == 0) {
+      fx0 = x;
+      fy0 = y;
+    } else if (fn >= 2) {
+      const double ar = triangle_area6(fx0, fy0, fx1, fy1, x, y);
+      if (!(ar < tol)) {
+        sink(fx0, fy0, fx1, fy1, x, y, ar);
+        kept++;
+      }
+    }
+    fx1 = x;
+    fy1 = y;
+    fn++;
+  };
+  int n = NA;
+#pragma unroll
+  for (int s = 0; s < NL; s++) {
+    const int src = (s + 1) & 1, dst = s & 1;
+    const int cap = dst ? P::kCap1 : P::kCap0;
+    const bool last = s == NL - 1;
+    const Line l = L[s];
+    const double fx = s == 0 ? A[0] : X(src, 0), fy = s == 0 ? A[1] : Y(src, 0);
+    double fd = l.nx * (fx - l.px);
+    fd = fd + l.ny * (fy - l.py);
+    double x1 = fx, y1 = fy, d1 = fd;
+    int m = 0;
+    bool over = false;
+    auto store = [&](double u, double v) {
+      if (m < cap) {
+        X(dst, m) = u;
+        Y(dst, m) = v;
+      } else {
+        over = true;
+      }
+    };
+    auto edge = [&](double x2, double y2, double d2) {
+      if (last) clip_edge(x1, y1, d1, x2, y2, d2, m, fan_push);
+      else clip_edge(x1, y1, d1, x2, y2, d2, m, store);
+      x1 = x2;
+      y1 = y2;
+      d1 = d2;
+    };
+    if (s == 0) {
+#pragma unroll
+      for (int i = 1; i < NA; i++) {
+        double d2 = l.nx * (A[2 * i] - l.px);
+        d2 = d2 + l.ny * (A[2 * i + 1] - l.py);
+        edge(A[2 * i], A[2 * i + 1], d2);
+      }
+    } else {
+      for (int i = 1; i < n; i++) {
+        const double x2 = X(src, i), y2 = Y(src, i);
+        double d2 = l.nx * (x2 - l.px);
+        d2 = d2 + l.ny * (y2 - l.py);
+        edge(x2, y2, d2);
+      }
+    }
+    edge(fx, fy, fd);
+    if (over) return -1;
+    if (m < 3) return 0;
+    n = m;
+  }
+  return kept;
+}
+
+constexpr int kTriBlock = 128;
+constexpr int kTriGrid = 4096;
+
+// 2D pairs, one per lane: the count pass (counts, measures, mass blocks and
+// mass row slots) or the fill pass, as k_clip<2, ...>, on clip_poly_lds.
+// Pairs that outgrow the LDS polygon go to g.olist for k_clip.
+template <bool MASS, bool FILL>
+__global__ void __launch_bounds__(kTriBlock) k_clip_tri(ClipArgs g) {
+  __shared__ double poly[LdsPoly<3, 3>::kPoints * 2 * kTriBlock];
+  double* q = poly + threadIdx.x;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < g.npairs; p += stride) {
+    const long a = g.rows[p], b = g.cols[p];
+    long off = 0;
+    int cap = 0;
+    if (FILL) {
+      cap = g.counts[p];
+      if (cap == 0) continue;
+      off = g.offsets[p];
+    }
+    double A[6], Bv[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) A[k] = g.acoords[a * 6 + k];
+#pragma unroll
+    for (int k = 0; k < 6; k++) Bv[k] = g.bcoords[b * 6 + k];
+    const double tol = 10.0 * fmin_f(g.asp[a], g.bsp[b]);
+    double m = 0.0;
+    double blk[9], affa[6], affb[6];
+    if (MASS) {
+#pragma unroll
+      for (int k = 0; k < 9; k++) blk[k] = 0.0;
+#pragma unroll
+      for (int k = 0; k < 6; k++) {
+        affa[k] = g.aaff[a * 6 + k];
+        affb[k] = g.baff[b * 6 + k];
+      }
+    }
+    Line L[3];
+    tri_lines(Bv, L);
+    int k = 0;
+    const int nk = clip_poly_lds<3, 3, kTriBlock>(
+        A, L, tol, q, [&](double x0, double y0, double x1, double y1, double x2, double y2, double ar) {
+          m = m + ar;
+          if (MASS) {
+            const double el[6] = {x0, y0, x1, y1, x2, y2};
+            mass_element<2>(el, ar, affa, affb, blk);
+          }
+          if (FILL && k < cap) {
+            double* o = g.elements + (off + k) * 6;
+            o[0] = x0;
+            o[1] = y0;
+            o[2] = x1;
+            o[3] = y1;
+            o[4] = x2;
+            o[5] = y2;
+          }
+          k++;
+        });
+    if (nk < 0) {
+      if (!FILL) g.olist[atomicAdd(g.nolist, 1ULL)] = (int)p;
+      continue;
+    }
+    if (FILL) continue;
+    g.counts[p] = nk;
+    g.measure[p] = nk == 0 ? 0.0 : m;
+    if (MASS) {
+#pragma unroll
+      for (int i = 0; i < 9; i++) g.blocks[p * 9 + i] = nk == 0 ? 0.0 : blk[i];
+      if (g.rkey) {
+        const long r = (p - g.indptr[a]) * 3;
+        int nodes[3];
+#pragma unroll
+        for (int j = 0; j < 3; j++) nodes[j] = nk == 0 ? -1 : g.enlb[b * 3 + j];
+#pragma unroll
+        for (int li = 0; li < 3; li++) {
+          const long o = g.rbase[a * 3 + li] + r;
+#pragma unroll
+          for (int j = 0; j < 3; j++) {
+            g.rkey[o + j] = nodes[j];
+            g.rval[o + j] = nk == 0 ? 0.0 : blk[li * 3 + j];
           }
         }
       }
@@ -591,12 +781,22 @@ struct MixedArgs {
   double* measure;
   const long* offsets;
   double* elements;
+  // pairs whose polygon outgrew its LDS slots (STREAM pass: the list to do)
+  int* olist;
+  unsigned long long* nolist;
 };
 
-template <int LA, int LB, bool FILL>
-__global__ void __launch_bounds__(kClipBlock) k_clip_mixed(MixedArgs g) {
+// STREAM false: every pair on clip_poly_lds, handing overflows to olist;
+// STREAM true: the pairs of olist on the streaming clip_poly.
+template <int LA, int LB, bool FILL, bool STREAM>
+__global__ void __launch_bounds__(kTriBlock) k_clip_mixed(MixedArgs g) {
+  constexpr int NA = LA == 4 && LB == 4 ? 4 : 3;
+  __shared__ double poly[STREAM ? 1 : LdsPoly<NA, 4>::kPoints * 2 * kTriBlock];
+  double* q = poly + threadIdx.x;
   const long stride = (long)gridDim.x * blockDim.x;
-  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < g.npairs; p += stride) {
+  const long total = STREAM ? (long)*g.nolist : g.npairs;
+  for (long w = (long)blockIdx.x * blockDim.x + threadIdx.x; w < total; w += stride) {
+    const long p = STREAM ? g.olist[w] : w;
     long off = 0;
     int cap = 0;
     if (FILL) {
@@ -626,16 +826,24 @@ __global__ void __launch_bounds__(kClipBlock) k_clip_mixed(MixedArgs g) {
       k++;
     };
     Line L[4];
-    int n = 0;
+    const double* poly_in = A;
+    double tol = 10.0 * 2.2250738585072014e-308;
     if constexpr (LA == 3 && LB == 4) {
       quad_lines(B, L);
-      n = clip_poly<4, 3>(A, L, 10.0 * fspacing(triangle_area(A)), sink);
+      tol = 10.0 * fspacing(triangle_area(A));
     } else if constexpr (LA == 4 && LB == 3) {
       quad_lines(A, L);
-      n = clip_poly<4, 3>(B, L, 10.0 * fspacing(triangle_area(B)), sink);
+      poly_in = B;
+      tol = 10.0 * fspacing(triangle_area(B));
     } else {
       quad_lines(B, L);
-      n = clip_poly<4, 4>(A, L, 10.0 * 2.2250738585072014e-308, sink);
+    }
+    int n;
+    if constexpr (STREAM) n = clip_poly<4, NA>(poly_in, L, tol, sink);
+    else n = clip_poly_lds<4, NA, kTriBlock>(poly_in, L, tol, q, sink);
+    if (n < 0) {
+      if (!FILL) g.olist[atomicAdd(g.nolist, 1ULL)] = (int)p;
+      continue;
     }
     if (FILL) continue;
     // a pass left with fewer than 3 points: no output, whatever was fanned
@@ -768,13 +976,24 @@ __global__ void __launch_bounds__(256) k_hex_planes(long n, const double* __rest
   for (int i = 0; i < 24; i++) planes[e * 24 + i] = pl[i];
 }
 
-template <bool FILL>
-static void launch_mixed(Context& C, int la, int lb, const MixedArgs& g) {
-  const dim3 grid(kClipGrid), block(kClipBlock);
-  if (la == 3 && lb == 4) hipLaunchKernelGGL((k_clip_mixed<3, 4, FILL>), grid, block, 0, C.stream, g);
-  else if (la == 4 && lb == 3) hipLaunchKernelGGL((k_clip_mixed<4, 3, FILL>), grid, block, 0, C.stream, g);
-  else hipLaunchKernelGGL((k_clip_mixed<4, 4, FILL>), grid, block, 0, C.stream, g);
+template <bool FILL, bool STREAM>
+static void launch_mixed_as(Context& C, int la, int lb, const MixedArgs& g) {
+  const dim3 grid(STREAM ? 64 : kTriGrid), block(kTriBlock);
+  if (la == 3 && lb == 4) hipLaunchKernelGGL((k_clip_mixed<3, 4, FILL, STREAM>), grid, block, 0, C.stream, g);
+  else if (la == 4 && lb == 3) hipLaunchKernelGGL((k_clip_mixed<4, 3, FILL, STREAM>), grid, block, 0, C.stream, g);
+  else hipLaunchKernelGGL((k_clip_mixed<4, 4, FILL, STREAM>), grid, block, 0, C.stream, g);
   LSM_HIP(hipGetLastError());
+}
+
+// LDS clipper over all pairs, then the streaming one over the pairs it handed
+// back (counted on the device; usually none).
+template <bool FILL>
+static void launch_mixed(Context& C, int la, int lb, MixedArgs g) {
+  g.olist = C.work.as<int>((size_t)(g.npairs > 0 ? g.npairs : 1));
+  g.nolist = C.scalars.as<unsigned long long>(16) + 2;
+  if (!FILL) LSM_HIP(hipMemsetAsync(g.nolist, 0, sizeof(unsigned long long), C.stream));
+  launch_mixed_as<FILL, false>(C, la, lb, g);
+  launch_mixed_as<FILL, true>(C, la, lb, g);
 }
 
 // Persistent lanes for the hexahedron walk: one full wave of resident blocks.
@@ -972,6 +1191,27 @@ void batch_sums(Context& C, long ngroups, const long* a_offsets, const long* ind
 template <int DIM>
 static void launch_clip(Context& C, const ClipArgs& args, bool mass, bool fill) {
   dim3 grid(kClipGrid), block(kClipBlock);
+  if (DIM == 2 && args.work == nullptr) {
+    // LDS clipper over all pairs, then the streaming clipper over the pairs
+    // it handed back (a device-side count: usually none, no host sync)
+    ClipArgs g = args;
+    g.olist = C.work.as<int>((size_t)(args.npairs > 0 ? args.npairs : 1));
+    g.nolist = C.scalars.as<unsigned long long>(16) + 2;
+    if (!fill) LSM_HIP(hipMemsetAsync(g.nolist, 0, sizeof(unsigned long long), C.stream));
+    const dim3 tgrid(kTriGrid), tblock(kTriBlock);
+    if (fill) hipLaunchKernelGGL((k_clip_tri<false, true>), tgrid, tblock, 0, C.stream, g);
+    else if (mass) hipLaunchKernelGGL((k_clip_tri<true, false>), tgrid, tblock, 0, C.stream, g);
+    else hipLaunchKernelGGL((k_clip_tri<false, false>), tgrid, tblock, 0, C.stream, g);
+    LSM_HIP(hipGetLastError());
+    g.work = g.olist;
+    g.nwork = g.nolist;
+    const dim3 fgrid(64);
+    if (fill) hipLaunchKernelGGL((k_clip<DIM, false, true>), fgrid, block, 0, C.stream, g);
+    else if (mass) hipLaunchKernelGGL((k_clip<DIM, true, false>), fgrid, block, 0, C.stream, g);
+    else hipLaunchKernelGGL((k_clip<DIM, false, false>), fgrid, block, 0, C.stream, g);
+    LSM_HIP(hipGetLastError());
+    return;
+  }
   if (fill) {
     hipLaunchKernelGGL((k_clip<DIM, false, true>), grid, block, 0, C.stream, args);
   } else if (mass) {

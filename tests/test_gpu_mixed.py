@@ -172,3 +172,18 @@ def test_bulk_mixed_rejects_mass_and_bad_loc():
     bad = gpu.DeviceMesh(pa, np.zeros((2, 5), dtype=np.int32))
     with pytest.raises(gpu.GpuError, match="loc"):
         ctx.supermesh_mixed(m, bad)
+
+
+def test_bulk_mixed_concave_quad():
+    # an arrowhead quad whose vertices alternate sides of B's first line: the
+    # first pass emits 6 points, more than a convex quad can (the library's
+    # per-lane polygon slots), so the pair is clipped by the streaming path;
+    # the result must still be the reference's (intersect_quads)
+    pa = np.array([[0.0, 0.0], [2.0, 1.0], [0.0, 2.0], [0.5, 1.0]])
+    ea = np.array([[0, 1, 2, 3]], dtype=np.int32)
+    pb = np.array([[0.3, -1.0], [0.3, 3.0], [-1.0, 3.0], [-1.0, -1.0]])
+    eb = np.array([[0, 1, 2, 3]], dtype=np.int32)
+    ref = O.intersect_quads(pa[ea[0]], pb[eb[0]])
+    assert len(ref) >= 3
+    r = _check_bulk(pa, ea, pb, eb, gpu.FINDER_RTREE, True)
+    assert r.nout == len(ref)
