@@ -534,6 +534,21 @@ __global__ void __launch_bounds__(kTriBlock) k_clip_tri(ClipArgs g) {
 // with stk = stack + t (consecutive lanes, consecutive words).
 constexpr int kWalkBlock = 128;
 
+// The CDNA4 dispatcher hands consecutive workgroups to its 8 XCDs (each with
+// its own L2) in turn.  A grid whose size is a multiple of 8 can give XCD x
+// the items [x * per, (x + 1) * per) of n: this lane starts at w, stops at
+// end and steps by stride.
+constexpr int kXcds = 8;
+__device__ __forceinline__ void xcd_share(long n, long& w, long& end, long& stride) {
+  const int nx = gridDim.x % kXcds == 0 ? kXcds : 1;
+  const long xcd = blockIdx.x % nx, lb = blockIdx.x / nx;
+  const long per = (n + nx - 1) / nx;
+  const long begin = xcd * per;
+  end = begin + per < n ? begin + per : n;
+  stride = (long)(gridDim.x / nx) * blockDim.x;
+  w = begin + lb * blockDim.x + threadIdx.x;
+}
+
 // NP planes: class of the tet at each depth 0..NP-1, current depth, and the
 // child index reached at each depth 1..NP (2 bits each).
 template <int NP>
@@ -596,9 +611,10 @@ template <bool MASS, bool FILL, bool LEARNED = false>
 __global__ void __launch_bounds__(kWalkBlock) k_clip_tet_walk(ClipArgs g) {
   __shared__ double stack[3 * 12 * kWalkBlock];
   double* stk = stack + threadIdx.x;
-  const long n = (long)*g.nwork;
-  const long stride = (long)gridDim.x * blockDim.x;
-  long w = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  // blocks go to the XCDs round robin: each XCD takes one contiguous share
+  // of the work list, so that its L2 holds one region of A and B
+  long w, n, stride;
+  xcd_share((long)*g.nwork, w, n, stride);
   long p = 0, off = 0;
   int cap = 0, k = 0;
   // single pass: this lane's arena region and its fill level
@@ -891,9 +907,8 @@ template <int MODE, bool FILL>
 __global__ void __launch_bounds__(kHexBlock) k_clip_hex_walk(HexWalkArgs g) {
   __shared__ double stack[5 * 12 * kHexBlock];
   double* stk = stack + threadIdx.x;
-  const long n = g.npairs;
-  const long stride = (long)gridDim.x * blockDim.x;
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long p, n, stride;
+  xcd_share(g.npairs, p, n, stride);
   long off = 0, a = 0;
   int cap = 0, k = 0, sub = 0;
   double T[12], tol = 0.0, m = 0.0;
@@ -1010,7 +1025,8 @@ static void launch_hex_walk_as(Context& C, const HexWalkArgs& g) {
   LSM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, C.device));
   const long lanes = (long)blocks_per_cu * (cus > 0 ? cus : 256) * kHexBlock;
   const long need = (g.npairs + kHexBlock - 1) / kHexBlock;
-  const unsigned blocks = (unsigned)std::max(1L, std::min(lanes / kHexBlock, need));
+  unsigned blocks = (unsigned)std::max(1L, std::min(lanes / kHexBlock, need));
+  if (blocks >= kXcds) blocks -= blocks % kXcds;  // whole XCD shares (xcd_share)
   hipLaunchKernelGGL((k_clip_hex_walk<MODE, FILL>), dim3(blocks), dim3(kHexBlock), 0, C.stream, g);
   LSM_HIP(hipGetLastError());
 }
