@@ -49,6 +49,7 @@ static void init_context(Context& C, int device) {
   LSM_HIP(hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking));
   for (int i = 0; i < 16; i++) LSM_HIP(hipEventCreate(&C.ev[i]));
   C.events = true;
+  LSM_HIP(hipHostMalloc(reinterpret_cast<void**>(&C.hstat), 64 * sizeof(long), hipHostMallocDefault));
 }
 
 static void free_context(Context& C) {
@@ -73,6 +74,8 @@ static void free_context(Context& C) {
   for (DevBuf* b : bufs) b->release();
   if (C.pinned) hipHostFree(C.pinned);
   C.pinned = nullptr;
+  if (C.hstat) hipHostFree(C.hstat);
+  C.hstat = nullptr;
   if (C.events)
     for (int i = 0; i < 16; i++) hipEventDestroy(C.ev[i]);
   C.events = false;
@@ -159,15 +162,25 @@ static void run_supermesh(Context& C, const libsupermesh_gpu_mesh* a, const libs
   Phase ph(C, flags & LIBSUPERMESH_GPU_TIMING);
   C.timing = ph.on;
   ph.mark();
-  stage_mesh(C, C.mesh_a, a->dim, loc_a, a->nnodes, a->nelements, a->positions, a->enlist, false, mass);
+  // the node-index check of both meshes is read back with the candidate count
+  // (a bad index is clamped on the device, so the search in between is safe)
+  int* bad = stage_flag(C);
+  LSM_HIP(hipMemsetAsync(bad, 0, sizeof(int), C.stream));
+  stage_mesh(C, C.mesh_a, a->dim, loc_a, a->nnodes, a->nelements, a->positions, a->enlist, false, mass, true);
   stage_mesh(C, C.mesh_b, b->dim, loc_b, b->nnodes, b->nelements, b->positions, b->enlist,
-             clip && !mixed, mass);
+             clip && !mixed, mass, true);
+  C.hstat[0] = 0;
+  LSM_HIP(hipMemcpyAsync(C.hstat, bad, sizeof(int), hipMemcpyDeviceToHost, C.stream));
   stage_groups(C, C.mesh_a, ngroups, a_off);
   stage_groups(C, C.mesh_b, ngroups, b_off);
   ph.mark();
   build_bvh(C, C.mesh_b);
   ph.mark();
+  // find_pairs synchronises after reading the candidate count, which also
+  // completes the copy of the flag queued before it
   long npairs = find_pairs(C, C.mesh_a, C.mesh_b, finder, flags & LIBSUPERMESH_GPU_SORTED);
+  LSM_REQUIRE(*reinterpret_cast<const int*>(C.hstat) == 0, LIBSUPERMESH_GPU_ERROR_INVALID,
+              "Invalid node: element-node index out of range");
   ph.mark();
   out->nelements_a = a->nelements;
   out->npairs = npairs;

@@ -8,7 +8,7 @@
 // R-tree finder uses closed intervals.  Both predicates run here in FP64 on
 // exact min/max boxes.
 //
-// Design: B's element boxes are sorted by the 63/64-bit Morton code of their
+// Design: B's element boxes are sorted by the leading 32 bits of the Morton code of their
 // centres and grouped into an implicit F-ary hierarchy (node i of level l
 // covers children [F*i, F*i+F) of level l-1; level 0 = elements).  Internal
 // boxes are exact unions, so pruning with the same predicate never loses a
@@ -136,8 +136,11 @@ k_stage(long ne, long nnodes, const double* __restrict__ pos, const int* __restr
   }
 }
 
+int* stage_flag(Context& C) { return C.scalars.as<int>(64); }
+
 void stage_mesh(Context& C, DevMesh& M, int dim, int loc, long nnodes, long ne,
-                const double* positions, const int* enlist, bool need_planes, bool need_affine) {
+                const double* positions, const int* enlist, bool need_planes, bool need_affine,
+                bool defer_check) {
   LSM_REQUIRE(dim == 2 || dim == 3, LIBSUPERMESH_GPU_ERROR_INVALID, "Invalid dimension");
   const bool simplex = loc == dim + 1;
   LSM_REQUIRE(simplex || loc == (1 << dim), LIBSUPERMESH_GPU_ERROR_INVALID,
@@ -160,8 +163,8 @@ void stage_mesh(Context& C, DevMesh& M, int dim, int loc, long nnodes, long ne,
   double* scale = M.scale.as<double>((size_t)ne);
   double* planes = need_planes && dim == 3 ? M.planes.as<double>((size_t)ne * 16) : nullptr;
   double* affine = need_affine ? M.affine.as<double>((size_t)ne * (dim + dim * dim)) : nullptr;
-  int* bad = C.scalars.as<int>(64);
-  LSM_HIP(hipMemsetAsync(bad, 0, sizeof(int), C.stream));
+  int* bad = stage_flag(C);
+  if (!defer_check) LSM_HIP(hipMemsetAsync(bad, 0, sizeof(int), C.stream));
   if (ne > 0) {
 #define LSM_STAGE(D, L)                                                                         \
   hipLaunchKernelGGL((k_stage<D, L>), dim3(grid_for(ne, kBlock)), dim3(kBlock), 0, C.stream, ne,  \
@@ -176,6 +179,7 @@ void stage_mesh(Context& C, DevMesh& M, int dim, int loc, long nnodes, long ne,
 #undef LSM_STAGE
     LSM_HIP(hipGetLastError());
   }
+  if (defer_check) return;
   int hbad = 0;
   LSM_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, C.stream));
   LSM_HIP(hipStreamSynchronize(C.stream));
@@ -311,7 +315,7 @@ __device__ __forceinline__ uint64_t spread3(uint64_t x) {  // 21 bits -> every t
 template <int DIM>
 __global__ void __launch_bounds__(kBlock)
 k_morton(long ne, const double* __restrict__ bbox, const double* __restrict__ bounds,
-         const int* __restrict__ group, int gbits, uint64_t* __restrict__ keys,
+         const int* __restrict__ group, int gbits, uint32_t* __restrict__ keys,
          int* __restrict__ vals) {
   long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= ne) return;
@@ -327,11 +331,13 @@ k_morton(long ne, const double* __restrict__ bbox, const double* __restrict__ bo
     uint64_t q = (uint64_t)(t * maxq);
     code |= (DIM == 2 ? spread2(q) : spread3(q)) << d;
   }
-  if (group && gbits > 0) {
-    constexpr int KEYBITS = DIM == 2 ? 64 : 63;
-    code = ((uint64_t)group[e] << (KEYBITS - gbits)) | (code >> gbits);
-  }
-  keys[e] = code;
+  constexpr int KEYBITS = DIM == 2 ? 64 : 63;
+  if (group && gbits > 0) code = ((uint64_t)group[e] << (KEYBITS - gbits)) | (code >> gbits);
+  // the leading 32 bits (16 per axis in 2D, 10-11 in 3D, below a batch's
+  // mesh id) order the elements finely enough for the hierarchy; ties keep
+  // element order (the sort is stable), and the candidate sets do not depend
+  // on the order at all
+  keys[e] = (uint32_t)(code >> (KEYBITS - 32));
   vals[e] = (int)e;
 }
 
@@ -342,8 +348,8 @@ static void morton_order(Context& C, const DevMesh& M, DevBuf& perm) {
   int dim = M.dim;
   int nparts = (int)std::min<long>(1024, std::max<long>(1, (ne + kBlock - 1) / kBlock));
   double* part = C.partials.as<double>((size_t)nparts * 6 + 6);
-  uint64_t* keys = C.keys.as<uint64_t>(ne);
-  uint64_t* keys2 = C.keys_alt.as<uint64_t>(ne);
+  uint32_t* keys = C.keys.as<uint32_t>(ne);
+  uint32_t* keys2 = C.keys_alt.as<uint32_t>(ne);
   int* vals = C.vals.as<int>(ne);
   int* out = perm.as<int>(ne);
   const double* bbox = static_cast<const double*>(M.bbox.ptr);
@@ -362,12 +368,12 @@ static void morton_order(Context& C, const DevMesh& M, DevBuf& perm) {
                        part, group, gbits, keys, vals);
   }
   LSM_HIP(hipGetLastError());
-  unsigned end_bit = dim == 2 ? 64 : 63;
+  const unsigned begin_bit = 0, end_bit = 32;
   size_t tmp_bytes = 0;
-  LSM_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys, keys2, vals, out, (size_t)ne, 0u,
+  LSM_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys, keys2, vals, out, (size_t)ne, begin_bit,
                                     end_bit, C.stream));
   void* tmp = C.sort_tmp.get(tmp_bytes);
-  LSM_HIP(rocprim::radix_sort_pairs(tmp, tmp_bytes, keys, keys2, vals, out, (size_t)ne, 0u, end_bit,
+  LSM_HIP(rocprim::radix_sort_pairs(tmp, tmp_bytes, keys, keys2, vals, out, (size_t)ne, begin_bit, end_bit,
                                     C.stream));
 }
 

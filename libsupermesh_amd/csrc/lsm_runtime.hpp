@@ -139,6 +139,9 @@ struct Context {
   DevBuf one_in, one_out;
   void* pinned = nullptr;
   size_t pinned_bytes = 0;
+  // pinned status words the pipeline copies back asynchronously and reads
+  // after its next synchronisation: [0] staging flag, [1] longest mass row
+  long* hstat = nullptr;
   Predictor predictor;
   // 2D mixed mass: rows laid out before the clip and filled by it
   bool mass_rows_ready = false;
@@ -152,8 +155,12 @@ struct Context {
 Context* default_context();
 
 // Mesh staging + BVH (finder.hip).
+// With defer_check the node-index check is left to the caller: it clears the
+// flag (stage_flag) before staging and reads it back later.
 void stage_mesh(Context& C, DevMesh& M, int dim, int loc, long nnodes, long nelements,
-                const double* positions, const int* enlist, bool need_planes, bool need_affine);
+                const double* positions, const int* enlist, bool need_planes, bool need_affine,
+                bool defer_check = false);
+int* stage_flag(Context& C);
 // Mesh id of every element of a batch from its element offsets (device,
 // [ngroups + 1]); validates the offsets.
 void stage_groups(Context& C, DevMesh& M, long ngroups, const long* offsets);

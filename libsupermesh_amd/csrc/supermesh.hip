@@ -2074,6 +2074,13 @@ static void assemble_mass_slots(Context& C, const DevMesh& A, const DevMesh& B, 
   long* ucount = C.mvals_alt.as<long>(nn + 1);
   int* ovf = C.scalars.as<int>(64) + 4;
   LSM_HIP(hipMemsetAsync(ovf, 0, sizeof(int), C.stream));
+  // the longest row, copied back during the clip (the clip synchronised since)
+  C.mass_maxraw = C.hstat[1];
+  LSM_REQUIRE(C.mass_maxraw >= 0, LIBSUPERMESH_GPU_ERROR_STATE, "mass row bound not read back");
+  if (C.mass_maxraw > 2048) {  // rows too long for the row kernels' LDS
+    assemble_mass_sorted(C, A, B, out->npairs, out);
+    return;
+  }
   int cus = 0;
   LSM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, C.device));
   const dim3 grid((unsigned)std::max<long>(1, std::min<long>((long)(cus > 0 ? cus : 256) * 16, nn)));
@@ -2084,23 +2091,24 @@ static void assemble_mass_slots(Context& C, const DevMesh& A, const DevMesh& B, 
     hipLaunchKernelGGL((k_mass_rows_gather<3, 2048, 2, true>), grid, dim3(128), 0, C.stream, nn, nptr, nullptr,
                        nullptr, nullptr, nullptr, nullptr, nullptr, raw_off, tkey, tval, ucount, ovf);
   LSM_HIP(hipGetLastError());
-  int hovf = 0;
-  LSM_HIP(hipMemcpyAsync(&hovf, ovf, sizeof(int), hipMemcpyDeviceToHost, C.stream));
+  // row lengths scanned before the overflow flag is known: one read-back for
+  // both (an overflowing row leaves the scan meaningless, and it is unused)
+  inclusive_scan_long(C, ucount, mindptr + 1, nn);
+  C.hstat[2] = 0;
+  C.hstat[3] = 0;
+  LSM_HIP(hipMemcpyAsync(C.hstat + 2, ovf, sizeof(int), hipMemcpyDeviceToHost, C.stream));
+  LSM_HIP(hipMemcpyAsync(C.hstat + 3, mindptr + nn, sizeof(long), hipMemcpyDeviceToHost, C.stream));
   LSM_HIP(hipStreamSynchronize(C.stream));
-  if (hovf) {  // a row with more distinct columns than the row kernel holds
+  if (*reinterpret_cast<const int*>(C.hstat + 2)) {  // a row with more distinct columns than the kernel holds
     assemble_mass_sorted(C, A, B, out->npairs, out);
     return;
   }
-  inclusive_scan_long(C, ucount, mindptr + 1, nn);
-  long nnz = 0;
-  LSM_HIP(hipMemcpyAsync(&nnz, mindptr + nn, sizeof(long), hipMemcpyDeviceToHost, C.stream));
-  LSM_HIP(hipStreamSynchronize(C.stream));
+  const long nnz = C.hstat[3];
   int* mind = C.mindices.as<int>(nnz);
   double* mval = C.uvals.as<double>(nnz);
   hipLaunchKernelGGL(k_mass_copy, dim3(grid_for(nn, 4)), dim3(256), 0, C.stream, nn, raw_off, mindptr, tkey, tval,
                      mind, mval);
   LSM_HIP(hipGetLastError());
-  LSM_HIP(hipStreamSynchronize(C.stream));
   out->mass_nnz = nnz;
   out->mass_indices = mind;
   out->mass_values = mval;
@@ -2154,20 +2162,22 @@ bool mass_prepare_rows(Context& C, const DevMesh& A, const DevMesh& B, const lon
   long* rbase = C.row_count.as<long>(nent + 1);
   hipLaunchKernelGGL(k_elem_base, dim3(grid_for(nent, 256)), dim3(256), 0, C.stream, nent, nlist, slot, rbase);
   LSM_HIP(hipGetLastError());
-  long maxraw = 0, total_raw = 0;
+  // every (element, local vertex) holds LOC slots per candidate pair of the
+  // element, so the slots number LOC^2 per pair; the longest row decides the
+  // row kernel and is copied back for the assembly (read after the clip's
+  // synchronisation, no wait here)
+  const long total_raw = (long)loc * loc * g.npairs;
   {
     size_t bytes = 0;
     long* dmax = reinterpret_cast<long*>(C.scalars.as<long>(64)) + 8;
     LSM_HIP(rocprim::reduce(nullptr, bytes, raw, dmax, 0L, (size_t)nn, rocprim::maximum<long>(), C.stream));
     void* tmp = C.sort_tmp.get(bytes);
     LSM_HIP(rocprim::reduce(tmp, bytes, raw, dmax, 0L, (size_t)nn, rocprim::maximum<long>(), C.stream));
-    LSM_HIP(hipMemcpyAsync(&maxraw, dmax, sizeof(long), hipMemcpyDeviceToHost, C.stream));
-    LSM_HIP(hipMemcpyAsync(&total_raw, raw_off + nn, sizeof(long), hipMemcpyDeviceToHost, C.stream));
-    LSM_HIP(hipStreamSynchronize(C.stream));
+    C.hstat[1] = -1;
+    LSM_HIP(hipMemcpyAsync(C.hstat + 1, dmax, sizeof(long), hipMemcpyDeviceToHost, C.stream));
   }
-  if (maxraw > 2048) return false;
   C.mass_rows_ready = true;
-  C.mass_maxraw = maxraw;
+  C.mass_maxraw = -1;
   g.rbase = rbase;
   g.indptr = indptr;
   g.enlb = B.enlist_dev;

@@ -285,6 +285,10 @@ def test_invalid_inputs_raise(ctx):
     bad[3, 1] = pa.shape[0] + 7
     with pytest.raises(gpu.GpuError, match="Invalid node"):
         run(ctx, pa, bad, pa, ea)
+    bad_b = ea.copy()
+    bad_b[0, 0] = -1
+    with pytest.raises(gpu.GpuError, match="Invalid node"):  # in B, with mass
+        ctx.supermesh(gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pa, bad_b), flags=gpu.MASS)
     p3, e3 = meshgen.split_grid(3, 2, 0)
     with pytest.raises(gpu.GpuError, match="Invalid dimension"):
         run(ctx, pa, ea, p3, e3)
