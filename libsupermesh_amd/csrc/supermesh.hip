@@ -90,10 +90,38 @@ __device__ __forceinline__ void basis(const double* __restrict__ aff, const doub
   }
 }
 
-// Adds the element's quadrature contributions to blk[LOC][LOC].
+// Adds the element's contributions to blk[LOC][LOC].  2D: the product of two
+// P1 functions is quadratic, so the 3-point rule of the reference tests
+// (test_parallel_p1_inner_product_2d.F90:59-68) equals the closed form
+// int_c f g = |c|/12 (sum_v f_v g_v + (sum_v f_v)(sum_v g_v)) over c's
+// vertex values; this evaluates the latter (about half the operations; sums
+// may use fused multiply-adds: the mass is compared within 1e-12, not
+// bitwise).  3D: the reference's 4-point rule as written.
 template <int DIM>
 __device__ __forceinline__ void mass_element(const double* el, double meas, const double* affa,
                                              const double* affb, double* blk) {
+  if constexpr (DIM == 2) {
+    double fa[3][3], fb[3][3];  // [vertex][basis]
+#pragma unroll
+    for (int v = 0; v < 3; v++) {
+      basis<2>(affa, el + 2 * v, fa[v]);
+      basis<2>(affb, el + 2 * v, fb[v]);
+    }
+    const double w = meas * (1.0 / 12.0);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      const double sa = fa[0][i] + fa[1][i] + fa[2][i];
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        const double sb = fb[0][j] + fb[1][j] + fb[2][j];
+        double d = __builtin_fma(fa[0][i], fb[0][j], sa * sb);
+        d = __builtin_fma(fa[1][i], fb[1][j], d);
+        d = __builtin_fma(fa[2][i], fb[2][j], d);
+        blk[i * 3 + j] = __builtin_fma(w, d, blk[i * 3 + j]);
+      }
+    }
+    return;
+  }
   constexpr int LOC = DIM + 1;
   constexpr int NQ = DIM == 2 ? 3 : 4;
   const double wq = DIM == 2 ? 1.0 / 3.0 : 0.25;
