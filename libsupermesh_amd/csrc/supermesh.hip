@@ -1683,16 +1683,19 @@ k_elem_nonempty(long ne, const long* __restrict__ indptr, const int* __restrict_
   out[e] = c;
 }
 
+// Moves each row's reduced (column, value) list from its slot range to the
+// CSR: 16 lanes per row (rows hold a few dozen columns).
+constexpr int kCopyLanes = 16;
 __global__ void __launch_bounds__(256)
 k_mass_copy(long nn, const long* __restrict__ raw_off, const long* __restrict__ mptr,
             const int* __restrict__ tkey, const double* __restrict__ tval, int* __restrict__ mind,
             double* __restrict__ mval) {
-  const long i = (long)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const long i = ((long)blockIdx.x * blockDim.x + threadIdx.x) / kCopyLanes;
   if (i >= nn) return;
-  const long n = mptr[i + 1] - mptr[i];
-  for (long t = threadIdx.x & 63; t < n; t += 64) {
-    mind[mptr[i] + t] = tkey[raw_off[i] + t];
-    mval[mptr[i] + t] = tval[raw_off[i] + t];
+  const long o = mptr[i], n = mptr[i + 1] - o, r = raw_off[i];
+  for (long t = threadIdx.x % kCopyLanes; t < n; t += kCopyLanes) {
+    mind[o + t] = tkey[r + t];
+    mval[o + t] = tval[r + t];
   }
 }
 
@@ -1820,18 +1823,22 @@ constexpr int kMaxDistinct = 256;  // columns a row may hold on this path
 // than kMaxDistinct columns.  One wave; `ht` has kHashSlots entries.
 __device__ __forceinline__ int row_reduce(const int* rk, const double* rv, int n, int* ht, int* dk, int lane,
                                           int* __restrict__ tkey, double* __restrict__ tval, long o) {
-  for (int t = lane; t < kHashSlots; t += 64) ht[t] = -1;
+  // table of 128, 256 or 512 slots: at least 4/3 of the entries (n is the
+  // same on every lane), so short rows clear and scan less
+  const int tbits = n <= 96 ? 7 : (n <= 192 ? 8 : 9);
+  const int tsize = 1 << tbits;
+  for (int t = lane; t < tsize; t += 64) ht[t] = -1;
   __syncthreads();
   bool full = false;
   for (int t = lane; t < n; t += 64) {
     const int key = rk[t];
-    unsigned h = ((unsigned)key * 2654435761u) >> (32 - 9);
+    unsigned h = ((unsigned)key * 2654435761u) >> (32 - tbits);
     int tries = 0;
     while (true) {
       const int prev = atomicCAS(&ht[h], -1, key);
       if (prev == -1 || prev == key) break;
-      h = (h + 1) & (kHashSlots - 1);
-      if (++tries >= kHashSlots) {
+      h = (h + 1) & (tsize - 1);
+      if (++tries >= tsize) {
         full = true;
         break;
       }
@@ -1839,7 +1846,7 @@ __device__ __forceinline__ int row_reduce(const int* rk, const double* rv, int n
   }
   __syncthreads();
   int nd = 0;
-  for (int s0 = 0; s0 < kHashSlots; s0 += 64) {
+  for (int s0 = 0; s0 < tsize; s0 += 64) {
     const int key = ht[s0 + lane];
     const bool occ = key != -1;
     const unsigned long long m = __ballot(occ);
@@ -2134,8 +2141,8 @@ static void assemble_mass_slots(Context& C, const DevMesh& A, const DevMesh& B, 
   const long nnz = C.hstat[3];
   int* mind = C.mindices.as<int>(nnz);
   double* mval = C.uvals.as<double>(nnz);
-  hipLaunchKernelGGL(k_mass_copy, dim3(grid_for(nn, 4)), dim3(256), 0, C.stream, nn, raw_off, mindptr, tkey, tval,
-                     mind, mval);
+  hipLaunchKernelGGL(k_mass_copy, dim3(grid_for(nn, 256 / kCopyLanes)), dim3(256), 0, C.stream, nn, raw_off,
+                     mindptr, tkey, tval, mind, mval);
   LSM_HIP(hipGetLastError());
   out->mass_nnz = nnz;
   out->mass_indices = mind;
@@ -2317,7 +2324,7 @@ void assemble_mass(Context& C, const DevMesh& A, const DevMesh& B, long npairs,
   LSM_HIP(hipStreamSynchronize(C.stream));
   int* mind = C.mindices.as<int>(nnz);
   double* mval = C.uvals.as<double>(nnz);
-  hipLaunchKernelGGL(k_mass_copy, dim3(grid_for(nn, 4)), dim3(256), 0, C.stream, nn, raw_off, mindptr, tkey,
+  hipLaunchKernelGGL(k_mass_copy, dim3(grid_for(nn, 256 / kCopyLanes)), dim3(256), 0, C.stream, nn, raw_off, mindptr, tkey,
                      tval, mind, mval);
   LSM_HIP(hipGetLastError());
   LSM_HIP(hipStreamSynchronize(C.stream));
