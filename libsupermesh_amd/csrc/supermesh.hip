@@ -1866,7 +1866,18 @@ __device__ __forceinline__ int row_reduce(const int* rk, const double* rv, int n
   const int n8 = (n + 7) & ~7;
   if (nd <= 64) {
     wave_bitonic_keys<1>(key, lane);
-    for (int t = 0; t < n8; t += 8) {
+    // few columns: the entries split into G consecutive parts; lane group g
+    // sums part g for the column of rank lane % (64 / G), and the parts are
+    // then added in part order -- a fixed order, so still reproducible
+    const int G = nd <= 16 ? 4 : (nd <= 32 ? 2 : 1);
+    const int W = 64 / G;
+    const int r = lane % W;
+    const int mykey = __shfl(key[0], r, 64);
+    const int chunk = ((n8 / 8 + G - 1) / G) * 8;
+    const int t0 = (lane / W) * chunk;
+    const int t1 = t0 + chunk < n8 ? t0 + chunk : n8;
+    double part = 0.0;
+    for (int t = t0; t < t1; t += 8) {
       const int4 k0 = *reinterpret_cast<const int4*>(rk + t), k1 = *reinterpret_cast<const int4*>(rk + t + 4);
       const double2 v0 = *reinterpret_cast<const double2*>(rv + t);
       const double2 v1 = *reinterpret_cast<const double2*>(rv + t + 2);
@@ -1876,8 +1887,11 @@ __device__ __forceinline__ int row_reduce(const int* rk, const double* rv, int n
       const double vv[8] = {v0.x, v0.y, v1.x, v1.y, v2.x, v2.y, v3.x, v3.y};
 #pragma unroll
       for (int u = 0; u < 8; u++)
-        if (kk[u] == key[0]) acc[0] = acc[0] + vv[u];
+        if (kk[u] == mykey) part = part + vv[u];
     }
+    double tot = part;
+    for (int q = 1; q < G; q++) tot = tot + __shfl(part, r + q * W, 64);
+    acc[0] = tot;  // lanes below W (all of rank < nd among them) hold their column's total
   } else {
     wave_bitonic_keys<4>(key, lane);
     for (int t = 0; t < n8; t += 8) {
