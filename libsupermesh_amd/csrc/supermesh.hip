@@ -1869,7 +1869,7 @@ __device__ __forceinline__ int row_reduce(const int* rk, const double* rv, int n
     // few columns: the entries split into G consecutive parts; lane group g
     // sums part g for the column of rank lane % (64 / G), and the parts are
     // then added in part order -- a fixed order, so still reproducible
-    const int G = nd <= 16 ? 4 : (nd <= 32 ? 2 : 1);
+    const int G = nd <= 8 ? 8 : 64 / nd;
     const int W = 64 / G;
     const int r = lane % W;
     const int mykey = __shfl(key[0], r, 64);
