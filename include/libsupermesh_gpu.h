@@ -132,7 +132,7 @@ int libsupermesh_gpu_supermesh(libsupermesh_gpu_context* ctx, const libsupermesh
                                libsupermesh_gpu_result* out);
 
 /* The same on meshes of any element type: loc_a / loc_b are dim+1 (triangles,
- * tetrahedra) or 2^dim (quadrilaterals, hexahedra, convex, nodes in the
+ * tetrahedra) or 2^dim (quadrilaterals, hexahedra; nodes in the
  * reference's order), and each pair is clipped as intersect_elements does
  * (Supermesh.F90:162-209): tri x quad as intersect_tri_quad, hex x hex as
  * intersect_hexes (Supermesh.F90:303-334), and so on.  The output elements are
@@ -142,6 +142,23 @@ int libsupermesh_gpu_supermesh(libsupermesh_gpu_context* ctx, const libsupermesh
 int libsupermesh_gpu_supermesh_mixed(libsupermesh_gpu_context* ctx, const libsupermesh_gpu_mesh* a,
                                      int loc_a, const libsupermesh_gpu_mesh* b, int loc_b, int finder,
                                      int flags, libsupermesh_gpu_result* out);
+
+/* A search hierarchy over mesh B kept on the device for queries one element
+ * at a time: the reference's tree_type / rtree_type allocate, query and
+ * deallocate (src/Intersection_Finder.F90:481-541, 645-712), which the
+ * Fortran module's stateful *_intersection_finder_set_input / _find /
+ * _query_output / _get_output / _reset are built on.  finder =
+ * LIBSUPERMESH_GPU_FINDER_TREE (strict overlap) or _RTREE (closed); host
+ * arrays, enlist long[nelements][loc] 0-based.  A query of one element
+ * (element double[loc][dim]) returns the count; _result copies the B ids,
+ * ascending, 0-based.  Each query is a few device launches: for many
+ * elements use libsupermesh_gpu_intersection_finder. */
+typedef struct libsupermesh_gpu_tree libsupermesh_gpu_tree;
+int libsupermesh_gpu_tree_create(int finder, int dim, long nnodes, const double* positions, long nelements,
+                                 int loc, const long* enlist, libsupermesh_gpu_tree** tree);
+int libsupermesh_gpu_tree_query(libsupermesh_gpu_tree* tree, int loc, const double* element, long* neles);
+int libsupermesh_gpu_tree_result(libsupermesh_gpu_tree* tree, long neles, long* eles);
+void libsupermesh_gpu_tree_destroy(libsupermesh_gpu_tree* tree);
 
 /* A batch of independent meshes of one dimension, concatenated: mesh m owns
  * elements element_offsets[m] .. element_offsets[m+1]-1 (device array,

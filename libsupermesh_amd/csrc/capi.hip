@@ -24,6 +24,15 @@ struct libsupermesh_gpu_context {
   lsm::Context ctx;
 };
 
+// A search hierarchy kept on the device for one-element queries: B staged
+// and its hierarchy built once, in a context of its own.
+struct libsupermesh_gpu_tree {
+  lsm::Context ctx;
+  int finder = 0;
+  int dim = 0;
+  std::vector<int> result;
+};
+
 namespace lsm {
 
 static thread_local int tl_code = 0;
@@ -948,6 +957,70 @@ void libsupermesh_sort_intersection_finder_query_output(long* nindices) {
 void libsupermesh_sort_intersection_finder_get_output(long* nelements, long* nindices, long* indices,
                                                       long* ind_ptr) {
   guarded([&] { finder_get_output(g_sort, *nelements, *nindices, indices, ind_ptr); });
+}
+
+int libsupermesh_gpu_tree_create(int finder, int dim, long nnodes, const double* positions, long nelements,
+                                 int loc, const long* enlist, libsupermesh_gpu_tree** tree) {
+  return guarded([&] {
+    LSM_REQUIRE(tree != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID, "null tree");
+    *tree = nullptr;
+    LSM_REQUIRE(finder == LIBSUPERMESH_GPU_FINDER_TREE || finder == LIBSUPERMESH_GPU_FINDER_RTREE,
+                LIBSUPERMESH_GPU_ERROR_INVALID, "unknown finder");
+    auto t = std::make_unique<libsupermesh_gpu_tree>();
+    t->finder = finder;
+    t->dim = dim;
+    Context& C = t->ctx;
+    init_context(C, 0);
+    try {
+      libsupermesh_gpu_mesh vb;
+      host_mesh_to_device(C, C.mesh_b, nnodes, dim, nelements, loc, positions, enlist, vb);
+      stage_mesh(C, C.mesh_b, dim, loc, nnodes, nelements, vb.positions, vb.enlist, false, false);
+      build_bvh(C, C.mesh_b);
+      LSM_HIP(hipStreamSynchronize(C.stream));
+    } catch (...) {
+      free_context(C);
+      throw;
+    }
+    *tree = t.release();
+  });
+}
+
+int libsupermesh_gpu_tree_query(libsupermesh_gpu_tree* tree, int loc, const double* element, long* neles) {
+  return guarded([&] {
+    LSM_REQUIRE(tree != nullptr && neles != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID, "null argument");
+    *neles = 0;
+    tree->result.clear();
+    Context& C = tree->ctx;
+    LSM_HIP(hipSetDevice(C.device));
+    const int dim = tree->dim;
+    std::vector<long> enl((size_t)loc);
+    for (int v = 0; v < loc; v++) enl[v] = v;
+    libsupermesh_gpu_mesh va;
+    host_mesh_to_device(C, C.mesh_a, loc, dim, 1, loc, element, enl.data(), va);
+    stage_mesh(C, C.mesh_a, dim, loc, loc, 1, va.positions, va.enlist, false, false);
+    const long n = find_pairs(C, C.mesh_a, C.mesh_b, tree->finder, LIBSUPERMESH_GPU_SORTED);
+    tree->result.resize((size_t)n);
+    if (n > 0)
+      LSM_HIP(hipMemcpyAsync(tree->result.data(), C.indices.ptr, sizeof(int) * n, hipMemcpyDeviceToHost,
+                             C.stream));
+    LSM_HIP(hipStreamSynchronize(C.stream));
+    *neles = n;
+  });
+}
+
+int libsupermesh_gpu_tree_result(libsupermesh_gpu_tree* tree, long neles, long* eles) {
+  return guarded([&] {
+    LSM_REQUIRE(tree != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID, "null tree");
+    LSM_REQUIRE(neles == (long)tree->result.size() && (neles == 0 || eles != nullptr),
+                LIBSUPERMESH_GPU_ERROR_INVALID, "result size differs from the last query's");
+    for (long i = 0; i < neles; i++) eles[i] = tree->result[i];
+  });
+}
+
+void libsupermesh_gpu_tree_destroy(libsupermesh_gpu_tree* tree) {
+  if (!tree) return;
+  free_context(tree->ctx);
+  delete tree;
 }
 
 void libsupermesh_advancing_front_intersection_finder_set_input(

@@ -26,9 +26,52 @@ module libsupermesh
 
   public :: tree_intersection_finder, rtree_intersection_finder, &
     & sort_intersection_finder, advancing_front_intersection_finder, &
-    & intersection_finder, intersect_tris, intersect_tets, &
+    & intersection_finder, brute_force_intersection_finder, intersect_tris, intersect_tets, &
     & triangle_area, tetrahedron_volume, intersect_intervals, interval_size, &
     & intersect_tri_quad, intersect_tet_hex, supermesh, libsupermesh_gpu_error
+  public :: intersections, tree_type, rtree_type, allocate, query, deallocate
+  public :: tree_intersection_finder_set_input, tree_intersection_finder_find, &
+    & tree_intersection_finder_query_output, tree_intersection_finder_get_output, &
+    & tree_intersection_finder_reset
+  public :: rtree_intersection_finder_set_input, rtree_intersection_finder_find, &
+    & rtree_intersection_finder_query_output, rtree_intersection_finder_get_output, &
+    & rtree_intersection_finder_reset
+
+  ! Ragged candidate lists, one per A element (Intersections.F90:35-40).
+  type intersections
+    integer, dimension(:), pointer :: v => null()
+    integer :: n = 0
+  end type intersections
+
+  ! A search hierarchy over one mesh, kept on the device
+  ! (libsupermesh_gpu_tree; Intersection_Finder.F90:164-172).
+  type tree_type
+    integer :: dim = 0
+    type(c_ptr) :: handle = c_null_ptr
+  end type tree_type
+
+  type rtree_type
+    type(c_ptr) :: handle = c_null_ptr
+  end type rtree_type
+
+  interface allocate
+    module procedure allocate_tree, allocate_rtree
+  end interface allocate
+
+  interface query
+    module procedure query_tree_allocatable, query_tree_pointer, query_rtree_allocatable, &
+      & query_rtree_pointer
+  end interface query
+
+  interface deallocate
+    module procedure deallocate_tree, deallocate_rtree, deallocate_intersections, &
+      & deallocate_intersections_rank_1
+  end interface deallocate
+
+  ! state of the set_input / find / query_output / get_output / reset calls
+  type(tree_type), save :: state_tree
+  type(rtree_type), save :: state_rtree
+  integer, dimension(:), allocatable, save :: state_tree_eles, state_rtree_eles
 
   ! The per-pair callback of supermesh: the interface of the reference's
   ! intersection_calculation (Parallel_Supermesh.F90:97-113).
@@ -56,12 +99,17 @@ module libsupermesh
   type(c_ptr), save :: gpu_context = c_null_ptr
 
   interface tree_intersection_finder
-    module procedure tree_intersection_finder_csr_sparsity
+    module procedure tree_intersection_finder_csr_sparsity, tree_intersection_finder_intersections
   end interface tree_intersection_finder
 
   interface rtree_intersection_finder
-    module procedure rtree_intersection_finder_csr_sparsity
+    module procedure rtree_intersection_finder_csr_sparsity, rtree_intersection_finder_intersections
   end interface rtree_intersection_finder
+
+  interface brute_force_intersection_finder
+    module procedure brute_force_intersection_finder_csr_sparsity, &
+      & brute_force_intersection_finder_intersections
+  end interface brute_force_intersection_finder
 
   interface intersect_tris
     module procedure intersect_tris_real
@@ -72,15 +120,16 @@ module libsupermesh
   end interface intersect_tets
 
   interface sort_intersection_finder
-    module procedure sort_intersection_finder_csr_sparsity
+    module procedure sort_intersection_finder_csr_sparsity, sort_intersection_finder_intersections
   end interface sort_intersection_finder
 
   interface advancing_front_intersection_finder
-    module procedure advancing_front_intersection_finder_csr_sparsity
+    module procedure advancing_front_intersection_finder_csr_sparsity, &
+      & advancing_front_intersection_finder_intersections
   end interface advancing_front_intersection_finder
 
   interface intersection_finder
-    module procedure intersection_finder_csr_sparsity
+    module procedure intersection_finder_csr_sparsity, intersection_finder_intersections
   end interface intersection_finder
 
   ! finders behind finder_csr
@@ -254,6 +303,35 @@ module libsupermesh
       import :: c_int
       integer(kind = c_int) :: c_last_error_code
     end function c_last_error_code
+    function c_tree_create(finder, dim, nnodes, positions, nelements, loc, enlist, tree) &
+        & bind(c, name = "libsupermesh_gpu_tree_create")
+      import :: c_int, c_long, c_double, c_ptr
+      integer(kind = c_int), value :: finder, dim, loc
+      integer(kind = c_long), value :: nnodes, nelements
+      real(kind = c_double), dimension(*) :: positions
+      integer(kind = c_long), dimension(*) :: enlist
+      type(c_ptr) :: tree
+      integer(kind = c_int) :: c_tree_create
+    end function c_tree_create
+    function c_tree_query(tree, loc, element, neles) bind(c, name = "libsupermesh_gpu_tree_query")
+      import :: c_int, c_long, c_double, c_ptr
+      type(c_ptr), value :: tree
+      integer(kind = c_int), value :: loc
+      real(kind = c_double), dimension(*) :: element
+      integer(kind = c_long) :: neles
+      integer(kind = c_int) :: c_tree_query
+    end function c_tree_query
+    function c_tree_result(tree, neles, eles) bind(c, name = "libsupermesh_gpu_tree_result")
+      import :: c_int, c_long, c_ptr
+      type(c_ptr), value :: tree
+      integer(kind = c_long), value :: neles
+      integer(kind = c_long), dimension(*) :: eles
+      integer(kind = c_int) :: c_tree_result
+    end function c_tree_result
+    subroutine c_tree_destroy(tree) bind(c, name = "libsupermesh_gpu_tree_destroy")
+      import :: c_ptr
+      type(c_ptr), value :: tree
+    end subroutine c_tree_destroy
   end interface
 
 contains
@@ -581,5 +659,266 @@ contains
     t = tet
     call c_tetrahedron_volume(t, volume)
   end function tetrahedron_volume
+
+
+  ! ---------------------------------------------------------------------
+  ! Device trees for one-element queries (Intersection_Finder.F90:481-760).
+  ! Results are ascending B element ids (1-based).
+  ! ---------------------------------------------------------------------
+
+  subroutine tree_create(finder, positions, enlist, handle)
+    integer, intent(in) :: finder
+    real(kind = real_kind), dimension(:, :), intent(in) :: positions
+    integer, dimension(:, :), intent(in) :: enlist
+    type(c_ptr), intent(out) :: handle
+    integer(kind = c_long), dimension(:, :), allocatable :: e
+    real(kind = c_double), dimension(:, :), allocatable :: p
+    integer(kind = c_int) :: rc
+    allocate(p(size(positions, 1), size(positions, 2)), e(size(enlist, 1), size(enlist, 2)))
+    p = positions
+    e = enlist - 1
+    handle = c_null_ptr
+    rc = c_tree_create(int(finder, c_int), int(size(positions, 1), c_int), int(size(positions, 2), c_long), &
+      & p, int(size(enlist, 2), c_long), int(size(enlist, 1), c_int), e, handle)
+  end subroutine tree_create
+
+  subroutine tree_query(handle, element_a, eles_b)
+    type(c_ptr), intent(in) :: handle
+    real(kind = real_kind), dimension(:, :), intent(in) :: element_a
+    integer, dimension(:), intent(out), allocatable :: eles_b
+    real(kind = c_double), dimension(size(element_a, 1), size(element_a, 2)) :: el
+    integer(kind = c_long), dimension(:), allocatable :: r
+    integer(kind = c_long) :: n
+    integer(kind = c_int) :: rc
+    el = element_a
+    n = 0
+    if(c_associated(handle)) rc = c_tree_query(handle, int(size(element_a, 2), c_int), el, n)
+    allocate(r(max(n, 1_c_long)))
+    if(n > 0) rc = c_tree_result(handle, n, r)
+    allocate(eles_b(n))
+    eles_b = int(r(:n)) + 1
+  end subroutine tree_query
+
+  subroutine allocate_tree(tree, positions, enlist)
+    type(tree_type), intent(out) :: tree
+    real(kind = real_kind), dimension(:, :), intent(in) :: positions
+    integer, dimension(:, :), intent(in) :: enlist
+    tree%dim = size(positions, 1)
+    call tree_create(0, positions, enlist, tree%handle)
+  end subroutine allocate_tree
+
+  subroutine allocate_rtree(rtree, positions, enlist)
+    type(rtree_type), intent(out) :: rtree
+    real(kind = real_kind), dimension(:, :), intent(in) :: positions
+    integer, dimension(:, :), intent(in) :: enlist
+    call tree_create(1, positions, enlist, rtree%handle)
+  end subroutine allocate_rtree
+
+  subroutine query_tree_allocatable(tree, element_a, eles_b)
+    type(tree_type), intent(inout) :: tree
+    real(kind = real_kind), dimension(:, :), intent(in) :: element_a
+    integer, dimension(:), allocatable, intent(out) :: eles_b
+    call tree_query(tree%handle, element_a, eles_b)
+  end subroutine query_tree_allocatable
+
+  subroutine query_tree_pointer(tree, element_a, eles_b)
+    type(tree_type), intent(inout) :: tree
+    real(kind = real_kind), dimension(:, :), intent(in) :: element_a
+    integer, dimension(:), pointer, intent(out) :: eles_b
+    integer, dimension(:), allocatable :: r
+    call tree_query(tree%handle, element_a, r)
+    allocate(eles_b(size(r)))
+    eles_b = r
+  end subroutine query_tree_pointer
+
+  subroutine query_rtree_allocatable(rtree, element_a, eles_b)
+    type(rtree_type), intent(inout) :: rtree
+    real(kind = real_kind), dimension(:, :), intent(in) :: element_a
+    integer, dimension(:), allocatable, intent(out) :: eles_b
+    call tree_query(rtree%handle, element_a, eles_b)
+  end subroutine query_rtree_allocatable
+
+  subroutine query_rtree_pointer(rtree, element_a, eles_b)
+    type(rtree_type), intent(inout) :: rtree
+    real(kind = real_kind), dimension(:, :), intent(in) :: element_a
+    integer, dimension(:), pointer, intent(out) :: eles_b
+    integer, dimension(:), allocatable :: r
+    call tree_query(rtree%handle, element_a, r)
+    allocate(eles_b(size(r)))
+    eles_b = r
+  end subroutine query_rtree_pointer
+
+  subroutine deallocate_tree(tree)
+    type(tree_type), intent(inout) :: tree
+    if(c_associated(tree%handle)) call c_tree_destroy(tree%handle)
+    tree%handle = c_null_ptr
+  end subroutine deallocate_tree
+
+  subroutine deallocate_rtree(rtree)
+    type(rtree_type), intent(inout) :: rtree
+    if(c_associated(rtree%handle)) call c_tree_destroy(rtree%handle)
+    rtree%handle = c_null_ptr
+  end subroutine deallocate_rtree
+
+  subroutine tree_intersection_finder_set_input(positions, enlist)
+    real(kind = real_kind), dimension(:, :), intent(in) :: positions
+    integer, dimension(:, :), intent(in) :: enlist
+    call tree_intersection_finder_reset()
+    call allocate(state_tree, positions, enlist)
+  end subroutine tree_intersection_finder_set_input
+
+  subroutine tree_intersection_finder_find(element_a)
+    real(kind = real_kind), dimension(:, :), intent(in) :: element_a
+    if(allocated(state_tree_eles)) deallocate(state_tree_eles)
+    call query(state_tree, element_a, state_tree_eles)
+  end subroutine tree_intersection_finder_find
+
+  subroutine tree_intersection_finder_query_output(neles_b)
+    integer, intent(out) :: neles_b
+    neles_b = 0
+    if(allocated(state_tree_eles)) neles_b = size(state_tree_eles)
+  end subroutine tree_intersection_finder_query_output
+
+  subroutine tree_intersection_finder_get_output(ele_b, i)
+    integer, intent(out) :: ele_b
+    integer, intent(in) :: i
+    ele_b = state_tree_eles(i)
+  end subroutine tree_intersection_finder_get_output
+
+  subroutine tree_intersection_finder_reset()
+    if(allocated(state_tree_eles)) deallocate(state_tree_eles)
+    call deallocate(state_tree)
+  end subroutine tree_intersection_finder_reset
+
+  subroutine rtree_intersection_finder_set_input(positions, enlist)
+    real(kind = real_kind), dimension(:, :), intent(in) :: positions
+    integer, dimension(:, :), intent(in) :: enlist
+    call rtree_intersection_finder_reset()
+    call allocate(state_rtree, positions, enlist)
+  end subroutine rtree_intersection_finder_set_input
+
+  subroutine rtree_intersection_finder_find(element_a)
+    real(kind = real_kind), dimension(:, :), intent(in) :: element_a
+    if(allocated(state_rtree_eles)) deallocate(state_rtree_eles)
+    call query(state_rtree, element_a, state_rtree_eles)
+  end subroutine rtree_intersection_finder_find
+
+  subroutine rtree_intersection_finder_query_output(neles_b)
+    integer, intent(out) :: neles_b
+    neles_b = 0
+    if(allocated(state_rtree_eles)) neles_b = size(state_rtree_eles)
+  end subroutine rtree_intersection_finder_query_output
+
+  subroutine rtree_intersection_finder_get_output(ele_b, i)
+    integer, intent(out) :: ele_b
+    integer, intent(in) :: i
+    ele_b = state_rtree_eles(i)
+  end subroutine rtree_intersection_finder_get_output
+
+  subroutine rtree_intersection_finder_reset()
+    if(allocated(state_rtree_eles)) deallocate(state_rtree_eles)
+    call deallocate(state_rtree)
+  end subroutine rtree_intersection_finder_reset
+
+  ! ---------------------------------------------------------------------
+  ! Ragged (intersections) forms of the finders: the CSR result split per
+  ! A element (Intersections.F90:48-71 run backwards).
+  ! ---------------------------------------------------------------------
+
+  subroutine csr_to_intersections(indices, indptr, map_ab)
+    integer, dimension(:), intent(in) :: indices, indptr
+    type(intersections), dimension(:), intent(out) :: map_ab
+    integer :: a, n
+    do a = 1, size(map_ab)
+      n = indptr(a + 1) - indptr(a)
+      allocate(map_ab(a)%v(n))
+      map_ab(a)%v = indices(indptr(a):indptr(a + 1) - 1)
+      map_ab(a)%n = n
+    end do
+  end subroutine csr_to_intersections
+
+  subroutine deallocate_intersections(ints)
+    type(intersections), intent(inout) :: ints
+    if(associated(ints%v)) deallocate(ints%v)
+    ints%n = 0
+  end subroutine deallocate_intersections
+
+  subroutine deallocate_intersections_rank_1(ints)
+    type(intersections), dimension(:), intent(inout) :: ints
+    integer :: i
+    do i = 1, size(ints)
+      call deallocate_intersections(ints(i))
+    end do
+  end subroutine deallocate_intersections_rank_1
+
+  subroutine ragged(kind, positions_a, enlist_a, positions_b, enlist_b, map_ab)
+    integer, intent(in) :: kind
+    real(kind = real_kind), dimension(:, :), intent(in) :: positions_a, positions_b
+    integer, dimension(:, :), intent(in) :: enlist_a, enlist_b
+    type(intersections), dimension(:), intent(out) :: map_ab
+    integer, dimension(:), allocatable :: indices
+    integer, dimension(size(enlist_a, 2) + 1) :: indptr
+    call finder_csr(kind, positions_a, enlist_a, positions_b, enlist_b, indices, indptr)
+    call csr_to_intersections(indices, indptr, map_ab)
+  end subroutine ragged
+
+  subroutine tree_intersection_finder_intersections(positions_a, enlist_a, positions_b, enlist_b, map_ab)
+    real(kind = real_kind), dimension(:, :), intent(in) :: positions_a, positions_b
+    integer, dimension(:, :), intent(in) :: enlist_a, enlist_b
+    type(intersections), dimension(:), intent(out) :: map_ab
+    call ragged(finder_tree, positions_a, enlist_a, positions_b, enlist_b, map_ab)
+  end subroutine tree_intersection_finder_intersections
+
+  subroutine rtree_intersection_finder_intersections(positions_a, enlist_a, positions_b, enlist_b, map_ab)
+    real(kind = real_kind), dimension(:, :), intent(in) :: positions_a, positions_b
+    integer, dimension(:, :), intent(in) :: enlist_a, enlist_b
+    type(intersections), dimension(:), intent(out) :: map_ab
+    call ragged(finder_rtree, positions_a, enlist_a, positions_b, enlist_b, map_ab)
+  end subroutine rtree_intersection_finder_intersections
+
+  subroutine sort_intersection_finder_intersections(positions_a, enlist_a, positions_b, enlist_b, map_ab)
+    real(kind = real_kind), dimension(:, :), intent(in) :: positions_a, positions_b
+    integer, dimension(:, :), intent(in) :: enlist_a, enlist_b
+    type(intersections), dimension(:), intent(out) :: map_ab
+    call ragged(finder_sort, positions_a, enlist_a, positions_b, enlist_b, map_ab)
+  end subroutine sort_intersection_finder_intersections
+
+  subroutine advancing_front_intersection_finder_intersections(positions_a, enlist_a, positions_b, &
+      & enlist_b, map_ab)
+    real(kind = real_kind), dimension(:, :), intent(in) :: positions_a, positions_b
+    integer, dimension(:, :), intent(in) :: enlist_a, enlist_b
+    type(intersections), dimension(:), intent(out) :: map_ab
+    call ragged(finder_front, positions_a, enlist_a, positions_b, enlist_b, map_ab)
+  end subroutine advancing_front_intersection_finder_intersections
+
+  subroutine intersection_finder_intersections(positions_a, enlist_a, positions_b, enlist_b, map_ab)
+    real(kind = real_kind), dimension(:, :), intent(in) :: positions_a, positions_b
+    integer, dimension(:, :), intent(in) :: enlist_a, enlist_b
+    type(intersections), dimension(:), intent(out) :: map_ab
+    integer :: kind
+    kind = finder_rtree
+    if(size(positions_a, 1) == 1) kind = finder_sort
+    call ragged(kind, positions_a, enlist_a, positions_b, enlist_b, map_ab)
+  end subroutine intersection_finder_intersections
+
+  ! Brute force (Intersection_Finder.F90:990-1050): the strict-overlap set,
+  ! rows ascending -- the tree finder's result exactly.
+  subroutine brute_force_intersection_finder_csr_sparsity(positions_a, enlist_a, positions_b, enlist_b, &
+      & map_ab_indices, map_ab_indptr)
+    real(kind = real_kind), dimension(:, :), intent(in) :: positions_a, positions_b
+    integer, dimension(:, :), intent(in) :: enlist_a, enlist_b
+    integer, dimension(:), allocatable, intent(out) :: map_ab_indices
+    integer, dimension(:), intent(out) :: map_ab_indptr
+    call finder_csr(finder_tree, positions_a, enlist_a, positions_b, enlist_b, map_ab_indices, &
+      & map_ab_indptr)
+  end subroutine brute_force_intersection_finder_csr_sparsity
+
+  subroutine brute_force_intersection_finder_intersections(positions_a, enlist_a, positions_b, enlist_b, &
+      & map_ab)
+    real(kind = real_kind), dimension(:, :), intent(in) :: positions_a, positions_b
+    integer, dimension(:, :), intent(in) :: enlist_a, enlist_b
+    type(intersections), dimension(:), intent(out) :: map_ab
+    call ragged(finder_tree, positions_a, enlist_a, positions_b, enlist_b, map_ab)
+  end subroutine brute_force_intersection_finder_intersections
 
 end module libsupermesh

@@ -8,6 +8,10 @@
 !   mode "mixed": simplex mesh A against quad / hex mesh B (R-tree finder,
 !     intersect_tri_quad / intersect_tet_hex), or 1D meshes (sort finder,
 !     intersect_intervals);
+!   mode "tree_api": one-element queries through the stateful tree finder
+!     (set_input / find / query_output / get_output / reset) and an
+!     rtree_type (allocate / query / deallocate), checked against the CSR
+!     finders, plus the ragged brute force form; prints the pair count;
 !   mode "front" / "finder": advancing_front_intersection_finder /
 !     intersection_finder only; the third number printed is then a checksum
 !     of the CSR, sum of indices(i) * i plus sum of indptr(i) * (i + 7).
@@ -64,6 +68,9 @@ program test_intersector_gpu
   real(kind = real_kind), dimension(2) :: interval_c
   integer :: a, i, b, c, n
   integer(kind = 8) :: chk
+  integer, dimension(:), allocatable :: indices2, indptr2, eles
+  type(intersections), dimension(:), allocatable :: map_ab
+  type(rtree_type) :: rt
 
   call get_command_argument(1, fa)
   call get_command_argument(2, fb)
@@ -111,6 +118,45 @@ program test_intersector_gpu
       end do
     end do
     call check_error()
+  case("tree_api")
+    allocate(indptr(size(ea, 2) + 1), indptr2(size(ea, 2) + 1), map_ab(size(ea, 2)))
+    call tree_intersection_finder(pa, ea, pb, eb, indices, indptr)
+    call rtree_intersection_finder(pa, ea, pb, eb, indices2, indptr2)
+    call brute_force_intersection_finder(pa, ea, pb, eb, map_ab)
+    call check_error()
+    call tree_intersection_finder_set_input(pb, eb)
+    call allocate(rt, pb, eb)
+    call check_error()
+    do a = 1, size(ea, 2)
+      call tree_intersection_finder_find(pa(:, ea(:, a)))
+      call tree_intersection_finder_query_output(n)
+      if(n /= indptr(a + 1) - indptr(a) .or. map_ab(a)%n /= n) then
+        print "(a,i0)", "error: tree row length, element ", a
+        stop 1
+      end if
+      do i = 1, n
+        call tree_intersection_finder_get_output(b, i)
+        if(b /= indices(indptr(a) + i - 1) .or. map_ab(a)%v(i) /= b) then
+          print "(a,i0)", "error: tree row entry, element ", a
+          stop 1
+        end if
+      end do
+      call query(rt, pa(:, ea(:, a)), eles)
+      if(size(eles) /= indptr2(a + 1) - indptr2(a)) then
+        print "(a,i0)", "error: rtree row length, element ", a
+        stop 1
+      end if
+      if(any(eles /= indices2(indptr2(a):indptr2(a + 1) - 1))) then
+        print "(a,i0)", "error: rtree row entry, element ", a
+        stop 1
+      end if
+      npairs = npairs + n
+      npieces = npieces + size(eles)
+    end do
+    call check_error()
+    call tree_intersection_finder_reset()
+    call deallocate(rt)
+    call deallocate(map_ab)
   case("front", "finder")
     allocate(indptr(size(ea, 2) + 1))
     if(trim(mode) == "front") then

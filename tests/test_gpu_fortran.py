@@ -101,3 +101,22 @@ def test_fortran_front_and_default_finder(tmp_path, fixture_meshes, mode, a, b):
     ix = np.asarray(ix, dtype=np.int64) + 1
     ref = int(np.sum(ix * np.arange(1, len(ix) + 1)) + np.sum((np.asarray(ip) + 1) * np.arange(8, len(ip) + 8)))
     assert int(n) == len(ix) and int(chk) == ref
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran interface not built (no amdflang)")
+@pytest.mark.parametrize("a,b", [("square.1", "square.2"), ("cube.1", "cube.2"), ("square.1", "square.4")])
+def test_fortran_tree_types_and_stateful_finders(tmp_path, fixture_meshes, a, b):
+    # tree_type / rtree_type allocate-query-deallocate and the stateful
+    # *_intersection_finder_set_input / find / query_output / get_output /
+    # reset (Intersection_Finder.F90:481-760), element by element against the
+    # CSR finders inside the driver; the counts against the oracle here
+    pa, ea = fixture_meshes[a]
+    pb, eb = fixture_meshes[b]
+    fa, fb = tmp_path / "a.bin", tmp_path / "b.bin"
+    write_mesh(fa, pa, ea)
+    write_mesh(fb, pb, eb)
+    r = subprocess.run([EXE, str(fa), str(fb), "tree_api"], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    dim, n_strict, n_closed, _ = r.stdout.split()
+    assert int(n_strict) == len(O.brute_force(pa, ea, pb, eb)[1])
+    assert int(n_closed) == len(O.brute_force(pa, ea, pb, eb, closed=True)[1])
