@@ -80,6 +80,12 @@ void libsupermesh_intersect_tet_hex(c_kind* tet_a, c_kind* hex_b, c_kind* tets_c
  * Replaces src/C_bindings.F90:54-64 (include/libsupermesh-c.h:31). */
 void libsupermesh_intersect_tri_quad(c_kind* tri_a, c_kind* quad_b, c_kind* tris_c, int* n_tris_c);
 
+/* Quad-quad and hex-hex intersection (intersect_quads / intersect_hexes,
+ * src/Supermesh.F90:231-248, 303-334; the reference exports them from Fortran
+ * only): tris_c[62][3][2], tets_c[5 * 729][4][3]. */
+void libsupermesh_intersect_quads(c_kind* quad_a, c_kind* quad_b, c_kind* tris_c, int* n_tris_c);
+void libsupermesh_intersect_hexes(c_kind* hex_a, c_kind* hex_b, c_kind* tets_c, int* n_tets_c);
+
 /* The advancing front finder, in the tree finder's set_input / query_output /
  * get_output form.  The reference has it in Fortran only
  * (advancing_front_intersection_finder, src/Intersection_Finder.F90:138-141,

@@ -354,7 +354,7 @@ __global__ void k_one(int kind, const double* __restrict__ in, double* __restric
       k++;
     });
     out[0] = (double)n;
-  } else {  // intersect_tet_hex: tet (12) then hex (24)
+  } else if (kind == 5) {  // intersect_tet_hex: tet (12) then hex (24)
     int k = 0;
     int n = clip_tet_hex(in, in + 12, [&](const double* t, double) {
       if (k < kTetHexBufSize)
@@ -362,11 +362,41 @@ __global__ void k_one(int kind, const double* __restrict__ in, double* __restric
       k++;
     });
     out[0] = (double)n;
+  } else if (kind == 6) {  // intersect_quads: quad A (8) then quad B (8), tol 10 * tiny
+    Line L[4];
+    quad_lines(in + 8, L);
+    int k = 0;
+    int n = clip_poly<4, 4>(in, L, 10.0 * 2.2250738585072014e-308,
+                            [&](double x0, double y0, double x1, double y1, double x2, double y2, double) {
+                              if (k < kQuadsBufSize) {
+                                double* o = out + 1 + 6 * k;
+                                o[0] = x0;
+                                o[1] = y0;
+                                o[2] = x1;
+                                o[3] = y1;
+                                o[4] = x2;
+                                o[5] = y2;
+                              }
+                              k++;
+                            });
+    out[0] = (double)n;
+  } else {  // intersect_hexes: hex A (24) then hex B (24), A's five tets in turn
+    int k = 0;
+    for (int s = 0; s < 5; s++) {
+      double T[12];
+      hex_split_tet(in, s, T);
+      clip_tet_hex(T, in + 24, [&](const double* t, double) {
+        if (k < kHexesBufSize)
+          for (int i = 0; i < 12; i++) out[1 + 12 * k + i] = t[i];
+        k++;
+      });
+    }
+    out[0] = (double)k;
   }
 }
 
-constexpr int kOneIn = 36;                          // largest input: tet + hex
-constexpr int kOneOut = 1 + 12 * kTetHexBufSize;  // largest output: 729 tets
+constexpr int kOneIn = 48;                        // largest input: two hexahedra
+constexpr int kOneOut = 1 + 12 * kHexesBufSize;  // largest output: 5 x 729 tets
 
 // Runs k_one on the default context: nin doubles in, nout doubles back.
 static void one_call(int kind, const double* in, int nin, double* result, int nout) {
@@ -915,6 +945,36 @@ void libsupermesh_intersect_tet_hex(double* tet_a, double* hex_b, double* tets_c
     std::memcpy(in + 12, hex_b, sizeof(double) * 24);
     one_call(5, in, 36, res.data(), 1 + 12 * kTetHexBufSize);
     int n = (int)res[0];
+    *n_tets_c = n;
+    std::memcpy(tets_c, res.data() + 1, sizeof(double) * 12 * n);
+  });
+  if (rc != LIBSUPERMESH_GPU_OK) *n_tets_c = 0;
+}
+
+void libsupermesh_intersect_quads(double* quad_a, double* quad_b, double* tris_c, int* n_tris_c) {
+  int rc = guarded([&] {
+    double in[16];
+    std::memcpy(in, quad_a, sizeof(double) * 8);
+    std::memcpy(in + 8, quad_b, sizeof(double) * 8);
+    std::vector<double> res(1 + 6 * kQuadsBufSize);
+    one_call(6, in, 16, res.data(), (int)res.size());
+    const int n = (int)res[0];
+    LSM_REQUIRE(n <= kQuadsBufSize, LIBSUPERMESH_GPU_ERROR_OVERFLOW, "intersect_quads: more than 62 triangles");
+    *n_tris_c = n;
+    std::memcpy(tris_c, res.data() + 1, sizeof(double) * 6 * n);
+  });
+  if (rc != LIBSUPERMESH_GPU_OK) *n_tris_c = 0;
+}
+
+void libsupermesh_intersect_hexes(double* hex_a, double* hex_b, double* tets_c, int* n_tets_c) {
+  int rc = guarded([&] {
+    double in[48];
+    std::memcpy(in, hex_a, sizeof(double) * 24);
+    std::memcpy(in + 24, hex_b, sizeof(double) * 24);
+    std::vector<double> res(1 + 12 * kHexesBufSize);
+    one_call(7, in, 48, res.data(), (int)res.size());
+    const int n = (int)res[0];
+    LSM_REQUIRE(n <= kHexesBufSize, LIBSUPERMESH_GPU_ERROR_OVERFLOW, "intersect_hexes: more than 3645 tets");
     *n_tets_c = n;
     std::memcpy(tets_c, res.data() + 1, sizeof(double) * 12 * n);
   });

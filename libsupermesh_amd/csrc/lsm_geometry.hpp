@@ -19,6 +19,8 @@ constexpr int kTriBufSize = 22;  // TRI_BUF_SIZE, include/libsupermesh-c.h:3
 constexpr int kTetBufSize = 81;  // TET_BUF_SIZE, include/libsupermesh-c.h:4
 constexpr int kTriQuadBufSize = 46;  // tris_c(2, 3, 46), C_bindings.F90:59
 constexpr int kTetHexBufSize = 729;  // tets_c(3, 4, 729), C_bindings.F90:47
+constexpr int kQuadsBufSize = 62;    // max_n_tris_c(4, 4), Supermesh.F90:129
+constexpr int kHexesBufSize = 5 * kTetHexBufSize;  // Supermesh.F90:131
 
 // ---------------------------------------------------------------------------
 // Fortran intrinsics
@@ -875,6 +877,19 @@ LSM_HD void hex_planes(const double* h, double* planes) {
     planes[4 * f + 1] = n[1];
     planes[4 * f + 2] = n[2];
     planes[4 * f + 3] = dot3(o[0], o[1], o[2], n[0], n[1], n[2]);
+  }
+}
+
+// Split tet s of a hexahedron (intersect_hexes, Supermesh.F90:318-332):
+// vertex ids 2567, 0237, 0457, 0125, 0257, one nibble each.
+LSM_HD void hex_split_tet(const double* hex, int s, double* T) {
+  const unsigned code = s == 0 ? 0x7652u : s == 1 ? 0x7320u : s == 2 ? 0x7540u : s == 3 ? 0x5210u : 0x7520u;
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const int hv = (code >> (4 * v)) & 15;
+    T[3 * v] = hex[3 * hv];
+    T[3 * v + 1] = hex[3 * hv + 1];
+    T[3 * v + 2] = hex[3 * hv + 2];
   }
 }
 

@@ -915,19 +915,6 @@ struct HexWalkArgs {
   double* elements;
 };
 
-// Split tet s of a hexahedron (intersect_hexes, Supermesh.F90:318-332):
-// vertex ids 2567, 0237, 0457, 0125, 0257, one nibble each.
-__device__ __forceinline__ void hex_split_tet(const double* hex, int s, double* T) {
-  const unsigned code = s == 0 ? 0x7652u : s == 1 ? 0x7320u : s == 2 ? 0x7540u : s == 3 ? 0x5210u : 0x7520u;
-#pragma unroll
-  for (int v = 0; v < 4; v++) {
-    const int hv = (code >> (4 * v)) & 15;
-    T[3 * v] = hex[3 * hv];
-    T[3 * v + 1] = hex[3 * hv + 1];
-    T[3 * v + 2] = hex[3 * hv + 2];
-  }
-}
-
 // One wave per block: 30 KB of LDS stack per block, five blocks per CU.
 constexpr int kHexBlock = 64;
 

@@ -30,6 +30,16 @@ module libsupermesh
     & triangle_area, tetrahedron_volume, intersect_intervals, interval_size, &
     & intersect_tri_quad, intersect_tet_hex, supermesh, libsupermesh_gpu_error
   public :: intersections, tree_type, rtree_type, allocate, query, deallocate
+  public :: intersect_quads, intersect_hexes, intersect_simplices, intersect_elements, &
+    & simplex_volume, max_n_tris_c, max_n_tets_c, max_n_simplices_c
+
+  interface max_n_tris_c
+    module procedure max_n_tris_c_tri, max_n_tris_c_poly
+  end interface max_n_tris_c
+
+  interface max_n_simplices_c
+    module procedure max_n_simplices_c_simplices, max_n_simplices_c_elements
+  end interface max_n_simplices_c
   public :: tree_intersection_finder_set_input, tree_intersection_finder_find, &
     & tree_intersection_finder_query_output, tree_intersection_finder_get_output, &
     & tree_intersection_finder_reset
@@ -250,6 +260,18 @@ module libsupermesh
       real(kind = c_double), dimension(*) :: tet_a, hex_b, tets_c
       integer(kind = c_int) :: n_tets_c
     end subroutine c_intersect_tet_hex
+    subroutine c_intersect_quads(quad_a, quad_b, tris_c, n_tris_c) &
+        & bind(c, name = "libsupermesh_intersect_quads")
+      import :: c_double, c_int
+      real(kind = c_double), dimension(*) :: quad_a, quad_b, tris_c
+      integer(kind = c_int) :: n_tris_c
+    end subroutine c_intersect_quads
+    subroutine c_intersect_hexes(hex_a, hex_b, tets_c, n_tets_c) &
+        & bind(c, name = "libsupermesh_intersect_hexes")
+      import :: c_double, c_int
+      real(kind = c_double), dimension(*) :: hex_a, hex_b, tets_c
+      integer(kind = c_int) :: n_tets_c
+    end subroutine c_intersect_hexes
     subroutine c_intersect_intervals(interval_a, interval_b, intervals_c, n_intervals_c) &
         & bind(c, name = "libsupermesh_intersect_intervals")
       import :: c_double, c_int
@@ -920,5 +942,155 @@ contains
     type(intersections), dimension(:), intent(out) :: map_ab
     call ragged(finder_tree, positions_a, enlist_a, positions_b, enlist_b, map_ab)
   end subroutine brute_force_intersection_finder_intersections
+
+
+  ! ---------------------------------------------------------------------
+  ! Element dispatch (Supermesh.F90:89-209, 231-248, 303-351) and the buffer
+  ! sizes of the per-pair calls (Triangle_Intersection.F90:106-123,
+  ! Tetrahedron_Intersection.F90:181-188).
+  ! ---------------------------------------------------------------------
+
+  pure elemental function max_n_tris_c_tri(n_lines_b) result(max_n)
+    integer, intent(in) :: n_lines_b
+    integer :: max_n
+    max_n = 3 * (2 ** n_lines_b) - 2
+  end function max_n_tris_c_tri
+
+  pure elemental function max_n_tris_c_poly(n_lines_a, n_lines_b) result(max_n)
+    integer, intent(in) :: n_lines_a, n_lines_b
+    integer :: max_n
+    max_n = n_lines_a * (2 ** n_lines_b) - 2
+  end function max_n_tris_c_poly
+
+  pure elemental function max_n_tets_c(n_planes_b) result(max_n)
+    integer, intent(in) :: n_planes_b
+    integer :: max_n
+    max_n = 3 ** n_planes_b
+  end function max_n_tets_c
+
+  pure elemental function max_n_simplices_c_simplices(dim) result(max_n)
+    integer, intent(in) :: dim
+    integer :: max_n
+    select case(dim)
+      case(1)
+        max_n = interval_buf_size
+      case(2)
+        max_n = tri_buf_size
+      case(3)
+        max_n = tet_buf_size
+      case default
+        max_n = 0
+    end select
+  end function max_n_simplices_c_simplices
+
+  pure elemental function max_n_simplices_c_elements(dim, loc_a, loc_b) result(max_n)
+    integer, intent(in) :: dim, loc_a, loc_b
+    integer :: max_n
+    max_n = 0
+    if(loc_a == dim + 1 .and. loc_b == dim + 1) then
+      max_n = max_n_simplices_c_simplices(dim)
+    else if((loc_a == dim + 1 .and. loc_b == 2 ** dim) .or. (loc_b == dim + 1 .and. loc_a == 2 ** dim)) then
+      if(dim == 2) max_n = max_n_tris_c_tri(4)
+      if(dim == 3) max_n = max_n_tets_c(6)
+    else if(loc_a == 2 ** dim .and. loc_b == 2 ** dim) then
+      if(dim == 2) max_n = max_n_tris_c_poly(4, 4)
+      if(dim == 3) max_n = 5 * max_n_tets_c(6)
+    end if
+  end function max_n_simplices_c_elements
+
+  ! intersect_quads (Supermesh.F90:231-248): tris_c(2, 3, 62).
+  subroutine intersect_quads(quad_a, quad_b, tris_c, n_tris_c)
+    real(kind = real_kind), dimension(2, 4), intent(in) :: quad_a, quad_b
+    real(kind = real_kind), dimension(:, :, :), intent(inout) :: tris_c
+    integer, intent(out) :: n_tris_c
+    real(kind = c_double), dimension(2, 4) :: a, b
+    real(kind = c_double), dimension(2, 3, 62) :: c
+    integer(kind = c_int) :: n
+    a = quad_a
+    b = quad_b
+    call c_intersect_quads(a, b, c, n)
+    n_tris_c = n
+    if(n > 0) tris_c(:, :, :n) = c(:, :, :n)
+  end subroutine intersect_quads
+
+  ! intersect_hexes (Supermesh.F90:303-334): tets_c(3, 4, 5 * 729).
+  subroutine intersect_hexes(hex_a, hex_b, tets_c, n_tets_c)
+    real(kind = real_kind), dimension(3, 8), intent(in) :: hex_a, hex_b
+    real(kind = real_kind), dimension(:, :, :), intent(inout) :: tets_c
+    integer, intent(out) :: n_tets_c
+    real(kind = c_double), dimension(3, 8) :: a, b
+    real(kind = c_double), dimension(:, :, :), allocatable :: c
+    integer(kind = c_int) :: n
+    allocate(c(3, 4, 5 * 729))
+    a = hex_a
+    b = hex_b
+    call c_intersect_hexes(a, b, c, n)
+    n_tets_c = n
+    if(n > 0) tets_c(:, :, :n) = c(:, :, :n)
+  end subroutine intersect_hexes
+
+  ! intersect_simplices (Supermesh.F90:141-160): intervals, triangles or
+  ! tetrahedra by the dimension.
+  subroutine intersect_simplices(simplex_a, simplex_b, simplices_c, n_simplices_c)
+    real(kind = real_kind), dimension(:, :), intent(in) :: simplex_a, simplex_b
+    real(kind = real_kind), dimension(:, :, :), intent(inout) :: simplices_c
+    integer, intent(out) :: n_simplices_c
+    real(kind = real_kind), dimension(2, 3, tri_buf_size) :: tc
+    real(kind = real_kind), dimension(3, 4, tet_buf_size) :: ec
+    real(kind = real_kind), dimension(2) :: ic
+    n_simplices_c = 0
+    select case(size(simplex_a, 1))
+      case(1)
+        call intersect_intervals(simplex_a(1, :), simplex_b(1, :), ic, n_simplices_c)
+        if(n_simplices_c > 0) simplices_c(1, :, 1) = ic
+      case(2)
+        call intersect_tris_real(simplex_a, simplex_b, tc, n_simplices_c)
+        if(n_simplices_c > 0) simplices_c(:, :, :n_simplices_c) = tc(:, :, :n_simplices_c)
+      case(3)
+        call intersect_tets_real(simplex_a, simplex_b, ec, n_simplices_c)
+        if(n_simplices_c > 0) simplices_c(:, :, :n_simplices_c) = ec(:, :, :n_simplices_c)
+    end select
+  end subroutine intersect_simplices
+
+  ! intersect_elements (Supermesh.F90:162-209): simplices, simplex x quad /
+  ! hex either way round, quads or hexes, by the vertex counts.
+  subroutine intersect_elements(element_a, element_b, elements_c, nelements_c)
+    real(kind = real_kind), dimension(:, :), intent(in) :: element_a, element_b
+    real(kind = real_kind), dimension(:, :, :), intent(inout) :: elements_c
+    integer, intent(out) :: nelements_c
+    integer :: dim, loc_a, loc_b
+    dim = size(element_a, 1)
+    loc_a = size(element_a, 2)
+    loc_b = size(element_b, 2)
+    nelements_c = 0
+    if(loc_a == dim + 1 .and. loc_b == dim + 1) then
+      call intersect_simplices(element_a, element_b, elements_c, nelements_c)
+    else if(loc_a == dim + 1 .and. loc_b == 2 ** dim) then
+      if(dim == 2) call intersect_tri_quad(element_a, element_b, elements_c, nelements_c)
+      if(dim == 3) call intersect_tet_hex(element_a, element_b, elements_c, nelements_c)
+    else if(loc_a == 2 ** dim .and. loc_b == dim + 1) then
+      if(dim == 2) call intersect_tri_quad(element_b, element_a, elements_c, nelements_c)
+      if(dim == 3) call intersect_tet_hex(element_b, element_a, elements_c, nelements_c)
+    else if(loc_a == 2 ** dim .and. loc_b == 2 ** dim) then
+      if(dim == 2) call intersect_quads(element_a, element_b, elements_c, nelements_c)
+      if(dim == 3) call intersect_hexes(element_a, element_b, elements_c, nelements_c)
+    end if
+  end subroutine intersect_elements
+
+  ! simplex_volume (Supermesh.F90:336-351).
+  function simplex_volume(simplex) result(volume)
+    real(kind = real_kind), dimension(:, :), intent(in) :: simplex
+    real(kind = real_kind) :: volume
+    select case(size(simplex, 1))
+      case(1)
+        volume = interval_size(simplex(1, :))
+      case(2)
+        volume = triangle_area(simplex)
+      case(3)
+        volume = tetrahedron_volume(simplex)
+      case default
+        volume = -huge(volume)
+    end select
+  end function simplex_volume
 
 end module libsupermesh

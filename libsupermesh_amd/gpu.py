@@ -311,6 +311,16 @@ def intersect_tet_hex(tet_a, hex_b):
     return _pair_call(lib().libsupermesh_intersect_tet_hex, tet_a, hex_b, (729, 4, 3))
 
 
+def intersect_quads(quad_a, quad_b):
+    """Triangles of quad_a ^ quad_b (intersect_quads), ``float64[n, 3, 2]`` (at most 62)."""
+    return _pair_call(lib().libsupermesh_intersect_quads, quad_a, quad_b, (62, 3, 2))
+
+
+def intersect_hexes(hex_a, hex_b):
+    """Tetrahedra of hex_a ^ hex_b (intersect_hexes), ``float64[n, 4, 3]`` (at most 3645)."""
+    return _pair_call(lib().libsupermesh_intersect_hexes, hex_a, hex_b, (3645, 4, 3))
+
+
 def intersect_intervals(interval_a, interval_b):
     """``(n, interval)``: n is 0 or 1.  Host code in the library."""
     a = np.ascontiguousarray(interval_a, dtype=np.float64).reshape(2)

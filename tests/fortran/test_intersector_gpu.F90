@@ -8,6 +8,8 @@
 !   mode "mixed": simplex mesh A against quad / hex mesh B (R-tree finder,
 !     intersect_tri_quad / intersect_tet_hex), or 1D meshes (sort finder,
 !     intersect_intervals);
+!   mode "elements": every closed-overlap pair through intersect_elements
+!     (any element types), summing simplex_volume of the pieces;
 !   mode "tree_api": one-element queries through the stateful tree finder
 !     (set_input / find / query_output / get_output / reset) and an
 !     rtree_type (allocate / query / deallocate), checked against the CSR
@@ -114,6 +116,23 @@ program test_intersector_gpu
             total = total + tetrahedron_volume(tets_h(:, :, c))
           end do
         end if
+        npieces = npieces + n
+      end do
+    end do
+    call check_error()
+  case("elements")
+    allocate(indptr(size(ea, 2) + 1))
+    call intersection_finder(pa, ea, pb, eb, indices, indptr)
+    call check_error()
+    allocate(tets_h(dim, dim + 1, max_n_simplices_c(dim, size(ea, 1), size(eb, 1))))
+    do a = 1, size(ea, 2)
+      do i = indptr(a), indptr(a + 1) - 1
+        b = indices(i)
+        npairs = npairs + 1
+        call intersect_elements(pa(:, ea(:, a)), pb(:, eb(:, b)), tets_h, n)
+        do c = 1, n
+          total = total + simplex_volume(tets_h(:, :, c))
+        end do
         npieces = npieces + n
       end do
     end do

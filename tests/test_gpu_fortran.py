@@ -120,3 +120,27 @@ def test_fortran_tree_types_and_stateful_finders(tmp_path, fixture_meshes, a, b)
     dim, n_strict, n_closed, _ = r.stdout.split()
     assert int(n_strict) == len(O.brute_force(pa, ea, pb, eb)[1])
     assert int(n_closed) == len(O.brute_force(pa, ea, pb, eb, closed=True)[1])
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran interface not built (no amdflang)")
+@pytest.mark.parametrize("a,b,total", [("square.1", "square.2", 1.0), ("square.4", "square.1", 1.0),
+                                       ("square.3", "square.4", 1.0), ("cube.1", "cube.2", 1.0),
+                                       ("cube.3", "cube.2", 0.5), ("cube.3", "cube.4", 0.5),
+                                       ("line.1", "line.2", 1.0)])
+def test_fortran_intersect_elements(tmp_path, fixture_meshes, a, b, total):
+    # src/tests/test_element_intersector.F90: intersect_elements over every
+    # pair, simplex_volume summed; the piece count against the oracle
+    pa, ea = fixture_meshes[a]
+    pb, eb = fixture_meshes[b]
+    fa, fb = tmp_path / "a.bin", tmp_path / "b.bin"
+    write_mesh(fa, pa, ea)
+    write_mesh(fb, pb, eb)
+    r = subprocess.run([EXE, str(fa), str(fb), "elements"], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    dim, n, pieces, s = r.stdout.split()
+    if int(dim) == 1:
+        ip, ix = O.sort_finder(pa, ea, pb, eb)
+    else:
+        ip, ix = O.brute_force(pa, ea, pb, eb, closed=True)
+    assert int(n) == len(ix)
+    assert abs(float(s) - total) < 1e-12

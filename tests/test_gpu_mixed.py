@@ -52,6 +52,23 @@ def test_tet_hex_bit_exact():
     assert nonempty > 100
 
 
+def test_quads_and_hexes_bit_exact():
+    # the per-pair intersect_quads / intersect_hexes against the oracle
+    rng = np.random.default_rng(33)
+    nq = nh = 0
+    for i in range(300):
+        qa, qb = _quad(rng), _quad(rng)
+        got, ref = gpu.intersect_quads(qa, qb), O.intersect_quads(qa, qb)
+        assert got.shape == ref.shape and np.array_equal(got, ref), i
+        nq += len(ref) > 0
+    for i in range(60):
+        ha, hb = _hex(rng), _hex(rng)
+        got, ref = gpu.intersect_hexes(ha, hb), O.intersect_hexes(ha, hb)
+        assert got.shape == ref.shape and np.array_equal(got, ref), i
+        nh += len(ref) > 0
+    assert nq > 100 and nh > 20
+
+
 @pytest.mark.parametrize("names,expected", [(("square.1", "square.4"), 1.0),
                                             (("cube.1", "cube.3"), 0.5)])
 def test_reference_mixed_meshes(fixture_meshes, names, expected):
