@@ -473,79 +473,98 @@ template <bool MASS, bool FILL>
 __global__ void __launch_bounds__(kTriBlock) k_clip_tri(ClipArgs g) {
   __shared__ double poly[LdsPoly<3, 3>::kPoints * 2 * kTriBlock];
   double* q = poly + threadIdx.x;
+  // count pass with mass: the 9-value blocks of the block's pairs leave
+  // through the (then idle) polygon slots as one contiguous, coalesced run
+  constexpr bool STAGE = MASS && !FILL;
+  static_assert(LdsPoly<3, 3>::kPoints * 2 >= 9, "room for a block per lane");
   const long stride = (long)gridDim.x * blockDim.x;
-  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < g.npairs; p += stride) {
-    const long a = g.rows[p], b = g.cols[p];
-    long off = 0;
-    int cap = 0;
-    if (FILL) {
-      cap = g.counts[p];
-      if (cap == 0) continue;
-      off = g.offsets[p];
-    }
-    double A[6], Bv[6];
-#pragma unroll
-    for (int k = 0; k < 6; k++) A[k] = g.acoords[a * 6 + k];
-#pragma unroll
-    for (int k = 0; k < 6; k++) Bv[k] = g.bcoords[b * 6 + k];
-    const double tol = 10.0 * fmin_f(g.asp[a], g.bsp[b]);
-    double m = 0.0;
-    double blk[9], affa[6], affb[6];
-    if (MASS) {
-#pragma unroll
-      for (int k = 0; k < 9; k++) blk[k] = 0.0;
-#pragma unroll
-      for (int k = 0; k < 6; k++) {
-        affa[k] = g.aaff[a * 6 + k];
-        affb[k] = g.baff[b * 6 + k];
+  // the same trip count for every lane of the block (barriers below)
+  for (long base = (long)blockIdx.x * blockDim.x; base < g.npairs; base += stride) {
+    const long p = base + threadIdx.x;
+    double blk[9];
+    bool keep = false;  // this lane's block is written from here
+    do {
+      if (p >= g.npairs) break;
+      const long a = g.rows[p], b = g.cols[p];
+      long off = 0;
+      int cap = 0;
+      if (FILL) {
+        cap = g.counts[p];
+        if (cap == 0) break;
+        off = g.offsets[p];
       }
-    }
-    Line L[3];
-    tri_lines(Bv, L);
-    int k = 0;
-    const int nk = clip_poly_lds<3, 3, kTriBlock>(
-        A, L, tol, q, [&](double x0, double y0, double x1, double y1, double x2, double y2, double ar) {
-          m = m + ar;
-          if (MASS) {
-            const double el[6] = {x0, y0, x1, y1, x2, y2};
-            mass_element<2>(el, ar, affa, affb, blk);
-          }
-          if (FILL && k < cap) {
-            double* o = g.elements + (off + k) * 6;
-            o[0] = x0;
-            o[1] = y0;
-            o[2] = x1;
-            o[3] = y1;
-            o[4] = x2;
-            o[5] = y2;
-          }
-          k++;
-        });
-    if (nk < 0) {
-      if (!FILL) g.olist[atomicAdd(g.nolist, 1ULL)] = (int)p;
-      continue;
-    }
-    if (FILL) continue;
-    g.counts[p] = nk;
-    g.measure[p] = nk == 0 ? 0.0 : m;
-    if (MASS) {
+      double A[6], Bv[6];
 #pragma unroll
-      for (int i = 0; i < 9; i++) g.blocks[p * 9 + i] = nk == 0 ? 0.0 : blk[i];
-      if (g.rkey) {
-        const long r = (p - g.indptr[a]) * 3;
-        int nodes[3];
+      for (int k = 0; k < 6; k++) A[k] = g.acoords[a * 6 + k];
 #pragma unroll
-        for (int j = 0; j < 3; j++) nodes[j] = nk == 0 ? -1 : g.enlb[b * 3 + j];
+      for (int k = 0; k < 6; k++) Bv[k] = g.bcoords[b * 6 + k];
+      const double tol = 10.0 * fmin_f(g.asp[a], g.bsp[b]);
+      double m = 0.0;
+      double affa[6], affb[6];
+      if (MASS) {
 #pragma unroll
-        for (int li = 0; li < 3; li++) {
-          const long o = g.rbase[a * 3 + li] + r;
+        for (int k = 0; k < 9; k++) blk[k] = 0.0;
 #pragma unroll
-          for (int j = 0; j < 3; j++) {
-            g.rkey[o + j] = nodes[j];
-            g.rval[o + j] = nk == 0 ? 0.0 : blk[li * 3 + j];
+        for (int k = 0; k < 6; k++) {
+          affa[k] = g.aaff[a * 6 + k];
+          affb[k] = g.baff[b * 6 + k];
+        }
+      }
+      Line L[3];
+      tri_lines(Bv, L);
+      int k = 0;
+      const int nk = clip_poly_lds<3, 3, kTriBlock>(
+          A, L, tol, q, [&](double x0, double y0, double x1, double y1, double x2, double y2, double ar) {
+            m = m + ar;
+            if (MASS) {
+              const double el[6] = {x0, y0, x1, y1, x2, y2};
+              mass_element<2>(el, ar, affa, affb, blk);
+            }
+            if (FILL && k < cap) {
+              double* o = g.elements + (off + k) * 6;
+              o[0] = x0;
+              o[1] = y0;
+              o[2] = x1;
+              o[3] = y1;
+              o[4] = x2;
+              o[5] = y2;
+            }
+            k++;
+          });
+      if (nk < 0) {  // the streaming clipper writes this pair's results later
+        if (!FILL) g.olist[atomicAdd(g.nolist, 1ULL)] = (int)p;
+        break;
+      }
+      if (FILL) break;
+      g.counts[p] = nk;
+      g.measure[p] = nk == 0 ? 0.0 : m;
+      if (MASS) {
+        keep = nk > 0;
+        if (g.rkey) {
+          const long r = (p - g.indptr[a]) * 3;
+          int nodes[3];
+#pragma unroll
+          for (int j = 0; j < 3; j++) nodes[j] = nk == 0 ? -1 : g.enlb[b * 3 + j];
+#pragma unroll
+          for (int li = 0; li < 3; li++) {
+            const long o = g.rbase[a * 3 + li] + r;
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+              g.rkey[o + j] = nodes[j];
+              g.rval[o + j] = nk == 0 ? 0.0 : blk[li * 3 + j];
+            }
           }
         }
       }
+    } while (false);
+    if constexpr (STAGE) {
+      __syncthreads();  // every lane is done with its polygon slots
+#pragma unroll
+      for (int i = 0; i < 9; i++) poly[threadIdx.x * 9 + i] = keep ? blk[i] : 0.0;
+      __syncthreads();
+      const long nv = g.npairs - base < kTriBlock ? g.npairs - base : kTriBlock;
+      for (long t = threadIdx.x; t < 9 * nv; t += kTriBlock) g.blocks[base * 9 + t] = poly[t];
+      __syncthreads();  // the slots hold polygons again
     }
   }
 }
