@@ -586,19 +586,33 @@ __device__ __forceinline__ unsigned child_mask(const LevelInfo& info, const doub
   return m;
 }
 
+// OR of an 8-bit mask over the wave's active lanes (uniform result).
+__device__ __forceinline__ unsigned wave_or8(unsigned x) {
+  unsigned r = 0;
+#pragma unroll
+  for (int b = 0; b < 8; b++) r |= (__ballot((x >> b) & 1u) != 0ULL ? 1u : 0u) << b;
+  return r;
+}
+
 // Single-pass search: the same depth-first order as k_query, but a node's
 // children are tested together (their boxes are contiguous) and the walk
 // keeps, per level, a byte of the hit children not yet visited; the node
 // index is implicit (parent = child / F).  The first `cap` results of a query
-// go to scratch (slot-major: scratch[k * na + t]); all are counted.
+// go to scratch (slot-major: scratch[k * na + t]); all are counted.  The wave
+// walks one path: the union of its 64 queries' paths (consecutive queries are
+// close in Morton order, so their paths mostly coincide).  Every lane tests the children of the wave's current node
+// (the same boxes: one broadcast load per box, no divergent gathers) and
+// keeps its own per-level hit masks, so its hits, and their order, are those
+// of its own depth-first walk in k_query_mask: each level's union mask is
+// consumed lowest child first, which is each lane's own order.
 template <int DIM, bool CLOSED, bool GROUPED>
 __global__ void __launch_bounds__(kBlock)
-k_query_mask(long na, const int* __restrict__ qperm, const double* __restrict__ abox, LevelInfo info,
+k_query_wave(long na, const int* __restrict__ qperm, const double* __restrict__ abox, LevelInfo info,
              const double* __restrict__ boxes, const int* __restrict__ perm, int cap,
              int* __restrict__ scratch, long* __restrict__ row_count, const int* __restrict__ agroup,
              const int* __restrict__ granges) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= na) return;
+  if (t >= na) return;  // the rest of the wave walks on without these lanes
   const int a = qperm[t];
   const int g = GROUPED ? agroup[a] : 0;
   double qlo[DIM], qhi[DIM];
@@ -615,50 +629,62 @@ k_query_mask(long na, const int* __restrict__ qperm, const double* __restrict__ 
   const int top = info.levels - 1;
   if (top == 0) {
     if (child_mask<DIM, CLOSED, GROUPED>(info, boxes, granges, 0, 0, qlo, qhi, g) & 1u) emit(0);
-  } else if (child_mask<DIM, CLOSED, GROUPED>(info, boxes, granges, top, 0, qlo, qhi, g) & 1u) {
-    // masks of levels 1..16, a byte each
-    unsigned long long mlo = 0, mhi = 0;
-    auto get = [&](int L) -> unsigned {
-      return L <= 8 ? (unsigned)(mlo >> (8 * (L - 1))) & 0xffu : (unsigned)(mhi >> (8 * (L - 9))) & 0xffu;
+  } else {
+    const bool root = child_mask<DIM, CLOSED, GROUPED>(info, boxes, granges, top, 0, qlo, qhi, g) & 1u;
+    // per lane: masks of levels 1..16, a byte each; per wave: their union
+    unsigned long long mlo = 0, mhi = 0, ulo = 0, uhi = 0;
+    auto get = [](unsigned long long lo, unsigned long long hi, int L) -> unsigned {
+      return L <= 8 ? (unsigned)(lo >> (8 * (L - 1))) & 0xffu : (unsigned)(hi >> (8 * (L - 9))) & 0xffu;
     };
-    auto set = [&](int L, unsigned m) {
+    auto put = [](unsigned long long& lo, unsigned long long& hi, int L, unsigned m) {
       if (L <= 8) {
         const int sh = 8 * (L - 1);
-        mlo = (mlo & ~(0xffULL << sh)) | ((unsigned long long)m << sh);
+        lo = (lo & ~(0xffULL << sh)) | ((unsigned long long)m << sh);
       } else {
         const int sh = 8 * (L - 9);
-        mhi = (mhi & ~(0xffULL << sh)) | ((unsigned long long)m << sh);
+        hi = (hi & ~(0xffULL << sh)) | ((unsigned long long)m << sh);
       }
     };
     int L = top;
     long node = 0;
-    set(L, child_mask<DIM, CLOSED, GROUPED>(info, boxes, granges, top - 1, 0, qlo, qhi, g));
+    {
+      unsigned m = child_mask<DIM, CLOSED, GROUPED>(info, boxes, granges, top - 1, 0, qlo, qhi, g);
+      if (!root) m = 0;
+      put(mlo, mhi, L, m);
+      put(ulo, uhi, L, wave_or8(m));
+    }
     while (true) {
-      unsigned m = get(L);
-      if (m == 0) {
+      const unsigned um = get(ulo, uhi, L);
+      if (um == 0) {
         if (L == top) break;
         node /= kFanout;
         L++;
         continue;
       }
-      if (L == 1) {  // children are elements
+      if (L == 1) {  // children are elements: each lane its own hits
+        unsigned m = get(mlo, mhi, 1);
         while (m) {
           const int c = __builtin_ctz(m);
           m &= m - 1;
           emit(node * kFanout + c);
         }
-        set(1, 0);
+        put(mlo, mhi, 1, 0);
+        put(ulo, uhi, 1, 0);
         continue;
       }
-      const int c = __builtin_ctz(m);
-      set(L, m & (m - 1));
+      const int c = __builtin_ctz(um);
+      put(ulo, uhi, L, um & (um - 1));
+      const unsigned mine = get(mlo, mhi, L);
+      put(mlo, mhi, L, mine & ~(1u << c));
       const long child = node * kFanout + c;
-      const unsigned mm = child_mask<DIM, CLOSED, GROUPED>(info, boxes, granges, L - 2, child * kFanout, qlo,
-                                                           qhi, g);
-      if (mm) {
+      unsigned mm = child_mask<DIM, CLOSED, GROUPED>(info, boxes, granges, L - 2, child * kFanout, qlo, qhi, g);
+      if (!((mine >> c) & 1u)) mm = 0;
+      const unsigned umm = wave_or8(mm);
+      if (umm) {
         node = child;
         L--;
-        set(L, mm);
+        put(mlo, mhi, L, mm);
+        put(ulo, uhi, L, umm);
       }
     }
   }
@@ -775,10 +801,10 @@ long find_pairs(Context& C, const DevMesh& A, const DevMesh& B, int finder, int 
 #define LSM_MASK(DIM, CL)                                                                           \
   do {                                                                                            \
     if (H.grouped)                                                                                \
-      hipLaunchKernelGGL((k_query_mask<DIM, CL, true>), grid, block, 0, C.stream, na, qperm, abox,  \
+      hipLaunchKernelGGL((k_query_wave<DIM, CL, true>), grid, block, 0, C.stream, na, qperm, abox,  \
                          info, boxes, perm, cap, scratch, row_count, agroup, granges);            \
     else                                                                                          \
-      hipLaunchKernelGGL((k_query_mask<DIM, CL, false>), grid, block, 0, C.stream, na, qperm, abox, \
+      hipLaunchKernelGGL((k_query_wave<DIM, CL, false>), grid, block, 0, C.stream, na, qperm, abox, \
                          info, boxes, perm, cap, scratch, row_count, agroup, granges);            \
   } while (0)
   int* indices = nullptr;
