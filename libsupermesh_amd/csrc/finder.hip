@@ -603,7 +603,7 @@ __device__ __forceinline__ unsigned wave_or8(unsigned x) {
 // close in Morton order, so their paths mostly coincide).  Every lane tests the children of the wave's current node
 // (the same boxes: one broadcast load per box, no divergent gathers) and
 // keeps its own per-level hit masks, so its hits, and their order, are those
-// of its own depth-first walk in k_query_mask: each level's union mask is
+// of its own depth-first walk alone: each level's union mask is
 // consumed lowest child first, which is each lane's own order.
 template <int DIM, bool CLOSED, bool GROUPED>
 __global__ void __launch_bounds__(kBlock)
