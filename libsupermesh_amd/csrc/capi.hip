@@ -1026,6 +1026,8 @@ int libsupermesh_gpu_tree_create(int finder, int dim, long nnodes, const double*
     *tree = nullptr;
     LSM_REQUIRE(finder == LIBSUPERMESH_GPU_FINDER_TREE || finder == LIBSUPERMESH_GPU_FINDER_RTREE,
                 LIBSUPERMESH_GPU_ERROR_INVALID, "unknown finder");
+    LSM_REQUIRE(positions != nullptr && (nelements == 0 || enlist != nullptr), LIBSUPERMESH_GPU_ERROR_INVALID,
+                "null mesh arrays");
     auto t = std::make_unique<libsupermesh_gpu_tree>();
     t->finder = finder;
     t->dim = dim;
@@ -1047,10 +1049,12 @@ int libsupermesh_gpu_tree_create(int finder, int dim, long nnodes, const double*
 
 int libsupermesh_gpu_tree_query(libsupermesh_gpu_tree* tree, int loc, const double* element, long* neles) {
   return guarded([&] {
-    LSM_REQUIRE(tree != nullptr && neles != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID, "null argument");
+    LSM_REQUIRE(tree != nullptr && neles != nullptr && element != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID,
+                "null argument");
+    Context& C = tree->ctx;
+    std::lock_guard<std::mutex> lk(C.mu);
     *neles = 0;
     tree->result.clear();
-    Context& C = tree->ctx;
     LSM_HIP(hipSetDevice(C.device));
     const int dim = tree->dim;
     std::vector<long> enl((size_t)loc);
@@ -1071,6 +1075,7 @@ int libsupermesh_gpu_tree_query(libsupermesh_gpu_tree* tree, int loc, const doub
 int libsupermesh_gpu_tree_result(libsupermesh_gpu_tree* tree, long neles, long* eles) {
   return guarded([&] {
     LSM_REQUIRE(tree != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID, "null tree");
+    std::lock_guard<std::mutex> lk(tree->ctx.mu);
     LSM_REQUIRE(neles == (long)tree->result.size() && (neles == 0 || eles != nullptr),
                 LIBSUPERMESH_GPU_ERROR_INVALID, "result size differs from the last query's");
     for (long i = 0; i < neles; i++) eles[i] = tree->result[i];

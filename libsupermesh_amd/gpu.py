@@ -229,6 +229,44 @@ def sort_intersection_finder(pos_a, enl_a, pos_b, enl_b):
     return _finder("libsupermesh_sort_intersection_finder", pos_a, enl_a, pos_b, enl_b)
 
 
+class DeviceTree:
+    """A search hierarchy over mesh B kept on the GPU, queried one element at
+    a time (the reference's tree_type / rtree_type); rows ascending, 0-based."""
+
+    def __init__(self, positions, enlist, finder=FINDER_TREE):
+        pos = np.ascontiguousarray(positions, dtype=np.float64)
+        enl = np.ascontiguousarray(enlist, dtype=np.int64)
+        self.dim = pos.shape[1]
+        self.handle = c_void_p()
+        L = lib()
+        L.libsupermesh_gpu_tree_create.argtypes = [c_int, c_int, c_long, c_void_p, c_long, c_int, c_void_p,
+                                                   POINTER(c_void_p)]
+        L.libsupermesh_gpu_tree_query.argtypes = [c_void_p, c_int, c_void_p, POINTER(c_long)]
+        L.libsupermesh_gpu_tree_result.argtypes = [c_void_p, c_long, c_void_p]
+        L.libsupermesh_gpu_tree_destroy.argtypes = [c_void_p]
+        _check(L.libsupermesh_gpu_tree_create(finder, self.dim, pos.shape[0], pos.ctypes.data, enl.shape[0],
+                                              enl.shape[1], enl.ctypes.data, ctypes.byref(self.handle)))
+
+    def query(self, element):
+        el = np.ascontiguousarray(element, dtype=np.float64)
+        n = c_long(0)
+        _check(lib().libsupermesh_gpu_tree_query(self.handle, el.shape[0], el.ctypes.data, ctypes.byref(n)))
+        out = np.zeros(max(n.value, 1), dtype=np.int64)
+        _check(lib().libsupermesh_gpu_tree_result(self.handle, n.value, out.ctypes.data))
+        return out[:n.value]
+
+    def close(self):
+        if self.handle:
+            lib().libsupermesh_gpu_tree_destroy(self.handle)
+            self.handle = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def advancing_front_intersection_finder(pos_a, enl_a, pos_b, enl_b):
     """Candidate pairs of the reference's advancing front finder (closed
     overlap, rows in the front's visiting order), as CSR.  The front runs on

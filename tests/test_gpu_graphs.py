@@ -123,3 +123,24 @@ def test_advancing_front_empty():
     pa, ea = meshgen.split_grid(2, 3, 1)
     gp, gi = gpu.advancing_front_intersection_finder(pa, ea, pa[:0], np.zeros((0, 3), dtype=np.int64))
     assert np.array_equal(gp, np.zeros(len(ea) + 1)) and len(gi) == 0
+
+
+@pytest.mark.parametrize("finder,closed", [(gpu.FINDER_TREE, False), (gpu.FINDER_RTREE, True)])
+def test_device_tree_queries(fixture_meshes, finder, closed):
+    # libsupermesh_gpu_tree_*: one-element queries equal the brute-force rows
+    pa, ea = fixture_meshes["square.1"]
+    pb, eb = fixture_meshes["square.2"]
+    ip, ix = O.brute_force(pa, ea, pb, eb, closed=closed)
+    t = gpu.DeviceTree(pb, eb, finder=finder)
+    for a in range(len(ea)):
+        assert list(t.query(pa[ea[a]])) == list(ix[ip[a]:ip[a + 1]]), a
+    # a quad query against the simplex tree, and an element outside
+    q = np.array([[0.1, 0.1], [0.4, 0.1], [0.4, 0.3], [0.1, 0.3]])
+    got = t.query(q)
+    qa = np.array([[0, 1, 2, 3]])
+    ip2, ix2 = O.brute_force(q, qa, pb, eb, closed=closed)
+    assert list(got) == list(ix2)
+    assert len(t.query(np.array([[5.0, 5.0], [6.0, 5.0], [5.0, 6.0]]))) == 0
+    with pytest.raises(gpu.GpuError, match="differs"):
+        gpu._check(gpu.lib().libsupermesh_gpu_tree_result(t.handle, 7, None))
+    t.close()
