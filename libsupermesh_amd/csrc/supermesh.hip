@@ -1826,12 +1826,15 @@ constexpr int kMaxDistinct = 256;  // columns a row may hold on this path
 // Deduplicates and sums one row held in LDS (rk/rv, n entries): distinct
 // columns through an LDS hash table, sorted, each summed in entry order.
 // Writes them at tkey/tval + o; returns the count, or -1 if the row has more
-// than kMaxDistinct columns.  One wave; `ht` has kHashSlots entries.
+// than kMaxDistinct columns.  One wave; `ht` has 1 << HBITS entries.
+template <int HBITS = 9>
 __device__ __forceinline__ int row_reduce(const int* rk, const double* rv, int n, int* ht, int* dk, int lane,
                                           int* __restrict__ tkey, double* __restrict__ tval, long o) {
   // table of 128, 256 or 512 slots: at least 4/3 of the entries (n is the
-  // same on every lane), so short rows clear and scan less
-  const int tbits = n <= 96 ? 7 : (n <= 192 ? 8 : 9);
+  // same on every lane), so short rows clear and scan less; at most
+  // 1 << HBITS (a table that fills sends the row to the sorted path)
+  const int tb = n <= 96 ? 7 : (n <= 192 ? 8 : 9);
+  const int tbits = tb < HBITS ? tb : HBITS;
   const int tsize = 1 << tbits;
   for (int t = lane; t < tsize; t += 64) ht[t] = -1;
   __syncthreads();
@@ -2029,14 +2032,17 @@ k_mass_rows_gather(long nn, const long* __restrict__ nptr, const int* __restrict
 
 // Rows laid out by the clip pass (2D): one wave per block, rows i, i + grid,
 // ...; while a row is reduced, the next row's entries (up to 64 * PF) and the
-// one after's bounds are already being loaded.
-template <int CAP, int PF>
+// one after's bounds are already being loaded.  Rows hold at most CAP entries
+// and the hash table 1 << HBITS slots: the smaller variant takes less LDS, so
+// more waves share a CU.
+template <int CAP, int PF, int HBITS>
 __global__ void __launch_bounds__(64)
 k_mass_rows_slots(long nn, const long* __restrict__ raw_off, int* __restrict__ tkey, double* __restrict__ tval,
                   long* __restrict__ ucount, int* __restrict__ overflow) {
+  static_assert(HBITS >= 7 && HBITS <= 9, "hash table of 128 to 512 slots");
   __shared__ __align__(16) int rk[CAP + 8];
   __shared__ __align__(16) double rv[CAP + 8];
-  __shared__ int ht[kHashSlots];
+  __shared__ int ht[1 << HBITS];
   __shared__ int dk[kMaxDistinct];
   const int lane = threadIdx.x;
   const long stride = gridDim.x;
@@ -2099,7 +2105,7 @@ k_mass_rows_slots(long nn, const long* __restrict__ raw_off, int* __restrict__ t
     }
     if (lane < 8) rk[n + lane] = -2;  // tail of the last 8-entry group
     __syncthreads();
-    const int nd = row_reduce(rk, rv, n, ht, dk, lane, tkey, tval, a0);
+    const int nd = row_reduce<HBITS>(rk, rv, n, ht, dk, lane, tkey, tval, a0);
     if (nd < 0) {
       if (lane == 0) atomicOr(overflow, 1);
     } else if (lane == 0) {
@@ -2139,8 +2145,11 @@ static void assemble_mass_slots(Context& C, const DevMesh& A, const DevMesh& B, 
   int cus = 0;
   LSM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, C.device));
   const dim3 grid((unsigned)std::max<long>(1, std::min<long>((long)(cus > 0 ? cus : 256) * 16, nn)));
-  if (C.mass_maxraw <= 512)
-    hipLaunchKernelGGL((k_mass_rows_slots<512, 4>), dim3(grid.x * 4), dim3(64), 0, C.stream, nn, raw_off, tkey,
+  if (C.mass_maxraw <= 320)
+    hipLaunchKernelGGL((k_mass_rows_slots<320, 4, 8>), dim3(grid.x * 4), dim3(64), 0, C.stream, nn, raw_off, tkey,
+                       tval, ucount, ovf);
+  else if (C.mass_maxraw <= 512)
+    hipLaunchKernelGGL((k_mass_rows_slots<512, 4, 9>), dim3(grid.x * 4), dim3(64), 0, C.stream, nn, raw_off, tkey,
                        tval, ucount, ovf);
   else
     hipLaunchKernelGGL((k_mass_rows_gather<3, 2048, 2, true>), grid, dim3(128), 0, C.stream, nn, nptr, nullptr,
