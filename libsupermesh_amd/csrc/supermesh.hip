@@ -1826,9 +1826,10 @@ constexpr int kMaxDistinct = 256;  // columns a row may hold on this path
 // Deduplicates and sums one row held in LDS (rk/rv, n entries): distinct
 // columns through an LDS hash table, sorted, each summed in entry order.
 // Writes them at tkey/tval + o; returns the count, or -1 if the row has more
-// than kMaxDistinct columns.  One wave; `ht` has 1 << HBITS entries.
+// than kMaxDistinct columns.  One wave; `ht` has 1 << HBITS entries (at
+// least kMaxDistinct): the distinct columns are compacted into its front.
 template <int HBITS = 9>
-__device__ __forceinline__ int row_reduce(const int* rk, const double* rv, int n, int* ht, int* dk, int lane,
+__device__ __forceinline__ int row_reduce(const int* rk, const double* rv, int n, int* ht, int lane,
                                           int* __restrict__ tkey, double* __restrict__ tval, long o) {
   // table of 128, 256 or 512 slots: at least 4/3 of the entries (n is the
   // same on every lane), so short rows clear and scan less; at most
@@ -1854,13 +1855,15 @@ __device__ __forceinline__ int row_reduce(const int* rk, const double* rv, int n
     }
   }
   __syncthreads();
+  // compaction in place: slot pos <= s0 + lane, and every slot below s0 + 64
+  // was read before the ballot that places the stores
   int nd = 0;
   for (int s0 = 0; s0 < tsize; s0 += 64) {
     const int key = ht[s0 + lane];
     const bool occ = key != -1;
     const unsigned long long m = __ballot(occ);
     const int pos = nd + __popcll(m & ((1ULL << lane) - 1));
-    if (occ && pos < kMaxDistinct) dk[pos] = key;
+    if (occ && pos < kMaxDistinct) ht[pos] = key;
     nd += __popcll(m);
   }
   const bool bad = __ballot(full) != 0ULL || nd > kMaxDistinct;
@@ -1868,10 +1871,10 @@ __device__ __forceinline__ int row_reduce(const int* rk, const double* rv, int n
   if (bad) return -1;
   int key[4];
 #pragma unroll
-  for (int r = 0; r < 4; r++) key[r] = lane + 64 * r < nd ? dk[lane + 64 * r] : 0x7fffffff;
+  for (int r = 0; r < 4; r++) key[r] = lane + 64 * r < nd ? ht[lane + 64 * r] : 0x7fffffff;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  // entries are read 8 at a time (the tail past n holds key -2, which matches
-  // no column) so that the LDS loads overlap
+  // entries are read 4 at a time up to a multiple of 8 (the tail past n holds
+  // key -2, which matches no column) so that the LDS loads overlap
   const int n8 = (n + 7) & ~7;
   if (nd <= 64) {
     wave_bitonic_keys<1>(key, lane);
@@ -1886,16 +1889,14 @@ __device__ __forceinline__ int row_reduce(const int* rk, const double* rv, int n
     const int t0 = (lane / W) * chunk;
     const int t1 = t0 + chunk < n8 ? t0 + chunk : n8;
     double part = 0.0;
-    for (int t = t0; t < t1; t += 8) {
-      const int4 k0 = *reinterpret_cast<const int4*>(rk + t), k1 = *reinterpret_cast<const int4*>(rk + t + 4);
+    for (int t = t0; t < t1; t += 4) {
+      const int4 k0 = *reinterpret_cast<const int4*>(rk + t);
       const double2 v0 = *reinterpret_cast<const double2*>(rv + t);
       const double2 v1 = *reinterpret_cast<const double2*>(rv + t + 2);
-      const double2 v2 = *reinterpret_cast<const double2*>(rv + t + 4);
-      const double2 v3 = *reinterpret_cast<const double2*>(rv + t + 6);
-      const int kk[8] = {k0.x, k0.y, k0.z, k0.w, k1.x, k1.y, k1.z, k1.w};
-      const double vv[8] = {v0.x, v0.y, v1.x, v1.y, v2.x, v2.y, v3.x, v3.y};
+      const int kk[4] = {k0.x, k0.y, k0.z, k0.w};
+      const double vv[4] = {v0.x, v0.y, v1.x, v1.y};
 #pragma unroll
-      for (int u = 0; u < 8; u++)
+      for (int u = 0; u < 4; u++)
         if (kk[u] == mykey) part = part + vv[u];
     }
     double tot = part;
@@ -1903,16 +1904,14 @@ __device__ __forceinline__ int row_reduce(const int* rk, const double* rv, int n
     acc[0] = tot;  // lanes below W (all of rank < nd among them) hold their column's total
   } else {
     wave_bitonic_keys<4>(key, lane);
-    for (int t = 0; t < n8; t += 8) {
-      const int4 k0 = *reinterpret_cast<const int4*>(rk + t), k1 = *reinterpret_cast<const int4*>(rk + t + 4);
+    for (int t = 0; t < n8; t += 4) {
+      const int4 k0 = *reinterpret_cast<const int4*>(rk + t);
       const double2 v0 = *reinterpret_cast<const double2*>(rv + t);
       const double2 v1 = *reinterpret_cast<const double2*>(rv + t + 2);
-      const double2 v2 = *reinterpret_cast<const double2*>(rv + t + 4);
-      const double2 v3 = *reinterpret_cast<const double2*>(rv + t + 6);
-      const int kk[8] = {k0.x, k0.y, k0.z, k0.w, k1.x, k1.y, k1.z, k1.w};
-      const double vv[8] = {v0.x, v0.y, v1.x, v1.y, v2.x, v2.y, v3.x, v3.y};
+      const int kk[4] = {k0.x, k0.y, k0.z, k0.w};
+      const double vv[4] = {v0.x, v0.y, v1.x, v1.y};
 #pragma unroll
-      for (int u = 0; u < 8; u++) {
+      for (int u = 0; u < 4; u++) {
 #pragma unroll
         for (int r = 0; r < 4; r++)
           if (kk[u] == key[r]) acc[r] = acc[r] + vv[u];
@@ -1940,12 +1939,10 @@ k_mass_rows_gather(long nn, const long* __restrict__ nptr, const int* __restrict
   __shared__ __align__(16) int s_rk[WAVES][CAP + 8];
   __shared__ __align__(16) double s_rv[WAVES][CAP + 8];
   __shared__ int s_ht[WAVES][kHashSlots];
-  __shared__ int s_dk[WAVES][kMaxDistinct];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   int* rk = s_rk[wv];
   double* rv = s_rv[wv];
   int* ht = s_ht[wv];
-  int* dk = s_dk[wv];
   const long nwaves = (long)gridDim.x * WAVES;
   // the same trip count and barrier sequence for every wave of the block
   for (long base = (long)blockIdx.x * WAVES; base < nn; base += nwaves) {
@@ -2021,7 +2018,7 @@ k_mass_rows_gather(long nn, const long* __restrict__ nptr, const int* __restrict
     __syncthreads();
     // 2-4. distinct columns, sorted, summed in gather order (every wave of the
     // block calls this, valid row or not, so that the barriers match)
-    const int nd = row_reduce(rk, rv, valid ? n : 0, ht, dk, lane, tkey, tval, valid ? raw_off[i] : 0);
+    const int nd = row_reduce(rk, rv, valid ? n : 0, ht, lane, tkey, tval, valid ? raw_off[i] : 0);
     if (valid && lane == 0) {
       if (nd < 0) atomicOr(overflow, 1);
       else ucount[i] = nd;
@@ -2039,11 +2036,10 @@ template <int CAP, int PF, int HBITS>
 __global__ void __launch_bounds__(64)
 k_mass_rows_slots(long nn, const long* __restrict__ raw_off, int* __restrict__ tkey, double* __restrict__ tval,
                   long* __restrict__ ucount, int* __restrict__ overflow) {
-  static_assert(HBITS >= 7 && HBITS <= 9, "hash table of 128 to 512 slots");
+  static_assert(HBITS >= 8 && HBITS <= 9, "hash table of 256 or 512 slots, room for kMaxDistinct columns");
   __shared__ __align__(16) int rk[CAP + 8];
   __shared__ __align__(16) double rv[CAP + 8];
   __shared__ int ht[1 << HBITS];
-  __shared__ int dk[kMaxDistinct];
   const int lane = threadIdx.x;
   const long stride = gridDim.x;
   long i = blockIdx.x;
@@ -2105,7 +2101,7 @@ k_mass_rows_slots(long nn, const long* __restrict__ raw_off, int* __restrict__ t
     }
     if (lane < 8) rk[n + lane] = -2;  // tail of the last 8-entry group
     __syncthreads();
-    const int nd = row_reduce<HBITS>(rk, rv, n, ht, dk, lane, tkey, tval, a0);
+    const int nd = row_reduce<HBITS>(rk, rv, n, ht, lane, tkey, tval, a0);
     if (nd < 0) {
       if (lane == 0) atomicOr(overflow, 1);
     } else if (lane == 0) {
