@@ -215,6 +215,40 @@ def test_mass_blocks_and_csr(ctx, dim):
     assert abs(f @ (M @ g) - 0.25) < 1e-12
 
 
+# A coarse A over a finer B gives long mass rows: node i's row holds 3 slots
+# per candidate pair of each element around i.  The ranges pick the 2D row
+# kernels: up to 320 slots, up to 512, the gathered rows up to 2048, and the
+# global sort beyond (also taken by rows with more than 256 columns).
+@pytest.mark.parametrize("na,nb,lo,hi", [(12, 20, 321, 512), (8, 24, 513, 2048), (4, 40, 2049, 10 ** 9)])
+def test_mass_long_rows(ctx, na, nb, lo, hi):
+    import scipy.sparse as sp
+    pa, ea = meshgen.split_grid(2, na, 5)
+    pb, eb = meshgen.split_grid(2, nb, 6, mesh_id=1)
+    A, B = gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pb, eb)
+    r = ctx.supermesh(A, B, flags=gpu.MASS)
+    out = gpu.fetch(r, 2, ["indptr", "rows", "indices", "counts"])
+    raw = np.zeros(pa.shape[0], np.int64)
+    for v in range(3):
+        np.add.at(raw, ea[:, v], 3 * np.diff(out["indptr"]))
+    assert lo <= raw.max() <= hi, raw.max()
+    want = O.mass_pairs(pa, ea, pb, eb, out["rows"], out["indices"])
+    scale = np.abs(want).max()
+    indptr, indices, values = gpu.fetch_mass(r, pa.shape[0])
+    assert np.all(np.diff(indptr) >= 0)
+    for i in range(pa.shape[0]):   # columns ascending within each row
+        assert np.all(np.diff(indices[indptr[i]:indptr[i + 1]]) > 0)
+    M = sp.csr_matrix((values, indices, indptr), shape=(pa.shape[0], pb.shape[0]))
+    ne = out["counts"] > 0
+    rr = np.repeat(ea[out["rows"][ne]], 3, axis=1).ravel()
+    cc = np.tile(eb[out["indices"][ne]], (1, 3)).ravel()
+    W = sp.coo_matrix((want[ne].ravel(), (rr, cc)), shape=M.shape).tocsr()
+    W.sum_duplicates()
+    assert M.nnz == W.nnz
+    assert abs(M - W).max() <= 1e-12 * scale
+    assert abs(M.sum() - 1.0) < 1e-12
+    assert abs(pa[:, 0] @ (M @ pb[:, 1]) - 0.25) < 1e-12
+
+
 def test_reference_integrals_gpu(ctx, fixture_meshes):
     ref = json.load(open(os.path.join(HERE, "golden", "reference_stats.json")))
     for key, fa_of, dim in [("integral_2d", lambda p: p[:, 0], 2),
