@@ -215,21 +215,29 @@ def test_mass_blocks_and_csr(ctx, dim):
     assert abs(f @ (M @ g) - 0.25) < 1e-12
 
 
-# A coarse A over a finer B gives long mass rows: node i's row holds 3 slots
-# per candidate pair of each element around i.  The ranges pick the 2D row
-# kernels: up to 320 slots, up to 512, the gathered rows up to 2048, and the
-# global sort beyond (also taken by rows with more than 256 columns).
-@pytest.mark.parametrize("na,nb,lo,hi", [(12, 20, 321, 512), (8, 24, 513, 2048), (4, 40, 2049, 10 ** 9)])
-def test_mass_long_rows(ctx, na, nb, lo, hi):
+# Mesh-size ratios give mass rows of chosen lengths, so that every row kernel
+# runs.  2D: node i's row holds 3 slots per candidate pair of each element
+# around i (up to 320 slots, up to 512, gathered rows up to 2048, the global
+# sort beyond).  3D: 4 entries per non-empty pair (up to 512, up to 2048, the
+# global sort).  Rows with more than 256 columns also take the global sort.
+@pytest.mark.parametrize("dim,na,nb,lo,hi", [(2, 12, 20, 321, 512), (2, 8, 24, 513, 2048),
+                                             (2, 4, 40, 2049, 10 ** 9), (3, 1, 2, 1, 512),
+                                             (3, 2, 3, 513, 2048), (3, 6, 8, 2049, 10 ** 9)])
+def test_mass_long_rows(ctx, dim, na, nb, lo, hi):
     import scipy.sparse as sp
-    pa, ea = meshgen.split_grid(2, na, 5)
-    pb, eb = meshgen.split_grid(2, nb, 6, mesh_id=1)
+    pa, ea = meshgen.split_grid(dim, na, 5)
+    pb, eb = meshgen.split_grid(dim, nb, 6, mesh_id=1)
+    loc = dim + 1
     A, B = gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pb, eb)
     r = ctx.supermesh(A, B, flags=gpu.MASS)
-    out = gpu.fetch(r, 2, ["indptr", "rows", "indices", "counts"])
+    out = gpu.fetch(r, dim, ["indptr", "rows", "indices", "counts"])
+    if dim == 2:
+        per_element = np.diff(out["indptr"])
+    else:
+        per_element = np.bincount(out["rows"][out["counts"] > 0], minlength=ea.shape[0])
     raw = np.zeros(pa.shape[0], np.int64)
-    for v in range(3):
-        np.add.at(raw, ea[:, v], 3 * np.diff(out["indptr"]))
+    for v in range(loc):
+        np.add.at(raw, ea[:, v], loc * per_element)
     assert lo <= raw.max() <= hi, raw.max()
     want = O.mass_pairs(pa, ea, pb, eb, out["rows"], out["indices"])
     scale = np.abs(want).max()
@@ -239,8 +247,8 @@ def test_mass_long_rows(ctx, na, nb, lo, hi):
         assert np.all(np.diff(indices[indptr[i]:indptr[i + 1]]) > 0)
     M = sp.csr_matrix((values, indices, indptr), shape=(pa.shape[0], pb.shape[0]))
     ne = out["counts"] > 0
-    rr = np.repeat(ea[out["rows"][ne]], 3, axis=1).ravel()
-    cc = np.tile(eb[out["indices"][ne]], (1, 3)).ravel()
+    rr = np.repeat(ea[out["rows"][ne]], loc, axis=1).ravel()
+    cc = np.tile(eb[out["indices"][ne]], (1, loc)).ravel()
     W = sp.coo_matrix((want[ne].ravel(), (rr, cc)), shape=M.shape).tocsr()
     W.sum_duplicates()
     assert M.nnz == W.nnz
