@@ -392,9 +392,8 @@ def cpu_baseline(name, meshes):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on exactly one GPU: the path shards by "
-                          "Morton ranges of A across GPUs only by design (see DESIGN.md); "
-                          "run with --gpus 1"}))
+        print(json.dumps({"error": "runs on exactly one GPU: sharding A by Morton ranges "
+                          "across GPUs is designed but not built; run with --gpus 1"}))
         sys.exit(2)
     import numpy as np
     from libsupermesh_amd import gpu
