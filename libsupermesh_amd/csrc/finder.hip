@@ -693,13 +693,17 @@ k_query_wave(long na, const int* __restrict__ qperm, const double* __restrict__ 
 
 // Rows that fit the scratch: copy them into the CSR arrays.  A wave takes 64
 // consecutive queries and transposes their slot-major scratch through LDS in
-// 64-slot tiles, so that both the scratch reads and the row writes are
-// contiguous across the wave.
+// J-slot tiles, so that both the scratch reads and the row writes are
+// contiguous across the wave.  J is the smallest of 16, 32, 64 that holds the
+// longest row (or the scratch): short rows take a small tile, and more waves
+// share a CU.
+template <int J>
 __global__ void __launch_bounds__(kBlock)
 k_rows_from_scratch(long na, const int* __restrict__ qperm, int cap, const int* __restrict__ scratch,
                     const long* __restrict__ row_count, const long* __restrict__ indptr,
                     int* __restrict__ indices, int* __restrict__ rows) {
-  __shared__ int tile[kBlock / 64][64][65];
+  static_assert(J == 16 || J == 32 || J == 64, "tile height");
+  __shared__ int tile[kBlock / 64][J][65];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const long base = ((long)blockIdx.x * (kBlock / 64) + wv) * 64;
   if (base >= na) return;  // whole waves only; no block barriers below
@@ -718,9 +722,9 @@ k_rows_from_scratch(long na, const int* __restrict__ qperm, int cap, const int* 
     nmax = v > nmax ? v : nmax;
   }
   int (*T)[65] = tile[wv];
-  for (long s0 = 0; s0 < nmax; s0 += 64) {
+  for (long s0 = 0; s0 < nmax; s0 += J) {
     // tile[j][q] = slot s0 + j of query base + q
-    const int nj = nmax - s0 < 64 ? (int)(nmax - s0) : 64;
+    const int nj = nmax - s0 < J ? (int)(nmax - s0) : J;
     for (int j = 0; j < nj; j++) {
       const long s = s0 + j;
       T[j][lane] = (s < cap && t < na) ? scratch[s * na + t] : 0;
@@ -731,7 +735,7 @@ k_rows_from_scratch(long na, const int* __restrict__ qperm, int cap, const int* 
       const long nq = __shfl(n, q, 64), oq = __shfl(o, q, 64);
       const int aq = __shfl(a, q, 64);
       const long s = s0 + lane;
-      if (s < nq) {
+      if (lane < J && s < nq) {
         indices[oq + s] = T[lane][q];
         rows[oq + s] = aq;
       }
@@ -846,8 +850,16 @@ long find_pairs(Context& C, const DevMesh& A, const DevMesh& B, int finder, int 
   indices = C.indices.as<int>(npairs);
   rows = C.rows.as<int>(npairs);
   if (npairs > 0) {
-    hipLaunchKernelGGL(k_rows_from_scratch, grid, block, 0, C.stream, na, qperm, cap, scratch, row_count,
-                       indptr, indices, rows);
+    const long tall = maxrow < cap ? maxrow : cap;  // the most slots a copied row holds
+    if (tall <= 16)
+      hipLaunchKernelGGL(k_rows_from_scratch<16>, grid, block, 0, C.stream, na, qperm, cap, scratch, row_count,
+                         indptr, indices, rows);
+    else if (tall <= 32)
+      hipLaunchKernelGGL(k_rows_from_scratch<32>, grid, block, 0, C.stream, na, qperm, cap, scratch, row_count,
+                         indptr, indices, rows);
+    else
+      hipLaunchKernelGGL(k_rows_from_scratch<64>, grid, block, 0, C.stream, na, qperm, cap, scratch, row_count,
+                         indptr, indices, rows);
     if (maxrow > cap) {
       if (A.dim == 2) {
         if (closed) LSM_QUERY(2, true, true, cap);
