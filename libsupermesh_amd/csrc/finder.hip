@@ -694,15 +694,16 @@ k_query_wave(long na, const int* __restrict__ qperm, const double* __restrict__ 
 // Rows that fit the scratch: copy them into the CSR arrays.  A wave takes 64
 // consecutive queries and transposes their slot-major scratch through LDS in
 // J-slot tiles, so that both the scratch reads and the row writes are
-// contiguous across the wave.  J is the smallest of 16, 32, 64 that holds the
-// longest row (or the scratch): short rows take a small tile, and more waves
-// share a CU.
+// contiguous across the wave.  The tile is 16 slots tall when that holds the
+// longest row, else 32 (longer rows take several tiles): a small tile lets
+// more waves share a CU (8 and 4 per SIMD; 2 with 64 slots, which measured
+// slower on 3D rows).
 template <int J>
 __global__ void __launch_bounds__(kBlock)
 k_rows_from_scratch(long na, const int* __restrict__ qperm, int cap, const int* __restrict__ scratch,
                     const long* __restrict__ row_count, const long* __restrict__ indptr,
                     int* __restrict__ indices, int* __restrict__ rows) {
-  static_assert(J == 16 || J == 32 || J == 64, "tile height");
+  static_assert(J == 16 || J == 32, "tile height");
   __shared__ int tile[kBlock / 64][J][65];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const long base = ((long)blockIdx.x * (kBlock / 64) + wv) * 64;
@@ -854,11 +855,8 @@ long find_pairs(Context& C, const DevMesh& A, const DevMesh& B, int finder, int 
     if (tall <= 16)
       hipLaunchKernelGGL(k_rows_from_scratch<16>, grid, block, 0, C.stream, na, qperm, cap, scratch, row_count,
                          indptr, indices, rows);
-    else if (tall <= 32)
-      hipLaunchKernelGGL(k_rows_from_scratch<32>, grid, block, 0, C.stream, na, qperm, cap, scratch, row_count,
-                         indptr, indices, rows);
     else
-      hipLaunchKernelGGL(k_rows_from_scratch<64>, grid, block, 0, C.stream, na, qperm, cap, scratch, row_count,
+      hipLaunchKernelGGL(k_rows_from_scratch<32>, grid, block, 0, C.stream, na, qperm, cap, scratch, row_count,
                          indptr, indices, rows);
     if (maxrow > cap) {
       if (A.dim == 2) {
