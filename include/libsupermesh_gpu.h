@@ -102,7 +102,9 @@ void libsupermesh_gpu_context_destroy(libsupermesh_gpu_context* ctx);
 /* The context's hipStream_t, as void*. */
 void* libsupermesh_gpu_context_stream(libsupermesh_gpu_context* ctx);
 
-/* Text of the last error on this thread ("" if none) and its code. */
+/* Text of the last error on this thread and its code: every call but
+ * libsupermesh_gpu_free clears them when it succeeds ("" and 0), so after a
+ * failed call its message survives the clean-up of the caller's buffers. */
 const char* libsupermesh_gpu_last_error(void);
 int libsupermesh_gpu_last_error_code(void);
 
@@ -248,8 +250,20 @@ int libsupermesh_gpu_supermesh_callback(libsupermesh_gpu_context* ctx,
                                         const libsupermesh_gpu_mesh* a,
                                         const libsupermesh_gpu_mesh* b,
                                         libsupermesh_gpu_pair_callback callback, void* user);
+/* The same on meshes of any element type, as parallel_supermesh takes them
+ * (its intersection_calculation gets element_a(dim, loc_a) and
+ * element_b(dim, loc_b), Parallel_Supermesh.F90:105-120): loc_a / loc_b are
+ * dim+1 or 2^dim nodes per element (enlist holds loc ids per element), each
+ * pair is clipped as intersect_elements does (Supermesh.F90:162-209), and the
+ * callback receives element_a[loc_a][dim], element_b[loc_b][dim] and
+ * nodes_b[loc_b]; the clipped elements are simplices. */
+int libsupermesh_gpu_supermesh_callback_mixed(libsupermesh_gpu_context* ctx,
+                                              const libsupermesh_gpu_mesh* a, int loc_a,
+                                              const libsupermesh_gpu_mesh* b, int loc_b,
+                                              libsupermesh_gpu_pair_callback callback, void* user);
 
-/* Clip an explicit list of pairs (device arrays pair_a[n], pair_b[n]);
+/* Clip an explicit list of pairs (device arrays pair_a[n], pair_b[n], n <
+ * 2^31; ids are checked against the meshes' element counts);
  * fills counts / measure (and elements, blocks per flags; no mass CSR). */
 int libsupermesh_gpu_clip_pairs(libsupermesh_gpu_context* ctx, const libsupermesh_gpu_mesh* a,
                                 const libsupermesh_gpu_mesh* b, long npairs, const int* pair_a,

@@ -36,8 +36,20 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def _compile(src):
+def _stale(src, obj):
+    """True when obj is missing or older than src or any header it may include."""
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    deps = [os.path.join(CSRC, src)] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".hpp")]
+    deps += [os.path.join(ROOT, "include", h) for h in os.listdir(os.path.join(ROOT, "include"))]
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def _compile(src, only_stale=False):
     obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
+    if only_stale and not _stale(src, obj):
+        return obj
     cmd = [hipcc(), *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -64,8 +76,10 @@ def build_fortran(lib_path, verbose=True):
         raise RuntimeError(f"Fortran build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     test_src = os.path.join(ROOT, "tests", "fortran", "test_intersector_gpu.F90")
     exe = os.path.join(LIBDIR, "test_intersector_gpu")
-    cmd = [fc, "-O2", "-I", mod_dir, test_src, "-o", exe, "-L" + LIBDIR, "-lsupermesh_fortran",
-           "-lsupermesh_gpu", "-Wl,-rpath,$ORIGIN"]
+    test_mod_dir = os.path.join(mod_dir, "test")
+    os.makedirs(test_mod_dir, exist_ok=True)
+    cmd = [fc, "-O2", "-I", mod_dir, "-module-dir", test_mod_dir, test_src, "-o", exe, "-L" + LIBDIR,
+           "-lsupermesh_fortran", "-lsupermesh_gpu", "-Wl,-rpath,$ORIGIN"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"Fortran test build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
@@ -74,10 +88,12 @@ def build_fortran(lib_path, verbose=True):
     return flib
 
 
-def build(verbose=True):
+def build(verbose=True, only_stale=False):
+    """Compiles every source (only_stale: only those older than their sources
+    or headers) and links the library and the Fortran interface."""
     os.makedirs(OBJDIR, exist_ok=True)
     with concurrent.futures.ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+        objs = list(ex.map(lambda src: _compile(src, only_stale), SOURCES))
     out = os.path.join(LIBDIR, LIBNAME)
     tmp = out + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
@@ -92,5 +108,5 @@ def build(verbose=True):
 
 
 if __name__ == "__main__":
-    build()
+    build(only_stale="--stale" in sys.argv)
     sys.exit(0)

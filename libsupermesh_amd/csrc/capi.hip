@@ -72,7 +72,7 @@ static void free_context(Context& C) {
                     &C.row_count, &C.indptr, &C.indices, &C.rows, &C.counts, &C.measure,
                     &C.partials, &C.scalars, &C.offsets, &C.elements, &C.blocks, &C.mflag, &C.mslot, &C.work, &C.nwork,
                     &C.mkeys, &C.mvals, &C.mkeys_alt, &C.mvals_alt, &C.ukeys, &C.uvals, &C.nuniq,
-                    &C.mindptr, &C.mindices, &C.one_in, &C.one_out, &C.mesh_a.scale,
+                    &C.mindptr, &C.mindices, &C.mesh_a.scale,
                     &C.mesh_b.scale, &C.mesh_a.group, &C.mesh_b.group, &C.bvh.granges, &C.gpairs,
                     &C.gsum, &C.predictor.weights, &C.predictor.bias, &C.predictor.master,
                     &C.predictor.adam_m, &C.predictor.adam_v, &C.predictor.grad,
@@ -81,8 +81,6 @@ static void free_context(Context& C) {
                     &C.predictor.slots, &C.predictor.slot_elems, &C.predictor.overflow,
                     &C.predictor.arena, &C.predictor.aoff, &C.fscratch};
   for (DevBuf* b : bufs) b->release();
-  if (C.pinned) hipHostFree(C.pinned);
-  C.pinned = nullptr;
   if (C.hstat) hipHostFree(C.hstat);
   C.hstat = nullptr;
   if (C.events)
@@ -223,18 +221,27 @@ static void run_supermesh(Context& C, const libsupermesh_gpu_mesh* a, const libs
 // (Parallel_Supermesh.F90:436-453): closed-overlap candidates with A as the
 // searched mesh, B elements in order, the A candidates of each in ascending
 // id; every candidate pair, empty or not, is handed to the callback with
-// the clipped elements of intersect_elements(A, B).  Pairs are clipped on the
-// device in chunks and the callbacks run on the calling thread.
-static void run_callback_driver(Context& C, const libsupermesh_gpu_mesh* a,
-                                const libsupermesh_gpu_mesh* b, libsupermesh_gpu_pair_callback cb,
+// the clipped elements of intersect_elements(A, B) (Supermesh.F90:162-209:
+// simplices, quadrilaterals or hexahedra on either side, loc_a / loc_b
+// nodes per element).  Pairs are clipped on the device in chunks and the
+// callbacks run on the calling thread.
+static void run_callback_driver(Context& C, const libsupermesh_gpu_mesh* a, int loc_a,
+                                const libsupermesh_gpu_mesh* b, int loc_b, libsupermesh_gpu_pair_callback cb,
                                 void* user) {
+  check_mesh(a, "a");
+  check_mesh(b, "b");
+  const int dim = a->dim;
+  LSM_REQUIRE((loc_a == dim + 1 || loc_a == 1 << dim) && (loc_b == dim + 1 || loc_b == 1 << dim),
+              LIBSUPERMESH_GPU_ERROR_INVALID, "Invalid element type: loc must be dim+1 or 2^dim");
   libsupermesh_gpu_result r;
-  run_supermesh(C, b, a, LIBSUPERMESH_GPU_FINDER_RTREE, LIBSUPERMESH_GPU_SORTED, false, &r);
+  run_supermesh(C, b, a, LIBSUPERMESH_GPU_FINDER_RTREE, LIBSUPERMESH_GPU_SORTED, false, &r, 0, nullptr, nullptr,
+                loc_b, loc_a);
   const long npairs = r.npairs;
-  const int dim = a->dim, loc = dim + 1, per = loc * dim;
+  const int per = (dim + 1) * dim;
+  const bool mixed = loc_a != dim + 1 || loc_b != dim + 1;
   // host copies of both meshes, for the element coordinates and nodes_b
   std::vector<double> pos_a((size_t)a->nnodes * dim), pos_b((size_t)b->nnodes * dim);
-  std::vector<int> enl_a((size_t)a->nelements * loc), enl_b((size_t)b->nelements * loc);
+  std::vector<int> enl_a((size_t)a->nelements * loc_a), enl_b((size_t)b->nelements * loc_b);
   auto d2h = [&](void* dst, const void* src, size_t bytes) {
     if (bytes) LSM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, C.stream));
   };
@@ -246,14 +253,14 @@ static void run_callback_driver(Context& C, const libsupermesh_gpu_mesh* a,
   // pair lists: B element (CSR row) and A element (CSR entry) of each pair
   const int* d_pair_b = r.rows;
   const int* d_pair_a = r.indices;
-  stage_mesh(C, C.mesh_a, dim, loc, a->nnodes, a->nelements, a->positions, a->enlist, false, false);
-  stage_mesh(C, C.mesh_b, dim, loc, b->nnodes, b->nelements, b->positions, b->enlist, true, false);
+  stage_mesh(C, C.mesh_a, dim, loc_a, a->nnodes, a->nelements, a->positions, a->enlist, false, false);
+  stage_mesh(C, C.mesh_b, dim, loc_b, b->nnodes, b->nelements, b->positions, b->enlist, !mixed, false);
   constexpr long kChunk = 1L << 21;
   std::vector<int> pa, pb, counts;
   std::vector<long> offs;
   std::vector<double> elems;
-  double ea[4 * 3], eb[4 * 3];
-  long nodes_b[4];
+  double ea[8 * 3], eb[8 * 3];
+  long nodes_b[8];
   for (long p0 = 0; p0 < npairs; p0 += kChunk) {
     const long n = std::min(kChunk, npairs - p0);
     libsupermesh_gpu_result cr;
@@ -261,7 +268,8 @@ static void run_callback_driver(Context& C, const libsupermesh_gpu_mesh* a,
     cr.npairs = n;
     cr.rows = d_pair_a + p0;
     cr.indices = d_pair_b + p0;
-    clip_pairs(C, C.mesh_a, C.mesh_b, n, LIBSUPERMESH_GPU_ELEMENTS, &cr);
+    if (mixed) clip_pairs_mixed(C, C.mesh_a, C.mesh_b, n, LIBSUPERMESH_GPU_ELEMENTS, &cr);
+    else clip_pairs(C, C.mesh_a, C.mesh_b, n, LIBSUPERMESH_GPU_ELEMENTS, &cr);
     pa.resize(n);
     pb.resize(n);
     counts.resize(n);
@@ -275,17 +283,41 @@ static void run_callback_driver(Context& C, const libsupermesh_gpu_mesh* a,
     LSM_HIP(hipStreamSynchronize(C.stream));
     for (long i = 0; i < n; i++) {
       const long ia = pa[i], ib = pb[i];
-      for (int v = 0; v < loc; v++) {
-        const long na = enl_a[(size_t)ia * loc + v], nb = enl_b[(size_t)ib * loc + v];
+      for (int v = 0; v < loc_a; v++) {
+        const long na = enl_a[(size_t)ia * loc_a + v];
+        for (int d = 0; d < dim; d++) ea[v * dim + d] = pos_a[(size_t)na * dim + d];
+      }
+      for (int v = 0; v < loc_b; v++) {
+        const long nb = enl_b[(size_t)ib * loc_b + v];
         nodes_b[v] = nb;
-        for (int d = 0; d < dim; d++) {
-          ea[v * dim + d] = pos_a[(size_t)na * dim + d];
-          eb[v * dim + d] = pos_b[(size_t)nb * dim + d];
-        }
+        for (int d = 0; d < dim; d++) eb[v * dim + d] = pos_b[(size_t)nb * dim + d];
       }
       cb(ea, eb, elems.data() + (size_t)offs[i] * per, counts[i], nodes_b, ia, ib, user);
     }
   }
+}
+
+// Flags (bit 0) any pair whose A id is outside [0, na) or B id outside
+// [0, nb).
+__global__ void __launch_bounds__(256) k_check_pair_ids(long n, const int* __restrict__ pa,
+                                                        const int* __restrict__ pb, long na, long nb,
+                                                        int* __restrict__ flag) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && (pa[i] < 0 || pa[i] >= na || pb[i] < 0 || pb[i] >= nb)) *flag = 1;
+}
+
+// Checks caller-supplied pair lists against the mesh sizes before any kernel
+// reads coordinates through them.
+static void check_pair_ids(Context& C, long n, const int* pa, const int* pb, long na, long nb) {
+  if (n <= 0) return;
+  int* flag = stage_flag(C);
+  LSM_HIP(hipMemsetAsync(flag, 0, sizeof(int), C.stream));
+  hipLaunchKernelGGL(k_check_pair_ids, dim3(grid_for(n, 256)), dim3(256), 0, C.stream, n, pa, pb, na, nb, flag);
+  LSM_HIP(hipGetLastError());
+  int h = 0;
+  LSM_HIP(hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, C.stream));
+  LSM_HIP(hipStreamSynchronize(C.stream));
+  LSM_REQUIRE(h == 0, LIBSUPERMESH_GPU_ERROR_INVALID, "Invalid element: pair id out of range");
 }
 
 __global__ void __launch_bounds__(256) k_low_byte(long n, int* __restrict__ v) {
@@ -294,132 +326,111 @@ __global__ void __launch_bounds__(256) k_low_byte(long n, int* __restrict__ v) {
 }
 
 // ---------------------------------------------------------------------------
-// Per-pair kernels behind the reference's one-pair C functions
+// The reference's one-pair C functions (C_bindings.F90:42-64,134-174) run on
+// the calling thread: the same __host__ __device__ clippers as the kernels,
+// compiled for the host with the same flags (IEEE double, no contraction), so
+// the results are bitwise those of the bulk path.  No device, no locks: the
+// calls are re-entrant.  Each writes at most `cap` elements of `stride`
+// doubles to out and returns the number the reference reports.
 // ---------------------------------------------------------------------------
 
-__global__ void k_one(int kind, const double* __restrict__ in, double* __restrict__ out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  if (kind == 0) {  // intersect_tris_real
-    Line L[3];
-    tri_lines(in + 6, L);
-    double tol = tri_tol(triangle_area(in), triangle_area(in + 6));
-    int k = 0;
-    int n = clip_tri(in, L, tol, [&](double x0, double y0, double x1, double y1, double x2, double y2,
-                                     double) {
-      if (k < kTriBufSize) {
-        double* o = out + 1 + 6 * k;
-        o[0] = x0;
-        o[1] = y0;
-        o[2] = x1;
-        o[3] = y1;
-        o[4] = x2;
-        o[5] = y2;
-      }
-      k++;
-    });
-    out[0] = (double)n;
-  } else if (kind == 1) {  // intersect_tets_real
-    Planes P;
-    tet_planes(in + 12, P);
-    Tet A;
-    for (int i = 0; i < 12; i++) A.v[i] = in[i];
-    double tol = tet_tol(tet_volume_tet(in), tet_volume_tet(in + 12));
-    int k = 0;
-    int n = clip_tet(A, P, tol, [&](const Tet& T, double) {
-      if (k < kTetBufSize)
-        for (int i = 0; i < 12; i++) out[1 + 12 * k + i] = T.v[i];
-      k++;
-    });
-    out[0] = (double)n;
-  } else if (kind == 2) {
-    out[0] = triangle_area(in);
-  } else if (kind == 3) {
-    out[0] = tet_volume_real(in);
-  } else if (kind == 4) {  // intersect_tri_quad: tri (6) then quad (8)
-    Line L[4];
-    quad_lines(in + 6, L);
-    const double tol = 10.0 * fspacing(triangle_area(in));
-    int k = 0;
-    int n = clip_poly<4>(in, L, tol, [&](double x0, double y0, double x1, double y1, double x2,
-                                         double y2, double) {
-      if (k < kTriQuadBufSize) {
-        double* o = out + 1 + 6 * k;
-        o[0] = x0;
-        o[1] = y0;
-        o[2] = x1;
-        o[3] = y1;
-        o[4] = x2;
-        o[5] = y2;
-      }
-      k++;
-    });
-    out[0] = (double)n;
-  } else if (kind == 5) {  // intersect_tet_hex: tet (12) then hex (24)
-    int k = 0;
-    int n = clip_tet_hex(in, in + 12, [&](const double* t, double) {
-      if (k < kTetHexBufSize)
-        for (int i = 0; i < 12; i++) out[1 + 12 * k + i] = t[i];
-      k++;
-    });
-    out[0] = (double)n;
-  } else if (kind == 6) {  // intersect_quads: quad A (8) then quad B (8), tol 10 * tiny
-    Line L[4];
-    quad_lines(in + 8, L);
-    int k = 0;
-    int n = clip_poly<4, 4>(in, L, 10.0 * 2.2250738585072014e-308,
-                            [&](double x0, double y0, double x1, double y1, double x2, double y2, double) {
-                              if (k < kQuadsBufSize) {
-                                double* o = out + 1 + 6 * k;
-                                o[0] = x0;
-                                o[1] = y0;
-                                o[2] = x1;
-                                o[3] = y1;
-                                o[4] = x2;
-                                o[5] = y2;
-                              }
-                              k++;
-                            });
-    out[0] = (double)n;
-  } else {  // intersect_hexes: hex A (24) then hex B (24), A's five tets in turn
-    int k = 0;
-    for (int s = 0; s < 5; s++) {
-      double T[12];
-      hex_split_tet(in, s, T);
-      clip_tet_hex(T, in + 24, [&](const double* t, double) {
-        if (k < kHexesBufSize)
-          for (int i = 0; i < 12; i++) out[1 + 12 * k + i] = t[i];
-        k++;
-      });
+static int one_tris(const double* tri_a, const double* tri_b, double* out, int cap) {
+  Line L[3];
+  tri_lines(tri_b, L);
+  const double tol = tri_tol(triangle_area(tri_a), triangle_area(tri_b));
+  int k = 0;
+  return clip_tri(tri_a, L, tol, [&](double x0, double y0, double x1, double y1, double x2, double y2, double) {
+    if (k < cap) {
+      double* o = out + 6 * k;
+      o[0] = x0;
+      o[1] = y0;
+      o[2] = x1;
+      o[3] = y1;
+      o[4] = x2;
+      o[5] = y2;
     }
-    out[0] = (double)k;
-  }
+    k++;
+  });
 }
 
-constexpr int kOneIn = 48;                        // largest input: two hexahedra
-constexpr int kOneOut = 1 + 12 * kHexesBufSize;  // largest output: 5 x 729 tets
+static int one_tets(const double* tet_a, const double* tet_b, double* out, int cap) {
+  Planes P;
+  tet_planes(tet_b, P);
+  Tet A;
+  for (int i = 0; i < 12; i++) A.v[i] = tet_a[i];
+  const double tol = tet_tol(tet_volume_tet(tet_a), tet_volume_tet(tet_b));
+  int k = 0;
+  return clip_tet(A, P, tol, [&](const Tet& T, double) {
+    if (k < cap)
+      for (int i = 0; i < 12; i++) out[12 * k + i] = T.v[i];
+    k++;
+  });
+}
 
-// Runs k_one on the default context: nin doubles in, nout doubles back.
-static void one_call(int kind, const double* in, int nin, double* result, int nout) {
-  Context& C = *default_context();
-  std::lock_guard<std::mutex> lk(C.mu);
-  LSM_HIP(hipSetDevice(C.device));
-  LSM_REQUIRE(nin <= kOneIn && nout <= kOneOut, LIBSUPERMESH_GPU_ERROR_INVALID, "per-pair buffer size");
-  size_t need = sizeof(double) * (size_t)(kOneIn + kOneOut);
-  if (!C.pinned) {
-    LSM_HIP(hipHostMalloc(&C.pinned, 2 * need, hipHostMallocDefault));
-    C.pinned_bytes = 2 * need;
+// intersect_tri_quad (Supermesh.F90:211-230): the triangle clipped by the
+// quad's lines, tolerance from the triangle's area only.
+static int one_tri_quad(const double* tri, const double* quad, double* out, int cap) {
+  Line L[4];
+  quad_lines(quad, L);
+  const double tol = 10.0 * fspacing(triangle_area(tri));
+  int k = 0;
+  return clip_poly<4>(tri, L, tol, [&](double x0, double y0, double x1, double y1, double x2, double y2, double) {
+    if (k < cap) {
+      double* o = out + 6 * k;
+      o[0] = x0;
+      o[1] = y0;
+      o[2] = x1;
+      o[3] = y1;
+      o[4] = x2;
+      o[5] = y2;
+    }
+    k++;
+  });
+}
+
+// intersect_quads (Supermesh.F90:231-248): quad A by quad B's lines, tolerance
+// 10 * tiny.
+static int one_quads(const double* qa, const double* qb, double* out, int cap) {
+  Line L[4];
+  quad_lines(qb, L);
+  int k = 0;
+  return clip_poly<4, 4>(qa, L, 10.0 * 2.2250738585072014e-308,
+                         [&](double x0, double y0, double x1, double y1, double x2, double y2, double) {
+                           if (k < cap) {
+                             double* o = out + 6 * k;
+                             o[0] = x0;
+                             o[1] = y0;
+                             o[2] = x1;
+                             o[3] = y1;
+                             o[4] = x2;
+                             o[5] = y2;
+                           }
+                           k++;
+                         });
+}
+
+static int one_tet_hex(const double* tet, const double* hex, double* out, int cap) {
+  int k = 0;
+  return clip_tet_hex(tet, hex, [&](const double* t, double) {
+    if (k < cap)
+      for (int i = 0; i < 12; i++) out[12 * k + i] = t[i];
+    k++;
+  });
+}
+
+// intersect_hexes (Supermesh.F90:303-334): A's five tets in turn.
+static int one_hexes(const double* ha, const double* hb, double* out, int cap) {
+  int k = 0;
+  for (int s = 0; s < 5; s++) {
+    double T[12];
+    hex_split_tet(ha, s, T);
+    clip_tet_hex(T, hb, [&](const double* t, double) {
+      if (k < cap)
+        for (int i = 0; i < 12; i++) out[12 * k + i] = t[i];
+      k++;
+    });
   }
-  double* hin = static_cast<double*>(C.pinned);
-  double* hout = hin + need / sizeof(double);
-  std::memcpy(hin, in, sizeof(double) * nin);
-  double* din = C.one_in.as<double>(kOneIn);
-  double* dout = C.one_out.as<double>(kOneOut);
-  LSM_HIP(hipMemcpyAsync(din, hin, sizeof(double) * nin, hipMemcpyHostToDevice, C.stream));
-  hipLaunchKernelGGL(k_one, dim3(1), dim3(64), 0, C.stream, kind, din, dout);
-  LSM_HIP(hipGetLastError());
-  LSM_HIP(hipMemcpyAsync(hout, dout, sizeof(double) * nout, hipMemcpyDeviceToHost, C.stream));
-  LSM_HIP(hipStreamSynchronize(C.stream));
-  std::memcpy(result, hout, sizeof(double) * nout);
+  return k;
 }
 
 // ---------------------------------------------------------------------------
@@ -587,9 +598,13 @@ int libsupermesh_gpu_malloc(int device, size_t bytes, void** ptr) {
 }
 
 int libsupermesh_gpu_free(void* ptr) {
-  return guarded([&] {
-    if (ptr) LSM_HIP(hipFree(ptr));
-  });
+  // a successful free leaves the last error as it was: callers free their
+  // buffers after a failed call and still read its message
+  if (!ptr) return LIBSUPERMESH_GPU_OK;
+  const hipError_t e = hipFree(ptr);
+  if (e == hipSuccess) return LIBSUPERMESH_GPU_OK;
+  set_last_error(LIBSUPERMESH_GPU_ERROR_HIP, (std::string("hipFree failed: ") + hipGetErrorString(e)).c_str());
+  return LIBSUPERMESH_GPU_ERROR_HIP;
 }
 
 int libsupermesh_gpu_memcpy(void* dst, const void* src, size_t bytes, int kind) {
@@ -763,6 +778,7 @@ int libsupermesh_gpu_predictor_infer(libsupermesh_gpu_context* ctx, const libsup
     std::lock_guard<std::mutex> lk(C.mu);
     LSM_HIP(hipSetDevice(C.device));
     LSM_REQUIRE(C.predictor.ready, LIBSUPERMESH_GPU_ERROR_STATE, "no predictor weights loaded");
+    check_pair_ids(C, npairs, pair_a, pair_b, a->nelements, b->nelements);
     stage_mesh(C, C.mesh_a, 3, 4, a->nnodes, a->nelements, a->positions, a->enlist, false, false);
     stage_mesh(C, C.mesh_b, 3, 4, b->nnodes, b->nelements, b->positions, b->enlist, false, false);
     PredictArgs pa = {};
@@ -811,9 +827,23 @@ int libsupermesh_gpu_supermesh_callback(libsupermesh_gpu_context* ctx, const lib
   return guarded([&] {
     LSM_REQUIRE(ctx != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID, "null context");
     LSM_REQUIRE(callback != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID, "null callback");
+    check_mesh(a, "a");
+    check_mesh(b, "b");
     Context& C = ctx->ctx;
     std::lock_guard<std::mutex> lk(C.mu);
-    run_callback_driver(C, a, b, callback, user);
+    run_callback_driver(C, a, a->dim + 1, b, b->dim + 1, callback, user);
+  });
+}
+
+int libsupermesh_gpu_supermesh_callback_mixed(libsupermesh_gpu_context* ctx, const libsupermesh_gpu_mesh* a,
+                                              int loc_a, const libsupermesh_gpu_mesh* b, int loc_b,
+                                              libsupermesh_gpu_pair_callback callback, void* user) {
+  return guarded([&] {
+    LSM_REQUIRE(ctx != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID, "null context");
+    LSM_REQUIRE(callback != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID, "null callback");
+    Context& C = ctx->ctx;
+    std::lock_guard<std::mutex> lk(C.mu);
+    run_callback_driver(C, a, loc_a, b, loc_b, callback, user);
   });
 }
 
@@ -828,10 +858,13 @@ int libsupermesh_gpu_clip_pairs(libsupermesh_gpu_context* ctx, const libsupermes
     LSM_REQUIRE(npairs >= 0 && (npairs == 0 || (pair_a && pair_b)), LIBSUPERMESH_GPU_ERROR_INVALID,
                 "invalid pair list");
     LSM_REQUIRE(out != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID, "null result");
+    // the clip kernels index their work lists with 32-bit pair numbers
+    LSM_REQUIRE(npairs < (1L << 31), LIBSUPERMESH_GPU_ERROR_OVERFLOW, "more than 2^31 - 1 pairs in one call");
     Context& C = ctx->ctx;
     std::lock_guard<std::mutex> lk(C.mu);
     LSM_HIP(hipSetDevice(C.device));
     std::memset(out, 0, sizeof(*out));
+    check_pair_ids(C, npairs, pair_a, pair_b, a->nelements, b->nelements);
     const bool mass = flags & LIBSUPERMESH_GPU_MASS;
     stage_mesh(C, C.mesh_a, a->dim, a->dim + 1, a->nnodes, a->nelements, a->positions, a->enlist,
                false, mass);
@@ -890,93 +923,39 @@ void libsupermesh_rtree_intersection_finder_get_output(long* nelements, long* ni
 }
 
 void libsupermesh_intersect_tris_real(double* tri_a, double* tri_b, double* tris_c, int* n_tris_c) {
-  int rc = guarded([&] {
-    double in[12], res[1 + 6 * kTriBufSize];
-    std::memcpy(in, tri_a, sizeof(double) * 6);
-    std::memcpy(in + 6, tri_b, sizeof(double) * 6);
-    one_call(0, in, 12, res, 1 + 6 * kTriBufSize);
-    int n = (int)res[0];
-    *n_tris_c = n;
-    std::memcpy(tris_c, res + 1, sizeof(double) * 6 * n);
-  });
-  if (rc != LIBSUPERMESH_GPU_OK) *n_tris_c = 0;
+  *n_tris_c = one_tris(tri_a, tri_b, tris_c, kTriBufSize);
 }
 
 void libsupermesh_intersect_tets_real(double* tet_a, double* tet_b, double* tets_c, int* n_tets_c) {
-  int rc = guarded([&] {
-    double in[24];
-    std::vector<double> res(1 + 12 * kTetBufSize);
-    std::memcpy(in, tet_a, sizeof(double) * 12);
-    std::memcpy(in + 12, tet_b, sizeof(double) * 12);
-    one_call(1, in, 24, res.data(), 1 + 12 * kTetBufSize);
-    int n = (int)res[0];
-    *n_tets_c = n;
-    std::memcpy(tets_c, res.data() + 1, sizeof(double) * 12 * n);
-  });
-  if (rc != LIBSUPERMESH_GPU_OK) *n_tets_c = 0;
+  *n_tets_c = one_tets(tet_a, tet_b, tets_c, kTetBufSize);
 }
 
-void libsupermesh_triangle_area(double* tri, double* area) {
-  guarded([&] { one_call(2, tri, 6, area, 1); });
-}
+void libsupermesh_triangle_area(double* tri, double* area) { *area = triangle_area(tri); }
 
-void libsupermesh_tetrahedron_volume(double* tet, double* volume) {
-  guarded([&] { one_call(3, tet, 12, volume, 1); });
-}
+void libsupermesh_tetrahedron_volume(double* tet, double* volume) { *volume = tet_volume_real(tet); }
 
 void libsupermesh_intersect_tri_quad(double* tri_a, double* quad_b, double* tris_c, int* n_tris_c) {
-  int rc = guarded([&] {
-    double in[14], res[1 + 6 * kTriQuadBufSize];
-    std::memcpy(in, tri_a, sizeof(double) * 6);
-    std::memcpy(in + 6, quad_b, sizeof(double) * 8);
-    one_call(4, in, 14, res, 1 + 6 * kTriQuadBufSize);
-    int n = (int)res[0];
-    *n_tris_c = n;
-    std::memcpy(tris_c, res + 1, sizeof(double) * 6 * n);
-  });
-  if (rc != LIBSUPERMESH_GPU_OK) *n_tris_c = 0;
+  *n_tris_c = one_tri_quad(tri_a, quad_b, tris_c, kTriQuadBufSize);
 }
 
 void libsupermesh_intersect_tet_hex(double* tet_a, double* hex_b, double* tets_c, int* n_tets_c) {
-  int rc = guarded([&] {
-    double in[36];
-    std::vector<double> res(1 + 12 * kTetHexBufSize);
-    std::memcpy(in, tet_a, sizeof(double) * 12);
-    std::memcpy(in + 12, hex_b, sizeof(double) * 24);
-    one_call(5, in, 36, res.data(), 1 + 12 * kTetHexBufSize);
-    int n = (int)res[0];
-    *n_tets_c = n;
-    std::memcpy(tets_c, res.data() + 1, sizeof(double) * 12 * n);
-  });
-  if (rc != LIBSUPERMESH_GPU_OK) *n_tets_c = 0;
+  *n_tets_c = one_tet_hex(tet_a, hex_b, tets_c, kTetHexBufSize);
 }
 
 void libsupermesh_intersect_quads(double* quad_a, double* quad_b, double* tris_c, int* n_tris_c) {
   int rc = guarded([&] {
-    double in[16];
-    std::memcpy(in, quad_a, sizeof(double) * 8);
-    std::memcpy(in + 8, quad_b, sizeof(double) * 8);
-    std::vector<double> res(1 + 6 * kQuadsBufSize);
-    one_call(6, in, 16, res.data(), (int)res.size());
-    const int n = (int)res[0];
+    const int n = one_quads(quad_a, quad_b, tris_c, kQuadsBufSize);
     LSM_REQUIRE(n <= kQuadsBufSize, LIBSUPERMESH_GPU_ERROR_OVERFLOW, "intersect_quads: more than 62 triangles");
     *n_tris_c = n;
-    std::memcpy(tris_c, res.data() + 1, sizeof(double) * 6 * n);
   });
   if (rc != LIBSUPERMESH_GPU_OK) *n_tris_c = 0;
 }
 
 void libsupermesh_intersect_hexes(double* hex_a, double* hex_b, double* tets_c, int* n_tets_c) {
   int rc = guarded([&] {
-    double in[48];
-    std::memcpy(in, hex_a, sizeof(double) * 24);
-    std::memcpy(in + 24, hex_b, sizeof(double) * 24);
-    std::vector<double> res(1 + 12 * kHexesBufSize);
-    one_call(7, in, 48, res.data(), (int)res.size());
-    const int n = (int)res[0];
+    const int n = one_hexes(hex_a, hex_b, tets_c, kHexesBufSize);
     LSM_REQUIRE(n <= kHexesBufSize, LIBSUPERMESH_GPU_ERROR_OVERFLOW, "intersect_hexes: more than 3645 tets");
     *n_tets_c = n;
-    std::memcpy(tets_c, res.data() + 1, sizeof(double) * 12 * n);
   });
   if (rc != LIBSUPERMESH_GPU_OK) *n_tets_c = 0;
 }

@@ -135,10 +135,6 @@ struct Context {
   DevBuf mflag, mslot, mkeys, mvals, mkeys_alt, mvals_alt, ukeys, uvals, nuniq, mindptr, mindices;
   // batches: per-mesh pair offsets and measure sums
   DevBuf gpairs, gsum;
-  // per-pair host API staging
-  DevBuf one_in, one_out;
-  void* pinned = nullptr;
-  size_t pinned_bytes = 0;
   // pinned status words the pipeline copies back asynchronously and reads
   // after its next synchronisation: [0] staging flag, [1] longest mass row
   long* hstat = nullptr;

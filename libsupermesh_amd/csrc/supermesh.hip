@@ -1074,6 +1074,8 @@ static void launch_hex_walk(Context& C, int la, int lb, const HexWalkArgs& g) {
 
 void clip_pairs_mixed(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int flags,
                       libsupermesh_gpu_result* out) {
+  LSM_REQUIRE(npairs < (1L << 31), LIBSUPERMESH_GPU_ERROR_OVERFLOW,
+              "more than 2^31 - 1 candidate pairs to clip in one call");
   const int dim = A.dim;
   MixedArgs g = {};
   g.npairs = npairs;
@@ -1366,6 +1368,9 @@ bool mass_prepare_rows(Context& C, const DevMesh& A, const DevMesh& B, const lon
 
 void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int flags,
                 libsupermesh_gpu_result* out) {
+  // work lists, overflow lists and the single pass's items are 32-bit pair numbers
+  LSM_REQUIRE(npairs < (1L << 31), LIBSUPERMESH_GPU_ERROR_OVERFLOW,
+              "more than 2^31 - 1 candidate pairs to clip in one call");
   const int dim = A.dim, loc = dim + 1;
   const bool mass = flags & LIBSUPERMESH_GPU_MASS;
   const bool elems = flags & LIBSUPERMESH_GPU_ELEMENTS;

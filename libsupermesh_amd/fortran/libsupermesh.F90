@@ -105,7 +105,7 @@ module libsupermesh
   end type gpu_mesh
 
   procedure(intersection_calculation_interface), pointer, save :: user_calculation => null()
-  integer, save :: callback_dim = 0
+  integer, save :: callback_dim = 0, callback_loc_a = 0, callback_loc_b = 0
   type(c_ptr), save :: gpu_context = c_null_ptr
 
   interface tree_intersection_finder
@@ -317,6 +317,16 @@ module libsupermesh
       type(c_ptr), value :: user
       integer(kind = c_int) :: c_supermesh_callback
     end function c_supermesh_callback
+    function c_supermesh_callback_mixed(ctx, a, loc_a, b, loc_b, callback, user) &
+        & bind(c, name = "libsupermesh_gpu_supermesh_callback_mixed")
+      import :: c_int, c_ptr, c_funptr, gpu_mesh
+      type(c_ptr), value :: ctx
+      type(gpu_mesh) :: a, b
+      integer(kind = c_int), value :: loc_a, loc_b
+      type(c_funptr), value :: callback
+      type(c_ptr), value :: user
+      integer(kind = c_int) :: c_supermesh_callback_mixed
+    end function c_supermesh_callback_mixed
     function c_last_error() bind(c, name = "libsupermesh_gpu_last_error")
       import :: c_ptr
       type(c_ptr) :: c_last_error
@@ -566,14 +576,18 @@ contains
   ! Callback driver: the single-GPU form of the self-self loop of
   ! parallel_supermesh (Parallel_Supermesh.F90:436-453) without the MPI
   ! exchange.  intersection_calculation is called for every closed-overlap
-  ! candidate pair, B element by B element, with local = .true.
+  ! candidate pair, B element by B element, with local = .true.  Elements may
+  ! be simplices, quadrilaterals or hexahedra (loc = size(enlist, 1)); each
+  ! pair is clipped as intersect_elements does.  On failure (bad node ids,
+  ! element types, device errors) no callback is made past the failure and
+  ! libsupermesh_gpu_error() holds the message.
   subroutine supermesh(positions_a, enlist_a, positions_b, enlist_b, intersection_calculation)
     real(kind = real_kind), dimension(:, :), intent(in) :: positions_a, positions_b
     integer, dimension(:, :), intent(in) :: enlist_a, enlist_b
     procedure(intersection_calculation_interface) :: intersection_calculation
     type(gpu_mesh) :: ma, mb
     type(c_ptr) :: bufs(4)
-    integer(kind = c_int) :: rc
+    integer(kind = c_int) :: rc, rc_free
     integer :: i
     bufs = c_null_ptr
     if(.not. c_associated(gpu_context)) then
@@ -585,11 +599,15 @@ contains
     if(rc == 0) then
       user_calculation => intersection_calculation
       callback_dim = size(positions_a, 1)
-      rc = c_supermesh_callback(gpu_context, ma, mb, c_funloc(callback_trampoline), c_null_ptr)
+      callback_loc_a = size(enlist_a, 1)
+      callback_loc_b = size(enlist_b, 1)
+      rc = c_supermesh_callback_mixed(gpu_context, ma, int(callback_loc_a, c_int), mb, &
+        & int(callback_loc_b, c_int), c_funloc(callback_trampoline), c_null_ptr)
       user_calculation => null()
     end if
+    ! a successful free leaves the message of a failed call above readable
     do i = 1, 4
-      if(c_associated(bufs(i))) rc = c_gpu_free(bufs(i))
+      if(c_associated(bufs(i))) rc_free = c_gpu_free(bufs(i))
     end do
   end subroutine supermesh
 
@@ -629,9 +647,9 @@ contains
     integer(kind = c_long), dimension(:), pointer :: nb
     integer :: d
     d = callback_dim
-    call c_f_pointer(element_a, ea, [d, d + 1])
-    call c_f_pointer(element_b, eb, [d, d + 1])
-    call c_f_pointer(nodes_b, nb, [d + 1])
+    call c_f_pointer(element_a, ea, [d, callback_loc_a])
+    call c_f_pointer(element_b, eb, [d, callback_loc_b])
+    call c_f_pointer(nodes_b, nb, [callback_loc_b])
     if(n_elements_c > 0) then
       call c_f_pointer(elements_c, ec, [d, d + 1, int(n_elements_c)])
     else

@@ -86,6 +86,9 @@ def lib():
         L.libsupermesh_gpu_supermesh_callback.argtypes = [c_void_p, POINTER(Mesh), POINTER(Mesh),
                                                           CALLBACK, c_void_p]
         L.libsupermesh_gpu_supermesh_callback.restype = c_int
+        L.libsupermesh_gpu_supermesh_callback_mixed.argtypes = [c_void_p, POINTER(Mesh), c_int,
+                                                                POINTER(Mesh), c_int, CALLBACK, c_void_p]
+        L.libsupermesh_gpu_supermesh_callback_mixed.restype = c_int
         L.libsupermesh_gpu_supermesh_batched.argtypes = [c_void_p, POINTER(MeshBatch),
                                                          POINTER(MeshBatch), c_int, c_int,
                                                          POINTER(Result), POINTER(BatchResult)]
@@ -596,27 +599,30 @@ class Context:
         """The callback driver: ``fn(element_a, element_b, elements_c, nodes_b,
         ele_a, ele_b)`` for every closed-overlap candidate pair, B element by B
         element, A candidates ascending (0-based ids, numpy arrays valid only
-        during the call)."""
+        during the call).  Meshes may hold any element type (a.loc, b.loc):
+        element_a is [a.loc, dim], element_b and nodes_b follow b.loc, and the
+        clipped elements are simplices, as intersect_elements gives them."""
         dim = a.struct.dim
         loc = dim + 1
+        la, lb = a.loc, b.loc
         err = []
 
         def tramp(pa, pb, pc, n, nodes, ea, eb, user):
             if err:
                 return
             try:
-                A = np.ctypeslib.as_array(pa, shape=(loc, dim))
-                B = np.ctypeslib.as_array(pb, shape=(loc, dim))
+                A = np.ctypeslib.as_array(pa, shape=(la, dim))
+                B = np.ctypeslib.as_array(pb, shape=(lb, dim))
                 Cc = (np.ctypeslib.as_array(pc, shape=(n, loc, dim)) if n > 0
                       else np.zeros((0, loc, dim)))
-                nb = np.ctypeslib.as_array(nodes, shape=(loc,))
+                nb = np.ctypeslib.as_array(nodes, shape=(lb,))
                 fn(A, B, Cc, nb, ea, eb)
             except BaseException as e:  # surfaced after the driver returns
                 err.append(e)
 
         cb = CALLBACK(tramp)
-        _check(lib().libsupermesh_gpu_supermesh_callback(self.handle, ctypes.byref(a.struct),
-                                                         ctypes.byref(b.struct), cb, None))
+        _check(lib().libsupermesh_gpu_supermesh_callback_mixed(self.handle, ctypes.byref(a.struct), la,
+                                                               ctypes.byref(b.struct), lb, cb, None))
         if err:
             raise err[0]
 

@@ -4,7 +4,10 @@
 ! Usage: test_intersector_gpu mesh_a.bin mesh_b.bin [mode]
 !   mode "pairs" (default): tree finder, then intersect_tris / intersect_tets
 !     per pair, summing triangle_area / tetrahedron_volume of the pieces;
-!   mode "callback": the supermesh callback driver, summing in the callback;
+!   mode "callback": the supermesh callback driver, summing in the callback
+!     (any element types);
+!   mode "badnode": the callback driver on mesh A with an out-of-range node
+!     id, which must leave an error message and make no callback;
 !   mode "mixed": simplex mesh A against quad / hex mesh B (R-tree finder,
 !     intersect_tri_quad / intersect_tet_hex), or 1D meshes (sort finder,
 !     intersect_intervals);
@@ -89,6 +92,14 @@ program test_intersector_gpu
   case("callback")
     call supermesh(pa, ea, pb, eb, accumulate)
     call check_error()
+  case("badnode")
+    ! an out-of-range node id must fail with a message and no callbacks
+    ea(1, 1) = size(pa, 2) + 5
+    call supermesh(pa, ea, pb, eb, accumulate)
+    if(len(libsupermesh_gpu_error()) == 0 .or. npairs /= 0) then
+      print "(a)", "error: supermesh accepted an invalid node id"
+      stop 1
+    end if
   case("mixed")
     allocate(indptr(size(ea, 2) + 1))
     if(dim == 1) then

@@ -58,6 +58,39 @@ def test_fortran_mixed_and_1d(tmp_path, fixture_meshes, a, b, total):
     assert abs(float(s) - total) < 100.0 * max(np.finfo(float).eps, O.spacing(total))
 
 
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran interface not built (no amdflang)")
+@pytest.mark.parametrize("a,b,total", [("square.1", "square.4", 1.0), ("cube.1", "cube.3", 0.5),
+                                       ("square.3", "square.4", 1.0)])
+def test_fortran_callback_driver_mixed(tmp_path, fixture_meshes, a, b, total):
+    # parallel_supermesh takes any element type (dim x loc_a, dim x loc_b) and
+    # clips with intersect_elements: quads / hexahedra reach the callback whole
+    pa, ea = fixture_meshes[a]
+    pb, eb = fixture_meshes[b]
+    fa, fb = tmp_path / "a.bin", tmp_path / "b.bin"
+    write_mesh(fa, pa, ea)
+    write_mesh(fb, pb, eb)
+    r = subprocess.run([EXE, str(fa), str(fb), "callback"], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    dim, n, pieces, s = r.stdout.split()
+    ip, ix = O.brute_force(pa, ea, pb, eb, closed=True)
+    assert int(n) == len(ix)
+    assert abs(float(s) - total) < 100.0 * max(np.finfo(float).eps, O.spacing(total))
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran interface not built (no amdflang)")
+def test_fortran_callback_driver_reports_errors(tmp_path, fixture_meshes):
+    # an out-of-range node id: supermesh() returns with the message readable
+    # through libsupermesh_gpu_error() after its buffers are freed
+    pa, ea = fixture_meshes["cube.1"]
+    pb, eb = fixture_meshes["cube.2"]
+    fa, fb = tmp_path / "a.bin", tmp_path / "b.bin"
+    write_mesh(fa, pa, ea)
+    write_mesh(fb, pb, eb)
+    r = subprocess.run([EXE, str(fa), str(fb), "badnode"], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.split()[1] == "0"
+
+
 def _run_driver(tmp_path, fixture_meshes, a, b, npairs, npieces, total, mode, closed):
     pa, ea = fixture_meshes[a]
     pb, eb = fixture_meshes[b]

@@ -5,6 +5,7 @@ bit-exact; per-pair measures and output element coordinates bit-exact (the
 clippers round exactly like the reference); global sums within 1e-12; mixed
 mass within 1e-12 relative.
 """
+import ctypes
 import json
 import os
 
@@ -361,6 +362,74 @@ def test_callback_driver(fixture_meshes, names):
     rows, cols = O.csr_to_pairs(ip, ix)
     expect = [(int(r), int(c)) for r, c in zip(rows, cols)]
     assert seen == expect
+
+
+@pytest.mark.parametrize("names", [("square.1", "square.4"), ("square.3", "square.2"),
+                                   ("cube.1", "cube.3"), ("cube.3", "cube.4")])
+def test_callback_driver_mixed_elements(fixture_meshes, names):
+    """Callback driver on quadrilateral / hexahedral meshes: the whole elements
+    reach the callback and every pair's pieces equal the oracle's
+    intersect_elements dispatch (Supermesh.F90:162-209)."""
+    pa, ea = fixture_meshes[names[0]]
+    pb, eb = fixture_meshes[names[1]]
+    dim = pa.shape[1]
+    ctx = gpu.Context()
+    A, B = gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pb, eb)
+    seen = []
+    total = [0.0]
+
+    def ref_pieces(xa, xb):
+        la, lb = len(xa), len(xb)
+        if dim == 2:
+            if la == 3 and lb == 3:
+                return O.intersect_tris(xa, xb)
+            if la == 3:
+                return O.intersect_tri_quad(xa, xb)
+            if lb == 3:
+                return O.intersect_tri_quad(xb, xa)
+            return O.intersect_quads(xa, xb)
+        if la == 4 and lb == 4:
+            return O.intersect_tets(xa, xb)
+        if la == 4:
+            return O.intersect_tet_hex(xa, xb)
+        if lb == 4:
+            return O.intersect_tet_hex(xb, xa)
+        return O.intersect_hexes(xa, xb)
+
+    def fn(xa, xb, xc, nodes_b, ia, ib):
+        assert np.array_equal(xa, pa[ea[ia]]) and np.array_equal(xb, pb[eb[ib]])
+        assert np.array_equal(nodes_b, eb[ib])
+        assert np.array_equal(xc, ref_pieces(xa, xb))
+        for t in xc:
+            total[0] += O.triangle_area(t) if dim == 2 else O.tet_volume_real(t)
+        seen.append((ib, ia))
+
+    ctx.supermesh_callback(A, B, fn)
+    ip, ix = O.brute_force(pb, eb, pa, ea, closed=True)
+    rows, cols = O.csr_to_pairs(ip, ix)
+    assert seen == [(int(r), int(c)) for r, c in zip(rows, cols)]
+    expect = {"square": 1.0, "cube": 0.5}[names[0].split(".")[0]]
+    assert abs(total[0] - expect) < 1e-12
+
+
+def test_clip_pairs_rejects_bad_pair_lists():
+    """Explicit pair lists: ids outside the meshes and lists of 2^31 pairs or
+    more are refused before any kernel reads through them."""
+    pa, ea = meshgen.split_grid(2, 4, 0)
+    ctx = gpu.Context()
+    A = gpu.DeviceMesh(pa, ea)
+    good = gpu.DeviceArray(np.zeros(3, dtype=np.int32))
+    bad = gpu.DeviceArray(np.array([0, 1, ea.shape[0]], dtype=np.int32))
+    with pytest.raises(gpu.GpuError, match="out of range"):
+        ctx.clip_pairs(A, A, good, bad)
+    with pytest.raises(gpu.GpuError, match="out of range"):
+        ctx.clip_pairs(A, A, gpu.DeviceArray(np.array([0, -1, 2], dtype=np.int32)), good)
+    r = gpu.Result()
+    rc = gpu.lib().libsupermesh_gpu_clip_pairs(ctx.handle, ctypes.byref(A.struct), ctypes.byref(A.struct),
+                                               1 << 31, good.ptr, good.ptr, 0, ctypes.byref(r))
+    assert rc == 4 and b"2^31" in gpu.lib().libsupermesh_gpu_last_error()
+    r = ctx.clip_pairs(A, A, good, good)   # the context is still usable
+    assert r.npairs == 3
 
 
 def test_device_checksum_matches_numpy():
