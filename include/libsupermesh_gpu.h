@@ -91,9 +91,13 @@ typedef struct {
    * [6] total, [7] the clip count kernel alone (the single pass with LEARNED) */
   double ms[8];
   /* LIBSUPERMESH_GPU_LEARNED: output slots allocated from the predicted
-   * budgets, and pairs clipped again because they outgrew their budget */
+   * budgets, pairs clipped again because they outgrew their budget, and the
+   * budget rule used: the quantile whose slots fit in device memory (the
+   * configured one or a lower level), or 0 when none fit and the plain
+   * count-then-fill path ran */
   long learned_slots;
   long learned_overflow;
+  double learned_quantile;
 } libsupermesh_gpu_result;
 
 /* Context on HIP device `device` with its own stream (default priority). */

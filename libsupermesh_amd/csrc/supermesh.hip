@@ -1466,6 +1466,7 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
     } else {
       g.bshift = 8 * level;
       P.level = level;
+      out->learned_quantile = pa.levels[level];
       LSM_HIP(hipMemsetAsync(slots, 0, sizeof(long), C.stream));
       if (nwork > 0) {
         long* wide = C.row_count.as<long>((size_t)nwork);

@@ -56,7 +56,8 @@ class Result(ctypes.Structure):
                 ("blocks", c_void_p), ("mass_nnz", c_long), ("mass_indptr", c_void_p),
                 ("mass_indices", c_void_p), ("mass_values", c_void_p), ("sum", c_double),
                 ("nonempty", c_long), ("nout", c_long), ("max_count", c_int),
-                ("ms", c_double * 8), ("learned_slots", c_long), ("learned_overflow", c_long)]
+                ("ms", c_double * 8), ("learned_slots", c_long), ("learned_overflow", c_long),
+                ("learned_quantile", c_double)]
 
 
 # libsupermesh_gpu_pair_callback

@@ -33,6 +33,8 @@
  *                                 closed variant = R-tree predicate,
  *                                 R-Tree_Intersection_Finder_C++.h:233-251
  *                                 with libspatialindex closed Region overlap
+ *   orc_grid_finder               the brute-force set above, pruned by a
+ *                                 uniform grid of cells (full-size checks)
  *   orc_tree_*                    Quadtree_Intersection_Finder.F90:174-420,
  *                                 Octree_Intersection_Finder.F90:153-411
  *   orc_mass_pairs,               test_parallel_p1_inner_product_2d.F90:59-68,
@@ -789,6 +791,180 @@ long orc_brute_force_finder(int dim, int closed,
     indptr[a + 1] = nnz;
   }
   free(bbs);
+  return nnz;
+}
+
+/* The same strict / closed brute-force set, found through a uniform grid of
+ * cells so that it runs at the full workload sizes.  B elements are binned by
+ * the cell of their box's low corner; cells are at least as wide as every
+ * box of A and B, so a B box that overlaps an A box has its low corner within
+ * (A.lo - E, A.hi] (E the widest B box) per axis, i.e. in a cell between
+ * cell(A.lo - E) and cell(A.hi).  One cell of slack on each side covers the
+ * rounding of the cell arithmetic, and every B element in those cells is
+ * tested with the exact predicate of orc_brute_force_finder, so the set is
+ * that of the all-pairs loop.  Rows come out ascending.  Threads take
+ * contiguous ranges of A. */
+typedef struct {
+  int dim, closed, ncell[3];
+  double lo[3], width[3];
+  const double *bbs_b;
+  const long *cstart;
+  const int *cele;
+  long a0, a1;
+  int loc_a;
+  const double *pos_a;
+  const int *enl_a;
+  double ext_b[3];
+  long *counts; /* [a1 - a0] */
+  int *hits;    /* grown as needed */
+  long nhits, cap;
+} grid_job;
+
+static int cell_of(const grid_job *g, int d, double x) {
+  double f = floor((x - g->lo[d]) / g->width[d]);
+  if (f < 0) return 0;
+  if (f > g->ncell[d] - 1) return g->ncell[d] - 1;
+  return (int)f;
+}
+
+static int cmp_int(const void *x, const void *y) {
+  int a = *(const int *)x, b = *(const int *)y;
+  return (a > b) - (a < b);
+}
+
+static void *grid_worker(void *arg) {
+  grid_job *g = (grid_job *)arg;
+  const int dim = g->dim;
+  for (long a = g->a0; a < g->a1; a++) {
+    double ba[6];
+    element_bbox(dim, g->loc_a, g->pos_a, g->enl_a + a * g->loc_a, ba);
+    int c0[3] = {0, 0, 0}, c1[3] = {0, 0, 0};
+    for (int d = 0; d < dim; d++) {
+      c0[d] = cell_of(g, d, ba[d] - g->ext_b[d]) - 1;
+      c1[d] = cell_of(g, d, ba[3 + d]) + 1;
+      if (c0[d] < 0) c0[d] = 0;
+      if (c1[d] > g->ncell[d] - 1) c1[d] = g->ncell[d] - 1;
+    }
+    const long start = g->nhits;
+    for (int z = c0[2]; z <= c1[2]; z++)
+      for (int y = c0[1]; y <= c1[1]; y++)
+        for (int x = c0[0]; x <= c1[0]; x++) {
+          const long c = ((long)z * g->ncell[1] + y) * g->ncell[0] + x;
+          for (long k = g->cstart[c]; k < g->cstart[c + 1]; k++) {
+            const int b = g->cele[k];
+            const double *bb = g->bbs_b + 6 * (long)b;
+            int hit = g->closed ? boxes_closed(dim, ba, bb) : boxes_strict(dim, ba, bb);
+            if (hit) {
+              if (g->nhits == g->cap) {
+                g->cap = g->cap * 2 + 1024;
+                g->hits = (int *)realloc(g->hits, sizeof(int) * (size_t)g->cap);
+              }
+              g->hits[g->nhits++] = b;
+            }
+          }
+        }
+    qsort(g->hits + start, (size_t)(g->nhits - start), sizeof(int), cmp_int);
+    g->counts[a - g->a0] = g->nhits - start;
+  }
+  return NULL;
+}
+
+long orc_grid_finder(int dim, int closed, int nthreads,
+                     long ne_a, int loc_a, const double *pos_a, const int *enl_a,
+                     long ne_b, int loc_b, const double *pos_b, const int *enl_b,
+                     long *indptr, int **indices_out) {
+  double *bbs = (double *)malloc(sizeof(double) * 6 * (size_t)(ne_b > 0 ? ne_b : 1));
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}, ext[3] = {0, 0, 0};
+  for (long b = 0; b < ne_b; b++) {
+    double *bb = bbs + 6 * b;
+    element_bbox(dim, loc_b, pos_b, enl_b + b * loc_b, bb);
+    for (int d = 0; d < dim; d++) {
+      if (b == 0 || bb[d] < lo[d]) lo[d] = bb[d];
+      if (b == 0 || bb[d] > hi[d]) hi[d] = bb[d];
+      if (bb[3 + d] - bb[d] > ext[d]) ext[d] = bb[3 + d] - bb[d];
+    }
+  }
+  for (long a = 0; a < ne_a; a++) {
+    double ba[6];
+    element_bbox(dim, loc_a, pos_a, enl_a + a * loc_a, ba);
+    for (int d = 0; d < dim; d++)
+      if (ba[3 + d] - ba[d] > ext[d]) ext[d] = ba[3 + d] - ba[d];
+  }
+  grid_job proto;
+  memset(&proto, 0, sizeof(proto));
+  proto.dim = dim;
+  proto.closed = closed;
+  const int cap = dim == 2 ? 4096 : 256;
+  long ncells = 1;
+  for (int d = 0; d < 3; d++) {
+    proto.ncell[d] = 1;
+    proto.lo[d] = lo[d];
+    proto.width[d] = 1.0;
+    if (d < dim) {
+      double w = ext[d] > 0 ? ext[d] : 1.0;
+      double span = hi[d] - lo[d];
+      long n = (long)(span / w) + 1;
+      if (n > cap) {
+        n = cap;
+        w = span / (double)(cap - 1);
+        if (w < ext[d]) w = ext[d];
+      }
+      if (n < 1) n = 1;
+      proto.ncell[d] = (int)n;
+      proto.width[d] = w;
+      proto.ext_b[d] = ext[d];
+    }
+    ncells *= proto.ncell[d];
+  }
+  long *cstart = (long *)calloc((size_t)ncells + 1, sizeof(long));
+  int *cele = (int *)malloc(sizeof(int) * (size_t)(ne_b > 0 ? ne_b : 1));
+  long *cell_of_b = (long *)malloc(sizeof(long) * (size_t)(ne_b > 0 ? ne_b : 1));
+  for (long b = 0; b < ne_b; b++) {
+    long c = 0;
+    for (int d = dim - 1; d >= 0; d--) c = c * proto.ncell[d] + cell_of(&proto, d, bbs[6 * b + d]);
+    cell_of_b[b] = c;
+    cstart[c + 1]++;
+  }
+  for (long c = 0; c < ncells; c++) cstart[c + 1] += cstart[c];
+  long *fill = (long *)malloc(sizeof(long) * (size_t)ncells);
+  memcpy(fill, cstart, sizeof(long) * (size_t)ncells);
+  for (long b = 0; b < ne_b; b++) cele[fill[cell_of_b[b]]++] = (int)b;
+  free(fill);
+  free(cell_of_b);
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 256) nthreads = 256;
+  grid_job jobs[256];
+  pthread_t th[256];
+  for (int t = 0; t < nthreads; t++) {
+    jobs[t] = proto;
+    jobs[t].bbs_b = bbs;
+    jobs[t].cstart = cstart;
+    jobs[t].cele = cele;
+    jobs[t].loc_a = loc_a;
+    jobs[t].pos_a = pos_a;
+    jobs[t].enl_a = enl_a;
+    jobs[t].a0 = ne_a * t / nthreads;
+    jobs[t].a1 = ne_a * (t + 1) / nthreads;
+    jobs[t].counts = indptr + 1 + jobs[t].a0;
+  }
+  if (nthreads == 1) {
+    grid_worker(&jobs[0]);
+  } else {
+    for (int t = 0; t < nthreads; t++) pthread_create(&th[t], NULL, grid_worker, &jobs[t]);
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+  }
+  indptr[0] = 0;
+  for (long a = 0; a < ne_a; a++) indptr[a + 1] += indptr[a];
+  const long nnz = indptr[ne_a];
+  int *out = (int *)malloc(sizeof(int) * (size_t)(nnz > 0 ? nnz : 1));
+  for (int t = 0; t < nthreads; t++) {
+    if (jobs[t].nhits) memcpy(out + indptr[jobs[t].a0], jobs[t].hits, sizeof(int) * (size_t)jobs[t].nhits);
+    free(jobs[t].hits);
+  }
+  free(cstart);
+  free(cele);
+  free(bbs);
+  *indices_out = out;
   return nnz;
 }
 

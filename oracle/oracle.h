@@ -89,6 +89,12 @@ long orc_brute_force_finder(int dim, int closed,
                             long *indptr, int *indices);
 
 typedef struct orc_tree orc_tree;
+/* The brute-force set through a uniform grid of cells (any size, threaded).
+ * Rows ascending; *indices is allocated here (free with orc_free). */
+long orc_grid_finder(int dim, int closed, int nthreads,
+                     long ne_a, int loc_a, const double *pos_a, const int *enl_a,
+                     long ne_b, int loc_b, const double *pos_b, const int *enl_b,
+                     long *indptr, int **indices);
 orc_tree *orc_tree_build(int dim, long nnodes_b, const double *pos_b,
                          long ne_b, int loc_b, const int *enl_b);
 void orc_tree_free(orc_tree *t);

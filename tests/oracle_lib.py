@@ -85,6 +85,11 @@ def lib():
                                                  _l, _l, ctypes.c_int, _dp, _lp, _lp,
                                                  ctypes.POINTER(_lp)]
         L.orc_free.argtypes = [ctypes.c_void_p]
+        L.orc_grid_finder.restype = ctypes.c_long
+        L.orc_grid_finder.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_long, ctypes.c_int, _dp, _ip,
+                                      ctypes.c_long, ctypes.c_int, _dp, _ip,
+                                      _lp, ctypes.POINTER(_ip)]
         _lib = L
     return _lib
 
@@ -227,6 +232,23 @@ def brute_force(pos_a, enl_a, pos_b, enl_b, closed=False):
     indices = np.zeros(max(int(indptr[-1]), 1), dtype=np.int32)
     L.orc_brute_force_finder(*args, indptr.ctypes.data_as(_lp), _i(indices))
     return indptr, indices[:indptr[-1]]
+
+
+def grid_finder(pos_a, enl_a, pos_b, enl_b, closed=False, nthreads=8):
+    """The brute-force set (rows ascending) through a uniform grid of cells:
+    the same predicate as brute_force, fast enough for the full workloads."""
+    pos_a, pos_b = _f64(pos_a), _f64(pos_b)
+    enl_a, enl_b = _i32(enl_a), _i32(enl_b)
+    ne_a = enl_a.shape[0]
+    indptr = np.zeros(ne_a + 1, dtype=np.int64)
+    out = _ip()
+    L = lib()
+    nnz = L.orc_grid_finder(pos_a.shape[1], int(closed), nthreads, ne_a, enl_a.shape[1], _d(pos_a), _i(enl_a),
+                            enl_b.shape[0], enl_b.shape[1], _d(pos_b), _i(enl_b), indptr.ctypes.data_as(_lp),
+                            ctypes.byref(out))
+    indices = np.ctypeslib.as_array(out, shape=(max(nnz, 1),))[:nnz].copy()
+    L.orc_free(ctypes.cast(out, ctypes.c_void_p))
+    return indptr, indices
 
 
 def tree_finder(pos_a, enl_a, pos_b, enl_b):

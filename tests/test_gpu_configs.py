@@ -1,0 +1,149 @@
+"""The benchmark workloads at full size, through the C ABI on the GPU.
+
+* 2D tri 1024^2 x 2 vs 1000^2 x 2 (jitter 0.3h, seed 1) with the P1 x P1
+  mixed mass: the candidate set equals the all-pairs strict set, every
+  pair's output count and area is bitwise the oracle's, |sum - 1| <= 1e-12,
+  1^T M 1 = 1 and f^T M g = int_0^1 int_0^1 x y = 1/4 for f = x, g = y
+  (exact for the degree-2 quadrature), and a sample of blocks matches the
+  oracle's quadrature within 1e-12 relative.
+* 3D tet 64^3 x 6 vs 60^3 x 6 (jitter 0.25h, seed 2): the same pair-set and
+  per-pair bars on all 97 M candidate pairs, |sum - 1| <= 1e-12.
+* 4096 mesh pairs of uneven sizes in batched calls: per mesh, the pair
+  count, the per-pair counts and measures, and the measure sum equal those
+  of one plain call per mesh pair; every |sum - 1| <= 1e-12.
+* tet 96^3 x 6 vs 90^3 x 6 (seed 4), materialised supermesh: the learned
+  single pass gives the plain count-then-fill path's counts, offsets and
+  output tets (device checksums), and reports the budget level it used.
+
+These follow the reference's serial loops (src/tests/test_tri_intersector.F90:89-104,
+test_tet_intersector.F90:89-104) at the benchmark sizes.  Each test prints
+its workload before its GPU step.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from libsupermesh_amd import gpu, meshgen
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+WEIGHTS = os.path.join(ROOT, "libsupermesh_amd", "data", "tet_counts_mlp.f32")
+
+
+def _threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def _say(msg):
+    print(f"[configs] {msg}", flush=True)
+
+
+def _check_pairs_and_clips(ctx, name, mass):
+    pa, ea, pb, eb = meshgen.config_meshes(name)
+    dim = pa.shape[1]
+    A, B = gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pb, eb)
+    _say(f"{name}: supermesh of {ea.shape[0]} x {eb.shape[0]} elements, mass={mass}")
+    r = ctx.supermesh(A, B, flags=gpu.SORTED | (gpu.MASS if mass else 0))
+    out = gpu.fetch(r, dim, ("indptr", "indices", "counts", "measure"))
+    nt = _threads()
+    ip, ix = O.grid_finder(pa, ea, pb, eb, nthreads=nt)
+    assert np.array_equal(out["indptr"], ip)
+    assert np.array_equal(out["indices"], ix)
+    rows, cols = O.csr_to_pairs(ip, ix)
+    cnt, meas = O.clip_pairs(pa, ea, pb, eb, rows, cols, nthreads=nt)
+    assert np.array_equal(out["counts"], cnt)
+    assert np.array_equal(out["measure"], meas)
+    assert abs(r.sum - 1.0) <= 1e-12
+    assert r.nout == int(cnt.sum()) and r.nonempty == int(np.count_nonzero(cnt))
+    _say(f"{name}: {r.npairs} pairs, {r.nout} elements, |sum-1| = {abs(r.sum - 1.0):.2e}")
+    return r, (pa, ea, pb, eb), (rows, cols)
+
+
+def test_tri_1024_vs_1000_with_mass():
+    ctx = gpu.Context()
+    try:
+        r, (pa, ea, pb, eb), (rows, cols) = _check_pairs_and_clips(ctx, "tri_1024_vs_1000", mass=True)
+        mi, mj, mv = gpu.fetch_mass(r, pa.shape[0])
+        assert mi[-1] == r.mass_nnz
+        assert abs(mv.sum() - 1.0) <= 1e-12
+        # f = x on A's nodes, g = y on B's: the P1 fields are exact, so
+        # f^T M g = int x y over the unit square
+        row_of = np.repeat(np.arange(pa.shape[0]), np.diff(mi))
+        fmg = np.sum(pa[row_of, 0] * mv * pb[mj, 1])
+        assert abs(fmg - 0.25) <= 1e-12
+        # blocks of a deterministic sample of pairs against the oracle's
+        # 3-point quadrature
+        blocks = gpu.fetch(r, 2, ("blocks",))["blocks"]
+        sel = np.arange(0, rows.shape[0], 19)
+        ref = O.mass_pairs(pa, ea, pb, eb, rows[sel], cols[sel], nthreads=_threads())
+        scale = np.abs(ref).max()
+        assert np.abs(blocks[sel] - ref).max() <= 1e-12 * scale
+    finally:
+        ctx.close()
+
+
+def test_tet_64_vs_60():
+    ctx = gpu.Context()
+    try:
+        _check_pairs_and_clips(ctx, "tet_64_vs_60", mass=False)
+    finally:
+        ctx.close()
+
+
+def test_batched_4096_mesh_pairs():
+    pairs = meshgen.batched_pairs(count=4096, seed=3)
+    ctx = gpu.Context()
+    try:
+        for dim in (2, 3):
+            sel = [k for k, p in enumerate(pairs) if p[0] == dim]
+            BA = gpu.DeviceMeshBatch([(pairs[k][1], pairs[k][2]) for k in sel])
+            BB = gpu.DeviceMeshBatch([(pairs[k][3], pairs[k][4]) for k in sel])
+            _say(f"batched {dim}D: {len(sel)} mesh pairs, {BA.element_offsets[-1]} x {BB.element_offsets[-1]} "
+                 "elements")
+            r, br = ctx.supermesh_batched(BA, BB, flags=gpu.SORTED)
+            poff, sums = gpu.fetch_batch(br)
+            got = gpu.fetch(r, dim, ("counts", "measure"))
+            assert np.all(np.abs(sums - 1.0) <= 1e-12), np.abs(sums - 1.0).max()
+            _say(f"batched {dim}D: {r.npairs} pairs; plain calls per mesh pair")
+            for m, k in enumerate(sel):
+                _, pa, ea, pb, eb = pairs[k]
+                rp = ctx.supermesh(gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pb, eb), flags=gpu.SORTED)
+                p0, p1 = int(poff[m]), int(poff[m + 1])
+                assert p1 - p0 == rp.npairs, k
+                plain = gpu.fetch(rp, dim, ("counts", "measure"))
+                assert np.array_equal(got["counts"][p0:p1], plain["counts"]), k
+                assert np.array_equal(got["measure"][p0:p1], plain["measure"]), k
+                assert abs(sums[m] - rp.sum) <= 1e-12, k
+    finally:
+        ctx.close()
+
+
+def test_learned_tet_96_vs_90_identical_to_plain():
+    pa, ea, pb, eb = meshgen.config_meshes("tet_96_vs_90")
+    ctx = gpu.Context()
+    try:
+        A, B = gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pb, eb)
+        ctx.predictor_set(np.fromfile(WEIGHTS, dtype=np.float32))
+        res = {}
+        for mode, flags in (("plain", gpu.ELEMENTS), ("learned", gpu.ELEMENTS | gpu.LEARNED)):
+            _say(f"tet_96_vs_90 {mode}: {ea.shape[0]} x {eb.shape[0]} tets, materialised")
+            r = ctx.supermesh(A, B, flags=flags)
+            res[mode] = (r.npairs, r.nout, r.sum, ctx.checksum(r.counts, 4 * r.npairs),
+                         ctx.checksum(r.offsets, 8 * (r.npairs + 1)), ctx.checksum(r.elements, 96 * r.nout))
+            if mode == "learned":
+                _say(f"learned: budget quantile {r.learned_quantile}, {r.learned_slots} slots, "
+                     f"{r.learned_overflow} pairs clipped again")
+                assert 0.0 < r.learned_quantile <= 0.9
+            assert abs(r.sum - 1.0) <= 1e-12
+        assert res["plain"] == res["learned"]
+        ip, _ = O.grid_finder(pa, ea, pb, eb, nthreads=_threads())
+        assert res["plain"][0] == ip[-1]
+    finally:
+        ctx.close()

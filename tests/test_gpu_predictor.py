@@ -102,3 +102,38 @@ def test_learned_single_pass_is_identical(meshes, weights, quantile, arena):
     if weights == "random" and arena == "tiny":
         assert r1.learned_overflow > 0    # the fallback path was exercised
     assert r1.learned_slots >= 0
+    assert 0.0 < r1.learned_quantile <= quantile
+
+
+@pytest.mark.parametrize("weights", ["trained", "random"])
+def test_learned_single_pass_with_mass_is_identical(meshes, weights):
+    """LEARNED with MASS (the single pass that also integrates the mixed mass
+    blocks): counts, elements, blocks and the assembled mass CSR equal the
+    plain path's."""
+    pa, ea, pb, eb = meshes
+    ctx = gpu.Context()
+    A, B = gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pb, eb)
+    if weights == "trained":
+        ctx.predictor_collect(A, B, 1 << 20)
+        ctx.predictor_train(epochs=6, batch=512, lr=3e-3, seed=1)
+    else:
+        ctx.predictor_set(R.init(9, scale=3.0))
+    ctx.predictor_arena(0.0, 1)   # force overflow pairs through the fallback too
+    what = ["indptr", "indices", "counts", "measure", "offsets", "elements", "blocks"]
+    flags = gpu.ELEMENTS | gpu.SORTED | gpu.MASS
+    r0 = ctx.supermesh(A, B, flags=flags)
+    plain = gpu.fetch(r0, 3, what)
+    m0 = gpu.fetch_mass(r0, pa.shape[0])
+    r1 = ctx.supermesh(A, B, flags=flags | gpu.LEARNED)
+    learned = gpu.fetch(r1, 3, what)
+    m1 = gpu.fetch_mass(r1, pa.shape[0])
+    assert r1.learned_quantile > 0
+    for k in what:
+        assert np.array_equal(plain[k], learned[k]), k
+    for x, y in zip(m0, m1):
+        assert np.array_equal(x, y)
+    # and the blocks against the oracle's quadrature
+    rows, cols = O.csr_to_pairs(plain["indptr"], plain["indices"])
+    ref = O.mass_pairs(pa, ea, pb, eb, rows, cols)
+    scale = np.abs(ref).max()
+    assert np.abs(learned["blocks"] - ref).max() <= 1e-12 * scale

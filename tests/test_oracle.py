@@ -197,6 +197,40 @@ def test_tree_deep_split():
     assert np.array_equal(tree, brute)
 
 
+@pytest.mark.parametrize("closed", [False, True])
+@pytest.mark.parametrize("case", ["tri_64_vs_50", "fixture_2d", "fixture_3d", "touching", "far", "tiny",
+                                  "mixed_sizes"])
+def test_grid_finder_equals_brute_force(fixture_meshes, case, closed):
+    """The cell-pruned finder used for full-size checks returns exactly the
+    all-pairs set, rows ascending, on jittered and exact-touching grids,
+    meshes far from the origin, tiny meshes, and A and B of very different
+    element sizes."""
+    if case == "tri_64_vs_50":
+        pa, ea, pb, eb = meshgen.config_meshes("tri_64_vs_50")
+    elif case == "fixture_2d":
+        (pa, ea), (pb, eb) = fixture_meshes["triangle_0_01"], fixture_meshes["square_0_01"]
+    elif case == "fixture_3d":
+        (pa, ea), (pb, eb) = fixture_meshes["cube.1"], fixture_meshes["pyramid_0_05"]
+    elif case == "touching":
+        pa, ea = meshgen.split_grid(3, 6, 0, alpha=0.0, diagonal=0)
+        pb, eb = meshgen.split_grid(3, 3, 0, alpha=0.0, diagonal=1)
+    elif case == "far":
+        pa, ea = meshgen.split_grid(2, 20, 1)
+        pb, eb = meshgen.split_grid(2, 17, 2, mesh_id=1, diagonal=1)
+        pa, pb = pa * 3.0 + 1e6, pb * 3.0 + 1e6
+    elif case == "tiny":
+        pa, ea = meshgen.split_grid(3, 4, 1)
+        pb, eb = meshgen.split_grid(3, 5, 2, mesh_id=1)
+        pa, pb = pa * 1e-9, pb * 1e-9
+    else:
+        pa, ea = meshgen.split_grid(2, 40, 1)
+        pb, eb = meshgen.split_grid(2, 3, 2, mesh_id=1, diagonal=1)
+    ip, ix = O.grid_finder(pa, ea, pb, eb, closed=closed, nthreads=3)
+    bp, bx = O.brute_force(pa, ea, pb, eb, closed=closed)
+    assert np.array_equal(ip, bp)
+    assert np.array_equal(ix, bx)   # brute force rows are ascending too
+
+
 # --------------------------------------------------------------------------
 # Clippers against exact arithmetic and invariants
 # --------------------------------------------------------------------------
