@@ -26,16 +26,17 @@ METRIC = "clipped candidate pairs/s (2D tri, 3D tet); learned+batched vs plain, 
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix), FMA counted as 2 flops
 HBM_PEAK_GBS = 8000.0
 
-# FP64 operations (add, sub, mul, div, sqrt, compare, min/max, abs) the
-# reference algorithm executes per candidate pair, counted with the
-# instrumented CPU restatement (oracle/count_flops.py) on these workloads.
-FLOPS_PER_PAIR = {"tri_1024_vs_1000": 415.8, "tet_64_vs_60": 691.1}
+# FP64 arithmetic operations (add, sub, mul, div, sqrt; compares, abs, min
+# and max are not counted) the reference algorithm executes per candidate
+# pair, counted with the instrumented CPU restatement (oracle/count_flops.py)
+# on these workloads: clip + (2D) the mixed-mass quadrature.
+FLOPS_PER_PAIR = {"tri_1024_vs_1000": 108.6 + 268.0, "tet_64_vs_60": 544.6}
 
 # Algorithmic bytes per candidate pair of the clip count pass: the B element
 # gathered (tri 6 doubles + 3 ids = 60 B; tet 12 doubles + 4 ids = 112 B),
-# the pair's two ids (8 B), its count (4 B) and measure (8 B); with the mixed
-# mass (2D) also its 3x3 block (72 B) and its 9 row-slot entries (9 x 12 B).
-BYTES_PER_PAIR = {"tri_1024_vs_1000": 60 + 8 + 4 + 8 + 72 + 108, "tet_64_vs_60": 112 + 8 + 4 + 8}
+# the pair's CSR index (4 B), its count (4 B) and measure (8 B); with the
+# mixed mass (2D) also its 3x3 block (72 B).
+BYTES_PER_PAIR = {"tri_1024_vs_1000": 60 + 4 + 4 + 8 + 72, "tet_64_vs_60": 112 + 4 + 4 + 8}
 
 # HBM bytes per launch of the same kernel on the same workload, from the two
 # counter runs summarised in profiles/traffic_tri1024_tet64.md (FETCH_SIZE x
@@ -51,6 +52,7 @@ def parse():
     p.add_argument("--workload", default="tri_1024_vs_1000")
     p.add_argument("--no-3d", action="store_true", help="skip the 3D workload")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--no-dropin", action="store_true", help="skip the drop-in host-array finder timing")
     p.add_argument("--no-batched", action="store_true", help="skip the batched workload")
     p.add_argument("--no-mixed", action="store_true", help="skip the tri/quad and tet/hex workloads")
     p.add_argument("--batch-count", type=int, default=4096, help="mesh pairs in the batched workload")
@@ -66,12 +68,31 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def host_threads():
+# A one-GPU box gives its process a share of 16 host cores; worker pools are
+# sized to that share whatever the machine's core count.
+BOX_CPU_SHARE = 16
+
+
+def host_info():
     try:
-        n = len(os.sched_getaffinity(0))
+        aff = len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+        aff = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity_cores": aff, "cpu_model": model,
+            "threads_used": max(1, min(aff, BOX_CPU_SHARE))}
+
+
+def host_threads():
+    return host_info()["threads_used"]
 
 
 def gpu_workload(ctx, name, steps, warmup, mass):
@@ -286,7 +307,10 @@ def clip_roofline(name, w):
            "bytes_per_pair": bpp, "pairs_per_launch": w["pairs"], "kernel_ms": w["clip_kernel_ms"]}
     if fpp:
         tf = fpp * w["pairs"] / secs / 1e12
+        # bit-exactness forbids fused multiply-adds: half the FMA-counted peak
+        # is the attainable one for this code
         out["fp64"] = {"achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tf / FP64_PEAK_TFLOPS,
+                       "peak_without_fma": FP64_PEAK_TFLOPS / 2, "frac_without_fma": tf / (FP64_PEAK_TFLOPS / 2),
                        "flops_per_pair": fpp}
     return out
 
@@ -371,22 +395,54 @@ def learned_batched_workload(ctx, steps, warmup, weights, count=1024):
             "identical_outputs": same}
 
 
-def cpu_baseline(name, meshes):
-    """The CPU restatement of the reference's serial loop (tree finder, then
-    the per-pair clip), one thread per host core over contiguous ranges of A."""
+def cpu_baseline(name, meshes, mass):
+    """The CPU restatement of the reference's serial loop on the host cores
+    (tree finder, then the per-pair clip, and with mass the per-pair P1 x P1
+    block of the reference tests' quadrature contracted with linear fields),
+    one thread per core over contiguous ranges of A, on the whole workload;
+    and one thread on a bounded sample (the first tenth of A's elements)."""
+    import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     pa, ea, pb, eb = meshes
-    n = host_threads()
-    stats, s, (tb, tq) = O.supermesh(pa, ea, pb, eb, finder=0, nthreads=n)
-    return {
+    info = host_info()
+    n = info["threads_used"]
+    stats, s, (tb, tq) = O.supermesh(pa, ea, pb, eb, finder=0, nthreads=n, mass=mass)
+    sub = ea[: max(1, ea.shape[0] // 10)]
+    st1, _, (tb1, tq1) = O.supermesh(pa, sub, pb, eb, finder=0, nthreads=1, mass=mass)
+    what = "tree query + clip" + (" + mixed-mass blocks (f^T M g)" if mass else "")
+    out = {
         "value": stats["candidates"] / tq, "unit": "pairs/s", "cores": n, "kind": "port",
-        "sample": (f"whole {name} workload: serial tree build on B ({tb:.2f} s, excluded), "
-                   f"then tree query + clip of all {stats['candidates']} candidate pairs on "
-                   f"{n} threads ({tq:.2f} s); no mixed mass"),
+        "sample": (f"whole {name} workload: serial tree build on B ({tb:.2f} s, excluded), then {what} "
+                   f"of all {stats['candidates']} candidate pairs on {n} threads ({tq:.2f} s); "
+                   "single core: the same on the first tenth of A's elements "
+                   f"({st1['candidates']} pairs, {tq1:.2f} s); no mass CSR assembly"),
         "end_to_end_pairs_per_s": stats["candidates"] / (tb + tq),
+        "single_core_pairs_per_s": st1["candidates"] / tq1,
         "sum_abs_err": abs(s - 1.0),
+        "host": info,
     }
+    if mass:
+        out["fmg_abs_err"] = abs(stats["fmg"] - 0.25)
+    return out
+
+
+def dropin_finder(meshes, steps=3):
+    """The reference's drop-in C finder on host arrays:
+    libsupermesh_tree_intersection_finder_set_input -> query_output ->
+    get_output (C_bindings.F90:346-391), i.e. the H2D copy of both meshes, the
+    device search and the D2H copy of the CSR, all inside the timed call."""
+    import numpy as np
+    from libsupermesh_amd import gpu
+    pa, ea, pb, eb = meshes
+    ea64, eb64 = ea.astype(np.int64), eb.astype(np.int64)
+    ip, ix = gpu.tree_intersection_finder(pa, ea64, pb, eb64)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ip, ix = gpu.tree_intersection_finder(pa, ea64, pb, eb64)
+    dt = (time.perf_counter() - t0) / steps
+    return {"ms_per_call": dt * 1e3, "pairs": int(ix.shape[0]), "pairs_per_s": ix.shape[0] / dt,
+            "note": "host arrays in, host CSR out (PCIe included); not the bench value"}
 
 
 def main():
@@ -427,13 +483,17 @@ def main():
     roofline = clip_roofline(args.workload, main_w)
     if extra:
         extra["roofline"] = clip_roofline(extra["workload"], extra)
+    for w in (main_w, extra):
+        if w and not args.no_dropin:
+            w["dropin_finder"] = dropin_finder(w["_meshes"])
+            log(f"[bench] drop-in host finder {w['workload']}: {w['dropin_finder']['ms_per_call']:.1f} ms/call")
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline ...")
-        cpu = cpu_baseline(args.workload, main_w["_meshes"])
+        cpu = cpu_baseline(args.workload, main_w["_meshes"], mass=main_w["mass_nnz"] is not None)
         log(f"[bench] CPU baseline {cpu['value']:.3e} pairs/s on {cpu['cores']} threads")
         if extra:
-            extra["cpu_baseline"] = cpu_baseline(extra["workload"], extra["_meshes"])
+            extra["cpu_baseline"] = cpu_baseline(extra["workload"], extra["_meshes"], mass=False)
             log(f"[bench] CPU baseline {extra['workload']}: {extra['cpu_baseline']['value']:.3e} pairs/s")
     for w in (main_w, extra):
         if w:

@@ -44,8 +44,10 @@ void set_last_error(int code, const char* msg) {
 }
 
 static void clear_error() {
-  tl_code = 0;
-  tl_msg.clear();
+  if (tl_code != 0 || !tl_msg.empty()) {
+    tl_code = 0;
+    tl_msg.clear();
+  }
 }
 
 static void init_context(Context& C, int device) {
@@ -923,22 +925,32 @@ void libsupermesh_rtree_intersection_finder_get_output(long* nelements, long* ni
 }
 
 void libsupermesh_intersect_tris_real(double* tri_a, double* tri_b, double* tris_c, int* n_tris_c) {
+  clear_error();
   *n_tris_c = one_tris(tri_a, tri_b, tris_c, kTriBufSize);
 }
 
 void libsupermesh_intersect_tets_real(double* tet_a, double* tet_b, double* tets_c, int* n_tets_c) {
+  clear_error();
   *n_tets_c = one_tets(tet_a, tet_b, tets_c, kTetBufSize);
 }
 
-void libsupermesh_triangle_area(double* tri, double* area) { *area = triangle_area(tri); }
+void libsupermesh_triangle_area(double* tri, double* area) {
+  clear_error();
+  *area = triangle_area(tri);
+}
 
-void libsupermesh_tetrahedron_volume(double* tet, double* volume) { *volume = tet_volume_real(tet); }
+void libsupermesh_tetrahedron_volume(double* tet, double* volume) {
+  clear_error();
+  *volume = tet_volume_real(tet);
+}
 
 void libsupermesh_intersect_tri_quad(double* tri_a, double* quad_b, double* tris_c, int* n_tris_c) {
+  clear_error();
   *n_tris_c = one_tri_quad(tri_a, quad_b, tris_c, kTriQuadBufSize);
 }
 
 void libsupermesh_intersect_tet_hex(double* tet_a, double* hex_b, double* tets_c, int* n_tets_c) {
+  clear_error();
   *n_tets_c = one_tet_hex(tet_a, hex_b, tets_c, kTetHexBufSize);
 }
 

@@ -1460,7 +1460,7 @@ static double now_s(void) {
 }
 
 typedef struct {
-  int dim, finder;
+  int dim, finder, mass;
   long a0, a1;
   const double *pos_a, *pos_b;
   const int *enl_a, *enl_b;
@@ -1468,8 +1468,40 @@ typedef struct {
   const orc_tree *tree;
   const double *bbs_b;
   long long stats[4];
-  orc_ksum sum;
+  orc_ksum sum, fmg;
 } sm_job;
+
+/* The P1 x P1 mixed-mass block of one clipped pair (the reference tests'
+ * quadrature, as mass_worker) and its contribution to f^T M g for the linear
+ * fields f = x on A's nodes and g = y on B's. */
+static void pair_mass(int dim, const double *ea, const double *eb, const double *buf, int n, orc_ksum *fmg) {
+  int loc = dim + 1, esz = dim == 2 ? 6 : 12, nq = dim == 2 ? 3 : 4;
+  double wq = dim == 2 ? 1.0 / 3.0 : 1.0 / 4.0;
+  double blk[16];
+  for (int k = 0; k < loc * loc; k++) blk[k] = 0.0;
+  for (int c = 0; c < n; c++) {
+    const double *el = buf + esz * c;
+    double meas = measure_real(dim, el);
+    for (int q = 0; q < nq; q++) {
+      double x[3], fa[4], fb[4];
+      quad_point(dim, q, el, x);
+      if (dim == 2) {
+        basis_p1_2d(ea, x, fa);
+        basis_p1_2d(eb, x, fb);
+      } else {
+        basis_p1_3d(ea, x, fa);
+        basis_p1_3d(eb, x, fb);
+      }
+      double wm = wq * meas;
+      for (int a = 0; a < loc; a++)
+        for (int b = 0; b < loc; b++) blk[a * loc + b] = blk[a * loc + b] + wm * fa[a] * fb[b];
+    }
+  }
+  double s = 0.0;
+  for (int a = 0; a < loc; a++)
+    for (int b = 0; b < loc; b++) s = s + ea[a * dim] * blk[a * loc + b] * eb[b * dim + 1];
+  orc_ksum_add(fmg, s);
+}
 
 static void *sm_worker(void *arg) {
   sm_job *j = (sm_job *)arg;
@@ -1478,6 +1510,7 @@ static void *sm_worker(void *arg) {
   int *cand = (int *)malloc(sizeof(int) * (size_t)(j->ne_b > 0 ? j->ne_b : 1));
   double ea[12], eb[12], buf[ORC_TET_BUF_SIZE * 12];
   orc_ksum_init(&j->sum);
+  orc_ksum_init(&j->fmg);
   memset(j->stats, 0, sizeof(j->stats));
   for (long a = j->a0; a < j->a1; a++) {
     gather(dim, j->pos_a, j->enl_a, a, ea);
@@ -1501,6 +1534,7 @@ static void *sm_worker(void *arg) {
       j->stats[2] += n;
       if (n > j->stats[3]) j->stats[3] = n;
       for (int k = 0; k < n; k++) orc_ksum_add(&j->sum, measure_real(dim, buf + esz * k));
+      if (j->mass && n > 0) pair_mass(dim, ea, eb, buf, n, &j->fmg);
     }
   }
   free(seen);
@@ -1508,7 +1542,7 @@ static void *sm_worker(void *arg) {
   return NULL;
 }
 
-void orc_supermesh(int dim, int finder, int nthreads,
+void orc_supermesh(int dim, int finder, int nthreads, int mass,
                    long nnodes_a, long ne_a, const double *pos_a, const int *enl_a,
                    long nnodes_b, long ne_b, const double *pos_b, const int *enl_b,
                    long long *stats, double *sum_out, double *times) {
@@ -1532,6 +1566,7 @@ void orc_supermesh(int dim, int finder, int nthreads,
     sm_job *j = &jobs[t];
     j->dim = dim;
     j->finder = finder;
+    j->mass = mass;
     j->a0 = tree || finder != 0 ? ne_a * t / nthreads : 0;
     j->a1 = tree || finder != 0 ? ne_a * (t + 1) / nthreads : 0;
     j->pos_a = pos_a;
@@ -1549,15 +1584,19 @@ void orc_supermesh(int dim, int finder, int nthreads,
   }
   double t2 = now_s();
   memset(stats, 0, 4 * sizeof(long long));
-  orc_ksum total;
+  orc_ksum total, fmg;
   orc_ksum_init(&total);
+  orc_ksum_init(&fmg);
   for (int t = 0; t < nthreads; t++) {
     for (int k = 0; k < 3; k++) stats[k] += jobs[t].stats[k];
     if (jobs[t].stats[3] > stats[3]) stats[3] = jobs[t].stats[3];
     orc_ksum_add(&total, jobs[t].sum.sum);
     orc_ksum_add(&total, jobs[t].sum.error);
+    orc_ksum_add(&fmg, jobs[t].fmg.sum);
+    orc_ksum_add(&fmg, jobs[t].fmg.error);
   }
-  *sum_out = orc_ksum_value(&total);
+  sum_out[0] = orc_ksum_value(&total);
+  sum_out[1] = orc_ksum_value(&fmg);
   times[0] = t1 - t0;
   times[1] = t2 - t1;
   free(jobs);

@@ -140,9 +140,13 @@ void orc_inner_product(int dim, long npairs, const int *pair_a, const int *pair_
  * loop.  finder: 0 tree (strict), 1 brute force strict, 2 brute force closed.
  * Threads split the A elements into contiguous ranges; the tree is built once
  * (serially, as the reference does) and shared read-only.
+ * With mass, each non-empty pair's P1 x P1 mixed-mass block is integrated
+ * too (the reference tests' quadrature) and contracted with f = x on A's
+ * nodes and g = y on B's.
  * stats[0]=candidates, [1]=non-empty pairs, [2]=output elements,
- * [3]=max per pair.  times[0]=build seconds, [1]=query+clip seconds. */
-void orc_supermesh(int dim, int finder, int nthreads,
+ * [3]=max per pair.  sum_out[0] = Sigma measures, [1] = f^T M g (mass).
+ * times[0]=build seconds, [1]=query+clip(+mass) seconds. */
+void orc_supermesh(int dim, int finder, int nthreads, int mass,
                    long nnodes_a, long ne_a, const double *pos_a, const int *enl_a,
                    long nnodes_b, long ne_b, const double *pos_b, const int *enl_b,
                    long long *stats, double *sum_out, double *times);

@@ -60,7 +60,7 @@ def lib():
                                      _dp, ctypes.c_int]
         L.orc_inner_product.argtypes = [ctypes.c_int, ctypes.c_long, _ip, _ip, _dp, _ip, _dp,
                                         _dp, _ip, _dp, _dp, _dp]
-        L.orc_supermesh.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+        L.orc_supermesh.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_long, ctypes.c_long, _dp, _ip,
                                     ctypes.c_long, ctypes.c_long, _dp, _ip, _llp, _dp, _dp]
         for name in ("orc_intersect_tri_quad", "orc_intersect_quads", "orc_intersect_tet_hex",
@@ -313,19 +313,23 @@ def inner_product(pos_a, enl_a, field_a, pos_b, enl_b, field_b, pa, pb):
     return float(m[0]), float(s[0])
 
 
-def supermesh(pos_a, enl_a, pos_b, enl_b, finder=0, nthreads=1):
+def supermesh(pos_a, enl_a, pos_b, enl_b, finder=0, nthreads=1, mass=False):
     """finder 0: tree (strict); 1: brute force strict; 2: brute force closed.
-    Returns (stats dict, sum, (build_s, query_clip_s))."""
+    Returns (stats dict, sum, (build_s, query_clip_s)); with mass the stats
+    also hold "fmg", f^T M g for f = x on A and g = y on B."""
     pos_a, pos_b = _f64(pos_a), _f64(pos_b)
     enl_a, enl_b = _i32(enl_a), _i32(enl_b)
     stats = np.zeros(4, dtype=np.int64)
-    s = np.zeros(1)
+    s = np.zeros(2)
     t = np.zeros(2)
-    lib().orc_supermesh(pos_a.shape[1], finder, nthreads, pos_a.shape[0], enl_a.shape[0],
+    lib().orc_supermesh(pos_a.shape[1], finder, nthreads, int(mass), pos_a.shape[0], enl_a.shape[0],
                         _d(pos_a), _i(enl_a), pos_b.shape[0], enl_b.shape[0], _d(pos_b),
                         _i(enl_b), stats.ctypes.data_as(_llp), _d(s), _d(t))
     keys = ("candidates", "nonempty", "elements", "max_per_pair")
-    return dict(zip(keys, (int(x) for x in stats))), float(s[0]), (float(t[0]), float(t[1]))
+    out = dict(zip(keys, (int(x) for x in stats)))
+    if mass:
+        out["fmg"] = float(s[1])
+    return out, float(s[0]), (float(t[0]), float(t[1]))
 
 
 def load_fixture_meshes():

@@ -137,3 +137,14 @@ def test_reentrant_across_threads(L):
     for th in threads:
         th.join()
     assert not bad
+
+
+def test_success_clears_the_last_error(L):
+    """Like every other call, a per-pair call that succeeds clears the thread's
+    last error (a failed call before it leaves one)."""
+    L.libsupermesh_gpu_last_error.restype = ctypes.c_char_p
+    n = ctypes.c_long(0)
+    L.libsupermesh_sort_intersection_finder_query_output(ctypes.byref(n))   # before set_input
+    assert L.libsupermesh_gpu_last_error_code() == 3
+    _call(L.libsupermesh_intersect_tris_real, [[0, 0], [1, 0], [0, 1]], [[0, 0], [1, 0], [0, 1]], (22, 3, 2))
+    assert L.libsupermesh_gpu_last_error_code() == 0 and L.libsupermesh_gpu_last_error() == b""
