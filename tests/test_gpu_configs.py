@@ -140,7 +140,7 @@ def test_learned_tet_96_vs_90_identical_to_plain():
             if mode == "learned":
                 _say(f"learned: budget quantile {r.learned_quantile}, {r.learned_slots} slots, "
                      f"{r.learned_overflow} pairs clipped again")
-                assert 0.0 < r.learned_quantile <= 0.9
+                assert 0.0 < r.learned_quantile <= 0.9 + 1e-6   # a float32 level
             assert abs(r.sum - 1.0) <= 1e-12
         assert res["plain"] == res["learned"]
         ip, _ = O.grid_finder(pa, ea, pb, eb, nthreads=_threads())

@@ -102,7 +102,7 @@ def test_learned_single_pass_is_identical(meshes, weights, quantile, arena):
     if weights == "random" and arena == "tiny":
         assert r1.learned_overflow > 0    # the fallback path was exercised
     assert r1.learned_slots >= 0
-    assert 0.0 < r1.learned_quantile <= quantile
+    assert 0.0 < r1.learned_quantile <= quantile + 1e-6   # a float32 level
 
 
 @pytest.mark.parametrize("weights", ["trained", "random"])
