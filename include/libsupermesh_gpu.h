@@ -273,6 +273,24 @@ int libsupermesh_gpu_clip_pairs(libsupermesh_gpu_context* ctx, const libsupermes
                                 const libsupermesh_gpu_mesh* b, long npairs, const int* pair_a,
                                 const int* pair_b, int flags, libsupermesh_gpu_result* out);
 
+/* libsupermesh_gpu_clip_pairs with no device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Every argument is the caller's: device arrays pair_a[n],
+ * pair_b[n] in, counts[n], measure[n] and sum[1] (the compensated sum of the
+ * measures) out, all written asynchronously on `stream` (a hipStream_t; null
+ * for the null stream) of GPU `device`, which becomes the calling thread's
+ * current device.  Simplex meshes (dim 2 or 3) as libsupermesh_gpu_mesh,
+ * read directly (nothing is staged).  Counts and measures equal
+ * libsupermesh_gpu_clip_pairs's bit for bit; a pair whose element or node ids
+ * are out of range gets count -1 and measure 0.  n is at most 4194304
+ * (2^22): longer lists are refused (split them).  The pass's working memory
+ * is one fixed 16.1 MiB array per GPU, kept until the process ends and
+ * shared by every call on that GPU: calls that overlap on one GPU race. */
+int libsupermesh_gpu_clip_pairs_noalloc(int device, void* stream, const libsupermesh_gpu_mesh* a,
+                                        const libsupermesh_gpu_mesh* b, long npairs, const int* pair_a,
+                                        const int* pair_b, int* counts, double* measure, double* sum);
+
 #ifdef __cplusplus
 }
 #endif

@@ -880,6 +880,16 @@ int libsupermesh_gpu_clip_pairs(libsupermesh_gpu_context* ctx, const libsupermes
   });
 }
 
+int libsupermesh_gpu_clip_pairs_noalloc(int device, void* stream, const libsupermesh_gpu_mesh* a,
+                                        const libsupermesh_gpu_mesh* b, long npairs, const int* pair_a,
+                                        const int* pair_b, int* counts, double* measure, double* sum) {
+  return guarded([&] {
+    LSM_REQUIRE(a != nullptr && b != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID, "null mesh");
+    clip_pairs_noalloc(device, static_cast<hipStream_t>(stream), *a, *b, npairs, pair_a, pair_b, counts, measure,
+                       sum);
+  });
+}
+
 // ---- drop-in symbols (include/libsupermesh-c.h) ---------------------------
 
 void libsupermesh_tree_intersection_finder_set_input(long* nnodes_a, int* dim_a, long* nelements_a,

@@ -186,6 +186,11 @@ void advancing_front_finder(Context& C, int dim, long nn_a, long ne_a, int loc_a
 
 // Clipping of candidate pairs where A or B is not a simplex mesh (quads in
 // 2D, hexahedra in 3D).
+// Explicit pair list, no allocation: asynchronous on `stream`, results and
+// the measure sum written to the caller's device arrays (supermesh.hip).
+void clip_pairs_noalloc(int device, hipStream_t stream, const libsupermesh_gpu_mesh& a,
+                        const libsupermesh_gpu_mesh& b, long npairs, const int* pair_a, const int* pair_b,
+                        int* counts, double* measure, double* sum);
 void clip_pairs_mixed(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int flags,
                       libsupermesh_gpu_result* out);
 // Order-sensitive checksum of a device buffer (whole 32-bit words).

@@ -1569,6 +1569,232 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
 }
 
 // ---------------------------------------------------------------------------
+// Clipping an explicit pair list with no allocation, for callers that cannot
+// allocate device memory (code under stream capture, serving loops).  The
+// kernels read the meshes' positions and element lists directly (no staged
+// copies), each lane gathering its pair's two elements; the pass's only device
+// memory beyond the caller's inputs and outputs is the file-scope `scratch`
+// below: the 2D pairs handed from the LDS clipper to the streaming one, their
+// count, and the partial sums of the measure reduction.  Every call on a GPU
+// shares that GPU's copy, so calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+
+constexpr long kNoAllocMaxPairs = 1L << 22;
+constexpr size_t kScratchOlist = 0;                                         // int [kNoAllocMaxPairs]
+constexpr size_t kScratchCount = kScratchOlist + sizeof(int) * kNoAllocMaxPairs;  // unsigned long long
+constexpr size_t kScratchParts = kScratchCount + 256;                        // Partial [kClipGrid]
+constexpr size_t kScratchBytes = kScratchParts + sizeof(Partial) * kClipGrid;
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+struct DirectArgs {
+  long npairs;
+  const int* pair_a;
+  const int* pair_b;
+  long na, nb, nnodes_a, nnodes_b;
+  const double* pos_a;
+  const int* enl_a;
+  const double* pos_b;
+  const int* enl_b;
+  int* counts;
+  double* measure;
+  int* olist;
+  unsigned long long* nolist;
+};
+
+// The coordinates of element e (LOC = DIM + 1 nodes); false when a node id is
+// outside [0, nnodes) (node 0 is read instead).
+template <int DIM>
+__device__ __forceinline__ bool gather_element(const double* __restrict__ pos, const int* __restrict__ enl,
+                                               long nnodes, long e, double* c) {
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < DIM + 1; j++) {
+    long v = enl[e * (DIM + 1) + j];
+    if (v < 0 || v >= nnodes) {
+      ok = false;
+      v = 0;
+    }
+#pragma unroll
+    for (int d = 0; d < DIM; d++) c[j * DIM + d] = pos[v * DIM + d];
+  }
+  return ok;
+}
+
+// Both elements of pair p; false (and nothing read through a bad id) when an
+// element or node id is out of range.
+template <int DIM>
+__device__ __forceinline__ bool gather_pair(const DirectArgs& g, long p, double* A, double* B) {
+  const long a = g.pair_a[p], b = g.pair_b[p];
+  if (a < 0 || a >= g.na || b < 0 || b >= g.nb) return false;
+  const bool oa = gather_element<DIM>(g.pos_a, g.enl_a, g.nnodes_a, a, A);
+  const bool ob = gather_element<DIM>(g.pos_b, g.enl_b, g.nnodes_b, b, B);
+  return oa && ob;
+}
+
+__global__ void k_noalloc_zero(unsigned long long* __restrict__ n) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *n = 0;
+}
+
+// 2D: clip_poly_lds as k_clip_tri; pairs that outgrow the LDS polygon go to
+// the scratch list.  Invalid pairs get count -1 and measure 0.
+__global__ void __launch_bounds__(kTriBlock) k_clip_direct_tri(DirectArgs g) {
+  __shared__ double poly[LdsPoly<3, 3>::kPoints * 2 * kTriBlock];
+  double* q = poly + threadIdx.x;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < g.npairs; p += stride) {
+    double A[6], Bv[6];
+    if (!gather_pair<2>(g, p, A, Bv)) {
+      g.counts[p] = -1;
+      g.measure[p] = 0.0;
+      continue;
+    }
+    const double tol = tri_tol(triangle_area(A), triangle_area(Bv));
+    Line L[3];
+    tri_lines(Bv, L);
+    double m = 0.0;
+    const int nk = clip_poly_lds<3, 3, kTriBlock>(
+        A, L, tol, q, [&](double, double, double, double, double, double, double ar) { m = m + ar; });
+    if (nk < 0) {
+      g.olist[atomicAdd(g.nolist, 1ULL)] = (int)p;
+      continue;
+    }
+    g.counts[p] = nk;
+    g.measure[p] = nk == 0 ? 0.0 : m;
+  }
+}
+
+// 2D pairs of the scratch list on the streaming clipper.
+__global__ void __launch_bounds__(kClipBlock) k_clip_direct_tri_stream(DirectArgs g) {
+  const long n = (long)*g.nolist;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long w = (long)blockIdx.x * blockDim.x + threadIdx.x; w < n; w += stride) {
+    const long p = g.olist[w];
+    double A[6], Bv[6];
+    gather_pair<2>(g, p, A, Bv);  // valid: checked by k_clip_direct_tri
+    const double tol = tri_tol(triangle_area(A), triangle_area(Bv));
+    Line L[3];
+    tri_lines(Bv, L);
+    double m = 0.0;
+    const int nk = clip_poly<3>(A, L, tol, [&](double, double, double, double, double, double, double ar) {
+      m = m + ar;
+    });
+    g.counts[p] = nk;
+    g.measure[p] = nk == 0 ? 0.0 : m;
+  }
+}
+
+// 3D: B's planes per pair (get_planes_tet, as staged for the bulk path) and
+// the depth-first walk of k_clip_tet_walk, with the planes beside the tet
+// stack in LDS (17 words per lane: consecutive lanes fall in different banks).
+__global__ void __launch_bounds__(kWalkBlock) k_clip_direct_tet(DirectArgs g) {
+  __shared__ double stack[3 * 12 * kWalkBlock];
+  __shared__ double plane_lds[17 * kWalkBlock];
+  double* stk = stack + threadIdx.x;
+  double* planes = plane_lds + 17 * threadIdx.x;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < g.npairs; p += stride) {
+    double A[12], Bv[12];
+    if (!gather_pair<3>(g, p, A, Bv)) {
+      g.counts[p] = -1;
+      g.measure[p] = 0.0;
+      continue;
+    }
+    Planes P;
+    tet_planes(Bv, P);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+#pragma unroll
+      for (int k = 0; k < 3; k++) planes[4 * i + k] = P.n[i][k];
+      planes[4 * i + 3] = P.c[i];
+    }
+    const double tol = tet_tol(tet_volume_tet(A), tet_volume_tet(Bv));
+    TetWalkLds W;
+    double d0[4];
+    walk_distances(A, planes, d0);
+    W.cls[0] = pack_class(d0);
+    W.level = 1;
+    W.digits = 0;
+    double m = 0.0;
+    int nk = 0;
+    while (walk_step_lds<4, kWalkBlock>(W, A, stk, planes, tol, [&](const double* t, double) {
+      m = m + tet_volume_real(t);
+      nk++;
+    })) {
+    }
+    g.counts[p] = nk;
+    g.measure[p] = nk == 0 ? 0.0 : m;
+  }
+}
+
+// Sum of the measures: the double-double of k_reduce_partials.
+__global__ void k_noalloc_sum(const Partial* __restrict__ parts, double* __restrict__ sum) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *sum = parts[0].hi + parts[0].lo;
+}
+
+// This GPU's address of `scratch`, looked up once per device with that
+// device current (each GPU holds its own copy).  The array's name is a host
+// stand-in: host code uses only this address.
+static void* scratch_address(int device) {
+  static std::mutex mu;
+  static void* addr[64] = {};
+  LSM_REQUIRE(device >= 0 && device < 64, LIBSUPERMESH_GPU_ERROR_INVALID, "invalid device index");
+  std::lock_guard<std::mutex> lk(mu);
+  if (!addr[device]) {
+    void* ptr = nullptr;
+    LSM_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    LSM_REQUIRE(ptr != nullptr, LIBSUPERMESH_GPU_ERROR_HIP, "no device address for the scratch array");
+    addr[device] = ptr;
+  }
+  return addr[device];
+}
+
+void clip_pairs_noalloc(int device, hipStream_t stream, const libsupermesh_gpu_mesh& a,
+                        const libsupermesh_gpu_mesh& b, long npairs, const int* pair_a, const int* pair_b,
+                        int* counts, double* measure, double* sum) {
+  LSM_REQUIRE(a.dim == b.dim && (a.dim == 2 || a.dim == 3), LIBSUPERMESH_GPU_ERROR_INVALID, "Invalid dimension");
+  LSM_REQUIRE(npairs >= 0 && npairs <= kNoAllocMaxPairs, LIBSUPERMESH_GPU_ERROR_INVALID,
+              "npairs exceeds the no-allocation limit of 4194304 pairs per call (split the list)");
+  LSM_REQUIRE(sum != nullptr && (npairs == 0 || (pair_a && pair_b && counts && measure)),
+              LIBSUPERMESH_GPU_ERROR_INVALID, "null argument");
+  LSM_REQUIRE(a.nelements >= 0 && b.nelements >= 0 && (a.nelements == 0 || (a.nnodes > 0 && a.positions && a.enlist)) &&
+                  (b.nelements == 0 || (b.nnodes > 0 && b.positions && b.enlist)),
+              LIBSUPERMESH_GPU_ERROR_INVALID, "invalid mesh");
+  LSM_HIP(hipSetDevice(device));
+  unsigned char* base = static_cast<unsigned char*>(scratch_address(device));
+  DirectArgs g;
+  g.npairs = npairs;
+  g.pair_a = pair_a;
+  g.pair_b = pair_b;
+  g.na = a.nelements;
+  g.nb = b.nelements;
+  g.nnodes_a = a.nnodes;
+  g.nnodes_b = b.nnodes;
+  g.pos_a = a.positions;
+  g.enl_a = a.enlist;
+  g.pos_b = b.positions;
+  g.enl_b = b.enlist;
+  g.counts = counts;
+  g.measure = measure;
+  g.olist = reinterpret_cast<int*>(base + kScratchOlist);
+  g.nolist = reinterpret_cast<unsigned long long*>(base + kScratchCount);
+  Partial* parts = reinterpret_cast<Partial*>(base + kScratchParts);
+  if (a.dim == 2) {
+    hipLaunchKernelGGL(k_noalloc_zero, dim3(1), dim3(64), 0, stream, g.nolist);
+    if (npairs > 0) {
+      hipLaunchKernelGGL(k_clip_direct_tri, dim3(grid_for(npairs, kTriBlock)), dim3(kTriBlock), 0, stream, g);
+      hipLaunchKernelGGL(k_clip_direct_tri_stream, dim3(64), dim3(kClipBlock), 0, stream, g);
+    }
+  } else if (npairs > 0) {
+    hipLaunchKernelGGL(k_clip_direct_tet, dim3(grid_for(npairs, kWalkBlock)), dim3(kWalkBlock), 0, stream, g);
+  }
+  hipLaunchKernelGGL(k_reduce_pairs, dim3(kClipGrid), dim3(kClipBlock), 0, stream, npairs, counts, measure, parts);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kClipBlock), 0, stream, kClipGrid, parts);
+  hipLaunchKernelGGL(k_noalloc_sum, dim3(1), dim3(64), 0, stream, parts, sum);
+  LSM_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
 // Mixed mass assembly: (node_a, node_b, value) entries of non-empty pairs,
 // sorted by key (stable, so equal keys keep pair order), then summed per key
 // by a sequential walk -- deterministic.
