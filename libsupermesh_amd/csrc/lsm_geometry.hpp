@@ -644,80 +644,46 @@ struct Recipe {
 };
 
 LSM_HD Recipe make_recipe(int c, int ci) {
+  // Written without branches: lanes of a wave are in different split cases,
+  // and every case is a handful of integer selects.
   const int kind = c & 7;
   const int p0 = (c >> 5) & 3, p1 = (c >> 7) & 3, p2 = (c >> 9) & 3;
   const int n0 = (c >> 11) & 3, n1 = (c >> 13) & 3, n2 = (c >> 15) & 3;
   const int z0 = (c >> 17) & 3;
+  const bool k3 = kind == 3, k5 = kind == 5, k6 = kind == 6;
+  const bool k3c1 = k3 && ci == 1, k3c2 = k3 && ci == 2, k5c1 = k5 && ci == 1, k5c2 = k5 && ci == 2,
+             k6c1 = k6 && ci == 1;
   Recipe r;
-  r.sp[0] = r.sp[1] = r.sp[2] = 0;
-  r.sn[0] = r.sn[1] = r.sn[2] = 0;
-  r.src[0] = 0;
-  r.src[1] = 1;
-  r.src[2] = 2;
-  r.src[3] = 3;
-  // Slot-preserving children replace the outside vertices in place.
-  auto put = [&](int slot, int s) {
-    r.src[0] = slot == 0 ? s : r.src[0];
-    r.src[1] = slot == 1 ? s : r.src[1];
-    r.src[2] = slot == 2 ? s : r.src[2];
-    r.src[3] = slot == 3 ? s : r.src[3];
-  };
-  if (kind == 2) {  // +++-: outside vertices move to the plane
-    r.sp[0] = p0; r.sn[0] = n0;
-    r.sp[1] = p1; r.sn[1] = n0;
-    r.sp[2] = p2; r.sn[2] = n0;
-    put(p0, 4);
-    put(p1, 5);
-    put(p2, 6);
-  } else if (kind == 4) {  // ++-0
-    r.sp[0] = p0; r.sn[0] = n0;
-    r.sp[1] = p1; r.sn[1] = n0;
-    put(p0, 4);
-    put(p1, 5);
-  } else if (kind == 7) {  // +-00
-    r.sp[0] = p0; r.sn[0] = n0;
-    put(p0, 4);
-  } else if (kind == 3) {  // ++--
-    if (ci == 0) {
-      r.sp[0] = p0; r.sn[0] = n1;
-      r.sp[1] = p1; r.sn[1] = n0;
-      put(p0, 4);
-      put(p1, 5);
-    } else if (ci == 1) {
-      r.sp[0] = p1; r.sn[0] = n1;
-      r.sp[1] = p0; r.sn[1] = n1;
-      r.sp[2] = p1; r.sn[2] = n0;
-      r.src[0] = n1; r.src[1] = 4; r.src[2] = 5; r.src[3] = 6;
-    } else {
-      r.sp[0] = p0; r.sn[0] = n0;
-      r.sp[1] = p1; r.sn[1] = n0;
-      r.sp[2] = p0; r.sn[2] = n1;
-      r.src[0] = n0; r.src[1] = 4; r.src[2] = 5; r.src[3] = 6;
-    }
-  } else if (kind == 5) {  // +---
-    if (ci == 0) {
-      r.sp[0] = p0; r.sn[0] = n0;
-      put(p0, 4);
-    } else if (ci == 1) {
-      r.sp[0] = p0; r.sn[0] = n0;
-      r.sp[1] = p0; r.sn[1] = n1;
-      r.src[0] = 4; r.src[1] = n1; r.src[2] = n2; r.src[3] = 5;
-    } else {
-      r.sp[0] = p0; r.sn[0] = n1;
-      r.sp[1] = p0; r.sn[1] = n2;
-      r.sp[2] = p0; r.sn[2] = n0;
-      r.src[0] = n2; r.src[1] = 4; r.src[2] = 5; r.src[3] = 6;
-    }
-  } else if (kind == 6) {  // +--0
-    if (ci == 0) {
-      r.sp[0] = p0; r.sn[0] = n0;
-      put(p0, 4);
-    } else {
-      r.sp[0] = p0; r.sn[0] = n1;
-      r.sp[1] = p0; r.sn[1] = n0;
-      r.src[0] = 4; r.src[1] = z0; r.src[2] = n1; r.src[3] = 5;
-    }
+  // split points (outside, inside): +++- by default, (p_s, n0)
+  r.sp[0] = k3c1 ? p1 : p0;
+  r.sn[0] = (k3 && ci == 0) || k3c1 || k5c2 || k6c1 ? n1 : n0;
+  r.sp[1] = k3c1 ? p0 : (k3c2 ? p1 : (k5c1 || k5c2 || k6c1 ? p0 : p1));
+  r.sn[1] = k3c1 || k5c1 ? n1 : (k5c2 ? n2 : n0);
+  r.sp[2] = k3c1 ? p1 : (k3c2 || k5c2 ? p0 : p2);
+  r.sn[2] = k3c2 ? n1 : n0;
+  // children of the in-place kind keep the parent's slots and move the
+  // outside vertex of each split onto the plane: +++- (3 splits), ++-0 (2),
+  // +-00 (1), and child 0 of ++-- (2), +--- (1), +--0 (1)
+  const int nsplit = kind == 2 ? 3 : (kind == 4 || (k3 && ci == 0) ? 2 : (kind == 7 || ((k5 || k6) && ci == 0) ? 1 : 0));
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    int v = j;
+    v = nsplit >= 3 && j == r.sp[2] ? 6 : v;
+    v = nsplit >= 2 && j == r.sp[1] ? 5 : v;
+    v = nsplit >= 1 && j == r.sp[0] ? 4 : v;
+    r.src[j] = v;
   }
+  // the other children list their four slots: ++-- (n1 | n0, 4, 5, 6),
+  // +--- (4, n1, n2, 5) and (n2, 4, 5, 6), +--0 (4, z0, n1, 5)
+  const bool expl = k3c1 || k3c2 || k5c1 || k5c2 || k6c1;
+  const int e0 = k5c1 || k6c1 ? 4 : (k3c1 ? n1 : (k3c2 ? n0 : n2));
+  const int e1 = k5c1 ? n1 : (k6c1 ? z0 : 4);
+  const int e2 = k5c1 ? n2 : (k6c1 ? n1 : 5);
+  const int e3 = k5c1 || k6c1 ? 5 : 6;
+  r.src[0] = expl ? e0 : r.src[0];
+  r.src[1] = expl ? e1 : r.src[1];
+  r.src[2] = expl ? e2 : r.src[2];
+  r.src[3] = expl ? e3 : r.src[3];
   return r;
 }
 
@@ -764,83 +730,124 @@ LSM_HD void build_child(const double* T, const double* d, const Recipe& r, doubl
   }
 }
 
-// Walk state of one pair clipped by NP planes: tets at depths 1..NP-1 (depth
-// 0, tet A, is passed to each step), their packed classes against the next
-// plane, the depth of the next child to build (1..NP) and the next child index
-// per depth.  A parent's distances to its plane are recomputed when a child is
-// built (the same operations, hence the same values) rather than kept, to save
-// registers.  The planes are read through `planes` (NP x [n0 n1 n2 c]).
+// Depth-first walk over the clip tree of one pair and NP planes, in which
+// every step builds exactly one child: the next child is built (and, at depth
+// NP, handed to the sink; otherwise pushed and classified against the next
+// plane), then the walk climbs over every depth whose children are all done.
+// The lanes of a wave stay in lockstep: the work of a step does not depend on
+// the depth, the split case or whether the child is a leaf.  State: the class
+// of the current tet at each depth 0..NP-1 against the plane below it, the
+// depth of the next child to build (0 once the walk is over) and the index
+// of the next child at each depth 1..NP (2 bits each).  Tets at depths
+// 1..NP-1 live in the caller's stack: depth d at stk[((d - 1) * 12 + i) *
+// STRIDE], so a wave's lanes can interleave their stacks in LDS (STRIDE =
+// lanes) or a host caller use a plain array (STRIDE = 1).
 template <int NP>
-struct TetWalkN {
-  double T[NP - 1][12];
+struct WalkState {
   int cls[NP];
   int level;
-  int digits;  // 2 bits per depth 1..NP
+  int digits;
 };
-using TetWalk = TetWalkN<4>;
 
 LSM_HD void walk_distances(const double* T, const double* pl, double* d) {
 #pragma unroll
   for (int v = 0; v < 4; v++) d[v] = dot3(pl[0], pl[1], pl[2], T[3 * v], T[3 * v + 1], T[3 * v + 2]) - pl[3];
 }
 
+// Climbs while the children at the current depth are all done.
 template <int NP>
-LSM_HD void walk_start(TetWalkN<NP>& W, const double* A, const double* planes) {
-  double d[4];
-  walk_distances(A, planes, d);
-  W.cls[0] = pack_class(d);
+LSM_HD void walk_climb(WalkState<NP>& W) {
+#pragma unroll
+  for (int j = NP; j >= 1; j--) {
+    const bool done = W.level == j && ((W.digits >> (2 * j)) & 3) >= class_nchild(W.cls[j - 1]);
+    W.level = done ? j - 1 : W.level;
+  }
+}
+
+// Starts the walk of tet A, whose class against the first plane is cls0.
+template <int NP>
+LSM_HD void walk_begin(WalkState<NP>& W, int cls0) {
+  W.cls[0] = cls0;
+#pragma unroll
+  for (int j = 1; j < NP; j++) W.cls[j] = 0;
   W.level = 1;
   W.digits = 0;
+  walk_climb(W);
 }
 
-// One step.  Returns false once the pair's walk is complete.  Calls
-// sink(tet, vol_tet) for each kept output tet, in the reference's order.
-template <int NP, class Sink>
-LSM_HD bool walk_step(TetWalkN<NP>& W, const double* A, const double* planes, double tol, Sink&& sink) {
+// One step (W.level >= 1).  plane(j, out) gives plane j (0..NP-1) as
+// [n0 n1 n2 c].  Calls sink(tet, vol_tet, keep) once per step: keep is true
+// for the kept output tets, in the reference's order, and false otherwise
+// (the tet is then not an output).
+template <int NP, int STRIDE, class PlaneFn, class Sink>
+LSM_HD void walk_next(WalkState<NP>& W, const double* A, double* stk, PlaneFn&& plane, double tol,
+                      Sink&& sink) {
   const int k = W.level;
   const int ci = (W.digits >> (2 * k)) & 3;
-  int cls = in_reg_i(W.cls[0]);
+  int c = in_reg_i(W.cls[0]);
 #pragma unroll
-  for (int j = 1; j < NP; j++) cls = k == j + 1 ? in_reg_i(W.cls[j]) : cls;
-  if (ci >= class_nchild(cls)) {
-    // all children of this parent done; its own index was advanced on descent
-    if (k == 1) return false;
-    W.level = k - 1;
-    return true;
-  }
-  double parent[12], pd[4];
+  for (int j = 1; j < NP; j++) c = k == j + 1 ? in_reg_i(W.cls[j]) : c;
+  // the parent: A at depth 0, else the stack's depth k-1, read in any case
+  // so that the loads are issued together
+  const double* src = stk + (k >= 2 ? k - 2 : 0) * 12 * STRIDE;
+  double parent[12];
 #pragma unroll
   for (int i = 0; i < 12; i++) {
-    double v = in_reg(A[i]);
-#pragma unroll
-    for (int j = 0; j < NP - 1; j++) v = k == j + 2 ? in_reg(W.T[j][i]) : v;
-    parent[i] = v;
+    const double v = in_reg(src[i * STRIDE]);
+    parent[i] = k == 1 ? in_reg(A[i]) : v;
   }
-  walk_distances(parent, planes + 4 * (k - 1), pd);
+  double pl[4], pd[4];
+  plane(k - 1, pl);
+  walk_distances(parent, pl, pd);
   double child[12];
-  build_child(parent, pd, make_recipe(cls, ci), child);
-  if (k == NP) {
-    W.digits += 1 << (2 * NP);
-    double vol = tet_volume_tet(child);
-    if (vol >= tol) sink(child, vol);
-    return true;
-  }
-  // descend: store the child at depth k, classify it against plane k+1
-  double dc[4];
-  walk_distances(child, planes + 4 * k, dc);
+  build_child(parent, pd, make_recipe(c, ci), child);
+  W.digits += 1 << (2 * k);
+  const bool leaf = k == NP;
+  const double vol = tet_volume_tet(child);
+  sink(child, vol, leaf && vol >= tol);
+  // the child's class against the next plane (computed at depth NP too, and
+  // unused there)
+  double cp[4], dc[4];
+  plane(k < NP ? k : NP - 1, cp);
+  walk_distances(child, cp, dc);
   const int cc = pack_class(dc);
+  if (!leaf) {
+    double* dst = stk + (k - 1) * 12 * STRIDE;
 #pragma unroll
-  for (int j = 0; j < NP - 1; j++) {
-#pragma unroll
-    for (int i = 0; i < 12; i++) W.T[j][i] = (k == j + 1) ? child[i] : W.T[j][i];
-    W.cls[j + 1] = (k == j + 1) ? cc : W.cls[j + 1];
+    for (int i = 0; i < 12; i++) dst[i * STRIDE] = child[i];
   }
-  W.digits = (W.digits + (1 << (2 * k))) & ~(3 << (2 * (k + 1)));
-  W.level = k + 1;
-  return true;
+#pragma unroll
+  for (int j = 1; j < NP; j++) W.cls[j] = !leaf && k == j ? cc : W.cls[j];
+  W.digits = leaf ? W.digits : (W.digits & ~(3 << (2 * (k + 1))));
+  W.level = leaf ? k : k + 1;
+  walk_climb(W);
 }
 
-// Whole clip of one pair with the flat walk (host checks and one-pair calls).
+// Whole walk of one pair on the host or in a single lane: sink(tet, vol_tet)
+// for each kept output tet; returns how many.  planes: NP x [n0 n1 n2 c].
+template <int NP, class Sink>
+LSM_HD int walk_whole(const double* A, const double* planes, double tol, Sink&& sink) {
+  double stk[12 * (NP - 1)];
+  double d0[4];
+  walk_distances(A, planes, d0);
+  WalkState<NP> W;
+  walk_begin(W, pack_class(d0));
+  int kept = 0;
+  auto plane = [&](int j, double* o) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) o[i] = planes[4 * j + i];
+  };
+  while (W.level > 0)
+    walk_next<NP, 1>(W, A, stk, plane, tol, [&](const double* t, double v, bool keep) {
+      if (keep) {
+        sink(t, v);
+        kept++;
+      }
+    });
+  return kept;
+}
+
+// Whole clip of one pair with the walk (host checks).
 template <class Sink>
 LSM_HD int clip_tet_walk(const double* A, const Planes& P, double tol, Sink&& sink) {
   double planes[16];
@@ -848,15 +855,7 @@ LSM_HD int clip_tet_walk(const double* A, const Planes& P, double tol, Sink&& si
     for (int k = 0; k < 3; k++) planes[4 * i + k] = P.n[i][k];
     planes[4 * i + 3] = P.c[i];
   }
-  TetWalk W;
-  walk_start<4>(W, A, planes);
-  int kept = 0;
-  while (walk_step<4>(W, A, planes, tol, [&](const double* t, double v) {
-    sink(t, v);
-    kept++;
-  })) {
-  }
-  return kept;
+  return walk_whole<4>(A, planes, tol, sink);
 }
 
 
@@ -900,15 +899,7 @@ LSM_HD int clip_tet_hex(const double* A, const double* hex, Sink&& sink) {
   double planes[24];
   hex_planes(hex, planes);
   const double tol = 10.0 * fspacing(tet_volume_tet(A));
-  TetWalkN<6> W;
-  walk_start<6>(W, A, planes);
-  int kept = 0;
-  while (walk_step<6>(W, A, planes, tol, [&](const double* t, double v) {
-    sink(t, v);
-    kept++;
-  })) {
-  }
-  return kept;
+  return walk_whole<6>(A, planes, tol, sink);
 }
 
 // True when tet A lies beyond one of B's face planes by a margin that no

@@ -596,60 +596,6 @@ __device__ __forceinline__ void xcd_share(long n, long& w, long& end, long& stri
   w = begin + lb * blockDim.x + threadIdx.x;
 }
 
-// NP planes: class of the tet at each depth 0..NP-1, current depth, and the
-// child index reached at each depth 1..NP (2 bits each).
-template <int NP>
-struct TetWalkLdsN {
-  int cls[NP];
-  int level;
-  int digits;
-};
-using TetWalkLds = TetWalkLdsN<4>;
-
-// One step of the walk over NP planes with the depth 1..NP-1 tets in LDS;
-// lane stride LANES between consecutive words of one lane.
-template <int NP, int LANES, class Sink>
-__device__ __forceinline__ bool walk_step_lds(TetWalkLdsN<NP>& W, const double* A, double* stk,
-                                              const double* planes, double tol, Sink&& sink) {
-  const int k = W.level;
-  const int ci = (W.digits >> (2 * k)) & 3;
-  int cls = in_reg_i(W.cls[0]);
-#pragma unroll
-  for (int j = 1; j < NP; j++) cls = k == j + 1 ? in_reg_i(W.cls[j]) : cls;
-  if (ci >= class_nchild(cls)) {
-    if (k == 1) return false;
-    W.level = k - 1;
-    return true;
-  }
-  double parent[12], pd[4];
-  const double* src = stk + (k >= 2 ? k - 2 : 0) * 12 * LANES;
-#pragma unroll
-  for (int i = 0; i < 12; i++) {
-    const double v = src[i * LANES];
-    parent[i] = k == 1 ? in_reg(A[i]) : v;
-  }
-  walk_distances(parent, planes + 4 * (k - 1), pd);
-  double child[12];
-  build_child(parent, pd, make_recipe(cls, ci), child);
-  if (k == NP) {
-    W.digits += 1 << (2 * NP);
-    const double vol = tet_volume_tet(child);
-    if (vol >= tol) sink(child, vol);
-    return true;
-  }
-  double dc[4];
-  walk_distances(child, planes + 4 * k, dc);
-  const int cc = pack_class(dc);
-  double* dst = stk + (k - 1) * 12 * LANES;
-#pragma unroll
-  for (int i = 0; i < 12; i++) dst[i * LANES] = child[i];
-#pragma unroll
-  for (int j = 1; j < NP; j++) W.cls[j] = k == j ? cc : W.cls[j];
-  W.digits = (W.digits + (1 << (2 * k))) & ~(3 << (2 * (k + 1)));
-  W.level = k + 1;
-  return true;
-}
-
 // MODES: count (FILL and LEARNED false): counts, measures (and mass blocks);
 // fill (FILL): writes each pair's counts[p] outputs at offsets[p];
 // single pass (LEARNED): the count pass that also writes up to budget[w]
@@ -670,8 +616,27 @@ __global__ void __launch_bounds__(kWalkBlock) k_clip_tet_walk(ClipArgs g) {
   bool lost = false;
   double A[12], tol = 0.0, m = 0.0;
   double blk[16], affa[12], affb[12];
-  const double* planes = nullptr;
-  TetWalkLds W;
+  // B's four planes: in registers, loaded once per pair (with the mass
+  // blocks' registers also live, read from memory at each step instead)
+  double P[16];
+  const double* bpl = nullptr;
+  auto plane = [&](int j, double* o) {
+    if constexpr (MASS) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) o[i] = bpl[4 * j + i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        double v = in_reg(P[12 + i]);
+        v = j == 2 ? in_reg(P[8 + i]) : v;
+        v = j == 1 ? in_reg(P[4 + i]) : v;
+        v = j == 0 ? in_reg(P[i]) : v;
+        o[i] = v;
+      }
+    }
+  };
+  WalkState<4> W;
+  W.level = 0;
   bool active = false;
   while (true) {
     if (!active) {
@@ -693,7 +658,11 @@ __global__ void __launch_bounds__(kWalkBlock) k_clip_tet_walk(ClipArgs g) {
       const long a = g.rows[p], b = g.cols[p];
 #pragma unroll
       for (int i = 0; i < 12; i++) A[i] = g.acoords[a * 12 + i];
-      planes = g.bplanes + b * 16;
+      bpl = g.bplanes + b * 16;
+      if constexpr (!MASS) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) P[i] = bpl[i];
+      }
       tol = 10.0 * fmin_f(g.asp[a], g.bsp[b]);
       m = 0.0;
       k = 0;
@@ -706,36 +675,39 @@ __global__ void __launch_bounds__(kWalkBlock) k_clip_tet_walk(ClipArgs g) {
           affb[i] = g.baff[b * 12 + i];
         }
       }
-      {
-        double d0[4];
-        walk_distances(A, planes, d0);
-        W.cls[0] = pack_class(d0);
-        W.level = 1;
-        W.digits = 0;
-      }
+      double p0[4], d0[4];
+      plane(0, p0);
+      walk_distances(A, p0, d0);
+      walk_begin(W, pack_class(d0));
       active = true;
     }
-    const bool more = walk_step_lds<4, kWalkBlock>(W, A, stk, planes, tol, [&](const double* t, double) {
-      const double vr = tet_volume_real(t);
-      m = m + vr;
-      if (MASS) mass_element<3>(t, vr, affa, affb, blk);
-      if ((FILL || LEARNED) && k < cap) {
-        double* o = (LEARNED ? g.slot_elems : g.elements) + (off + k) * 12;
+    if (W.level > 0) {
+      walk_next<4, kWalkBlock>(W, A, stk, plane, tol, [&](const double* t, double, bool keep) {
+        // every step hands over its child; only kept leaves count, and the
+        // measure adds 0 for the others (exactly the sum of the kept ones)
+        const double vr = tet_volume_real(t);
+        m = m + (keep ? vr : 0.0);
+        if (keep) {
+          if (MASS) mass_element<3>(t, vr, affa, affb, blk);
+          if ((FILL || LEARNED) && k < cap) {
+            double* o = (LEARNED ? g.slot_elems : g.elements) + (off + k) * 12;
 #pragma unroll
-        for (int i = 0; i < 12; i++) o[i] = t[i];
-      } else if (LEARNED) {
-        const long e = atop + (k - cap);
-        if (e < g.arena_per_lane) {
-          double* o = g.arena + (abase + e) * 12;
+            for (int i = 0; i < 12; i++) o[i] = t[i];
+          } else if (LEARNED) {
+            const long e = atop + (k - cap);
+            if (e < g.arena_per_lane) {
+              double* o = g.arena + (abase + e) * 12;
 #pragma unroll
-          for (int i = 0; i < 12; i++) o[i] = t[i];
-        } else {
-          lost = true;
+              for (int i = 0; i < 12; i++) o[i] = t[i];
+            } else {
+              lost = true;
+            }
+          }
         }
-      }
-      k++;
-    });
-    if (more) continue;
+        k += keep ? 1 : 0;
+      });
+      if (W.level > 0) continue;
+    }
     if (LEARNED) {
       const bool extra = k > cap && !lost;
       g.aoff[w] = extra ? abase + atop : -1;
@@ -916,7 +888,7 @@ __global__ void __launch_bounds__(kTriBlock) k_clip_mixed(MixedArgs g) {
 }
 
 // 3D: one pair per lane at a time over the six planes of the hexahedron
-// (walk_step_lds<6>, depths 1..5 in LDS); a lane moves on to its next pair
+// (walk_next<6>, depths 1..5 in LDS); a lane moves on to its next pair
 // as soon as its walk ends.  MODE 0: tet A x hex B; 1: hex A x tet B (the
 // tet is B's); 2: hex A x hex B, A's five tets in turn, outputs and measure
 // accumulated over them in order.  hplanes: planes of the hexahedra,
@@ -947,15 +919,18 @@ __global__ void __launch_bounds__(kHexBlock) k_clip_hex_walk(HexWalkArgs g) {
   int cap = 0, k = 0, sub = 0;
   double T[12], tol = 0.0, m = 0.0;
   const double* planes = nullptr;
-  TetWalkLdsN<6> W;
+  auto plane = [&](int j, double* o) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) o[i] = planes[4 * j + i];
+  };
+  WalkState<6> W;
+  W.level = 0;
   bool active = false;
   auto start = [&]() {
     tol = 10.0 * fspacing(tet_volume_tet(T));
     double d0[4];
     walk_distances(T, planes, d0);
-    W.cls[0] = pack_class(d0);
-    W.level = 1;
-    W.digits = 0;
+    walk_begin(W, pack_class(d0));
   };
   while (true) {
     if (!active) {
@@ -987,16 +962,19 @@ __global__ void __launch_bounds__(kHexBlock) k_clip_hex_walk(HexWalkArgs g) {
       start();
       active = true;
     }
-    const bool more = walk_step_lds<6, kHexBlock>(W, T, stk, planes, tol, [&](const double* t, double) {
-      m = m + tet_volume_real(t);
-      if (FILL && k < cap) {
-        double* o = g.elements + (off + k) * 12;
+    if (W.level > 0) {
+      walk_next<6, kHexBlock>(W, T, stk, plane, tol, [&](const double* t, double, bool keep) {
+        const double vr = tet_volume_real(t);
+        m = m + (keep ? vr : 0.0);
+        if (FILL && keep && k < cap) {
+          double* o = g.elements + (off + k) * 12;
 #pragma unroll
-        for (int i = 0; i < 12; i++) o[i] = t[i];
-      }
-      k++;
-    });
-    if (more) continue;
+          for (int i = 0; i < 12; i++) o[i] = t[i];
+        }
+        k += keep ? 1 : 0;
+      });
+      if (W.level > 0) continue;
+    }
     if (MODE == 2 && sub < 4) {
       sub++;
       hex_split_tet(g.acoords + a * 24, sub, T);
@@ -1685,13 +1663,10 @@ __global__ void __launch_bounds__(kClipBlock) k_clip_direct_tri_stream(DirectArg
 }
 
 // 3D: B's planes per pair (get_planes_tet, as staged for the bulk path) and
-// the depth-first walk of k_clip_tet_walk, with the planes beside the tet
-// stack in LDS (17 words per lane: consecutive lanes fall in different banks).
+// the depth-first walk of k_clip_tet_walk.
 __global__ void __launch_bounds__(kWalkBlock) k_clip_direct_tet(DirectArgs g) {
   __shared__ double stack[3 * 12 * kWalkBlock];
-  __shared__ double plane_lds[17 * kWalkBlock];
   double* stk = stack + threadIdx.x;
-  double* planes = plane_lds + 17 * threadIdx.x;
   const long stride = (long)gridDim.x * blockDim.x;
   for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < g.npairs; p += stride) {
     double A[12], Bv[12];
@@ -1702,26 +1677,35 @@ __global__ void __launch_bounds__(kWalkBlock) k_clip_direct_tet(DirectArgs g) {
     }
     Planes P;
     tet_planes(Bv, P);
+    auto plane = [&](int j, double* o) {
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-#pragma unroll
-      for (int k = 0; k < 3; k++) planes[4 * i + k] = P.n[i][k];
-      planes[4 * i + 3] = P.c[i];
-    }
+      for (int i = 0; i < 3; i++) {
+        double v = in_reg(P.n[3][i]);
+        v = j == 2 ? in_reg(P.n[2][i]) : v;
+        v = j == 1 ? in_reg(P.n[1][i]) : v;
+        v = j == 0 ? in_reg(P.n[0][i]) : v;
+        o[i] = v;
+      }
+      double c = in_reg(P.c[3]);
+      c = j == 2 ? in_reg(P.c[2]) : c;
+      c = j == 1 ? in_reg(P.c[1]) : c;
+      c = j == 0 ? in_reg(P.c[0]) : c;
+      o[3] = c;
+    };
     const double tol = tet_tol(tet_volume_tet(A), tet_volume_tet(Bv));
-    TetWalkLds W;
-    double d0[4];
-    walk_distances(A, planes, d0);
-    W.cls[0] = pack_class(d0);
-    W.level = 1;
-    W.digits = 0;
+    double p0[4], d0[4];
+    plane(0, p0);
+    walk_distances(A, p0, d0);
+    WalkState<4> W;
+    walk_begin(W, pack_class(d0));
     double m = 0.0;
     int nk = 0;
-    while (walk_step_lds<4, kWalkBlock>(W, A, stk, planes, tol, [&](const double* t, double) {
-      m = m + tet_volume_real(t);
-      nk++;
-    })) {
-    }
+    while (W.level > 0)
+      walk_next<4, kWalkBlock>(W, A, stk, plane, tol, [&](const double* t, double, bool keep) {
+        const double vr = tet_volume_real(t);
+        m = m + (keep ? vr : 0.0);
+        nk += keep ? 1 : 0;
+      });
     g.counts[p] = nk;
     g.measure[p] = nk == 0 ? 0.0 : m;
   }
