@@ -46,6 +46,9 @@ extern "C" {
                                          learned predictor and clip once (pairs
                                          that outgrow their budget are clipped
                                          again); the output is identical */
+#define LIBSUPERMESH_GPU_BLOCKS 32    /* with MASS: also keep the per-pair
+                                         blocks (result.blocks); 3D, and the
+                                         explicit pair lists, always keep them */
 
 typedef struct libsupermesh_gpu_context libsupermesh_gpu_context;
 
@@ -73,7 +76,8 @@ typedef struct {
   const long* offsets;       /* [npairs + 1] first output element of each pair */
   const double* elements;    /* [nout][dim+1][dim] */
   /* P1 x P1 mixed mass (LIBSUPERMESH_GPU_MASS):
-   * blocks[p][i][j] = int over the pair's supermesh of phi^A_i phi^B_j,
+   * blocks[p][i][j] = int over the pair's supermesh of phi^A_i phi^B_j
+   * (null for a 2D supermesh unless LIBSUPERMESH_GPU_BLOCKS),
    * and the assembled matrix M[nnodes_a][nnodes_b] in CSR. */
   const double* blocks;      /* [npairs][dim+1][dim+1] */
   long mass_nnz;

@@ -74,7 +74,7 @@ static void free_context(Context& C) {
                     &C.row_count, &C.indptr, &C.indices, &C.rows, &C.counts, &C.measure,
                     &C.partials, &C.scalars, &C.offsets, &C.elements, &C.blocks, &C.mflag, &C.mslot, &C.work, &C.nwork,
                     &C.mkeys, &C.mvals, &C.mkeys_alt, &C.mvals_alt, &C.ukeys, &C.uvals, &C.nuniq,
-                    &C.mindptr, &C.mindices, &C.mesh_a.scale,
+                    &C.mindptr, &C.mindices, &C.mrbase, &C.mesh_a.scale,
                     &C.mesh_b.scale, &C.mesh_a.group, &C.mesh_b.group, &C.bvh.granges, &C.gpairs,
                     &C.gsum, &C.predictor.weights, &C.predictor.bias, &C.predictor.master,
                     &C.predictor.adam_m, &C.predictor.adam_v, &C.predictor.grad,

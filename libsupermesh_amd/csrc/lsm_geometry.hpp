@@ -688,9 +688,23 @@ LSM_HD Recipe make_recipe(int c, int ci) {
 }
 
 // Child of parent T (distances d to the plane) from recipe r.  Unused split
-// points are computed from vertex 0 twice and never selected.
+// points are computed from vertex 0 twice and never selected.  The slots
+// taken from the parent are filled first; each split point then goes into
+// the slots that name it as soon as it is computed, so only one split point
+// is live at a time.
 LSM_HD void build_child(const double* T, const double* d, const Recipe& r, double* out) {
-  double S[3][3];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int src = r.src[j];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      double v = in_reg(T[9 + k]);
+      v = src == 2 ? in_reg(T[6 + k]) : v;
+      v = src == 1 ? in_reg(T[3 + k]) : v;
+      v = src == 0 ? in_reg(T[k]) : v;
+      out[3 * j + k] = v;
+    }
+  }
 #pragma unroll
   for (int s = 0; s < 3; s++) {
     const int ip = r.sp[s], in = r.sn[s];
@@ -710,22 +724,9 @@ LSM_HD void build_child(const double* T, const double* d, const Recipe& r, doubl
       vn = in == 2 ? t2 : vn;
       vn = in == 1 ? t1 : vn;
       vn = in == 0 ? t0 : vn;
-      S[s][k] = w0 * vp + w1 * vn;
-    }
-  }
+      const double x = w0 * vp + w1 * vn;
 #pragma unroll
-  for (int j = 0; j < 4; j++) {
-    const int src = r.src[j];
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      double v = in_reg(S[2][k]);
-      v = src == 5 ? in_reg(S[1][k]) : v;
-      v = src == 4 ? in_reg(S[0][k]) : v;
-      v = src == 3 ? in_reg(T[9 + k]) : v;
-      v = src == 2 ? in_reg(T[6 + k]) : v;
-      v = src == 1 ? in_reg(T[3 + k]) : v;
-      v = src == 0 ? in_reg(T[k]) : v;
-      out[3 * j + k] = v;
+      for (int j = 0; j < 4; j++) out[3 * j + k] = r.src[j] == 4 + s ? x : in_reg(out[3 * j + k]);
     }
   }
 }
@@ -739,9 +740,12 @@ LSM_HD void build_child(const double* T, const double* d, const Recipe& r, doubl
 // of the current tet at each depth 0..NP-1 against the plane below it, the
 // depth of the next child to build (0 once the walk is over) and the index
 // of the next child at each depth 1..NP (2 bits each).  Tets at depths
-// 1..NP-1 live in the caller's stack: depth d at stk[((d - 1) * 12 + i) *
-// STRIDE], so a wave's lanes can interleave their stacks in LDS (STRIDE =
-// lanes) or a host caller use a plain array (STRIDE = 1).
+// 1..NREG live in the caller's registers R (depth d at R[(d - 1) * 12 + i]),
+// and those at depths NREG+1..NP-1 in the caller's stack: depth d at
+// stk[((d - NREG - 1) * 12 + i) * STRIDE], so a wave's lanes can interleave
+// their stacks in LDS (STRIDE = lanes) or a host caller use a plain array
+// (STRIDE = 1).  Register depths cost a select per coordinate and step; each
+// takes 96 bytes per lane off the stack.
 template <int NP>
 struct WalkState {
   int cls[NP];
@@ -779,21 +783,26 @@ LSM_HD void walk_begin(WalkState<NP>& W, int cls0) {
 // [n0 n1 n2 c].  Calls sink(tet, vol_tet, keep) once per step: keep is true
 // for the kept output tets, in the reference's order, and false otherwise
 // (the tet is then not an output).
-template <int NP, int STRIDE, class PlaneFn, class Sink>
-LSM_HD void walk_next(WalkState<NP>& W, const double* A, double* stk, PlaneFn&& plane, double tol,
-                      Sink&& sink) {
+template <int NP, int STRIDE, int NREG = 0, class PlaneFn, class Sink>
+LSM_HD void walk_next(WalkState<NP>& W, const double* A, double* R, double* stk, PlaneFn&& plane,
+                      double tol, Sink&& sink) {
+  static_assert(NREG >= 0 && NREG < NP, "register depths 1..NREG, NREG < NP");
   const int k = W.level;
   const int ci = (W.digits >> (2 * k)) & 3;
   int c = in_reg_i(W.cls[0]);
 #pragma unroll
   for (int j = 1; j < NP; j++) c = k == j + 1 ? in_reg_i(W.cls[j]) : c;
-  // the parent: A at depth 0, else the stack's depth k-1, read in any case
-  // so that the loads are issued together
-  const double* src = stk + (k >= 2 ? k - 2 : 0) * 12 * STRIDE;
+  // the parent (depth k-1): A at depth 0, a register tet at depths 1..NREG,
+  // else the stack's; the stack word is read in any case so that the loads
+  // are issued together
+  const int sd = k - 2 - NREG;
+  const double* src = stk + (sd > 0 ? sd : 0) * 12 * STRIDE;
   double parent[12];
 #pragma unroll
   for (int i = 0; i < 12; i++) {
-    const double v = in_reg(src[i * STRIDE]);
+    double v = in_reg(src[i * STRIDE]);
+#pragma unroll
+    for (int r = NREG; r >= 1; r--) v = k == r + 1 ? in_reg(R[(r - 1) * 12 + i]) : v;
     parent[i] = k == 1 ? in_reg(A[i]) : v;
   }
   double pl[4], pd[4];
@@ -811,8 +820,13 @@ LSM_HD void walk_next(WalkState<NP>& W, const double* A, double* stk, PlaneFn&& 
   plane(k < NP ? k : NP - 1, cp);
   walk_distances(child, cp, dc);
   const int cc = pack_class(dc);
-  if (!leaf) {
-    double* dst = stk + (k - 1) * 12 * STRIDE;
+#pragma unroll
+  for (int r = 1; r <= NREG; r++) {
+#pragma unroll
+    for (int i = 0; i < 12; i++) R[(r - 1) * 12 + i] = k == r ? child[i] : R[(r - 1) * 12 + i];
+  }
+  if (!leaf && k > NREG) {
+    double* dst = stk + (k - 1 - NREG) * 12 * STRIDE;
 #pragma unroll
     for (int i = 0; i < 12; i++) dst[i * STRIDE] = child[i];
   }
@@ -838,7 +852,7 @@ LSM_HD int walk_whole(const double* A, const double* planes, double tol, Sink&& 
     for (int i = 0; i < 4; i++) o[i] = planes[4 * j + i];
   };
   while (W.level > 0)
-    walk_next<NP, 1>(W, A, stk, plane, tol, [&](const double* t, double v, bool keep) {
+    walk_next<NP, 1>(W, A, nullptr, stk, plane, tol, [&](const double* t, double v, bool keep) {
       if (keep) {
         sink(t, v);
         kept++;

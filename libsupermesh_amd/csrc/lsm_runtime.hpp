@@ -133,6 +133,8 @@ struct Context {
   DevBuf work, nwork;            // compacted list of pairs that need the full clip
   // mass assembly
   DevBuf mflag, mslot, mkeys, mvals, mkeys_alt, mvals_alt, ukeys, uvals, nuniq, mindptr, mindices;
+  // 2D row slots: each (element, vertex) row's first slot
+  DevBuf mrbase;
   // batches: per-mesh pair offsets and measure sums
   DevBuf gpairs, gsum;
   // pinned status words the pipeline copies back asynchronously and reads

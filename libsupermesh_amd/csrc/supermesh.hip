@@ -186,7 +186,9 @@ struct ClipArgs {
   unsigned long long* nolist;
   // mass row slots (2D): the count pass also writes each pair's contributions
   // (B node, value) to the rows of A's nodes, at
-  // rbase[a * LOC + li] + (p - indptr[a]) * LOC + j; empty pairs write node -1
+  // rbase[a * LOC + li] + (p - indptr[a]) * LOC + j, j = 0..LOC-1 (B's local
+  // nodes); an empty pair writes node -1 and no values.  Per-pair blocks
+  // only when `blocks` is set.
   const long* rbase;
   const long* indptr;
   const int* enlb;
@@ -343,20 +345,20 @@ __global__ void __launch_bounds__(kClipBlock) k_clip(ClipArgs g) {
     g.counts[p] = nk;
     g.measure[p] = m;
     if (MASS) {
+      if (g.blocks) {
 #pragma unroll
-      for (int k = 0; k < LOC * LOC; k++) g.blocks[p * LOC * LOC + k] = nk == 0 ? 0.0 : blk[k];
+        for (int k = 0; k < LOC * LOC; k++) g.blocks[p * LOC * LOC + k] = nk == 0 ? 0.0 : blk[k];
+      }
       if (g.rkey) {
         const long r = (p - g.indptr[a]) * LOC;
-        int nodes[LOC];
-#pragma unroll
-        for (int j = 0; j < LOC; j++) nodes[j] = nk == 0 ? -1 : g.enlb[b * LOC + j];
 #pragma unroll
         for (int li = 0; li < LOC; li++) {
           const long o = g.rbase[a * LOC + li] + r;
 #pragma unroll
-          for (int j = 0; j < LOC; j++) {
-            g.rkey[o + j] = nodes[j];
-            g.rval[o + j] = nk == 0 ? 0.0 : blk[li * LOC + j];
+          for (int j = 0; j < LOC; j++) g.rkey[o + j] = nk == 0 ? -1 : g.enlb[b * LOC + j];
+          if (nk > 0) {
+#pragma unroll
+            for (int j = 0; j < LOC; j++) g.rval[o + j] = blk[li * LOC + j];
           }
         }
       }
@@ -542,22 +544,20 @@ __global__ void __launch_bounds__(kTriBlock) k_clip_tri(ClipArgs g) {
         keep = nk > 0;
         if (g.rkey) {
           const long r = (p - g.indptr[a]) * 3;
-          int nodes[3];
-#pragma unroll
-          for (int j = 0; j < 3; j++) nodes[j] = nk == 0 ? -1 : g.enlb[b * 3 + j];
 #pragma unroll
           for (int li = 0; li < 3; li++) {
             const long o = g.rbase[a * 3 + li] + r;
 #pragma unroll
-            for (int j = 0; j < 3; j++) {
-              g.rkey[o + j] = nodes[j];
-              g.rval[o + j] = nk == 0 ? 0.0 : blk[li * 3 + j];
+            for (int j = 0; j < 3; j++) g.rkey[o + j] = nk == 0 ? -1 : g.enlb[b * 3 + j];
+            if (nk > 0) {
+#pragma unroll
+              for (int j = 0; j < 3; j++) g.rval[o + j] = blk[li * 3 + j];
             }
           }
         }
       }
     } while (false);
-    if constexpr (STAGE) {
+    if (STAGE && g.blocks) {  // the same for the whole grid: the barriers match
       __syncthreads();  // every lane is done with its polygon slots
 #pragma unroll
       for (int i = 0; i < 9; i++) poly[threadIdx.x * 9 + i] = keep ? blk[i] : 0.0;
@@ -574,12 +574,16 @@ __global__ void __launch_bounds__(kTriBlock) k_clip_tri(ClipArgs g) {
 // and classifies or measures it) and, when its pair is done, moves on to its
 // next work item by itself, so lanes never wait for the slowest pair of the
 // wave -- only for the slowest chain of work items, which evens out.
-// The walk of k_clip_tet_walk with its stack of tets (depths 1..3) in LDS
-// instead of registers: a parent is one LDS read per coordinate and a child
-// one write, where the register stack needed a select over every depth.
-// Lane t's depth-j tet, coordinate i, is stk[((j - 1) * 12 + i) * kWalkBlock]
-// with stk = stack + t (consecutive lanes, consecutive words).
+// The walk of k_clip_tet_walk keeps its depth-1 tet in registers and its
+// depth-2 and depth-3 tets in LDS: a parent there is one LDS read per
+// coordinate and a child one write.  The stack is what limits how many waves
+// share a CU: 192 bytes per lane let three waves per SIMD run (two with all
+// three depths in LDS), at the cost of a register select per coordinate and
+// step.  Lane t's depth-j tet (j = 2, 3), coordinate i, is
+// stk[((j - 2) * 12 + i) * kWalkBlock] with stk = stack + t (consecutive
+// lanes, consecutive words).
 constexpr int kWalkBlock = 128;
+constexpr int kWalkReg = 1;
 
 // The CDNA4 dispatcher hands consecutive workgroups to its 8 XCDs (each with
 // its own L2) in turn.  A grid whose size is a multiple of 8 can give XCD x
@@ -601,9 +605,11 @@ __device__ __forceinline__ void xcd_share(long n, long& w, long& end, long& stri
 // single pass (LEARNED): the count pass that also writes up to budget[w]
 // outputs of work item w at slots[w] of the slot buffer.
 template <bool MASS, bool FILL, bool LEARNED = false>
-__global__ void __launch_bounds__(kWalkBlock) k_clip_tet_walk(ClipArgs g) {
-  __shared__ double stack[3 * 12 * kWalkBlock];
+__global__ void __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(MASS ? 2 : 3)))
+k_clip_tet_walk(ClipArgs g) {
+  __shared__ double stack[(3 - kWalkReg) * 12 * kWalkBlock];
   double* stk = stack + threadIdx.x;
+  double R[12 * kWalkReg];
   // blocks go to the XCDs round robin: each XCD takes one contiguous share
   // of the work list, so that its L2 holds one region of A and B
   long w, n, stride;
@@ -616,24 +622,12 @@ __global__ void __launch_bounds__(kWalkBlock) k_clip_tet_walk(ClipArgs g) {
   bool lost = false;
   double A[12], tol = 0.0, m = 0.0;
   double blk[16], affa[12], affb[12];
-  // B's four planes: in registers, loaded once per pair (with the mass
-  // blocks' registers also live, read from memory at each step instead)
-  double P[16];
+  // B's four planes are read from memory at each step (the pair's 128 bytes
+  // stay in cache): held in registers they would cost the third wave per SIMD
   const double* bpl = nullptr;
   auto plane = [&](int j, double* o) {
-    if constexpr (MASS) {
 #pragma unroll
-      for (int i = 0; i < 4; i++) o[i] = bpl[4 * j + i];
-    } else {
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        double v = in_reg(P[12 + i]);
-        v = j == 2 ? in_reg(P[8 + i]) : v;
-        v = j == 1 ? in_reg(P[4 + i]) : v;
-        v = j == 0 ? in_reg(P[i]) : v;
-        o[i] = v;
-      }
-    }
+    for (int i = 0; i < 4; i++) o[i] = bpl[4 * j + i];
   };
   WalkState<4> W;
   W.level = 0;
@@ -659,10 +653,6 @@ __global__ void __launch_bounds__(kWalkBlock) k_clip_tet_walk(ClipArgs g) {
 #pragma unroll
       for (int i = 0; i < 12; i++) A[i] = g.acoords[a * 12 + i];
       bpl = g.bplanes + b * 16;
-      if constexpr (!MASS) {
-#pragma unroll
-        for (int i = 0; i < 16; i++) P[i] = bpl[i];
-      }
       tol = 10.0 * fmin_f(g.asp[a], g.bsp[b]);
       m = 0.0;
       k = 0;
@@ -682,7 +672,7 @@ __global__ void __launch_bounds__(kWalkBlock) k_clip_tet_walk(ClipArgs g) {
       active = true;
     }
     if (W.level > 0) {
-      walk_next<4, kWalkBlock>(W, A, stk, plane, tol, [&](const double* t, double, bool keep) {
+      walk_next<4, kWalkBlock, kWalkReg>(W, A, R, stk, plane, tol, [&](const double* t, double, bool keep) {
         // every step hands over its child; only kept leaves count, and the
         // measure adds 0 for the others (exactly the sum of the kept ones)
         const double vr = tet_volume_real(t);
@@ -888,7 +878,7 @@ __global__ void __launch_bounds__(kTriBlock) k_clip_mixed(MixedArgs g) {
 }
 
 // 3D: one pair per lane at a time over the six planes of the hexahedron
-// (walk_next<6>, depths 1..5 in LDS); a lane moves on to its next pair
+// (walk_next<6>, depths 1..2 in registers, 3..5 in LDS); a lane moves on to its next pair
 // as soon as its walk ends.  MODE 0: tet A x hex B; 1: hex A x tet B (the
 // tet is B's); 2: hex A x hex B, A's five tets in turn, outputs and measure
 // accumulated over them in order.  hplanes: planes of the hexahedra,
@@ -906,13 +896,17 @@ struct HexWalkArgs {
   double* elements;
 };
 
-// One wave per block: 30 KB of LDS stack per block, five blocks per CU.
+// One wave per block; the walk keeps its depth-1 and depth-2 tets in
+// registers and depths 3..5 in LDS (18 KB per block, eight blocks per CU).
 constexpr int kHexBlock = 64;
+constexpr int kHexReg = 2;
 
 template <int MODE, bool FILL>
-__global__ void __launch_bounds__(kHexBlock) k_clip_hex_walk(HexWalkArgs g) {
-  __shared__ double stack[5 * 12 * kHexBlock];
+__global__ void __launch_bounds__(kHexBlock) __attribute__((amdgpu_waves_per_eu(2)))
+k_clip_hex_walk(HexWalkArgs g) {
+  __shared__ double stack[(5 - kHexReg) * 12 * kHexBlock];
   double* stk = stack + threadIdx.x;
+  double R[12 * kHexReg];
   long p, n, stride;
   xcd_share(g.npairs, p, n, stride);
   long off = 0, a = 0;
@@ -963,7 +957,7 @@ __global__ void __launch_bounds__(kHexBlock) k_clip_hex_walk(HexWalkArgs g) {
       active = true;
     }
     if (W.level > 0) {
-      walk_next<6, kHexBlock>(W, T, stk, plane, tol, [&](const double* t, double, bool keep) {
+      walk_next<6, kHexBlock, kHexReg>(W, T, R, stk, plane, tol, [&](const double* t, double, bool keep) {
         const double vr = tet_volume_real(t);
         m = m + (keep ? vr : 0.0);
         if (FILL && keep && k < cap) {
@@ -1367,7 +1361,12 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
   g.baff = static_cast<const double*>(B.affine.ptr);
   g.counts = C.counts.as<int>(npairs);
   g.measure = C.measure.as<double>(npairs);
-  g.blocks = mass ? C.blocks.as<double>((size_t)npairs * loc * loc) : nullptr;
+  // 2D: the mass goes to the rows of A's nodes laid out before the clip;
+  // per-pair blocks only on request (or without rows, as the only output)
+  C.mass_rows_ready = false;
+  if (mass && dim == 2 && out->indptr) mass_prepare_rows(C, A, B, out->indptr, g);
+  const bool blocks = mass && (dim == 3 || !C.mass_rows_ready || (flags & LIBSUPERMESH_GPU_BLOCKS));
+  g.blocks = blocks ? C.blocks.as<double>((size_t)npairs * loc * loc) : nullptr;
   Partial* parts = C.partials.as<Partial>(kClipGrid);
   if (dim == 3 && npairs > 0) {
     const long chunk = (npairs + kClipGrid - 1) / kClipGrid;
@@ -1471,8 +1470,6 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
       LSM_HIP(hipMemsetAsync(g.ndropped, 0, sizeof(unsigned long long), C.stream));
     }
   }
-  C.mass_rows_ready = false;
-  if (mass && dim == 2 && out->indptr) mass_prepare_rows(C, A, B, out->indptr, g);
   if (C.timing) LSM_HIP(hipEventRecord(C.ev[14], C.stream));
   if (dim == 2) launch_clip<2>(C, g, mass, false);
   else if (learned) launch_tet_walk(C, g, mass ? 4 : 3);
@@ -1665,8 +1662,9 @@ __global__ void __launch_bounds__(kClipBlock) k_clip_direct_tri_stream(DirectArg
 // 3D: B's planes per pair (get_planes_tet, as staged for the bulk path) and
 // the depth-first walk of k_clip_tet_walk.
 __global__ void __launch_bounds__(kWalkBlock) k_clip_direct_tet(DirectArgs g) {
-  __shared__ double stack[3 * 12 * kWalkBlock];
+  __shared__ double stack[(3 - kWalkReg) * 12 * kWalkBlock];
   double* stk = stack + threadIdx.x;
+  double R[12 * kWalkReg];
   const long stride = (long)gridDim.x * blockDim.x;
   for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < g.npairs; p += stride) {
     double A[12], Bv[12];
@@ -1701,7 +1699,7 @@ __global__ void __launch_bounds__(kWalkBlock) k_clip_direct_tet(DirectArgs g) {
     double m = 0.0;
     int nk = 0;
     while (W.level > 0)
-      walk_next<4, kWalkBlock>(W, A, stk, plane, tol, [&](const double* t, double, bool keep) {
+      walk_next<4, kWalkBlock, kWalkReg>(W, A, R, stk, plane, tol, [&](const double* t, double, bool keep) {
         const double vr = tet_volume_real(t);
         m = m + (keep ? vr : 0.0);
         nk += keep ? 1 : 0;
@@ -2174,7 +2172,7 @@ k_mass_rows_gather(long nn, const long* __restrict__ nptr, const int* __restrict
       for (long rb = r0; rb < r1; rb += 64) {
         const long r = rb + lane;
         const int key = r < r1 ? tkey[r] : -1;
-        const double v = r < r1 ? tval[r] : 0.0;
+        const double v = key >= 0 ? tval[r] : 0.0;
         const unsigned long long m = __ballot(key >= 0);
         if (key >= 0) {
           const int pos = n + __popcll(m & ((1ULL << lane) - 1));
@@ -2267,7 +2265,7 @@ k_mass_rows_slots(long nn, const long* __restrict__ raw_off, int* __restrict__ t
   for (int q = 0; q < PF; q++) {
     const long r = a0 + lane + 64 * q;
     ck[q] = r < a1 ? tkey[r] : -1;
-    cv[q] = r < a1 ? tval[r] : 0.0;
+    cv[q] = ck[q] >= 0 ? tval[r] : 0.0;
   }
   long j = i + stride;
   long b0 = 0, b1 = 0;
@@ -2291,7 +2289,7 @@ k_mass_rows_slots(long nn, const long* __restrict__ raw_off, int* __restrict__ t
     for (long rb = a0 + 64 * PF; rb < a1; rb += 64) {  // the rest of a long row
       const long r = rb + lane;
       const int key = r < a1 ? tkey[r] : -1;
-      const double v = r < a1 ? tval[r] : 0.0;
+      const double v = key >= 0 ? tval[r] : 0.0;
       const unsigned long long m = __ballot(key >= 0);
       if (key >= 0) {
         const int pos = n + __popcll(m & ((1ULL << lane) - 1));
@@ -2308,7 +2306,7 @@ k_mass_rows_slots(long nn, const long* __restrict__ raw_off, int* __restrict__ t
       for (int q = 0; q < PF; q++) {
         const long r = b0 + lane + 64 * q;
         ck[q] = r < b1 ? tkey[r] : -1;
-        cv[q] = r < b1 ? tval[r] : 0.0;
+        cv[q] = ck[q] >= 0 ? tval[r] : 0.0;
       }
       if (jn < nn) {
         c0 = raw_off[jn];
@@ -2331,6 +2329,38 @@ k_mass_rows_slots(long nn, const long* __restrict__ raw_off, int* __restrict__ t
   }
 }
 
+// Per-pair blocks from the row slots (2D), for the sorted assembly when the
+// count pass wrote rows only: block[p][li][j] is row (a, li)'s slot j of the
+// pair; empty pairs get zeros.
+__global__ void __launch_bounds__(256)
+k_blocks_from_rows(long npairs, const int* __restrict__ rows, const long* __restrict__ indptr,
+                   const int* __restrict__ counts, const long* __restrict__ rbase, const double* __restrict__ rval,
+                   double* __restrict__ blocks) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npairs) return;
+  const long a = rows[p];
+  const bool ne = counts[p] > 0;
+  const long r = (p - indptr[a]) * 3;
+#pragma unroll
+  for (int li = 0; li < 3; li++) {
+    const long o = rbase[a * 3 + li] + r;
+#pragma unroll
+    for (int j = 0; j < 3; j++) blocks[p * 9 + li * 3 + j] = ne ? rval[o + j] : 0.0;
+  }
+}
+
+static void blocks_from_rows(Context& C, const DevMesh& A, libsupermesh_gpu_result* out) {
+  if (out->blocks) return;
+  const long npairs = out->npairs;
+  double* blocks = C.blocks.as<double>((size_t)(npairs > 0 ? npairs : 1) * 9);
+  if (npairs > 0)
+    hipLaunchKernelGGL(k_blocks_from_rows, dim3(grid_for(npairs, 256)), dim3(256), 0, C.stream, npairs, out->rows,
+                       out->indptr, out->counts, static_cast<const long*>(C.mrbase.ptr),
+                       static_cast<const double*>(C.mvals.ptr), blocks);
+  LSM_HIP(hipGetLastError());
+  out->blocks = blocks;
+}
+
 // After the clip pass filled the row slots (mass_prepare_rows).
 static void assemble_mass_slots(Context& C, const DevMesh& A, const DevMesh& B, libsupermesh_gpu_result* out) {
   const long nn = A.nnodes;
@@ -2351,6 +2381,7 @@ static void assemble_mass_slots(Context& C, const DevMesh& A, const DevMesh& B, 
   C.mass_maxraw = C.hstat[1];
   LSM_REQUIRE(C.mass_maxraw >= 0, LIBSUPERMESH_GPU_ERROR_STATE, "mass row bound not read back");
   if (C.mass_maxraw > 2048) {  // rows too long for the row kernels' LDS
+    blocks_from_rows(C, A, out);
     assemble_mass_sorted(C, A, B, out->npairs, out);
     return;
   }
@@ -2381,6 +2412,7 @@ static void assemble_mass_slots(Context& C, const DevMesh& A, const DevMesh& B, 
   LSM_HIP(hipMemcpyAsync(C.hstat + 3, mindptr + nn, sizeof(long), hipMemcpyDeviceToHost, C.stream));
   LSM_HIP(hipStreamSynchronize(C.stream));
   if (*reinterpret_cast<const int*>(C.hstat + 2)) {  // a row with more distinct columns than the kernel holds
+    blocks_from_rows(C, A, out);
     assemble_mass_sorted(C, A, B, out->npairs, out);
     return;
   }
@@ -2440,7 +2472,7 @@ bool mass_prepare_rows(Context& C, const DevMesh& A, const DevMesh& B, const lon
   inclusive_scan_long(C, contrib, slot + 1, nent);
   hipLaunchKernelGGL(k_row_bounds, dim3(grid_for(nn + 1, 256)), dim3(256), 0, C.stream, nn, nptr, slot, raw_off,
                      raw);
-  long* rbase = C.row_count.as<long>(nent + 1);
+  long* rbase = C.mrbase.as<long>(nent + 1);
   hipLaunchKernelGGL(k_elem_base, dim3(grid_for(nent, 256)), dim3(256), 0, C.stream, nent, nlist, slot, rbase);
   LSM_HIP(hipGetLastError());
   // every (element, local vertex) holds LOC slots per candidate pair of the

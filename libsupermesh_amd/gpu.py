@@ -31,6 +31,7 @@ MASS = 2
 TIMING = 4
 SORTED = 8
 LEARNED = 16
+BLOCKS = 32
 PREDICTOR_PARAMS = 12065
 
 _lib = None

@@ -79,7 +79,11 @@ def test_tri_1024_vs_1000_with_mass():
         fmg = np.sum(pa[row_of, 0] * mv * pb[mj, 1])
         assert abs(fmg - 0.25) <= 1e-12
         # blocks of a deterministic sample of pairs against the oracle's
-        # 3-point quadrature
+        # 3-point quadrature (kept on request), and the same matrix
+        A, B = gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pb, eb)
+        r = ctx.supermesh(A, B, flags=gpu.SORTED | gpu.MASS | gpu.BLOCKS)
+        for x, y in zip((mi, mj, mv), gpu.fetch_mass(r, pa.shape[0])):
+            assert np.array_equal(x, y)
         blocks = gpu.fetch(r, 2, ("blocks",))["blocks"]
         sel = np.arange(0, rows.shape[0], 19)
         ref = O.mass_pairs(pa, ea, pb, eb, rows[sel], cols[sel], nthreads=_threads())

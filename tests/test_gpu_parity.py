@@ -36,6 +36,8 @@ def ctx():
 def run(ctx, pa, ea, pb, eb, finder=gpu.FINDER_TREE, flags=0):
     A = gpu.DeviceMesh(pa, ea)
     B = gpu.DeviceMesh(pb, eb)
+    if flags & gpu.MASS:
+        flags |= gpu.BLOCKS
     r = ctx.supermesh(A, B, finder=finder, flags=flags | gpu.SORTED)
     dim = pa.shape[1]
     what = ["indptr", "indices", "rows", "counts", "measure"]
@@ -196,12 +198,18 @@ def test_mass_blocks_and_csr(ctx, dim):
         pa, ea = meshgen.split_grid(3, 8, 11)
         pb, eb = meshgen.split_grid(3, 7, 12, mesh_id=1)
     A, B = gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pb, eb)
-    r = ctx.supermesh(A, B, flags=gpu.MASS)
+    # MASS alone: the assembled matrix (2D: no per-pair blocks kept)
+    r0 = ctx.supermesh(A, B, flags=gpu.MASS)
+    assert (r0.blocks is None) == (dim == 2)
+    csr0 = gpu.fetch_mass(r0, pa.shape[0])
+    r = ctx.supermesh(A, B, flags=gpu.MASS | gpu.BLOCKS)
     out = gpu.fetch(r, dim, ["rows", "indices", "counts", "blocks"])
     want = O.mass_pairs(pa, ea, pb, eb, out["rows"], out["indices"])
     scale = np.abs(want).max()
     assert np.all(np.abs(out["blocks"] - want) <= 1e-12 * scale)
     indptr, indices, values = gpu.fetch_mass(r, pa.shape[0])
+    for x, y in zip(csr0, (indptr, indices, values)):   # the same matrix either way
+        assert np.array_equal(x, y)
     M = sp.csr_matrix((values, indices, indptr), shape=(pa.shape[0], pb.shape[0]))
     loc = dim + 1
     ne = out["counts"] > 0

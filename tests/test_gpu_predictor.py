@@ -120,7 +120,7 @@ def test_learned_single_pass_with_mass_is_identical(meshes, weights):
         ctx.predictor_set(R.init(9, scale=3.0))
     ctx.predictor_arena(0.0, 1)   # force overflow pairs through the fallback too
     what = ["indptr", "indices", "counts", "measure", "offsets", "elements", "blocks"]
-    flags = gpu.ELEMENTS | gpu.SORTED | gpu.MASS
+    flags = gpu.ELEMENTS | gpu.SORTED | gpu.MASS | gpu.BLOCKS
     r0 = ctx.supermesh(A, B, flags=flags)
     plain = gpu.fetch(r0, 3, what)
     m0 = gpu.fetch_mass(r0, pa.shape[0])
