@@ -197,16 +197,23 @@ def batched_workload(ctx, steps, warmup, count=4096):
             sums[k] = r.sum
         return tot, sums
 
-    def run_batched():
+    def run_batched(flags=0):
         tot, sums = 0, np.zeros(len(pairs))
         for dim, (sel, BA, BB) in batches.items():
-            r, br = ctx.supermesh_batched(BA, BB)
+            r, br = ctx.supermesh_batched(BA, BB, flags=flags)
             tot += r.npairs
             sums[sel] = gpu.fetch_batch(br)[1]
+            if flags & gpu.GRAPH:
+                graphs[dim] = int(r.count_graph)
         return tot, sums
 
+    graphs = {}
     out = {}
-    for name, fn in (("plain", run_plain), ("batched", run_batched)):
+    # batched_graph: the count pass of each batched call as a recorded graph,
+    # in the launch shape the first call measured fastest (measured in the
+    # warm-up, outside the timed steps)
+    for name, fn in (("plain", run_plain), ("batched", run_batched),
+                     ("batched_graph", lambda: run_batched(gpu.GRAPH))):
         for _ in range(warmup):
             fn()
         gpu.synchronize()
@@ -221,15 +228,19 @@ def batched_workload(ctx, steps, warmup, count=4096):
     dev = np.abs(out["plain"]["_sums"] - out["batched"]["_sums"])
     ref = np.maximum(1.0, np.abs(out["plain"]["_sums"]))
     unit_err = float(np.abs(out["batched"]["_sums"] - 1.0).max())   # every mesh covers [0,1]^d
+    graph_same = bool(np.array_equal(out["batched"]["_sums"], out["batched_graph"]["_sums"]))
     for v in out.values():
         v.pop("_sums")
     n2 = len(batches.get(2, ([],))[0])
     return {
         "workload": f"{len(pairs)} mesh pairs ({n2} 2D, {len(pairs) - n2} 3D), "
                     "elements log-uniform in [50, 50000] per mesh, seed 3",
-        "plain": out["plain"], "batched": out["batched"],
-        "speedup": out["plain"]["ms_per_step"] / out["batched"]["ms_per_step"],
-        "same_pairs": out["plain"]["pairs"] == out["batched"]["pairs"],
+        "plain": out["plain"], "batched": out["batched"], "batched_graph": out["batched_graph"],
+        "speedup": out["plain"]["ms_per_step"] / out["batched_graph"]["ms_per_step"],
+        "speedup_without_graph": out["plain"]["ms_per_step"] / out["batched"]["ms_per_step"],
+        "graph": {"count_pass_replayed": graphs, "same_sums_as_eager": graph_same,
+                  "launch_shapes": {f"{d}d": ctx.count_shape(d) for d in batches}},
+        "same_pairs": out["plain"]["pairs"] == out["batched"]["pairs"] == out["batched_graph"]["pairs"],
         "max_rel_sum_diff": float((dev / ref).max()),
         "max_abs_sum_err": unit_err,
     }
@@ -238,8 +249,11 @@ def batched_workload(ctx, steps, warmup, count=4096):
 def learned_workload(ctx, name, steps, warmup, weights, quantiles=(0.7, 0.9, 0.99)):
     """Materialised tet supermesh (LIBSUPERMESH_GPU_ELEMENTS): plain
     count-then-fill against the single pass sized by the learned predictor
-    (LIBSUPERMESH_GPU_LEARNED).  Outputs are compared by device checksums of
-    the per-pair counts, the offsets and every output tet."""
+    (LIBSUPERMESH_GPU_LEARNED), which leaves each pair's run where its budget
+    placed it, and against the same single pass compacted into the plain
+    layout in the call (LIBSUPERMESH_GPU_DENSE).  Outputs are compared by
+    device checksums of the per-pair counts, the offsets and every output
+    tet, the single pass's runs after compaction."""
     import numpy as np
     from libsupermesh_amd import gpu, meshgen
     t0 = time.time()
@@ -250,6 +264,12 @@ def learned_workload(ctx, name, steps, warmup, weights, quantiles=(0.7, 0.9, 0.9
     out = {}
     modes = [("plain", gpu.ELEMENTS | gpu.TIMING, None)]
     modes += [(f"learned_q{q}", gpu.ELEMENTS | gpu.LEARNED | gpu.TIMING, q) for q in quantiles]
+    modes += [("learned_dense", gpu.ELEMENTS | gpu.LEARNED | gpu.DENSE | gpu.TIMING, 0.9)]
+
+    def sums(r):
+        return (ctx.checksum(r.counts, 4 * r.npairs), ctx.checksum(r.offsets, 8 * (r.npairs + 1)),
+                ctx.checksum(r.elements, 96 * r.nout))
+
     for mode, flags, q in modes:
         if q is not None:
             ctx.predictor_quantile(q)
@@ -267,27 +287,38 @@ def learned_workload(ctx, name, steps, warmup, weights, quantiles=(0.7, 0.9, 0.9
             phases.append(list(r.ms))
         gpu.synchronize()
         dt = (time.perf_counter() - t1) / steps
-        sums = (ctx.checksum(r.counts, 4 * r.npairs), ctx.checksum(r.offsets, 8 * (r.npairs + 1)),
-                ctx.checksum(r.elements, 96 * r.nout))
+        if q is not None and not flags & gpu.DENSE:
+            # the runs of this very call, compacted (untimed) for the comparison
+            level = r.learned_quantile
+            slots, ovf, size = int(r.learned_slots), int(r.learned_overflow), int(r.elements_size)
+            r = ctx.supermesh(A, B, flags=flags | gpu.DENSE)
         out[mode] = {"ms_per_step": dt * 1e3, "pairs": int(r.npairs), "supermesh_elements": int(r.nout),
                      "pairs_per_s": r.npairs / dt, "elements_per_s": r.nout / dt,
-                     "first_walk_ms": sum(walk) / steps, "_sums": sums,
+                     "first_walk_ms": sum(walk) / steps, "_sums": sums(r),
                      "phase_ms": {k: round(sum(p[i] for p in phases) / steps, 3)
                                   for k, i in (("search", 2), ("clip", 3), ("total", 6))}}
         if q is not None:
-            out[mode].update({"quantile": q, "predictor_ms": sum(pred) / steps, "slots": int(r.learned_slots),
-                              "overflow_pairs": int(r.learned_overflow)})
+            if flags & gpu.DENSE:
+                level = r.learned_quantile
+                slots, ovf, size = int(r.learned_slots), int(r.learned_overflow), int(r.elements_size)
+            out[mode].update({"quantile_requested": q, "quantile_used": round(float(level), 6),
+                              "predictor_ms": sum(pred) / steps, "slots": slots, "overflow_pairs": ovf,
+                              "elements_array": size})
         log(f"[bench] learned {name}, {mode}: {r.npairs} pairs, {r.nout} tets, {dt * 1e3:.1f} ms/step, "
             f"phases {out[mode]['phase_ms']}")
     ref = out["plain"].pop("_sums")
     same = all(v.pop("_sums") == ref for k, v in out.items() if k != "plain")
-    best = min((k for k in out if k != "plain"), key=lambda k: out[k]["ms_per_step"])
+    runs = [k for k in out if k.startswith("learned_q")]
+    best = min(runs, key=lambda k: out[k]["ms_per_step"])
     return {"workload": name, "plain": out["plain"], "learned": out[best],
             "speedup": out["plain"]["ms_per_step"] / out[best]["ms_per_step"],
+            "learned_dense": out["learned_dense"],
+            "speedup_dense": out["plain"]["ms_per_step"] / out["learned_dense"]["ms_per_step"],
             "identical_outputs": same, "weights": os.path.relpath(weights, ROOT),
-            "quantile_sweep": {k: {"ms_per_step": round(v["ms_per_step"], 3), "slots": v["slots"],
-                                   "overflow_pairs": v["overflow_pairs"]}
-                               for k, v in out.items() if k != "plain"}}
+            "quantile_sweep": {k: {"ms_per_step": round(out[k]["ms_per_step"], 3),
+                                   "quantile_used": out[k]["quantile_used"], "slots": out[k]["slots"],
+                                   "overflow_pairs": out[k]["overflow_pairs"]}
+                               for k in runs}}
 
 
 def clip_roofline(name, w):
@@ -319,8 +350,9 @@ def learned_batched_workload(ctx, steps, warmup, weights, count=1024):
     """The materialised supermesh (ELEMENTS, rows sorted) of `count` mesh pairs
     of the batched generator: plain = one count-then-fill call per mesh pair;
     learned+batched = one batched call per dimension, the 3D one sized by the
-    predictor.  Outputs are compared mesh by mesh by device checksums of each
-    mesh's per-pair counts and output elements."""
+    predictor (its runs left where the budgets placed them).  Outputs are
+    compared mesh by mesh by device checksums of each mesh's per-pair counts
+    and output elements, the single pass compacted in the call."""
     import numpy as np
     from libsupermesh_amd import gpu, meshgen
     t0 = time.time()
@@ -348,7 +380,8 @@ def learned_batched_workload(ctx, steps, warmup, weights, count=1024):
     def sums_batched():
         out = [None] * len(pairs)
         for dim, (sel, BA, BB) in batches.items():
-            r, br = ctx.supermesh_batched(BA, BB, flags=base | (gpu.LEARNED if dim == 3 else 0))
+            # the single pass's runs compacted in the call, for the comparison
+            r, br = ctx.supermesh_batched(BA, BB, flags=base | (gpu.LEARNED | gpu.DENSE if dim == 3 else 0))
             poff, _ = gpu.fetch_batch(br)
             offs = gpu.device_to_numpy(r.offsets, r.npairs + 1, np.int64)
             esz = 8 * (dim + 1) * dim

@@ -42,13 +42,33 @@ extern "C" {
 #define LIBSUPERMESH_GPU_SORTED 8     /* sort each candidate row by B id (otherwise
                                          rows keep the hierarchy's traversal
                                          order, which is deterministic) */
-#define LIBSUPERMESH_GPU_LEARNED 16   /* 3D with ELEMENTS: size the output from the
-                                         learned predictor and clip once (pairs
-                                         that outgrow their budget are clipped
-                                         again); the output is identical */
+#define LIBSUPERMESH_GPU_LEARNED 16   /* 3D with ELEMENTS: place each pair's run
+                                         of output elements from the learned
+                                         predictor's budget and clip once
+                                         (outputs beyond a budget go to an
+                                         arena and the run is then joined in a
+                                         spill reserve; pairs that outgrow the
+                                         arena are clipped again).  Every
+                                         pair's elements are those of the
+                                         plain path, in the same order, but
+                                         the runs lie apart: pair p's are
+                                         elements[offsets[p] .. offsets[p] +
+                                         counts[p]) */
 #define LIBSUPERMESH_GPU_BLOCKS 32    /* with MASS: also keep the per-pair
                                          blocks (result.blocks); 3D, and the
                                          explicit pair lists, always keep them */
+#define LIBSUPERMESH_GPU_DENSE 64     /* with LEARNED: compact the runs into the
+                                         plain path's layout (offsets = prefix
+                                         sums of the counts), byte for byte the
+                                         plain path's arrays */
+#define LIBSUPERMESH_GPU_GRAPH 128    /* the count pass (counts, measures, sums;
+                                         not with ELEMENTS, MASS or LEARNED)
+                                         of a simplex supermesh or batch in
+                                         the launch shape measured fastest on
+                                         the context's first such call, run as
+                                         a recorded hipGraph that later calls
+                                         with the same buffers replay; results
+                                         are those of the plain count pass */
 
 typedef struct libsupermesh_gpu_context libsupermesh_gpu_context;
 
@@ -72,9 +92,14 @@ typedef struct {
   /* per-pair clip results */
   const int* counts;         /* [npairs] output elements per pair */
   const double* measure;     /* [npairs] sum of output areas / volumes */
-  /* materialised supermesh (LIBSUPERMESH_GPU_ELEMENTS) */
+  /* materialised supermesh (LIBSUPERMESH_GPU_ELEMENTS): pair p's output
+   * elements are elements[offsets[p] .. offsets[p] + counts[p]).  Plain
+   * layout: offsets are the prefix sums of the counts (offsets[npairs] =
+   * nout) and elements_size = nout.  LIBSUPERMESH_GPU_LEARNED without
+   * LIBSUPERMESH_GPU_DENSE: runs lie apart (budgets, arena, spill) in an
+   * array of elements_size elements; offsets[npairs] is then unspecified. */
   const long* offsets;       /* [npairs + 1] first output element of each pair */
-  const double* elements;    /* [nout][dim+1][dim] */
+  const double* elements;    /* [elements_size][dim+1][dim] */
   /* P1 x P1 mixed mass (LIBSUPERMESH_GPU_MASS):
    * blocks[p][i][j] = int over the pair's supermesh of phi^A_i phi^B_j
    * (null for a 2D supermesh unless LIBSUPERMESH_GPU_BLOCKS),
@@ -94,14 +119,19 @@ typedef struct {
    * element fill), [4] mass assembly, [5] the predictor kernel (LEARNED),
    * [6] total, [7] the clip count kernel alone (the single pass with LEARNED) */
   double ms[8];
-  /* LIBSUPERMESH_GPU_LEARNED: output slots allocated from the predicted
-   * budgets, pairs clipped again because they outgrew their budget, and the
-   * budget rule used: the quantile whose slots fit in device memory (the
-   * configured one or a lower level), or 0 when none fit and the plain
-   * count-then-fill path ran */
+  /* LIBSUPERMESH_GPU_LEARNED: output elements placed from the predicted
+   * budgets (their sum), pairs whose output outgrew the budget (their runs
+   * joined from the arena, or clipped again when it ran out), and the budget
+   * rule used: the quantile whose runs fit in device memory (the configured
+   * one or a lower level), or 0 when none fit and the plain count-then-fill
+   * path ran */
   long learned_slots;
   long learned_overflow;
   double learned_quantile;
+  long elements_size;        /* elements in the `elements` array */
+  int count_graph;           /* LIBSUPERMESH_GPU_GRAPH: 1 when the count pass ran
+                                as the recorded graph (0: eagerly, in the same
+                                launch shape) */
 } libsupermesh_gpu_result;
 
 /* Context on HIP device `device` with its own stream (default priority). */
@@ -123,6 +153,22 @@ int libsupermesh_gpu_malloc(int device, size_t bytes, void** ptr);
 int libsupermesh_gpu_free(void* ptr);
 int libsupermesh_gpu_memcpy(void* dst, const void* src, size_t bytes, int kind);
 int libsupermesh_gpu_synchronize(int device);
+
+/* The count pass's launch shape the context measured fastest for dimension
+ * dim (LIBSUPERMESH_GPU_GRAPH; zeros before the first such call): threads per
+ * block, blocks in the grid (3D: the walk's persistent lanes / block; 2D:
+ * blocks of the grid-stride loop, i.e. the pairs each wave takes), the tet
+ * depths whose stack lives in LDS (3D: 2 or 3; the rest in registers) and the
+ * time of that shape on the sample it was measured on. */
+int libsupermesh_gpu_count_shape(libsupermesh_gpu_context* ctx, int dim, int* block, int* grid, int* lds_depths,
+                                 float* ms);
+
+/* The plain layout of a LIBSUPERMESH_GPU_LEARNED result (what
+ * LIBSUPERMESH_GPU_DENSE gives in the same call): rewrites result->offsets,
+ * ->elements and ->elements_size to arrays of the context holding the runs
+ * end to end.  The result must be the context's latest; the buffers it
+ * pointed to are left as they were. */
+int libsupermesh_gpu_compact(libsupermesh_gpu_context* ctx, libsupermesh_gpu_result* result);
 
 /* Order-sensitive checksum of `bytes` (a multiple of 4) of device memory:
  * the sum of its 32-bit words w_i times (2 i + 1), modulo 2^64.  For comparing
@@ -221,8 +267,11 @@ int libsupermesh_gpu_predictor_set(libsupermesh_gpu_context* ctx, const float* p
 int libsupermesh_gpu_predictor_get(libsupermesh_gpu_context* ctx, float* params);
 int libsupermesh_gpu_predictor_quantile(libsupermesh_gpu_context* ctx, float q);
 /* Outputs beyond a pair's budget go to an arena of fraction x the predicted
- * output plus min_tets tets (defaults 0.25 and 2^20), split among the clip's
- * lanes; pairs that find their lane's share full are clipped again. */
+ * output plus min_tets tets (defaults 0.25 and 2^20) split among the clip's
+ * lanes; such runs are then joined in a spill reserve of the same size, and
+ * pairs that find their lane's share full are clipped again into it.  When
+ * the spill reserve is too small every pair is filled again in the plain
+ * layout (elements_size = nout). */
 int libsupermesh_gpu_predictor_arena(libsupermesh_gpu_context* ctx, float fraction, long min_tets);
 int libsupermesh_gpu_predictor_collect(libsupermesh_gpu_context* ctx, const libsupermesh_gpu_mesh* a,
                                        const libsupermesh_gpu_mesh* b, long max_samples,

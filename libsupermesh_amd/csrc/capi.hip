@@ -80,8 +80,9 @@ static void free_context(Context& C) {
                     &C.predictor.adam_m, &C.predictor.adam_v, &C.predictor.grad,
                     &C.predictor.features, &C.predictor.labels, &C.predictor.act,
                     &C.predictor.loss, &C.predictor.index, &C.predictor.budget,
-                    &C.predictor.slots, &C.predictor.slot_elems, &C.predictor.overflow,
-                    &C.predictor.arena, &C.predictor.aoff, &C.fscratch};
+                    &C.predictor.overflow, &C.predictor.spill, &C.predictor.aoff, &C.offsets_dense,
+                    &C.elements_dense, &C.fscratch};
+  release_graph(C);
   for (DevBuf* b : bufs) b->release();
   if (C.hstat) hipHostFree(C.hstat);
   C.hstat = nullptr;
@@ -196,8 +197,16 @@ static void run_supermesh(Context& C, const libsupermesh_gpu_mesh* a, const libs
   out->indptr = static_cast<const long*>(C.indptr.ptr);
   out->indices = static_cast<const int*>(C.indices.get(4));
   out->rows = static_cast<const int*>(C.rows.get(4));
+  LSM_REQUIRE(!(clip && (flags & LIBSUPERMESH_GPU_GRAPH) &&
+                (mixed || (flags & (LIBSUPERMESH_GPU_ELEMENTS | LIBSUPERMESH_GPU_MASS | LIBSUPERMESH_GPU_LEARNED)))),
+              LIBSUPERMESH_GPU_ERROR_INVALID,
+              "LIBSUPERMESH_GPU_GRAPH records the count pass of simplex meshes: not with ELEMENTS, MASS or LEARNED");
   if (clip && mixed) {
     clip_pairs_mixed(C, C.mesh_a, C.mesh_b, npairs, flags, out);
+    ph.mark();
+    ph.mark();
+  } else if (clip && (flags & LIBSUPERMESH_GPU_GRAPH) && npairs > 0) {
+    count_pass_graph(C, C.mesh_a, C.mesh_b, npairs, out, ngroups, a_off);
     ph.mark();
     ph.mark();
   } else if (clip) {
@@ -684,6 +693,34 @@ int libsupermesh_gpu_supermesh_batched(libsupermesh_gpu_context* ctx,
     batch->nmeshes = a->nmeshes;
     batch->pair_offsets = static_cast<const long*>(C.gpairs.ptr);
     batch->sums = static_cast<const double*>(C.gsum.ptr);
+  });
+}
+
+int libsupermesh_gpu_count_shape(libsupermesh_gpu_context* ctx, int dim, int* block, int* grid, int* lds_depths,
+                                 float* ms) {
+  return guarded([&] {
+    LSM_REQUIRE(ctx != nullptr && (dim == 2 || dim == 3), LIBSUPERMESH_GPU_ERROR_INVALID, "invalid argument");
+    std::lock_guard<std::mutex> lk(ctx->ctx.mu);
+    const CountShape& s = ctx->ctx.shape[dim - 2];
+    if (block) *block = s.block;
+    if (grid) *grid = s.grid;
+    if (lds_depths) *lds_depths = s.block == 0 ? 0 : (dim == 3 ? 3 - s.reg : 0);
+    if (ms) *ms = s.ms;
+  });
+}
+
+int libsupermesh_gpu_compact(libsupermesh_gpu_context* ctx, libsupermesh_gpu_result* result) {
+  return guarded([&] {
+    LSM_REQUIRE(ctx != nullptr && result != nullptr, LIBSUPERMESH_GPU_ERROR_INVALID, "null argument");
+    Context& C = ctx->ctx;
+    std::lock_guard<std::mutex> lk(C.mu);
+    LSM_HIP(hipSetDevice(C.device));
+    if (result->elements != nullptr && result->elements == C.elements_dense.ptr) return;  // already compact
+    LSM_REQUIRE(result->offsets != nullptr && result->offsets == C.offsets.ptr &&
+                    result->elements == C.elements.ptr && result->counts == C.counts.ptr,
+                LIBSUPERMESH_GPU_ERROR_STATE,
+                "libsupermesh_gpu_compact needs the context's latest result with ELEMENTS");
+    compact_result(C, C.mesh_a.dim, result);
   });
 }
 

@@ -114,7 +114,14 @@ struct Predictor {
   DevBuf features, labels;  // collected training samples
   long nsamples = 0;
   DevBuf act, loss, index;  // training scratch
-  DevBuf budget, slots, slot_elems, overflow, arena, aoff;  // single-pass clip buffers
+  DevBuf budget, overflow, spill, aoff;  // single-pass clip: budgets, flags, spill offsets, arena offsets
+};
+
+// Launch shape of the count-pass kernel: block size, grid (blocks), and in
+// 3D the walk's tet depths held in registers (the rest of 1..3 in LDS).
+struct CountShape {
+  int block = 0, grid = 0, reg = 0;
+  float ms = 0.f;  // its time on the sample it was measured on
 };
 
 struct Context {
@@ -130,6 +137,7 @@ struct Context {
   DevBuf fscratch;               // single-pass search results (slot-major)
   // clipping
   DevBuf counts, measure, partials, scalars, offsets, elements, blocks;
+  DevBuf offsets_dense, elements_dense;  // LIBSUPERMESH_GPU_DENSE: the single pass's runs compacted
   DevBuf work, nwork;            // compacted list of pairs that need the full clip
   // mass assembly
   DevBuf mflag, mslot, mkeys, mvals, mkeys_alt, mvals_alt, ukeys, uvals, nuniq, mindptr, mindices;
@@ -148,6 +156,15 @@ struct Context {
   hipEvent_t ev[16];
   bool events = false;
   bool timing = false;  // record per-phase / clip-kernel events this call
+  // count pass, per dimension ([0] 2D, [1] 3D): the launch shape measured
+  // fastest, and the pass recorded as a graph (LIBSUPERMESH_GPU_GRAPH) with
+  // the arguments it was recorded for
+  CountShape shape[2];
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+  std::vector<const void*> graph_key[2];
+  long graph_cap = 0;          // pairs the recorded passes' buffers hold
+  bool graph_refused = false;  // a recording held other than kernel nodes: not used again
 };
 
 Context* default_context();
@@ -197,6 +214,13 @@ void clip_pairs_mixed(Context& C, const DevMesh& A, const DevMesh& B, long npair
                       libsupermesh_gpu_result* out);
 // Order-sensitive checksum of a device buffer (whole 32-bit words).
 unsigned long long checksum(Context& C, const void* ptr, size_t bytes);
+void compact_result(Context& C, int dim, libsupermesh_gpu_result* out);
+// The count pass of a simplex supermesh (counts, measures, sums, per-mesh
+// sums of a batch) in the launch shape measured fastest, as one recorded
+// graph (LIBSUPERMESH_GPU_GRAPH).
+void count_pass_graph(Context& C, const DevMesh& A, const DevMesh& B, long npairs, libsupermesh_gpu_result* out,
+                      long ngroups, const long* a_off);
+void release_graph(Context& C);
 // Work list (pairs that needed the full clip) of the last 3D clip_pairs call.
 long last_work_list(Context& C, const int** work);
 // Batches: per-mesh candidate pair offsets (gpairs, from the CSR row pointers

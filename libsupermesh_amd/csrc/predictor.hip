@@ -9,7 +9,7 @@
 //   * k_predict: features + forward + softmax; writes each pair's output
 //     budget, the smallest count whose predicted cumulative probability
 //     reaches the quantile q (81 = the tet clip's maximum when that is the
-//     last class);
+//     last class); activations stay in registers from layer to layer;
 //   * k_train_step / k_train_wgrad / k_train_adam: one minibatch of training
 //     (forward, softmax cross-entropy, backward through the ReLU layers,
 //     weight gradients as MFMA reductions over the batch, Adam in FP32 on the
@@ -37,11 +37,35 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 // BF16 weight layouts (element offsets into one array):
 //   forward, [out][in] with in padded to 32 / 64: F1 [64][32], F2 [64][64],
 //   F3 [64][64], F4 [64][64] (rows >= 33 zero);
-//   backward, [in][out]: G2 [64][64], G3 [64][64], G4 [64][64] (cols >= 33 zero).
+//   backward, [in][out]: G2 [64][64], G3 [64][64], G4 [64][64] (cols >= 33 zero);
+//   inference, [out][in] with the inputs permuted (in_pos, feat_pos) so that
+//   a layer's MFMA accumulators are, lane by lane, the next layer's B
+//   operand: I1 [64][32], I2 [64][64], I3 [64][64], I4 [64][64].
 constexpr int kF1 = 0, kF2 = kF1 + 64 * 32, kF3 = kF2 + 64 * 64, kF4 = kF3 + 64 * 64;
 constexpr int kFwdSize = kF4 + 64 * 64;  // 14336
 constexpr int kG2 = kFwdSize, kG3 = kG2 + 64 * 64, kG4 = kG3 + 64 * 64;
-constexpr int kAllSize = kG4 + 64 * 64;
+constexpr int kI1 = kG4 + 64 * 64, kI2 = kI1 + 64 * 32, kI3 = kI2 + 64 * 64, kI4 = kI3 + 64 * 64;
+constexpr int kInfSize = kFwdSize;
+constexpr int kAllSize = kI4 + 64 * 64;
+
+// A 32 x 32 MFMA accumulator holds, in lane (column c, half h = lane / 32),
+// element r (0..15) at row acc_row(r) = 8 (r / 4) + 4 h + r % 4.  A B operand
+// supplies, in lane (column c, half h), the 8 consecutive k of its k-step
+// from 8 h.  Hidden unit n (tile t = n / 32 of a layer's output) sits in the
+// accumulator of its lane's half at r = 4 ((n % 32) / 8) + n % 4; the next
+// layer takes it as input position in_pos(n) = 16 s + 8 h + j with
+// s = 2 t + r / 8, j = r % 8, so lane (c, h) packs accumulator elements
+// 8 (s % 2) .. + 7 of tile s / 2 as its k-step s.
+__host__ __device__ constexpr int in_pos(int n) {
+  return 16 * (2 * (n >> 5) + ((4 * ((n & 31) >> 3) + (n & 3)) >> 3)) + 8 * (((n & 31) >> 2) & 1) +
+         ((4 * ((n & 31) >> 3) + (n & 3)) & 7);
+}
+// Features 0..11 (A's vertices) and 12..23 (B's) at the positions the
+// feature lanes supply: lane half 0 A's coordinates 0-7 (k-step 0) and 8-11
+// (k-step 1), half 1 B's alike; positions 20-23 and 28-31 are zero.
+__host__ __device__ constexpr int feat_pos(int f) {
+  return f < 8 ? f : (f < 12 ? 16 + (f - 8) : (f < 20 ? 8 + (f - 12) : 24 + (f - 20)));
+}
 constexpr int kLd = 64;  // row stride (elements) of activation tiles in LDS
 
 __device__ __forceinline__ int lane64() { return threadIdx.x & 63; }
@@ -107,17 +131,25 @@ __device__ __forceinline__ void pair_features(const double* __restrict__ ta, con
 // reaches it, or the clip's maximum (81) when only the last class ("32 or
 // more") does.  Also returns 1024 x the expected count.
 __device__ __forceinline__ unsigned budgets_of(const float* logit, const float* levels, unsigned& expect) {
+  // unrolled throughout: the logits stay in registers
   float m = logit[0];
+#pragma unroll
   for (int c = 1; c < 33; c++) m = fmaxf(m, logit[c]);
+  float ev[33];
   float s = 0.f;
-  for (int c = 0; c < 33; c++) s += __expf(logit[c] - m);
+#pragma unroll
+  for (int c = 0; c < 33; c++) {
+    ev[c] = __expf(logit[c] - m);
+    s += ev[c];
+  }
   float t[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) t[k] = levels[k] * s;
   unsigned b[4] = {(unsigned)kTetBufSize, (unsigned)kTetBufSize, (unsigned)kTetBufSize, (unsigned)kTetBufSize};
   float acc = 0.f, ex = 0.f;
+#pragma unroll
   for (int c = 0; c < 33; c++) {
-    const float e = __expf(logit[c] - m);
+    const float e = ev[c];
     acc += e;
     ex += (float)c * e;
     if (c < 32) {
@@ -142,80 +174,144 @@ void budget_levels(float q, float* levels) {
 // ---------------------------------------------------------------------------
 
 constexpr int kPredWaves = 4;
+constexpr int kPad = 8;  // LDS row padding (bf16) of the inference weights: rows 16 B apart mod 128
 
-__global__ void __launch_bounds__(64 * kPredWaves) k_predict(PredictArgs g) {
-  __shared__ __align__(16) __bf16 sW[kFwdSize];
-  __shared__ float sB[4 * 64];
-  __shared__ __align__(16) __bf16 sX[kPredWaves][2][32 * kLd];
-  for (int i = threadIdx.x; i < kFwdSize; i += blockDim.x) sW[i] = g.weights[i];
-  for (int i = threadIdx.x; i < 4 * 64; i += blockDim.x) sB[i] = g.bias[i];
-  const int wv = threadIdx.x >> 6, l = lane64();
-  __bf16* X0 = sX[wv][0];
-  __bf16* X1 = sX[wv][1];
-  float* L = reinterpret_cast<float*>(sX[wv][0]);  // 32 x 33 logits over both buffers
-  __shared__ unsigned long long sTot[kPredWaves][5];
-  unsigned long long tot[5] = {0, 0, 0, 0, 0};
-  const long ntiles = (g.nwork + 31) / 32;
-  // uniform trip count per block, so the block barriers below are safe
-  const long per_iter = (long)gridDim.x * kPredWaves;
-  const long iters = (ntiles + per_iter - 1) / per_iter;
-  __syncthreads();
-  for (long it = 0; it < iters; it++) {
-    const long tile = (it * gridDim.x + blockIdx.x) * kPredWaves + wv;
-    const long w = tile * 32 + (l & 31);
-    const bool valid = tile < ntiles && w < g.nwork;
-    if (l < 32) {
-      __bf16* row = X0 + l * kLd;
-      if (valid) {
-        const long p = g.work ? g.work[w] : w;
-        pair_features(g.acoords + (long)g.rows[p] * 12, g.bcoords + (long)g.cols[p] * 12, row);
-      } else {
-        for (int k = 0; k < 32; k++) row[k] = (__bf16)0.f;
-      }
-    }
-    __syncthreads();
-    dense<32, true>(X0, sW + kF1, sB, X1);
-    __syncthreads();
-    dense<64, true>(X1, sW + kF2, sB + 64, X0);
-    __syncthreads();
-    dense<64, true>(X0, sW + kF3, sB + 128, X1);
-    __syncthreads();
-    f32x16 a0 = {}, a1 = {};
-    a0 = mma_tile(X1, kLd, sW + kF4, 64, 64, a0);
-    a1 = mma_tile(X1, kLd, sW + kF4 + 32 * 64, 64, 64, a1);
-    __syncthreads();
-    {
-      const int j = l & 31;
+// B operand of a k-step from accumulator elements rb .. rb + 7 of tile acc:
+// bias (by the hidden unit each element holds) + ReLU, rounded to BF16.
+__device__ __forceinline__ bf16x8 relu_pack(const f32x16& acc, int rb, const float* bias) {
+  bf16x8 v;
 #pragma unroll
-      for (int r = 0; r < 16; r++) L[acc_row(r) * 33 + j] = a0[r] + sB[192 + j];
-      if (j == 0) {
-#pragma unroll
-        for (int r = 0; r < 16; r++) L[acc_row(r) * 33 + 32] = a1[r] + sB[192 + 32];
-      }
-    }
-    __syncthreads();
-    if (l < 32 && valid) {
-      unsigned ex = 0;
-      const unsigned b = budgets_of(L + l * 33, g.levels, ex);
-      g.budget[w] = b;
-#pragma unroll
-      for (int k = 0; k < 4; k++) tot[k] += (b >> (8 * k)) & 0xffu;
-      tot[4] += ex;
-    }
-    __syncthreads();
+  for (int j = 0; j < 8; j++) {
+    const float x = acc[rb + j] + bias[acc_row(rb + j)];
+    v[j] = (__bf16)(x > 0.f ? x : 0.f);
   }
-  // block totals: lanes of a wave, then the block's waves, then one atomic each
+  return v;
+}
+
+// acc[t] (units 32 t .. 32 t + 31 x 32 pairs) = W (inference layout, rows
+// of K + kPad) x the 32 pairs' inputs, given as the B operands of the K / 16
+// k-steps.
+template <int K>
+__device__ __forceinline__ void layer(const __bf16* W, const bf16x8* in, f32x16* acc) {
+  const int i = lane64() & 31, h = (lane64() >> 5) * 8;
+#pragma unroll
+  for (int t = 0; t < 2; t++) {
+    f32x16 a = {};
+#pragma unroll
+    for (int s = 0; s < K / 16; s++) {
+      const bf16x8 va = *reinterpret_cast<const bf16x8*>(W + (32 * t + i) * (K + kPad) + 16 * s + h);
+      a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(va, in[s], a, 0, 0, 0);
+    }
+    acc[t] = a;
+  }
+}
+
+// A wave takes 32 work items at a time with no block barrier: lanes c and
+// c + 32 load pair c's two tets (A, B) and build its features, the four
+// layers run on the MFMA units with every activation kept in registers (the
+// inference layouts put each accumulator where the next layer's operand
+// wants it), and both lanes of a pair evaluate the softmax and budgets.
+__global__ void __launch_bounds__(64 * kPredWaves) __attribute__((amdgpu_waves_per_eu(4))) k_predict(PredictArgs g) {
+  __shared__ __align__(16) __bf16 sW[64 * (32 + kPad) + 3 * 64 * (64 + kPad)];
+  __shared__ float sB[4 * 64];
+  constexpr int o1 = 0, o2 = 64 * (32 + kPad), o3 = o2 + 64 * (64 + kPad), o4 = o3 + 64 * (64 + kPad);
+  for (int x = threadIdx.x; x < 64 * 32; x += blockDim.x) sW[o1 + (x >> 5) * (32 + kPad) + (x & 31)] = g.weights[kI1 + x];
+  for (int x = threadIdx.x; x < 64 * 64; x += blockDim.x) {
+    const int r = x >> 6, c = x & 63;
+    sW[o2 + r * (64 + kPad) + c] = g.weights[kI2 + x];
+    sW[o3 + r * (64 + kPad) + c] = g.weights[kI3 + x];
+    sW[o4 + r * (64 + kPad) + c] = g.weights[kI4 + x];
+  }
+  for (int x = threadIdx.x; x < 4 * 64; x += blockDim.x) sB[x] = g.bias[x];
+  __syncthreads();
+  const int l = lane64(), c = l & 31, half = l >> 5;
+  const long ntiles = (g.nwork + 31) / 32;
+  const long nwaves = (long)gridDim.x * kPredWaves;
+  unsigned long long tot[5] = {0, 0, 0, 0, 0};
+  // the element of this lane's tet of its pair in tile `tile` (A's in half
+  // 0, B's in half 1), -1 past the end: looked up one tile ahead, so that
+  // each tile waits on one load (the coordinates) instead of three
+  auto element_of = [&](long tile) -> long {
+    const long w = tile * 32 + c;
+    if (tile >= ntiles || w >= g.nwork) return -1;
+    const long p = g.work ? g.work[w] : w;
+    return half ? g.cols[p] : g.rows[p];
+  };
+  long tile = (long)blockIdx.x * kPredWaves + (threadIdx.x >> 6);
+  long e = element_of(tile);
+  for (; tile < ntiles; tile += nwaves) {
+    // the weights are read from LDS at each tile: held in registers across
+    // tiles (where the compiler would hoist them) they would not fit
+    int zero = 0;
+    asm volatile("" : "+v"(zero));
+    const __bf16* W = sW + zero;
+    const long w = tile * 32 + c;
+    const bool valid = w < g.nwork;
+    double T[12];
+    {
+      const double* src = (half ? g.bcoords : g.acoords) + (e >= 0 ? e : 0) * 12;
+#pragma unroll
+      for (int k = 0; k < 12; k++) T[k] = e >= 0 ? src[k] : 0.0;
+    }
+    e = element_of(tile + nwaves);
+    // features: both tets normalised to their joint box (pair_features)
+    bf16x8 x[2];
+    {
+      double lo[3], hi[3], sc[3];
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        lo[d] = fmin_f(fmin_f(T[d], T[3 + d]), fmin_f(T[6 + d], T[9 + d]));
+        hi[d] = fmax_f(fmax_f(T[d], T[3 + d]), fmax_f(T[6 + d], T[9 + d]));
+        lo[d] = fmin_f(lo[d], __shfl_xor(lo[d], 32, 64));
+        hi[d] = fmax_f(hi[d], __shfl_xor(hi[d], 32, 64));
+        sc[d] = hi[d] > lo[d] ? 1.0 / (hi[d] - lo[d]) : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < 8; k++) x[0][k] = (__bf16)(float)((T[k] - lo[k % 3]) * sc[k % 3]);
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        x[1][k] = k < 4 ? (__bf16)(float)((T[8 + k] - lo[(8 + k) % 3]) * sc[(8 + k) % 3]) : (__bf16)0.f;
+    }
+    f32x16 acc[2];
+    bf16x8 b[4];
+    layer<32>(W + o1, x, acc);
+#pragma unroll
+    for (int s = 0; s < 4; s++) b[s] = relu_pack(acc[s >> 1], 8 * (s & 1), sB + 32 * (s >> 1));
+    layer<64>(W + o2, b, acc);
+#pragma unroll
+    for (int s = 0; s < 4; s++) b[s] = relu_pack(acc[s >> 1], 8 * (s & 1), sB + 64 + 32 * (s >> 1));
+    layer<64>(W + o3, b, acc);
+#pragma unroll
+    for (int s = 0; s < 4; s++) b[s] = relu_pack(acc[s >> 1], 8 * (s & 1), sB + 128 + 32 * (s >> 1));
+    layer<64>(W + o4, b, acc);
+    // logits of pair c, classes 0..32, in both of its lanes
+    float L[33];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const float mine = acc[0][r], other = __shfl_xor(mine, 32, 64);
+      // class 8 (r / 4) + 4 hh + r % 4 comes from the lane of half hh
+      L[8 * (r >> 2) + (r & 3)] = half == 0 ? mine : other;
+      L[8 * (r >> 2) + 4 + (r & 3)] = half == 0 ? other : mine;
+    }
+    L[32] = __shfl(acc[1][0], c, 64);  // unit 32 = tile 1, row 0: half 0's element 0
+#pragma unroll
+    for (int k = 0; k < 33; k++) L[k] += sB[192 + k];
+    if (valid) {
+      unsigned ex = 0;
+      const unsigned bud = budgets_of(L, g.levels, ex);
+      if (half == 0) {
+        g.budget[w] = bud;
+#pragma unroll
+        for (int k = 0; k < 4; k++) tot[k] += (bud >> (8 * k)) & 0xffu;
+        tot[4] += ex;
+      }
+    }
+  }
 #pragma unroll
   for (int k = 0; k < 5; k++) {
     unsigned long long v = tot[k];
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    if (l == 0) sTot[wv][k] = v;
-  }
-  __syncthreads();
-  if (threadIdx.x < 5) {
-    unsigned long long v = 0;
-    for (int i = 0; i < kPredWaves; i++) v += sTot[i][threadIdx.x];
-    atomicAdd(g.sums + threadIdx.x, v);
+    if (l == 0 && v) atomicAdd(g.sums + k, v);
   }
 }
 
@@ -369,24 +465,28 @@ __device__ __forceinline__ void publish(int i, float w, __bf16* W, float* bias) 
   if (i < kOffB1) {
     const int o = (i - kOffW1) / 24, n = (i - kOffW1) % 24;
     W[kF1 + o * 32 + n] = v;
+    W[kI1 + o * 32 + feat_pos(n)] = v;
   } else if (i < kOffW2) {
     bias[i - kOffB1] = w;
   } else if (i < kOffB2) {
     const int o = (i - kOffW2) / 64, n = (i - kOffW2) % 64;
     W[kF2 + o * 64 + n] = v;
     W[kG2 + n * 64 + o] = v;
+    W[kI2 + o * 64 + in_pos(n)] = v;
   } else if (i < kOffW3) {
     bias[64 + i - kOffB2] = w;
   } else if (i < kOffB3) {
     const int o = (i - kOffW3) / 64, n = (i - kOffW3) % 64;
     W[kF3 + o * 64 + n] = v;
     W[kG3 + n * 64 + o] = v;
+    W[kI3 + o * 64 + in_pos(n)] = v;
   } else if (i < kOffW4) {
     bias[128 + i - kOffB3] = w;
   } else if (i < kOffB4) {
     const int o = (i - kOffW4) / 64, n = (i - kOffW4) % 64;
     W[kF4 + o * 64 + n] = v;
     W[kG4 + n * 64 + o] = v;
+    W[kI4 + o * 64 + in_pos(n)] = v;
   } else {
     bias[192 + i - kOffB4] = w;
   }
@@ -613,7 +713,7 @@ void predict_budgets(Context& C, const PredictArgs& args) {
   int cus = 0;
   LSM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, C.device));
   const long tiles = (args.nwork + 31) / 32;
-  long blocks = std::min<long>((long)(cus > 0 ? cus : 256) * 2, (tiles + kPredWaves - 1) / kPredWaves);
+  long blocks = std::min<long>((long)(cus > 0 ? cus : 256) * 4, (tiles + kPredWaves - 1) / kPredWaves);
   blocks = std::max<long>(blocks, 1);
   hipLaunchKernelGGL(k_predict, dim3((unsigned)blocks), dim3(64 * kPredWaves), 0, C.stream, args);
   LSM_HIP(hipGetLastError());

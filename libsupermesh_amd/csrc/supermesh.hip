@@ -148,6 +148,7 @@ __device__ __forceinline__ void mass_element(const double* el, double meas, cons
 
 struct ClipArgs {
   long npairs;
+  const long* npairs_dev;  // when set, the pair count is read here (a recorded graph's count pass)
   const int* rows;
   const int* cols;
   const int* work;    // when non-null: process pairs work[0..nwork)
@@ -166,20 +167,21 @@ struct ClipArgs {
   double* blocks;
   const long* offsets;
   double* elements;
-  // single-pass (predicted budget) clip: per work item budget and slot offset
+  // single pass (predicted budget) clip: per work item budget; each pair's
+  // run starts at offsets[p] (the budgets laid end to end)
   const unsigned* budget;  // packed budget levels; this call uses byte bshift / 8
   int bshift;
-  const long* slots;
-  double* slot_elems;
-  // outputs beyond a budget go to the walking lane's own arena region (no
-  // atomics: a lane writes one pair at a time, so a pair's extra outputs are
-  // contiguous); aoff[w] = their start, or -1
-  double* arena;
+  // outputs beyond a budget continue in the walking lane's own arena region
+  // (no atomics: a lane finishes one pair before the next), elements
+  // [arena_base + lane * arena_per_lane, + arena_per_lane); aoff[w] = where
+  // work item w's continue.  dropped[w]: 1 the run is split (budget, then
+  // arena), 2 the lane's arena ran out (the pair is clipped again)
+  long arena_base;
   long arena_per_lane;
   long* aoff;
-  unsigned char* dropped;      // single pass: the lane's arena ran out, outputs dropped
-  unsigned long long* ndropped;
-  const unsigned char* ovf;  // fill pass: when non-null, only items flagged here
+  unsigned char* dropped;
+  unsigned long long* ndropped;  // [0] split runs, [1] pairs clipped again
+  const unsigned char* ovf;  // fill pass: when non-null, only items flagged 2 here
   // 2D: pairs whose working polygon outgrew its LDS slots, for the
   // streaming clipper
   int* olist;
@@ -205,18 +207,22 @@ __device__ __forceinline__ void load_planes(const double* __restrict__ bp, long 
   }
 }
 
+__device__ __forceinline__ long pair_count(const ClipArgs& g) { return g.npairs_dev ? *g.npairs_dev : g.npairs; }
+
 // Tet pairs: pairs whose A lies provably beyond one of B's face planes get
 // their (empty) result here; the rest are listed for the full clip.  Block b
-// owns pairs [b*chunk, (b+1)*chunk) and writes the survivors, in pair order,
-// to its own region work[b*chunk ..] (no atomics); k_gather_work then packs
-// the regions into one dense list.
+// owns pairs [b*chunk, (b+1)*chunk), chunk = ceil(npairs / grid), and writes
+// the survivors, in pair order, to its own region work[b*chunk ..] (no
+// atomics); k_gather_work then packs the regions into one dense list.
 template <bool MASS>
-__global__ void __launch_bounds__(kClipBlock) k_tet_triage(ClipArgs g, long chunk, int* __restrict__ work,
+__global__ void __launch_bounds__(kClipBlock) k_tet_triage(ClipArgs g, int* __restrict__ work,
                                                            long* __restrict__ block_count) {
   __shared__ int wave_count[kClipBlock / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long np = pair_count(g);
+  const long chunk = (np + gridDim.x - 1) / gridDim.x;
   const long lo = (long)blockIdx.x * chunk;
-  const long hi = lo + chunk < g.npairs ? lo + chunk : g.npairs;
+  const long hi = lo + chunk < np ? lo + chunk : np;
   long kept = 0;
   for (long base = lo; base < hi; base += kClipBlock) {
     const long p = base + threadIdx.x;
@@ -256,11 +262,35 @@ __global__ void __launch_bounds__(kClipBlock) k_tet_triage(ClipArgs g, long chun
 }
 
 __global__ void __launch_bounds__(kClipBlock)
-k_gather_work(long chunk, const long* __restrict__ block_offset, const int* __restrict__ src,
-              int* __restrict__ dst) {
+k_gather_work(long npairs, const long* __restrict__ npairs_dev, const long* __restrict__ block_offset,
+              const int* __restrict__ src, int* __restrict__ dst) {
+  const long np = npairs_dev ? *npairs_dev : npairs;
+  const long chunk = (np + gridDim.x - 1) / gridDim.x;
   const long b = blockIdx.x;
   const long o = block_offset[b], n = block_offset[b + 1] - o;
   for (long i = threadIdx.x; i < n; i += kClipBlock) dst[o + i] = src[b * chunk + i];
+}
+
+// Exclusive offsets of the triage's kClipGrid block counts (one block):
+// offset[0] = 0, offset[b + 1] = count[0] + .. + count[b]; offset[kClipGrid]
+// is the work list's length.
+__global__ void __launch_bounds__(1024) k_scan_blocks(const long* __restrict__ count, long* __restrict__ offset) {
+  static_assert(kClipGrid == 2048, "two counts per thread");
+  __shared__ long part[1024];
+  const int t = threadIdx.x;
+  const long c0 = count[2 * t], c1 = count[2 * t + 1];
+  part[t] = c0 + c1;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {  // inclusive scan of the pair sums (Hillis-Steele)
+    const long v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  const long before = t > 0 ? part[t - 1] : 0;
+  if (t == 0) offset[0] = 0;
+  offset[2 * t + 1] = before + c0;
+  offset[2 * t + 2] = before + c0 + c1;
 }
 
 template <int DIM, bool MASS, bool FILL>
@@ -268,7 +298,7 @@ __global__ void __launch_bounds__(kClipBlock) k_clip(ClipArgs g) {
   constexpr int LOC = DIM + 1;
   constexpr int ESZ = LOC * DIM;
   const long stride = (long)gridDim.x * blockDim.x;
-  const long n = g.work ? (long)*g.nwork : g.npairs;
+  const long n = g.work ? (long)*g.nwork : pair_count(g);
   for (long w = (long)blockIdx.x * blockDim.x + threadIdx.x; w < n; w += stride) {
     const long p = g.work ? g.work[w] : w;
     const long a = g.rows[p], b = g.cols[p];
@@ -471,22 +501,30 @@ constexpr int kTriGrid = 4096;
 // 2D pairs, one per lane: the count pass (counts, measures, mass blocks and
 // mass row slots) or the fill pass, as k_clip<2, ...>, on clip_poly_lds.
 // Pairs that outgrow the LDS polygon go to g.olist for k_clip.
-template <bool MASS, bool FILL>
-__global__ void __launch_bounds__(kTriBlock) k_clip_tri(ClipArgs g) {
-  __shared__ double poly[LdsPoly<3, 3>::kPoints * 2 * kTriBlock];
+template <bool MASS, bool FILL, int BLOCK = kTriBlock>
+__global__ void __launch_bounds__(BLOCK) k_clip_tri(ClipArgs g) {
+  __shared__ double poly[LdsPoly<3, 3>::kPoints * 2 * BLOCK];
   double* q = poly + threadIdx.x;
-  // count pass with mass: the 9-value blocks of the block's pairs leave
-  // through the (then idle) polygon slots as one contiguous, coalesced run
+  const long np = pair_count(g);
+  // count pass with mass: the mass row slots and the 9-value blocks of the
+  // block's pairs leave through the (then idle) polygon slots, written by
+  // consecutive threads to consecutive words
   constexpr bool STAGE = MASS && !FILL;
-  static_assert(LdsPoly<3, 3>::kPoints * 2 >= 9, "room for a block per lane");
+  static_assert(LdsPoly<3, 3>::kPoints * 2 >= 9 + 3 + 2, "room for a lane's rows: values, targets, keys");
+  double* svals = poly;                                       // [BLOCK][9]
+  long* sdst = reinterpret_cast<long*>(poly + 9 * BLOCK);     // [BLOCK][3]: row slot of (pair, li), or -1
+  int* skey = reinterpret_cast<int*>(poly + 12 * BLOCK);      // [BLOCK][3]: B node, or -1 (empty pair)
   const long stride = (long)gridDim.x * blockDim.x;
   // the same trip count for every lane of the block (barriers below)
-  for (long base = (long)blockIdx.x * blockDim.x; base < g.npairs; base += stride) {
+  for (long base = (long)blockIdx.x * blockDim.x; base < np; base += stride) {
     const long p = base + threadIdx.x;
     double blk[9];
     bool keep = false;  // this lane's block is written from here
+    bool rows = false;  // this lane's row slots are written from here
+    long rdst[3] = {0, 0, 0};
+    int rkey[3] = {0, 0, 0};
     do {
-      if (p >= g.npairs) break;
+      if (p >= np) break;
       const long a = g.rows[p], b = g.cols[p];
       long off = 0;
       int cap = 0;
@@ -515,7 +553,7 @@ __global__ void __launch_bounds__(kTriBlock) k_clip_tri(ClipArgs g) {
       Line L[3];
       tri_lines(Bv, L);
       int k = 0;
-      const int nk = clip_poly_lds<3, 3, kTriBlock>(
+      const int nk = clip_poly_lds<3, 3, BLOCK>(
           A, L, tol, q, [&](double x0, double y0, double x1, double y1, double x2, double y2, double ar) {
             m = m + ar;
             if (MASS) {
@@ -543,27 +581,49 @@ __global__ void __launch_bounds__(kTriBlock) k_clip_tri(ClipArgs g) {
       if (MASS) {
         keep = nk > 0;
         if (g.rkey) {
+          rows = true;
           const long r = (p - g.indptr[a]) * 3;
 #pragma unroll
-          for (int li = 0; li < 3; li++) {
-            const long o = g.rbase[a * 3 + li] + r;
+          for (int li = 0; li < 3; li++) rdst[li] = g.rbase[a * 3 + li] + r;
 #pragma unroll
-            for (int j = 0; j < 3; j++) g.rkey[o + j] = nk == 0 ? -1 : g.enlb[b * 3 + j];
-            if (nk > 0) {
-#pragma unroll
-              for (int j = 0; j < 3; j++) g.rval[o + j] = blk[li * 3 + j];
-            }
-          }
+          for (int j = 0; j < 3; j++) rkey[j] = nk == 0 ? -1 : g.enlb[b * 3 + j];
         }
       }
     } while (false);
+    if (STAGE && g.rkey) {  // the same for the whole grid: the barriers match
+      __syncthreads();  // every lane is done with its polygon slots
+      const int t = threadIdx.x;
+#pragma unroll
+      for (int i = 0; i < 9; i++) svals[t * 9 + i] = keep ? blk[i] : 0.0;
+#pragma unroll
+      for (int li = 0; li < 3; li++) sdst[t * 3 + li] = rows ? rdst[li] : -1;
+#pragma unroll
+      for (int j = 0; j < 3; j++) skey[t * 3 + j] = rkey[j];
+      __syncthreads();
+      // row li of every pair: slot j of pair q at sdst[q][li] + j, pairs in
+      // order, so the threads' writes run along each A element's row chunk
+      const int nv = np - base < BLOCK ? (int)(np - base) : BLOCK;
+#pragma unroll
+      for (int li = 0; li < 3; li++) {
+        for (int x = t; x < 3 * nv; x += BLOCK) {
+          const int q = x / 3, j = x - 3 * q;
+          const long d = sdst[q * 3 + li];
+          if (d >= 0) {
+            const int key = skey[q * 3 + j];
+            g.rkey[d + j] = key;
+            if (key >= 0) g.rval[d + j] = svals[q * 9 + li * 3 + j];
+          }
+        }
+      }
+      __syncthreads();  // the slots hold polygons again
+    }
     if (STAGE && g.blocks) {  // the same for the whole grid: the barriers match
       __syncthreads();  // every lane is done with its polygon slots
 #pragma unroll
       for (int i = 0; i < 9; i++) poly[threadIdx.x * 9 + i] = keep ? blk[i] : 0.0;
       __syncthreads();
-      const long nv = g.npairs - base < kTriBlock ? g.npairs - base : kTriBlock;
-      for (long t = threadIdx.x; t < 9 * nv; t += kTriBlock) g.blocks[base * 9 + t] = poly[t];
+      const long nv = np - base < BLOCK ? np - base : BLOCK;
+      for (long t = threadIdx.x; t < 9 * nv; t += BLOCK) g.blocks[base * 9 + t] = poly[t];
       __syncthreads();  // the slots hold polygons again
     }
   }
@@ -604,26 +664,33 @@ __device__ __forceinline__ void xcd_share(long n, long& w, long& end, long& stri
 // fill (FILL): writes each pair's counts[p] outputs at offsets[p];
 // single pass (LEARNED): the count pass that also writes up to budget[w]
 // outputs of work item w at slots[w] of the slot buffer.
-template <bool MASS, bool FILL, bool LEARNED = false>
-__global__ void __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(MASS ? 2 : 3)))
+// BLOCK and REG (tet depths held in registers, 0 or 1; the other depths of
+// 1..3 in LDS) are launch shapes the count pass can be tuned over.
+template <bool MASS, bool FILL, bool LEARNED = false, int BLOCK = kWalkBlock, int REG = kWalkReg>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MASS || REG == 0 ? 2 : 3)))
 k_clip_tet_walk(ClipArgs g) {
-  __shared__ double stack[(3 - kWalkReg) * 12 * kWalkBlock];
+  static_assert(REG == 0 || REG == 1, "depth 1 in registers or not");
+  __shared__ double stack[(3 - REG) * 12 * BLOCK];
   double* stk = stack + threadIdx.x;
-  double R[12 * kWalkReg];
+  double R[REG > 0 ? 12 * REG : 1];
   // blocks go to the XCDs round robin: each XCD takes one contiguous share
   // of the work list, so that its L2 holds one region of A and B
   long w, n, stride;
   xcd_share((long)*g.nwork, w, n, stride);
   long p = 0, off = 0;
   int cap = 0, k = 0;
-  // single pass: this lane's arena region and its fill level
-  const long abase = LEARNED ? ((long)blockIdx.x * blockDim.x + threadIdx.x) * g.arena_per_lane : 0;
+  // single pass: this lane's arena region, its fill level, and whether the
+  // current pair's output outgrew budget and arena
+  const long abase =
+      LEARNED ? g.arena_base + ((long)blockIdx.x * blockDim.x + threadIdx.x) * g.arena_per_lane : 0;
   long atop = 0;
   bool lost = false;
   double A[12], tol = 0.0, m = 0.0;
   double blk[16], affa[12], affb[12];
   // B's four planes are read from memory at each step (the pair's 128 bytes
-  // stay in cache): held in registers they would cost the third wave per SIMD
+  // stay in cache): held in registers they would cost the third wave per
+  // SIMD, and even planes 2 and 3 alone (16 registers more, no fewer waves)
+  // made the count pass slower (30.2 against 28.1 ms on tet 64^3 vs 60^3)
   const double* bpl = nullptr;
   auto plane = [&](int j, double* o) {
 #pragma unroll
@@ -639,14 +706,14 @@ k_clip_tet_walk(ClipArgs g) {
         p = g.work[w];
         if (!FILL) break;
         cap = g.counts[p];
-        if (cap > 0 && (!g.ovf || g.ovf[w])) break;
+        if (cap > 0 && (!g.ovf || g.ovf[w] == 2)) break;
         w += stride;
       }
       if (w >= n) break;
       if (FILL) off = g.offsets[p];
       if (LEARNED) {
         cap = (int)((g.budget[w] >> g.bshift) & 0xffu);
-        off = g.slots[w];
+        off = g.offsets[p];
         lost = false;
       }
       const long a = g.rows[p], b = g.cols[p];
@@ -672,7 +739,7 @@ k_clip_tet_walk(ClipArgs g) {
       active = true;
     }
     if (W.level > 0) {
-      walk_next<4, kWalkBlock, kWalkReg>(W, A, R, stk, plane, tol, [&](const double* t, double, bool keep) {
+      walk_next<4, BLOCK, REG>(W, A, R, stk, plane, tol, [&](const double* t, double, bool keep) {
         // every step hands over its child; only kept leaves count, and the
         // measure adds 0 for the others (exactly the sum of the kept ones)
         const double vr = tet_volume_real(t);
@@ -680,13 +747,14 @@ k_clip_tet_walk(ClipArgs g) {
         if (keep) {
           if (MASS) mass_element<3>(t, vr, affa, affb, blk);
           if ((FILL || LEARNED) && k < cap) {
-            double* o = (LEARNED ? g.slot_elems : g.elements) + (off + k) * 12;
+            double* o = g.elements + (off + k) * 12;
 #pragma unroll
             for (int i = 0; i < 12; i++) o[i] = t[i];
           } else if (LEARNED) {
+            // beyond the budget: the run continues in this lane's arena
             const long e = atop + (k - cap);
             if (e < g.arena_per_lane) {
-              double* o = g.arena + (abase + e) * 12;
+              double* o = g.elements + (abase + e) * 12;
 #pragma unroll
               for (int i = 0; i < 12; i++) o[i] = t[i];
             } else {
@@ -699,11 +767,12 @@ k_clip_tet_walk(ClipArgs g) {
       if (W.level > 0) continue;
     }
     if (LEARNED) {
-      const bool extra = k > cap && !lost;
-      g.aoff[w] = extra ? abase + atop : -1;
-      if (extra) atop += k - cap;
-      g.dropped[w] = lost;
-      if (lost) atomicAdd(g.ndropped, 1ULL);
+      const bool split = k > cap && !lost;
+      g.aoff[w] = split ? abase + atop : -1;
+      if (split) atop += k - cap;
+      g.dropped[w] = lost ? 2 : (split ? 1 : 0);
+      if (split) atomicAdd(g.ndropped, 1ULL);
+      if (lost) atomicAdd(g.ndropped + 1, 1ULL);
     }
     if (!FILL) {
       g.counts[p] = k;
@@ -721,8 +790,9 @@ k_clip_tet_walk(ClipArgs g) {
 // Totals over pairs in pair order: fixed grid, fixed tree, so the sum is
 // reproducible for a given input.
 __global__ void __launch_bounds__(kClipBlock)
-k_reduce_pairs(long npairs, const int* __restrict__ counts, const double* __restrict__ measure,
-               Partial* __restrict__ partials) {
+k_reduce_pairs(long npairs, const long* __restrict__ npairs_dev, const int* __restrict__ counts,
+               const double* __restrict__ measure, Partial* __restrict__ partials) {
+  if (npairs_dev) npairs = *npairs_dev;
   double hi = 0.0, lo = 0.0;
   long long nonempty = 0, nout = 0, maxc = 0;
   const long stride = (long)gridDim.x * blockDim.x;
@@ -1080,7 +1150,7 @@ void clip_pairs_mixed(Context& C, const DevMesh& A, const DevMesh& B, long npair
     else launch_hex_walk<false>(C, A.loc, B.loc, h);
   }
   Partial* parts = C.partials.as<Partial>(kClipGrid);
-  hipLaunchKernelGGL(k_reduce_pairs, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, npairs, g.counts, g.measure,
+  hipLaunchKernelGGL(k_reduce_pairs, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, npairs, nullptr, g.counts, g.measure,
                      parts);
   hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kClipBlock), 0, C.stream, kClipGrid, parts);
   LSM_HIP(hipGetLastError());
@@ -1123,6 +1193,7 @@ void clip_pairs_mixed(Context& C, const DevMesh& A, const DevMesh& B, long npair
     LSM_HIP(hipStreamSynchronize(C.stream));
     out->offsets = offs;
     out->elements = static_cast<const double*>(C.elements.get(8));
+    out->elements_size = out->nout;
   }
 }
 
@@ -1212,6 +1283,10 @@ void batch_sums(Context& C, long ngroups, const long* a_offsets, const long* ind
   LSM_HIP(hipGetLastError());
 }
 
+__global__ void k_zero_u64(unsigned long long* __restrict__ p) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
+}
+
 template <int DIM>
 static void launch_clip(Context& C, const ClipArgs& args, bool mass, bool fill) {
   dim3 grid(kClipGrid), block(kClipBlock);
@@ -1221,7 +1296,7 @@ static void launch_clip(Context& C, const ClipArgs& args, bool mass, bool fill) 
     ClipArgs g = args;
     g.olist = C.work.as<int>((size_t)(args.npairs > 0 ? args.npairs : 1));
     g.nolist = C.scalars.as<unsigned long long>(16) + 2;
-    if (!fill) LSM_HIP(hipMemsetAsync(g.nolist, 0, sizeof(unsigned long long), C.stream));
+    if (!fill) hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, C.stream, g.nolist);
     const dim3 tgrid(kTriGrid), tblock(kTriBlock);
     if (fill) hipLaunchKernelGGL((k_clip_tri<false, true>), tgrid, tblock, 0, C.stream, g);
     else if (mass) hipLaunchKernelGGL((k_clip_tri<true, false>), tgrid, tblock, 0, C.stream, g);
@@ -1284,56 +1359,150 @@ static void launch_tet_walk(Context& C, const ClipArgs& args, bool mass, bool fi
   launch_tet_walk(C, args, fill ? 2 : (mass ? 1 : 0));
 }
 
-// Single pass: moves each work item's outputs (its slot, then its arena run)
-// to the final layout.  Items whose outputs were dropped are left to the fill
-// pass.  A wave takes 64 consecutive work items and copies them one after
-// another, its lanes on consecutive 16-byte words.
+// Single pass: each pair's budget (0 for pairs outside the work list) as the
+// run lengths whose prefix sums place the runs.
+__global__ void __launch_bounds__(256)
+k_pair_budgets(long nwork, const int* __restrict__ work, const unsigned* __restrict__ budget, int shift,
+               long* __restrict__ len) {
+  const long w = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w < nwork) len[work[w]] = (budget[w] >> shift) & 0xffu;
+}
+
+__global__ void __launch_bounds__(256) k_fill_long(long n, long v, long* __restrict__ out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = v;
+}
+
+// Single pass, pairs whose run is split or was dropped: their output lengths
+// (0 for the others), and then the dropped ones' runs placed from `base`.
+__global__ void __launch_bounds__(256)
+k_dropped_lengths(long nwork, const int* __restrict__ work, const unsigned char* __restrict__ dropped,
+                  const int* __restrict__ counts, long* __restrict__ len) {
+  const long w = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w < nwork) len[w] = dropped[w] ? counts[work[w]] : 0;
+}
+
+__global__ void __launch_bounds__(256)
+k_dropped_offsets(long nwork, const int* __restrict__ work, const unsigned char* __restrict__ dropped,
+                  const long* __restrict__ incl, const int* __restrict__ counts, long base,
+                  long* __restrict__ offsets) {
+  const long w = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w < nwork && dropped[w] == 2) {
+    const long p = work[w];
+    offsets[p] = base + incl[w] - counts[p];
+  }
+}
+
+// Single pass, split runs: the budget's part and the arena's part of each
+// joined end to end at base + incl[w] - counts[p], which becomes the pair's
+// offset.  A wave takes 64 consecutive work items and copies the split ones
+// one after another, its lanes on consecutive 16-byte words.
 __global__ void __launch_bounds__(kClipBlock)
-k_compact_slots(const unsigned long long* __restrict__ nwork, const int* __restrict__ work,
-                const int* __restrict__ counts, const unsigned* __restrict__ budget, int bshift,
-                const long* __restrict__ slots, const double* __restrict__ slot_elems,
-                const long* __restrict__ aoff, const double* __restrict__ arena,
-                const unsigned char* __restrict__ dropped, const long* __restrict__ offsets,
-                double* __restrict__ elements) {
-  const long n = (long)*nwork;
+k_join_runs(long nwork, const int* __restrict__ work, const unsigned char* __restrict__ dropped,
+            const int* __restrict__ counts, const unsigned* __restrict__ budget, int shift,
+            const long* __restrict__ aoff, const long* __restrict__ incl, long base, long* __restrict__ offsets,
+            double* __restrict__ el) {
   const int lane = threadIdx.x & 63;
   const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const long nwaves = ((long)gridDim.x * blockDim.x) >> 6;
-  for (long base = wave * 64; base < n; base += nwaves * 64) {
-    const long w = base + lane;
-    long src = 0, dst = 0, asrc = 0;
-    int c1 = 0, c2 = 0;
-    if (w < n && !dropped[w]) {
-      const long p = work[w];
+  for (long b0 = wave * 64; b0 < nwork; b0 += nwaves * 64) {
+    const long w = b0 + lane;
+    long p = 0, s1 = 0, s2 = 0, dst = 0;
+    int n1 = 0, n2 = 0;
+    if (w < nwork && dropped[w] == 1) {
+      p = work[w];
       const int c = counts[p];
-      const int b = (int)((budget[w] >> bshift) & 0xffu);
-      c1 = c < b ? c : b;
-      c2 = c - c1;
-      src = slots[w];
-      asrc = aoff[w];
-      dst = offsets[p];
+      n1 = (int)((budget[w] >> shift) & 0xffu);
+      n2 = c - n1;
+      s1 = offsets[p];
+      s2 = aoff[w];
+      dst = base + incl[w] - c;
     }
     for (int k = 0; k < 64; k++) {
-      const int n1 = __shfl(c1, k, 64), n2 = __shfl(c2, k, 64);
-      if (n1 + n2 == 0) continue;
-      const long dk = __shfl(dst, k, 64);
-      double2* d2 = reinterpret_cast<double2*>(elements + dk * 12);
-      if (n1 > 0) {
-        const double2* s2 = reinterpret_cast<const double2*>(slot_elems + __shfl(src, k, 64) * 12);
-        for (int i = lane; i < 6 * n1; i += 64) d2[i] = s2[i];
+      const int m1 = __shfl(n1, k, 64), m2 = __shfl(n2, k, 64);
+      if (m1 + m2 == 0) continue;
+      double2* d2 = reinterpret_cast<double2*>(el + __shfl(dst, k, 64) * 12);
+      const double2* a2 = reinterpret_cast<const double2*>(el + __shfl(s1, k, 64) * 12);
+      const double2* b2 = reinterpret_cast<const double2*>(el + __shfl(s2, k, 64) * 12);
+      for (int i = lane; i < 6 * m1; i += 64) d2[i] = a2[i];
+      for (int i = lane; i < 6 * m2; i += 64) d2[6 * m1 + i] = b2[i];
+    }
+    if (n1 + n2 > 0) offsets[p] = dst;
+  }
+}
+
+// Dense layout of a single pass's output: pair p's run (counts[p] elements
+// of EW doubles at src_off[p]) to dst_off[p], the prefix sums of the counts.
+// A wave takes 64 consecutive pairs and copies their runs one after
+// another, its lanes on consecutive 16-byte words.
+template <int EW>
+__global__ void __launch_bounds__(kClipBlock)
+k_compact_runs(long npairs, const int* __restrict__ counts, const long* __restrict__ src_off,
+               const double* __restrict__ src, const long* __restrict__ dst_off, double* __restrict__ dst) {
+  static_assert(EW % 2 == 0, "whole 16-byte words per element");
+  const int lane = threadIdx.x & 63;
+  const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long nwaves = ((long)gridDim.x * blockDim.x) >> 6;
+  for (long base = wave * 64; base < npairs; base += nwaves * 64) {
+    const long p = base + lane;
+    long so = 0, dof = 0;
+    int c = 0;
+    if (p < npairs) {
+      c = counts[p];
+      if (c > 0) {
+        so = src_off[p];
+        dof = dst_off[p];
       }
-      if (n2 > 0) {
-        const double2* a2 = reinterpret_cast<const double2*>(arena + __shfl(asrc, k, 64) * 12);
-        for (int i = lane; i < 6 * n2; i += 64) d2[6 * n1 + i] = a2[i];
-      }
+    }
+    for (int k = 0; k < 64; k++) {
+      const int n = __shfl(c, k, 64);
+      if (n == 0) continue;
+      const double2* s2 = reinterpret_cast<const double2*>(src + __shfl(so, k, 64) * EW);
+      double2* d2 = reinterpret_cast<double2*>(dst + __shfl(dof, k, 64) * EW);
+      for (int i = lane; i < (EW / 2) * n; i += 64) d2[i] = s2[i];
     }
   }
 }
 
-__global__ void __launch_bounds__(256)
-k_budget_to_long(long n, const unsigned* __restrict__ budget, int shift, long* __restrict__ out) {
-  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = (budget[i] >> shift) & 0xffu;
+void compact_runs(Context& C, int dim, long npairs, const int* counts, const long* src_off, const double* src,
+                  const long* dst_off, double* dst) {
+  if (npairs <= 0) return;
+  if (dim == 2)
+    hipLaunchKernelGGL(k_compact_runs<6>, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, npairs, counts, src_off,
+                       src, dst_off, dst);
+  else
+    hipLaunchKernelGGL(k_compact_runs<12>, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, npairs, counts,
+                       src_off, src, dst_off, dst);
+  LSM_HIP(hipGetLastError());
+}
+
+template <class T>
+static void inclusive_scan_long(Context& C, const T* in, long* out, long n) {
+  size_t bytes = 0;
+  LSM_HIP(rocprim::inclusive_scan(nullptr, bytes, in, out, (size_t)n, rocprim::plus<long>(), C.stream));
+  void* tmp = C.scan_tmp.get(bytes);
+  LSM_HIP(rocprim::inclusive_scan(tmp, bytes, in, out, (size_t)n, rocprim::plus<long>(), C.stream));
+}
+
+void compact_result(Context& C, int dim, libsupermesh_gpu_result* out);
+// Tet triage and the work list of the pairs that need the full clip
+// (g.work, g.nwork on the device); buffers sized for `cap` pairs.  Only
+// kernels, so that a graph can record it.
+static void triage_tets(Context& C, ClipArgs& g, bool mass, long cap) {
+  int* region = C.mflag.as<int>(cap);
+  long* bcount = C.nwork.as<long>(2 * kClipGrid + 2);
+  long* boff = bcount + kClipGrid + 1;
+  if (mass)
+    hipLaunchKernelGGL(k_tet_triage<true>, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, g, region, bcount);
+  else
+    hipLaunchKernelGGL(k_tet_triage<false>, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, g, region, bcount);
+  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, C.stream, bcount, boff);
+  int* work = C.work.as<int>(cap);
+  hipLaunchKernelGGL(k_gather_work, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, g.npairs, g.npairs_dev, boff,
+                     region, work);
+  LSM_HIP(hipGetLastError());
+  g.work = work;
+  g.nwork = reinterpret_cast<const unsigned long long*>(boff + kClipGrid);
 }
 
 bool mass_prepare_rows(Context& C, const DevMesh& A, const DevMesh& B, const long* indptr, ClipArgs& g);
@@ -1368,43 +1537,20 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
   const bool blocks = mass && (dim == 3 || !C.mass_rows_ready || (flags & LIBSUPERMESH_GPU_BLOCKS));
   g.blocks = blocks ? C.blocks.as<double>((size_t)npairs * loc * loc) : nullptr;
   Partial* parts = C.partials.as<Partial>(kClipGrid);
-  if (dim == 3 && npairs > 0) {
-    const long chunk = (npairs + kClipGrid - 1) / kClipGrid;
-    int* region = C.mflag.as<int>(npairs);
-    long* bcount = C.nwork.as<long>(2 * kClipGrid + 2);
-    long* boff = bcount + kClipGrid + 1;
-    if (mass)
-      hipLaunchKernelGGL(k_tet_triage<true>, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, g, chunk,
-                         region, bcount);
-    else
-      hipLaunchKernelGGL(k_tet_triage<false>, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, g, chunk,
-                         region, bcount);
-    LSM_HIP(hipGetLastError());
-    LSM_HIP(hipMemsetAsync(boff, 0, sizeof(long), C.stream));
-    size_t bytes = 0;
-    LSM_HIP(rocprim::inclusive_scan(nullptr, bytes, bcount, boff + 1, (size_t)kClipGrid,
-                                    rocprim::plus<long>(), C.stream));
-    void* tmp = C.scan_tmp.get(bytes);
-    LSM_HIP(rocprim::inclusive_scan(tmp, bytes, bcount, boff + 1, (size_t)kClipGrid,
-                                    rocprim::plus<long>(), C.stream));
-    int* work = C.work.as<int>(npairs);
-    hipLaunchKernelGGL(k_gather_work, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, chunk, boff, region,
-                       work);
-    LSM_HIP(hipGetLastError());
-    g.work = work;
-    g.nwork = reinterpret_cast<const unsigned long long*>(boff + kClipGrid);
-  }
+  if (dim == 3 && npairs > 0) triage_tets(C, g, mass, npairs);
   bool learned = elems && dim == 3 && (flags & LIBSUPERMESH_GPU_LEARNED) && npairs > 0;
-  long nwork = 0, nslots = 0, arena_tets = 0;
+  const bool dense = flags & LIBSUPERMESH_GPU_DENSE;
+  long nwork = 0, nslots = 0, nspill = 0, capacity = 0, arena_per_lane = 0;
+  long* offs = nullptr;
   if (learned) {
-    // single pass: predicted budgets -> slots -> one walk that counts and writes
+    // single pass: predicted budgets -> runs placed from them -> one walk
+    // that counts and writes
     Predictor& P = C.predictor;
     LSM_REQUIRE(P.ready, LIBSUPERMESH_GPU_ERROR_STATE,
                 "LIBSUPERMESH_GPU_LEARNED needs predictor weights (libsupermesh_gpu_predictor_set)");
     LSM_HIP(hipMemcpyAsync(&nwork, g.nwork, sizeof(long), hipMemcpyDeviceToHost, C.stream));
     LSM_HIP(hipStreamSynchronize(C.stream));
     unsigned* budget = P.budget.as<unsigned>((size_t)(nwork > 0 ? nwork : 1));
-    long* slots = P.slots.as<long>((size_t)nwork + 1);
     unsigned long long* sums = C.scalars.as<unsigned long long>(16) + 8;
     LSM_HIP(hipMemsetAsync(sums, 0, 5 * sizeof(unsigned long long), C.stream));
     PredictArgs pa = {};
@@ -1425,49 +1571,58 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
     unsigned long long hs[5] = {0, 0, 0, 0, 0};
     LSM_HIP(hipMemcpyAsync(hs, sums, sizeof(hs), hipMemcpyDeviceToHost, C.stream));
     LSM_HIP(hipStreamSynchronize(C.stream));
-    // the highest quantile level whose slots, with room for the expected
-    // output beside them, fit in the device memory left
+    // the highest quantile level whose runs, with the arena and the spill
+    // reserve behind them (and, for the dense layout, the compacted copy),
+    // fit in the device memory left.  The arena takes the outputs beyond a
+    // budget (a fraction of the expected output, spread over the walk's
+    // lanes), the spill reserve the joined runs of those pairs and of the
+    // pairs whose lane's arena ran out (clipped again).
+    if (!dense) {
+      C.elements_dense.release();
+      C.offsets_dense.release();
+    }
     size_t free_b = 0, total_b = 0;
     LSM_HIP(hipMemGetInfo(&free_b, &total_b));
-    const double avail = (double)free_b + (double)P.slot_elems.bytes + (double)P.arena.bytes +
-                         (double)C.elements.bytes - 4e9;
+    const double avail = (double)free_b + (double)C.elements.bytes + (double)C.elements_dense.bytes - 4e9;
     const double expect = (double)hs[4] / 1024.0;
-    // the arena holds outputs beyond the budgets: a quarter of the expected
-    // output, spread over the walk's lanes
-    arena_tets = (long)(P.arena_fraction * expect) + P.arena_min;
+    const long lanes = tet_walk_lanes(C, mass ? 4 : 3);  // the grid of the single-pass launch
+    arena_per_lane = std::max(1L, ((long)(P.arena_fraction * expect) + P.arena_min) / lanes);
+    nspill = (long)(P.arena_fraction * expect) + P.arena_min;
     int level = -1;
-    for (int k = 0; k < 4 && level < 0; k++)
-      if (((double)hs[k] + (double)arena_tets + 1.1 * expect + 1.0) * 96.0 <= avail) level = k;
+    for (int k = 0; k < 4 && level < 0; k++) {
+      const double need = (double)hs[k] + (double)(arena_per_lane * lanes) + (double)nspill +
+                          (dense ? 1.05 * expect : 0.0) + 1.0;
+      if (need * 96.0 <= avail) level = k;
+    }
     if (level < 0) {
       learned = false;  // no room: the plain count-then-fill path (same output)
     } else {
       g.bshift = 8 * level;
       P.level = level;
       out->learned_quantile = pa.levels[level];
-      LSM_HIP(hipMemsetAsync(slots, 0, sizeof(long), C.stream));
-      if (nwork > 0) {
-        long* wide = C.row_count.as<long>((size_t)nwork);
-        hipLaunchKernelGGL(k_budget_to_long, dim3(grid_for(nwork, 256)), dim3(256), 0, C.stream, nwork, budget,
-                           g.bshift, wide);
-        size_t bytes = 0;
-        LSM_HIP(rocprim::inclusive_scan(nullptr, bytes, wide, slots + 1, (size_t)nwork, rocprim::plus<long>(),
-                                        C.stream));
-        void* tmp = C.scan_tmp.get(bytes);
-        LSM_HIP(rocprim::inclusive_scan(tmp, bytes, wide, slots + 1, (size_t)nwork, rocprim::plus<long>(),
-                                        C.stream));
-      }
-      LSM_HIP(hipMemcpyAsync(&nslots, slots + nwork, sizeof(long), hipMemcpyDeviceToHost, C.stream));
+      // runs: budgets laid end to end in pair order (pairs outside the work
+      // list have none)
+      offs = C.offsets.as<long>(npairs + 1);
+      long* len = C.row_count.as<long>(npairs);
+      hipLaunchKernelGGL(k_fill_long, dim3(grid_for(npairs, 256)), dim3(256), 0, C.stream, npairs, 0L, len);
+      if (nwork > 0)
+        hipLaunchKernelGGL(k_pair_budgets, dim3(grid_for(nwork, 256)), dim3(256), 0, C.stream, nwork, g.work,
+                           budget, g.bshift, len);
+      hipLaunchKernelGGL(k_fill_long, dim3(1), dim3(64), 0, C.stream, 1L, 0L, offs);
+      LSM_HIP(hipGetLastError());
+      inclusive_scan_long(C, len, offs + 1, npairs);
+      LSM_HIP(hipMemcpyAsync(&nslots, offs + npairs, sizeof(long), hipMemcpyDeviceToHost, C.stream));
       LSM_HIP(hipStreamSynchronize(C.stream));
+      capacity = nslots + arena_per_lane * lanes + nspill;
+      g.offsets = offs;
+      g.elements = C.elements.as<double>((size_t)capacity * 12);
       g.budget = budget;
-      g.slots = slots;
-      g.slot_elems = P.slot_elems.as<double>((size_t)(nslots > 0 ? nslots : 1) * 12);
-      const long lanes = tet_walk_lanes(C, mass ? 4 : 3);  // the grid of the single-pass launch
-      g.arena_per_lane = std::max(1L, arena_tets / lanes);
-      g.arena = P.arena.as<double>((size_t)(g.arena_per_lane * lanes) * 12);
+      g.arena_base = nslots;
+      g.arena_per_lane = arena_per_lane;
       g.aoff = P.aoff.as<long>((size_t)(nwork > 0 ? nwork : 1));
       g.dropped = P.overflow.as<unsigned char>((size_t)(nwork > 0 ? nwork : 1));
-      g.ndropped = C.scalars.as<unsigned long long>(16) + 1;
-      LSM_HIP(hipMemsetAsync(g.ndropped, 0, sizeof(unsigned long long), C.stream));
+      g.ndropped = C.scalars.as<unsigned long long>(16) + 4;
+      LSM_HIP(hipMemsetAsync(g.ndropped, 0, 2 * sizeof(unsigned long long), C.stream));
     }
   }
   if (C.timing) LSM_HIP(hipEventRecord(C.ev[14], C.stream));
@@ -1475,7 +1630,7 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
   else if (learned) launch_tet_walk(C, g, mass ? 4 : 3);
   else launch_tet_walk(C, g, mass, false);
   if (C.timing) LSM_HIP(hipEventRecord(C.ev[15], C.stream));
-  hipLaunchKernelGGL(k_reduce_pairs, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, npairs, g.counts,
+  hipLaunchKernelGGL(k_reduce_pairs, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, npairs, nullptr, g.counts,
                      g.measure, parts);
   hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kClipBlock), 0, C.stream, kClipGrid, parts);
   LSM_HIP(hipGetLastError());
@@ -1500,47 +1655,365 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
   out->max_count = npairs > 0 ? (int)tot.maxc : 0;
   out->offsets = nullptr;
   out->elements = nullptr;
-  if (elems) {
-    long* offs = C.offsets.as<long>(npairs + 1);
-    LSM_HIP(hipMemsetAsync(offs, 0, sizeof(long), C.stream));
-    if (npairs > 0) {
-      long* tmp_counts = C.row_count.as<long>(npairs);
-      hipLaunchKernelGGL(k_count_to_long, dim3(grid_for(npairs, 256)), dim3(256), 0, C.stream, npairs,
-                         g.counts, tmp_counts);
-      size_t bytes = 0;
-      LSM_HIP(rocprim::inclusive_scan(nullptr, bytes, tmp_counts, offs + 1, (size_t)npairs,
-                                      rocprim::plus<long>(), C.stream));
-      void* tmp = C.scan_tmp.get(bytes);
-      LSM_HIP(rocprim::inclusive_scan(tmp, bytes, tmp_counts, offs + 1, (size_t)npairs,
-                                      rocprim::plus<long>(), C.stream));
-      g.offsets = offs;
-      g.elements = C.elements.as<double>((size_t)(out->nout > 0 ? out->nout : 1) * loc * dim);
-      if (learned) {
-        hipLaunchKernelGGL(k_compact_slots, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, g.nwork, g.work,
-                           g.counts, g.budget, g.bshift, g.slots, g.slot_elems, g.aoff, g.arena, g.dropped,
-                           offs, g.elements);
-        LSM_HIP(hipGetLastError());
-        unsigned long long h = 0;
-        LSM_HIP(hipMemcpyAsync(&h, g.ndropped, sizeof(h), hipMemcpyDeviceToHost, C.stream));
-        LSM_HIP(hipStreamSynchronize(C.stream));
-        if (h > 0) {
-          // pairs whose outputs did not fit budget and arena: the plain fill
-          // pass, for them only
+  out->elements_size = 0;
+  if (!elems) return;
+  if (learned) {
+    unsigned long long h[2] = {0, 0};
+    LSM_HIP(hipMemcpyAsync(h, g.ndropped, sizeof(h), hipMemcpyDeviceToHost, C.stream));
+    LSM_HIP(hipStreamSynchronize(C.stream));
+    out->learned_slots = nslots;
+    out->learned_overflow = (long)(h[0] + h[1]);
+    bool placed = true;
+    if (h[0] + h[1] > 0) {
+      // split runs are joined, and dropped pairs clipped again, into the
+      // spill reserve
+      long* len = C.row_count.as<long>((size_t)nwork);
+      long* incl = C.predictor.spill.as<long>((size_t)nwork);
+      hipLaunchKernelGGL(k_dropped_lengths, dim3(grid_for(nwork, 256)), dim3(256), 0, C.stream, nwork, g.work,
+                         g.dropped, g.counts, len);
+      LSM_HIP(hipGetLastError());
+      inclusive_scan_long(C, len, incl, nwork);
+      long need = 0;
+      LSM_HIP(hipMemcpyAsync(&need, incl + nwork - 1, sizeof(long), hipMemcpyDeviceToHost, C.stream));
+      LSM_HIP(hipStreamSynchronize(C.stream));
+      placed = need <= nspill;
+      if (placed) {
+        const long base = capacity - nspill;
+        if (h[0] > 0)
+          hipLaunchKernelGGL(k_join_runs, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, nwork, g.work, g.dropped,
+                             g.counts, g.budget, g.bshift, g.aoff, incl, base, offs, g.elements);
+        if (h[1] > 0) {
+          hipLaunchKernelGGL(k_dropped_offsets, dim3(grid_for(nwork, 256)), dim3(256), 0, C.stream, nwork,
+                             g.work, g.dropped, incl, g.counts, base, offs);
+          LSM_HIP(hipGetLastError());
           g.ovf = g.dropped;
           launch_tet_walk(C, g, 2);
           g.ovf = nullptr;
         }
-        out->learned_slots = nslots;
-        out->learned_overflow = (long)h;
-      } else if (dim == 2) {
-        launch_clip<2>(C, g, false, true);
-      } else {
-        launch_tet_walk(C, g, false, true);
+        LSM_HIP(hipGetLastError());
       }
     }
-    out->offsets = offs;
-    out->elements = static_cast<const double*>(C.elements.get(8));
+    if (placed) {
+      out->offsets = offs;
+      out->elements = static_cast<const double*>(C.elements.get(8));
+      out->elements_size = capacity;
+      if (dense) compact_result(C, dim, out);
+      return;
+    }
+    // the spill reserve is too small: the plain fill pass over every pair
+    // (the counts are known), into the dense layout
   }
+  long* dofs = C.offsets.as<long>(npairs + 1);
+  hipLaunchKernelGGL(k_fill_long, dim3(1), dim3(64), 0, C.stream, 1L, 0L, dofs);
+  LSM_HIP(hipGetLastError());
+  if (npairs > 0) {
+    long* tmp_counts = C.row_count.as<long>(npairs);
+    hipLaunchKernelGGL(k_count_to_long, dim3(grid_for(npairs, 256)), dim3(256), 0, C.stream, npairs, g.counts,
+                       tmp_counts);
+    inclusive_scan_long(C, tmp_counts, dofs + 1, npairs);
+    g.offsets = dofs;
+    g.elements = C.elements.as<double>((size_t)(out->nout > 0 ? out->nout : 1) * loc * dim);
+    if (dim == 2) launch_clip<2>(C, g, false, true);
+    else launch_tet_walk(C, g, false, true);
+  }
+  out->offsets = dofs;
+  out->elements = static_cast<const double*>(C.elements.get(8));
+  out->elements_size = out->nout;
+}
+
+// The dense layout of a result whose runs are placed apart (the learned
+// single pass): offsets become the prefix sums of the counts, elements the
+// runs end to end, in the context's second pair of output buffers.
+void compact_result(Context& C, int dim, libsupermesh_gpu_result* out) {
+  const long npairs = out->npairs;
+  long* dofs = C.offsets_dense.as<long>(npairs + 1);
+  hipLaunchKernelGGL(k_fill_long, dim3(1), dim3(64), 0, C.stream, 1L, 0L, dofs);
+  LSM_HIP(hipGetLastError());
+  const int ew = (dim + 1) * dim;
+  double* del = C.elements_dense.as<double>((size_t)(out->nout > 0 ? out->nout : 1) * ew);
+  if (npairs > 0) {
+    long* tmp_counts = C.row_count.as<long>(npairs);
+    hipLaunchKernelGGL(k_count_to_long, dim3(grid_for(npairs, 256)), dim3(256), 0, C.stream, npairs, out->counts,
+                       tmp_counts);
+    inclusive_scan_long(C, tmp_counts, dofs + 1, npairs);
+    compact_runs(C, dim, npairs, out->counts, out->offsets, out->elements, dofs, del);
+  }
+  LSM_HIP(hipStreamSynchronize(C.stream));
+  out->offsets = dofs;
+  out->elements = del;
+  out->elements_size = out->nout;
+}
+
+// ---------------------------------------------------------------------------
+// The count pass (counts, measures, sums; no elements, no mass) in a launch
+// shape picked by measurement, recorded as one hipGraph
+// (LIBSUPERMESH_GPU_GRAPH).  Every kernel of the recorded pass reads the pair
+// count from the device (indptr[na]) and everything it clears is cleared by
+// a kernel, so a replay serves every call whose arguments (buffers, staged
+// meshes, launch shape) are those it was recorded with, whatever its pairs.
+// ---------------------------------------------------------------------------
+
+__global__ void k_min_u64(const unsigned long long* __restrict__ n, unsigned long long cap,
+                          unsigned long long* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *out = *n < cap ? *n : cap;
+}
+
+template <int BLOCK, int REG>
+static int walk_blocks_per_cu() {
+  static int nb = 0;
+  if (nb == 0) {
+    LSM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &nb, reinterpret_cast<const void*>(k_clip_tet_walk<false, false, false, BLOCK, REG>), BLOCK, 0));
+    nb = nb > 0 ? nb : 1;
+  }
+  return nb;
+}
+
+static int device_cus(Context& C) {
+  int cus = 0;
+  LSM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, C.device));
+  return cus > 0 ? cus : 256;
+}
+
+template <int BLOCK, int REG>
+static void walk_count_as(Context& C, const ClipArgs& g, int grid) {
+  hipLaunchKernelGGL((k_clip_tet_walk<false, false, false, BLOCK, REG>), dim3(grid), dim3(BLOCK), 0, C.stream, g);
+}
+
+static void launch_count_walk(Context& C, const ClipArgs& g, const CountShape& s) {
+  switch (s.block * 2 + s.reg) {
+    case 128: walk_count_as<64, 0>(C, g, s.grid); break;
+    case 129: walk_count_as<64, 1>(C, g, s.grid); break;
+    case 256: walk_count_as<128, 0>(C, g, s.grid); break;
+    case 257: walk_count_as<128, 1>(C, g, s.grid); break;
+    case 512: walk_count_as<256, 0>(C, g, s.grid); break;
+    default: walk_count_as<256, 1>(C, g, s.grid); break;
+  }
+  LSM_HIP(hipGetLastError());
+}
+
+// 2D: the LDS clipper in shape s, then the streaming clipper over the pairs
+// it handed back (g.olist, g.nolist set by the caller).
+static void launch_count_tri(Context& C, const ClipArgs& g, const CountShape& s) {
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, C.stream, g.nolist);
+  if (s.block == 64) hipLaunchKernelGGL((k_clip_tri<false, false, 64>), dim3(s.grid), dim3(64), 0, C.stream, g);
+  else if (s.block == 128) hipLaunchKernelGGL((k_clip_tri<false, false, 128>), dim3(s.grid), dim3(128), 0, C.stream, g);
+  else hipLaunchKernelGGL((k_clip_tri<false, false, 256>), dim3(s.grid), dim3(256), 0, C.stream, g);
+  ClipArgs f = g;
+  f.work = g.olist;
+  f.nwork = g.nolist;
+  hipLaunchKernelGGL((k_clip<2, false, false>), dim3(64), dim3(kClipBlock), 0, C.stream, f);
+  LSM_HIP(hipGetLastError());
+}
+
+// Times every candidate shape on a prefix of the pairs (3D: of the work list
+// the triage left in g) and keeps the fastest in C.shape.  Candidates:
+// blocks of 64, 128 or 256 lanes; 3D: the depth-1 tet in registers or in
+// LDS, and every resident block (the walk's lanes are persistent) or half
+// of them; 2D: grids of 1024 to 8192 blocks (pairs per lane of the
+// grid-stride loop).  Each candidate runs once to warm up, then once timed.
+static void tune_count_pass(Context& C, ClipArgs g, int dim) {
+  constexpr unsigned long long kSample = 1ULL << 23;
+  std::vector<CountShape> cand;
+  const int cus = device_cus(C);
+  if (dim == 3) {
+    const int bpc[6] = {walk_blocks_per_cu<64, 0>(), walk_blocks_per_cu<64, 1>(), walk_blocks_per_cu<128, 0>(),
+                        walk_blocks_per_cu<128, 1>(), walk_blocks_per_cu<256, 0>(), walk_blocks_per_cu<256, 1>()};
+    const int blocks[3] = {64, 128, 256};
+    for (int b = 0; b < 3; b++)
+      for (int reg = 0; reg < 2; reg++)
+        for (int half = 0; half < 2; half++) {
+          CountShape c;
+          c.block = blocks[b];
+          c.reg = reg;
+          c.grid = std::max(kXcds, (bpc[2 * b + reg] * cus) >> half);
+          cand.push_back(c);
+        }
+    unsigned long long* ns = C.scalars.as<unsigned long long>(16) + 3;
+    hipLaunchKernelGGL(k_min_u64, dim3(1), dim3(64), 0, C.stream, g.nwork, kSample, ns);
+    g.nwork = ns;
+  } else {
+    for (int block : {64, 128, 256})
+      for (int grid : {1024, 2048, 4096, 8192}) {
+        CountShape c;
+        c.block = block;
+        c.grid = grid;
+        cand.push_back(c);
+      }
+    g.npairs = std::min<long>(g.npairs, (long)kSample);
+    g.npairs_dev = nullptr;
+  }
+  hipEvent_t e0, e1;
+  LSM_HIP(hipEventCreate(&e0));
+  LSM_HIP(hipEventCreate(&e1));
+  CountShape best;
+  for (CountShape& c : cand) {
+    for (int rep = 0; rep < 2; rep++) {
+      if (rep == 1) LSM_HIP(hipEventRecord(e0, C.stream));
+      if (dim == 3) launch_count_walk(C, g, c);
+      else launch_count_tri(C, g, c);
+      if (rep == 1) LSM_HIP(hipEventRecord(e1, C.stream));
+    }
+    LSM_HIP(hipEventSynchronize(e1));
+    LSM_HIP(hipEventElapsedTime(&c.ms, e0, e1));
+    if (best.block == 0 || c.ms < best.ms) best = c;
+  }
+  LSM_HIP(hipEventDestroy(e0));
+  LSM_HIP(hipEventDestroy(e1));
+  C.shape[dim - 2] = best;
+}
+
+// Everything the count pass launches, in shape s: only kernels.
+static void enqueue_count_pass(Context& C, ClipArgs& g, int dim, long cap, const CountShape& s, Partial* parts,
+                               long ngroups, const long* a_off, const long* indptr) {
+  if (dim == 3) {
+    triage_tets(C, g, false, cap);
+    launch_count_walk(C, g, s);
+  } else {
+    launch_count_tri(C, g, s);
+  }
+  hipLaunchKernelGGL(k_reduce_pairs, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, g.npairs, g.npairs_dev,
+                     g.counts, g.measure, parts);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kClipBlock), 0, C.stream, kClipGrid, parts);
+  LSM_HIP(hipGetLastError());
+  if (ngroups > 0) batch_sums(C, ngroups, a_off, indptr, g.measure);
+}
+
+// Records the count pass as a graph; false (and nothing kept) when the
+// recording fails or holds nodes other than kernels.
+static bool record_count_pass(Context& C, ClipArgs& g, int dim, long cap, const CountShape& s, Partial* parts,
+                              long ngroups, const long* a_off, const long* indptr) {
+  hipGraph_t graph = nullptr;
+  if (hipStreamBeginCapture(C.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  bool ok = true;
+  try {
+    enqueue_count_pass(C, g, dim, cap, s, parts, ngroups, a_off, indptr);
+  } catch (...) {
+    ok = false;
+  }
+  const hipError_t end = hipStreamEndCapture(C.stream, &graph);
+  if (end != hipSuccess || !ok || graph == nullptr) {
+    (void)hipGetLastError();
+    if (graph) (void)hipGraphDestroy(graph);
+    return false;
+  }
+  size_t n = 0;
+  LSM_HIP(hipGraphGetNodes(graph, nullptr, &n));
+  std::vector<hipGraphNode_t> nodes(n);
+  LSM_HIP(hipGraphGetNodes(graph, nodes.data(), &n));
+  for (hipGraphNode_t node : nodes) {
+    hipGraphNodeType t;
+    LSM_HIP(hipGraphNodeGetType(node, &t));
+    if (t != hipGraphNodeTypeKernel) ok = false;
+  }
+  hipGraphExec_t exec = nullptr;
+  if (ok && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    ok = false;
+  }
+  if (!ok) {
+    (void)hipGraphDestroy(graph);
+    C.graph_refused = true;
+    return false;
+  }
+  C.graph[dim - 2] = graph;
+  C.graph_exec[dim - 2] = exec;
+  return true;
+}
+
+static void release_graph(Context& C, int d) {
+  if (C.graph_exec[d]) (void)hipGraphExecDestroy(C.graph_exec[d]);
+  if (C.graph[d]) (void)hipGraphDestroy(C.graph[d]);
+  C.graph_exec[d] = nullptr;
+  C.graph[d] = nullptr;
+  C.graph_key[d].clear();
+}
+
+void release_graph(Context& C) {
+  release_graph(C, 0);
+  release_graph(C, 1);
+}
+
+void count_pass_graph(Context& C, const DevMesh& A, const DevMesh& B, long npairs, libsupermesh_gpu_result* out,
+                      long ngroups, const long* a_off) {
+  LSM_REQUIRE(npairs < (1L << 31), LIBSUPERMESH_GPU_ERROR_OVERFLOW,
+              "more than 2^31 - 1 candidate pairs to clip in one call");
+  const int dim = A.dim;
+  // buffers with headroom, so that later calls with a few more pairs replay
+  // the same recording
+  if (npairs > C.graph_cap) C.graph_cap = npairs + npairs / 8 + 1024;
+  const long cap = C.graph_cap;
+  ClipArgs g = {};
+  g.npairs = npairs;
+  g.npairs_dev = out->indptr + out->nelements_a;
+  g.rows = out->rows;
+  g.cols = out->indices;
+  g.acoords = static_cast<const double*>(A.coords.ptr);
+  g.bcoords = static_cast<const double*>(B.coords.ptr);
+  g.asp = static_cast<const double*>(A.meas_sp.ptr);
+  g.bsp = static_cast<const double*>(B.meas_sp.ptr);
+  g.ascale = static_cast<const double*>(A.scale.ptr);
+  g.bscale = static_cast<const double*>(B.scale.ptr);
+  g.bplanes = static_cast<const double*>(B.planes.ptr);
+  g.counts = C.counts.as<int>(cap);
+  g.measure = C.measure.as<double>(cap);
+  Partial* parts = C.partials.as<Partial>(kClipGrid);
+  // every buffer the pass touches, allocated before any recording
+  C.mflag.as<int>(cap);
+  C.work.as<int>(cap);
+  C.nwork.as<long>(2 * kClipGrid + 2);
+  unsigned long long* scal = C.scalars.as<unsigned long long>(16);
+  if (ngroups > 0) {
+    C.gpairs.as<long>((size_t)ngroups + 1);
+    C.gsum.as<double>((size_t)ngroups);
+  }
+  if (dim == 2) {
+    g.olist = static_cast<int*>(C.work.ptr);
+    g.nolist = scal + 2;
+  }
+  if (C.shape[dim - 2].block == 0) {
+    if (dim == 3) triage_tets(C, g, false, cap);  // the work list the walk's candidates run on
+    tune_count_pass(C, g, dim);
+  }
+  const CountShape s = C.shape[dim - 2];
+  const std::vector<const void*> key = {
+      reinterpret_cast<const void*>((long)dim), reinterpret_cast<const void*>(cap),
+      reinterpret_cast<const void*>(ngroups), reinterpret_cast<const void*>((long)(s.block * 4 + s.reg)),
+      reinterpret_cast<const void*>((long)s.grid), g.npairs_dev, g.rows, g.cols, g.acoords, g.bcoords, g.asp,
+      g.bsp, g.ascale, g.bscale, g.bplanes, g.counts, g.measure, parts, C.mflag.ptr, C.work.ptr, C.nwork.ptr,
+      scal, a_off, out->indptr, C.gpairs.ptr, C.gsum.ptr};
+  const int d = dim - 2;
+  if (C.graph_exec[d] && key != C.graph_key[d]) release_graph(C, d);
+  if (!C.graph_exec[d] && !C.graph_refused &&
+      record_count_pass(C, g, dim, cap, s, parts, ngroups, a_off, out->indptr))
+    C.graph_key[d] = key;
+  if (C.timing) LSM_HIP(hipEventRecord(C.ev[14], C.stream));
+  if (C.graph_exec[d]) {
+    LSM_HIP(hipGraphLaunch(C.graph_exec[d], C.stream));
+  } else {
+    ClipArgs e = g;
+    enqueue_count_pass(C, e, dim, cap, s, parts, ngroups, a_off, out->indptr);
+  }
+  if (C.timing) LSM_HIP(hipEventRecord(C.ev[15], C.stream));
+  Partial tot;
+  LSM_HIP(hipMemcpyAsync(&tot, parts, sizeof(Partial), hipMemcpyDeviceToHost, C.stream));
+  LSM_HIP(hipStreamSynchronize(C.stream));
+  if (C.timing) {
+    float ms = 0.f;
+    LSM_HIP(hipEventElapsedTime(&ms, C.ev[14], C.ev[15]));
+    out->ms[7] = ms;
+  }
+  out->counts = g.counts;
+  out->measure = g.measure;
+  out->blocks = nullptr;
+  out->sum = npairs > 0 ? tot.hi + tot.lo : 0.0;
+  out->nonempty = npairs > 0 ? tot.nonempty : 0;
+  out->nout = npairs > 0 ? tot.nout : 0;
+  out->max_count = npairs > 0 ? (int)tot.maxc : 0;
+  out->count_graph = C.graph_exec[d] != nullptr;
 }
 
 // ---------------------------------------------------------------------------
@@ -1770,7 +2243,7 @@ void clip_pairs_noalloc(int device, hipStream_t stream, const libsupermesh_gpu_m
   } else if (npairs > 0) {
     hipLaunchKernelGGL(k_clip_direct_tet, dim3(grid_for(npairs, kWalkBlock)), dim3(kWalkBlock), 0, stream, g);
   }
-  hipLaunchKernelGGL(k_reduce_pairs, dim3(kClipGrid), dim3(kClipBlock), 0, stream, npairs, counts, measure, parts);
+  hipLaunchKernelGGL(k_reduce_pairs, dim3(kClipGrid), dim3(kClipBlock), 0, stream, npairs, nullptr, counts, measure, parts);
   hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kClipBlock), 0, stream, kClipGrid, parts);
   hipLaunchKernelGGL(k_noalloc_sum, dim3(1), dim3(64), 0, stream, parts, sum);
   LSM_HIP(hipGetLastError());
@@ -1919,13 +2392,6 @@ k_mass_copy(long nn, const long* __restrict__ raw_off, const long* __restrict__ 
   }
 }
 
-template <class T>
-static void inclusive_scan_long(Context& C, const T* in, long* out, long n) {
-  size_t bytes = 0;
-  LSM_HIP(rocprim::inclusive_scan(nullptr, bytes, in, out, (size_t)n, rocprim::plus<long>(), C.stream));
-  void* tmp = C.scan_tmp.get(bytes);
-  LSM_HIP(rocprim::inclusive_scan(tmp, bytes, in, out, (size_t)n, rocprim::plus<long>(), C.stream));
-}
 
 static void assemble_mass_sorted(Context& C, const DevMesh& A, const DevMesh& B, long npairs,
                    libsupermesh_gpu_result* out) {

@@ -32,6 +32,8 @@ TIMING = 4
 SORTED = 8
 LEARNED = 16
 BLOCKS = 32
+DENSE = 64
+GRAPH = 128
 PREDICTOR_PARAMS = 12065
 
 _lib = None
@@ -58,7 +60,7 @@ class Result(ctypes.Structure):
                 ("mass_indices", c_void_p), ("mass_values", c_void_p), ("sum", c_double),
                 ("nonempty", c_long), ("nout", c_long), ("max_count", c_int),
                 ("ms", c_double * 8), ("learned_slots", c_long), ("learned_overflow", c_long),
-                ("learned_quantile", c_double)]
+                ("learned_quantile", c_double), ("elements_size", c_long), ("count_graph", c_int)]
 
 
 # libsupermesh_gpu_pair_callback
@@ -112,6 +114,11 @@ def lib():
         L.libsupermesh_gpu_checksum.argtypes = [c_void_p, c_void_p, ctypes.c_size_t,
                                                 POINTER(ctypes.c_ulonglong)]
         L.libsupermesh_gpu_checksum.restype = c_int
+        L.libsupermesh_gpu_compact.argtypes = [c_void_p, POINTER(Result)]
+        L.libsupermesh_gpu_compact.restype = c_int
+        L.libsupermesh_gpu_count_shape.argtypes = [c_void_p, c_int, POINTER(c_int), POINTER(c_int), POINTER(c_int),
+                                                   POINTER(ctypes.c_float)]
+        L.libsupermesh_gpu_count_shape.restype = c_int
         for name in ("set", "get", "quantile", "collect", "train", "clear", "infer", "add_samples"):
             getattr(L, "libsupermesh_gpu_predictor_" + name).restype = c_int
         L.libsupermesh_gpu_malloc.argtypes = [c_int, ctypes.c_size_t, POINTER(c_void_p)]
@@ -168,6 +175,14 @@ class DeviceArray:
 
     def numpy(self):
         return device_to_numpy(self.ptr.value, int(np.prod(self.shape)), self.dtype).reshape(self.shape)
+
+    def write(self, host):
+        """Overwrite the allocation in place (same device address) with a
+        host array of the same shape and type: e.g. a mesh whose nodes moved."""
+        host = np.ascontiguousarray(host, dtype=self.dtype)
+        assert host.shape == self.shape
+        if host.nbytes:
+            _check(lib().libsupermesh_gpu_memcpy(self.ptr, host.ctypes.data, host.nbytes, 1))
 
     def free(self):
         if self.ptr:
@@ -530,6 +545,21 @@ class Context:
                                                           ctypes.byref(r)))
         return r
 
+    def count_shape(self, dim):
+        """The count pass's launch shape measured fastest for `dim`
+        (GRAPH calls): {block, grid, lds_depths, ms}; zeros before the first."""
+        b, g, d = c_int(0), c_int(0), c_int(0)
+        ms = ctypes.c_float(0)
+        _check(lib().libsupermesh_gpu_count_shape(self.handle, dim, ctypes.byref(b), ctypes.byref(g),
+                                                  ctypes.byref(d), ctypes.byref(ms)))
+        return {"block": b.value, "grid": g.value, "lds_depths": d.value, "ms": round(ms.value, 4)}
+
+    def compact(self, r: Result):
+        """The plain layout (offsets = prefix sums of the counts) of the
+        context's latest result, in place; a no-op for a compact one."""
+        _check(lib().libsupermesh_gpu_compact(self.handle, ctypes.byref(r)))
+        return r
+
     def checksum(self, ptr, nbytes):
         """Order-sensitive 64-bit checksum of device memory (whole 32-bit words)."""
         h = ctypes.c_ulonglong(0)
@@ -657,7 +687,7 @@ def fetch(r: Result, dim, what=("indptr", "indices", "rows", "counts", "measure"
         elif w == "offsets":
             out[w] = device_to_numpy(r.offsets, r.npairs + 1, np.int64)
         elif w == "elements":
-            out[w] = device_to_numpy(r.elements, r.nout * loc * dim, np.float64).reshape(-1, loc, dim)
+            out[w] = device_to_numpy(r.elements, r.elements_size * loc * dim, np.float64).reshape(-1, loc, dim)
         elif w == "blocks":
             out[w] = device_to_numpy(r.blocks, r.npairs * loc * loc, np.float64).reshape(-1, loc, loc)
         else:

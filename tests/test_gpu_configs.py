@@ -12,8 +12,9 @@
   count, the per-pair counts and measures, and the measure sum equal those
   of one plain call per mesh pair; every |sum - 1| <= 1e-12.
 * tet 96^3 x 6 vs 90^3 x 6 (seed 4), materialised supermesh: the learned
-  single pass gives the plain count-then-fill path's counts, offsets and
-  output tets (device checksums), and reports the budget level it used.
+  single pass, compacted in the call (LIBSUPERMESH_GPU_DENSE), gives the plain
+  count-then-fill path's counts, offsets and output tets (device checksums),
+  and reports the budget level it used.
 
 These follow the reference's serial loops (src/tests/test_tri_intersector.F90:89-104,
 test_tet_intersector.F90:89-104) at the benchmark sizes.  Each test prints
@@ -136,7 +137,7 @@ def test_learned_tet_96_vs_90_identical_to_plain():
         A, B = gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pb, eb)
         ctx.predictor_set(np.fromfile(WEIGHTS, dtype=np.float32))
         res = {}
-        for mode, flags in (("plain", gpu.ELEMENTS), ("learned", gpu.ELEMENTS | gpu.LEARNED)):
+        for mode, flags in (("plain", gpu.ELEMENTS), ("learned", gpu.ELEMENTS | gpu.LEARNED | gpu.DENSE)):
             _say(f"tet_96_vs_90 {mode}: {ea.shape[0]} x {eb.shape[0]} tets, materialised")
             r = ctx.supermesh(A, B, flags=flags)
             res[mode] = (r.npairs, r.nout, r.sum, ctx.checksum(r.counts, 4 * r.npairs),

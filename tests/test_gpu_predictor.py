@@ -1,6 +1,9 @@
 """The learned output-size predictor: MFMA inference and training kernels
 against a numpy restatement, and the single-pass (LEARNED) tet clip against
-the plain count-then-fill path -- identical outputs whatever the weights."""
+the plain count-then-fill path -- every pair's output elements identical
+whatever the weights, in the run layout (runs placed apart) and compacted
+(LIBSUPERMESH_GPU_DENSE, libsupermesh_gpu_compact: the plain arrays byte for
+byte)."""
 import numpy as np
 import pytest
 
@@ -9,6 +12,25 @@ import oracle_lib as O
 from libsupermesh_amd import gpu, meshgen
 
 pytestmark = pytest.mark.gpu
+
+
+def runs_end_to_end(out):
+    """Every pair's run (elements[offsets[p] .. offsets[p] + counts[p])), in
+    pair order: the plain layout's element array."""
+    offs, cnt, el = out["offsets"], out["counts"].astype(np.int64), out["elements"]
+    excl = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+    idx = np.repeat(offs[:-1] - excl, cnt) + np.arange(int(cnt.sum()))
+    return el[idx]
+
+
+def check_runs_apart(r, out):
+    """Runs lie inside the element array and do not overlap."""
+    offs, cnt = out["offsets"][:-1], out["counts"]
+    nz = cnt > 0
+    start, end = offs[nz], offs[nz] + cnt[nz]
+    assert np.all(start >= 0) and np.all(end <= r.elements_size)
+    order = np.argsort(start, kind="stable")
+    assert np.all(end[order][:-1] <= start[order][1:])
 
 
 @pytest.fixture(scope="module")
@@ -72,12 +94,14 @@ def test_training_learns_clip_counts(meshes):
 
 @pytest.mark.parametrize("weights", ["trained", "random", "zero"])
 @pytest.mark.parametrize("quantile", [0.5, 0.99])
-@pytest.mark.parametrize("arena", ["default", "tiny"])
+@pytest.mark.parametrize("arena", ["default", "spill", "tiny"])
 def test_learned_single_pass_is_identical(meshes, weights, quantile, arena):
     pa, ea, pb, eb = meshes
     ctx = gpu.Context()
-    if arena == "tiny":
-        ctx.predictor_arena(0.0, 1)   # one tet per lane: most extras are dropped and re-clipped
+    if arena == "spill":
+        ctx.predictor_arena(0.0, 1 << 20)   # a few arena tets per lane: some pairs are clipped again
+    elif arena == "tiny":
+        ctx.predictor_arena(0.0, 1)   # a one-tet spill reserve: the plain fill of every pair
     A, B = gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pb, eb)
     if weights == "trained":
         ctx.predictor_collect(A, B, 1 << 20)
@@ -90,17 +114,32 @@ def test_learned_single_pass_is_identical(meshes, weights, quantile, arena):
     what = ["indptr", "indices", "counts", "measure", "offsets", "elements"]
     r0 = ctx.supermesh(A, B, flags=gpu.ELEMENTS | gpu.SORTED)
     plain = gpu.fetch(r0, 3, what)
+    assert r0.elements_size == r0.nout
     n0 = (r0.npairs, r0.nout, r0.sum)
     r1 = ctx.supermesh(A, B, flags=gpu.ELEMENTS | gpu.SORTED | gpu.LEARNED)
     learned = gpu.fetch(r1, 3, what)
     assert (r1.npairs, r1.nout, r1.sum) == n0
-    for k in what:
+    for k in ("indptr", "indices", "counts", "measure"):
         assert np.array_equal(plain[k], learned[k]), k
+    refilled = r1.elements_size == r1.nout and np.array_equal(learned["offsets"], plain["offsets"])
+    if not refilled:
+        check_runs_apart(r1, learned)
+    assert np.array_equal(runs_end_to_end(learned), plain["elements"])
+    # compacted after the call, and in the call: the plain arrays byte for byte
+    ctx.compact(r1)
+    assert (r1.elements_size, r1.nout) == (r0.nout, r0.nout)
+    compact = gpu.fetch(r1, 3, ["offsets", "elements"])
+    r2 = ctx.supermesh(A, B, flags=gpu.ELEMENTS | gpu.SORTED | gpu.LEARNED | gpu.DENSE)
+    dense = gpu.fetch(r2, 3, ["offsets", "elements"])
+    for k in ("offsets", "elements"):
+        assert np.array_equal(compact[k], plain[k]), k
+        assert np.array_equal(dense[k], plain[k]), k
     if weights == "zero" and quantile == 0.99:
         # uniform prediction: the 0.99 quantile is the last class, budget = maximum
-        assert r1.learned_overflow == 0
-    if weights == "random" and arena == "tiny":
-        assert r1.learned_overflow > 0    # the fallback path was exercised
+        assert r1.learned_overflow == 0 and not refilled
+    if weights == "random":
+        assert r1.learned_overflow > 0    # outputs beyond their budgets: joined or re-clipped runs
+        assert refilled == (arena == "tiny")
     assert r1.learned_slots >= 0
     assert 0.0 < r1.learned_quantile <= quantile + 1e-6   # a float32 level
 
@@ -124,7 +163,7 @@ def test_learned_single_pass_with_mass_is_identical(meshes, weights):
     r0 = ctx.supermesh(A, B, flags=flags)
     plain = gpu.fetch(r0, 3, what)
     m0 = gpu.fetch_mass(r0, pa.shape[0])
-    r1 = ctx.supermesh(A, B, flags=flags | gpu.LEARNED)
+    r1 = ctx.supermesh(A, B, flags=flags | gpu.LEARNED | gpu.DENSE)
     learned = gpu.fetch(r1, 3, what)
     m1 = gpu.fetch_mass(r1, pa.shape[0])
     assert r1.learned_quantile > 0
