@@ -80,7 +80,7 @@ static void free_context(Context& C) {
                     &C.predictor.adam_m, &C.predictor.adam_v, &C.predictor.grad,
                     &C.predictor.features, &C.predictor.labels, &C.predictor.act,
                     &C.predictor.loss, &C.predictor.index, &C.predictor.budget,
-                    &C.predictor.overflow, &C.predictor.spill, &C.predictor.aoff, &C.offsets_dense,
+                    &C.predictor.overflow, &C.predictor.spill, &C.predictor.aoff, &C.offsets_dense, &C.queue,
                     &C.elements_dense, &C.fscratch};
   release_graph(C);
   for (DevBuf* b : bufs) b->release();

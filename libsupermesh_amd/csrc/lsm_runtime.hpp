@@ -139,6 +139,7 @@ struct Context {
   DevBuf counts, measure, partials, scalars, offsets, elements, blocks;
   DevBuf offsets_dense, elements_dense;  // LIBSUPERMESH_GPU_DENSE: the single pass's runs compacted
   DevBuf work, nwork;            // compacted list of pairs that need the full clip
+  DevBuf queue;                  // the tet walk's work-item counters, one per XCD share
   // mass assembly
   DevBuf mflag, mslot, mkeys, mvals, mkeys_alt, mvals_alt, ukeys, uvals, nuniq, mindptr, mindices;
   // 2D row slots: each (element, vertex) row's first slot

@@ -146,6 +146,60 @@ __device__ __forceinline__ void mass_element(const double* el, double meas, cons
   }
 }
 
+// 2D P1 x P1 mass of a clipped polygon from its moments, the polygon's fan
+// triangles accumulated in coordinates d = x - o local to A's first vertex o:
+// mom = {int 1, int dx, int dy, int dx^2, int dx dy, int dy^2}.  Per
+// triangle (area ar, vertices d_0..d_2; exact for quadratics):
+// int dx = ar/3 sum dx_v, int dx dy = ar/12 (sum dx_v dy_v + sum dx_v sum dy_v).
+__device__ __forceinline__ void tri_moments(double x0, double y0, double x1, double y1, double x2, double y2,
+                                            double ar, double ox, double oy, double* mom) {
+  const double a0 = x0 - ox, b0 = y0 - oy, a1 = x1 - ox, b1 = y1 - oy, a2 = x2 - ox, b2 = y2 - oy;
+  const double sa = a0 + a1 + a2, sb = b0 + b1 + b2;
+  const double w3 = ar * (1.0 / 3.0), w12 = ar * (1.0 / 12.0);
+  mom[0] += ar;
+  mom[1] += w3 * sa;
+  mom[2] += w3 * sb;
+  mom[3] += w12 * (a0 * a0 + a1 * a1 + a2 * a2 + sa * sa);
+  mom[4] += w12 * (a0 * b0 + a1 * b1 + a2 * b2 + sa * sb);
+  mom[5] += w12 * (b0 * b0 + b1 * b1 + b2 * b2 + sb * sb);
+}
+
+// blk[i][j] = int phi^A_i phi^B_j over the polygon of moments mom (origin o =
+// A's first vertex, where phi^A = (1, 0, 0)): with phi^A_i = alpha_i + g_i.d
+// and phi^B_j = beta_j + h_j.d, the integral is alpha_i beta_j I + alpha_i
+// h_j.Id + beta_j g_i.Id + g_i^T Idd h_j.  The same integral as summing
+// mass_element<2> over the fan, in exact arithmetic.
+__device__ __forceinline__ void mass_from_moments(const double* mom, const double* affa, const double* affb,
+                                                  double* blk) {
+  double g[3][2], h[3][2], beta[3];
+  g[1][0] = affa[2];
+  g[1][1] = affa[3];
+  g[2][0] = affa[4];
+  g[2][1] = affa[5];
+  g[0][0] = -(g[1][0] + g[2][0]);
+  g[0][1] = -(g[1][1] + g[2][1]);
+  h[1][0] = affb[2];
+  h[1][1] = affb[3];
+  h[2][0] = affb[4];
+  h[2][1] = affb[5];
+  h[0][0] = -(h[1][0] + h[2][0]);
+  h[0][1] = -(h[1][1] + h[2][1]);
+  const double o[2] = {affa[0], affa[1]};
+  basis<2>(affb, o, beta);
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    const double hd = h[j][0] * mom[1] + h[j][1] * mom[2];            // h_j . Id
+    const double tx = mom[3] * h[j][0] + mom[4] * h[j][1];            // Idd h_j
+    const double ty = mom[4] * h[j][0] + mom[5] * h[j][1];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      double v = beta[j] * (g[i][0] * mom[1] + g[i][1] * mom[2]) + (g[i][0] * tx + g[i][1] * ty);
+      if (i == 0) v += beta[j] * mom[0] + hd;  // alpha = (1, 0, 0)
+      blk[i * 3 + j] = v;
+    }
+  }
+}
+
 struct ClipArgs {
   long npairs;
   const long* npairs_dev;  // when set, the pair count is read here (a recorded graph's count pass)
@@ -182,6 +236,8 @@ struct ClipArgs {
   unsigned char* dropped;
   unsigned long long* ndropped;  // [0] split runs, [1] pairs clipped again
   const unsigned char* ovf;  // fill pass: when non-null, only items flagged 2 here
+  // tet walks: one work-item counter per XCD share (zeroed before each launch)
+  unsigned long long* queue;
   // 2D: pairs whose working polygon outgrew its LDS slots, for the
   // streaming clipper
   int* olist;
@@ -540,15 +596,12 @@ __global__ void __launch_bounds__(BLOCK) k_clip_tri(ClipArgs g) {
       for (int k = 0; k < 6; k++) Bv[k] = g.bcoords[b * 6 + k];
       const double tol = 10.0 * fmin_f(g.asp[a], g.bsp[b]);
       double m = 0.0;
-      double affa[6], affb[6];
+      // mass: the polygon's moments about A's first vertex (the origin of
+      // its barycentric map), turned into the block once the fan is done
+      double mom[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, ox = 0.0, oy = 0.0;
       if (MASS) {
-#pragma unroll
-        for (int k = 0; k < 9; k++) blk[k] = 0.0;
-#pragma unroll
-        for (int k = 0; k < 6; k++) {
-          affa[k] = g.aaff[a * 6 + k];
-          affb[k] = g.baff[b * 6 + k];
-        }
+        ox = g.aaff[a * 6];
+        oy = g.aaff[a * 6 + 1];
       }
       Line L[3];
       tri_lines(Bv, L);
@@ -556,10 +609,7 @@ __global__ void __launch_bounds__(BLOCK) k_clip_tri(ClipArgs g) {
       const int nk = clip_poly_lds<3, 3, BLOCK>(
           A, L, tol, q, [&](double x0, double y0, double x1, double y1, double x2, double y2, double ar) {
             m = m + ar;
-            if (MASS) {
-              const double el[6] = {x0, y0, x1, y1, x2, y2};
-              mass_element<2>(el, ar, affa, affb, blk);
-            }
+            if (MASS) tri_moments(x0, y0, x1, y1, x2, y2, ar, ox, oy, mom);
             if (FILL && k < cap) {
               double* o = g.elements + (off + k) * 6;
               o[0] = x0;
@@ -580,6 +630,15 @@ __global__ void __launch_bounds__(BLOCK) k_clip_tri(ClipArgs g) {
       g.measure[p] = nk == 0 ? 0.0 : m;
       if (MASS) {
         keep = nk > 0;
+        if (keep) {
+          double affa[6], affb[6];
+#pragma unroll
+          for (int i = 0; i < 6; i++) {
+            affa[i] = g.aaff[a * 6 + i];
+            affb[i] = g.baff[b * 6 + i];
+          }
+          mass_from_moments(mom, affa, affb, blk);
+        }
         if (g.rkey) {
           rows = true;
           const long r = (p - g.indptr[a]) * 3;
@@ -660,10 +719,55 @@ __device__ __forceinline__ void xcd_share(long n, long& w, long& end, long& stri
   w = begin + lb * blockDim.x + threadIdx.x;
 }
 
+// The walk's lanes take work items from queues (counters handing out
+// kQueueBatch consecutive items at a time): each XCD's share of the work list
+// is cut into kSubQueues contiguous parts, each with its own counter, so the
+// items in flight on an XCD stay a few compact windows of the list whose A
+// and B data its L2 holds (with fixed strides the lanes drift apart over the
+// whole share), and no counter takes more than a fraction of the atomics.
+constexpr int kQueueBatch = 16;
+constexpr int kSubQueues = 4;
+constexpr int kQueues = kXcds * kSubQueues;
+struct WalkQueue {
+  unsigned long long* counter;
+  long begin, end;  // this queue's part of the work list
+  long w = 0, wend = 0;
+  __device__ __forceinline__ bool next() {  // advance to the next item; false when none is left
+    if (++w < wend) return true;
+    const long s = begin + (long)atomicAdd(counter, (unsigned long long)kQueueBatch);
+    if (s >= end) return false;
+    w = s;
+    wend = s + kQueueBatch < end ? s + kQueueBatch : end;
+    return true;
+  }
+};
+
+__device__ __forceinline__ WalkQueue walk_queue(unsigned long long* queues, long n) {
+  const int nx = gridDim.x % kXcds == 0 ? kXcds : 1;
+  // every part needs blocks to drain it: a grid with fewer blocks per share
+  // than parts keeps one part per share
+  const int nsub = (int)(gridDim.x / nx) >= kSubQueues ? kSubQueues : 1;
+  const long xcd = blockIdx.x % nx, sub = (blockIdx.x / nx) % nsub;
+  const long per = (n + nx - 1) / nx;
+  const long b0 = xcd * per, e0 = b0 + per < n ? b0 + per : n;
+  const long sper = (e0 - b0 + nsub - 1) / nsub;
+  WalkQueue q;
+  q.counter = queues + xcd * kSubQueues + sub;
+  q.begin = b0 + sub * sper < e0 ? b0 + sub * sper : e0;
+  q.end = q.begin + sper < e0 ? q.begin + sper : e0;
+  q.w = -1;
+  q.wend = 0;
+  return q;
+}
+
+__global__ void k_zero_queues(unsigned long long* __restrict__ q) {
+  if (threadIdx.x < kQueues && blockIdx.x == 0) q[threadIdx.x] = 0;
+}
+
 // MODES: count (FILL and LEARNED false): counts, measures (and mass blocks);
 // fill (FILL): writes each pair's counts[p] outputs at offsets[p];
 // single pass (LEARNED): the count pass that also writes up to budget[w]
-// outputs of work item w at slots[w] of the slot buffer.
+// outputs of work item w at offsets[p] and the rest to the lane's arena.
 // BLOCK and REG (tet depths held in registers, 0 or 1; the other depths of
 // 1..3 in LDS) are launch shapes the count pass can be tuned over.
 template <bool MASS, bool FILL, bool LEARNED = false, int BLOCK = kWalkBlock, int REG = kWalkReg>
@@ -675,8 +779,8 @@ k_clip_tet_walk(ClipArgs g) {
   double R[REG > 0 ? 12 * REG : 1];
   // blocks go to the XCDs round robin: each XCD takes one contiguous share
   // of the work list, so that its L2 holds one region of A and B
-  long w, n, stride;
-  xcd_share((long)*g.nwork, w, n, stride);
+  WalkQueue Q = walk_queue(g.queue, (long)*g.nwork);
+  bool more = Q.next();
   long p = 0, off = 0;
   int cap = 0, k = 0;
   // single pass: this lane's arena region, its fill level, and whether the
@@ -702,14 +806,15 @@ k_clip_tet_walk(ClipArgs g) {
   while (true) {
     if (!active) {
       // next work item of this lane (fill pass: skip pairs with no output)
-      while (w < n) {
-        p = g.work[w];
+      while (more) {
+        p = g.work[Q.w];
         if (!FILL) break;
         cap = g.counts[p];
-        if (cap > 0 && (!g.ovf || g.ovf[w] == 2)) break;
-        w += stride;
+        if (cap > 0 && (!g.ovf || g.ovf[Q.w] == 2)) break;
+        more = Q.next();
       }
-      if (w >= n) break;
+      if (!more) break;
+      const long w = Q.w;
       if (FILL) off = g.offsets[p];
       if (LEARNED) {
         cap = (int)((g.budget[w] >> g.bshift) & 0xffu);
@@ -768,9 +873,9 @@ k_clip_tet_walk(ClipArgs g) {
     }
     if (LEARNED) {
       const bool split = k > cap && !lost;
-      g.aoff[w] = split ? abase + atop : -1;
+      g.aoff[Q.w] = split ? abase + atop : -1;
       if (split) atop += k - cap;
-      g.dropped[w] = lost ? 2 : (split ? 1 : 0);
+      g.dropped[Q.w] = lost ? 2 : (split ? 1 : 0);
       if (split) atomicAdd(g.ndropped, 1ULL);
       if (lost) atomicAdd(g.ndropped + 1, 1ULL);
     }
@@ -782,7 +887,7 @@ k_clip_tet_walk(ClipArgs g) {
         for (int i = 0; i < 16; i++) g.blocks[p * 16 + i] = k == 0 ? 0.0 : blk[i];
       }
     }
-    w += stride;
+    more = Q.next();
     active = false;
   }
 }
@@ -1343,7 +1448,16 @@ static unsigned tet_walk_blocks(Context& C, int which) {
 
 static long tet_walk_lanes(Context& C, int which) { return (long)tet_walk_blocks(C, which) * kWalkBlock; }
 
-static void launch_tet_walk(Context& C, const ClipArgs& args, int which) {
+// The walk's per-XCD work queues, zeroed by a kernel before each launch.
+static unsigned long long* walk_queues(Context& C) {
+  unsigned long long* q = C.queue.as<unsigned long long>(kQueues);
+  hipLaunchKernelGGL(k_zero_queues, dim3(1), dim3(64), 0, C.stream, q);
+  return q;
+}
+
+static void launch_tet_walk(Context& C, const ClipArgs& a, int which) {
+  ClipArgs args = a;
+  args.queue = walk_queues(C);
   dim3 grid(tet_walk_blocks(C, which)), block(kWalkBlock);
   switch (which) {
     case 0: hipLaunchKernelGGL((k_clip_tet_walk<false, false>), grid, block, 0, C.stream, args); break;
@@ -1780,7 +1894,9 @@ static void walk_count_as(Context& C, const ClipArgs& g, int grid) {
   hipLaunchKernelGGL((k_clip_tet_walk<false, false, false, BLOCK, REG>), dim3(grid), dim3(BLOCK), 0, C.stream, g);
 }
 
-static void launch_count_walk(Context& C, const ClipArgs& g, const CountShape& s) {
+static void launch_count_walk(Context& C, const ClipArgs& a, const CountShape& s) {
+  ClipArgs g = a;
+  g.queue = walk_queues(C);
   switch (s.block * 2 + s.reg) {
     case 128: walk_count_as<64, 0>(C, g, s.grid); break;
     case 129: walk_count_as<64, 1>(C, g, s.grid); break;
@@ -1965,6 +2081,7 @@ void count_pass_graph(Context& C, const DevMesh& A, const DevMesh& B, long npair
   C.mflag.as<int>(cap);
   C.work.as<int>(cap);
   C.nwork.as<long>(2 * kClipGrid + 2);
+  C.queue.as<unsigned long long>(kQueues);
   unsigned long long* scal = C.scalars.as<unsigned long long>(16);
   if (ngroups > 0) {
     C.gpairs.as<long>((size_t)ngroups + 1);
@@ -1984,7 +2101,7 @@ void count_pass_graph(Context& C, const DevMesh& A, const DevMesh& B, long npair
       reinterpret_cast<const void*>(ngroups), reinterpret_cast<const void*>((long)(s.block * 4 + s.reg)),
       reinterpret_cast<const void*>((long)s.grid), g.npairs_dev, g.rows, g.cols, g.acoords, g.bcoords, g.asp,
       g.bsp, g.ascale, g.bscale, g.bplanes, g.counts, g.measure, parts, C.mflag.ptr, C.work.ptr, C.nwork.ptr,
-      scal, a_off, out->indptr, C.gpairs.ptr, C.gsum.ptr};
+      C.queue.ptr, scal, a_off, out->indptr, C.gpairs.ptr, C.gsum.ptr};
   const int d = dim - 2;
   if (C.graph_exec[d] && key != C.graph_key[d]) release_graph(C, d);
   if (!C.graph_exec[d] && !C.graph_refused &&
