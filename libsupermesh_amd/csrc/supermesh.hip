@@ -689,7 +689,7 @@ __global__ void __launch_bounds__(BLOCK) k_clip_tri(ClipArgs g) {
 }
 
 // Tet pairs on the compacted work list, one pair per lane at a time.  Each
-// lane walks its pair's clip tree with walk_step (every step builds one child
+// lane walks its pair's clip tree with walk_next (every step builds one child
 // and classifies or measures it) and, when its pair is done, moves on to its
 // next work item by itself, so lanes never wait for the slowest pair of the
 // wave -- only for the slowest chain of work items, which evens out.
@@ -705,19 +705,10 @@ constexpr int kWalkBlock = 128;
 constexpr int kWalkReg = 1;
 
 // The CDNA4 dispatcher hands consecutive workgroups to its 8 XCDs (each with
-// its own L2) in turn.  A grid whose size is a multiple of 8 can give XCD x
-// the items [x * per, (x + 1) * per) of n: this lane starts at w, stops at
-// end and steps by stride.
+// its own L2) in turn: with a grid whose size is a multiple of 8, block b
+// runs on XCD b % 8, which takes the share [x * per, (x + 1) * per) of the
+// work list.
 constexpr int kXcds = 8;
-__device__ __forceinline__ void xcd_share(long n, long& w, long& end, long& stride) {
-  const int nx = gridDim.x % kXcds == 0 ? kXcds : 1;
-  const long xcd = blockIdx.x % nx, lb = blockIdx.x / nx;
-  const long per = (n + nx - 1) / nx;
-  const long begin = xcd * per;
-  end = begin + per < n ? begin + per : n;
-  stride = (long)(gridDim.x / nx) * blockDim.x;
-  w = begin + lb * blockDim.x + threadIdx.x;
-}
 
 // The walk's lanes take work items from queues (counters handing out
 // kQueueBatch consecutive items at a time): each XCD's share of the work list
@@ -762,6 +753,13 @@ __device__ __forceinline__ WalkQueue walk_queue(unsigned long long* queues, long
 
 __global__ void k_zero_queues(unsigned long long* __restrict__ q) {
   if (threadIdx.x < kQueues && blockIdx.x == 0) q[threadIdx.x] = 0;
+}
+
+// The walks' work queues, zeroed by a kernel before each launch.
+static unsigned long long* walk_queues(Context& C) {
+  unsigned long long* q = C.queue.as<unsigned long long>(kQueues);
+  hipLaunchKernelGGL(k_zero_queues, dim3(1), dim3(64), 0, C.stream, q);
+  return q;
 }
 
 // MODES: count (FILL and LEARNED false): counts, measures (and mass blocks);
@@ -1069,6 +1067,7 @@ struct HexWalkArgs {
   double* measure;
   const long* offsets;
   double* elements;
+  unsigned long long* queue;  // the walk's work-item counters (zeroed before each launch)
 };
 
 // One wave per block; the walk keeps its depth-1 and depth-2 tets in
@@ -1082,9 +1081,10 @@ k_clip_hex_walk(HexWalkArgs g) {
   __shared__ double stack[(5 - kHexReg) * 12 * kHexBlock];
   double* stk = stack + threadIdx.x;
   double R[12 * kHexReg];
-  long p, n, stride;
-  xcd_share(g.npairs, p, n, stride);
-  long off = 0, a = 0;
+  // pairs from the walk's queues, as k_clip_tet_walk takes its work items
+  WalkQueue Q = walk_queue(g.queue, g.npairs);
+  bool more = Q.next();
+  long p = 0, off = 0, a = 0;
   int cap = 0, k = 0, sub = 0;
   double T[12], tol = 0.0, m = 0.0;
   const double* planes = nullptr;
@@ -1103,13 +1103,14 @@ k_clip_hex_walk(HexWalkArgs g) {
   };
   while (true) {
     if (!active) {
-      while (p < n) {
+      while (more) {
+        p = Q.w;
         if (!FILL) break;
         cap = g.counts[p];
         if (cap > 0) break;
-        p += stride;
+        more = Q.next();
       }
-      if (p >= n) break;
+      if (!more) break;
       if (FILL) off = g.offsets[p];
       a = g.rows[p];
       const long b = g.cols[p];
@@ -1154,7 +1155,7 @@ k_clip_hex_walk(HexWalkArgs g) {
       g.counts[p] = k;
       g.measure[p] = k == 0 ? 0.0 : m;
     }
-    p += stride;
+    more = Q.next();
     active = false;
   }
 }
@@ -1207,8 +1208,10 @@ static void launch_hex_walk_as(Context& C, const HexWalkArgs& g) {
   const long lanes = (long)blocks_per_cu * (cus > 0 ? cus : 256) * kHexBlock;
   const long need = (g.npairs + kHexBlock - 1) / kHexBlock;
   unsigned blocks = (unsigned)std::max(1L, std::min(lanes / kHexBlock, need));
-  if (blocks >= kXcds) blocks -= blocks % kXcds;  // whole XCD shares (xcd_share)
-  hipLaunchKernelGGL((k_clip_hex_walk<MODE, FILL>), dim3(blocks), dim3(kHexBlock), 0, C.stream, g);
+  if (blocks >= kXcds) blocks -= blocks % kXcds;  // whole XCD shares (walk_queue)
+  HexWalkArgs h = g;
+  h.queue = walk_queues(C);
+  hipLaunchKernelGGL((k_clip_hex_walk<MODE, FILL>), dim3(blocks), dim3(kHexBlock), 0, C.stream, h);
   LSM_HIP(hipGetLastError());
 }
 
@@ -1448,12 +1451,6 @@ static unsigned tet_walk_blocks(Context& C, int which) {
 
 static long tet_walk_lanes(Context& C, int which) { return (long)tet_walk_blocks(C, which) * kWalkBlock; }
 
-// The walk's per-XCD work queues, zeroed by a kernel before each launch.
-static unsigned long long* walk_queues(Context& C) {
-  unsigned long long* q = C.queue.as<unsigned long long>(kQueues);
-  hipLaunchKernelGGL(k_zero_queues, dim3(1), dim3(64), 0, C.stream, q);
-  return q;
-}
 
 static void launch_tet_walk(Context& C, const ClipArgs& a, int which) {
   ClipArgs args = a;
