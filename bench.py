@@ -40,8 +40,8 @@ BYTES_PER_PAIR = {"tri_1024_vs_1000": 60 + 4 + 4 + 8 + 72, "tet_64_vs_60": 112 +
 
 # HBM bytes per launch of the same kernel on the same workload, from the two
 # counter runs summarised in profiles/traffic_tri1024_tet64.md (FETCH_SIZE x
-# 1.9997 + WRITE_SIZE x 1.0, the factors a 256 MiB copy calibrates).
-PROFILED_TRAFFIC = {"tri_1024_vs_1000": 5.37e9, "tet_64_vs_60": 31.48e9}
+# 1.9992 + WRITE_SIZE x 1.0, the factors a 256 MiB copy calibrates).
+PROFILED_TRAFFIC = {"tri_1024_vs_1000": 3.63e9, "tet_64_vs_60": 32.25e9}
 
 
 def parse():
