@@ -246,7 +246,7 @@ def batched_workload(ctx, steps, warmup, count=4096):
     }
 
 
-def learned_workload(ctx, name, steps, warmup, weights, quantiles=(0.7, 0.9, 0.99)):
+def learned_workload(ctx, name, steps, warmup, weights, quantiles=(0.7, 0.8, 0.85, 0.9, 0.99)):
     """Materialised tet supermesh (LIBSUPERMESH_GPU_ELEMENTS): plain
     count-then-fill against the single pass sized by the learned predictor
     (LIBSUPERMESH_GPU_LEARNED), which leaves each pair's run where its budget

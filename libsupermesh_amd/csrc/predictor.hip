@@ -70,6 +70,19 @@ constexpr int kLd = 64;  // row stride (elements) of activation tiles in LDS
 
 __device__ __forceinline__ int lane64() { return threadIdx.x & 63; }
 
+// The value lane l ^ 32 holds (the other half of the wave), by a VALU swap
+// of the halves instead of an LDS permute.
+__device__ __forceinline__ float other_half(float v) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+  return __uint_as_float(lane64() < 32 ? r[1] : r[0]);
+}
+
+__device__ __forceinline__ double other_half(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const float lo = other_half(__uint_as_float((unsigned)b)), hi = other_half(__uint_as_float((unsigned)(b >> 32)));
+  return __longlong_as_double((long long)(((unsigned long long)__float_as_uint(hi) << 32) | __float_as_uint(lo)));
+}
+
 // Row of accumulator element r in a 32x32 MFMA result (column = lane & 31).
 __device__ __forceinline__ int acc_row(int r) { return 8 * (r >> 2) + 4 * (lane64() >> 5) + (r & 3); }
 
@@ -126,12 +139,13 @@ __device__ __forceinline__ void pair_features(const double* __restrict__ ta, con
   for (int k = 24; k < 32; k++) row[k] = (__bf16)0.f;
 }
 
-// Budgets at four quantile levels (highest first), packed one per byte: for
-// each level, the smallest count whose cumulative predicted probability
-// reaches it, or the clip's maximum (81) when only the last class ("32 or
-// more") does.  Also returns 1024 x the expected count.
-__device__ __forceinline__ unsigned budgets_of(const float* logit, const float* levels, unsigned& expect) {
-  // unrolled throughout: the logits stay in registers
+// Budgets at two quantile levels, one per byte (bits 0-7, 8-15): for each
+// level, the smallest count whose cumulative predicted probability reaches
+// it, or the clip's maximum (81) when only the last class ("32 or more")
+// does; also 1024 x the expected count.  The cumulative probabilities only
+// grow with the count, so that smallest count is the number of counts below
+// 32 whose cumulative probability stays under the level (all 32: 81).
+__device__ __forceinline__ unsigned budgets_pair(const float* logit, const float* levels, unsigned& expect) {
   float m = logit[0];
 #pragma unroll
   for (int c = 1; c < 33; c++) m = fmaxf(m, logit[c]);
@@ -142,24 +156,21 @@ __device__ __forceinline__ unsigned budgets_of(const float* logit, const float* 
     ev[c] = __expf(logit[c] - m);
     s += ev[c];
   }
-  float t[4];
-#pragma unroll
-  for (int k = 0; k < 4; k++) t[k] = levels[k] * s;
-  unsigned b[4] = {(unsigned)kTetBufSize, (unsigned)kTetBufSize, (unsigned)kTetBufSize, (unsigned)kTetBufSize};
+  const float t0 = levels[0] * s, t1 = levels[1] * s;
   float acc = 0.f, ex = 0.f;
+  unsigned n0 = 0, n1 = 0;
 #pragma unroll
   for (int c = 0; c < 33; c++) {
-    const float e = ev[c];
-    acc += e;
-    ex += (float)c * e;
+    acc += ev[c];
+    ex += (float)c * ev[c];
     if (c < 32) {
-#pragma unroll
-      for (int k = 0; k < 4; k++)
-        if (b[k] == (unsigned)kTetBufSize && acc >= t[k]) b[k] = (unsigned)c;
+      n0 += acc < t0 ? 1u : 0u;
+      n1 += acc < t1 ? 1u : 0u;
     }
   }
   expect = (unsigned)(1024.f * ex / s);
-  return b[0] | (b[1] << 8) | (b[2] << 16) | (b[3] << 24);
+  const unsigned b0 = n0 == 32 ? (unsigned)kTetBufSize : n0, b1 = n1 == 32 ? (unsigned)kTetBufSize : n1;
+  return b0 | (b1 << 8);
 }
 
 void budget_levels(float q, float* levels) {
@@ -262,8 +273,8 @@ __global__ void __launch_bounds__(64 * kPredWaves) __attribute__((amdgpu_waves_p
       for (int d = 0; d < 3; d++) {
         lo[d] = fmin_f(fmin_f(T[d], T[3 + d]), fmin_f(T[6 + d], T[9 + d]));
         hi[d] = fmax_f(fmax_f(T[d], T[3 + d]), fmax_f(T[6 + d], T[9 + d]));
-        lo[d] = fmin_f(lo[d], __shfl_xor(lo[d], 32, 64));
-        hi[d] = fmax_f(hi[d], __shfl_xor(hi[d], 32, 64));
+        lo[d] = fmin_f(lo[d], other_half(lo[d]));
+        hi[d] = fmax_f(hi[d], other_half(hi[d]));
         sc[d] = hi[d] > lo[d] ? 1.0 / (hi[d] - lo[d]) : 0.0;
       }
 #pragma unroll
@@ -288,17 +299,21 @@ __global__ void __launch_bounds__(64 * kPredWaves) __attribute__((amdgpu_waves_p
     float L[33];
 #pragma unroll
     for (int r = 0; r < 16; r++) {
-      const float mine = acc[0][r], other = __shfl_xor(mine, 32, 64);
+      const float mine = acc[0][r], other = other_half(mine);
       // class 8 (r / 4) + 4 hh + r % 4 comes from the lane of half hh
       L[8 * (r >> 2) + (r & 3)] = half == 0 ? mine : other;
       L[8 * (r >> 2) + 4 + (r & 3)] = half == 0 ? other : mine;
     }
-    L[32] = __shfl(acc[1][0], c, 64);  // unit 32 = tile 1, row 0: half 0's element 0
+    L[32] = half == 0 ? acc[1][0] : other_half(acc[1][0]);  // unit 32 = tile 1, row 0: half 0's element 0
 #pragma unroll
     for (int k = 0; k < 33; k++) L[k] += sB[192 + k];
     if (valid) {
+      // the two halves' lanes hold the same pair: each finds two of the four
+      // budget levels (half 0 levels 0 and 1, half 1 levels 2 and 3)
       unsigned ex = 0;
-      const unsigned bud = budgets_of(L, g.levels, ex);
+      const unsigned two = budgets_pair(L, g.levels + 2 * half, ex);
+      const unsigned other = __float_as_uint(other_half(__uint_as_float(two)));
+      const unsigned bud = half == 0 ? (two | (other << 16)) : 0u;
       if (half == 0) {
         g.budget[w] = bud;
 #pragma unroll
