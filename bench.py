@@ -301,6 +301,8 @@ def learned_workload(ctx, name, steps, warmup, weights, quantiles=(0.7, 0.8, 0.8
             if flags & gpu.DENSE:
                 level = r.learned_quantile
                 slots, ovf, size = int(r.learned_slots), int(r.learned_overflow), int(r.elements_size)
+                out[mode]["dense_route"] = {1: "moved in place", 2: "copied", 3: "filled again"}.get(
+                    int(r.dense_route), "plain path")
             out[mode].update({"quantile_requested": q, "quantile_used": round(float(level), 6),
                               "predictor_ms": sum(pred) / steps, "slots": slots, "overflow_pairs": ovf,
                               "elements_array": size})

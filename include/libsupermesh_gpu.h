@@ -60,7 +60,8 @@ extern "C" {
 #define LIBSUPERMESH_GPU_DENSE 64     /* with LEARNED: compact the runs into the
                                          plain path's layout (offsets = prefix
                                          sums of the counts), byte for byte the
-                                         plain path's arrays */
+                                         plain path's arrays; moved within the
+                                         runs' own buffer (no second copy) */
 #define LIBSUPERMESH_GPU_GRAPH 128    /* the count pass (counts, measures, sums;
                                          not with ELEMENTS, MASS or LEARNED)
                                          of a simplex supermesh or batch in
@@ -132,6 +133,11 @@ typedef struct {
   int count_graph;           /* LIBSUPERMESH_GPU_GRAPH: 1 when the count pass ran
                                 as the recorded graph (0: eagerly, in the same
                                 launch shape) */
+  int dense_route;           /* LIBSUPERMESH_GPU_LEARNED | LIBSUPERMESH_GPU_DENSE:
+                                how the plain layout was made: 1 the runs moved
+                                within their buffer, 2 copied to a second one
+                                (some pair's plain start lay beyond its run's),
+                                3 every pair filled again (0 otherwise) */
 } libsupermesh_gpu_result;
 
 /* Context on HIP device `device` with its own stream (default priority). */

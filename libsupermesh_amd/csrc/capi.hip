@@ -81,7 +81,7 @@ static void free_context(Context& C) {
                     &C.predictor.features, &C.predictor.labels, &C.predictor.act,
                     &C.predictor.loss, &C.predictor.index, &C.predictor.budget,
                     &C.predictor.overflow, &C.predictor.spill, &C.predictor.aoff, &C.offsets_dense, &C.queue,
-                    &C.elements_dense, &C.fscratch};
+                    &C.elements_dense, &C.dense_ctl, &C.fscratch};
   release_graph(C);
   for (DevBuf* b : bufs) b->release();
   if (C.hstat) hipHostFree(C.hstat);
@@ -715,7 +715,10 @@ int libsupermesh_gpu_compact(libsupermesh_gpu_context* ctx, libsupermesh_gpu_res
     Context& C = ctx->ctx;
     std::lock_guard<std::mutex> lk(C.mu);
     LSM_HIP(hipSetDevice(C.device));
-    if (result->elements != nullptr && result->elements == C.elements_dense.ptr) return;  // already compact
+    LSM_REQUIRE(result->elements != nullptr, LIBSUPERMESH_GPU_ERROR_STATE,
+                "libsupermesh_gpu_compact needs a result with ELEMENTS");
+    if (!C.runs_layout || result->elements == C.elements_dense.ptr || result->offsets == C.offsets_dense.ptr)
+      return;  // the plain layout already
     LSM_REQUIRE(result->offsets != nullptr && result->offsets == C.offsets.ptr &&
                     result->elements == C.elements.ptr && result->counts == C.counts.ptr,
                 LIBSUPERMESH_GPU_ERROR_STATE,

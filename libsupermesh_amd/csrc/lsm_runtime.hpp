@@ -138,6 +138,8 @@ struct Context {
   // clipping
   DevBuf counts, measure, partials, scalars, offsets, elements, blocks;
   DevBuf offsets_dense, elements_dense;  // LIBSUPERMESH_GPU_DENSE: the single pass's runs compacted
+  DevBuf dense_ctl;              // the dense move's fit flag and step cursor
+  bool runs_layout = false;      // the latest result holds the single pass's runs apart
   DevBuf work, nwork;            // compacted list of pairs that need the full clip
   DevBuf queue;                  // the tet walk's work-item counters, one per XCD share
   // mass assembly

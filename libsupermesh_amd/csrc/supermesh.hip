@@ -235,7 +235,9 @@ struct ClipArgs {
   long* aoff;
   unsigned char* dropped;
   unsigned long long* ndropped;  // [0] split runs, [1] pairs clipped again
-  const unsigned char* ovf;  // fill pass: when non-null, only items flagged 2 here
+  // fill pass: when non-null, only items flagged 2 here, each written at
+  // aoff[w] (its place in the spill reserve)
+  const unsigned char* ovf;
   // tet walks: one work-item counter per XCD share (zeroed before each launch)
   unsigned long long* queue;
   // 2D: pairs whose working polygon outgrew its LDS slots, for the
@@ -813,7 +815,7 @@ k_clip_tet_walk(ClipArgs g) {
       }
       if (!more) break;
       const long w = Q.w;
-      if (FILL) off = g.offsets[p];
+      if (FILL) off = g.ovf ? g.aoff[w] : g.offsets[p];
       if (LEARNED) {
         cap = (int)((g.budget[w] >> g.bshift) & 0xffu);
         off = g.offsets[p];
@@ -1484,23 +1486,27 @@ __global__ void __launch_bounds__(256) k_fill_long(long n, long v, long* __restr
   if (i < n) out[i] = v;
 }
 
-// Single pass, pairs whose run is split or was dropped: their output lengths
-// (0 for the others), and then the dropped ones' runs placed from `base`.
+// Single pass, pairs flagged `from` or above in dropped (1: split runs and
+// dropped pairs, 2: dropped pairs only): their output lengths (0 for the
+// others), and then the dropped ones' runs placed from `base` (aoff[w], and
+// offsets[p] when set_offsets).
 __global__ void __launch_bounds__(256)
 k_dropped_lengths(long nwork, const int* __restrict__ work, const unsigned char* __restrict__ dropped,
-                  const int* __restrict__ counts, long* __restrict__ len) {
+                  unsigned char from, const int* __restrict__ counts, long* __restrict__ len) {
   const long w = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (w < nwork) len[w] = dropped[w] ? counts[work[w]] : 0;
+  if (w < nwork) len[w] = dropped[w] >= from ? counts[work[w]] : 0;
 }
 
 __global__ void __launch_bounds__(256)
 k_dropped_offsets(long nwork, const int* __restrict__ work, const unsigned char* __restrict__ dropped,
                   const long* __restrict__ incl, const int* __restrict__ counts, long base,
-                  long* __restrict__ offsets) {
+                  long* __restrict__ aoff, long* __restrict__ offsets, bool set_offsets) {
   const long w = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (w < nwork && dropped[w] == 2) {
     const long p = work[w];
-    offsets[p] = base + incl[w] - counts[p];
+    const long o = base + incl[w] - counts[p];
+    aoff[w] = o;
+    if (set_offsets) offsets[p] = o;
   }
 }
 
@@ -1587,6 +1593,119 @@ void compact_runs(Context& C, int dim, long npairs, const int* counts, const lon
   LSM_HIP(hipGetLastError());
 }
 
+// The dense layout straight from a single pass (LIBSUPERMESH_GPU_DENSE),
+// moved inside its own buffer.  Pair p's run starts at roff[p] of `src` (the
+// budgets laid end to end, roff[npairs] = the slots; src lies `gap` elements
+// into the buffer) and goes to doff[p] of `dst` (the buffer's start; doff =
+// the prefix sums of the counts).  When doff[p] <= gap + roff[p] for every p
+// in [0, npairs], no run lands beyond its own start, and the runs move in
+// steps from the first pair p0 not yet moved: every run up to p1, the last
+// pair with doff[p1] <= gap + roff[p0], lands below every run not yet read
+// (runs lie in pair order, the arena and spill reserve above them all), so
+// those runs move at once; when not even p0 + 1 qualifies, p0's run moves
+// alone, one wave copying it forward (a word is written only after the
+// source words below it have been read).  A run that outgrew its budget
+// moves its budget's part here and the rest afterwards (k_dense_overflow).
+
+// bad = 1 when some p has doff[p] > gap + roff[p].
+__global__ void __launch_bounds__(256)
+k_dense_fits(long n, const long* __restrict__ roff, long gap, const long* __restrict__ doff,
+             unsigned long long* __restrict__ bad) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n && doff[p] > gap + roff[p]) atomicOr(bad, 1ULL);
+}
+
+// One step (cursor[step & 1] = p0; block 0 leaves p1 in cursor[(step+1) & 1]),
+// or every pair at once into a separate buffer (cursor null).  src and dst
+// may share memory: no __restrict__ on them.
+template <int EW>
+__global__ void __launch_bounds__(kClipBlock)
+k_dense_step(long npairs, const int* __restrict__ counts, const long* __restrict__ roff, long gap,
+             const long* __restrict__ doff, long* __restrict__ cursor, int step, const double* src, double* dst) {
+  static_assert(EW % 2 == 0, "whole 16-byte words per element");
+  __shared__ long range[2];
+  if (threadIdx.x == 0) {
+    long p0 = 0, p1 = npairs;
+    if (cursor) {
+      p0 = cursor[step & 1];
+      p1 = p0;
+      if (p0 < npairs) {
+        const long bound = gap + roff[p0];
+        p1 = p0 + 1;
+        if (doff[p1] <= bound) {
+          long hi = npairs;  // the last q in [p0 + 1, npairs] with doff[q] <= bound
+          while (p1 < hi) {
+            const long mid = (p1 + hi + 1) >> 1;
+            if (doff[mid] <= bound) p1 = mid;
+            else hi = mid - 1;
+          }
+        }
+      }
+      if (blockIdx.x == 0) cursor[(step + 1) & 1] = p1;
+    }
+    range[0] = p0;
+    range[1] = p1;
+  }
+  __syncthreads();
+  const long p0 = range[0], p1 = range[1];
+  const int lane = threadIdx.x & 63;
+  const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long nwaves = ((long)gridDim.x * blockDim.x) >> 6;
+  for (long b0 = p0 + wave * 64; b0 < p1; b0 += nwaves * 64) {
+    const long p = b0 + lane;
+    long so = 0, dof = 0;
+    int n = 0;
+    if (p < p1) {
+      const int c = counts[p];
+      if (c > 0) {
+        so = roff[p];
+        const long b = roff[p + 1] - so;
+        n = c < b ? c : (int)b;
+        dof = doff[p];
+      }
+    }
+    for (int k = 0; k < 64; k++) {
+      const int m = __shfl(n, k, 64);
+      if (m == 0) continue;
+      const double2* s2 = reinterpret_cast<const double2*>(src + __shfl(so, k, 64) * EW);
+      double2* d2 = reinterpret_cast<double2*>(dst + __shfl(dof, k, 64) * EW);
+      for (int i = lane; i < (EW / 2) * m; i += 64) d2[i] = s2[i];
+    }
+  }
+}
+
+// Then the runs that outgrew their budget: a split run (dropped 1) its arena
+// part, behind its budget's part; a pair clipped again (dropped 2) its whole
+// run from the spill reserve; both at aoff[w] of src.
+template <int EW>
+__global__ void __launch_bounds__(kClipBlock)
+k_dense_overflow(long nwork, const int* __restrict__ work, const unsigned char* __restrict__ dropped,
+                 const int* __restrict__ counts, const long* __restrict__ roff, const long* __restrict__ aoff,
+                 const long* __restrict__ doff, const double* __restrict__ src, double* __restrict__ dst) {
+  const int lane = threadIdx.x & 63;
+  const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long nwaves = ((long)gridDim.x * blockDim.x) >> 6;
+  for (long b0 = wave * 64; b0 < nwork; b0 += nwaves * 64) {
+    const long w = b0 + lane;
+    long so = 0, dof = 0;
+    int n = 0;
+    if (w < nwork && dropped[w]) {
+      const long p = work[w];
+      const long b = dropped[w] == 1 ? roff[p + 1] - roff[p] : 0;
+      n = counts[p] - (int)b;
+      so = aoff[w];
+      dof = doff[p] + b;
+    }
+    for (int k = 0; k < 64; k++) {
+      const int m = __shfl(n, k, 64);
+      if (m == 0) continue;
+      const double2* s2 = reinterpret_cast<const double2*>(src + __shfl(so, k, 64) * EW);
+      double2* d2 = reinterpret_cast<double2*>(dst + __shfl(dof, k, 64) * EW);
+      for (int i = lane; i < (EW / 2) * m; i += 64) d2[i] = s2[i];
+    }
+  }
+}
+
 template <class T>
 static void inclusive_scan_long(Context& C, const T* in, long* out, long n) {
   size_t bytes = 0;
@@ -1617,6 +1736,66 @@ static void triage_tets(Context& C, ClipArgs& g, bool mass, long cap) {
 }
 
 bool mass_prepare_rows(Context& C, const DevMesh& A, const DevMesh& B, const long* indptr, ClipArgs& g);
+
+// LIBSUPERMESH_GPU_DENSE after a single pass whose runs are placed: runs at
+// roff[p] (split runs' arena parts and the runs clipped again at aoff[w]) of
+// g.elements, which lies `gap` elements into C.elements, moved to the prefix
+// sums of the counts at C.elements' start (k_dense_step), or, when some pair
+// lies ahead of its run, copied to the second buffer if device memory holds
+// it.  False when neither: the caller fills every pair again.
+static bool dense_from_runs(Context& C, const ClipArgs& g, long gap, long nwork, const long* roff,
+                            libsupermesh_gpu_result* out) {
+  constexpr int kMaxSteps = 512;  // in-place steps before giving up (a few dozen in practice)
+  const long npairs = out->npairs, nout = out->nout;
+  long* dofs = C.offsets_dense.as<long>(npairs + 1);
+  long* tmp_counts = C.row_count.as<long>(npairs);
+  hipLaunchKernelGGL(k_fill_long, dim3(1), dim3(64), 0, C.stream, 1L, 0L, dofs);
+  hipLaunchKernelGGL(k_count_to_long, dim3(grid_for(npairs, 256)), dim3(256), 0, C.stream, npairs, g.counts,
+                     tmp_counts);
+  LSM_HIP(hipGetLastError());
+  inclusive_scan_long(C, tmp_counts, dofs + 1, npairs);
+  long* ctl = C.dense_ctl.as<long>(4);  // [0] pairs ahead of their runs, [2..3] the step cursor
+  hipLaunchKernelGGL(k_fill_long, dim3(1), dim3(64), 0, C.stream, 4L, 0L, ctl);
+  hipLaunchKernelGGL(k_dense_fits, dim3(grid_for(npairs + 1, 256)), dim3(256), 0, C.stream, npairs + 1, roff, gap,
+                     dofs, reinterpret_cast<unsigned long long*>(ctl));
+  LSM_HIP(hipGetLastError());
+  long ahead = 0;
+  LSM_HIP(hipMemcpyAsync(&ahead, ctl, sizeof(long), hipMemcpyDeviceToHost, C.stream));
+  LSM_HIP(hipStreamSynchronize(C.stream));
+  const double* src = g.elements;
+  double* dst = static_cast<double*>(C.elements.ptr);
+  if (ahead == 0) {
+    long* cursor = ctl + 2;
+    long p0 = 0;
+    int step = 0;
+    while (p0 < npairs && step < kMaxSteps) {
+      for (int j = 0; j < 8; j++, step++)
+        hipLaunchKernelGGL(k_dense_step<12>, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, npairs, g.counts,
+                           roff, gap, dofs, cursor, step, src, dst);
+      LSM_HIP(hipGetLastError());
+      LSM_HIP(hipMemcpyAsync(&p0, cursor + (step & 1), sizeof(long), hipMemcpyDeviceToHost, C.stream));
+      LSM_HIP(hipStreamSynchronize(C.stream));
+    }
+    if (p0 < npairs) return false;
+  } else {
+    size_t free_b = 0, total_b = 0;
+    LSM_HIP(hipMemGetInfo(&free_b, &total_b));
+    if ((double)nout * 96.0 + 1e9 > (double)free_b) return false;
+    dst = C.elements_dense.as<double>((size_t)(nout > 0 ? nout : 1) * 12);
+    hipLaunchKernelGGL(k_dense_step<12>, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, npairs, g.counts, roff,
+                       gap, dofs, nullptr, 0, src, dst);
+  }
+  if (nwork > 0)
+    hipLaunchKernelGGL(k_dense_overflow<12>, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, nwork, g.work,
+                       g.dropped, g.counts, roff, g.aoff, dofs, src, dst);
+  LSM_HIP(hipGetLastError());
+  LSM_HIP(hipStreamSynchronize(C.stream));
+  out->offsets = dofs;
+  out->elements = dst;
+  out->elements_size = nout;
+  out->dense_route = ahead == 0 ? 1 : 2;
+  return true;
+}
 
 void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int flags,
                 libsupermesh_gpu_result* out) {
@@ -1651,8 +1830,10 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
   if (dim == 3 && npairs > 0) triage_tets(C, g, mass, npairs);
   bool learned = elems && dim == 3 && (flags & LIBSUPERMESH_GPU_LEARNED) && npairs > 0;
   const bool dense = flags & LIBSUPERMESH_GPU_DENSE;
-  long nwork = 0, nslots = 0, nspill = 0, capacity = 0, arena_per_lane = 0;
+  long nwork = 0, nslots = 0, nspill = 0, capacity = 0, arena_per_lane = 0, gap = 0;
   long* offs = nullptr;
+  C.runs_layout = false;
+  out->dense_route = 0;
   if (learned) {
     // single pass: predicted budgets -> runs placed from them -> one walk
     // that counts and writes
@@ -1683,26 +1864,28 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
     LSM_HIP(hipMemcpyAsync(hs, sums, sizeof(hs), hipMemcpyDeviceToHost, C.stream));
     LSM_HIP(hipStreamSynchronize(C.stream));
     // the highest quantile level whose runs, with the arena and the spill
-    // reserve behind them (and, for the dense layout, the compacted copy),
-    // fit in the device memory left.  The arena takes the outputs beyond a
-    // budget (a fraction of the expected output, spread over the walk's
-    // lanes), the spill reserve the joined runs of those pairs and of the
-    // pairs whose lane's arena ran out (clipped again).
-    if (!dense) {
-      C.elements_dense.release();
-      C.offsets_dense.release();
-    }
+    // reserve behind them, fit in the device memory left.  The arena takes
+    // the outputs beyond a budget (a fraction of the expected output, spread
+    // over the walk's lanes), the spill reserve the joined runs of those
+    // pairs (run layout only) and the runs of the pairs whose lane's arena
+    // ran out (clipped again).  The dense layout is moved into place in the
+    // same buffer afterwards; its runs start a gap into the buffer, so that
+    // the pairs ahead of their budgets do not reach runs not yet moved.
+    C.elements_dense.release();
+    if (dense) C.offsets_dense.as<long>(npairs + 1);  // the dense layout's offsets
+    else C.offsets_dense.release();
     size_t free_b = 0, total_b = 0;
     LSM_HIP(hipMemGetInfo(&free_b, &total_b));
-    const double avail = (double)free_b + (double)C.elements.bytes + (double)C.elements_dense.bytes - 4e9;
+    const double avail = (double)free_b + (double)C.elements.bytes - 4e9;
     const double expect = (double)hs[4] / 1024.0;
     const long lanes = tet_walk_lanes(C, mass ? 4 : 3);  // the grid of the single-pass launch
     arena_per_lane = std::max(1L, ((long)(P.arena_fraction * expect) + P.arena_min) / lanes);
     nspill = (long)(P.arena_fraction * expect) + P.arena_min;
+    gap = dense ? (long)(expect / 256.0) + 4096 : 0;
     int level = -1;
     for (int k = 0; k < 4 && level < 0; k++) {
-      const double need = (double)hs[k] + (double)(arena_per_lane * lanes) + (double)nspill +
-                          (dense ? 1.05 * expect : 0.0) + 1.0;
+      const double need =
+          (double)gap + (double)hs[k] + (double)(arena_per_lane * lanes) + (double)nspill + 1.0;
       if (need * 96.0 <= avail) level = k;
     }
     if (level < 0) {
@@ -1726,7 +1909,7 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
       LSM_HIP(hipStreamSynchronize(C.stream));
       capacity = nslots + arena_per_lane * lanes + nspill;
       g.offsets = offs;
-      g.elements = C.elements.as<double>((size_t)capacity * 12);
+      g.elements = C.elements.as<double>((size_t)(gap + capacity) * 12) + gap * 12;
       g.budget = budget;
       g.arena_base = nslots;
       g.arena_per_lane = arena_per_lane;
@@ -1775,13 +1958,14 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
     out->learned_slots = nslots;
     out->learned_overflow = (long)(h[0] + h[1]);
     bool placed = true;
-    if (h[0] + h[1] > 0) {
-      // split runs are joined, and dropped pairs clipped again, into the
-      // spill reserve
+    // the spill reserve takes the runs of the pairs clipped again and, for
+    // the run layout, the split runs joined (the dense move takes a split
+    // run's two parts where they lie)
+    if (h[1] > 0 || (!dense && h[0] > 0)) {
       long* len = C.row_count.as<long>((size_t)nwork);
       long* incl = C.predictor.spill.as<long>((size_t)nwork);
       hipLaunchKernelGGL(k_dropped_lengths, dim3(grid_for(nwork, 256)), dim3(256), 0, C.stream, nwork, g.work,
-                         g.dropped, g.counts, len);
+                         g.dropped, (unsigned char)(dense ? 2 : 1), g.counts, len);
       LSM_HIP(hipGetLastError());
       inclusive_scan_long(C, len, incl, nwork);
       long need = 0;
@@ -1790,12 +1974,12 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
       placed = need <= nspill;
       if (placed) {
         const long base = capacity - nspill;
-        if (h[0] > 0)
+        if (!dense && h[0] > 0)
           hipLaunchKernelGGL(k_join_runs, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, nwork, g.work, g.dropped,
                              g.counts, g.budget, g.bshift, g.aoff, incl, base, offs, g.elements);
         if (h[1] > 0) {
           hipLaunchKernelGGL(k_dropped_offsets, dim3(grid_for(nwork, 256)), dim3(256), 0, C.stream, nwork,
-                             g.work, g.dropped, incl, g.counts, base, offs);
+                             g.work, g.dropped, incl, g.counts, base, g.aoff, offs, !dense);
           LSM_HIP(hipGetLastError());
           g.ovf = g.dropped;
           launch_tet_walk(C, g, 2);
@@ -1804,15 +1988,18 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
         LSM_HIP(hipGetLastError());
       }
     }
-    if (placed) {
+    if (placed && !dense) {
       out->offsets = offs;
       out->elements = static_cast<const double*>(C.elements.get(8));
       out->elements_size = capacity;
-      if (dense) compact_result(C, dim, out);
+      C.runs_layout = true;
       return;
     }
-    // the spill reserve is too small: the plain fill pass over every pair
-    // (the counts are known), into the dense layout
+    if (placed && dense_from_runs(C, g, gap, nwork, offs, out)) return;
+    out->dense_route = dense ? 3 : 0;
+    // the spill reserve is too small, or the dense layout could not be moved
+    // into place nor copied: the plain fill pass over every pair (the counts
+    // are known), into the dense layout
   }
   long* dofs = C.offsets.as<long>(npairs + 1);
   hipLaunchKernelGGL(k_fill_long, dim3(1), dim3(64), 0, C.stream, 1L, 0L, dofs);

@@ -60,7 +60,8 @@ class Result(ctypes.Structure):
                 ("mass_indices", c_void_p), ("mass_values", c_void_p), ("sum", c_double),
                 ("nonempty", c_long), ("nout", c_long), ("max_count", c_int),
                 ("ms", c_double * 8), ("learned_slots", c_long), ("learned_overflow", c_long),
-                ("learned_quantile", c_double), ("elements_size", c_long), ("count_graph", c_int)]
+                ("learned_quantile", c_double), ("elements_size", c_long), ("count_graph", c_int),
+                ("dense_route", c_int)]
 
 
 # libsupermesh_gpu_pair_callback

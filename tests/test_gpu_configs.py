@@ -12,9 +12,10 @@
   count, the per-pair counts and measures, and the measure sum equal those
   of one plain call per mesh pair; every |sum - 1| <= 1e-12.
 * tet 96^3 x 6 vs 90^3 x 6 (seed 4), materialised supermesh: the learned
-  single pass, compacted in the call (LIBSUPERMESH_GPU_DENSE), gives the plain
-  count-then-fill path's counts, offsets and output tets (device checksums),
-  and reports the budget level it used.
+  single pass, compacted in the call (LIBSUPERMESH_GPU_DENSE, the runs moved
+  within their own buffer), gives the plain count-then-fill path's counts,
+  offsets and output tets (device checksums), and reports the budget level
+  it used.
 
 These follow the reference's serial loops (src/tests/test_tri_intersector.F90:89-104,
 test_tet_intersector.F90:89-104) at the benchmark sizes.  Each test prints
@@ -144,8 +145,9 @@ def test_learned_tet_96_vs_90_identical_to_plain():
                          ctx.checksum(r.offsets, 8 * (r.npairs + 1)), ctx.checksum(r.elements, 96 * r.nout))
             if mode == "learned":
                 _say(f"learned: budget quantile {r.learned_quantile}, {r.learned_slots} slots, "
-                     f"{r.learned_overflow} pairs clipped again")
+                     f"{r.learned_overflow} pairs outgrew their budget, dense route {r.dense_route}")
                 assert 0.0 < r.learned_quantile <= 0.9 + 1e-6   # a float32 level
+                assert r.dense_route == 1   # moved within the runs' buffer
             assert abs(r.sum - 1.0) <= 1e-12
         assert res["plain"] == res["learned"]
         ip, _ = O.grid_finder(pa, ea, pb, eb, nthreads=_threads())

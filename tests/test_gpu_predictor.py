@@ -144,6 +144,40 @@ def test_learned_single_pass_is_identical(meshes, weights, quantile, arena):
     assert 0.0 < r1.learned_quantile <= quantile + 1e-6   # a float32 level
 
 
+@pytest.mark.parametrize("case,route", [("roomy", 1), ("no_budgets", 2), ("tiny_spill", 3)])
+def test_dense_routes(meshes, case, route):
+    """LIBSUPERMESH_GPU_DENSE's three ways to the plain layout, each forced,
+    each byte for byte the plain arrays: the runs moved within their buffer
+    (every budget the maximum: no pair's plain start beyond its run's), copied
+    to a second buffer (every budget 0: the plain layout lies beyond the
+    runs' start), every pair filled again (a one-tet spill reserve that the
+    pairs clipped again outgrow)."""
+    pa, ea, pb, eb = meshes
+    ctx = gpu.Context()
+    A, B = gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pb, eb)
+    if case == "tiny_spill":
+        ctx.predictor_set(R.init(9, scale=3.0))
+        ctx.predictor_arena(0.0, 1)
+        ctx.predictor_quantile(0.5)
+    else:
+        ctx.predictor_set(np.zeros(gpu.PREDICTOR_PARAMS, np.float32))
+        ctx.predictor_quantile(0.99 if case == "roomy" else 0.01)
+    what = ["counts", "offsets", "elements"]
+    plain = gpu.fetch(ctx.supermesh(A, B, flags=gpu.ELEMENTS), 3, what)
+    r = ctx.supermesh(A, B, flags=gpu.ELEMENTS | gpu.LEARNED | gpu.DENSE)
+    print(f"[dense] {case}: route {r.dense_route}, {r.learned_slots} slots, {r.learned_overflow} overflow, "
+          f"{r.nout} tets", flush=True)
+    assert r.dense_route == route
+    assert r.elements_size == r.nout
+    got = gpu.fetch(r, 3, what)
+    for k in what:
+        assert np.array_equal(got[k], plain[k]), k
+    if case == "no_budgets":
+        assert r.learned_slots == 0 and r.learned_overflow > 0
+    ctx.compact(r)   # already plain: left as it is
+    assert r.elements_size == r.nout and np.array_equal(gpu.fetch(r, 3, ["elements"])["elements"], plain["elements"])
+
+
 @pytest.mark.parametrize("weights", ["trained", "random"])
 def test_learned_single_pass_with_mass_is_identical(meshes, weights):
     """LEARNED with MASS (the single pass that also integrates the mixed mass
