@@ -262,17 +262,22 @@ def learned_workload(ctx, name, steps, warmup, weights, quantiles=(0.7, 0.8, 0.8
     A, B = gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pb, eb)
     ctx.predictor_set(np.fromfile(weights, dtype=np.float32))
     out = {}
-    modes = [("plain", gpu.ELEMENTS | gpu.TIMING, None)]
-    modes += [(f"learned_q{q}", gpu.ELEMENTS | gpu.LEARNED | gpu.TIMING, q) for q in quantiles]
-    modes += [("learned_dense", gpu.ELEMENTS | gpu.LEARNED | gpu.DENSE | gpu.TIMING, 0.9)]
+    # arena: None = the default (a quarter of the expected output, spread over
+    # the walk's lanes), 0 = none (every pair beyond its budget clipped again)
+    modes = [("plain", gpu.ELEMENTS | gpu.TIMING, None, None)]
+    modes += [(f"learned_q{q}", gpu.ELEMENTS | gpu.LEARNED | gpu.TIMING, q, None) for q in quantiles]
+    modes += [("learned_q0.9_noarena", gpu.ELEMENTS | gpu.LEARNED | gpu.TIMING, 0.9, 0)]
+    modes += [("learned_dense", gpu.ELEMENTS | gpu.LEARNED | gpu.DENSE | gpu.TIMING, 0.9, None)]
+    modes += [("learned_dense_noarena", gpu.ELEMENTS | gpu.LEARNED | gpu.DENSE | gpu.TIMING, 0.9, 0)]
 
     def sums(r):
         return (ctx.checksum(r.counts, 4 * r.npairs), ctx.checksum(r.offsets, 8 * (r.npairs + 1)),
                 ctx.checksum(r.elements, 96 * r.nout))
 
-    for mode, flags, q in modes:
+    for mode, flags, q, arena in modes:
         if q is not None:
             ctx.predictor_quantile(q)
+            ctx.predictor_arena(*((0.25, 1 << 20) if arena is None else (0.0, 0)))
         for _ in range(warmup):
             ctx.supermesh(A, B, flags=flags)
         gpu.synchronize()
@@ -312,15 +317,16 @@ def learned_workload(ctx, name, steps, warmup, weights, quantiles=(0.7, 0.8, 0.8
     same = all(v.pop("_sums") == ref for k, v in out.items() if k != "plain")
     runs = [k for k in out if k.startswith("learned_q")]
     best = min(runs, key=lambda k: out[k]["ms_per_step"])
-    return {"workload": name, "plain": out["plain"], "learned": out[best],
+    dense = min((k for k in out if k.startswith("learned_dense")), key=lambda k: out[k]["ms_per_step"])
+    return {"workload": name, "plain": out["plain"], "learned": out[best], "learned_mode": best,
             "speedup": out["plain"]["ms_per_step"] / out[best]["ms_per_step"],
-            "learned_dense": out["learned_dense"],
-            "speedup_dense": out["plain"]["ms_per_step"] / out["learned_dense"]["ms_per_step"],
+            "learned_dense": out[dense], "learned_dense_mode": dense,
+            "speedup_dense": out["plain"]["ms_per_step"] / out[dense]["ms_per_step"],
             "identical_outputs": same, "weights": os.path.relpath(weights, ROOT),
             "quantile_sweep": {k: {"ms_per_step": round(out[k]["ms_per_step"], 3),
                                    "quantile_used": out[k]["quantile_used"], "slots": out[k]["slots"],
                                    "overflow_pairs": out[k]["overflow_pairs"]}
-                               for k in runs}}
+                               for k in runs + [k for k in out if k.startswith("learned_dense")]}}
 
 
 def clip_roofline(name, w):

@@ -273,12 +273,15 @@ int libsupermesh_gpu_predictor_set(libsupermesh_gpu_context* ctx, const float* p
 int libsupermesh_gpu_predictor_get(libsupermesh_gpu_context* ctx, float* params);
 int libsupermesh_gpu_predictor_quantile(libsupermesh_gpu_context* ctx, float q);
 /* Outputs beyond a pair's budget go to an arena of fraction x the predicted
- * output plus min_tets tets (defaults 0.25 and 2^20) split among the clip's
- * lanes; such runs are then joined in a spill reserve of the same size, and
- * pairs that find their lane's share full are clipped again into it.  When
- * the spill reserve is too small every pair is filled again in the plain
- * layout (elements_size = nout). */
+ * output plus min_tets tets (defaults 0.25 and 2^20; 0 and 0: no arena)
+ * split among the clip's lanes; such runs are then joined in the spill
+ * reserve, and pairs that find their lane's share full (every pair beyond
+ * its budget, without an arena) are clipped again into it. */
 int libsupermesh_gpu_predictor_arena(libsupermesh_gpu_context* ctx, float fraction, long min_tets);
+/* The spill reserve: fraction x the predicted output plus min_tets tets
+ * (defaults 0.25 and 2^20).  When the runs to place there outgrow it every
+ * pair is filled again in the plain layout (elements_size = nout). */
+int libsupermesh_gpu_predictor_spill(libsupermesh_gpu_context* ctx, float fraction, long min_tets);
 int libsupermesh_gpu_predictor_collect(libsupermesh_gpu_context* ctx, const libsupermesh_gpu_mesh* a,
                                        const libsupermesh_gpu_mesh* b, long max_samples,
                                        long* nsamples);

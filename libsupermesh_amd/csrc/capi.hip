@@ -768,11 +768,21 @@ int libsupermesh_gpu_predictor_quantile(libsupermesh_gpu_context* ctx, float q) 
 
 int libsupermesh_gpu_predictor_arena(libsupermesh_gpu_context* ctx, float fraction, long min_tets) {
   return guarded([&] {
-    LSM_REQUIRE(ctx != nullptr && fraction >= 0.f && min_tets >= 1, LIBSUPERMESH_GPU_ERROR_INVALID,
+    LSM_REQUIRE(ctx != nullptr && fraction >= 0.f && min_tets >= 0, LIBSUPERMESH_GPU_ERROR_INVALID,
                 "invalid arena size");
     std::lock_guard<std::mutex> lk(ctx->ctx.mu);
     ctx->ctx.predictor.arena_fraction = fraction;
     ctx->ctx.predictor.arena_min = min_tets;
+  });
+}
+
+int libsupermesh_gpu_predictor_spill(libsupermesh_gpu_context* ctx, float fraction, long min_tets) {
+  return guarded([&] {
+    LSM_REQUIRE(ctx != nullptr && fraction >= 0.f && min_tets >= 1, LIBSUPERMESH_GPU_ERROR_INVALID,
+                "invalid spill reserve size");
+    std::lock_guard<std::mutex> lk(ctx->ctx.mu);
+    ctx->ctx.predictor.spill_fraction = fraction;
+    ctx->ctx.predictor.spill_min = min_tets;
   });
 }
 

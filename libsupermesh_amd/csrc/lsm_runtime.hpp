@@ -108,7 +108,9 @@ struct Predictor {
   float quantile = 0.9f;    // budget = this quantile of the predicted count
   int level = 0;            // quantile level the last single pass used (0 = quantile)
   float arena_fraction = 0.25f;  // arena size: this fraction of the expected output ...
-  long arena_min = 1L << 20;     // ... plus this many tets
+  long arena_min = 1L << 20;     // ... plus this many tets (both 0: no arena)
+  float spill_fraction = 0.25f;  // spill reserve: this fraction of the expected output ...
+  long spill_min = 1L << 20;     // ... plus this many tets
   DevBuf weights, bias;     // BF16 layouts, FP32 biases [4][64]
   DevBuf master, adam_m, adam_v, grad;
   DevBuf features, labels;  // collected training samples

@@ -1879,8 +1879,8 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
     const double avail = (double)free_b + (double)C.elements.bytes - 4e9;
     const double expect = (double)hs[4] / 1024.0;
     const long lanes = tet_walk_lanes(C, mass ? 4 : 3);  // the grid of the single-pass launch
-    arena_per_lane = std::max(1L, ((long)(P.arena_fraction * expect) + P.arena_min) / lanes);
-    nspill = (long)(P.arena_fraction * expect) + P.arena_min;
+    arena_per_lane = ((long)(P.arena_fraction * expect) + P.arena_min) / lanes;
+    nspill = (long)(P.spill_fraction * expect) + P.spill_min;
     gap = dense ? (long)(expect / 256.0) + 4096 : 0;
     int level = -1;
     for (int k = 0; k < 4 && level < 0; k++) {

@@ -112,6 +112,8 @@ def lib():
         L.libsupermesh_gpu_predictor_add_samples.argtypes = [c_void_p, c_long, fp, POINTER(c_int)]
         L.libsupermesh_gpu_predictor_arena.argtypes = [c_void_p, ctypes.c_float, c_long]
         L.libsupermesh_gpu_predictor_arena.restype = c_int
+        L.libsupermesh_gpu_predictor_spill.argtypes = [c_void_p, ctypes.c_float, c_long]
+        L.libsupermesh_gpu_predictor_spill.restype = c_int
         L.libsupermesh_gpu_checksum.argtypes = [c_void_p, c_void_p, ctypes.c_size_t,
                                                 POINTER(ctypes.c_ulonglong)]
         L.libsupermesh_gpu_checksum.restype = c_int
@@ -583,6 +585,9 @@ class Context:
 
     def predictor_arena(self, fraction, min_tets):
         _check(lib().libsupermesh_gpu_predictor_arena(self.handle, ctypes.c_float(fraction), min_tets))
+
+    def predictor_spill(self, fraction, min_tets):
+        _check(lib().libsupermesh_gpu_predictor_spill(self.handle, ctypes.c_float(fraction), min_tets))
 
     def predictor_collect(self, a: DeviceMesh, b: DeviceMesh, max_samples):
         n = c_long(0)

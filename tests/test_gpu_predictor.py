@@ -94,14 +94,17 @@ def test_training_learns_clip_counts(meshes):
 
 @pytest.mark.parametrize("weights", ["trained", "random", "zero"])
 @pytest.mark.parametrize("quantile", [0.5, 0.99])
-@pytest.mark.parametrize("arena", ["default", "spill", "tiny"])
+@pytest.mark.parametrize("arena", ["default", "spill", "none", "tiny"])
 def test_learned_single_pass_is_identical(meshes, weights, quantile, arena):
     pa, ea, pb, eb = meshes
     ctx = gpu.Context()
     if arena == "spill":
         ctx.predictor_arena(0.0, 1 << 20)   # a few arena tets per lane: some pairs are clipped again
+    elif arena == "none":
+        ctx.predictor_arena(0.0, 0)   # no arena: every pair beyond its budget is clipped again
     elif arena == "tiny":
-        ctx.predictor_arena(0.0, 1)   # a one-tet spill reserve: the plain fill of every pair
+        ctx.predictor_arena(0.0, 1)
+        ctx.predictor_spill(0.0, 1)   # a one-tet spill reserve: the plain fill of every pair
     A, B = gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pb, eb)
     if weights == "trained":
         ctx.predictor_collect(A, B, 1 << 20)
@@ -157,7 +160,8 @@ def test_dense_routes(meshes, case, route):
     A, B = gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pb, eb)
     if case == "tiny_spill":
         ctx.predictor_set(R.init(9, scale=3.0))
-        ctx.predictor_arena(0.0, 1)
+        ctx.predictor_arena(0.0, 0)
+        ctx.predictor_spill(0.0, 1)
         ctx.predictor_quantile(0.5)
     else:
         ctx.predictor_set(np.zeros(gpu.PREDICTOR_PARAMS, np.float32))
@@ -191,7 +195,8 @@ def test_learned_single_pass_with_mass_is_identical(meshes, weights):
         ctx.predictor_train(epochs=6, batch=512, lr=3e-3, seed=1)
     else:
         ctx.predictor_set(R.init(9, scale=3.0))
-    ctx.predictor_arena(0.0, 1)   # force overflow pairs through the fallback too
+    ctx.predictor_arena(0.0, 0)
+    ctx.predictor_spill(0.0, 1)   # force overflow pairs through the fallback too
     what = ["indptr", "indices", "counts", "measure", "offsets", "elements", "blocks"]
     flags = gpu.ELEMENTS | gpu.SORTED | gpu.MASS | gpu.BLOCKS
     r0 = ctx.supermesh(A, B, flags=flags)
