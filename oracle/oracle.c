@@ -64,8 +64,12 @@
  *     the discovery order of hand-worked meshes; transpose == scipy's CSR
  *     transpose, twice == identity.
  *   advancing front: on connected meshes every row's set equals the closed
- *     brute-force set; the row order (the front's visiting order) is pinned
- *     only by a hand-worked two-mesh example.
+ *     brute-force set; the row order (the front's stack order, each row
+ *     grown from its clue's, a new seed per part, closed boxes) by a mesh
+ *     pair worked by hand through the reference's loop.
+ *   sort finder: on meshes the brute-force strict set; on intervals that
+ *     overlap within a mesh (where the sweep skips pairs) cases worked by
+ *     hand through the reference's sweep, and merge orders with ties.
  */
 #include "oracle.h"
 

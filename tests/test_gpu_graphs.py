@@ -119,6 +119,15 @@ def test_advancing_front_matches_oracle(fixture_meshes):
     assert n > 100000
 
 
+def test_advancing_front_worked_case():
+    """The rows worked out by hand through the reference's loop
+    (test_oracle_graphs.af_worked_case): the stack order of the A front."""
+    from test_oracle_graphs import af_worked_case
+    pa, ea, pb, eb, rows = af_worked_case()
+    gp, gi = gpu.advancing_front_intersection_finder(pa, ea, pb, eb)
+    assert [list(gi[gp[a]:gp[a + 1]]) for a in range(len(ea))] == rows
+
+
 def test_advancing_front_empty():
     pa, ea = meshgen.split_grid(2, 3, 1)
     gp, gi = gpu.advancing_front_intersection_finder(pa, ea, pa[:0], np.zeros((0, 3), dtype=np.int64))

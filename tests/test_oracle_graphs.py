@@ -105,6 +105,48 @@ def test_transpose_is_scipy_transpose():
     assert np.array_equal(ptr2, A.indptr) and np.array_equal(idx2, A.indices)
 
 
+def af_worked_case():
+    """A small mesh pair whose advancing-front rows were worked out by hand
+    through the reference's loop (Intersection_Finder.F90:270-454).
+
+    A: the square [-1, 1]^2 cut into four triangles around its centre, e0
+    bottom, e1 right, e2 top, e3 left (face neighbours of e0: [e1, e3]), plus
+    e4 = (0.4, 0.5), (0.9, 0.5), (0.4, 0.9), which shares no face (a second
+    seed).  B: three vertical strips x in [-1, -0.4], [-0.4, 0.4], [0.4, 1],
+    each cut into a lower-right and an upper-left triangle (b0 b1, b2 b3, b4
+    b5); B's face neighbours: b0 [b1, b3], b1 [b0], b2 [b3, b5], b3 [b2, b0],
+    b4 [b5], b5 [b4, b2].  Boxes meet when closed intervals do.
+
+    * e0 (seed): brute force in B order, [0 1 2 3 4 5].  It pushes e1, then
+      e3; the front is a stack, so e3 comes first, with e0's row as its clue.
+    * e3 (box x in [-1, 0]): e0's row filtered: [0 1 2 3].  It pushes e2.
+    * e2 (clue e3): starts from [0 1 2 3]; b2's unseen neighbour b5 joins
+      the front, then b5's unseen b4: [0 1 2 3 5 4].  (Had e2 been reached
+      from e1 -- a queue instead of a stack -- it would start from e1's row
+      and read [2 3 4 5 0 1].)
+    * e1 (clue e0, box x in [0, 1]): [2 3 4 5].
+    * e4, the next unseen element, is a new seed: brute force, and its box's
+      x = 0.4 touches the middle strip's: [2 3 4 5]."""
+    pa = np.array([[-1, -1], [1, -1], [1, 1], [-1, 1], [0, 0], [0.4, 0.5], [0.9, 0.5], [0.4, 0.9]], float)
+    ea = np.array([[0, 1, 4], [1, 2, 4], [2, 3, 4], [3, 0, 4], [5, 6, 7]], np.int32)
+    xs = [-1.0, -0.4, 0.4, 1.0]
+    pb = np.array([[x, -1.0] for x in xs] + [[x, 1.0] for x in xs])
+    eb = np.array([[s, s + 1, s + 5] if k == 0 else [s, s + 5, s + 4] for s in range(3) for k in range(2)],
+                  np.int32)
+    rows = [[0, 1, 2, 3, 4, 5], [2, 3, 4, 5], [0, 1, 2, 3, 5, 4], [0, 1, 2, 3], [2, 3, 4, 5]]
+    return pa, ea, pb, eb, rows
+
+
+def test_advancing_front_worked_case():
+    pa, ea, pb, eb, rows = af_worked_case()
+    ptr, idx = O.mesh_eelist(pa.shape[0], ea, 2, 3)
+    assert [list(idx[ptr[i]:ptr[i + 1]]) for i in range(len(ea))] == [[1, 3], [0, 2], [1, 3], [0, 2], []]
+    ptr, idx = O.mesh_eelist(pb.shape[0], eb, 2, 3)
+    assert [list(idx[ptr[i]:ptr[i + 1]]) for i in range(len(eb))] == [[1, 3], [0], [3, 5], [2, 0], [5], [4, 2]]
+    ptr, idx = O.advancing_front(pa, ea, pb, eb)
+    assert [list(idx[ptr[a]:ptr[a + 1]]) for a in range(len(ea))] == rows
+
+
 def _closed_sets(pa, ea, pb, eb):
     ip, ix = O.brute_force(pa, ea, pb, eb, closed=True)
     return [set(ix[ip[a]:ip[a + 1]].tolist()) for a in range(len(ea))]
