@@ -163,7 +163,8 @@ int libsupermesh_gpu_synchronize(int device);
 /* The count pass's launch shape the context measured fastest for dimension
  * dim (LIBSUPERMESH_GPU_GRAPH; zeros before the first such call): threads per
  * block, blocks in the grid (3D: the walk's persistent lanes / block; 2D:
- * blocks of the grid-stride loop, i.e. the pairs each wave takes), the tet
+ * blocks of the grid-stride loop, i.e. the pairs each wave takes, or 0: one
+ * block per `block` pairs, each lane one pair), the tet
  * depths whose stack lives in LDS (3D: 2 or 3; the rest in registers) and the
  * time of that shape on the sample it was measured on. */
 int libsupermesh_gpu_count_shape(libsupermesh_gpu_context* ctx, int dim, int* block, int* grid, int* lds_depths,
