@@ -265,6 +265,31 @@ __device__ __forceinline__ void load_planes(const double* __restrict__ bp, long 
   }
 }
 
+// Output elements are written once and not read again by the pass that
+// writes them: streaming (nontemporal) stores keep them from evicting the
+// meshes' coordinates and planes that the clippers read from L2 (materialised
+// tet 96^3 vs 90^3: the fill walk and the single pass each 6 ms faster).
+__device__ __forceinline__ void stream_tri(double* o, double x0, double y0, double x1, double y1, double x2,
+                                           double y2) {
+  __builtin_nontemporal_store(x0, o);
+  __builtin_nontemporal_store(y0, o + 1);
+  __builtin_nontemporal_store(x1, o + 2);
+  __builtin_nontemporal_store(y1, o + 3);
+  __builtin_nontemporal_store(x2, o + 4);
+  __builtin_nontemporal_store(y2, o + 5);
+}
+
+__device__ __forceinline__ void stream_tet(double* o, const double* t) {
+#pragma unroll
+  for (int i = 0; i < 12; i++) __builtin_nontemporal_store(t[i], o + i);
+}
+
+// One 16-byte word of a run copy, stored the same way.
+__device__ __forceinline__ void stream_word(double2* d, const double2* s) {
+  using v2 = double __attribute__((ext_vector_type(2)));
+  __builtin_nontemporal_store(*reinterpret_cast<const v2*>(s), reinterpret_cast<v2*>(d));
+}
+
 __device__ __forceinline__ long pair_count(const ClipArgs& g) { return g.npairs_dev ? *g.npairs_dev : g.npairs; }
 
 // Tet pairs: pairs whose A lies provably beyond one of B's face planes get
@@ -399,13 +424,7 @@ __global__ void __launch_bounds__(kClipBlock) k_clip(ClipArgs g) {
           mass_element<2>(el, ar, affa, affb, blk);
         }
         if (FILL && k < cap) {
-          double* o = g.elements + (off + k) * 6;
-          o[0] = x0;
-          o[1] = y0;
-          o[2] = x1;
-          o[3] = y1;
-          o[4] = x2;
-          o[5] = y2;
+          stream_tri(g.elements + (off + k) * 6, x0, y0, x1, y1, x2, y2);
         }
         k++;
       });
@@ -421,9 +440,7 @@ __global__ void __launch_bounds__(kClipBlock) k_clip(ClipArgs g) {
         m = m + vr;
         if (MASS) mass_element<3>(T.v, vr, affa, affb, blk);
         if (FILL && k < cap) {
-          double* o = g.elements + (off + k) * 12;
-#pragma unroll
-          for (int kk = 0; kk < 12; kk++) o[kk] = T.v[kk];
+          stream_tet(g.elements + (off + k) * 12, T.v);
         }
         k++;
       });
@@ -622,13 +639,7 @@ __global__ void __launch_bounds__(BLOCK) k_clip_tri(ClipArgs g) {
             m = m + ar;
             if (MASS) tri_moments(x0, y0, x1, y1, x2, y2, ar, ox, oy, mom);
             if (FILL && k < cap) {
-              double* o = g.elements + (off + k) * 6;
-              o[0] = x0;
-              o[1] = y0;
-              o[2] = x1;
-              o[3] = y1;
-              o[4] = x2;
-              o[5] = y2;
+              stream_tri(g.elements + (off + k) * 6, x0, y0, x1, y1, x2, y2);
             }
             k++;
           });
@@ -861,16 +872,12 @@ k_clip_tet_walk(ClipArgs g) {
         if (keep) {
           if (MASS) mass_element<3>(t, vr, affa, affb, blk);
           if ((FILL || LEARNED) && k < cap) {
-            double* o = g.elements + (off + k) * 12;
-#pragma unroll
-            for (int i = 0; i < 12; i++) o[i] = t[i];
+            stream_tet(g.elements + (off + k) * 12, t);
           } else if (LEARNED) {
             // beyond the budget: the run continues in this lane's arena
             const long e = atop + (k - cap);
             if (e < g.arena_per_lane) {
-              double* o = g.elements + (abase + e) * 12;
-#pragma unroll
-              for (int i = 0; i < 12; i++) o[i] = t[i];
+              stream_tet(g.elements + (abase + e) * 12, t);
             } else {
               lost = true;
             }
@@ -1024,13 +1031,7 @@ __global__ void __launch_bounds__(kTriBlock) k_clip_mixed(MixedArgs g) {
     auto sink = [&](double x0, double y0, double x1, double y1, double x2, double y2, double ar) {
       m = m + ar;
       if (FILL && k < cap) {
-        double* o = g.elements + (off + k) * 6;
-        o[0] = x0;
-        o[1] = y0;
-        o[2] = x1;
-        o[3] = y1;
-        o[4] = x2;
-        o[5] = y2;
+        stream_tri(g.elements + (off + k) * 6, x0, y0, x1, y1, x2, y2);
       }
       k++;
     };
@@ -1148,9 +1149,7 @@ k_clip_hex_walk(HexWalkArgs g) {
         const double vr = tet_volume_real(t);
         m = m + (keep ? vr : 0.0);
         if (FILL && keep && k < cap) {
-          double* o = g.elements + (off + k) * 12;
-#pragma unroll
-          for (int i = 0; i < 12; i++) o[i] = t[i];
+          stream_tet(g.elements + (off + k) * 12, t);
         }
         k += keep ? 1 : 0;
       });
@@ -1550,8 +1549,8 @@ k_join_runs(long nwork, const int* __restrict__ work, const unsigned char* __res
       double2* d2 = reinterpret_cast<double2*>(el + __shfl(dst, k, 64) * 12);
       const double2* a2 = reinterpret_cast<const double2*>(el + __shfl(s1, k, 64) * 12);
       const double2* b2 = reinterpret_cast<const double2*>(el + __shfl(s2, k, 64) * 12);
-      for (int i = lane; i < 6 * m1; i += 64) d2[i] = a2[i];
-      for (int i = lane; i < 6 * m2; i += 64) d2[6 * m1 + i] = b2[i];
+      for (int i = lane; i < 6 * m1; i += 64) stream_word(d2 + i, a2 + i);
+      for (int i = lane; i < 6 * m2; i += 64) stream_word(d2 + 6 * m1 + i, b2 + i);
     }
     if (n1 + n2 > 0) offsets[p] = dst;
   }
@@ -1585,7 +1584,7 @@ k_compact_runs(long npairs, const int* __restrict__ counts, const long* __restri
       if (n == 0) continue;
       const double2* s2 = reinterpret_cast<const double2*>(src + __shfl(so, k, 64) * EW);
       double2* d2 = reinterpret_cast<double2*>(dst + __shfl(dof, k, 64) * EW);
-      for (int i = lane; i < (EW / 2) * n; i += 64) d2[i] = s2[i];
+      for (int i = lane; i < (EW / 2) * n; i += 64) stream_word(d2 + i, s2 + i);
     }
   }
 }
@@ -1678,7 +1677,7 @@ k_dense_step(long npairs, const int* __restrict__ counts, const long* __restrict
       if (m == 0) continue;
       const double2* s2 = reinterpret_cast<const double2*>(src + __shfl(so, k, 64) * EW);
       double2* d2 = reinterpret_cast<double2*>(dst + __shfl(dof, k, 64) * EW);
-      for (int i = lane; i < (EW / 2) * m; i += 64) d2[i] = s2[i];
+      for (int i = lane; i < (EW / 2) * m; i += 64) stream_word(d2 + i, s2 + i);
     }
   }
 }
@@ -1710,7 +1709,7 @@ k_dense_overflow(long nwork, const int* __restrict__ work, const unsigned char* 
       if (m == 0) continue;
       const double2* s2 = reinterpret_cast<const double2*>(src + __shfl(so, k, 64) * EW);
       double2* d2 = reinterpret_cast<double2*>(dst + __shfl(dof, k, 64) * EW);
-      for (int i = lane; i < (EW / 2) * m; i += 64) d2[i] = s2[i];
+      for (int i = lane; i < (EW / 2) * m; i += 64) stream_word(d2 + i, s2 + i);
     }
   }
 }
