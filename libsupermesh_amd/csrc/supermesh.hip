@@ -320,8 +320,8 @@ __global__ void __launch_bounds__(kClipBlock) k_tet_triage(ClipArgs g, int* __re
       const double sc = fmax_f(g.ascale[a], g.bscale[b]);
       keep = !tet_separated_by_b(A, P, sc);
       if (!keep) {
-        g.counts[p] = 0;
-        g.measure[p] = 0.0;
+        __builtin_nontemporal_store(0, g.counts + p);
+        __builtin_nontemporal_store(0.0, g.measure + p);
         if (MASS) {
 #pragma unroll
           for (int k = 0; k < 16; k++) g.blocks[p * 16 + k] = 0.0;
@@ -896,8 +896,8 @@ k_clip_tet_walk(ClipArgs g) {
       if (lost) atomicAdd(g.ndropped + 1, 1ULL);
     }
     if (!FILL) {
-      g.counts[p] = k;
-      g.measure[p] = k == 0 ? 0.0 : m;
+      __builtin_nontemporal_store(k, g.counts + p);
+      __builtin_nontemporal_store(k == 0 ? 0.0 : m, g.measure + p);
       if (MASS) {
 #pragma unroll
         for (int i = 0; i < 16; i++) g.blocks[p * 16 + i] = k == 0 ? 0.0 : blk[i];
