@@ -41,7 +41,7 @@ BYTES_PER_PAIR = {"tri_1024_vs_1000": 60 + 4 + 4 + 8 + 72, "tet_64_vs_60": 112 +
 # HBM bytes per launch of the same kernel on the same workload, from the two
 # counter runs summarised in profiles/traffic_tri1024_tet64.md (FETCH_SIZE x
 # 1.9992 + WRITE_SIZE x 1.0, the factors a 256 MiB copy calibrates).
-PROFILED_TRAFFIC = {"tri_1024_vs_1000": 3.55e9, "tet_64_vs_60": 32.26e9}
+PROFILED_TRAFFIC = {"tri_1024_vs_1000": 3.55e9, "tet_64_vs_60": 28.46e9}
 
 
 def parse():
