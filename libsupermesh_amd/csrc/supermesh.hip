@@ -1162,8 +1162,8 @@ k_clip_hex_walk(HexWalkArgs g) {
       continue;
     }
     if (!FILL) {
-      g.counts[p] = k;
-      g.measure[p] = k == 0 ? 0.0 : m;
+      __builtin_nontemporal_store(k, g.counts + p);
+      __builtin_nontemporal_store(k == 0 ? 0.0 : m, g.measure + p);
     }
     more = Q.next();
     active = false;
