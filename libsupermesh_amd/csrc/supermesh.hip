@@ -15,6 +15,8 @@
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_reduce.hpp>
 #include <rocprim/device/device_scan.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
+#include <rocprim/iterator/transform_iterator.hpp>
 
 #include "lsm_geometry.hpp"
 #include "lsm_runtime.hpp"
@@ -1495,15 +1497,6 @@ static void launch_tet_walk(Context& C, const ClipArgs& args, bool mass, bool fi
   launch_tet_walk(C, args, fill ? 2 : (mass ? 1 : 0));
 }
 
-// Single pass: each pair's budget (0 for pairs outside the work list) as the
-// run lengths whose prefix sums place the runs.
-__global__ void __launch_bounds__(256)
-k_pair_budgets(long nwork, const int* __restrict__ work, const unsigned* __restrict__ budget, int shift,
-               long* __restrict__ len) {
-  const long w = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (w < nwork) len[work[w]] = (budget[w] >> shift) & 0xffu;
-}
-
 __global__ void __launch_bounds__(256) k_fill_long(long n, long v, long* __restrict__ out) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = v;
@@ -1731,11 +1724,54 @@ k_dense_overflow(long nwork, const int* __restrict__ work, const unsigned char* 
 }
 
 template <class T>
-static void inclusive_scan_long(Context& C, const T* in, long* out, long n) {
+static void inclusive_scan_long(Context& C, T in, long* out, long n) {
   size_t bytes = 0;
   LSM_HIP(rocprim::inclusive_scan(nullptr, bytes, in, out, (size_t)n, rocprim::plus<long>(), C.stream));
   void* tmp = C.scan_tmp.get(bytes);
   LSM_HIP(rocprim::inclusive_scan(tmp, bytes, in, out, (size_t)n, rocprim::plus<long>(), C.stream));
+}
+
+// Single pass: work item w's budget at the chosen level (byte shift / 8 of
+// the packed levels), as the run length the placement scans.
+struct BudgetAt {
+  const unsigned* budget;
+  int shift;
+  __host__ __device__ long operator()(long w) const { return (long)((budget[w] >> shift) & 0xffu); }
+};
+
+// Single pass: every pair's run start from the work items' inclusive budget
+// sums (pairs off the work list, all empty, start where the next work
+// item's run does; offsets[npairs] = the slots).  Item w owns the pairs from
+// just after item w - 1's pair up to its own (w = nwork: the tail up to
+// npairs).  A wave takes 64 consecutive items, whose pairs are one
+// contiguous range, and writes it with its lanes on consecutive offsets,
+// each finding its pair's item by a binary search over the lanes' ranges.
+__global__ void __launch_bounds__(256)
+k_place_runs(long npairs, long nwork, const int* __restrict__ work, const long* __restrict__ incl,
+             long* __restrict__ offsets) {
+  const int lane = threadIdx.x & 63;
+  const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long nwaves = ((long)gridDim.x * blockDim.x) >> 6;
+  for (long w0 = wave * 64; w0 <= nwork; w0 += nwaves * 64) {
+    const long w = w0 + lane;
+    const int last = nwork - w0 < 63 ? (int)(nwork - w0) : 63;  // the wave's last item (lane)
+    long hi = 0x7fffffffffffffffL, base = 0;
+    if (w <= nwork) {
+      hi = w < nwork ? (long)work[w] : npairs;
+      base = w == 0 ? 0 : incl[w - 1];
+    }
+    const long first = w0 == 0 ? 0 : (long)work[w0 - 1] + 1;
+    const long end = __shfl(hi, last, 64);
+    for (long b = first; b <= end; b += 64) {
+      const long p = b + lane <= end ? b + lane : end;
+      int j = 0;  // the first lane whose range ends at or after p
+#pragma unroll
+      for (int step = 32; step > 0; step >>= 1)
+        if (__shfl(hi, j + step - 1, 64) < p) j += step;
+      const long v = __shfl(base, j, 64);
+      if (b + lane <= end) offsets[p] = v;
+    }
+  }
 }
 
 void compact_result(Context& C, int dim, libsupermesh_gpu_result* out);
@@ -1921,14 +1957,14 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
       // runs: budgets laid end to end in pair order (pairs outside the work
       // list have none)
       offs = C.offsets.as<long>(npairs + 1);
-      long* len = C.row_count.as<long>(npairs);
-      hipLaunchKernelGGL(k_fill_long, dim3(grid_for(npairs, 256)), dim3(256), 0, C.stream, npairs, 0L, len);
+      long* incl = C.row_count.as<long>((size_t)(nwork > 0 ? nwork : 1));
       if (nwork > 0)
-        hipLaunchKernelGGL(k_pair_budgets, dim3(grid_for(nwork, 256)), dim3(256), 0, C.stream, nwork, g.work,
-                           budget, g.bshift, len);
-      hipLaunchKernelGGL(k_fill_long, dim3(1), dim3(64), 0, C.stream, 1L, 0L, offs);
+        inclusive_scan_long(C, rocprim::make_transform_iterator(rocprim::counting_iterator<long>(0),
+                                                                BudgetAt{budget, g.bshift}),
+                            incl, nwork);
+      hipLaunchKernelGGL(k_place_runs, dim3(grid_for(nwork + 1, 256)), dim3(256), 0, C.stream, npairs, nwork,
+                         g.work, incl, offs);
       LSM_HIP(hipGetLastError());
-      inclusive_scan_long(C, len, offs + 1, npairs);
       LSM_HIP(hipMemcpyAsync(&nslots, offs + npairs, sizeof(long), hipMemcpyDeviceToHost, C.stream));
       LSM_HIP(hipStreamSynchronize(C.stream));
       capacity = nslots + arena_per_lane * lanes + nspill;
