@@ -1502,16 +1502,18 @@ __global__ void __launch_bounds__(256) k_fill_long(long n, long v, long* __restr
   if (i < n) out[i] = v;
 }
 
-// Single pass, pairs flagged `from` or above in dropped (1: split runs and
-// dropped pairs, 2: dropped pairs only): their output lengths (0 for the
-// others), and then the dropped ones' runs placed from `base` (aoff[w], and
-// offsets[p] when set_offsets).
-__global__ void __launch_bounds__(256)
-k_dropped_lengths(long nwork, const int* __restrict__ work, const unsigned char* __restrict__ dropped,
-                  unsigned char from, const int* __restrict__ counts, long* __restrict__ len) {
-  const long w = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (w < nwork) len[w] = dropped[w] >= from ? counts[work[w]] : 0;
-}
+// Single pass, work items flagged `from` or above in dropped (1: split runs
+// and dropped pairs, 2: dropped pairs only): their output lengths (0 for the
+// others), scanned to place their runs in the spill reserve; then the
+// dropped ones' runs placed from `base` (aoff[w], and offsets[p] when
+// set_offsets).
+struct DroppedLength {
+  const int* work;
+  const unsigned char* dropped;
+  unsigned char from;
+  const int* counts;
+  __host__ __device__ long operator()(long w) const { return dropped[w] >= from ? (long)counts[work[w]] : 0L; }
+};
 
 __global__ void __launch_bounds__(256)
 k_dropped_offsets(long nwork, const int* __restrict__ work, const unsigned char* __restrict__ dropped,
@@ -2022,12 +2024,11 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
     // the run layout, the split runs joined (the dense move takes a split
     // run's two parts where they lie)
     if (h[1] > 0 || (!dense && h[0] > 0)) {
-      long* len = C.row_count.as<long>((size_t)nwork);
       long* incl = C.predictor.spill.as<long>((size_t)nwork);
-      hipLaunchKernelGGL(k_dropped_lengths, dim3(grid_for(nwork, 256)), dim3(256), 0, C.stream, nwork, g.work,
-                         g.dropped, (unsigned char)(dense ? 2 : 1), g.counts, len);
-      LSM_HIP(hipGetLastError());
-      inclusive_scan_long(C, len, incl, nwork);
+      inclusive_scan_long(C, rocprim::make_transform_iterator(
+                                 rocprim::counting_iterator<long>(0),
+                                 DroppedLength{g.work, g.dropped, (unsigned char)(dense ? 2 : 1), g.counts}),
+                          incl, nwork);
       long need = 0;
       LSM_HIP(hipMemcpyAsync(&need, incl + nwork - 1, sizeof(long), hipMemcpyDeviceToHost, C.stream));
       LSM_HIP(hipStreamSynchronize(C.stream));
