@@ -139,40 +139,6 @@ __device__ __forceinline__ void pair_features(const double* __restrict__ ta, con
   for (int k = 24; k < 32; k++) row[k] = (__bf16)0.f;
 }
 
-// Budgets at two quantile levels, one per byte (bits 0-7, 8-15): for each
-// level, the smallest count whose cumulative predicted probability reaches
-// it, or the clip's maximum (81) when only the last class ("32 or more")
-// does; also 1024 x the expected count.  The cumulative probabilities only
-// grow with the count, so that smallest count is the number of counts below
-// 32 whose cumulative probability stays under the level (all 32: 81).
-__device__ __forceinline__ unsigned budgets_pair(const float* logit, const float* levels, unsigned& expect) {
-  float m = logit[0];
-#pragma unroll
-  for (int c = 1; c < 33; c++) m = fmaxf(m, logit[c]);
-  float ev[33];
-  float s = 0.f;
-#pragma unroll
-  for (int c = 0; c < 33; c++) {
-    ev[c] = __expf(logit[c] - m);
-    s += ev[c];
-  }
-  const float t0 = levels[0] * s, t1 = levels[1] * s;
-  float acc = 0.f, ex = 0.f;
-  unsigned n0 = 0, n1 = 0;
-#pragma unroll
-  for (int c = 0; c < 33; c++) {
-    acc += ev[c];
-    ex += (float)c * ev[c];
-    if (c < 32) {
-      n0 += acc < t0 ? 1u : 0u;
-      n1 += acc < t1 ? 1u : 0u;
-    }
-  }
-  expect = (unsigned)(1024.f * ex / s);
-  const unsigned b0 = n0 == 32 ? (unsigned)kTetBufSize : n0, b1 = n1 == 32 ? (unsigned)kTetBufSize : n1;
-  return b0 | (b1 << 8);
-}
-
 void budget_levels(float q, float* levels) {
   levels[0] = q;
   levels[1] = q < 0.9f ? q : 0.9f;
@@ -295,31 +261,78 @@ __global__ void __launch_bounds__(64 * kPredWaves) __attribute__((amdgpu_waves_p
 #pragma unroll
     for (int s = 0; s < 4; s++) b[s] = relu_pack(acc[s >> 1], 8 * (s & 1), sB + 128 + 32 * (s >> 1));
     layer<64>(W + o4, b, acc);
-    // logits of pair c, classes 0..32, in both of its lanes
-    float L[33];
+    // logits of pair c: this lane's 16 classes 8 (r / 4) + 4 half + r % 4,
+    // and unit 32 (tile 1, row 0) in half 0.  The softmax, its running sums
+    // and the four budget levels are split between the pair's two lanes:
+    // each exponentiates its own classes, and the halves swap their maximum,
+    // sums, the sums of their four 4-class blocks and their level counts.
+    float v[16];
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
-      const float mine = acc[0][r], other = other_half(mine);
-      // class 8 (r / 4) + 4 hh + r % 4 comes from the lane of half hh
-      L[8 * (r >> 2) + (r & 3)] = half == 0 ? mine : other;
-      L[8 * (r >> 2) + 4 + (r & 3)] = half == 0 ? other : mine;
-    }
-    L[32] = half == 0 ? acc[1][0] : other_half(acc[1][0]);  // unit 32 = tile 1, row 0: half 0's element 0
+    for (int r = 0; r < 16; r++) v[r] = acc[0][r] + sB[192 + 8 * (r >> 2) + 4 * half + (r & 3)];
+    const float v32 = acc[1][0] + sB[192 + 32];
+    float mloc = half == 0 ? v32 : v[0];
 #pragma unroll
-    for (int k = 0; k < 33; k++) L[k] += sB[192 + k];
-    if (valid) {
-      // the two halves' lanes hold the same pair: each finds two of the four
-      // budget levels (half 0 levels 0 and 1, half 1 levels 2 and 3)
-      unsigned ex = 0;
-      const unsigned two = budgets_pair(L, g.levels + 2 * half, ex);
-      const unsigned other = __float_as_uint(other_half(__uint_as_float(two)));
-      const unsigned bud = half == 0 ? (two | (other << 16)) : 0u;
-      if (half == 0) {
-        g.budget[w] = bud;
+    for (int r = 0; r < 16; r++) mloc = fmaxf(mloc, v[r]);
+    const float mx = fmaxf(mloc, other_half(mloc));
+    float e[16], bs[4];
+    float sloc = 0.f, exloc = 0.f;
 #pragma unroll
-        for (int k = 0; k < 4; k++) tot[k] += (bud >> (8 * k)) & 0xffu;
-        tot[4] += ex;
+    for (int jb = 0; jb < 4; jb++) {
+      bs[jb] = 0.f;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int r = 4 * jb + i;
+        e[r] = __expf(v[r] - mx);
+        bs[jb] += e[r];
+        exloc += (float)(8 * jb + 4 * half + i) * e[r];
       }
+      sloc += bs[jb];
+    }
+    const float e32 = half == 0 ? __expf(v32 - mx) : 0.f;
+    sloc += e32;
+    exloc += 32.f * e32;
+    const float ssum = sloc + other_half(sloc), exsum = exloc + other_half(exloc);
+    // probability before each of this lane's blocks: block 2 jb + h (classes
+    // 8 jb + 4 h .. + 3) is half h's block jb
+    float obs[4];  // the other half's block sums (swapped by every lane alike)
+#pragma unroll
+    for (int jb = 0; jb < 4; jb++) obs[jb] = other_half(bs[jb]);
+    float before[4] = {0.f, 0.f, 0.f, 0.f};
+    float run = 0.f;
+#pragma unroll
+    for (int B = 0; B < 8; B++) {
+      const bool mine = (B & 1) == half;
+      if (mine) before[B >> 1] = run;
+      run += mine ? bs[B >> 1] : obs[B >> 1];
+    }
+    float thr[4];
+#pragma unroll
+    for (int lv = 0; lv < 4; lv++) thr[lv] = g.levels[lv] * ssum;
+    // classes below 32 whose cumulative probability stays under each level,
+    // one byte per level (at most 16 per lane)
+    unsigned cnt = 0;
+#pragma unroll
+    for (int jb = 0; jb < 4; jb++) {
+      float cum = before[jb];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        cum += e[4 * jb + i];
+#pragma unroll
+        for (int lv = 0; lv < 4; lv++) cnt += cum < thr[lv] ? (1u << (8 * lv)) : 0u;
+      }
+    }
+    cnt += __float_as_uint(other_half(__uint_as_float(cnt)));
+    unsigned bud = 0;
+#pragma unroll
+    for (int lv = 0; lv < 4; lv++) {
+      const unsigned n = (cnt >> (8 * lv)) & 0xffu;
+      bud |= (n == 32 ? (unsigned)kTetBufSize : n) << (8 * lv);
+    }
+    if (valid && half == 0) {
+      g.budget[w] = bud;
+#pragma unroll
+      for (int k = 0; k < 4; k++) tot[k] += (bud >> (8 * k)) & 0xffu;
+      tot[4] += (unsigned)(1024.f * exsum / ssum);
     }
   }
 #pragma unroll
