@@ -26,6 +26,8 @@ namespace lsm {
 
 constexpr int kClipBlock = 256;
 constexpr int kClipGrid = 2048;  // fixed: the reduction tree depends only on npairs
+// The tet triage's blocks: each compacts one contiguous chunk of the pairs
+constexpr int kTriageGrid = 16384;
 
 struct Partial {
   double hi, lo;
@@ -356,26 +358,35 @@ k_gather_work(long npairs, const long* __restrict__ npairs_dev, const long* __re
   for (long i = threadIdx.x; i < n; i += kClipBlock) dst[o + i] = src[b * chunk + i];
 }
 
-// Exclusive offsets of the triage's kClipGrid block counts (one block):
-// offset[0] = 0, offset[b + 1] = count[0] + .. + count[b]; offset[kClipGrid]
-// is the work list's length.
+// Exclusive offsets of the triage's kTriageGrid block counts (one block):
+// offset[0] = 0, offset[b + 1] = count[0] + .. + count[b];
+// offset[kTriageGrid] is the work list's length.
 __global__ void __launch_bounds__(1024) k_scan_blocks(const long* __restrict__ count, long* __restrict__ offset) {
-  static_assert(kClipGrid == 2048, "two counts per thread");
+  constexpr int K = kTriageGrid / 1024;  // consecutive counts per thread
+  static_assert(kTriageGrid % 1024 == 0, "whole counts per thread");
   __shared__ long part[1024];
   const int t = threadIdx.x;
-  const long c0 = count[2 * t], c1 = count[2 * t + 1];
-  part[t] = c0 + c1;
+  long c[K], sum = 0;
+#pragma unroll
+  for (int i = 0; i < K; i++) {
+    c[i] = count[K * t + i];
+    sum += c[i];
+  }
+  part[t] = sum;
   __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {  // inclusive scan of the pair sums (Hillis-Steele)
+  for (int o = 1; o < 1024; o <<= 1) {  // inclusive scan of the threads' sums (Hillis-Steele)
     const long v = t >= o ? part[t - o] : 0;
     __syncthreads();
     part[t] += v;
     __syncthreads();
   }
-  const long before = t > 0 ? part[t - 1] : 0;
+  long run = t > 0 ? part[t - 1] : 0;
   if (t == 0) offset[0] = 0;
-  offset[2 * t + 1] = before + c0;
-  offset[2 * t + 2] = before + c0 + c1;
+#pragma unroll
+  for (int i = 0; i < K; i++) {
+    run += c[i];
+    offset[K * t + i + 1] = run;
+  }
 }
 
 template <int DIM, bool MASS, bool FILL>
@@ -1400,7 +1411,7 @@ unsigned long long checksum(Context& C, const void* ptr, size_t bytes) {
 long last_work_list(Context& C, const int** work) {
   long n = 0;
   const long* bcount = static_cast<const long*>(C.nwork.ptr);
-  LSM_HIP(hipMemcpyAsync(&n, bcount + 2 * kClipGrid + 1, sizeof(long), hipMemcpyDeviceToHost, C.stream));
+  LSM_HIP(hipMemcpyAsync(&n, bcount + 2 * kTriageGrid + 1, sizeof(long), hipMemcpyDeviceToHost, C.stream));
   LSM_HIP(hipStreamSynchronize(C.stream));
   *work = static_cast<const int*>(C.work.ptr);
   return n;
@@ -1782,19 +1793,19 @@ void compact_result(Context& C, int dim, libsupermesh_gpu_result* out);
 // kernels, so that a graph can record it.
 static void triage_tets(Context& C, ClipArgs& g, bool mass, long cap) {
   int* region = C.mflag.as<int>(cap);
-  long* bcount = C.nwork.as<long>(2 * kClipGrid + 2);
-  long* boff = bcount + kClipGrid + 1;
+  long* bcount = C.nwork.as<long>(2 * kTriageGrid + 2);
+  long* boff = bcount + kTriageGrid + 1;
   if (mass)
-    hipLaunchKernelGGL(k_tet_triage<true>, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, g, region, bcount);
+    hipLaunchKernelGGL(k_tet_triage<true>, dim3(kTriageGrid), dim3(kClipBlock), 0, C.stream, g, region, bcount);
   else
-    hipLaunchKernelGGL(k_tet_triage<false>, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, g, region, bcount);
+    hipLaunchKernelGGL(k_tet_triage<false>, dim3(kTriageGrid), dim3(kClipBlock), 0, C.stream, g, region, bcount);
   hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, C.stream, bcount, boff);
   int* work = C.work.as<int>(cap);
-  hipLaunchKernelGGL(k_gather_work, dim3(kClipGrid), dim3(kClipBlock), 0, C.stream, g.npairs, g.npairs_dev, boff,
+  hipLaunchKernelGGL(k_gather_work, dim3(kTriageGrid), dim3(kClipBlock), 0, C.stream, g.npairs, g.npairs_dev, boff,
                      region, work);
   LSM_HIP(hipGetLastError());
   g.work = work;
-  g.nwork = reinterpret_cast<const unsigned long long*>(boff + kClipGrid);
+  g.nwork = reinterpret_cast<const unsigned long long*>(boff + kTriageGrid);
 }
 
 bool mass_prepare_rows(Context& C, const DevMesh& A, const DevMesh& B, const long* indptr, ClipArgs& g);
@@ -2329,7 +2340,7 @@ void count_pass_graph(Context& C, const DevMesh& A, const DevMesh& B, long npair
   // every buffer the pass touches, allocated before any recording
   C.mflag.as<int>(cap);
   C.work.as<int>(cap);
-  C.nwork.as<long>(2 * kClipGrid + 2);
+  C.nwork.as<long>(2 * kTriageGrid + 2);
   C.queue.as<unsigned long long>(kQueues);
   unsigned long long* scal = C.scalars.as<unsigned long long>(16);
   if (ngroups > 0) {
