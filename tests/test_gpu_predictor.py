@@ -215,3 +215,38 @@ def test_learned_single_pass_with_mass_is_identical(meshes, weights):
     ref = O.mass_pairs(pa, ea, pb, eb, rows, cols)
     scale = np.abs(ref).max()
     assert np.abs(learned["blocks"] - ref).max() <= 1e-12 * scale
+
+
+@pytest.mark.parametrize("dense", [False, True])
+def test_learned_batched_is_identical(dense):
+    """LIBSUPERMESH_GPU_LEARNED on a batch of 3D mesh pairs (one batched call,
+    the walk over every mesh's pairs at once): each mesh's counts and output
+    tets equal those of one plain count-then-fill call per mesh pair, in the
+    run layout (runs read where their offsets put them) and compacted."""
+    pairs = [p for p in meshgen.batched_pairs(count=24, seed=5, emin=50, emax=3000) if p[0] == 3]
+    assert len(pairs) >= 6
+    ctx = gpu.Context()
+    ctx.predictor_set(R.init(9, scale=3.0))   # budgets that many runs outgrow
+    ctx.predictor_quantile(0.5)
+    BA = gpu.DeviceMeshBatch([(p[1], p[2]) for p in pairs])
+    BB = gpu.DeviceMeshBatch([(p[3], p[4]) for p in pairs])
+    flags = gpu.ELEMENTS | gpu.SORTED | gpu.LEARNED | (gpu.DENSE if dense else 0)
+    r, br = ctx.supermesh_batched(BA, BB, flags=flags)
+    poff, sums = gpu.fetch_batch(br)
+    got = gpu.fetch(r, 3, ["counts", "offsets", "elements"])
+    assert r.learned_overflow > 0 and r.learned_quantile > 0
+    if dense:
+        assert r.elements_size == r.nout and r.dense_route in (1, 2, 3)
+    else:
+        check_runs_apart(r, got)
+    for m, (_, pa, ea, pb, eb) in enumerate(pairs):
+        rp = ctx.supermesh(gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pb, eb), flags=gpu.ELEMENTS | gpu.SORTED)
+        plain = gpu.fetch(rp, 3, ["counts", "elements"])
+        p0, p1 = int(poff[m]), int(poff[m + 1])
+        assert p1 - p0 == rp.npairs, m
+        cnt = got["counts"][p0:p1]
+        assert np.array_equal(cnt, plain["counts"]), m
+        runs = [got["elements"][got["offsets"][p]:got["offsets"][p] + got["counts"][p]] for p in range(p0, p1)]
+        mine = np.concatenate(runs) if runs else np.zeros((0, 4, 3))
+        assert np.array_equal(mine.reshape(-1, 4, 3), plain["elements"].reshape(-1, 4, 3)), m
+        assert abs(sums[m] - rp.sum) <= 1e-12, m
