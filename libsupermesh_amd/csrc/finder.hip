@@ -623,7 +623,12 @@ k_query_wave(long na, const int* __restrict__ qperm, const double* __restrict__ 
   }
   long cnt = 0;
   auto emit = [&](long leaf) {
-    if (cnt < cap) scratch[cnt * na + t] = perm[leaf];
+    if (cnt < cap) {
+      // 2D: streaming stores keep the hierarchy's boxes in L2 (0.96 -> 0.93 ms
+      // on tri 1024^2 vs 1000^2); in 3D they made the search slower
+      if constexpr (DIM == 2) __builtin_nontemporal_store(perm[leaf], scratch + cnt * na + t);
+      else scratch[cnt * na + t] = perm[leaf];
+    }
     cnt++;
   };
   const int top = info.levels - 1;
