@@ -267,7 +267,7 @@ def learned_workload(ctx, name, steps, warmup, weights, quantiles=(0.7, 0.8, 0.8
     modes = [("plain", gpu.ELEMENTS | gpu.TIMING, None, None)]
     modes += [(f"learned_q{q}", gpu.ELEMENTS | gpu.LEARNED | gpu.TIMING, q, None) for q in quantiles]
     modes += [("learned_q0.9_noarena", gpu.ELEMENTS | gpu.LEARNED | gpu.TIMING, 0.9, 0)]
-    modes += [("learned_dense", gpu.ELEMENTS | gpu.LEARNED | gpu.DENSE | gpu.TIMING, 0.9, None)]
+    modes += [("learned_dense", gpu.ELEMENTS | gpu.LEARNED | gpu.DENSE | gpu.TIMING, 0.99, None)]
     modes += [("learned_dense_noarena", gpu.ELEMENTS | gpu.LEARNED | gpu.DENSE | gpu.TIMING, 0.9, 0)]
 
     def sums(r):

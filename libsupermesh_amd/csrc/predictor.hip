@@ -139,11 +139,16 @@ __device__ __forceinline__ void pair_features(const double* __restrict__ ta, con
   for (int k = 24; k < 32; k++) row[k] = (__bf16)0.f;
 }
 
+// The quantile asked for and three below it, highest first: the single pass
+// takes the highest whose runs fit in memory.  Below about 0.8 so many runs
+// outgrow their budget that the spill reserve overflows and every pair is
+// filled again (tet 96^3 vs 90^3: 0.95 126 ms, 0.9 130 ms, 0.85 132 ms, 0.8
+// and 0.7 216 ms, plain count-then-fill 178 ms).
 void budget_levels(float q, float* levels) {
   levels[0] = q;
-  levels[1] = q < 0.9f ? q : 0.9f;
-  levels[2] = q < 0.7f ? q : 0.7f;
-  levels[3] = q < 0.5f ? q : 0.5f;
+  levels[1] = q < 0.95f ? q : 0.95f;
+  levels[2] = q < 0.9f ? q : 0.9f;
+  levels[3] = q < 0.8f ? q : 0.8f;
 }
 
 // ---------------------------------------------------------------------------
