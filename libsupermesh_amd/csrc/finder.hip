@@ -737,12 +737,14 @@ k_rows_from_scratch(long na, const int* __restrict__ qperm, int cap, const int* 
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-    for (int q = 0; q < 64; q++) {
+    // 64 / J queries at a time: lane l writes slot l % J of query q + l / J
+    for (int q0 = 0; q0 < 64; q0 += 64 / J) {
+      const int q = q0 + lane / J, sl = lane % J;
       const long nq = __shfl(n, q, 64), oq = __shfl(o, q, 64);
       const int aq = __shfl(a, q, 64);
-      const long s = s0 + lane;
-      if (lane < J && s < nq) {
-        indices[oq + s] = T[lane][q];
+      const long s = s0 + sl;
+      if (s < nq) {
+        indices[oq + s] = T[sl][q];
         rows[oq + s] = aq;
       }
     }
