@@ -845,6 +845,7 @@ int libsupermesh_gpu_predictor_infer(libsupermesh_gpu_context* ctx, const libsup
     // packed levels land in the caller's array; level 0 is the low byte
     pa.budget = reinterpret_cast<unsigned*>(budgets);
     pa.sums = C.scalars.as<unsigned long long>(16) + 8;
+    pa.spare = reinterpret_cast<unsigned*>(pa.sums + 5);
     LSM_HIP(hipMemsetAsync(pa.sums, 0, 5 * sizeof(unsigned long long), C.stream));
     predict_budgets(C, pa);
     if (npairs > 0)

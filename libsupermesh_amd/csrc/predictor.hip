@@ -193,6 +193,7 @@ __device__ __forceinline__ void layer(const __bf16* W, const bf16x8* in, f32x16*
 // layers run on the MFMA units with every activation kept in registers (the
 // inference layouts put each accumulator where the next layer's operand
 // wants it), and both lanes of a pair evaluate the softmax and budgets.
+template <bool WORK>
 __global__ void __launch_bounds__(64 * kPredWaves) __attribute__((amdgpu_waves_per_eu(4))) k_predict(PredictArgs g) {
   __shared__ __align__(16) __bf16 sW[64 * (32 + kPad) + 3 * 64 * (64 + kPad)];
   __shared__ float sB[4 * 64];
@@ -210,17 +211,25 @@ __global__ void __launch_bounds__(64 * kPredWaves) __attribute__((amdgpu_waves_p
   const long ntiles = (g.nwork + 31) / 32;
   const long nwaves = (long)gridDim.x * kPredWaves;
   unsigned long long tot[5] = {0, 0, 0, 0, 0};
-  // the element of this lane's tet of its pair in tile `tile` (A's in half
-  // 0, B's in half 1), -1 past the end: looked up one tile ahead, so that
-  // each tile waits on one load (the coordinates) instead of three
-  auto element_of = [&](long tile) -> long {
-    const long w = tile * 32 + c;
-    if (tile >= ntiles || w >= g.nwork) return -1;
-    const long p = g.work ? g.work[w] : w;
-    return half ? g.cols[p] : g.rows[p];
+  // the pair of this lane's work item in tile `tile` (past the end, that of
+  // the last item: a lane past the end computes on real data, and its pair's
+  // column of the MFMA tiles is never stored), and the element of this
+  // lane's tet of pair p (A's in half 0, B's in half 1).  Three tiles are in
+  // flight: each tile issues the work item load of the tile two ahead, the
+  // element load of the next tile and its own coordinate loads back to back
+  // and unconditionally, so that it waits on one round trip (loads complete
+  // in order: a wait for any of them waits for the earlier ones)
+  auto pair_of = [&](long tile) -> int {
+    const long w = std::min<long>(tile * 32 + c, g.nwork - 1);
+    return WORK ? g.work[w] : (int)w;
   };
+  const int* element = half ? g.cols : g.rows;
   long tile = (long)blockIdx.x * kPredWaves + (threadIdx.x >> 6);
-  long e = element_of(tile);
+  int e = element[pair_of(tile)];
+  int p_next = pair_of(tile + nwaves);
+  // waited for here, before the loop, so that at the loop head only the
+  // previous tile's budget store can be in flight, and no load waits for it
+  asm volatile("" : "+v"(e), "+v"(p_next));
   for (; tile < ntiles; tile += nwaves) {
     // the weights are read from LDS at each tile: held in registers across
     // tiles (where the compiler would hoist them) they would not fit
@@ -229,13 +238,20 @@ __global__ void __launch_bounds__(64 * kPredWaves) __attribute__((amdgpu_waves_p
     const __bf16* W = sW + zero;
     const long w = tile * 32 + c;
     const bool valid = w < g.nwork;
+    const int p_after = pair_of(tile + 2 * nwaves);
+    const int e_next = element[p_next];
     double T[12];
     {
-      const double* src = (half ? g.bcoords : g.acoords) + (e >= 0 ? e : 0) * 12;
+      const double2* src = reinterpret_cast<const double2*>((half ? g.bcoords : g.acoords) + (long)e * 12);
 #pragma unroll
-      for (int k = 0; k < 12; k++) T[k] = e >= 0 ? src[k] : 0.0;
+      for (int k = 0; k < 6; k++) {
+        const double2 v = src[k];
+        T[2 * k] = v.x;
+        T[2 * k + 1] = v.y;
+      }
     }
-    e = element_of(tile + nwaves);
+    e = e_next;
+    p_next = p_after;
     // features: both tets normalised to their joint box (pair_features)
     bf16x8 x[2];
     {
@@ -333,8 +349,11 @@ __global__ void __launch_bounds__(64 * kPredWaves) __attribute__((amdgpu_waves_p
       const unsigned n = (cnt >> (8 * lv)) & 0xffu;
       bud |= (n == 32 ? (unsigned)kTetBufSize : n) << (8 * lv);
     }
+    // every lane stores (both lanes of a pair hold its budget; past the end,
+    // to the spare word), so that the store is counted exactly and the next
+    // tile's loads do not wait for it to complete
+    *(valid ? g.budget + w : g.spare) = bud;
     if (valid && half == 0) {
-      g.budget[w] = bud;
 #pragma unroll
       for (int k = 0; k < 4; k++) tot[k] += (bud >> (8 * k)) & 0xffu;
       tot[4] += (unsigned)(1024.f * exsum / ssum);
@@ -748,7 +767,10 @@ void predict_budgets(Context& C, const PredictArgs& args) {
   const long tiles = (args.nwork + 31) / 32;
   long blocks = std::min<long>((long)(cus > 0 ? cus : 256) * 4, (tiles + kPredWaves - 1) / kPredWaves);
   blocks = std::max<long>(blocks, 1);
-  hipLaunchKernelGGL(k_predict, dim3((unsigned)blocks), dim3(64 * kPredWaves), 0, C.stream, args);
+  if (args.work)
+    hipLaunchKernelGGL(k_predict<true>, dim3((unsigned)blocks), dim3(64 * kPredWaves), 0, C.stream, args);
+  else
+    hipLaunchKernelGGL(k_predict<false>, dim3((unsigned)blocks), dim3(64 * kPredWaves), 0, C.stream, args);
   LSM_HIP(hipGetLastError());
 }
 

@@ -25,6 +25,7 @@ struct PredictArgs {
   const float* bias;      // [4][64]
   float levels[4];           // quantiles, highest first
   unsigned* budget;          // [nwork]: byte k = the budget at levels[k]
+  unsigned* spare;           // one word that the lanes past the end store to
   unsigned long long* sums;  // [5]: budget sum per level; 1024 x expected count sum
 };
 

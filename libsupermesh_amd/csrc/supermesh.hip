@@ -1930,6 +1930,7 @@ void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int
     budget_levels(P.quantile, pa.levels);
     pa.budget = budget;
     pa.sums = sums;
+    pa.spare = reinterpret_cast<unsigned*>(sums + 5);
     if (C.timing) LSM_HIP(hipEventRecord(C.ev[12], C.stream));
     predict_budgets(C, pa);
     if (C.timing) LSM_HIP(hipEventRecord(C.ev[13], C.stream));
