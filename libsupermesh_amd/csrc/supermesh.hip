@@ -745,12 +745,16 @@ constexpr int kWalkReg = 1;
 // work list.
 constexpr int kXcds = 8;
 
-// The walk's lanes take work items from queues (counters handing out
+// The walk's lanes take work items from queues (counters handing out up to
 // kQueueBatch consecutive items at a time): each XCD's share of the work list
 // is cut into kSubQueues contiguous parts, each with its own counter, so the
 // items in flight on an XCD stay a few compact windows of the list whose A
 // and B data its L2 holds (with fixed strides the lanes drift apart over the
 // whole share), and no counter takes more than a fraction of the atomics.
+// A short list is handed out in smaller batches (about a quarter of the
+// items per lane, at least one): with 16 at a time, a list of a few
+// thousand items went to a few hundred lanes that walked 16 pairs each in
+// turn while the rest of the resident grid idled.
 constexpr int kQueueBatch = 16;
 constexpr int kSubQueues = 4;
 constexpr int kQueues = kXcds * kSubQueues;
@@ -758,12 +762,13 @@ struct WalkQueue {
   unsigned long long* counter;
   long begin, end;  // this queue's part of the work list
   long w = 0, wend = 0;
+  long batch = kQueueBatch;  // items per counter increment
   __device__ __forceinline__ bool next() {  // advance to the next item; false when none is left
     if (++w < wend) return true;
-    const long s = begin + (long)atomicAdd(counter, (unsigned long long)kQueueBatch);
+    const long s = begin + (long)atomicAdd(counter, (unsigned long long)batch);
     if (s >= end) return false;
     w = s;
-    wend = s + kQueueBatch < end ? s + kQueueBatch : end;
+    wend = s + batch < end ? s + batch : end;
     return true;
   }
 };
@@ -783,6 +788,8 @@ __device__ __forceinline__ WalkQueue walk_queue(unsigned long long* queues, long
   q.end = q.begin + sper < e0 ? q.begin + sper : e0;
   q.w = -1;
   q.wend = 0;
+  const long per_lane = n / (4L * gridDim.x * blockDim.x);
+  q.batch = per_lane < 1 ? 1 : (per_lane < kQueueBatch ? per_lane : kQueueBatch);
   return q;
 }
 
