@@ -184,11 +184,16 @@ static void run_supermesh(Context& C, const libsupermesh_gpu_mesh* a, const libs
   stage_groups(C, C.mesh_a, ngroups, a_off);
   stage_groups(C, C.mesh_b, ngroups, b_off);
   ph.mark();
-  build_bvh(C, C.mesh_b);
+  // a coarse A over a fine B: B's elements query a hierarchy over A, many
+  // short walks instead of a few long ones (each of A's few queries would
+  // walk thousands of leaves on one wave); the rows come out sorted
+  const bool from_b = ngroups == 0 && C.mesh_a.nelements > 0 && C.mesh_a.nelements * 8 < C.mesh_b.nelements;
+  build_bvh(C, from_b ? C.mesh_a : C.mesh_b);
   ph.mark();
   // find_pairs synchronises after reading the candidate count, which also
   // completes the copy of the flag queued before it
-  long npairs = find_pairs(C, C.mesh_a, C.mesh_b, finder, flags & LIBSUPERMESH_GPU_SORTED);
+  long npairs = from_b ? find_pairs_from_b(C, C.mesh_a, C.mesh_b, finder)
+                       : find_pairs(C, C.mesh_a, C.mesh_b, finder, flags & LIBSUPERMESH_GPU_SORTED);
   LSM_REQUIRE(*reinterpret_cast<const int*>(C.hstat) == 0, LIBSUPERMESH_GPU_ERROR_INVALID,
               "Invalid node: element-node index out of range");
   ph.mark();

@@ -902,4 +902,49 @@ long find_pairs(Context& C, const DevMesh& A, const DevMesh& B, int finder, int 
   return npairs;
 }
 
+// indptr over A from the pairs sorted by A id (arow): indptr[r] = the first
+// position of row r, indptr[na] = n
+__global__ void __launch_bounds__(kBlock)
+k_rows_to_indptr(long n, const unsigned* __restrict__ arow, long na, long* __restrict__ indptr) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  const long lo = i == 0 ? 0 : (long)arow[i - 1] + 1;
+  const long hi = i == n ? na : (long)arow[i];
+  for (long r = lo; r <= hi; r++) indptr[r] = i;
+}
+
+long find_pairs_from_b(Context& C, const DevMesh& A, const DevMesh& B, int finder) {
+  // B's elements query the hierarchy over A (built by the caller): a CSR over
+  // B, in B id order, whose entries are A ids.  The overlap tests are
+  // symmetric in the two boxes, so the pair set is the one A's queries find.
+  const long npairs = find_pairs(C, B, A, finder, 0);
+  const long na = A.nelements;
+  long* indptr = C.indptr.as<long>(na + 1);
+  if (npairs == 0) {
+    LSM_HIP(hipMemsetAsync(indptr, 0, sizeof(long) * (na + 1), C.stream));
+    return 0;
+  }
+  LSM_REQUIRE(npairs < (1L << 32), LIBSUPERMESH_GPU_ERROR_OVERFLOW, "too many candidate pairs to regroup");
+  unsigned* a_sorted = C.keys_alt.as<unsigned>(npairs);
+  int* b_sorted = C.vals_alt.as<int>(npairs);
+  const unsigned* a_ids = static_cast<const unsigned*>(C.indices.ptr);
+  const int* b_ids = static_cast<const int*>(C.rows.ptr);
+  // regrouped by A id with a stable sort: each A row keeps the B id order of
+  // the input, ascending
+  unsigned end_bit = 1;
+  while (end_bit < 31 && (1L << end_bit) < na) end_bit++;
+  size_t bytes = 0;
+  LSM_HIP(rocprim::radix_sort_pairs(nullptr, bytes, a_ids, a_sorted, b_ids, b_sorted, (size_t)npairs, 0u, end_bit,
+                                    C.stream));
+  void* tmp = C.sort_tmp.get(bytes);
+  LSM_HIP(rocprim::radix_sort_pairs(tmp, bytes, a_ids, a_sorted, b_ids, b_sorted, (size_t)npairs, 0u, end_bit,
+                                    C.stream));
+  hipLaunchKernelGGL(k_rows_to_indptr, dim3(grid_for(npairs + 1, kBlock)), dim3(kBlock), 0, C.stream, npairs,
+                     a_sorted, na, indptr);
+  LSM_HIP(hipGetLastError());
+  std::swap(C.rows, C.keys_alt);     // the A id of each pair
+  std::swap(C.indices, C.vals_alt);  // its B id, ascending within each row
+  return npairs;
+}
+
 }  // namespace lsm

@@ -187,6 +187,9 @@ void stage_groups(Context& C, DevMesh& M, long ngroups, const long* offsets);
 void build_bvh(Context& C, const DevMesh& B);
 // Count / fill candidate pairs of A against the BVH of B.  Returns npairs.
 long find_pairs(Context& C, const DevMesh& A, const DevMesh& B, int finder, int sort_rows);
+// The same pairs found the other way round (B's elements query a hierarchy
+// over A), regrouped as a CSR over A with every row in ascending B id order.
+long find_pairs_from_b(Context& C, const DevMesh& A, const DevMesh& B, int finder);
 
 // Clipping (supermesh.hip).
 void clip_pairs(Context& C, const DevMesh& A, const DevMesh& B, long npairs, int flags,

@@ -154,6 +154,26 @@ def test_closed_finder_on_touching_grids(ctx):
         assert abs(r.sum - 1.0) <= 1e-12
 
 
+@pytest.mark.parametrize("dim,na,nb,alpha", [(2, 6, 40, None), (2, 4, 32, 0.0), (3, 3, 12, None), (3, 2, 10, 0.0)])
+def test_coarse_a_over_fine_b(ctx, dim, na, nb, alpha):
+    # A with under an eighth of B's elements: B's elements query a hierarchy
+    # over A and the pairs are regrouped by A.  Same pair sets, rows, per-pair
+    # results and elements as the oracle, with the strict and the closed
+    # predicate (unjittered grids: boxes that touch exactly), and the rows
+    # come out sorted without SORTED too.
+    pa, ea = meshgen.split_grid(dim, na, 7 + na, alpha=alpha)
+    pb, eb = meshgen.split_grid(dim, nb, 11 + nb, alpha=alpha, mesh_id=1)
+    assert 8 * ea.shape[0] < eb.shape[0]
+    for finder, closed in [(gpu.FINDER_TREE, False), (gpu.FINDER_RTREE, True)]:
+        r, out = run(ctx, pa, ea, pb, eb, finder=finder, flags=gpu.ELEMENTS)
+        check_against_oracle(pa, ea, pb, eb, r, out, closed=closed, elements=True)
+        assert abs(r.sum - 1.0) <= 1e-12
+        r2 = ctx.supermesh(gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pb, eb), finder=finder)
+        out2 = gpu.fetch(r2, dim, ["indptr", "indices", "rows", "counts", "measure"])
+        for k in out2:
+            assert np.array_equal(out2[k], out[k]), k
+
+
 def test_default_row_order_same_pairs_and_results(ctx):
     # Without SORTED, rows keep the traversal order: same pair set and the
     # same per-pair results once both sides are ordered by (row, col).
