@@ -57,6 +57,8 @@ def parse():
     p.add_argument("--no-mixed", action="store_true", help="skip the tri/quad and tet/hex workloads")
     p.add_argument("--batch-count", type=int, default=4096, help="mesh pairs in the batched workload")
     p.add_argument("--no-learned", action="store_true", help="skip the learned single-pass workload")
+    p.add_argument("--per-call-only", action="store_true",
+                   help="only time one plain call per small mesh pair (first 256 of the batched generator)")
     p.add_argument("--learned-workload", default="tet_96_vs_90")
     p.add_argument("--weights", default=os.path.join(ROOT, "libsupermesh_amd", "data", "tet_counts_mlp.f32"))
     p.add_argument("--calibrate-copy", action="store_true",
@@ -468,6 +470,42 @@ def cpu_baseline(name, meshes, mass):
     return out
 
 
+def per_call_workload(ctx, count=256):
+    """One plain libsupermesh_gpu_supermesh call per small mesh pair (the
+    first `count` mesh pairs of the batched generator, seed 3: element counts
+    log-uniform in [50, 50000], resident in HBM), timed per call with the
+    library's phase times: the cost a caller pays who keeps the reference
+    manual's loop of one call per mesh pair."""
+    import numpy as np
+    from libsupermesh_amd import gpu, meshgen
+    pairs = meshgen.batched_pairs(count=count, seed=3)
+    plain = [(d, gpu.DeviceMesh(pa, ea), gpu.DeviceMesh(pb, eb), ea.shape[0], eb.shape[0])
+             for d, pa, ea, pb, eb in pairs]
+    for _, A, B, _, _ in plain:
+        ctx.supermesh(A, B)
+    gpu.synchronize()
+    out = {"workload": f"first {count} mesh pairs of the batched generator, one call each"}
+    for dim in (2, 3):
+        sel = [p for p in plain if p[0] == dim]
+        if not sel:
+            continue
+        ph = np.zeros(8)
+        t0 = time.perf_counter()
+        for _, A, B, _, _ in sel:
+            r = ctx.supermesh(A, B, flags=gpu.TIMING)
+            ph += np.array(list(r.ms)[:8])
+        gpu.synchronize()
+        dt = (time.perf_counter() - t0) / len(sel)
+        ph /= len(sel)
+        out[f"{dim}d"] = {"calls": len(sel), "ms_per_call": dt * 1e3,
+                          "mean_elements": [float(np.mean([p[3] for p in sel])), float(np.mean([p[4] for p in sel]))],
+                          "phase_ms": {k: round(float(ph[i]), 3) for k, i in
+                                       (("stage", 0), ("hierarchy", 1), ("search", 2), ("clip", 3),
+                                        ("clip_kernel", 7))}}
+        log(f"[bench] per call {dim}D: {len(sel)} calls, {dt * 1e3:.3f} ms/call, phases {out[f'{dim}d']['phase_ms']}")
+    return out
+
+
 def dropin_finder(meshes, steps=3):
     """The reference's drop-in C finder on host arrays:
     libsupermesh_tree_intersection_finder_set_input -> query_output ->
@@ -502,6 +540,10 @@ def main():
         gpu._check(gpu.lib().libsupermesh_gpu_memcpy(dst.ptr, src.ptr, n, 3))
         gpu.synchronize()
         log(f"[bench] calibration copy of {n} bytes done")
+    if args.per_call_only:
+        print(json.dumps(per_call_workload(ctx)))
+        ctx.close()
+        return
     main_w = gpu_workload(ctx, args.workload, args.steps, args.warmup, mass=args.workload.startswith("tri"))
     extra = None
     if not args.no_3d and args.workload != "tet_64_vs_60":
@@ -516,6 +558,7 @@ def main():
     if not args.no_learned and not args.no_batched:
         learned_batched = learned_batched_workload(ctx, max(1, min(args.steps, 2)), 1, args.weights,
                                                    count=max(1, args.batch_count // 4))
+    per_call = None if args.no_batched else per_call_workload(ctx)
     learned = None
     if not args.no_learned:
         if not os.path.exists(args.weights):
@@ -553,6 +596,7 @@ def main():
         "tet_64_vs_60": extra,
         "mixed_types": mixed,
         "batched": batched,
+        "per_call": per_call,
         "learned": learned,
         "learned_batched": learned_batched,
     }
