@@ -41,7 +41,11 @@ extern "C" {
 #define LIBSUPERMESH_GPU_TIMING 4     /* fill ms[] with per-phase times */
 #define LIBSUPERMESH_GPU_SORTED 8     /* sort each candidate row by B id (otherwise
                                          rows keep the hierarchy's traversal
-                                         order, which is deterministic) */
+                                         order, which is deterministic; when
+                                         A has under 1/8 of B's elements and
+                                         the call is not batched, B's elements
+                                         query a hierarchy over A and the rows
+                                         come out sorted either way) */
 #define LIBSUPERMESH_GPU_LEARNED 16   /* 3D with ELEMENTS: place each pair's run
                                          of output elements from the learned
                                          predictor's budget and clip once
